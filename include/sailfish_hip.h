@@ -1,0 +1,175 @@
+/*
+ * sailfish_hip.h -- C ABI of the MI355X (gfx950) split-explicit hot path.
+ *
+ * Every entry point takes device pointers, plain sizes/scalars and an explicit
+ * hipStream_t (passed as void* so that this header needs no HIP include).  Nothing
+ * synchronises the stream.  Return value: SF_OK (0) or a negative SF_ERR_* code;
+ * sf_last_error_string() describes the last failure of the calling thread.
+ *
+ * Each declaration names the reference interface it replaces (path:line in the
+ * Sailfish sources).  The reference passes grid sizes through device globals set
+ * once by `initialize` (modules/mod_cppkernels.h:154-171); here they live in an
+ * sf_ctx, so any number of grids (ensembles) can coexist in one process.
+ */
+#ifndef SAILFISH_HIP_H
+#define SAILFISH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SF_OK = 0,
+    SF_ERR_ARG = -1,          /* invalid argument (null pointer, size, range)       */
+    SF_ERR_HIP = -2,          /* a HIP runtime call failed                           */
+    SF_ERR_UNSUPPORTED = -3,  /* valid input that this entry point does not handle   */
+    SF_ERR_NOMEM = -4
+};
+
+enum { SF_FP64 = 0, SF_FP32 = 1 };
+
+/* sf_grid_desc.flags */
+enum {
+    /* computeMomentumPred masks with i%szJ (modules/mod_cppkernels.cpp:162); on grids
+     * with szI != szJ that freezes interior columns.  With this flag the mask is
+     * i%szI, i.e. exactly column 0 and row 0 (what the square-grid case does). */
+    SF_FLAG_FIX_Q5_MASK = 1
+};
+
+typedef struct sf_ctx sf_ctx;
+
+/* Horizontal/vertical sizes and the metric fields the reference keeps as device
+ * globals (_on_u, _om_v, _pn, _pm: mod_cppkernels.h:40-41).  szI = L+1 (xi, fastest),
+ * szJ = M+1, szK = N.  Metric pointers are FP64 device arrays of szI*szJ. */
+typedef struct sf_grid_desc {
+    int32_t szI, szJ, szK;
+    int32_t flags;            /* SF_FLAG_* */
+    const double *on_u, *om_v, *pn, *pm;
+} sf_grid_desc;
+
+/* Replaces `initialize` (mod_cppkernels.h:154-171) + `initModule`
+ * (modules/mod_operators.py:31-45). */
+int sf_ctx_create(const sf_grid_desc *desc, sf_ctx **out);
+int sf_ctx_destroy(sf_ctx *ctx);
+const char *sf_last_error_string(void);
+/* Library/ABI version: major*10000 + minor*100 + patch. */
+int sf_version(void);
+
+/* ---------------- barotropic LF-AM3 kernels, one to one, FP64 ----------------- */
+
+/* computeZetaRHS (modules/mod_cppkernels.cpp:80-128).  cff1 = weight1[iif-1]
+ * (read on the host instead of from device memory). */
+int sf_compute_zeta_rhs(sf_ctx *ctx, const double *zeta, const double *h,
+                        const double *ubar, const double *vbar, double *DU_avg1,
+                        double *DV_avg1, double *Zt_avg1, int32_t is_predictor,
+                        double cff1, double *res, void *stream);
+
+/* computeZetaPred (modules/mod_cppkernels.cpp:133-150). */
+int sf_compute_zeta_pred(sf_ctx *ctx, double Dt, const double *zeta_t0,
+                         const double *zeta_t1, double *zeta_t2, const double *rzeta_t1,
+                         void *stream);
+
+/* computeMomentumRHSPred (modules/mod_cppkernels.cpp:12-42). */
+int sf_compute_momentum_rhs_pred(sf_ctx *ctx, const double *h, double *rhs_ubar,
+                                 double *rhs_vbar, const double *zeta_t0,
+                                 const double *zeta_t2, double g, double weight,
+                                 void *stream);
+
+/* computeMomentumRHSCorr (modules/mod_cppkernels.cpp:45-77). */
+int sf_compute_momentum_rhs_corr(sf_ctx *ctx, const double *h, double *rhs_ubar,
+                                 double *rhs_vbar, const double *zeta_t0,
+                                 const double *zeta_t1, const double *zeta_t2, double g,
+                                 void *stream);
+
+/* computeMomentumPred (modules/mod_cppkernels.cpp:155-186). */
+int sf_compute_momentum_pred(sf_ctx *ctx, double Dt, const double *u_t1, double *u_t2,
+                             const double *v_t1, double *v_t2, const double *rhsu,
+                             const double *rhsv, const double *h, const double *zeta_t1,
+                             const double *zeta_t2, void *stream);
+
+/* AdamsMoultonCorr3rd (modules/mod_cppkernels.cpp:190-209). */
+int sf_adams_moulton_corr3rd(sf_ctx *ctx, double Dt, double *v_t2, const double *rhs_t0,
+                             const double *rhs_t1, const double *rhs_t2, void *stream);
+
+/* AdamsMoultonCorr3rd2 (modules/mod_cppkernels.cpp:213-251). */
+int sf_adams_moulton_corr3rd2(sf_ctx *ctx, double Dt, double *u_t2, double *v_t2,
+                              const double *rhsu_t0, const double *rhsu_t1,
+                              const double *rhsu_t2, const double *rhsv_t0,
+                              const double *rhsv_t1, const double *rhsv_t2,
+                              const double *h, const double *zeta_t1,
+                              const double *zeta_t2, void *stream);
+
+/* ---------------- lateral boundary conditions ---------------------------------- */
+
+/* setBC (modules/mod_BC.c:6-13): var[idx[k]] = val. */
+int sf_bc_set(double *var, const int32_t *idx, int32_t n, double val, void *stream);
+
+/* A composed boundary map: the net effect of an ordered list of setBC/copyBC calls
+ * (modules/mod_BC.c:6-23, issued by nonlinear/zetabc.py:5-12 and
+ * nonlinear/barotropicVelocityBC.py:7-31) run in list order.  For each k:
+ * src[k] >= 0 -> var[dst[k]] = pre(var[src[k]]); -1 -> 0.0; -2 -> clamp_val.
+ * dst[] entries are unique; every source is read before any destination is written.
+ * n <= 16384. */
+int sf_bc_apply_map(double *var, const int32_t *dst, const int32_t *src, int32_t n,
+                    double clamp_val, void *stream);
+
+/* ---------------- fused barotropic substep --------------------------------------- */
+
+/* Pointers of one substep, after the caller's time-level rotation (cycleTimes2D,
+ * modules/mod_ocean.py:284-318): index k of each 3-slot array is time level t_k.
+ * Bulk fields have the precision passed to sf_step2d_substep (double or float);
+ * the averages Zt/DU/DV_avg1 are always double.  Slots the substep never touches
+ * may be null: ubar[0], vbar[0], rubar[2], rvbar[0], rvbar[2]. */
+typedef struct sf_step2d_fields {
+    const void *h, *on_u, *om_v, *pm, *pn;
+    void *zeta[3], *ubar[3], *vbar[3];
+    void *rzeta[3], *rubar[3], *rvbar[3];
+    double *Zt_avg1, *DU_avg1, *DV_avg1;
+} sf_step2d_fields;
+
+/* sf_step2d_substep flags */
+enum {
+    SF_SUBSTEP_ZERO_VBAR = 1   /* vbar_t2 := 0 after the corrector (nonlinear/main3d.py:189) */
+};
+
+/* Boundary maps of the fused substep, as composed by the host from the BC lists of
+ * nonlinear/zetabc.py:5-12 and nonlinear/barotropicVelocityBC.py:7-31 (same encoding
+ * as sf_bc_apply_map; host arrays).  Rejects (SF_ERR_UNSUPPORTED) maps the fused
+ * kernel cannot apply inside a tile: zeta sources must be neighbours in the same
+ * tile, velocity maps must be constants (closed/clamped walls). */
+int sf_ctx_set_bc_maps(sf_ctx *ctx, const int32_t *zeta_dst, const int32_t *zeta_src, int32_t nz,
+                       const int32_t *ubar_dst, const int32_t *ubar_src, int32_t nu,
+                       const int32_t *vbar_dst, const int32_t *vbar_src, int32_t nv);
+
+/* Host-only check (no GPU work) of composed boundary maps against the fused
+ * substep's tiling, for a szI x szJ grid whose computeMomentumPred mask stride is
+ * q5_stride.  SF_OK if sf_ctx_set_bc_maps would accept them. */
+int sf_step2d_check_maps(int32_t szI, int32_t szJ, int32_t q5_stride,
+                         const int32_t *zeta_dst, const int32_t *zeta_src, int32_t nz,
+                         const int32_t *ubar_dst, const int32_t *ubar_src, int32_t nu,
+                         const int32_t *vbar_dst, const int32_t *vbar_src, int32_t nv);
+
+/* One barotropic substep: step2dPredictor + step2dCorrector
+ * (nonlinear/step2d.py:23-127) in one fused kernel plus a small kernel over the
+ * points that computeMomentumPred masks.  precision: SF_FP64 or SF_FP32.
+ * cff1 = weight1[iif-1]; clamp_val = value of clamped velocity boundaries. */
+int sf_step2d_substep(sf_ctx *ctx, const sf_step2d_fields *fields, int32_t precision,
+                      double dtfast, double cff1, double clamp_val, int32_t flags, void *stream);
+
+/* The two launches of sf_step2d_substep issued separately (parts: 1 = tiled kernel,
+ * 2 = masked-points kernel, 3 = both), so that a profiler or benchmark can put
+ * events between them.  Issuing 1 then 2 equals sf_step2d_substep. */
+int sf_step2d_substep_parts(sf_ctx *ctx, const sf_step2d_fields *fields, int32_t precision,
+                            double dtfast, double cff1, double clamp_val, int32_t flags,
+                            int32_t parts, void *stream);
+
+/* Tile shape (xi, eta) of the fused substep kernel. */
+int sf_step2d_tile_shape(int32_t *ti, int32_t *tj);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SAILFISH_HIP_H */

@@ -1,0 +1,56 @@
+// sf_internal.h -- shared definitions of the gfx950 kernels and the C ABI wrappers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/sailfish_hip.h"
+
+// Floating-point contraction is off for the whole library (also passed as
+// -ffp-contract=off): FP64 results must match the CPU restatement bit for bit.
+#pragma clang fp contract(off)
+
+struct sf_ctx {
+    int szI, szJ, szK;
+    long long S2;
+    const double *on_u, *om_v, *pn, *pm;
+    int device;
+    int q5_stride;            // mask stride of computeMomentumPred (szJ, or szI when fixed)
+    // dense boundary-code fields for the fused substep (one int32 per rho point),
+    // allocated by sf_ctx_set_bc_codes; see step2d_fused.hip
+    int32_t *bc_zeta, *bc_ubar, *bc_vbar;
+    int bc_ready;
+    int bc_ubar_has_clamp, bc_vbar_has_clamp;
+};
+
+namespace sf {
+
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+
+inline int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, what);
+    return SF_OK;
+}
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+constexpr int kBlock1D = 256;
+
+inline unsigned grid1d(long long n, int block = kBlock1D)
+{
+    return (unsigned)((n + block - 1) / block);
+}
+
+}  // namespace sf
+
+#define SF_REQUIRE(cond, ...)                 \
+    do {                                      \
+        if (!(cond)) {                        \
+            sf::set_error(__VA_ARGS__);       \
+            return SF_ERR_ARG;                \
+        }                                     \
+    } while (0)

@@ -1,0 +1,566 @@
+// step2d_fused.hip -- one barotropic substep (predictor + corrector + both boundary
+// passes of nonlinear/step2d.py:23-127) in one tiled kernel plus one small kernel
+// over the points that computeMomentumPred masks.
+//
+// Layout and tiling.  A workgroup of 256 threads owns a TI x TJ tile T of rho points
+// (TI along xi, the contiguous axis) and stages what it needs in LDS on the tile
+// plus a 2-point halo (R2); predictor quantities are recomputed on the 1-point halo
+// (R1) instead of being exchanged between workgroups.  Each HBM field is read or
+// written once per substep (halo re-reads hit L2), so the kernel is bound by HBM
+// bandwidth.  Workgroup ids are remapped so that each XCD walks a contiguous band
+// of tiles (its L2 then serves the halo re-reads).
+//
+// Stages (each followed by a barrier), with the reference code they restate:
+//   A  D1 = zeta_t1 + h, h, zeta_t0 on R2
+//   B  FX = (ubar_t1*RtoU(D1))*on_u, FY = (vbar_t1*RtoV(D1))*om_v  (mod_cppkernels.cpp:80-128)
+//   C  predicted zeta before BC on R1: zeta_t0 + Dt*rzeta          (cpp:133-150)
+//   D  zeta boundary map on R1                                      (zetabc.py:5-12)
+//   E  rhs_ubar/rhs_vbar (cpp:12-42), ubar/vbar (cpp:155-186) on T + right column /
+//      top row, velocity boundary map (barotropicVelocityBC.py:7-31)
+//   F  corrector on T: rzeta + averages (cpp:80-128), zeta AM3 (cpp:190-209)
+//   G  zeta boundary map on T (zetabc.py:5-12), stores.
+// The masked-points kernel then finishes the points computeMomentumPred skips
+// (i%szJ mask, cpp:162): their velocity boundary values, main3d's vbar_t2 := 0
+// (main3d.py:189), and the AM3 update that step2d.py:120-123 applies to ubar_t1
+// through the handle bound at mod_operators.py:146.  That update is kept only at
+// masked points: everywhere else the next predictor overwrites it before any read.
+//
+// Race freedom: inside the tiled kernel, every array that is read at halo points
+// (ubar_t1, vbar_t1, ubar_t2/vbar_t2 at masked points, rzeta_t1/rubar_t1/rvbar_t1 at
+// non-interior points) is either not written by the kernel or written only where no
+// other workgroup reads it.
+//
+// FP64 arithmetic keeps the reference's operand order with contraction off, so the
+// FP64 instance reproduces the one-kernel-per-reference-kernel sequence bit for bit.
+// The FP32 instance keeps the same order in float and accumulates Zt/DU/DV in FP64.
+//
+// Not written (never read by any kernel of the path): the corrector's momentum RHS
+// (rubar_t2, rvbar_t2) and the predictor's rvbar_t1.
+#include "sf_internal.h"
+#include "step2d_fused.h"
+
+namespace {
+
+constexpr int TI = SF_FUSED_TI;
+constexpr int TJ = SF_FUSED_TJ;
+constexpr int NT = 256;
+constexpr int W2 = TI + 4;           // R2 frame width
+constexpr int H2 = TJ + 4;
+constexpr int N2 = W2 * H2;
+
+// dense boundary-code values
+constexpr int kKeep = -1;
+constexpr int kZero = -2;
+constexpr int kClamp = -3;
+
+template <typename T>
+struct Fields {
+    const T *h, *on_u, *om_v, *pm, *pn;
+    const T *zeta_t0, *zeta_t1;
+    T *zeta_t2;
+    T *ubar_t1, *ubar_t2;
+    const T *vbar_t1;
+    T *vbar_t2;
+    const T *rzeta_t0;
+    T *rzeta_t1;
+    const T *rzeta_t2;
+    const T *rubar_t0;
+    T *rubar_t1;
+    const T *rvbar_t1;
+    double *Zt_avg1, *DU_avg1, *DV_avg1;
+    const int32_t *bc_zeta, *bc_ubar, *bc_vbar;
+};
+
+struct Params {
+    int szI, szJ, q5_stride;
+    int tiles_i, tiles_j;
+    int off_i, off_j;        // tile origin offset (the first tile starts at -off)
+    long long S2;
+    double Dt, cff1, clamp;
+    int zero_vbar;
+};
+
+__device__ __forceinline__ int lidx(int li, int lj) { return lj * W2 + li; }
+
+// AdamsMoultonCorr3rd (cpp:190-209): x + Dt*(AM3_2*r2 + AM3_1*r1 + AM3_0*r0)
+template <typename T>
+__device__ __forceinline__ T am3(T x, T Dt, T r2, T r1, T r0)
+{
+    const T AM3_2 = T(5.0 / 12.0), AM3_1 = T(8.0 / 12.0), AM3_0 = T(-1.0 / 12.0);
+    return x + Dt * (AM3_2 * r2 + AM3_1 * r1 + AM3_0 * r0);
+}
+
+__device__ __forceinline__ bool is_masked(long long f, int stride)
+{
+    return (f % stride == 0) || (f / stride == 0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
+{
+    __shared__ T sD1[N2];    // zeta_t1 + h                                  (R2)
+    __shared__ T sH[N2];     // h                                            (R2)
+    __shared__ T sZ0[N2];    // zeta_t0                                      (R2)
+    __shared__ T sA[N2];     // B-C: FX        E-G: predicted ubar after BC
+    __shared__ T sB[N2];     // B-C: FY        E-G: predicted vbar after BC
+    __shared__ T sC[N2];     // C-D: predicted zeta before BC   F-G: corrected zeta before BC
+    __shared__ T sZp[N2];    // predicted zeta after BC                      (R1)
+    T *const sFX = sA, *const sUp = sA;
+    T *const sFY = sB, *const sVp = sB;
+    T *const sZq = sC, *const sZc = sC;
+
+    const int szI = P.szI, szJ = P.szJ;
+    const int nblk = P.tiles_i * P.tiles_j;
+    int b = blockIdx.x;
+    if ((nblk & 7) == 0) b = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
+    const int ti = b % P.tiles_i, tj = b / P.tiles_i;
+    const int i0 = ti * TI - P.off_i, j0 = tj * TJ - P.off_j;
+    const int gi0 = i0 - 2, gj0 = j0 - 2;   // global coords of the R2 frame origin
+    const int tid = threadIdx.x;
+    const T Dt = T(P.Dt);
+
+    auto inside = [&](int i, int j) { return i >= 0 && i < szI && j >= 0 && j < szJ; };
+    auto interior = [&](int i, int j) { return i > 0 && i < szI - 1 && j > 0 && j < szJ - 1; };
+    auto band = [&](int i, int j) { return i <= 1 || i >= szI - 2 || j <= 1 || j >= szJ - 2; };
+    auto in_tile = [&](int i, int j) { return i >= i0 && i < i0 + TI && j >= j0 && j < j0 + TJ; };
+
+    // ---- A -----------------------------------------------------------------------
+    for (int r = tid; r < N2; r += NT) {
+        const int li = r % W2, lj = r / W2;
+        const int i = gi0 + li, j = gj0 + lj;
+        T d1 = T(0), hh = T(0), z0 = T(0);
+        if (inside(i, j)) {
+            const long long f = (long long)j * szI + i;
+            hh = F.h[f];
+            d1 = F.zeta_t1[f] + hh;
+            z0 = F.zeta_t0[f];
+        }
+        sD1[r] = d1;
+        sH[r] = hh;
+        sZ0[r] = z0;
+    }
+    __syncthreads();
+
+    // ---- B: face fluxes on the R2 frame minus its first row/column -----------------
+    for (int r = tid; r < N2; r += NT) {
+        const int li = r % W2, lj = r / W2;
+        const int i = gi0 + li, j = gj0 + lj;
+        T fx = T(0), fy = T(0);
+        if (li >= 1 && lj >= 1 && inside(i, j) && i >= 1 && j >= 1) {
+            const long long f = (long long)j * szI + i;
+            const T D = sD1[r];
+            fx = (F.ubar_t1[f] * ((D + sD1[r - 1]) * T(0.5))) * F.on_u[f];
+            fy = (F.vbar_t1[f] * ((D + sD1[r - W2]) * T(0.5))) * F.om_v[f];
+        }
+        sFX[r] = fx;
+        sFY[r] = fy;
+    }
+    __syncthreads();
+
+    // ---- C: predicted zeta before BC on R1 ---------------------------------------
+    for (int r = tid; r < N2; r += NT) {
+        const int li = r % W2, lj = r / W2;
+        if (li < 1 || li > TI + 2 || lj < 1 || lj > TJ + 2) continue;
+        const int i = gi0 + li, j = gj0 + lj;
+        T zq = T(0);
+        if (inside(i, j)) {
+            const long long f = (long long)j * szI + i;
+            if (i == 0 || j == 0) {
+                // computeZetaPred leaves column 0 and row 0 alone: the buffer keeps its
+                // value.  Outside the own tile such points are never used (they are
+                // boundary-map destinations, never sources), so they are not read.
+                if (in_tile(i, j)) zq = F.zeta_t2[f];
+            } else {
+                T rz;
+                if (interior(i, j))
+                    rz = (((sFX[r + 1] - sFX[r]) + (sFY[r + W2] - sFY[r])) * F.pm[f]) * F.pn[f];
+                else
+                    rz = F.rzeta_t1[f];   // computeZetaRHS never writes non-interior points
+                zq = sZ0[r] + Dt * rz;
+            }
+        }
+        sZq[r] = zq;
+    }
+    __syncthreads();
+
+    // ---- D: zeta boundary map on R1 -----------------------------------------------
+    for (int r = tid; r < N2; r += NT) {
+        const int li = r % W2, lj = r / W2;
+        if (li < 1 || li > TI + 2 || lj < 1 || lj > TJ + 2) continue;
+        const int i = gi0 + li, j = gj0 + lj;
+        T z = sZq[r];
+        if (inside(i, j) && band(i, j)) {
+            const int s = F.bc_zeta[(long long)j * szI + i];
+            if (s >= 0) z = sZq[lidx(s % szI - gi0, s / szI - gj0)];
+        }
+        sZp[r] = z;
+    }
+    __syncthreads();
+
+    // ---- E: momentum RHS and predicted velocities on [i0, i0+TI] x [j0, j0+TJ] ----
+    const T w = T(2.0 / 5.0);
+    const T a = T(1) - w;
+    const T gc = T(10.0);
+    for (int r = tid; r < (TI + 1) * (TJ + 1); r += NT) {
+        const int ri = r % (TI + 1), rj = r / (TI + 1);
+        const int i = i0 + ri, j = j0 + rj;
+        const int l = lidx(ri + 2, rj + 2);
+        T up = T(0), vp = T(0);
+        if (inside(i, j)) {
+            const long long f = (long long)j * szI + i;
+            if (!is_masked(f, P.q5_stride)) {
+                T ru, rv;
+                if (interior(i, j)) {
+                    const T gz = a * sZp[l] + w * sZ0[l];
+                    const T gzl = a * sZp[l - 1] + w * sZ0[l - 1];
+                    const T gzd = a * sZp[l - W2] + w * sZ0[l - W2];
+                    const T hu = (sH[l] + sH[l - 1]) * T(0.5);
+                    const T hv = (sH[l] + sH[l - W2]) * T(0.5);
+                    ru = (gc * F.on_u[f]) * (hu * (gz - gzl) + T(0.5) * (gz * gz - gzl * gzl));
+                    rv = (gc * F.om_v[f]) * (hv * (gz - gzd) + T(0.5) * (gz * gz - gzd * gzd));
+                    if (ri < TI && rj < TJ) F.rubar_t1[f] = ru;
+                } else {
+                    ru = F.rubar_t1[f];   // never written at non-interior points
+                    rv = F.rvbar_t1[f];
+                }
+                const T D1 = sD1[l], D2 = sZp[l] + sH[l];
+                const T D1l = sD1[l - 1], D2l = sZp[l - 1] + sH[l - 1];
+                const T D1d = sD1[l - W2], D2d = sZp[l - W2] + sH[l - W2];
+                const T pm = F.pm[f], pn = F.pn[f];
+                up = (F.ubar_t1[f] * ((D1 + D1l) * T(0.5)) + ((Dt * ru) * pm) * pn) / ((D2 + D2l) * T(0.5));
+                vp = (F.vbar_t1[f] * ((D1 + D1d) * T(0.5)) + ((Dt * rv) * pm) * pn) / ((D2 + D2d) * T(0.5));
+            } else {
+                // masked: the t2 buffer keeps its value (the masked-points kernel
+                // finalises these points after this kernel)
+                up = F.ubar_t2[f];
+                vp = F.vbar_t2[f];
+                if (interior(i, j) && ri < TI && rj < TJ) {
+                    const T gz = a * sZp[l] + w * sZ0[l];
+                    const T gzl = a * sZp[l - 1] + w * sZ0[l - 1];
+                    const T hu = (sH[l] + sH[l - 1]) * T(0.5);
+                    F.rubar_t1[f] = (gc * F.on_u[f]) * (hu * (gz - gzl) + T(0.5) * (gz * gz - gzl * gzl));
+                }
+            }
+            if (band(i, j)) {
+                const int cu = F.bc_ubar[f], cv = F.bc_vbar[f];
+                if (cu == kZero) up = T(0);
+                else if (cu == kClamp) up = T(P.clamp);
+                if (cv == kZero) vp = T(0);
+                else if (cv == kClamp) vp = T(P.clamp);
+            }
+        }
+        sUp[l] = up;
+        sVp[l] = vp;
+    }
+    __syncthreads();
+
+    // ---- F: corrector on T -------------------------------------------------------
+    const double cff1 = P.cff1;
+    for (int r = tid; r < TI * TJ; r += NT) {
+        const int ri = r % TI, rj = r / TI;
+        const int i = i0 + ri, j = j0 + rj;
+        if (!inside(i, j)) continue;
+        const int l = lidx(ri + 2, rj + 2);
+        const long long f = (long long)j * szI + i;
+        T rz;
+        if (interior(i, j)) {
+            const T D = sZp[l] + sH[l];
+            const T Dl = sZp[l - 1] + sH[l - 1];
+            const T Dd = sZp[l - W2] + sH[l - W2];
+            const T Dr = sZp[l + 1] + sH[l + 1];
+            const T Du = sZp[l + W2] + sH[l + W2];
+            const T DU = sUp[l] * ((D + Dl) * T(0.5));
+            const T DUr = sUp[l + 1] * ((Dr + D) * T(0.5));
+            const T DV = sVp[l] * ((D + Dd) * T(0.5));
+            const T DVu = sVp[l + W2] * ((Du + D) * T(0.5));
+            const T onu = F.on_u[f], omv = F.om_v[f];
+            rz = (((DUr * F.on_u[f + 1] - DU * onu) + (DVu * F.om_v[f + szI] - DV * omv)) * F.pm[f]) * F.pn[f];
+            F.rzeta_t1[f] = rz;
+            F.Zt_avg1[f] = F.Zt_avg1[f] + cff1 * (double)sZp[l];
+            F.DU_avg1[f] = F.DU_avg1[f] + (cff1 * (double)DU) * (double)onu;
+            F.DV_avg1[f] = F.DV_avg1[f] + (cff1 * (double)DV) * (double)omv;
+        } else {
+            rz = F.rzeta_t1[f];
+        }
+        sZc[l] = am3(sZp[l], Dt, F.rzeta_t2[f], rz, F.rzeta_t0[f]);
+    }
+    __syncthreads();
+
+    // ---- G: zeta boundary map on T, stores ------------------------------------------
+    for (int r = tid; r < TI * TJ; r += NT) {
+        const int ri = r % TI, rj = r / TI;
+        const int i = i0 + ri, j = j0 + rj;
+        if (!inside(i, j)) continue;
+        const int l = lidx(ri + 2, rj + 2);
+        const long long f = (long long)j * szI + i;
+        T z = sZc[l];
+        if (band(i, j)) {
+            const int s = F.bc_zeta[f];
+            if (s >= 0) z = sZc[lidx(s % szI - gi0, s / szI - gj0)];
+        }
+        F.zeta_t2[f] = z;
+        if (!is_masked(f, P.q5_stride)) {
+            F.ubar_t2[f] = sUp[l];
+            F.vbar_t2[f] = P.zero_vbar ? T(0) : sVp[l];
+        }
+    }
+}
+
+// Masked points (f%stride == 0 or f < stride): velocity boundary constants,
+// main3d's vbar_t2 := 0, and the AM3 update of ubar_t1 from step2d.py:120-123.
+template <typename T>
+__global__ __launch_bounds__(256) void k_step2d_masked(Fields<T> F, Params P)
+{
+    const long long ncol = (P.S2 + P.q5_stride - 1) / P.q5_stride;   // f = k*stride
+    const long long n = ncol + (P.q5_stride - 1);                     // plus f = 1..stride-1
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const long long f = k < ncol ? k * P.q5_stride : k - ncol + 1;
+    if (f >= P.S2) return;
+    const int szI = P.szI, szJ = P.szJ;
+    const int i = (int)(f % szI), j = (int)(f / szI);
+    const bool band = i <= 1 || i >= szI - 2 || j <= 1 || j >= szJ - 2;
+    T up = F.ubar_t2[f], vp = F.vbar_t2[f];
+    if (band) {
+        const int cu = F.bc_ubar[f], cv = F.bc_vbar[f];
+        if (cu == kZero) up = T(0);
+        else if (cu == kClamp) up = T(P.clamp);
+        if (cv == kZero) vp = T(0);
+        else if (cv == kClamp) vp = T(P.clamp);
+    }
+    F.ubar_t2[f] = up;
+    F.vbar_t2[f] = P.zero_vbar ? T(0) : vp;
+    F.ubar_t1[f] = am3(F.ubar_t1[f], T(P.Dt), F.rubar_t1[f], F.rubar_t0[f], F.vbar_t1[f]);
+}
+
+// Tile origin: shift the tiling by one point where the last tile would otherwise be
+// a single row or column (the boundary maps copy from the neighbouring row/column,
+// which must then lie in the same tile).
+void tile_origin(int szI, int szJ, int *off_i, int *off_j)
+{
+    *off_i = (szI % TI == 1) ? 1 : 0;
+    *off_j = (szJ % TJ == 1) ? 1 : 0;
+}
+
+template <typename T>
+int launch(sf_ctx *c, const sf_step2d_fields *a, double dtfast, double cff1, double clamp, int flags,
+           int parts, hipStream_t s)
+{
+    Fields<T> F;
+    F.h = (const T *)a->h;
+    F.on_u = (const T *)a->on_u;
+    F.om_v = (const T *)a->om_v;
+    F.pm = (const T *)a->pm;
+    F.pn = (const T *)a->pn;
+    F.zeta_t0 = (const T *)a->zeta[0];
+    F.zeta_t1 = (const T *)a->zeta[1];
+    F.zeta_t2 = (T *)a->zeta[2];
+    F.ubar_t1 = (T *)a->ubar[1];
+    F.ubar_t2 = (T *)a->ubar[2];
+    F.vbar_t1 = (const T *)a->vbar[1];
+    F.vbar_t2 = (T *)a->vbar[2];
+    F.rzeta_t0 = (const T *)a->rzeta[0];
+    F.rzeta_t1 = (T *)a->rzeta[1];
+    F.rzeta_t2 = (const T *)a->rzeta[2];
+    F.rubar_t0 = (const T *)a->rubar[0];
+    F.rubar_t1 = (T *)a->rubar[1];
+    F.rvbar_t1 = (const T *)a->rvbar[1];
+    F.Zt_avg1 = a->Zt_avg1;
+    F.DU_avg1 = a->DU_avg1;
+    F.DV_avg1 = a->DV_avg1;
+    F.bc_zeta = c->bc_zeta;
+    F.bc_ubar = c->bc_ubar;
+    F.bc_vbar = c->bc_vbar;
+    Params P;
+    P.szI = c->szI;
+    P.szJ = c->szJ;
+    P.q5_stride = c->q5_stride;
+    tile_origin(c->szI, c->szJ, &P.off_i, &P.off_j);
+    P.tiles_i = (c->szI + P.off_i + TI - 1) / TI;
+    P.tiles_j = (c->szJ + P.off_j + TJ - 1) / TJ;
+    P.S2 = c->S2;
+    P.Dt = dtfast;
+    P.cff1 = cff1;
+    P.clamp = clamp;
+    P.zero_vbar = (flags & SF_SUBSTEP_ZERO_VBAR) ? 1 : 0;
+    if (parts & 1) {
+        hipLaunchKernelGGL(k_step2d_fused<T>, dim3(P.tiles_i * P.tiles_j), dim3(NT), 0, s, F, P);
+        int rc = sf::check_launch("sf_step2d_substep (tiles)");
+        if (rc != SF_OK) return rc;
+    }
+    if (parts & 2) {
+        const long long nm = (c->S2 + c->q5_stride - 1) / c->q5_stride + c->q5_stride - 1;
+        hipLaunchKernelGGL(k_step2d_masked<T>, dim3(sf::grid1d(nm)), dim3(256), 0, s, F, P);
+        return sf::check_launch("sf_step2d_substep (masked points)");
+    }
+    return SF_OK;
+}
+
+__global__ void k_fill_i32(int32_t *p, long long n, int32_t v)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+__global__ void k_scatter_codes(int32_t *p, const int32_t *dst, const int32_t *src, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int s = src[k];
+    p[dst[k]] = s >= 0 ? s : (s == -1 ? kZero : kClamp);
+}
+
+}  // namespace
+
+// Host-side validation of the composed maps against what the tiled kernel assumes.
+extern "C" int sf_step2d_check_maps(int32_t szI, int32_t szJ, int32_t q5_stride, const int32_t *zd,
+                                    const int32_t *zs, int32_t nz, const int32_t *ud, const int32_t *us,
+                                    int32_t nu, const int32_t *vd, const int32_t *vs, int32_t nv)
+{
+    SF_REQUIRE(szI >= 3 && szJ >= 3 && q5_stride > 0, "sf_step2d_check_maps: bad sizes");
+    SF_REQUIRE((nz == 0 || (zd && zs)) && (nu == 0 || (ud && us)) && (nv == 0 || (vd && vs)),
+               "sf_step2d_check_maps: null map array");
+    const long long S2 = (long long)szI * szJ;
+    int off_i, off_j;
+    tile_origin(szI, szJ, &off_i, &off_j);
+    auto tix = [&](int i) { return (i + off_i) / TI; };
+    auto tjx = [&](int j) { return (j + off_j) / TJ; };
+    auto band = [&](int i, int j) { return i <= 1 || i >= szI - 2 || j <= 1 || j >= szJ - 2; };
+    for (int k = 0; k < nz; k++) {
+        const int d = zd[k], s = zs[k];
+        if (d < 0 || d >= S2 || !band(d % szI, d / szI)) {
+            sf::set_error("zeta map destination %d is not on the boundary band", d);
+            return SF_ERR_UNSUPPORTED;
+        }
+        if (s < 0 || s >= S2) {
+            sf::set_error("zeta map entry %d -> %d: constants are not supported by the fused substep", d, s);
+            return SF_ERR_UNSUPPORTED;
+        }
+        const int di = d % szI, dj = d / szI, si = s % szI, sj = s / szI;
+        if (abs(si - di) > 1 || abs(sj - dj) > 1 || tix(si) != tix(di) || tjx(sj) != tjx(dj)) {
+            sf::set_error("zeta map source %d of %d is not a neighbour in the same %dx%d tile", s, d, TI, TJ);
+            return SF_ERR_UNSUPPORTED;
+        }
+        // (a source on column 0 or row 0, e.g. (0,L) <- (0,L-1), is read only inside its
+        // own tile; halo copies of such destinations are never consumed)
+        // every tile whose 1-point halo contains the destination must also hold the
+        // source on its halo
+        const int ntj = (szJ + off_j + TJ - 1) / TJ, nti = (szI + off_i + TI - 1) / TI;
+        for (int tj = tjx(dj > 0 ? dj - 1 : 0); tj <= tjx(dj + 1) && tj < ntj; tj++)
+            for (int ti = tix(di > 0 ? di - 1 : 0); ti <= tix(di + 1) && ti < nti; ti++) {
+                const int x0 = ti * TI - off_i, y0 = tj * TJ - off_j;
+                const int a0 = x0 - 1, a1 = x0 + TI, b0 = y0 - 1, b1 = y0 + TJ;
+                const bool dIn = di >= a0 && di <= a1 && dj >= b0 && dj <= b1;
+                const bool sIn = si >= a0 && si <= a1 && sj >= b0 && sj <= b1;
+                if (dIn && !sIn) {
+                    sf::set_error("zeta map source %d of %d falls outside the halo of tile (%d,%d)", s, d, ti, tj);
+                    return SF_ERR_UNSUPPORTED;
+                }
+            }
+    }
+    const int32_t *dd[2] = {ud, vd}, *ss[2] = {us, vs};
+    const int nn[2] = {nu, nv};
+    for (int q = 0; q < 2; q++)
+        for (int k = 0; k < nn[q]; k++) {
+            const int d = dd[q][k];
+            if (d < 0 || d >= S2 || !band(d % szI, d / szI) || ss[q][k] >= 0) {
+                sf::set_error("velocity map entry %d is not a boundary constant (the fused substep supports "
+                              "closed/clamped velocity boundaries only)", d);
+                return SF_ERR_UNSUPPORTED;
+            }
+        }
+    if (q5_stride != szI && (szI % q5_stride) != 0) {
+        sf::set_error("fused substep: the i%%szJ mask admits column-0 points on a %dx%d grid; "
+                      "use the one-kernel-per-step path or SF_FLAG_FIX_Q5_MASK", szI, szJ);
+        return SF_ERR_UNSUPPORTED;
+    }
+    return SF_OK;
+}
+
+extern "C" int sf_ctx_set_bc_maps(sf_ctx *c, const int32_t *zeta_dst, const int32_t *zeta_src, int32_t nz,
+                                  const int32_t *ubar_dst, const int32_t *ubar_src, int32_t nu,
+                                  const int32_t *vbar_dst, const int32_t *vbar_src, int32_t nv)
+{
+    SF_REQUIRE(c, "sf_ctx_set_bc_maps: null context");
+    SF_REQUIRE(nz >= 0 && nu >= 0 && nv >= 0, "sf_ctx_set_bc_maps: negative size");
+    SF_REQUIRE((nz == 0 || (zeta_dst && zeta_src)) && (nu == 0 || (ubar_dst && ubar_src)) &&
+                   (nv == 0 || (vbar_dst && vbar_src)),
+               "sf_ctx_set_bc_maps: null map array");
+    c->bc_ready = 0;
+    int rc = sf_step2d_check_maps(c->szI, c->szJ, c->q5_stride, zeta_dst, zeta_src, nz, ubar_dst, ubar_src, nu,
+                                  vbar_dst, vbar_src, nv);
+    if (rc != SF_OK) return rc;
+    hipError_t e;
+    int32_t **fields[3] = {&c->bc_zeta, &c->bc_ubar, &c->bc_vbar};
+    for (auto p : fields)
+        if (!*p) {
+            e = hipMalloc((void **)p, sizeof(int32_t) * c->S2);
+            if (e != hipSuccess) return sf::hip_fail(e, "sf_ctx_set_bc_maps: hipMalloc");
+        }
+    const int32_t *ds[3] = {zeta_dst, ubar_dst, vbar_dst}, *sr[3] = {zeta_src, ubar_src, vbar_src};
+    const int n[3] = {nz, nu, nv};
+    int32_t *tmp = nullptr;
+    const int nmax = nz > nu ? (nz > nv ? nz : nv) : (nu > nv ? nu : nv);
+    if (nmax > 0) {
+        e = hipMalloc((void **)&tmp, sizeof(int32_t) * 2 * nmax);
+        if (e != hipSuccess) return sf::hip_fail(e, "sf_ctx_set_bc_maps: hipMalloc");
+    }
+    for (int q = 0; q < 3; q++) {
+        int32_t *dptr = *fields[q];
+        hipLaunchKernelGGL(k_fill_i32, dim3(sf::grid1d(c->S2)), dim3(256), 0, 0, dptr, c->S2, kKeep);
+        if (n[q] > 0) {
+            e = hipMemcpy(tmp, ds[q], sizeof(int32_t) * n[q], hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(tmp + nmax, sr[q], sizeof(int32_t) * n[q], hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                hipFree(tmp);
+                return sf::hip_fail(e, "sf_ctx_set_bc_maps: copy");
+            }
+            hipLaunchKernelGGL(k_scatter_codes, dim3(sf::grid1d(n[q])), dim3(256), 0, 0, dptr, tmp, tmp + nmax, n[q]);
+            e = hipDeviceSynchronize();
+            if (e != hipSuccess) {
+                hipFree(tmp);
+                return sf::hip_fail(e, "sf_ctx_set_bc_maps: scatter");
+            }
+        }
+    }
+    e = hipDeviceSynchronize();
+    if (tmp) hipFree(tmp);
+    if (e != hipSuccess) return sf::hip_fail(e, "sf_ctx_set_bc_maps");
+    c->bc_ready = 1;
+    return SF_OK;
+}
+
+extern "C" int sf_step2d_substep_parts(sf_ctx *c, const sf_step2d_fields *a, int32_t precision,
+                                       double dtfast, double cff1, double clamp_val, int32_t flags,
+                                       int32_t parts, void *stream)
+{
+    SF_REQUIRE(parts >= 1 && parts <= 3, "sf_step2d_substep_parts: parts=%d not in 1..3", parts);
+    SF_REQUIRE(c && a, "sf_step2d_substep: null argument");
+    SF_REQUIRE(c->bc_ready, "sf_step2d_substep: boundary maps not set (sf_ctx_set_bc_maps)");
+    SF_REQUIRE(a->h && a->on_u && a->om_v && a->pm && a->pn, "sf_step2d_substep: null grid field");
+    for (int k = 0; k < 3; k++)
+        SF_REQUIRE(a->zeta[k] && a->rzeta[k], "sf_step2d_substep: null zeta/rzeta slot");
+    SF_REQUIRE(a->ubar[1] && a->ubar[2] && a->vbar[1] && a->vbar[2] && a->rubar[0] && a->rubar[1] &&
+                   a->rvbar[1] && a->Zt_avg1 && a->DU_avg1 && a->DV_avg1,
+               "sf_step2d_substep: null state field");
+    if (precision == SF_FP64)
+        return launch<double>(c, a, dtfast, cff1, clamp_val, flags, parts, sf::as_stream(stream));
+    if (precision == SF_FP32)
+        return launch<float>(c, a, dtfast, cff1, clamp_val, flags, parts, sf::as_stream(stream));
+    sf::set_error("sf_step2d_substep: unknown precision %d", precision);
+    return SF_ERR_ARG;
+}
+
+extern "C" int sf_step2d_substep(sf_ctx *c, const sf_step2d_fields *a, int32_t precision, double dtfast,
+                                 double cff1, double clamp_val, int32_t flags, void *stream)
+{
+    return sf_step2d_substep_parts(c, a, precision, dtfast, cff1, clamp_val, flags, 3, stream);
+}
+
+extern "C" int sf_step2d_tile_shape(int32_t *ti, int32_t *tj)
+{
+    SF_REQUIRE(ti && tj, "sf_step2d_tile_shape: null argument");
+    *ti = TI;
+    *tj = TJ;
+    return SF_OK;
+}
