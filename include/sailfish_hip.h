@@ -115,6 +115,56 @@ int sf_bc_set(double *var, const int32_t *idx, int32_t n, double val, void *stre
 int sf_bc_apply_map(double *var, const int32_t *dst, const int32_t *src, int32_t n,
                     double clamp_val, void *stream);
 
+/* ---------------- 3D kernels, one to one, FP64 ------------------------------------
+ * 3D arrays are flat (level, eta, xi) with level stride szI*szJ: u, v, ru, rv, W,
+ * Huon, Hvom, Akv, z_w have szK+1 levels, Hz and z_r have szK.  szK >= 4. */
+
+/* addCoriolis (nonlinear/rhs_kernels.cpp:6-35). */
+int sf_add_coriolis(sf_ctx *ctx, const double *fomn, const double *u, const double *v,
+                    double *ru, double *rv, const double *Hz, void *stream);
+
+/* horizontalAdvection (nonlinear/rhs_kernels.cpp:70-132).  launch_threads is the
+ * thread count of the reference launch ((S2/512+1)*512, nonlinear/rhs.py:62): its
+ * threads past S2 store 0 into ru/rv at their flat index, which is reproduced after
+ * the columns are done. */
+int sf_horizontal_advection(sf_ctx *ctx, const double *u, const double *v, const double *Huon,
+                            const double *Hvom, double *ru, double *rv, int64_t launch_threads,
+                            void *stream);
+
+/* verticalAdvection (nonlinear/rhs_kernels.cpp:135-197); BC = zeta bcIdxFieldIdx2. */
+int sf_vertical_advection(sf_ctx *ctx, const double *u, const double *v, const double *W,
+                          double *ru, double *rv, const int32_t *BC, void *stream);
+
+/* set_maxflux (nonlinear/step3D_uv.kernels.cpp:372-405). */
+int sf_set_maxflux(sf_ctx *ctx, const double *u, const double *v, double *Huon, double *Hvom,
+                   const double *Hz, void *stream);
+
+/* omega (nonlinear/step3D_uv.kernels.cpp:62-165); BC = zeta bcIdxFieldIdx2. */
+int sf_omega(sf_ctx *ctx, double *W, const double *Huon, const double *Hvom, const double *z_w,
+             const int32_t *BC, void *stream);
+
+/* set_depth (nonlinear/step3D_uv.kernels.cpp:225-291). */
+int sf_set_depth(sf_ctx *ctx, int32_t Vtransform, const double *Zt_avg1, double *z_w, double *z_r,
+                 const double *h, double hc, double *Hz, const double *sc_r, const double *sc_w,
+                 const double *Cs_r, const double *Cs_w, void *stream);
+
+/* set_zeta (nonlinear/step3D_uv.kernels.cpp:294-310). */
+int sf_set_zeta(sf_ctx *ctx, double *zeta_t1, double *zeta_t2, const double *Zt_avg1, void *stream);
+
+/* setLateralUVBCs (nonlinear/step3D_uv.kernels.cpp:314-368) for closed/clamped walls
+ * (uType/vType = bcIdxFieldType of the u and v index sets). */
+int sf_set_lateral_uv_bcs(sf_ctx *ctx, double t, double *u, double *v, const int32_t *uType,
+                          const int32_t *vType, void *stream);
+
+/* step3d_UV (nonlinear/step3D_uv.kernels.cpp:646-707) with its scratch arrays SD, D,
+ * RHS ((szK+1)*S2 doubles each, nonlinear/main3d.py:48-58). */
+int sf_step3d_uv(sf_ctx *ctx, double *u, double *v, const double *ru, const double *rv,
+                 double *ubar_t1, double *vbar_t1, double *ubar_t2, double *vbar_t2,
+                 const double *Hz, const double *Akv, const double *z_r, const double *DU_avg1,
+                 const double *DV_avg1, const double *tmpU, const double *tmpV, int32_t iic,
+                 int32_t ntfirst, double lambda, double Dt, double *SD, double *D, double *RHS,
+                 void *stream);
+
 /* ---------------- fused barotropic substep --------------------------------------- */
 
 /* Pointers of one substep, after the caller's time-level rotation (cycleTimes2D,

@@ -1,0 +1,456 @@
+/*
+ * sf_oracle3d.c -- CPU restatement (FP64) of the 3D kernels of the Sailfish ROMS port
+ * (nonlinear/rhs_kernels.cpp, nonlinear/step3D_uv.kernels.cpp), written from scratch.
+ *
+ * TEST INFRASTRUCTURE ONLY (see sf_oracle2d.c).
+ *
+ * 3D fields are flat arrays with level stride S2 = szI*szJ; a stencil offset
+ * (dk, dj, di) from column f at level K reads index (K+dk)*S2 + f + dj*szI + di with
+ * no wrapping rules beyond the flat arithmetic itself (the reference's Eval,
+ * mod_cppkernels.h:337-340).  Reads outside [0, n) of an array of n values are
+ * undefined in the reference; they return 0.0 here (r3()).
+ *
+ * Schedules fixed here where the reference races:
+ *  - horizontalAdvection: threads i >= S2 of the launch write 0 into ru/rv at flat
+ *    i (rhs_kernels.cpp:86-91), i.e. into level 1 of the first npad columns, where
+ *    npad = launch threads - S2; those writes are applied after all columns.
+ *  - step3d_UV's V pass at U-but-not-V nodes reuses the U pass's eliminated
+ *    tridiagonal system (solveTri guards with isUNode, step3D_uv.kernels.cpp:525);
+ *    each column owns its scratch, as in the reference.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct sfo_grid3 {
+    int szI, szJ, N;
+    const double *on_u, *om_v, *pn, *pm;
+} sfo_grid3;
+
+typedef struct {
+    const double *p;
+    ptrdiff_t n;
+} arr_t;
+
+static inline double r3(arr_t a, ptrdiff_t idx) { return (idx < 0 || idx >= a.n) ? 0.0 : a.p[idx]; }
+
+#define S2_OF(g) ((ptrdiff_t)(g)->szI * (g)->szJ)
+
+static inline int isRNode(const sfo_grid3 *g, ptrdiff_t f)
+{
+    const int i = (int)(f % g->szI), j = (int)(f / g->szI);
+    return i > 0 && j > 0;
+}
+static inline int isUNode(const sfo_grid3 *g, ptrdiff_t f)
+{
+    const int i = (int)(f % g->szI), j = (int)(f / g->szI);
+    return i > 1 && j > 0;
+}
+static inline int isVNode(const sfo_grid3 *g, ptrdiff_t f)
+{
+    const int i = (int)(f % g->szI), j = (int)(f / g->szI);
+    return i > 0 && j > 1 && j < g->szJ - 1;
+}
+static inline int isInnerCell(const sfo_grid3 *g, ptrdiff_t f)
+{
+    const int i = (int)(f % g->szI), j = (int)(f / g->szI);
+    return i > 1 && i < g->szI - 2 && j > 1 && j < g->szJ - 2;
+}
+
+/* ---------------------------------------------------------------------------
+ * addCoriolis (rhs_kernels.cpp:6-35), R nodes, K = 0..N-1:
+ *   UF = (Hz*fomn)*VtoR(v); VF = (Hz*fomn)*UtoR(u); ru += RtoU(UF); rv -= RtoV(VF)
+ * ------------------------------------------------------------------------- */
+static double cor_UF(const sfo_grid3 *g, arr_t fomn, arr_t v, arr_t Hz, ptrdiff_t K, ptrdiff_t f)
+{
+    const ptrdiff_t S2 = S2_OF(g), b = K * S2 + f;
+    return (r3(Hz, b) * r3(fomn, f)) * ((r3(v, b) + r3(v, b + g->szI)) * 0.5);
+}
+static double cor_VF(const sfo_grid3 *g, arr_t fomn, arr_t u, arr_t Hz, ptrdiff_t K, ptrdiff_t f)
+{
+    const ptrdiff_t S2 = S2_OF(g), b = K * S2 + f;
+    return (r3(Hz, b) * r3(fomn, f)) * ((r3(u, b) + r3(u, b + 1)) * 0.5);
+}
+
+void sfo_addCoriolis(const sfo_grid3 *g, const double *fomn, const double *u, const double *v,
+                     double *ru, double *rv, const double *Hz)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N, n3 = (N + 1) * S2;
+    arr_t F = {fomn, S2}, U = {u, n3}, V = {v, n3}, H = {Hz, N * S2};
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        if (!isRNode(g, f)) continue;
+        for (ptrdiff_t K = 0; K < N; K++) {
+            const ptrdiff_t b = K * S2 + f;
+            const double rtou = (cor_UF(g, F, V, H, K, f) + cor_UF(g, F, V, H, K, f - 1)) * 0.5;
+            const double rtov = (cor_VF(g, F, U, H, K, f) + cor_VF(g, F, U, H, K, f - g->szI)) * 0.5;
+            ru[b] = ru[b] + rtou;
+            rv[b] = rv[b] - rtov;
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * horizontalAdvection (rhs_kernels.cpp:70-132), every column, K = 0..N-1.
+ * Offsets below are flat (dj*szI + di) at level K.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    arr_t u, v, Hu, Hv;
+    ptrdiff_t szI;
+} hadv_t;
+
+#define AT(a, o) r3((a), base + (o))
+static double dxx(arr_t a, ptrdiff_t base, ptrdiff_t o) { return AT(a, o - 1) - 2.0 * AT(a, o) + AT(a, o + 1); }
+static double dee(arr_t a, ptrdiff_t base, ptrdiff_t o, ptrdiff_t W) { return AT(a, o - W) - 2.0 * AT(a, o) + AT(a, o + W); }
+
+/* the eight flux expressions, evaluated at flat offset o from base (level K) */
+static double h_uR(const hadv_t *h, ptrdiff_t base, ptrdiff_t o) { return (AT(h->u, o) + AT(h->u, o + 1)) * 0.5; }
+static double h_uP(const hadv_t *h, ptrdiff_t base, ptrdiff_t o) { return (AT(h->u, o) + AT(h->u, o - 1)) * 0.5; }
+static double h_vR(const hadv_t *h, ptrdiff_t base, ptrdiff_t o) { return (AT(h->v, o) + AT(h->v, o + h->szI)) * 0.5; }
+static double h_vP(const hadv_t *h, ptrdiff_t base, ptrdiff_t o) { return (AT(h->v, o) + AT(h->v, o - h->szI)) * 0.5; }
+static double h_HuR(const hadv_t *h, ptrdiff_t base, ptrdiff_t o) { return (AT(h->Hu, o) + AT(h->Hu, o + 1)) * 0.5; }
+static double h_HuP(const hadv_t *h, ptrdiff_t base, ptrdiff_t o) { return (AT(h->Hu, o) + AT(h->Hu, o - 1)) * 0.5; }
+static double h_HvP(const hadv_t *h, ptrdiff_t base, ptrdiff_t o) { return (AT(h->Hv, o) + AT(h->Hv, o - h->szI)) * 0.5; }
+static double h_HvR(const hadv_t *h, ptrdiff_t base, ptrdiff_t o) { return (AT(h->Hv, o) + AT(h->Hv, o + h->szI)) * 0.5; }
+/* upwind curvature terms (h:608-702) */
+static double h_uxxR(const hadv_t *h, ptrdiff_t base, ptrdiff_t o)
+{   /* upwindUtoR(DXXUtoU(u), uR): uR(o) + uR(o+1) >= 0 ? DXX(u)(o) : DXX(u)(o+1) */
+    return (h_uR(h, base, o) + h_uR(h, base, o + 1) >= 0) ? dxx(h->u, base, o) : dxx(h->u, base, o + 1);
+}
+static double h_ueeP(const hadv_t *h, ptrdiff_t base, ptrdiff_t o)
+{   /* upwindUtoP(DEEUtoU(u), HvomP): vel(o) + vel(o-W) >= 0 ? DEE(u)(o) : DEE(u)(o-W) */
+    const ptrdiff_t W = h->szI;
+    return (h_HvP(h, base, o) + h_HvP(h, base, o - W) >= 0) ? dee(h->u, base, o, W) : dee(h->u, base, o - W, W);
+}
+static double h_HuxxR(const hadv_t *h, ptrdiff_t base, ptrdiff_t o)
+{   /* UtoR(DXXUtoU(Huon)) */
+    return (dxx(h->Hu, base, o) + dxx(h->Hu, base, o + 1)) * 0.5;
+}
+static double h_HueeP(const hadv_t *h, ptrdiff_t base, ptrdiff_t o)
+{   /* UtoP(DEEUtoU(Huon)) */
+    return (dee(h->Hu, base, o, h->szI) + dee(h->Hu, base, o - 1, h->szI)) * 0.5;
+}
+static double h_HvxxP(const hadv_t *h, ptrdiff_t base, ptrdiff_t o)
+{   /* VtoP(DXXVtoV(Hvom)) */
+    return (dxx(h->Hv, base, o) + dxx(h->Hv, base, o - h->szI)) * 0.5;
+}
+static double h_HveeR(const hadv_t *h, ptrdiff_t base, ptrdiff_t o)
+{   /* VtoR(DEEVtoV(Hvom)) */
+    return (dee(h->Hv, base, o, h->szI) + dee(h->Hv, base, o + h->szI, h->szI)) * 0.5;
+}
+static const double Gadv = -0.5;
+/* UFxR = uR + ((Gadv*uxxR)*(HuonR + Gadv*HuxxR)) and the three siblings (:121-125) */
+static double UFxR(const hadv_t *h, ptrdiff_t b, ptrdiff_t o)
+{
+    return h_uR(h, b, o) + (Gadv * h_uxxR(h, b, o)) * (h_HuR(h, b, o) + Gadv * h_HuxxR(h, b, o));
+}
+static double UFeP(const hadv_t *h, ptrdiff_t b, ptrdiff_t o)
+{
+    return h_uP(h, b, o) + (Gadv * h_ueeP(h, b, o)) * (h_HvP(h, b, o) + Gadv * h_HvxxP(h, b, o));
+}
+static double VFxP(const hadv_t *h, ptrdiff_t b, ptrdiff_t o)
+{
+    return h_vP(h, b, o) + (Gadv * h_ueeP(h, b, o)) * (h_HuP(h, b, o) + Gadv * h_HueeP(h, b, o));
+}
+static double VFeR(const hadv_t *h, ptrdiff_t b, ptrdiff_t o)
+{
+    return h_vR(h, b, o) + (Gadv * h_uxxR(h, b, o)) * (h_HvR(h, b, o) + Gadv * h_HveeR(h, b, o));
+}
+#undef AT
+
+void sfo_horizontalAdvection(const sfo_grid3 *g, const double *u, const double *v, const double *Huon,
+                             const double *Hvom, double *ru, double *rv, int32_t launch_threads)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N, n3 = (N + 1) * S2, W = g->szI;
+    hadv_t h = {{u, n3}, {v, n3}, {Huon, n3}, {Hvom, n3}, W};
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        for (ptrdiff_t K = 0; K < N; K++) {
+            const ptrdiff_t b = K * S2 + f;
+            const double du = (UFxR(&h, b, 0) - UFxR(&h, b, -1)) + (UFeP(&h, b, 0) - UFeP(&h, b, -W));
+            const double dv = (VFxP(&h, b, 0) - VFxP(&h, b, -1)) + (VFeR(&h, b, 0) - VFeR(&h, b, -W));
+            ru[b] = ru[b] - du;
+            rv[b] = rv[b] - dv;
+        }
+    }
+    /* threads S2 .. launch_threads-1 store 0 at flat index i (level 1 of the first
+     * columns), scheduled after the columns */
+    for (ptrdiff_t t = S2; t < launch_threads && t < n3; t++) {
+        ru[t] = 0.0;
+        rv[t] = 0.0;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * verticalAdvection (rhs_kernels.cpp:135-197): inner cells that are not zeta
+ * boundary points.  F(K) = UtoUW_4th(u)(K) * WtoUW_4th(W)(K) (h:723, :767).
+ * ------------------------------------------------------------------------- */
+static double v4(arr_t a, ptrdiff_t S2, ptrdiff_t K, ptrdiff_t f)
+{   /* opRtoW_4th: 9/16 (a(K)+a(K+1)) - 1/16 (a(K-1)+a(K+2)) */
+    return (9.0 / 16.0) * (r3(a, K * S2 + f) + r3(a, (K + 1) * S2 + f)) -
+           (1.0 / 16.0) * (r3(a, (K - 1) * S2 + f) + r3(a, (K + 2) * S2 + f));
+}
+static double h4(arr_t a, ptrdiff_t S2, ptrdiff_t K, ptrdiff_t f, ptrdiff_t d)
+{   /* opRtoU_4th (d=1) / opRtoV_4th (d=szI): 9/16 (a(o)+a(o+d)) - 1/16 (a(o-d)+a(o+2d)) */
+    const ptrdiff_t b = K * S2 + f;
+    return (9.0 / 16.0) * (r3(a, b) + r3(a, b + d)) - (1.0 / 16.0) * (r3(a, b - d) + r3(a, b + 2 * d));
+}
+
+void sfo_verticalAdvection(const sfo_grid3 *g, const double *u, const double *v, const double *Wv,
+                           double *ru, double *rv, const int32_t *BC)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N, n3 = (N + 1) * S2, W = g->szI;
+    arr_t U = {u, n3}, V = {v, n3}, Wa = {Wv, n3};
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        if (BC[f] >= 0 || !isInnerCell(g, f)) continue;
+#define FU(k) (v4(U, S2, (k), f) * h4(Wa, S2, (k), f, 1))
+#define FV(k) (v4(V, S2, (k), f) * h4(Wa, S2, (k), f, W))
+        for (ptrdiff_t K = 2; K < N - 2; K++) {
+            ru[K * S2 + f] = ru[K * S2 + f] - (FU(K + 1) - FU(K));
+            rv[K * S2 + f] = rv[K * S2 + f] - (FV(K + 1) - FV(K));
+        }
+        /* K = 1 block (:169-176) */
+        rv[1 * S2 + f] -= (9.0 / 16.0) * (FV(1) + FV(2)) - (1.0 / 16.0) * (FV(1) + FV(3));
+        const double u0 = r3(U, 0 * S2 + f), u1 = r3(U, 1 * S2 + f), u2 = r3(U, 2 * S2 + f);
+        const double uuu = ((9.0 / 16.0) * (u0 + u1) - (1.0 / 16.0) * (0.0 + u2)) * h4(Wa, S2, 1, f, 1);
+        const double uuu2 = ((9.0 / 16.0) * (0.0 + u0) - (1.0 / 16.0) * (0.0 + u1)) * h4(Wa, S2, 0, f, 1);
+        ru[1 * S2 + f] -= (FU(2) - uuu);
+        ru[0 * S2 + f] -= (uuu - uuu2);
+        /* K = 0 block (:178-194) */
+        ru[(N - 2) * S2 + f] -= (9.0 / 16.0) * (FU(N - 1) + FU(N - 2)) - (1.0 / 16.0) * (FU(N - 1) + FU(N - 3));
+        rv[(N - 2) * S2 + f] -= (9.0 / 16.0) * (FV(N - 1) + FV(N - 2)) - (1.0 / 16.0) * (FV(N - 1) + FV(N - 3));
+        rv[0 * S2 + f] = rv[1 * S2 + f];
+        ru[(N - 2) * S2 + f] = ru[(N - 3) * S2 + f];
+        rv[(N - 2) * S2 + f] = rv[(N - 3) * S2 + f];
+        ru[(N - 1) * S2 + f] = ru[(N - 2) * S2 + f];
+        rv[(N - 1) * S2 + f] = rv[(N - 2) * S2 + f];
+        ru[N * S2 + f] = ru[(N - 2) * S2 + f];
+        rv[N * S2 + f] = rv[(N - 2) * S2 + f];
+#undef FU
+#undef FV
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * set_maxflux (step3D_uv.kernels.cpp:372-405)
+ * ------------------------------------------------------------------------- */
+void sfo_set_maxflux(const sfo_grid3 *g, const double *u, const double *v, double *Huon, double *Hvom,
+                     const double *Hz)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N;
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        if (isUNode(g, f))
+            for (ptrdiff_t K = 0; K < N; K++) {
+                const ptrdiff_t b = K * S2 + f;
+                Huon[b] = (((Hz[b] + Hz[b - 1]) * 0.5) * u[b]) * g->on_u[f];
+            }
+        if (isVNode(g, f))
+            for (ptrdiff_t K = 0; K < N; K++) {
+                const ptrdiff_t b = K * S2 + f;
+                Hvom[b] = (((Hz[b] + Hz[b - g->szI]) * 0.5) * v[b]) * g->om_v[f];
+            }
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * omega (step3D_uv.kernels.cpp:62-165)
+ * ------------------------------------------------------------------------- */
+void sfo_omega(const sfo_grid3 *g, double *Wv, const double *Huon, const double *Hvom, const double *z_w,
+               const int32_t *BC)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N, n3 = (N + 1) * S2;
+    arr_t Hu = {Huon, n3}, Hv = {Hvom, n3};
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        if (BC[f] >= 0) continue;
+        Wv[f] = 0.0;
+        for (ptrdiff_t K = 1; K < N; K++) {
+            const ptrdiff_t b = K * S2 + f;
+            Wv[b] = Wv[b - S2] - ((r3(Hu, b + 1) - r3(Hu, b)) + (r3(Hv, b + g->szI) - r3(Hv, b)));
+        }
+        const double zb = z_w[f];
+        const double h = z_w[(N - 1) * S2 + f] - zb;
+        const double Ws = Wv[(N - 1) * S2 + f];
+        for (ptrdiff_t K = N - 1; K > 0; K--) {
+            const ptrdiff_t b = K * S2 + f;
+            Wv[b] = Wv[b] - (Ws * (z_w[b] - zb)) / h;
+        }
+        Wv[N * S2 + f] = 0.0;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * set_depth (step3D_uv.kernels.cpp:225-291) and set_zeta (:294-310)
+ * ------------------------------------------------------------------------- */
+void sfo_set_depth(const sfo_grid3 *g, int Vtransform, const double *Zt, double *z_w, double *z_r,
+                   const double *h, double hc, double *Hz, const double *sc_r, const double *sc_w,
+                   const double *Cs_r, const double *Cs_w)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N;
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        if (Vtransform == 1) {
+            for (ptrdiff_t K = 0; K < N; K++) {
+                const double z_r0 = hc * (sc_r[K] - Cs_r[K]) + Cs_r[K] * h[f];
+                const double z_w0 = hc * (sc_w[K] - Cs_w[K]) + Cs_w[K] * h[f];
+                z_w[K * S2 + f] = z_w0 + Zt[f] * (1.0 + z_w0 / h[f]);
+                z_r[K * S2 + f] = z_r0 + Zt[f] * (1.0 + z_r0 / h[f]);
+            }
+            z_w[N * S2 + f] = Zt[f];
+        } else if (Vtransform == 2) {
+            z_w[N * S2 + f] = 0.0;
+            for (ptrdiff_t K = 0; K < N; K++) {
+                const double cr = (hc * sc_r[K] + Cs_r[K] * h[f]) / (hc + h[f]);
+                const double cw = (hc * sc_w[K] + Cs_w[K] * h[f]) / (hc + h[f]);
+                z_w[K * S2 + f] = Zt[f] + (Zt[f] + h[f]) * cw;
+                z_r[K * S2 + f] = Zt[f] + (Zt[f] + h[f]) * cr;
+            }
+            z_w[N * S2 + f] = Zt[f];
+        }
+        for (ptrdiff_t K = 0; K < N; K++) Hz[K * S2 + f] = z_w[(K + 1) * S2 + f] - z_w[K * S2 + f];
+    }
+}
+
+void sfo_set_zeta(const sfo_grid3 *g, double *zeta1, double *zeta2, const double *Zt)
+{
+    const ptrdiff_t S2 = S2_OF(g);
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        if (!isRNode(g, f)) continue;
+        zeta2[f] = Zt[f];
+        zeta1[f] = Zt[f];
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * setLateralUVBCs (step3D_uv.kernels.cpp:314-368); gradient v boundaries index the
+ * 2D index field at 3D offsets in the reference (out of bounds): rejected (-1).
+ * ------------------------------------------------------------------------- */
+int sfo_setLateralUVBCs(const sfo_grid3 *g, double t, double *u, double *v, const int32_t *uIdx2,
+                        const int32_t *vIdx2, const int32_t *uType, const int32_t *vType)
+{
+    (void)uIdx2;
+    (void)vIdx2;
+    const ptrdiff_t S2 = S2_OF(g), N = g->N;
+    for (ptrdiff_t f = 0; f < S2; f++)
+        if (vType[f] > 0 && !(vType[f] & (1 << 4)) && !(vType[f] & (1 << 3)) && (vType[f] & (1 << 6)))
+            return -1;
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        const int bu = uType[f], bv = vType[f];
+        if (bu <= 0 && bv <= 0) continue;
+        if (bu > 0 && !(bu & (1 << 4)) && (bu & (1 << 3))) {
+            const double val = 0.05 * sin(t * 0.0002);
+            for (ptrdiff_t k = 0; k < N; k++) u[f + k * S2] = val;
+        }
+        if (bv > 0 && ((bv & (1 << 4)) || (bv & (1 << 3))))
+            for (ptrdiff_t k = 0; k < N; k++) v[f + k * S2] = 0.0;
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * step3d_UV (step3D_uv.kernels.cpp:646-707) with createVertViscousOpMatrix
+ * (:568-639), solveTri (:520-565), correctBaroclinicMeanVel (:475-517) and
+ * adjustBarotropicVelocity (:408-469).  SD/D/RHS: caller scratch, (N+1)*S2 each.
+ * ------------------------------------------------------------------------- */
+static void build_matrix(const sfo_grid3 *g, int vdir, ptrdiff_t f, double cff, double Dt, double lambda,
+                         double *SD, double *D, double *RHS, const double *Hz, const double *Akv,
+                         const double *z_r, const double *u, const double *ru)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N;
+    const ptrdiff_t d = vdir ? g->szI : 1;   /* neighbour that RtoU / RtoV average with */
+    if (vdir ? !isVNode(g, f) : !isUNode(g, f)) return;
+#define HZU(K) ((Hz[(K) * S2 + f] + Hz[(K) * S2 + f - d]) * 0.5)
+#define AKVU(K) ((Akv[(K) * S2 + f] + Akv[(K) * S2 + f - d]) * 0.5)
+#define DZ(K) (((z_r[(K) * S2 + f] - z_r[((K) - 1) * S2 + f]) + (z_r[(K) * S2 + f - d] - z_r[((K) - 1) * S2 + f - d])) * 0.5)
+    const double pmU = (g->pm[f] + g->pm[f - d]) * 0.5;
+    const double pnU = (g->pn[f] + g->pn[f - d]) * 0.5;
+    const double cdt_mn = (cff * Dt) * (pmU * pnU);
+    SD[(N - 1) * S2 + f] = 0.0;
+    D[f] = HZU(0);
+    RHS[f] = 0.0;
+    for (ptrdiff_t K = 0; K < N - 1; K++) SD[K * S2 + f] = (((-lambda) * Dt) * AKVU(K)) / DZ(K + 1);
+    for (ptrdiff_t K = 1; K < N; K++) {
+        RHS[K * S2 + f] = HZU(K) * (u[K * S2 + f] + cdt_mn * ru[K * S2 + f]);
+        D[K * S2 + f] = (HZU(K) - SD[K * S2 + f]) - SD[(K - 1) * S2 + f];
+    }
+    D[(N - 1) * S2 + f] = HZU(N - 1) - SD[(N - 2) * S2 + f];
+    SD[f] = (((-lambda) * Dt) * AKVU(0)) / DZ(1);
+    D[f] = HZU(0) - 2.0 * SD[f];
+#undef HZU
+#undef AKVU
+#undef DZ
+}
+
+static void solve_tri(const sfo_grid3 *g, ptrdiff_t f, double *SD, double *D, double *RHS, double *x)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N;
+    if (!isUNode(g, f)) return;
+    for (ptrdiff_t k = 1; k < N; k++) {
+        const double w = SD[(k - 1) * S2 + f] / D[(k - 1) * S2 + f];
+        D[k * S2 + f] -= w * SD[(k - 1) * S2 + f];
+        RHS[k * S2 + f] -= w * RHS[(k - 1) * S2 + f];
+    }
+    x[(N - 1) * S2 + f] = RHS[(N - 1) * S2 + f] / D[(N - 1) * S2 + f];
+    for (ptrdiff_t k = N - 2; k >= 0; k--)
+        x[k * S2 + f] = (RHS[k * S2 + f] - SD[k * S2 + f] * x[(k + 1) * S2 + f]) / D[k * S2 + f];
+}
+
+static void correct_mean(const sfo_grid3 *g, int vdir, ptrdiff_t f, double *x, const double *Hz,
+                         const double *Davg, const double *metric, const double *xold)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N;
+    const ptrdiff_t d = vdir ? g->szI : 1;
+    if (vdir ? !isVNode(g, f) : !isUNode(g, f)) return;
+    double H = 0.0, Hx = 0.0;
+    for (ptrdiff_t K = 0; K < N; K++) {
+        const double hu = (Hz[K * S2 + f] + Hz[K * S2 + f - d]) * 0.5;
+        H += hu;
+        Hx += xold[K * S2 + f] * hu;
+    }
+    const double diff = (Hx - Davg[f] / metric[f]) / H;
+    for (ptrdiff_t K = 0; K < N; K++) x[K * S2 + f] = x[K * S2 + f] - diff;
+}
+
+void sfo_step3d_UV(const sfo_grid3 *g, double *u, double *v, const double *ru, const double *rv,
+                   double *ubar_t1, double *vbar_t1, double *ubar_t2, double *vbar_t2, const double *Hz,
+                   const double *Akv, const double *z_r, const double *DU_avg1, const double *DV_avg1,
+                   const double *tmpU, const double *tmpV, int iic, int ntfirst, double lambda, double Dt,
+                   double *SD, double *D, double *RHS)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N;
+    double cff;
+    if (iic == ntfirst) cff = 1.0;
+    else if (iic == ntfirst + 1) cff = 3.0 / 2.0;
+    else cff = 23.0 / 12.0;
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        build_matrix(g, 0, f, cff, Dt, lambda, SD, D, RHS, Hz, Akv, z_r, tmpU, ru);
+        solve_tri(g, f, SD, D, RHS, u);
+        correct_mean(g, 0, f, u, Hz, DU_avg1, g->on_u, tmpU);
+        build_matrix(g, 1, f, cff, Dt, lambda, SD, D, RHS, Hz, Akv, z_r, tmpV, rv);
+        solve_tri(g, f, SD, D, RHS, v);
+        correct_mean(g, 1, f, v, Hz, DV_avg1, g->om_v, tmpV);
+        if (!isUNode(g, f)) continue;
+        /* adjustBarotropicVelocity (:408-469) */
+        double HU = 0.0, HV = 0.0, HuU = 0.0, HvV = 0.0;
+        for (ptrdiff_t K = 0; K < N; K++) {
+            const ptrdiff_t b = K * S2 + f;
+            const double hzu = (Hz[b] + Hz[b - 1]) * 0.5;
+            const double hzv = (Hz[b] + Hz[b - g->szI]) * 0.5;
+            HU += hzu;
+            HV += hzv;
+            HuU += hzu * u[b];
+            HvV += hzv * v[b];
+        }
+        const double uUp = (HuU - DU_avg1[f] / g->on_u[f]) / HU;
+        const double vUp = (HvV - DV_avg1[f] / g->om_v[f]) / HV;
+        ubar_t2[f] = (DU_avg1[f] / g->on_u[f]) / HU + uUp;
+        vbar_t2[f] = (DV_avg1[f] / g->om_v[f]) / HV + vUp;
+        ubar_t1[f] = ubar_t2[f];
+        vbar_t1[f] = vbar_t2[f];
+    }
+}

@@ -53,6 +53,15 @@ _SIGS = {
     "sf_bc_set": [P, P, I32, F64, P],
     "sf_bc_apply_map": [P, P, P, I32, F64, P],
     "sf_step2d_check_maps": [I32, I32, I32, P, P, I32, P, P, I32, P, P, I32],
+    "sf_add_coriolis": [P, P, P, P, P, P, P, P],
+    "sf_horizontal_advection": [P, P, P, P, P, P, P, I64, P],
+    "sf_vertical_advection": [P, P, P, P, P, P, P, P],
+    "sf_set_maxflux": [P, P, P, P, P, P, P],
+    "sf_omega": [P, P, P, P, P, P, P],
+    "sf_set_depth": [P, I32, P, P, P, P, F64, P, P, P, P, P, P],
+    "sf_set_zeta": [P, P, P, P, P],
+    "sf_set_lateral_uv_bcs": [P, F64, P, P, P, P, P],
+    "sf_step3d_uv": [P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I32, I32, F64, F64, P, P, P, P],
 }
 _RESTYPES = {"sf_last_error_string": C.c_char_p}
 

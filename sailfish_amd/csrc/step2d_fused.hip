@@ -78,6 +78,7 @@ struct Params {
     long long S2;
     double Dt, cff1, clamp;
     int zero_vbar;
+    int fold_masked;         // masked points finished by the tiled kernel (stride == szI)
 };
 
 __device__ __forceinline__ int lidx(int li, int lj) { return lj * W2 + li; }
@@ -302,6 +303,13 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
         if (!is_masked(f, P.q5_stride)) {
             F.ubar_t2[f] = sUp[l];
             F.vbar_t2[f] = P.zero_vbar ? T(0) : sVp[l];
+        } else if (P.fold_masked) {
+            // stride == szI: the masked points are column 0 and row 0, which no
+            // other workgroup reads, so they are finished here (same work as
+            // k_step2d_masked)
+            F.ubar_t2[f] = sUp[l];
+            F.vbar_t2[f] = P.zero_vbar ? T(0) : sVp[l];
+            F.ubar_t1[f] = am3(F.ubar_t1[f], Dt, F.rubar_t1[f], F.rubar_t0[f], F.vbar_t1[f]);
         }
     }
 }
@@ -383,12 +391,13 @@ int launch(sf_ctx *c, const sf_step2d_fields *a, double dtfast, double cff1, dou
     P.cff1 = cff1;
     P.clamp = clamp;
     P.zero_vbar = (flags & SF_SUBSTEP_ZERO_VBAR) ? 1 : 0;
+    P.fold_masked = (c->q5_stride == c->szI) ? 1 : 0;
     if (parts & 1) {
         hipLaunchKernelGGL(k_step2d_fused<T>, dim3(P.tiles_i * P.tiles_j), dim3(NT), 0, s, F, P);
         int rc = sf::check_launch("sf_step2d_substep (tiles)");
         if (rc != SF_OK) return rc;
     }
-    if (parts & 2) {
+    if ((parts & 2) && !P.fold_masked) {
         const long long nm = (c->S2 + c->q5_stride - 1) / c->q5_stride + c->q5_stride - 1;
         hipLaunchKernelGGL(k_step2d_masked<T>, dim3(sf::grid1d(nm)), dim3(256), 0, s, F, P);
         return sf::check_launch("sf_step2d_substep (masked points)");

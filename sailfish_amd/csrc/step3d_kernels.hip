@@ -1,0 +1,508 @@
+// step3d_kernels.hip -- 3D kernels of the baroclinic step, one launch per reference
+// kernel: addCoriolis / horizontalAdvection / verticalAdvection
+// (nonlinear/rhs_kernels.cpp), set_maxflux / omega / set_depth / set_zeta /
+// setLateralUVBCs / step3d_UV (nonlinear/step3D_uv.kernels.cpp).
+//
+// Layout: 3D fields are flat with level stride S2 = szI*szJ (k slowest, xi
+// fastest, modules/mod_ocean.py:229-240).  One thread per water column (j,i), the
+// level loop inside the thread; consecutive threads take consecutive i, so every
+// level of every field is read with unit stride across a wavefront.  Stencil
+// offsets are flat (the reference's Eval arithmetic, mod_cppkernels.h:337-340);
+// reads outside an array are defined as 0.0.  FP64, reference operand order,
+// contraction off: results equal the CPU restatement bit for bit.
+#include "sf_internal.h"
+
+namespace {
+
+struct A {
+    const double *p;
+    long long n;
+};
+
+__device__ __forceinline__ double r3(A a, long long idx) { return (idx < 0 || idx >= a.n) ? 0.0 : a.p[idx]; }
+
+struct G3 {
+    int szI, szJ, N;
+    long long S2;
+    const double *on_u, *om_v, *pn, *pm;
+};
+
+__device__ __forceinline__ bool isRNode(const G3 &g, long long f) { return (f % g.szI) > 0 && (f / g.szI) > 0; }
+__device__ __forceinline__ bool isUNode(const G3 &g, long long f) { return (f % g.szI) > 1 && (f / g.szI) > 0; }
+__device__ __forceinline__ bool isVNode(const G3 &g, long long f)
+{
+    const long long j = f / g.szI;
+    return (f % g.szI) > 0 && j > 1 && j < g.szJ - 1;
+}
+__device__ __forceinline__ bool isInnerCell(const G3 &g, long long f)
+{
+    const long long i = f % g.szI, j = f / g.szI;
+    return i > 1 && i < g.szI - 2 && j > 1 && j < g.szJ - 2;
+}
+
+#define COLUMN_INDEX                                                      \
+    const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x; \
+    if (f >= g.S2) return;
+
+// ---- addCoriolis (rhs_kernels.cpp:6-35) -------------------------------------------
+__device__ __forceinline__ double cor_UF(const G3 &g, A fomn, A v, A Hz, long long K, long long f)
+{
+    const long long b = K * g.S2 + f;
+    return (r3(Hz, b) * r3(fomn, f)) * ((r3(v, b) + r3(v, b + g.szI)) * 0.5);
+}
+__device__ __forceinline__ double cor_VF(const G3 &g, A fomn, A u, A Hz, long long K, long long f)
+{
+    const long long b = K * g.S2 + f;
+    return (r3(Hz, b) * r3(fomn, f)) * ((r3(u, b) + r3(u, b + 1)) * 0.5);
+}
+
+__global__ __launch_bounds__(256) void k_addCoriolis(G3 g, A fomn, A u, A v, double *ru, double *rv, A Hz)
+{
+    COLUMN_INDEX
+    if (!isRNode(g, f)) return;
+    for (long long K = 0; K < g.N; K++) {
+        const long long b = K * g.S2 + f;
+        const double rtou = (cor_UF(g, fomn, v, Hz, K, f) + cor_UF(g, fomn, v, Hz, K, f - 1)) * 0.5;
+        const double rtov = (cor_VF(g, fomn, u, Hz, K, f) + cor_VF(g, fomn, u, Hz, K, f - g.szI)) * 0.5;
+        ru[b] = ru[b] + rtou;
+        rv[b] = rv[b] - rtov;
+    }
+}
+
+// ---- horizontalAdvection (rhs_kernels.cpp:70-132) ---------------------------------
+struct H {
+    A u, v, Hu, Hv;
+    long long W;
+};
+
+#define AT(a, o) r3((a), base + (o))
+__device__ __forceinline__ double dxx(A a, long long base, long long o)
+{
+    return AT(a, o - 1) - 2.0 * AT(a, o) + AT(a, o + 1);
+}
+__device__ __forceinline__ double dee(A a, long long base, long long o, long long W)
+{
+    return AT(a, o - W) - 2.0 * AT(a, o) + AT(a, o + W);
+}
+__device__ __forceinline__ double h_uR(const H &h, long long base, long long o) { return (AT(h.u, o) + AT(h.u, o + 1)) * 0.5; }
+__device__ __forceinline__ double h_uP(const H &h, long long base, long long o) { return (AT(h.u, o) + AT(h.u, o - 1)) * 0.5; }
+__device__ __forceinline__ double h_vR(const H &h, long long base, long long o) { return (AT(h.v, o) + AT(h.v, o + h.W)) * 0.5; }
+__device__ __forceinline__ double h_vP(const H &h, long long base, long long o) { return (AT(h.v, o) + AT(h.v, o - h.W)) * 0.5; }
+__device__ __forceinline__ double h_HuR(const H &h, long long base, long long o) { return (AT(h.Hu, o) + AT(h.Hu, o + 1)) * 0.5; }
+__device__ __forceinline__ double h_HuP(const H &h, long long base, long long o) { return (AT(h.Hu, o) + AT(h.Hu, o - 1)) * 0.5; }
+__device__ __forceinline__ double h_HvP(const H &h, long long base, long long o) { return (AT(h.Hv, o) + AT(h.Hv, o - h.W)) * 0.5; }
+__device__ __forceinline__ double h_HvR(const H &h, long long base, long long o) { return (AT(h.Hv, o) + AT(h.Hv, o + h.W)) * 0.5; }
+__device__ __forceinline__ double h_uxxR(const H &h, long long base, long long o)
+{
+    return (h_uR(h, base, o) + h_uR(h, base, o + 1) >= 0) ? dxx(h.u, base, o) : dxx(h.u, base, o + 1);
+}
+__device__ __forceinline__ double h_ueeP(const H &h, long long base, long long o)
+{
+    return (h_HvP(h, base, o) + h_HvP(h, base, o - h.W) >= 0) ? dee(h.u, base, o, h.W) : dee(h.u, base, o - h.W, h.W);
+}
+__device__ __forceinline__ double h_HuxxR(const H &h, long long base, long long o)
+{
+    return (dxx(h.Hu, base, o) + dxx(h.Hu, base, o + 1)) * 0.5;
+}
+__device__ __forceinline__ double h_HueeP(const H &h, long long base, long long o)
+{
+    return (dee(h.Hu, base, o, h.W) + dee(h.Hu, base, o - 1, h.W)) * 0.5;
+}
+__device__ __forceinline__ double h_HvxxP(const H &h, long long base, long long o)
+{
+    return (dxx(h.Hv, base, o) + dxx(h.Hv, base, o - h.W)) * 0.5;
+}
+__device__ __forceinline__ double h_HveeR(const H &h, long long base, long long o)
+{
+    return (dee(h.Hv, base, o, h.W) + dee(h.Hv, base, o + h.W, h.W)) * 0.5;
+}
+constexpr double Gadv = -0.5;
+__device__ __forceinline__ double UFxR(const H &h, long long b, long long o)
+{
+    return h_uR(h, b, o) + (Gadv * h_uxxR(h, b, o)) * (h_HuR(h, b, o) + Gadv * h_HuxxR(h, b, o));
+}
+__device__ __forceinline__ double UFeP(const H &h, long long b, long long o)
+{
+    return h_uP(h, b, o) + (Gadv * h_ueeP(h, b, o)) * (h_HvP(h, b, o) + Gadv * h_HvxxP(h, b, o));
+}
+__device__ __forceinline__ double VFxP(const H &h, long long b, long long o)
+{
+    return h_vP(h, b, o) + (Gadv * h_ueeP(h, b, o)) * (h_HuP(h, b, o) + Gadv * h_HueeP(h, b, o));
+}
+__device__ __forceinline__ double VFeR(const H &h, long long b, long long o)
+{
+    return h_vR(h, b, o) + (Gadv * h_uxxR(h, b, o)) * (h_HvR(h, b, o) + Gadv * h_HveeR(h, b, o));
+}
+#undef AT
+
+__global__ __launch_bounds__(256) void k_horizontalAdvection(G3 g, H h, double *ru, double *rv)
+{
+    COLUMN_INDEX
+    const long long W = g.szI;
+    for (long long K = 0; K < g.N; K++) {
+        const long long b = K * g.S2 + f;
+        const double du = (UFxR(h, b, 0) - UFxR(h, b, -1)) + (UFeP(h, b, 0) - UFeP(h, b, -W));
+        const double dv = (VFxP(h, b, 0) - VFxP(h, b, -1)) + (VFeR(h, b, 0) - VFeR(h, b, -W));
+        ru[b] = ru[b] - du;
+        rv[b] = rv[b] - dv;
+    }
+}
+
+// the reference's threads S2..launch-1 store 0 at flat index i (rhs_kernels.cpp:86-91)
+__global__ void k_hadv_tail(long long S2, long long end, double *ru, double *rv)
+{
+    const long long t = S2 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= end) return;
+    ru[t] = 0.0;
+    rv[t] = 0.0;
+}
+
+// ---- verticalAdvection (rhs_kernels.cpp:135-197) ----------------------------------
+__device__ __forceinline__ double v4(A a, long long S2, long long K, long long f)
+{
+    return (9.0 / 16.0) * (r3(a, K * S2 + f) + r3(a, (K + 1) * S2 + f)) -
+           (1.0 / 16.0) * (r3(a, (K - 1) * S2 + f) + r3(a, (K + 2) * S2 + f));
+}
+__device__ __forceinline__ double h4(A a, long long S2, long long K, long long f, long long d)
+{
+    const long long b = K * S2 + f;
+    return (9.0 / 16.0) * (r3(a, b) + r3(a, b + d)) - (1.0 / 16.0) * (r3(a, b - d) + r3(a, b + 2 * d));
+}
+
+__global__ __launch_bounds__(256) void k_verticalAdvection(G3 g, A U, A V, A Wa, double *ru, double *rv,
+                                                           const int32_t *BC)
+{
+    COLUMN_INDEX
+    if (BC[f] >= 0 || !isInnerCell(g, f)) return;
+    const long long S2 = g.S2, N = g.N, W = g.szI;
+#define FU(k) (v4(U, S2, (k), f) * h4(Wa, S2, (k), f, 1))
+#define FV(k) (v4(V, S2, (k), f) * h4(Wa, S2, (k), f, W))
+    for (long long K = 2; K < N - 2; K++) {
+        ru[K * S2 + f] = ru[K * S2 + f] - (FU(K + 1) - FU(K));
+        rv[K * S2 + f] = rv[K * S2 + f] - (FV(K + 1) - FV(K));
+    }
+    rv[1 * S2 + f] -= (9.0 / 16.0) * (FV(1) + FV(2)) - (1.0 / 16.0) * (FV(1) + FV(3));
+    const double u0 = r3(U, f), u1 = r3(U, S2 + f), u2 = r3(U, 2 * S2 + f);
+    const double uuu = ((9.0 / 16.0) * (u0 + u1) - (1.0 / 16.0) * (0.0 + u2)) * h4(Wa, S2, 1, f, 1);
+    const double uuu2 = ((9.0 / 16.0) * (0.0 + u0) - (1.0 / 16.0) * (0.0 + u1)) * h4(Wa, S2, 0, f, 1);
+    ru[1 * S2 + f] -= (FU(2) - uuu);
+    ru[f] -= (uuu - uuu2);
+    ru[(N - 2) * S2 + f] -= (9.0 / 16.0) * (FU(N - 1) + FU(N - 2)) - (1.0 / 16.0) * (FU(N - 1) + FU(N - 3));
+    rv[(N - 2) * S2 + f] -= (9.0 / 16.0) * (FV(N - 1) + FV(N - 2)) - (1.0 / 16.0) * (FV(N - 1) + FV(N - 3));
+    rv[f] = rv[S2 + f];
+    ru[(N - 2) * S2 + f] = ru[(N - 3) * S2 + f];
+    rv[(N - 2) * S2 + f] = rv[(N - 3) * S2 + f];
+    ru[(N - 1) * S2 + f] = ru[(N - 2) * S2 + f];
+    rv[(N - 1) * S2 + f] = rv[(N - 2) * S2 + f];
+    ru[N * S2 + f] = ru[(N - 2) * S2 + f];
+    rv[N * S2 + f] = rv[(N - 2) * S2 + f];
+#undef FU
+#undef FV
+}
+
+// ---- set_maxflux (step3D_uv.kernels.cpp:372-405) ----------------------------------
+__global__ __launch_bounds__(256) void k_set_maxflux(G3 g, const double *u, const double *v, double *Huon,
+                                                     double *Hvom, const double *Hz)
+{
+    COLUMN_INDEX
+    if (isUNode(g, f))
+        for (long long K = 0; K < g.N; K++) {
+            const long long b = K * g.S2 + f;
+            Huon[b] = (((Hz[b] + Hz[b - 1]) * 0.5) * u[b]) * g.on_u[f];
+        }
+    if (isVNode(g, f))
+        for (long long K = 0; K < g.N; K++) {
+            const long long b = K * g.S2 + f;
+            Hvom[b] = (((Hz[b] + Hz[b - g.szI]) * 0.5) * v[b]) * g.om_v[f];
+        }
+}
+
+// ---- omega (step3D_uv.kernels.cpp:62-165) -----------------------------------------
+__global__ __launch_bounds__(256) void k_omega(G3 g, double *W, A Hu, A Hv, const double *z_w,
+                                               const int32_t *BC)
+{
+    COLUMN_INDEX
+    if (BC[f] >= 0) return;
+    const long long S2 = g.S2, N = g.N;
+    double w = 0.0;
+    W[f] = w;
+    for (long long K = 1; K < N; K++) {
+        const long long b = K * S2 + f;
+        w = w - ((r3(Hu, b + 1) - r3(Hu, b)) + (r3(Hv, b + g.szI) - r3(Hv, b)));
+        W[b] = w;
+    }
+    const double zb = z_w[f];
+    const double h = z_w[(N - 1) * S2 + f] - zb;
+    const double Ws = N > 1 ? w : W[(N - 1) * S2 + f];
+    for (long long K = N - 1; K > 0; K--) {
+        const long long b = K * S2 + f;
+        W[b] = W[b] - (Ws * (z_w[b] - zb)) / h;
+    }
+    W[N * S2 + f] = 0.0;
+}
+
+// ---- set_depth / set_zeta (step3D_uv.kernels.cpp:225-310) -------------------------
+__global__ __launch_bounds__(256) void k_set_depth(G3 g, int Vtransform, const double *Zt, double *z_w,
+                                                   double *z_r, const double *h, double hc, double *Hz,
+                                                   const double *sc_r, const double *sc_w, const double *Cs_r,
+                                                   const double *Cs_w)
+{
+    COLUMN_INDEX
+    const long long S2 = g.S2, N = g.N;
+    const double zt = Zt[f], hh = h[f];
+    if (Vtransform == 1) {
+        for (long long K = 0; K < N; K++) {
+            const double z_r0 = hc * (sc_r[K] - Cs_r[K]) + Cs_r[K] * hh;
+            const double z_w0 = hc * (sc_w[K] - Cs_w[K]) + Cs_w[K] * hh;
+            z_w[K * S2 + f] = z_w0 + zt * (1.0 + z_w0 / hh);
+            z_r[K * S2 + f] = z_r0 + zt * (1.0 + z_r0 / hh);
+        }
+        z_w[N * S2 + f] = zt;
+    } else if (Vtransform == 2) {
+        z_w[N * S2 + f] = 0.0;
+        for (long long K = 0; K < N; K++) {
+            const double cr = (hc * sc_r[K] + Cs_r[K] * hh) / (hc + hh);
+            const double cw = (hc * sc_w[K] + Cs_w[K] * hh) / (hc + hh);
+            z_w[K * S2 + f] = zt + (zt + hh) * cw;
+            z_r[K * S2 + f] = zt + (zt + hh) * cr;
+        }
+        z_w[N * S2 + f] = zt;
+    }
+    for (long long K = 0; K < N; K++) Hz[K * S2 + f] = z_w[(K + 1) * S2 + f] - z_w[K * S2 + f];
+}
+
+__global__ __launch_bounds__(256) void k_set_zeta(G3 g, double *zeta1, double *zeta2, const double *Zt)
+{
+    COLUMN_INDEX
+    if (!isRNode(g, f)) return;
+    zeta2[f] = Zt[f];
+    zeta1[f] = Zt[f];
+}
+
+// ---- setLateralUVBCs (step3D_uv.kernels.cpp:314-368) ------------------------------
+__global__ __launch_bounds__(256) void k_lateral_uv(G3 g, double t, double *u, double *v, const int32_t *uType,
+                                                    const int32_t *vType)
+{
+    COLUMN_INDEX
+    const int bu = uType[f], bv = vType[f];
+    if (bu <= 0 && bv <= 0) return;
+    if (bu > 0 && !(bu & (1 << 4)) && (bu & (1 << 3))) {
+        const double val = 0.05 * sin(t * 0.0002);
+        for (long long k = 0; k < g.N; k++) u[f + k * g.S2] = val;
+    }
+    if (bv > 0 && ((bv & (1 << 4)) || (bv & (1 << 3))))
+        for (long long k = 0; k < g.N; k++) v[f + k * g.S2] = 0.0;
+}
+
+// ---- step3d_UV (step3D_uv.kernels.cpp:646-707) ------------------------------------
+__device__ void build_matrix(const G3 &g, bool vdir, long long f, double cff, double Dt, double lambda,
+                             double *SD, double *D, double *RHS, const double *Hz, const double *Akv,
+                             const double *z_r, const double *u, const double *ru)
+{
+    const long long S2 = g.S2, N = g.N;
+    const long long d = vdir ? g.szI : 1;
+    if (vdir ? !isVNode(g, f) : !isUNode(g, f)) return;
+#define HZU(K) ((Hz[(K) * S2 + f] + Hz[(K) * S2 + f - d]) * 0.5)
+#define AKVU(K) ((Akv[(K) * S2 + f] + Akv[(K) * S2 + f - d]) * 0.5)
+#define DZ(K) (((z_r[(K) * S2 + f] - z_r[((K) - 1) * S2 + f]) + (z_r[(K) * S2 + f - d] - z_r[((K) - 1) * S2 + f - d])) * 0.5)
+    const double pmU = (g.pm[f] + g.pm[f - d]) * 0.5;
+    const double pnU = (g.pn[f] + g.pn[f - d]) * 0.5;
+    const double cdt_mn = (cff * Dt) * (pmU * pnU);
+    SD[(N - 1) * S2 + f] = 0.0;
+    D[f] = HZU(0);
+    RHS[f] = 0.0;
+    for (long long K = 0; K < N - 1; K++) SD[K * S2 + f] = (((-lambda) * Dt) * AKVU(K)) / DZ(K + 1);
+    for (long long K = 1; K < N; K++) {
+        RHS[K * S2 + f] = HZU(K) * (u[K * S2 + f] + cdt_mn * ru[K * S2 + f]);
+        D[K * S2 + f] = (HZU(K) - SD[K * S2 + f]) - SD[(K - 1) * S2 + f];
+    }
+    D[(N - 1) * S2 + f] = HZU(N - 1) - SD[(N - 2) * S2 + f];
+    SD[f] = (((-lambda) * Dt) * AKVU(0)) / DZ(1);
+    D[f] = HZU(0) - 2.0 * SD[f];
+#undef HZU
+#undef AKVU
+#undef DZ
+}
+
+__device__ void solve_tri(const G3 &g, long long f, double *SD, double *D, double *RHS, double *x)
+{
+    const long long S2 = g.S2, N = g.N;
+    if (!isUNode(g, f)) return;
+    for (long long k = 1; k < N; k++) {
+        const double w = SD[(k - 1) * S2 + f] / D[(k - 1) * S2 + f];
+        D[k * S2 + f] -= w * SD[(k - 1) * S2 + f];
+        RHS[k * S2 + f] -= w * RHS[(k - 1) * S2 + f];
+    }
+    x[(N - 1) * S2 + f] = RHS[(N - 1) * S2 + f] / D[(N - 1) * S2 + f];
+    for (long long k = N - 2; k >= 0; k--)
+        x[k * S2 + f] = (RHS[k * S2 + f] - SD[k * S2 + f] * x[(k + 1) * S2 + f]) / D[k * S2 + f];
+}
+
+__device__ void correct_mean(const G3 &g, bool vdir, long long f, double *x, const double *Hz, const double *Davg,
+                             const double *metric, const double *xold)
+{
+    const long long S2 = g.S2, N = g.N;
+    const long long d = vdir ? g.szI : 1;
+    if (vdir ? !isVNode(g, f) : !isUNode(g, f)) return;
+    double Hs = 0.0, Hx = 0.0;
+    for (long long K = 0; K < N; K++) {
+        const double hu = (Hz[K * S2 + f] + Hz[K * S2 + f - d]) * 0.5;
+        Hs += hu;
+        Hx += xold[K * S2 + f] * hu;
+    }
+    const double diff = (Hx - Davg[f] / metric[f]) / Hs;
+    for (long long K = 0; K < N; K++) x[K * S2 + f] = x[K * S2 + f] - diff;
+}
+
+struct UVArgs {
+    double *u, *v;
+    const double *ru, *rv;
+    double *ubar_t1, *vbar_t1, *ubar_t2, *vbar_t2;
+    const double *Hz, *Akv, *z_r, *DU_avg1, *DV_avg1, *tmpU, *tmpV;
+    double *SD, *D, *RHS;
+};
+
+__global__ __launch_bounds__(256) void k_step3d_uv(G3 g, UVArgs a, double cff, double lambda, double Dt)
+{
+    COLUMN_INDEX
+    const long long S2 = g.S2, N = g.N;
+    build_matrix(g, false, f, cff, Dt, lambda, a.SD, a.D, a.RHS, a.Hz, a.Akv, a.z_r, a.tmpU, a.ru);
+    solve_tri(g, f, a.SD, a.D, a.RHS, a.u);
+    correct_mean(g, false, f, a.u, a.Hz, a.DU_avg1, g.on_u, a.tmpU);
+    build_matrix(g, true, f, cff, Dt, lambda, a.SD, a.D, a.RHS, a.Hz, a.Akv, a.z_r, a.tmpV, a.rv);
+    solve_tri(g, f, a.SD, a.D, a.RHS, a.v);
+    correct_mean(g, true, f, a.v, a.Hz, a.DV_avg1, g.om_v, a.tmpV);
+    if (!isUNode(g, f)) return;
+    double HU = 0.0, HV = 0.0, HuU = 0.0, HvV = 0.0;
+    for (long long K = 0; K < N; K++) {
+        const long long b = K * S2 + f;
+        const double hzu = (a.Hz[b] + a.Hz[b - 1]) * 0.5;
+        const double hzv = (a.Hz[b] + a.Hz[b - g.szI]) * 0.5;
+        HU += hzu;
+        HV += hzv;
+        HuU += hzu * a.u[b];
+        HvV += hzv * a.v[b];
+    }
+    const double uUp = (HuU - a.DU_avg1[f] / g.on_u[f]) / HU;
+    const double vUp = (HvV - a.DV_avg1[f] / g.om_v[f]) / HV;
+    const double ub = (a.DU_avg1[f] / g.on_u[f]) / HU + uUp;
+    const double vb = (a.DV_avg1[f] / g.om_v[f]) / HV + vUp;
+    a.ubar_t2[f] = ub;
+    a.vbar_t2[f] = vb;
+    a.ubar_t1[f] = ub;
+    a.vbar_t1[f] = vb;
+}
+
+G3 grid3(const sf_ctx *c)
+{
+    return G3{c->szI, c->szJ, c->szK, c->S2, c->on_u, c->om_v, c->pn, c->pm};
+}
+
+}  // namespace
+
+#define CTX3(c)                                                                  \
+    SF_REQUIRE((c) != nullptr, "%s: null context", __func__);                   \
+    SF_REQUIRE((c)->szK >= 4, "%s: needs at least 4 levels (N=%d)", __func__, (c)->szK)
+
+#define LAUNCH_COLUMNS(kern, c, stream, ...)                                                                  \
+    hipLaunchKernelGGL(kern, dim3(sf::grid1d((c)->S2)), dim3(sf::kBlock1D), 0, sf::as_stream(stream), __VA_ARGS__); \
+    return sf::check_launch(__func__)
+
+extern "C" int sf_add_coriolis(sf_ctx *c, const double *fomn, const double *u, const double *v, double *ru,
+                               double *rv, const double *Hz, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(fomn && u && v && ru && rv && Hz, "sf_add_coriolis: null field");
+    const long long n3 = (c->szK + 1LL) * c->S2;
+    LAUNCH_COLUMNS(k_addCoriolis, c, stream, grid3(c), A{fomn, c->S2}, A{u, n3}, A{v, n3}, ru, rv,
+                   A{Hz, (long long)c->szK * c->S2});
+}
+
+extern "C" int sf_horizontal_advection(sf_ctx *c, const double *u, const double *v, const double *Huon,
+                                       const double *Hvom, double *ru, double *rv, int64_t launch_threads,
+                                       void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(u && v && Huon && Hvom && ru && rv, "sf_horizontal_advection: null field");
+    const long long n3 = (c->szK + 1LL) * c->S2;
+    H h{A{u, n3}, A{v, n3}, A{Huon, n3}, A{Hvom, n3}, c->szI};
+    hipLaunchKernelGGL(k_horizontalAdvection, dim3(sf::grid1d(c->S2)), dim3(sf::kBlock1D), 0, sf::as_stream(stream),
+                       grid3(c), h, ru, rv);
+    int rc = sf::check_launch("sf_horizontal_advection");
+    if (rc != SF_OK) return rc;
+    const long long end = launch_threads < n3 ? (long long)launch_threads : n3;
+    if (end > c->S2) {
+        hipLaunchKernelGGL(k_hadv_tail, dim3(sf::grid1d(end - c->S2)), dim3(sf::kBlock1D), 0, sf::as_stream(stream),
+                           c->S2, end, ru, rv);
+        rc = sf::check_launch("sf_horizontal_advection (tail)");
+    }
+    return rc;
+}
+
+extern "C" int sf_vertical_advection(sf_ctx *c, const double *u, const double *v, const double *W, double *ru,
+                                     double *rv, const int32_t *BC, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(u && v && W && ru && rv && BC, "sf_vertical_advection: null field");
+    const long long n3 = (c->szK + 1LL) * c->S2;
+    LAUNCH_COLUMNS(k_verticalAdvection, c, stream, grid3(c), A{u, n3}, A{v, n3}, A{W, n3}, ru, rv, BC);
+}
+
+extern "C" int sf_set_maxflux(sf_ctx *c, const double *u, const double *v, double *Huon, double *Hvom,
+                              const double *Hz, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(u && v && Huon && Hvom && Hz, "sf_set_maxflux: null field");
+    LAUNCH_COLUMNS(k_set_maxflux, c, stream, grid3(c), u, v, Huon, Hvom, Hz);
+}
+
+extern "C" int sf_omega(sf_ctx *c, double *W, const double *Huon, const double *Hvom, const double *z_w,
+                        const int32_t *BC, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(W && Huon && Hvom && z_w && BC, "sf_omega: null field");
+    const long long n3 = (c->szK + 1LL) * c->S2;
+    LAUNCH_COLUMNS(k_omega, c, stream, grid3(c), W, A{Huon, n3}, A{Hvom, n3}, z_w, BC);
+}
+
+extern "C" int sf_set_depth(sf_ctx *c, int32_t Vtransform, const double *Zt_avg1, double *z_w, double *z_r,
+                            const double *h, double hc, double *Hz, const double *sc_r, const double *sc_w,
+                            const double *Cs_r, const double *Cs_w, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(Zt_avg1 && z_w && z_r && h && Hz && sc_r && sc_w && Cs_r && Cs_w, "sf_set_depth: null field");
+    SF_REQUIRE(Vtransform == 1 || Vtransform == 2, "sf_set_depth: Vtransform=%d", Vtransform);
+    LAUNCH_COLUMNS(k_set_depth, c, stream, grid3(c), Vtransform, Zt_avg1, z_w, z_r, h, hc, Hz, sc_r, sc_w, Cs_r,
+                   Cs_w);
+}
+
+extern "C" int sf_set_zeta(sf_ctx *c, double *zeta_t1, double *zeta_t2, const double *Zt_avg1, void *stream)
+{
+    SF_REQUIRE(c && zeta_t1 && zeta_t2 && Zt_avg1, "sf_set_zeta: null argument");
+    LAUNCH_COLUMNS(k_set_zeta, c, stream, grid3(c), zeta_t1, zeta_t2, Zt_avg1);
+}
+
+extern "C" int sf_set_lateral_uv_bcs(sf_ctx *c, double t, double *u, double *v, const int32_t *uType,
+                                     const int32_t *vType, void *stream)
+{
+    SF_REQUIRE(c && u && v && uType && vType, "sf_set_lateral_uv_bcs: null argument");
+    LAUNCH_COLUMNS(k_lateral_uv, c, stream, grid3(c), t, u, v, uType, vType);
+}
+
+extern "C" int sf_step3d_uv(sf_ctx *c, double *u, double *v, const double *ru, const double *rv, double *ubar_t1,
+                            double *vbar_t1, double *ubar_t2, double *vbar_t2, const double *Hz, const double *Akv,
+                            const double *z_r, const double *DU_avg1, const double *DV_avg1, const double *tmpU,
+                            const double *tmpV, int32_t iic, int32_t ntfirst, double lambda, double Dt, double *SD,
+                            double *D, double *RHS, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(u && v && ru && rv && ubar_t1 && vbar_t1 && ubar_t2 && vbar_t2 && Hz && Akv && z_r && DU_avg1 &&
+                   DV_avg1 && tmpU && tmpV && SD && D && RHS,
+               "sf_step3d_uv: null field");
+    double cff;
+    if (iic == ntfirst) cff = 1.0;
+    else if (iic == ntfirst + 1) cff = 3.0 / 2.0;
+    else cff = 23.0 / 12.0;
+    UVArgs a{u, v, ru, rv, ubar_t1, vbar_t1, ubar_t2, vbar_t2, Hz, Akv, z_r, DU_avg1, DV_avg1, tmpU, tmpV, SD, D, RHS};
+    LAUNCH_COLUMNS(k_step3d_uv, c, stream, grid3(c), a, cff, lambda, Dt);
+}

@@ -176,3 +176,104 @@ def setBC(grid, block, args):
     if n == 0:
         return
     _lib.call("sf_bc_set", _f64(var), _ptr(idx), n, float(val), _stream())
+
+
+# ---- 3D handles (nonlinear/rhs_kernels.cpp, nonlinear/step3D_uv.kernels.cpp) -------
+
+def _launch_threads(grid, block, n_default):
+    """Thread count of a reference launch ``kernel(grid, block, args)``."""
+    try:
+        return int(grid[0]) * int(block[0])
+    except (TypeError, IndexError):
+        return int(n_default)
+
+
+def addCoriolis(grid, block, args, ctx=None):
+    """rhs_kernels.cpp:6-35; args = (fomn, u, v, ru, rv, Hz)."""
+    fomn, u, v, ru, rv, Hz = args
+    c = ctx or ctx_of()
+    _lib.call("sf_add_coriolis", c.handle, _f64(fomn), _f64(u), _f64(v), _f64(ru), _f64(rv), _f64(Hz), _stream())
+
+
+def horizontalAdvection(grid, block, args, ctx=None):
+    """rhs_kernels.cpp:70-132; args = (u, v, Huon, Hvom, ru, rv, BC)."""
+    u, v, Hu, Hv, ru, rv, _BC = args
+    c = ctx or ctx_of()
+    S2 = c.GRID.szI * c.GRID.szJ
+    nt = _launch_threads(grid, block, (S2 // 512 + 1) * 512)
+    _lib.call("sf_horizontal_advection", c.handle, _f64(u), _f64(v), _f64(Hu), _f64(Hv), _f64(ru), _f64(rv),
+              nt, _stream())
+
+
+def verticalAdvection(grid, block, args, ctx=None):
+    """rhs_kernels.cpp:135-197; args = (u, v, W, ru, rv, BC)."""
+    u, v, W, ru, rv, BC = args
+    c = ctx or ctx_of()
+    _lib.call("sf_vertical_advection", c.handle, _f64(u), _f64(v), _f64(W), _f64(ru), _f64(rv), _ptr(BC),
+              _stream())
+
+
+def set_maxflux(grid, block, args, ctx=None):
+    """step3D_uv.kernels.cpp:372-405; args = (u, v, Huon, Hvom, Hz)."""
+    u, v, Hu, Hv, Hz = args
+    c = ctx or ctx_of()
+    _lib.call("sf_set_maxflux", c.handle, _f64(u), _f64(v), _f64(Hu), _f64(Hv), _f64(Hz), _stream())
+
+
+def omega(grid, block, args, ctx=None):
+    """step3D_uv.kernels.cpp:62-165; args = (W, u, v, Huon, Hvom, z_w, BC)."""
+    W, _u, _v, Hu, Hv, z_w, BC = args
+    c = ctx or ctx_of()
+    _lib.call("sf_omega", c.handle, _f64(W), _f64(Hu), _f64(Hv), _f64(z_w), _ptr(BC), _stream())
+
+
+def set_depth(grid, block, args, ctx=None):
+    """step3D_uv.kernels.cpp:225-291; args = (Vtransform, Zt_avg1, z_w, z_r, h, hc, Hz,
+    sc_r, sc_w, Cs_r, Cs_w)."""
+    Vt, Zt, z_w, z_r, h, hc, Hz, sc_r, sc_w, Cs_r, Cs_w = args
+    c = ctx or ctx_of()
+    _lib.call("sf_set_depth", c.handle, int(Vt), _f64(Zt), _f64(z_w), _f64(z_r), _f64(h), float(hc), _f64(Hz),
+              _f64(sc_r), _f64(sc_w), _f64(Cs_r), _f64(Cs_w), _stream())
+
+
+def set_zeta(grid, block, args, ctx=None):
+    """step3D_uv.kernels.cpp:294-310; args = (zeta_t1, zeta_t2, Zt_avg1)."""
+    z1, z2, Zt = args
+    c = ctx or ctx_of()
+    _lib.call("sf_set_zeta", c.handle, _f64(z1), _f64(z2), _f64(Zt), _stream())
+
+
+def setLateralUVBCs(grid, block, args, ctx=None):
+    """step3D_uv.kernels.cpp:314-368; args = (t, u, v, uIdx2, vIdx2, uType, vType)."""
+    t, u, v, _ui, _vi, ut, vt = args
+    c = ctx or ctx_of()
+    vt_host = vt.detach().cpu().numpy()
+    grad_only = (vt_host > 0) & ((vt_host & (1 << 4)) == 0) & ((vt_host & (1 << 3)) == 0) & ((vt_host & (1 << 6)) != 0)
+    if grad_only.any():
+        raise _lib.SailfishError("setLateralUVBCs: gradient v boundaries index the 2D index field at 3D "
+                                 "offsets in the reference (step3D_uv.kernels.cpp:363); not supported")
+    _lib.call("sf_set_lateral_uv_bcs", c.handle, float(t), _f64(u), _f64(v), _ptr(ut), _ptr(vt), _stream())
+
+
+_velEq = {}
+
+
+def setVerticalVelEq(grid, block, args, ctx=None):
+    """step3D_uv.kernels.cpp:21-27: remember the tridiagonal scratch (SD, D, RHS)."""
+    SD, D, RHS = args
+    c = ctx or ctx_of()
+    _velEq[id(c)] = (SD, D, RHS)
+
+
+def step3d_UV(grid, block, args, ctx=None):
+    """step3D_uv.kernels.cpp:646-707; args = (u_t2, v_t2, ru_t2, rv_t2, ubar_t1, vbar_t1,
+    ubar_t2, vbar_t2, Hz, AKv, z_r, DU_avg1, DV_avg1, tmpU, tmpV, h, iic, ntfirst,
+    lambda, AK, dt) as passed by main3d.py:304-306."""
+    (u, v, ru, rv, ub1, vb1, ub2, vb2, Hz, AKv, z_r, DU, DV, tmpU, tmpV, _h, iic, ntfirst, lam, _AK, dt) = args
+    c = ctx or ctx_of()
+    if id(c) not in _velEq:
+        raise _lib.SailfishError("setVerticalVelEq has not been called for this grid")
+    SD, D, RHS = _velEq[id(c)]
+    _lib.call("sf_step3d_uv", c.handle, _f64(u), _f64(v), _f64(ru), _f64(rv), _f64(ub1), _f64(vb1), _f64(ub2),
+              _f64(vb2), _f64(Hz), _f64(AKv), _f64(z_r), _f64(DU), _f64(DV), _f64(tmpU), _f64(tmpV), int(iic),
+              int(ntfirst), float(lam), float(dt), _f64(SD), _f64(D), _f64(RHS), _stream())
