@@ -218,6 +218,32 @@ int sf_step2d_substep_parts(sf_ctx *ctx, const sf_step2d_fields *fields, int32_t
 /* Tile shape (xi, eta) of the fused substep kernel. */
 int sf_step2d_tile_shape(int32_t *ti, int32_t *tj);
 
+/* ---------------- physics-mode barotropic substep -------------------------------- */
+
+/* Fields of one physics-mode substep (ROMS LF-AM3 in 2D form with C2 advection,
+ * Coriolis, wind stress and linear bottom drag, as written in
+ * obsolete/step2d_LF_AM3.h.py:917-2780; closed basin).  Bulk arrays have the
+ * precision passed to sf_phys_substep; the averages are double.  Slot k of each
+ * state array is time level t_k after the caller's rotation (t0 = n-1, t1 = n,
+ * t2 = n+1 is written).  scratch[0..3]: predicted zeta, ubar, vbar and the pressure
+ * surface (szI*szJ each, bulk precision). */
+typedef struct sf_phys_fields {
+    int32_t szI, szJ;
+    double g, rdrg;
+    const void *h, *pm, *pn, *fomn, *sustr, *svstr;
+    void *zeta[3], *ubar[3], *vbar[3];
+    void *rzeta[3], *rubar[3], *rvbar[3];
+    void *scratch[4];
+    double *Zt_avg1, *DU_avg1, *DV_avg1;
+} sf_phys_fields;
+
+/* One physics-mode substep: predictor (leapfrog; forward Euler when first != 0) and
+ * corrector (AM3; backward Euler when first != 0).  cff1 = weight of this substep
+ * in the time averages.  No reference kernel: the live reference has no 2D
+ * advection/Coriolis/stress terms (they exist only as Fortran text). */
+int sf_phys_substep(const sf_phys_fields *fields, int32_t precision, double dt, int32_t first,
+                    double cff1, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,75 @@
+"""ctypes binding of the physics-mode oracle (oracle/sf_oracle_phys.c) and a driver
+over the same slot rotation as T_OCEAN.  TEST INFRASTRUCTURE ONLY."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import oracle as O
+
+P = C.c_void_p
+F64 = C.c_double
+
+
+class sfo_phys(C.Structure):
+    _fields_ = [("szI", C.c_int), ("szJ", C.c_int), ("h", P), ("pm", P), ("pn", P), ("fomn", P),
+                ("sustr", P), ("svstr", P), ("g", F64), ("rdrg", F64)]
+
+
+_bound = False
+
+
+def lib():
+    global _bound
+    L = O.lib()
+    if not _bound:
+        L.sfo_phys_substep.argtypes = [C.POINTER(sfo_phys), F64, C.c_int, F64] + [P] * 22
+        L.sfo_phys_substep.restype = None
+        _bound = True
+    return L
+
+
+class OPhys:
+    """State slots like T_OCEAN (rotating t0/t1/t2 views) plus static fields."""
+
+    def __init__(self, cfg, fomn, rdrg=3e-4, g=10.0):
+        self.szI, self.szJ = int(cfg["szI"]), int(cfg["szJ"])
+        S2 = self.szI * self.szJ
+        flat = lambda a: np.ascontiguousarray(np.asarray(a, float).reshape(-1))
+        z = np.zeros(S2)
+        self.static = {"h": flat(cfg["h"]), "pm": flat(cfg["pm"]), "pn": flat(cfg["pn"]), "fomn": flat(fomn),
+                       "sustr": flat(cfg.get("sustr", z)), "svstr": flat(cfg.get("svstr", z))}
+        s = self.static
+        self.c = sfo_phys(self.szI, self.szJ, O.ptr(s["h"]), O.ptr(s["pm"]), O.ptr(s["pn"]), O.ptr(s["fomn"]),
+                          O.ptr(s["sustr"]), O.ptr(s["svstr"]), g, rdrg)
+        self.st = O.OState(O.OGrid(self.szI, self.szJ, 1, dict(h=s["h"], pm=s["pm"], pn=s["pn"],
+                                                                 on_u=s["pm"], om_v=s["pm"])))
+        for k in range(3):
+            self.st.zeta[k][:] = flat(cfg["zeta0"])
+            if "ubar0" in cfg:
+                self.st.ubar[k][:] = flat(cfg["ubar0"])
+            if "vbar0" in cfg:
+                self.st.vbar[k][:] = flat(cfg["vbar0"])
+        self.scr = [np.zeros(S2) for _ in range(4)]
+
+    def substep(self, dt, first, w1):
+        v = self.st.v
+        p = O.ptr
+        lib().sfo_phys_substep(C.byref(self.c), dt, int(first), w1,
+                               p(v("zeta", 0)), p(v("zeta", 1)), p(v("zeta", 2)),
+                               p(v("ubar", 0)), p(v("ubar", 1)), p(v("ubar", 2)),
+                               p(v("vbar", 0)), p(v("vbar", 1)), p(v("vbar", 2)),
+                               p(v("rzeta", 0)), p(v("rzeta", 1)), p(v("rubar", 0)), p(v("rubar", 1)),
+                               p(v("rvbar", 0)), p(v("rvbar", 1)),
+                               p(self.scr[0]), p(self.scr[1]), p(self.scr[2]), p(self.scr[3]),
+                               p(self.st.Zt_avg1), p(self.st.DU_avg1), p(self.st.DV_avg1))
+
+    def run(self, nsteps, dt, weight1, start_first=True, iif0=0):
+        iif = iif0
+        for n in range(int(nsteps)):
+            iif += 1
+            self.st.cycleTimes2D()
+            k = min(max(iif - 1, 0), len(weight1) - 1)
+            self.substep(dt, start_first and n == 0, float(weight1[k]))
+        return iif

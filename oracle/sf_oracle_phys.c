@@ -1,0 +1,234 @@
+/*
+ * sf_oracle_phys.c -- CPU restatement (FP64) of the ROMS barotropic LF-AM3 step in
+ * its 2D (no SOLVE3D) form, from the Fortran text the Sailfish port keeps in
+ * obsolete/step2d_LF_AM3.h.py:
+ *   free surface  :917-1014   (leapfrog predictor, AM3 corrector, zwrk weights)
+ *   pressure grad.:1107-1237  (rhs = 0.5*g*on_u*((h+h)*(gz-gz) + (gz^2-gz^2)))
+ *   advection     :1244-1290  (UV_C2ADVECTION branch) and :1497-1502 (divergence)
+ *   Coriolis      :1503-1530
+ *   bottom stress :2241-2251  (linear drag bustr = rdrg*ubar, nonlinear/set_vbc.py:8-13)
+ *   surface stress:2595-2606
+ *   momentum step :2623-2780  (first step FE/BE, then LF/AM3)
+ * on the same flat C-grid as the live kernels (u point i between rho i-1 and i,
+ * v point j between rho j-1 and j), with ROMS metrics derived from pm, pn:
+ *   on_u = 2/(pn(i-1)+pn(i)), om_u = 2/(pm(i-1)+pm(i)),
+ *   om_v = 2/(pm(j-1)+pm(j)), on_v = 2/(pn(j-1)+pn(j)).
+ * Closed basin: u = 0 on the walls i=1 and i=L, v = 0 on j=1 and j=M; free-slip
+ * ghost rows/columns (zero gradient) for u, v and zeta.  The barotropic averages
+ * accumulate the predicted state with weight1[iif-1], as the live corrector does
+ * (modules/mod_cppkernels.cpp:116-125).
+ *
+ * TEST INFRASTRUCTURE ONLY (see sf_oracle2d.c).  Operand order below is the order
+ * the GPU kernels use; built with -ffp-contract=off.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+
+typedef struct sfo_phys {
+    int szI, szJ;
+    const double *h, *pm, *pn, *fomn, *sustr, *svstr;
+    double g, rdrg;
+} sfo_phys;
+
+#define IDX(j, i) ((ptrdiff_t)(j) * W + (i))
+
+/* the momentum right-hand sides at u point (j,i) and v point (j,i) of state
+ * (z, u, v) with pressure surface gz; D = h + z. */
+static inline double Dat(const sfo_phys *p, const double *z, ptrdiff_t f) { return p->h[f] + z[f]; }
+
+static double DUon(const sfo_phys *p, const double *z, const double *u, int W, int j, int i)
+{
+    const ptrdiff_t f = IDX(j, i);
+    const double onu = 2.0 / (p->pn[f - 1] + p->pn[f]);
+    return ((0.5 * (Dat(p, z, f - 1) + Dat(p, z, f))) * u[f]) * onu;
+}
+static double DVom(const sfo_phys *p, const double *z, const double *v, int W, int j, int i)
+{
+    const ptrdiff_t f = IDX(j, i);
+    const double omv = 2.0 / (p->pm[f - W] + p->pm[f]);
+    return ((0.5 * (Dat(p, z, f - W) + Dat(p, z, f))) * v[f]) * omv;
+}
+
+double sfo_phys_rhs_zeta(const sfo_phys *p, const double *z, const double *u, const double *v, int j, int i)
+{
+    const int W = p->szI;
+    return (DUon(p, z, u, W, j, i) - DUon(p, z, u, W, j, i + 1)) +
+           (DVom(p, z, v, W, j, i) - DVom(p, z, v, W, j + 1, i));
+}
+
+double sfo_phys_rhs_u(const sfo_phys *p, const double *z, const double *u, const double *v, const double *gz,
+                      int j, int i)
+{
+    const int W = p->szI;
+    const ptrdiff_t f = IDX(j, i);
+    const double onu = 2.0 / (p->pn[f - 1] + p->pn[f]);
+    const double omu = 2.0 / (p->pm[f - 1] + p->pm[f]);
+    /* pressure gradient */
+    const double g0 = gz[f - 1], g1 = gz[f];
+    double r = ((0.5 * p->g) * onu) * ((p->h[f - 1] + p->h[f]) * (g0 - g1) + (g0 * g0 - g1 * g1));
+    /* C2 advection: UFx at rho i-1, i; UFe at psi (j,i), (j+1,i) */
+    const double UFx1 = 0.25 * (DUon(p, z, u, W, j, i) + DUon(p, z, u, W, j, i + 1)) * (u[f] + u[f + 1]);
+    const double UFx0 = 0.25 * (DUon(p, z, u, W, j, i - 1) + DUon(p, z, u, W, j, i)) * (u[f - 1] + u[f]);
+    const double UFe1 = 0.25 * (DVom(p, z, v, W, j + 1, i) + DVom(p, z, v, W, j + 1, i - 1)) * (u[f + W] + u[f]);
+    const double UFe0 = 0.25 * (DVom(p, z, v, W, j, i) + DVom(p, z, v, W, j, i - 1)) * (u[f] + u[f - W]);
+    r = r - ((UFx1 - UFx0) + (UFe1 - UFe0));
+    /* Coriolis: UFx = 0.5*D*fomn*(v(j)+v(j+1)) at rho i-1, i */
+    const double c1 = ((0.5 * Dat(p, z, f)) * p->fomn[f]) * (v[f] + v[f + W]);
+    const double c0 = ((0.5 * Dat(p, z, f - 1)) * p->fomn[f - 1]) * (v[f - 1] + v[f - 1 + W]);
+    r = r + 0.5 * (c1 + c0);
+    /* bottom drag and surface stress */
+    r = r - ((p->rdrg * u[f]) * omu) * onu;
+    r = r + (p->sustr[f] * omu) * onu;
+    return r;
+}
+
+double sfo_phys_rhs_v(const sfo_phys *p, const double *z, const double *u, const double *v, const double *gz,
+                      int j, int i)
+{
+    const int W = p->szI;
+    const ptrdiff_t f = IDX(j, i);
+    const double omv = 2.0 / (p->pm[f - W] + p->pm[f]);
+    const double onv = 2.0 / (p->pn[f - W] + p->pn[f]);
+    const double g0 = gz[f - W], g1 = gz[f];
+    double r = ((0.5 * p->g) * omv) * ((p->h[f - W] + p->h[f]) * (g0 - g1) + (g0 * g0 - g1 * g1));
+    /* C2 advection: VFx at psi (j,i), (j,i+1); VFe at rho j-1, j */
+    const double VFx1 = 0.25 * (DUon(p, z, u, W, j, i + 1) + DUon(p, z, u, W, j - 1, i + 1)) * (v[f + 1] + v[f]);
+    const double VFx0 = 0.25 * (DUon(p, z, u, W, j, i) + DUon(p, z, u, W, j - 1, i)) * (v[f] + v[f - 1]);
+    const double VFe1 = 0.25 * (DVom(p, z, v, W, j, i) + DVom(p, z, v, W, j + 1, i)) * (v[f] + v[f + W]);
+    const double VFe0 = 0.25 * (DVom(p, z, v, W, j - 1, i) + DVom(p, z, v, W, j, i)) * (v[f - W] + v[f]);
+    r = r - ((VFx1 - VFx0) + (VFe1 - VFe0));
+    /* Coriolis: VFe = 0.5*D*fomn*(u(i)+u(i+1)) at rho j-1, j */
+    const double c1 = ((0.5 * Dat(p, z, f)) * p->fomn[f]) * (u[f] + u[f + 1]);
+    const double c0 = ((0.5 * Dat(p, z, f - W)) * p->fomn[f - W]) * (u[f - W] + u[f - W + 1]);
+    r = r - 0.5 * (c1 + c0);
+    r = r - ((p->rdrg * v[f]) * omv) * onv;
+    r = r + (p->svstr[f] * omv) * onv;
+    return r;
+}
+
+/* closed-basin boundary values of one state (free-slip ghosts, zero normal flow) */
+void sfo_phys_bc(const sfo_phys *p, double *z, double *u, double *v)
+{
+    const int W = p->szI, L = p->szI - 1, M = p->szJ - 1;
+    for (int j = 1; j < M; j++) {
+        z[IDX(j, 0)] = z[IDX(j, 1)];
+        z[IDX(j, L)] = z[IDX(j, L - 1)];
+    }
+    for (int i = 0; i <= L; i++) {
+        z[IDX(0, i)] = z[IDX(1, i)];
+        z[IDX(M, i)] = z[IDX(M - 1, i)];
+    }
+    for (int j = 0; j <= M; j++) {
+        u[IDX(j, 0)] = 0.0;
+        u[IDX(j, 1)] = 0.0;
+        u[IDX(j, L)] = 0.0;
+    }
+    for (int i = 2; i < L; i++) {
+        u[IDX(0, i)] = u[IDX(1, i)];
+        u[IDX(M, i)] = u[IDX(M - 1, i)];
+    }
+    for (int i = 0; i <= L; i++) {
+        v[IDX(0, i)] = 0.0;
+        v[IDX(1, i)] = 0.0;
+        v[IDX(M, i)] = 0.0;
+    }
+    for (int j = 2; j < M; j++) {
+        v[IDX(j, 0)] = v[IDX(j, 1)];
+        v[IDX(j, L)] = v[IDX(j, L - 1)];
+    }
+}
+
+/* One substep.  Slots after the caller's rotation: index 0 = n-1, 1 = n, 2 = new.
+ * zs/us/vs: predicted-state scratch (S2 each).  first != 0: forward-Euler
+ * predictor and backward-Euler corrector (no history).  Writes z2/u2/v2, the
+ * predictor RHS into rz1/ru1/rv1, and accumulates the averages. */
+void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
+                      const double *z0, const double *z1, double *z2, const double *u0, const double *u1,
+                      double *u2, const double *v0, const double *v1, double *v2, const double *rz0, double *rz1,
+                      const double *ru0, double *ru1, const double *rv0, double *rv1, double *zs, double *us,
+                      double *vs, double *gz, double *Zt, double *DUa, double *DVa)
+{
+    const int W = p->szI, L = p->szI - 1, M = p->szJ - 1;
+    const ptrdiff_t S2 = (ptrdiff_t)W * p->szJ;
+    /* ---- predictor: zeta ---- */
+    const double cff4 = 4.0 / 25.0, cff5 = 1.0 - 2.0 * cff4;
+    memcpy(zs, z1, sizeof(double) * S2);
+    for (int j = 1; j < M; j++)
+        for (int i = 1; i < L; i++) {
+            const ptrdiff_t f = IDX(j, i);
+            const double r = sfo_phys_rhs_zeta(p, z1, u1, v1, j, i);
+            rz1[f] = r;
+            if (first) zs[f] = z1[f] + ((p->pm[f] * p->pn[f]) * dt) * r;
+            else zs[f] = z0[f] + ((p->pm[f] * p->pn[f]) * (2.0 * dt)) * r;
+        }
+    memcpy(us, u1, sizeof(double) * S2);
+    memcpy(vs, v1, sizeof(double) * S2);
+    sfo_phys_bc(p, zs, us, vs);   /* zeta ghosts of the predicted surface */
+    for (ptrdiff_t f = 0; f < S2; f++)
+        gz[f] = first ? 0.5 * (z1[f] + zs[f]) : cff5 * z1[f] + cff4 * (z0[f] + zs[f]);
+    /* ---- predictor: momentum ---- */
+    const double *zb = first ? z1 : z0, *ub = first ? u1 : u0, *vb = first ? v1 : v0;
+    const double c1 = first ? 0.5 * dt : dt;
+    for (int j = 1; j < M; j++)
+        for (int i = 2; i < L; i++) {
+            const ptrdiff_t f = IDX(j, i);
+            const double r = sfo_phys_rhs_u(p, z1, u1, v1, gz, j, i);
+            ru1[f] = r;
+            const double cff = (p->pm[f] + p->pm[f - 1]) * (p->pn[f] + p->pn[f - 1]);
+            const double fac = 1.0 / (Dat(p, zs, f) + Dat(p, zs, f - 1));
+            us[f] = (ub[f] * (Dat(p, zb, f) + Dat(p, zb, f - 1)) + (cff * c1) * r) * fac;
+        }
+    for (int j = 2; j < M; j++)
+        for (int i = 1; i < L; i++) {
+            const ptrdiff_t f = IDX(j, i);
+            const double r = sfo_phys_rhs_v(p, z1, u1, v1, gz, j, i);
+            rv1[f] = r;
+            const double cff = (p->pm[f] + p->pm[f - W]) * (p->pn[f] + p->pn[f - W]);
+            const double fac = 1.0 / (Dat(p, zs, f) + Dat(p, zs, f - W));
+            vs[f] = (vb[f] * (Dat(p, zb, f) + Dat(p, zb, f - W)) + (cff * c1) * r) * fac;
+        }
+    sfo_phys_bc(p, zs, us, vs);
+    /* ---- corrector: zeta ---- */
+    const double a1 = dt * 5.0 / 12.0, a2 = dt * 8.0 / 12.0, a3 = dt * 1.0 / 12.0;
+    memcpy(z2, z1, sizeof(double) * S2);
+    for (int j = 1; j < M; j++)
+        for (int i = 1; i < L; i++) {
+            const ptrdiff_t f = IDX(j, i);
+            const double r = sfo_phys_rhs_zeta(p, zs, us, vs, j, i);
+            const double pmn = p->pm[f] * p->pn[f];
+            if (first) z2[f] = z1[f] + pmn * (dt * r);
+            else z2[f] = z1[f] + pmn * ((a1 * r + a2 * rz1[f]) - a3 * rz0[f]);
+            /* averages of the predicted state (live corrector rule) */
+            Zt[f] = Zt[f] + cff1w * zs[f];
+            DUa[f] = DUa[f] + cff1w * DUon(p, zs, us, W, j, i);
+            DVa[f] = DVa[f] + cff1w * DVom(p, zs, vs, W, j, i);
+        }
+    memcpy(u2, u1, sizeof(double) * S2);
+    memcpy(v2, v1, sizeof(double) * S2);
+    sfo_phys_bc(p, z2, u2, v2);
+    const double cf4 = 2.0 / 5.0, cf5 = 1.0 - cf4;
+    for (ptrdiff_t f = 0; f < S2; f++) gz[f] = cf5 * z2[f] + cf4 * zs[f];
+    /* ---- corrector: momentum ---- */
+    const double b1 = 0.5 * dt * 5.0 / 12.0, b2 = 0.5 * dt * 8.0 / 12.0, b3 = 0.5 * dt * 1.0 / 12.0;
+    for (int j = 1; j < M; j++)
+        for (int i = 2; i < L; i++) {
+            const ptrdiff_t f = IDX(j, i);
+            const double r = sfo_phys_rhs_u(p, zs, us, vs, gz, j, i);
+            const double cff = (p->pm[f] + p->pm[f - 1]) * (p->pn[f] + p->pn[f - 1]);
+            const double fac = 1.0 / (Dat(p, z2, f) + Dat(p, z2, f - 1));
+            const double tend = first ? (0.5 * dt) * r : (b1 * r + b2 * ru1[f]) - b3 * ru0[f];
+            u2[f] = (u1[f] * (Dat(p, z1, f) + Dat(p, z1, f - 1)) + cff * tend) * fac;
+        }
+    for (int j = 2; j < M; j++)
+        for (int i = 1; i < L; i++) {
+            const ptrdiff_t f = IDX(j, i);
+            const double r = sfo_phys_rhs_v(p, zs, us, vs, gz, j, i);
+            const double cff = (p->pm[f] + p->pm[f - W]) * (p->pn[f] + p->pn[f - W]);
+            const double fac = 1.0 / (Dat(p, z2, f) + Dat(p, z2, f - W));
+            const double tend = first ? (0.5 * dt) * r : (b1 * r + b2 * rv1[f]) - b3 * rv0[f];
+            v2[f] = (v1[f] * (Dat(p, z1, f) + Dat(p, z1, f - W)) + cff * tend) * fac;
+        }
+    sfo_phys_bc(p, z2, u2, v2);
+}

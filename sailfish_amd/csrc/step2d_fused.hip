@@ -99,16 +99,17 @@ __device__ __forceinline__ bool is_masked(long long f, int stride)
 template <typename T>
 __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
 {
-    __shared__ T sD1[N2];    // zeta_t1 + h                                  (R2)
-    __shared__ T sH[N2];     // h                                            (R2)
-    __shared__ T sZ0[N2];    // zeta_t0                                      (R2)
-    __shared__ T sA[N2];     // B-C: FX        E-G: predicted ubar after BC
-    __shared__ T sB[N2];     // B-C: FY        E-G: predicted vbar after BC
-    __shared__ T sC[N2];     // C-D: predicted zeta before BC   F-G: corrected zeta before BC
-    __shared__ T sZp[N2];    // predicted zeta after BC                      (R1)
-    T *const sFX = sA, *const sUp = sA;
-    T *const sFY = sB, *const sVp = sB;
-    T *const sZq = sC, *const sZc = sC;
+    // six LDS planes on the R2 frame, reused as their contents die:
+    //   P1  A-B: D1 = zeta_t1 + h          D-E: D2 = predicted zeta + h
+    //   P2  A-E: h
+    //   P3  B-C: FX                         E-F: PX = DU*on_u of the predicted state
+    //   P4  B-C: FY                         E-F: PY = DV*om_v of the predicted state
+    //   P5  C-D: predicted zeta before BC   F-G: corrected zeta before BC
+    //   P6  D-E: gz = (1-w)*predicted zeta + w*zeta_t0
+    __shared__ T P1[N2], P2[N2], P3[N2], P4[N2], P5[N2], P6[N2];
+    T *const sD1 = P1, *const sD2 = P1, *const sH = P2;
+    T *const sFX = P3, *const sPX = P3, *const sFY = P4, *const sPY = P4;
+    T *const sZq = P5, *const sZc = P5, *const sGZ = P6;
 
     const int szI = P.szI, szJ = P.szJ;
     const int nblk = P.tiles_i * P.tiles_j;
@@ -123,43 +124,86 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
     auto inside = [&](int i, int j) { return i >= 0 && i < szI && j >= 0 && j < szJ; };
     auto interior = [&](int i, int j) { return i > 0 && i < szI - 1 && j > 0 && j < szJ - 1; };
     auto band = [&](int i, int j) { return i <= 1 || i >= szI - 2 || j <= 1 || j >= szJ - 2; };
-    auto in_tile = [&](int i, int j) { return i >= i0 && i < i0 + TI && j >= j0 && j < j0 + TJ; };
 
-    // ---- A -----------------------------------------------------------------------
-    for (int r = tid; r < N2; r += NT) {
+    // Every thread owns the same R2-frame points r = tid + k*NT in all stages, so
+    // per-point inputs stay in registers; only neighbour accesses go through LDS.
+    constexpr int KP = (N2 + NT - 1) / NT;
+    T ub1[KP], vb1[KP], onu[KP], omv[KP], pmr[KP], pnr[KP];
+    T rz0[KP], rz2[KP], up[KP], vp[KP], z0r[KP], hr[KP], zp[KP], dU1[KP], dV1[KP];
+    double zt[KP], du[KP], dv[KP];
+
+    // ---- prologue: every bulk load of the tile, issued back to back ----------------
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const int r = tid + k * NT;
         const int li = r % W2, lj = r / W2;
         const int i = gi0 + li, j = gj0 + lj;
-        T d1 = T(0), hh = T(0), z0 = T(0);
-        if (inside(i, j)) {
+        T z1 = T(0), hh = T(0), z0 = T(0);
+        ub1[k] = vb1[k] = onu[k] = omv[k] = pmr[k] = pnr[k] = rz0[k] = rz2[k] = T(0);
+        zt[k] = du[k] = dv[k] = 0.0;
+        if (r < N2 && inside(i, j)) {
             const long long f = (long long)j * szI + i;
+            z1 = F.zeta_t1[f];
             hh = F.h[f];
-            d1 = F.zeta_t1[f] + hh;
             z0 = F.zeta_t0[f];
+            ub1[k] = F.ubar_t1[f];
+            vb1[k] = F.vbar_t1[f];
+            onu[k] = F.on_u[f];
+            omv[k] = F.om_v[f];
+            if (li >= 1 && li <= TI + 2 && lj >= 1 && lj <= TJ + 2) {
+                pmr[k] = F.pm[f];
+                pnr[k] = F.pn[f];
+            }
+            if (li >= 2 && li <= TI + 1 && lj >= 2 && lj <= TJ + 1) {
+                rz0[k] = F.rzeta_t0[f];
+                rz2[k] = F.rzeta_t2[f];
+                if (interior(i, j)) {
+                    zt[k] = F.Zt_avg1[f];
+                    du[k] = F.DU_avg1[f];
+                    dv[k] = F.DV_avg1[f];
+                }
+            }
         }
-        sD1[r] = d1;
-        sH[r] = hh;
-        sZ0[r] = z0;
+        z0r[k] = z0;
+        hr[k] = hh;
+        if (r < N2) {
+            sD1[r] = z1 + hh;
+            sH[r] = hh;
+        }
     }
     __syncthreads();
 
-    // ---- B: face fluxes on the R2 frame minus its first row/column -----------------
-    for (int r = tid; r < N2; r += NT) {
+    // ---- B: face fluxes of the t1 state (computeZetaRHS, cpp:80-128) ---------------
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const int r = tid + k * NT;
+        if (r >= N2) continue;
         const int li = r % W2, lj = r / W2;
         const int i = gi0 + li, j = gj0 + lj;
         T fx = T(0), fy = T(0);
-        if (li >= 1 && lj >= 1 && inside(i, j) && i >= 1 && j >= 1) {
-            const long long f = (long long)j * szI + i;
+        dU1[k] = dV1[k] = T(0);
+        if (li >= 1 && lj >= 1 && inside(i, j)) {
             const T D = sD1[r];
-            fx = (F.ubar_t1[f] * ((D + sD1[r - 1]) * T(0.5))) * F.on_u[f];
-            fy = (F.vbar_t1[f] * ((D + sD1[r - W2]) * T(0.5))) * F.om_v[f];
+            // ubar_t1*RtoU(D1) is also the first term of computeMomentumPred's
+            // numerator (cpp:183): kept for stage E.  Row 0 reads row -1 as zero
+            // (the reference reads before the array there).
+            dU1[k] = ub1[k] * ((D + sD1[r - 1]) * T(0.5));
+            dV1[k] = vb1[k] * ((D + sD1[r - W2]) * T(0.5));
+            if (i >= 1 && j >= 1) {
+                fx = dU1[k] * onu[k];
+                fy = dV1[k] * omv[k];
+            }
         }
         sFX[r] = fx;
         sFY[r] = fy;
     }
     __syncthreads();
 
-    // ---- C: predicted zeta before BC on R1 ---------------------------------------
-    for (int r = tid; r < N2; r += NT) {
+    // ---- C: predicted zeta before BC on R1 (computeZetaPred, cpp:133-150) ----------
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const int r = tid + k * NT;
+        if (r >= N2) continue;
         const int li = r % W2, lj = r / W2;
         if (li < 1 || li > TI + 2 || lj < 1 || lj > TJ + 2) continue;
         const int i = gi0 + li, j = gj0 + lj;
@@ -167,25 +211,28 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
         if (inside(i, j)) {
             const long long f = (long long)j * szI + i;
             if (i == 0 || j == 0) {
-                // computeZetaPred leaves column 0 and row 0 alone: the buffer keeps its
-                // value.  Outside the own tile such points are never used (they are
-                // boundary-map destinations, never sources), so they are not read.
-                if (in_tile(i, j)) zq = F.zeta_t2[f];
+                // computeZetaPred leaves column 0 and row 0 alone (the buffer keeps its
+                // value).  Outside the own tile such points are boundary-map
+                // destinations whose halo copies are never consumed: not read.
+                if (li >= 2 && li <= TI + 1 && lj >= 2 && lj <= TJ + 1) zq = F.zeta_t2[f];
             } else {
                 T rz;
                 if (interior(i, j))
-                    rz = (((sFX[r + 1] - sFX[r]) + (sFY[r + W2] - sFY[r])) * F.pm[f]) * F.pn[f];
+                    rz = (((sFX[r + 1] - sFX[r]) + (sFY[r + W2] - sFY[r])) * pmr[k]) * pnr[k];
                 else
                     rz = F.rzeta_t1[f];   // computeZetaRHS never writes non-interior points
-                zq = sZ0[r] + Dt * rz;
+                zq = z0r[k] + Dt * rz;
             }
         }
         sZq[r] = zq;
     }
     __syncthreads();
 
-    // ---- D: zeta boundary map on R1 -----------------------------------------------
-    for (int r = tid; r < N2; r += NT) {
+    // ---- D: zeta boundary map on R1 (zetabc.py:5-12) --------------------------------
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const int r = tid + k * NT;
+        if (r >= N2) continue;
         const int li = r % W2, lj = r / W2;
         if (li < 1 || li > TI + 2 || lj < 1 || lj > TJ + 2) continue;
         const int i = gi0 + li, j = gj0 + lj;
@@ -194,125 +241,151 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
             const int s = F.bc_zeta[(long long)j * szI + i];
             if (s >= 0) z = sZq[lidx(s % szI - gi0, s / szI - gj0)];
         }
-        sZp[r] = z;
+        zp[k] = z;
+    }
+    __syncthreads();   // P1's D1 and P5's pre-map values are read above: write after
+    {
+        const T w = T(2.0 / 5.0);
+        const T a = T(1) - w;
+#pragma unroll
+        for (int k = 0; k < KP; k++) {
+            const int r = tid + k * NT;
+            if (r >= N2) continue;
+            const int li = r % W2, lj = r / W2;
+            if (li < 1 || li > TI + 2 || lj < 1 || lj > TJ + 2) continue;
+            sD2[r] = zp[k] + hr[k];
+            sGZ[r] = a * zp[k] + w * z0r[k];
+        }
     }
     __syncthreads();
 
-    // ---- E: momentum RHS and predicted velocities on [i0, i0+TI] x [j0, j0+TJ] ----
+    // ---- E: momentum RHS and predicted velocities on T + right column + top row
+    //      (cpp:12-42, cpp:155-186, barotropicVelocityBC.py:7-31); then the corrector's
+    //      face fluxes PX/PY and time averages (cpp:80-128) ---------------------------
     const T w = T(2.0 / 5.0);
     const T a = T(1) - w;
     const T gc = T(10.0);
-    for (int r = tid; r < (TI + 1) * (TJ + 1); r += NT) {
-        const int ri = r % (TI + 1), rj = r / (TI + 1);
-        const int i = i0 + ri, j = j0 + rj;
-        const int l = lidx(ri + 2, rj + 2);
-        T up = T(0), vp = T(0);
+    const double cff1 = P.cff1;
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const int r = tid + k * NT;
+        if (r >= N2) continue;
+        const int li = r % W2, lj = r / W2;
+        up[k] = vp[k] = T(0);
+        if (li < 2 || li > TI + 2 || lj < 2 || lj > TJ + 2) continue;
+        const int i = gi0 + li, j = gj0 + lj;
+        T px = T(0), py = T(0);
         if (inside(i, j)) {
             const long long f = (long long)j * szI + i;
+            const bool inT = li <= TI + 1 && lj <= TJ + 1;
+            T u2, v2;
             if (!is_masked(f, P.q5_stride)) {
                 T ru, rv;
                 if (interior(i, j)) {
-                    const T gz = a * sZp[l] + w * sZ0[l];
-                    const T gzl = a * sZp[l - 1] + w * sZ0[l - 1];
-                    const T gzd = a * sZp[l - W2] + w * sZ0[l - W2];
-                    const T hu = (sH[l] + sH[l - 1]) * T(0.5);
-                    const T hv = (sH[l] + sH[l - W2]) * T(0.5);
-                    ru = (gc * F.on_u[f]) * (hu * (gz - gzl) + T(0.5) * (gz * gz - gzl * gzl));
-                    rv = (gc * F.om_v[f]) * (hv * (gz - gzd) + T(0.5) * (gz * gz - gzd * gzd));
-                    if (ri < TI && rj < TJ) F.rubar_t1[f] = ru;
+                    const T gz = sGZ[r];
+                    const T gzl = sGZ[r - 1];
+                    const T gzd = sGZ[r - W2];
+                    const T hu = (sH[r] + sH[r - 1]) * T(0.5);
+                    const T hv = (sH[r] + sH[r - W2]) * T(0.5);
+                    ru = (gc * onu[k]) * (hu * (gz - gzl) + T(0.5) * (gz * gz - gzl * gzl));
+                    rv = (gc * omv[k]) * (hv * (gz - gzd) + T(0.5) * (gz * gz - gzd * gzd));
+                    if (inT) F.rubar_t1[f] = ru;
                 } else {
                     ru = F.rubar_t1[f];   // never written at non-interior points
                     rv = F.rvbar_t1[f];
                 }
-                const T D1 = sD1[l], D2 = sZp[l] + sH[l];
-                const T D1l = sD1[l - 1], D2l = sZp[l - 1] + sH[l - 1];
-                const T D1d = sD1[l - W2], D2d = sZp[l - W2] + sH[l - W2];
-                const T pm = F.pm[f], pn = F.pn[f];
-                up = (F.ubar_t1[f] * ((D1 + D1l) * T(0.5)) + ((Dt * ru) * pm) * pn) / ((D2 + D2l) * T(0.5));
-                vp = (F.vbar_t1[f] * ((D1 + D1d) * T(0.5)) + ((Dt * rv) * pm) * pn) / ((D2 + D2d) * T(0.5));
+                const T D2 = sD2[r], D2l = sD2[r - 1], D2d = sD2[r - W2];
+                u2 = (dU1[k] + ((Dt * ru) * pmr[k]) * pnr[k]) / ((D2 + D2l) * T(0.5));
+                v2 = (dV1[k] + ((Dt * rv) * pmr[k]) * pnr[k]) / ((D2 + D2d) * T(0.5));
             } else {
-                // masked: the t2 buffer keeps its value (the masked-points kernel
-                // finalises these points after this kernel)
-                up = F.ubar_t2[f];
-                vp = F.vbar_t2[f];
-                if (interior(i, j) && ri < TI && rj < TJ) {
-                    const T gz = a * sZp[l] + w * sZ0[l];
-                    const T gzl = a * sZp[l - 1] + w * sZ0[l - 1];
-                    const T hu = (sH[l] + sH[l - 1]) * T(0.5);
-                    F.rubar_t1[f] = (gc * F.on_u[f]) * (hu * (gz - gzl) + T(0.5) * (gz * gz - gzl * gzl));
+                // masked: the t2 buffer keeps its value
+                u2 = F.ubar_t2[f];
+                v2 = F.vbar_t2[f];
+                if (interior(i, j) && inT) {
+                    const T gz = sGZ[r];
+                    const T gzl = sGZ[r - 1];
+                    const T hu = (sH[r] + sH[r - 1]) * T(0.5);
+                    F.rubar_t1[f] = (gc * onu[k]) * (hu * (gz - gzl) + T(0.5) * (gz * gz - gzl * gzl));
                 }
             }
             if (band(i, j)) {
                 const int cu = F.bc_ubar[f], cv = F.bc_vbar[f];
-                if (cu == kZero) up = T(0);
-                else if (cu == kClamp) up = T(P.clamp);
-                if (cv == kZero) vp = T(0);
-                else if (cv == kClamp) vp = T(P.clamp);
+                if (cu == kZero) u2 = T(0);
+                else if (cu == kClamp) u2 = T(P.clamp);
+                if (cv == kZero) v2 = T(0);
+                else if (cv == kClamp) v2 = T(P.clamp);
+            }
+            up[k] = u2;
+            vp[k] = v2;
+            // corrector fluxes of the predicted state: DU = ubar*RtoU(D), DV = vbar*RtoV(D)
+            const T D = sD2[r];
+            const T Dl = sD2[r - 1];
+            const T Dd = sD2[r - W2];
+            const T DU = u2 * ((D + Dl) * T(0.5));
+            const T DV = v2 * ((D + Dd) * T(0.5));
+            px = DU * onu[k];
+            py = DV * omv[k];
+            if (inT && interior(i, j)) {
+                F.Zt_avg1[f] = zt[k] + cff1 * (double)zp[k];
+                F.DU_avg1[f] = du[k] + (cff1 * (double)DU) * (double)onu[k];
+                F.DV_avg1[f] = dv[k] + (cff1 * (double)DV) * (double)omv[k];
             }
         }
-        sUp[l] = up;
-        sVp[l] = vp;
+        sPX[r] = px;
+        sPY[r] = py;
     }
     __syncthreads();
 
-    // ---- F: corrector on T -------------------------------------------------------
-    const double cff1 = P.cff1;
-    for (int r = tid; r < TI * TJ; r += NT) {
-        const int ri = r % TI, rj = r / TI;
-        const int i = i0 + ri, j = j0 + rj;
+    // ---- F: corrector on T: rzeta (cpp:80-128), zeta AM3 (cpp:190-209) -----------------
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const int r = tid + k * NT;
+        if (r >= N2) continue;
+        const int li = r % W2, lj = r / W2;
+        if (li < 2 || li > TI + 1 || lj < 2 || lj > TJ + 1) continue;
+        const int i = gi0 + li, j = gj0 + lj;
         if (!inside(i, j)) continue;
-        const int l = lidx(ri + 2, rj + 2);
         const long long f = (long long)j * szI + i;
         T rz;
         if (interior(i, j)) {
-            const T D = sZp[l] + sH[l];
-            const T Dl = sZp[l - 1] + sH[l - 1];
-            const T Dd = sZp[l - W2] + sH[l - W2];
-            const T Dr = sZp[l + 1] + sH[l + 1];
-            const T Du = sZp[l + W2] + sH[l + W2];
-            const T DU = sUp[l] * ((D + Dl) * T(0.5));
-            const T DUr = sUp[l + 1] * ((Dr + D) * T(0.5));
-            const T DV = sVp[l] * ((D + Dd) * T(0.5));
-            const T DVu = sVp[l + W2] * ((Du + D) * T(0.5));
-            const T onu = F.on_u[f], omv = F.om_v[f];
-            rz = (((DUr * F.on_u[f + 1] - DU * onu) + (DVu * F.om_v[f + szI] - DV * omv)) * F.pm[f]) * F.pn[f];
+            rz = (((sPX[r + 1] - sPX[r]) + (sPY[r + W2] - sPY[r])) * pmr[k]) * pnr[k];
             F.rzeta_t1[f] = rz;
-            F.Zt_avg1[f] = F.Zt_avg1[f] + cff1 * (double)sZp[l];
-            F.DU_avg1[f] = F.DU_avg1[f] + (cff1 * (double)DU) * (double)onu;
-            F.DV_avg1[f] = F.DV_avg1[f] + (cff1 * (double)DV) * (double)omv;
         } else {
             rz = F.rzeta_t1[f];
         }
-        sZc[l] = am3(sZp[l], Dt, F.rzeta_t2[f], rz, F.rzeta_t0[f]);
+        sZc[r] = am3(zp[k], Dt, rz2[k], rz, rz0[k]);
     }
     __syncthreads();
 
-    // ---- G: zeta boundary map on T, stores ------------------------------------------
-    for (int r = tid; r < TI * TJ; r += NT) {
-        const int ri = r % TI, rj = r / TI;
-        const int i = i0 + ri, j = j0 + rj;
+    // ---- G: zeta boundary map on T (zetabc.py:5-12), stores ------------------------
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const int r = tid + k * NT;
+        if (r >= N2) continue;
+        const int li = r % W2, lj = r / W2;
+        if (li < 2 || li > TI + 1 || lj < 2 || lj > TJ + 1) continue;
+        const int i = gi0 + li, j = gj0 + lj;
         if (!inside(i, j)) continue;
-        const int l = lidx(ri + 2, rj + 2);
         const long long f = (long long)j * szI + i;
-        T z = sZc[l];
+        T z = sZc[r];
         if (band(i, j)) {
             const int s = F.bc_zeta[f];
             if (s >= 0) z = sZc[lidx(s % szI - gi0, s / szI - gj0)];
         }
         F.zeta_t2[f] = z;
         if (!is_masked(f, P.q5_stride)) {
-            F.ubar_t2[f] = sUp[l];
-            F.vbar_t2[f] = P.zero_vbar ? T(0) : sVp[l];
+            F.ubar_t2[f] = up[k];
+            F.vbar_t2[f] = P.zero_vbar ? T(0) : vp[k];
         } else if (P.fold_masked) {
             // stride == szI: the masked points are column 0 and row 0, which no
-            // other workgroup reads, so they are finished here (same work as
-            // k_step2d_masked)
-            F.ubar_t2[f] = sUp[l];
-            F.vbar_t2[f] = P.zero_vbar ? T(0) : sVp[l];
-            F.ubar_t1[f] = am3(F.ubar_t1[f], Dt, F.rubar_t1[f], F.rubar_t0[f], F.vbar_t1[f]);
+            // other workgroup reads, so they are finished here (as k_step2d_masked)
+            F.ubar_t2[f] = up[k];
+            F.vbar_t2[f] = P.zero_vbar ? T(0) : vp[k];
+            F.ubar_t1[f] = am3(ub1[k], Dt, F.rubar_t1[f], F.rubar_t0[f], vb1[k]);
         }
     }
 }
+
 
 // Masked points (f%stride == 0 or f < stride): velocity boundary constants,
 // main3d's vbar_t2 := 0, and the AM3 update of ubar_t1 from step2d.py:120-123.
