@@ -1,18 +1,25 @@
 #!/usr/bin/env python3
-"""Benchmark of the barotropic LF-AM3 substep (step2dPredictor + step2dCorrector of
-nonlinear/step2d.py) on one MI355X.
+"""Benchmark of the barotropic LF-AM3 substep on one MI355X.
 
-Workload (default): closed basin of 2048 x 1024 rho points, dx = 2 km, seeded
-bathymetry 200..4000 m, beta plane, seeded free-surface noise, dt = 5 s; FP64 state,
-fused substep kernel, reference discretisation (sf_step2d_substep), with the
-column-0/row-0 mask variant of computeMomentumPred (SF_FLAG_FIX_Q5_MASK) so that the
-non-square grid evolves like the square case.  A "step" is one barotropic substep
+Default workload (mode ``phys``): closed basin of 2048 x 1024 rho points, dx = 2 km,
+seeded Gaussian-filtered bathymetry mapped to 200..4000 m, beta plane
+f = 1e-4 + 2e-11 y, double-gyre wind tau_x = 0.1 cos(2 pi y / Ly), seeded 1 cm
+free-surface noise, linear drag 3e-4 m/s, dt = 5 s; the ROMS 2D LF-AM3 substep
+(predictor + corrector with C2 advection, Coriolis, stresses, drag) as ONE fused
+launch (sf_phys_substep_fused).  FP64 state by default; ``--dtype f32`` keeps the
+state in FP32 and the time averages in FP64.  A "step" is one barotropic substep
 over the whole grid; inputs are resident in HBM before timing starts.
 
-Prints one JSON line.  ``roofline`` is the dominant kernel (the tiled substep
-kernel): algorithmic bytes per launch over its mean launch time, measured with HIP
-events on the launch stream.  ``cpu_baseline`` times the CPU oracle (C restatement
-of the reference kernels, OpenMP) on a bounded prefix of the same workload.
+Mode ``faithful``: the reference discretisation of nonlinear/step2d.py
+(sf_step2d_substep, fused tile kernel) on the same grid, with the column-0/row-0
+mask variant of computeMomentumPred.
+
+Prints one JSON line.  ``roofline`` is the dominant kernel: algorithmic bytes per
+launch over its mean launch time, measured with HIP events on the launch stream.
+``cpu_baseline`` times the CPU oracle (C restatement, OpenMP) on a bounded number
+of substeps of the same workload.  ``secondary`` repeats the GPU measurement in the
+other precision.  ``--calib-copy`` instead runs a plain device copy of known size
+(the calibration run for the HBM byte counters).
 """
 from __future__ import annotations
 
@@ -29,28 +36,85 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-steps/s + % HBM roofline; closure-coarse vs fine run at equal zeta RMSE"
 
-# Algorithmic HBM bytes per rho point of the tiled substep kernel: every field read
-# or written once (boundary-only reads omitted).
+# Algorithmic HBM bytes per rho point and substep, every field read or written once.
+# physics substep (k_phys_fused):
+#   reads : h, pm, pn, fomn, sustr, svstr, zeta/ubar/vbar at t0 and t1,
+#           rzeta/rubar/rvbar at t0                                  (15 bulk)
+#           Zt_avg1, DU_avg1, DV_avg1                                (3 FP64)
+#   writes: zeta/ubar/vbar at t2, rzeta/rubar/rvbar at t1            (6 bulk)
+#           Zt_avg1, DU_avg1, DV_avg1                                (3 FP64)
+PHYS_BULK, PHYS_F64 = 21, 6
+# reference-discretisation substep (k_step2d_fused):
 #   reads : zeta_t1, zeta_t0, h, ubar_t1, vbar_t1, on_u, om_v, pm, pn,
-#           rzeta_t0, rzeta_t2                                  (11 bulk)
-#           Zt_avg1, DU_avg1, DV_avg1                           (3 FP64)
-#   writes: zeta_t2, ubar_t2, vbar_t2, rzeta_t1, rubar_t1       (5 bulk)
-#           Zt_avg1, DU_avg1, DV_avg1                           (3 FP64)
-BULK_FIELDS = 16
-F64_FIELDS = 6
+#           rzeta_t0, rzeta_t2 (11 bulk), Zt/DU/DV_avg1 (3 FP64)
+#   writes: zeta_t2, ubar_t2, vbar_t2, rzeta_t1, rubar_t1 (5 bulk), averages (3 FP64)
+FAITHFUL_BULK, FAITHFUL_F64 = 16, 6
 
 
-def bytes_per_point(bulk_bytes: int) -> int:
-    return BULK_FIELDS * bulk_bytes + F64_FIELDS * 8
+def _events(torch, n):
+    return [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
 
 
-def run_gpu(args):
+def run_phys_gpu(args, dtype_name, steps, warmup):
+    import torch
+    from sailfish_amd import configs
+    from sailfish_amd.model2d import build_model
+    from sailfish_amd.phys2d import PhysStepper
+
+    dtype = torch.float64 if dtype_name == "f64" else torch.float32
+    cfg = configs.basin2(args.szI, args.szJ, dx=args.dx)
+    ct, GRID, OCEAN, _B = build_model(cfg, dtype=dtype, ndtfast=30)
+    st = PhysStepper(GRID, OCEAN, rdrg=cfg["rdrg"], fused=True, tile_j=args.tile_j)
+    tile_j = args.tile_j or st.lib.sf_phys_fused_default_tile(st.precision)
+    S2 = GRID.szI * GRID.szJ
+    n = [0]
+
+    def substep():
+        ct.next2DTimeStep()
+        OCEAN.cycleTimes2D()
+        st.step(ct, OCEAN, first=(n[0] == 0))
+        n[0] += 1
+
+    for _ in range(max(warmup, 1)):
+        substep()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0, ev1 = _events(torch, 1)[0]
+    ev0.record()
+    for _ in range(steps):
+        substep()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = ev0.elapsed_time(ev1) / steps
+
+    nk = max(10, min(steps, 100))
+    evs = _events(torch, nk)
+    for a, b in evs:
+        ct.next2DTimeStep()
+        OCEAN.cycleTimes2D()
+        a.record()
+        st.step(ct, OCEAN, first=False)
+        b.record()
+    torch.cuda.synchronize()
+    kern_ms = sum(a.elapsed_time(b) for a, b in evs) / nk
+    finite = bool(torch.isfinite(OCEAN.zeta_t2).all()) and bool(torch.isfinite(OCEAN.ubar_t2).all())
+    zmax = float(OCEAN.zeta_t2.abs().max())
+    bulk = 8 if dtype_name == "f64" else 4
+    bpp = PHYS_BULK * bulk + PHYS_F64 * 8
+    achieved = bpp * S2 / (kern_ms * 1e-3) / 1e9
+    return dict(cfg=cfg, S2=S2, ms_per_step=ms_per_step, wall=wall, kern_ms=kern_ms, bpp=bpp,
+                achieved=achieved, finite=finite, zmax=zmax, nk=nk, kernel="k_phys_fused",
+                entry="sf_phys_substep_fused", tile=[64, tile_j], substeps=n[0] + nk)
+
+
+def run_faithful_gpu(args, dtype_name, steps, warmup):
     import torch
     from sailfish_amd import configs
     from sailfish_amd.model2d import build_model
     from sailfish_amd.step2d_fused import FusedStepper
 
-    dtype = torch.float64 if args.dtype == "f64" else torch.float32
+    dtype = torch.float64 if dtype_name == "f64" else torch.float32
     cfg = configs.basin2(args.szI, args.szJ, dx=args.dx)
     ct, GRID, OCEAN, BOUNDARY = build_model(cfg, dtype=dtype, ndtfast=30, fix_q5=True)
     st = FusedStepper(GRID, BOUNDARY, dtype)
@@ -61,27 +125,21 @@ def run_gpu(args):
         OCEAN.cycleTimes2D()
         st.step(ct, OCEAN, parts=parts)
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         substep()
     torch.cuda.synchronize()
-
-    # timed region: K whole substeps
     t0 = time.perf_counter()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0, ev1 = _events(torch, 1)[0]
     ev0.record()
-    for _ in range(args.steps):
+    for _ in range(steps):
         substep()
     ev1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms_total = ev0.elapsed_time(ev1)
-    ms_per_step = ms_total / args.steps
-
-    # dominant kernel: HIP events around each tiled launch (the masked-points
-    # launch follows outside the events), same stream as the launches
-    nk = max(10, min(args.steps, 100))
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(nk)]
+    ms_per_step = ev0.elapsed_time(ev1) / steps
+    # the tiled launch alone (the masked-points launch follows outside the events)
+    nk = max(10, min(steps, 100))
+    evs = _events(torch, nk)
     for a, b in evs:
         ct.next2DTimeStep()
         OCEAN.cycleTimes2D()
@@ -91,21 +149,52 @@ def run_gpu(args):
         st.step(ct, OCEAN, parts=2)
     torch.cuda.synchronize()
     kern_ms = sum(a.elapsed_time(b) for a, b in evs) / nk
-
-    z = OCEAN.zeta_t2.detach().float().cpu()
-    finite = bool(torch.isfinite(z).all())
-    bulk = 8 if args.dtype == "f64" else 4
-    bpp = bytes_per_point(bulk)
-    traffic = bpp * S2
-    achieved = traffic / (kern_ms * 1e-3) / 1e9
+    finite = bool(torch.isfinite(OCEAN.zeta_t2).all())
+    bulk = 8 if dtype_name == "f64" else 4
+    bpp = FAITHFUL_BULK * bulk + FAITHFUL_F64 * 8
+    achieved = bpp * S2 / (kern_ms * 1e-3) / 1e9
     return dict(cfg=cfg, S2=S2, ms_per_step=ms_per_step, wall=wall, kern_ms=kern_ms, bpp=bpp,
-                achieved=achieved, finite=finite, nk=nk)
+                achieved=achieved, finite=finite, zmax=float(OCEAN.zeta_t2.abs().max()), nk=nk,
+                kernel="k_step2d_fused", entry="sf_step2d_substep", tile=None, substeps=warmup + steps + nk)
 
 
-def run_cpu_baseline(args, budget_s: float):
-    """Oracle (one C function per reference kernel, OpenMP) on the same grid for a
-    bounded number of substeps."""
-    import numpy as np
+def cpu_threads():
+    return int(os.environ.get("SF_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+
+
+def run_cpu_phys(args, budget_s):
+    """Physics-mode oracle (oracle/sf_oracle_phys.c, FP64, OpenMP) on the same grid."""
+    from oracle import oracle as O
+    from oracle import oracle_phys as OP
+    from sailfish_amd import configs
+    from sailfish_amd.mod_comptimes import CompTimes
+    from sailfish_amd.mod_grid import update_metrics
+    from sailfish_amd.set_weights import set_weights
+
+    cfg = configs.basin2(args.szI, args.szJ, dx=args.dx)
+    threads = cpu_threads()
+    O.set_threads(threads)
+    ct = CompTimes(cfg["dtfast"] * 30, 30)
+    set_weights(ct)
+    fomn = update_metrics(cfg["pm"], cfg["pn"], cfg["f"])["fomn"]
+    o = OP.OPhys(cfg, fomn, rdrg=cfg["rdrg"])
+    iif = o.run(1, cfg["dtfast"], ct.weight1)            # first (Euler) substep, untimed
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        iif = o.run(1, cfg["dtfast"], ct.weight1, start_first=False, iif0=iif)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 100000:
+            break
+    S2 = cfg["szI"] * cfg["szJ"]
+    return dict(value=S2 * n / el, unit="cell-steps/s", cores=threads, kind="port",
+                sample=f"{n} physics substeps of the {cfg['szI']}x{cfg['szJ']} basin (oracle C restatement, "
+                       f"FP64, {threads} OpenMP threads, {el:.1f} s)")
+
+
+def run_cpu_faithful(args, budget_s):
+    """Reference-discretisation oracle (one C function per reference kernel, OpenMP)."""
     from oracle import oracle as O
     from sailfish_amd import configs
     from sailfish_amd.mod_comptimes import CompTimes
@@ -113,7 +202,7 @@ def run_cpu_baseline(args, budget_s: float):
     from sailfish_amd.set_weights import set_weights
 
     cfg = configs.basin2(args.szI, args.szJ, dx=args.dx)
-    threads = int(os.environ.get("SF_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     O.set_threads(threads)
     ct = CompTimes(cfg["dtfast"] * 30, 30)
     set_weights(ct)
@@ -125,14 +214,14 @@ def run_cpu_baseline(args, budget_s: float):
         s.zeta[k][:] = cfg["zeta0"].ravel()
     b = O.OBoundary(g)
     t = O.OTimes(cfg["dtfast"], ct.weight1, ct.nfast)
-    O.main2d(t, g, s, b, 1)    # warm-up
+    O.main2d(t, g, s, b, 1)
     n = 0
     t0 = time.perf_counter()
     while True:
         O.main2d(t, g, s, b, 1)
         n += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or n >= 1000:
+        if el >= budget_s or n >= 100000:
             break
     S2 = cfg["szI"] * cfg["szJ"]
     return dict(value=S2 * n / el, unit="cell-steps/s", cores=threads, kind="port",
@@ -140,26 +229,61 @@ def run_cpu_baseline(args, budget_s: float):
                        f"the reference kernels, FP64, {threads} OpenMP threads, {el:.1f} s)")
 
 
+def calib_copy(args):
+    """Plain device-to-device copy of 1 GiB, 5 times: the byte counters of this
+    kernel calibrate FETCH_SIZE / WRITE_SIZE (2^30 bytes read and written each)."""
+    import torch
+    n = 1 << 30
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calib_copy_bytes": n, "launches": 5}))
+
+
+def summary(r, dtype_name, steps, warmup):
+    value = r["S2"] / (r["ms_per_step"] * 1e-3)
+    return dict(value=value, ms_per_step=r["ms_per_step"], dtype=dtype_name, steps=steps, warmup=warmup,
+                kernel=r["kernel"], kernel_ms=r["kern_ms"], bytes_per_point=r["bpp"],
+                achieved_GBs=r["achieved"], frac=r["achieved"] / HBM_PEAK_GBS, finite_after_run=r["finite"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--mode", choices=["phys", "faithful"], default="phys")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
+    ap.add_argument("--tile-j", type=int, default=0)
     ap.add_argument("--szI", type=int, default=2048)
     ap.add_argument("--szJ", type=int, default=1024)
     ap.add_argument("--dx", type=float, default=2e3)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the barotropic path would shard as row slabs with halo exchange; see DESIGN.md)",
+              "(the barotropic path would shard as row slabs with a 3-row halo exchange per substep)",
               file=sys.stderr)
         sys.exit(2)
+    if args.calib_copy:
+        calib_copy(args)
+        return
 
-    r = run_gpu(args)
+    run = run_phys_gpu if args.mode == "phys" else run_faithful_gpu
+    r = run(args, args.dtype, args.steps, args.warmup)
     value = r["S2"] / (r["ms_per_step"] * 1e-3)
+    if args.mode == "phys":
+        workload = (f"ROMS 2D LF-AM3 barotropic substep (predictor+corrector, C2 advection, Coriolis, "
+                    f"wind, linear drag), one fused launch; closed basin {args.szI}x{args.szJ} rho points, "
+                    f"dx={args.dx/1e3:g} km, beta plane, dt=5 s")
+    else:
+        workload = (f"reference step2d discretisation (predictor+corrector), fused tile kernel; closed basin "
+                    f"{args.szI}x{args.szJ} rho points, dx={args.dx/1e3:g} km, dt=5 s, column-0/row-0 mask")
     out = {
         "metric": METRIC,
         "value": value,
@@ -174,13 +298,14 @@ def main():
         "dtype": args.dtype,
         "data": "synthetic",
         "config": {
-            "workload": f"barotropic LF-AM3 substep (step2d predictor+corrector, fused), closed basin "
-                        f"{args.szI}x{args.szJ} rho points, dx={args.dx/1e3:g} km, dt=5 s, "
-                        f"{'FP64' if args.dtype == 'f64' else 'FP32 bulk + FP64 averages'}",
+            "workload": workload,
             "grid": [args.szI, args.szJ],
-            "kernel": "sf_step2d_substep",
-            "mask": "SF_FLAG_FIX_Q5_MASK",
+            "precision": "FP64" if args.dtype == "f64" else "FP32 state, FP64 time averages",
+            "entry": r["entry"],
+            "tile": r["tile"],
             "finite_after_run": r["finite"],
+            "max_abs_zeta_after_run": r["zmax"],
+            "substeps_run": r["substeps"],
         },
         "roofline": {
             "bound": "hbm",
@@ -189,14 +314,19 @@ def main():
             "unit": "GB/s",
             "frac": r["achieved"] / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_step2d_fused",
+            "kernel": r["kernel"],
             "kernel_ms": r["kern_ms"],
             "bytes_per_point": r["bpp"],
+            "points_per_launch": r["S2"],
             "launches_timed": r["nk"],
         },
     }
+    if not args.no_secondary:
+        other = "f32" if args.dtype == "f64" else "f64"
+        s2 = run(args, other, min(args.steps, 200), args.warmup)
+        out["secondary"] = summary(s2, other, min(args.steps, 200), args.warmup)
     if not args.no_cpu:
-        out["cpu_baseline"] = run_cpu_baseline(args, args.cpu_budget)
+        out["cpu_baseline"] = (run_cpu_phys if args.mode == "phys" else run_cpu_faithful)(args, args.cpu_budget)
     print(json.dumps(out))
 
 

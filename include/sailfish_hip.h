@@ -244,6 +244,17 @@ typedef struct sf_phys_fields {
 int sf_phys_substep(const sf_phys_fields *fields, int32_t precision, double dt, int32_t first,
                     double cff1, void *stream);
 
+/* The same substep as ONE launch: each workgroup evaluates predictor and corrector
+ * on an LDS-staged tile of 64 x tile_j points (tile_j = 4, 8 or 16; 0 = default)
+ * with a 3/2-point halo, so no intermediate state goes to memory.  scratch[] is
+ * not used.  Results equal sf_phys_substep bit for bit in both precisions.  Same
+ * reference lineage as sf_phys_substep. */
+int sf_phys_substep_fused(const sf_phys_fields *fields, int32_t precision, double dt, int32_t first,
+                          double cff1, int32_t tile_j, void *stream);
+
+/* Default tile height of sf_phys_substep_fused for a precision. */
+int sf_phys_fused_default_tile(int32_t precision);
+
 #ifdef __cplusplus
 }
 #endif

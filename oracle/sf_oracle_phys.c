@@ -25,6 +25,7 @@
 #include <stddef.h>
 #include <stdint.h>
 #include <string.h>
+#include <omp.h>
 
 typedef struct sfo_phys {
     int szI, szJ;
@@ -155,6 +156,7 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
     /* ---- predictor: zeta ---- */
     const double cff4 = 4.0 / 25.0, cff5 = 1.0 - 2.0 * cff4;
     memcpy(zs, z1, sizeof(double) * S2);
+#pragma omp parallel for schedule(static)
     for (int j = 1; j < M; j++)
         for (int i = 1; i < L; i++) {
             const ptrdiff_t f = IDX(j, i);
@@ -166,11 +168,13 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
     memcpy(us, u1, sizeof(double) * S2);
     memcpy(vs, v1, sizeof(double) * S2);
     sfo_phys_bc(p, zs, us, vs);   /* zeta ghosts of the predicted surface */
+#pragma omp parallel for schedule(static)
     for (ptrdiff_t f = 0; f < S2; f++)
         gz[f] = first ? 0.5 * (z1[f] + zs[f]) : cff5 * z1[f] + cff4 * (z0[f] + zs[f]);
     /* ---- predictor: momentum ---- */
     const double *zb = first ? z1 : z0, *ub = first ? u1 : u0, *vb = first ? v1 : v0;
     const double c1 = first ? 0.5 * dt : dt;
+#pragma omp parallel for schedule(static)
     for (int j = 1; j < M; j++)
         for (int i = 2; i < L; i++) {
             const ptrdiff_t f = IDX(j, i);
@@ -180,6 +184,7 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
             const double fac = 1.0 / (Dat(p, zs, f) + Dat(p, zs, f - 1));
             us[f] = (ub[f] * (Dat(p, zb, f) + Dat(p, zb, f - 1)) + (cff * c1) * r) * fac;
         }
+#pragma omp parallel for schedule(static)
     for (int j = 2; j < M; j++)
         for (int i = 1; i < L; i++) {
             const ptrdiff_t f = IDX(j, i);
@@ -193,6 +198,7 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
     /* ---- corrector: zeta ---- */
     const double a1 = dt * 5.0 / 12.0, a2 = dt * 8.0 / 12.0, a3 = dt * 1.0 / 12.0;
     memcpy(z2, z1, sizeof(double) * S2);
+#pragma omp parallel for schedule(static)
     for (int j = 1; j < M; j++)
         for (int i = 1; i < L; i++) {
             const ptrdiff_t f = IDX(j, i);
@@ -209,9 +215,11 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
     memcpy(v2, v1, sizeof(double) * S2);
     sfo_phys_bc(p, z2, u2, v2);
     const double cf4 = 2.0 / 5.0, cf5 = 1.0 - cf4;
+#pragma omp parallel for schedule(static)
     for (ptrdiff_t f = 0; f < S2; f++) gz[f] = cf5 * z2[f] + cf4 * zs[f];
     /* ---- corrector: momentum ---- */
     const double b1 = 0.5 * dt * 5.0 / 12.0, b2 = 0.5 * dt * 8.0 / 12.0, b3 = 0.5 * dt * 1.0 / 12.0;
+#pragma omp parallel for schedule(static)
     for (int j = 1; j < M; j++)
         for (int i = 2; i < L; i++) {
             const ptrdiff_t f = IDX(j, i);
@@ -221,6 +229,7 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
             const double tend = first ? (0.5 * dt) * r : (b1 * r + b2 * ru1[f]) - b3 * ru0[f];
             u2[f] = (u1[f] * (Dat(p, z1, f) + Dat(p, z1, f - 1)) + cff * tend) * fac;
         }
+#pragma omp parallel for schedule(static)
     for (int j = 2; j < M; j++)
         for (int i = 1; i < L; i++) {
             const ptrdiff_t f = IDX(j, i);

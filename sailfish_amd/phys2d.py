@@ -1,7 +1,7 @@
 """Physics-mode barotropic model: ROMS LF-AM3 in 2D form with C2 momentum
 advection, Coriolis, wind stress and linear bottom drag, on the reference's grid
 layout and T_OCEAN arrays (obsolete/step2d_LF_AM3.h.py:917-2780).  One substep =
-four gfx950 launches (sf_phys_substep)."""
+four gfx950 launches (sf_phys_substep) or one fused launch (sf_phys_substep_fused)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -30,13 +30,22 @@ def _bind():
     lib = _lib.load()
     lib.sf_phys_substep.argtypes = [C.POINTER(sf_phys_fields), C.c_int32, C.c_double, C.c_int32, C.c_double, P]
     lib.sf_phys_substep.restype = C.c_int32
+    lib.sf_phys_substep_fused.argtypes = [C.POINTER(sf_phys_fields), C.c_int32, C.c_double, C.c_int32, C.c_double,
+                                          C.c_int32, P]
+    lib.sf_phys_substep_fused.restype = C.c_int32
+    lib.sf_phys_fused_default_tile.argtypes = [C.c_int32]
+    lib.sf_phys_fused_default_tile.restype = C.c_int32
     return lib
 
 
 class PhysStepper:
-    """Static fields in the state precision plus the predicted-state scratch."""
+    """Static fields in the state precision plus the predicted-state scratch.
 
-    def __init__(self, GRID, OCEAN, rdrg: float = 3e-4, g: float = G_CONST):
+    ``fused``: one launch per substep (tile height ``tile_j``, 0 = library default)
+    instead of the four-launch form."""
+
+    def __init__(self, GRID, OCEAN, rdrg: float = 3e-4, g: float = G_CONST, fused: bool = True,
+                 tile_j: int = 0):
         _lib.require_gpu()
         self.lib = _bind()
         dt = OCEAN.dtype
@@ -54,9 +63,11 @@ class PhysStepper:
             "sustr": torch.from_numpy(host.get("sustr", zeros).reshape(-1).copy()).to(dev, dt),
             "svstr": torch.from_numpy(host.get("svstr", zeros).reshape(-1).copy()).to(dev, dt),
         }
-        self.scratch = [torch.zeros(S2, dtype=dt, device=dev) for _ in range(4)]
         self.szI, self.szJ = GRID.szI, GRID.szJ
         self.g, self.rdrg = float(g), float(rdrg)
+        self.fused, self.tile_j = bool(fused), int(tile_j)
+        # predicted-state scratch of the four-launch form (the fused form keeps it in LDS)
+        self.scratch = [] if self.fused else [torch.zeros(S2, dtype=dt, device=dev) for _ in range(4)]
 
     def fields(self, OCEAN) -> sf_phys_fields:
         a = sf_phys_fields()
@@ -68,13 +79,19 @@ class PhysStepper:
             arr = getattr(a, name)
             for k in range(3):
                 arr[k] = getattr(OCEAN, f"{name}_t{k}").data_ptr()
-        for k in range(4):
-            a.scratch[k] = self.scratch[k].data_ptr()
+        for k, buf in enumerate(self.scratch):
+            a.scratch[k] = buf.data_ptr()
         a.Zt_avg1, a.DU_avg1, a.DV_avg1 = OCEAN.Zt_avg1.data_ptr(), OCEAN.DU_avg1.data_ptr(), OCEAN.DV_avg1.data_ptr()
         return a
 
     def step(self, compTimes, OCEAN, first: bool, stream=None):
         a = self.fields(OCEAN)
+        if self.fused:
+            rc = self.lib.sf_phys_substep_fused(C.byref(a), self.precision, float(compTimes.dtfast),
+                                                int(bool(first)), compTimes.cff1(), self.tile_j,
+                                                _lib.stream_handle(stream))
+            _lib.check(rc, "sf_phys_substep_fused")
+            return
         rc = self.lib.sf_phys_substep(C.byref(a), self.precision, float(compTimes.dtfast), int(bool(first)),
                                       compTimes.cff1(), _lib.stream_handle(stream))
         _lib.check(rc, "sf_phys_substep")

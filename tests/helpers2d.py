@@ -50,13 +50,14 @@ def oracle_arrays(s):
 def bitwise_equal(a, b):
     """Bit-for-bit equality, except that any NaN equals any NaN: the payload of a
     generated NaN differs between x86 (0xfff8...) and AMD GPUs (0x7ff8...)."""
-    a = np.asarray(a); b = np.asarray(b)
-    if a.shape != b.shape:
+    a = np.ascontiguousarray(a); b = np.ascontiguousarray(b)
+    if a.shape != b.shape or a.dtype != b.dtype:
         return False
     na, nb = np.isnan(a), np.isnan(b)
     if not np.array_equal(na, nb):
         return False
-    return np.array_equal(a[~na].view(np.uint64), b[~nb].view(np.uint64))
+    u = {8: np.uint64, 4: np.uint32, 2: np.uint16}[a.dtype.itemsize]
+    return np.array_equal(a[~na].view(u), b[~nb].view(u))
 
 
 def compare_states(prod, orc, names=STATE2D + ("Zt_avg1", "DU_avg1", "DV_avg1"), skip=()):

@@ -1,5 +1,7 @@
-"""GPU parity of the physics-mode substep (sf_phys_substep) against the CPU oracle:
-FP64 bit for bit on every state slot; FP32 within a stated tolerance."""
+"""GPU parity of the physics-mode substep against the CPU oracle: the four-launch
+form (sf_phys_substep) and the fused one-launch form (sf_phys_substep_fused, every
+tile height) equal the oracle bit for bit in FP64; in FP32 the fused form equals
+the four-launch form bit for bit and both stay within a stated tolerance of FP64."""
 from __future__ import annotations
 
 import sys
@@ -12,54 +14,104 @@ from tests.helpers2d import bitwise_equal
 
 pytestmark = pytest.mark.gpu
 
+FIELDS = ("zeta", "ubar", "vbar")
+RHS = ("rzeta", "rubar", "rvbar")
+AVG = ("Zt_avg1", "DU_avg1", "DV_avg1")
 
-def _run(cfg, nsteps, dtype):
+
+def _gpu_run(cfg, nsteps, dtype, fused, tile_j=0):
     import torch
-    from oracle import oracle_phys as OP
     from sailfish_amd.model2d import build_model
-    from sailfish_amd.phys2d import run_phys
+    from sailfish_amd.phys2d import PhysStepper, run_phys
 
     ct, GRID, OCEAN, _B = build_model(cfg, dtype=dtype, ndtfast=30)
-    run_phys(ct, GRID, OCEAN, nsteps)
+    st = PhysStepper(GRID, OCEAN, fused=fused, tile_j=tile_j)
+    run_phys(ct, GRID, OCEAN, nsteps, stepper=st)
     torch.cuda.synchronize()
+    return ct, GRID, OCEAN
+
+
+def _oracle_run(cfg, GRID, ct, nsteps):
+    from oracle import oracle_phys as OP
     o = OP.OPhys(cfg, GRID.host["fomn"])
     o.run(nsteps, cfg["dtfast"], ct.weight1)
-    return OCEAN, o
+    return o
 
 
-@pytest.mark.parametrize("szI,szJ", [(40, 24), (67, 33), (128, 64)])
-def test_phys_fp64_bitwise(szI, szJ):
-    import torch
-    print(f"[case] phys fp64 {szI}x{szJ}", file=sys.stderr, flush=True)
-    cfg = configs.random_smooth(szI, szJ, seed=szI * 3 + szJ) if szI != 128 else configs.basin1()
-    n = 12 if szI != 128 else 60
-    OCEAN, o = _run(cfg, n, torch.float64)
+def _mismatches(OCEAN, o):
     bad = []
-    for name in ("zeta", "ubar", "vbar"):
+    for name in FIELDS:
         for k in range(3):
-            if not bitwise_equal(getattr(OCEAN, name)[k].cpu().numpy().ravel(), getattr(o.st, name)[k]):
+            if not bitwise_equal(getattr(OCEAN, f"{name}_t{k}").cpu().numpy().ravel(), o.st.v(name, k)):
                 bad.append((name, k))
-    for name in ("rzeta", "rubar", "rvbar"):
-        for k in range(3):
-            if not bitwise_equal(getattr(OCEAN, name)[k].cpu().numpy().ravel(), getattr(o.st, name)[k]):
+    for name in RHS:
+        for k in range(2):
+            if not bitwise_equal(getattr(OCEAN, f"{name}_t{k}").cpu().numpy().ravel(), o.st.v(name, k)):
                 bad.append((name, k))
-    for name in ("Zt_avg1", "DU_avg1", "DV_avg1"):
+    for name in AVG:
         if not bitwise_equal(getattr(OCEAN, name).cpu().numpy(), getattr(o.st, name)):
+            bad.append(name)
+    return bad
+
+
+def _case_cfg(szI, szJ):
+    if (szI, szJ) == (128, 64):
+        return configs.basin1(), 60
+    return configs.random_smooth(szI, szJ, seed=szI * 3 + szJ), 12
+
+
+# sizes: small, sz % 64 == 1 and sz % TJ == 1 (shifted tile origin), non-square, config-1 basin
+SIZES = [(40, 24), (65, 17), (129, 33), (67, 41), (128, 64)]
+
+
+@pytest.mark.parametrize("szI,szJ", SIZES)
+@pytest.mark.parametrize("mode", ["unfused", "fused4", "fused8", "fused16"])
+def test_phys_fp64_bitwise(szI, szJ, mode):
+    import torch
+    print(f"[case] phys fp64 {mode} {szI}x{szJ}", file=sys.stderr, flush=True)
+    cfg, n = _case_cfg(szI, szJ)
+    fused = mode != "unfused"
+    tj = int(mode[5:]) if fused else 0
+    ct, GRID, OCEAN = _gpu_run(cfg, n, torch.float64, fused, tj)
+    o = _oracle_run(cfg, GRID, ct, n)
+    bad = _mismatches(OCEAN, o)
+    assert not bad, bad
+    assert np.isfinite(OCEAN.zeta_t2.cpu().numpy()).all()
+
+
+@pytest.mark.parametrize("szI,szJ", [(65, 17), (130, 66)])
+@pytest.mark.parametrize("tj", [4, 8, 16])
+def test_phys_fp32_fused_equals_unfused(szI, szJ, tj):
+    import torch
+    print(f"[case] phys fp32 fused{tj} vs unfused {szI}x{szJ}", file=sys.stderr, flush=True)
+    cfg = configs.random_smooth(szI, szJ, seed=7)
+    _, _, A = _gpu_run(cfg, 10, torch.float32, True, tj)
+    _, _, B = _gpu_run(cfg, 10, torch.float32, False)
+    bad = []
+    for name in FIELDS + RHS:
+        for k in range(3 if name in FIELDS else 2):
+            x = getattr(A, f"{name}_t{k}").cpu().numpy()
+            y = getattr(B, f"{name}_t{k}").cpu().numpy()
+            if not bitwise_equal(x, y):
+                bad.append((name, k))
+    for name in AVG:
+        if not bitwise_equal(getattr(A, name).cpu().numpy(), getattr(B, name).cpu().numpy()):
             bad.append(name)
     assert not bad, bad
 
 
 def test_phys_fp32_error_vs_fp64():
-    """basin1 (128x64, config 1 physics), 400 substeps of 20 s: FP32 state with FP64
-    averages against the FP64 oracle.  Tolerance: max|dzeta| <= 2e-4*max|zeta|,
-    max|dubar| <= 2e-3*max|ubar|, averages within 1e-4 relative."""
+    """basin1 (128x64 closed basin, 5 km, dt 20 s), 400 substeps: FP32 state with FP64
+    averages against the FP64 oracle.  Tolerance: max|dzeta| <= 2e-4 max|zeta|,
+    max|dubar| <= 2e-3 max|ubar|, Zt_avg1 within 1e-4 relative."""
     import torch
-    print("[case] phys fp32", file=sys.stderr, flush=True)
+    print("[case] phys fp32 error", file=sys.stderr, flush=True)
     cfg = configs.basin1()
-    OCEAN, o = _run(cfg, 400, torch.float32)
-    z = OCEAN.zeta_t2.cpu().double().numpy()
+    ct, GRID, OCEAN = _gpu_run(cfg, 400, torch.float32, True)
+    o = _oracle_run(cfg, GRID, ct, 400)
+    z = OCEAN.zeta_t2.cpu().double().numpy().ravel()
     zo = o.st.v("zeta", 2)
-    u = OCEAN.ubar_t2.cpu().double().numpy()
+    u = OCEAN.ubar_t2.cpu().double().numpy().ravel()
     uo = o.st.v("ubar", 2)
     ez = np.abs(z - zo).max() / np.abs(zo).max()
     eu = np.abs(u - uo).max() / np.abs(uo).max()
