@@ -97,12 +97,26 @@ struct PhysLds {
     static constexpr int FW = TI + 5, FH = TJ + 5, FN = FW * FH, PLANES = 7;
 };
 
+// Element (f + d) of p, addressed as the uniform base p + d plus a 32-bit byte offset:
+// every neighbour of a point then shares one offset register (global_load saddr form).
+template <typename T>
+__device__ __forceinline__ T ldo(const T *p, long long d, uint32_t boff)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p + d) + boff);
+}
+template <typename T>
+__device__ __forceinline__ void sto(T *p, uint32_t boff, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + boff) = v;
+}
+
 // One output tile (ti, tj) of grid P; sm = PLANES x FN values of LDS.
 template <typename T, int TI, int TJ, int NT>
 __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*sm)[PhysLds<TI, TJ>::FN])
 {
     constexpr int FW = TI + 5;
     constexpr int KC = TI * TJ / NT;
+    constexpr uint32_t ES = sizeof(T);
     T *const sZ1 = sm[0], *const sZ2 = sm[0];
     T *const sDU = sm[1], *const sDV = sm[2];
     T *const sZS = sm[3], *const sRZ = sm[4], *const sUS = sm[5], *const sVS = sm[6];
@@ -115,38 +129,64 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     const bool first = P.first != 0;
 
     auto inside = [&](int i, int j) { return i >= 0 && i <= L && j >= 0 && j <= M; };
-    auto gidx = [&](int i, int j) { return (long long)j * W + i; };
     auto clampi = [](int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); };
+    auto off = [&](int i, int j) { return (uint32_t)(j * szI + i) * ES; };
+    auto offd = [&](int i, int j) { return (uint32_t)(j * szI + i) * 8u; };
 
-    // core-point registers
+    // metric factors of a u point / v point / rho point at byte offset o
+    auto onu_at = [&](uint32_t o) { return T(2) / (ldo(P.pn, -1, o) + ldo(P.pn, 0, o)); };
+    auto omu_at = [&](uint32_t o) { return T(2) / (ldo(P.pm, -1, o) + ldo(P.pm, 0, o)); };
+    auto omv_at = [&](uint32_t o) { return T(2) / (ldo(P.pm, -W, o) + ldo(P.pm, 0, o)); };
+    auto onv_at = [&](uint32_t o) { return T(2) / (ldo(P.pn, -W, o) + ldo(P.pn, 0, o)); };
+    auto cffu_at = [&](uint32_t o) {
+        return (ldo(P.pm, 0, o) + ldo(P.pm, -1, o)) * (ldo(P.pn, 0, o) + ldo(P.pn, -1, o));
+    };
+    auto cffv_at = [&](uint32_t o) {
+        return (ldo(P.pm, 0, o) + ldo(P.pm, -W, o)) * (ldo(P.pn, 0, o) + ldo(P.pn, -W, o));
+    };
+
+    // core-point registers: state, predictor right-hand sides and metric factors
     T z1c[KC], u1c[KC], v1c[KC], zsc[KC], ruc[KC], rvc[KC];
+    T onuc[KC], omuc[KC], cffuc[KC], omvc[KC], onvc[KC], cffvc[KC], pmnc[KC];
 
     // ---- L: z1 over the frame ----------------------------------------------------
     for_region<TI, TJ, NT, 3, 2>(tid, [&](int k, int a, int b) {
         const int i = fi0 + a, j = fj0 + b;
-        const T v = inside(i, j) ? P.z1[gidx(i, j)] : T(0);
+        const T v = inside(i, j) ? ldo(P.z1, 0, off(i, j)) : T(0);
         sZ1[b * FW + a] = v;
         if (k >= 0) z1c[k] = v;
     });
     __syncthreads();
 
-    // ---- A: DU1 at u points, DV1 at v points ------------------------------------
+    // ---- A: DU1 at u points, DV1 at v points (+ core metric factors) ---------------
     for_region<TI, TJ, NT, 2, 2>(tid, [&](int k, int a, int b) {
         const int i = fi0 + a, j = fj0 + b;
         const int o = b * FW + a;
         T du = T(0), dv = T(0), u1 = T(0), v1 = T(0);
         if (inside(i, j)) {
-            const long long f = gidx(i, j);
-            u1 = P.u1[f];
-            v1 = P.v1[f];
+            const uint32_t f = off(i, j);
+            u1 = ldo(P.u1, 0, f);
+            v1 = ldo(P.v1, 0, f);
+            const T h0 = ldo(P.h, 0, f);
             if (i >= 1) {
-                const T onu = T(2) / (P.pn[f - 1] + P.pn[f]);
-                du = ((T(0.5) * ((P.h[f - 1] + sZ1[o - 1]) + (P.h[f] + sZ1[o]))) * u1) * onu;
+                const T onu = onu_at(f);
+                du = ((T(0.5) * ((ldo(P.h, -1, f) + sZ1[o - 1]) + (h0 + sZ1[o]))) * u1) * onu;
+                if (k >= 0) {
+                    onuc[k] = onu;
+                    omuc[k] = omu_at(f);
+                    cffuc[k] = cffu_at(f);
+                }
             }
             if (j >= 1) {
-                const T omv = T(2) / (P.pm[f - W] + P.pm[f]);
-                dv = ((T(0.5) * ((P.h[f - W] + sZ1[o - FW]) + (P.h[f] + sZ1[o]))) * v1) * omv;
+                const T omv = omv_at(f);
+                dv = ((T(0.5) * ((ldo(P.h, -W, f) + sZ1[o - FW]) + (h0 + sZ1[o]))) * v1) * omv;
+                if (k >= 0) {
+                    omvc[k] = omv;
+                    onvc[k] = onv_at(f);
+                    cffvc[k] = cffv_at(f);
+                }
             }
+            if (k >= 0) pmnc[k] = ldo(P.pm, 0, f) * ldo(P.pn, 0, f);
         }
         sDU[o] = du;
         sDV[o] = dv;
@@ -164,13 +204,14 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         T zs = T(0), r = T(0);
         if (inside(i, j)) {
             const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
+            const bool own = qi == i && qj == j;
             const int qo = o + (qj - j) * FW + (qi - i);
-            const long long fq = gidx(qi, qj);
+            const uint32_t fq = off(qi, qj);
             r = (sDU[qo] - sDU[qo + 1]) + (sDV[qo] - sDV[qo + FW]);
-            const T pmn = P.pm[fq] * P.pn[fq];
+            const T pmn = (k >= 0 && own) ? pmnc[k] : ldo(P.pm, 0, fq) * ldo(P.pn, 0, fq);
             if (first) zs = sZ1[qo] + (pmn * dt) * r;
-            else zs = P.z0[fq] + (pmn * (T(2) * dt)) * r;
-            if (k >= 0 && qi == i && qj == j) P.rz1[fq] = r;
+            else zs = ldo(P.z0, 0, fq) + (pmn * (T(2) * dt)) * r;
+            if (k >= 0 && own) sto(P.rz1, fq, r);
         }
         sZS[o] = zs;
         sRZ[o] = r;
@@ -181,12 +222,14 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     // ---- P2: predicted ubar / vbar ------------------------------------------------
     const T cff4 = T(4.0 / 25.0), cff5 = T(1) - T(2) * cff4;
     const T c1dt = first ? T(0.5) * dt : dt;
-    // pressure surface of the predictor at frame offset o, global index f
-    auto gz_pred = [&](int o, long long f) {
-        return first ? T(0.5) * (sZ1[o] + sZS[o]) : cff5 * sZ1[o] + cff4 * (P.z0[f] + sZS[o]);
+    // pressure surface of the predictor at LDS index o, global neighbour d of offset f
+    auto gz_pred = [&](int o, long long d, uint32_t f) {
+        return first ? T(0.5) * (sZ1[o] + sZS[o]) : cff5 * sZ1[o] + cff4 * (ldo(P.z0, d, f) + sZS[o]);
     };
     // depth of the "before" state of the predictor
-    auto Db = [&](int o, long long f) { return first ? P.h[f] + sZ1[o] : P.h[f] + P.z0[f]; };
+    auto Db = [&](int o, long long d, uint32_t f) {
+        return first ? ldo(P.h, d, f) + sZ1[o] : ldo(P.h, d, f) + ldo(P.z0, d, f);
+    };
 
     for_region<TI, TJ, NT, 1, 1>(tid, [&](int k, int a, int b) {
         const int i = fi0 + a, j = fj0 + b;
@@ -196,60 +239,68 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             // ubar
             if (i >= 2 && i <= L - 1) {
                 const int qj = clampi(j, 1, M - 1);
+                const bool own = k >= 0 && qj == j;
                 const int q = o + (qj - j) * FW;
-                const long long f = gidx(i, qj);
-                const T onu = T(2) / (P.pn[f - 1] + P.pn[f]);
-                const T omu = T(2) / (P.pm[f - 1] + P.pm[f]);
-                const T g0 = gz_pred(q - 1, f - 1), g1 = gz_pred(q, f);
-                T r = ((T(0.5) * P.g) * onu) * ((P.h[f - 1] + P.h[f]) * (g0 - g1) + (g0 * g0 - g1 * g1));
-                const T um = P.u1[f - 1], uc = P.u1[f], up = P.u1[f + 1], un = P.u1[f + W], us_ = P.u1[f - W];
+                const uint32_t f = off(i, qj);
+                const T onu = own ? onuc[k] : onu_at(f);
+                const T omu = own ? omuc[k] : omu_at(f);
+                const T g0 = gz_pred(q - 1, -1, f), g1 = gz_pred(q, 0, f);
+                const T hm = ldo(P.h, -1, f), hc = ldo(P.h, 0, f);
+                T r = ((T(0.5) * P.g) * onu) * ((hm + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
+                const T um = ldo(P.u1, -1, f), uc = ldo(P.u1, 0, f), up = ldo(P.u1, 1, f);
+                const T un = ldo(P.u1, W, f), usn = ldo(P.u1, -W, f);
                 const T UFx1 = T(0.25) * (sDU[q] + sDU[q + 1]) * (uc + up);
                 const T UFx0 = T(0.25) * (sDU[q - 1] + sDU[q]) * (um + uc);
                 const T UFe1 = T(0.25) * (sDV[q + FW] + sDV[q + FW - 1]) * (un + uc);
-                const T UFe0 = T(0.25) * (sDV[q] + sDV[q - 1]) * (uc + us_);
+                const T UFe0 = T(0.25) * (sDV[q] + sDV[q - 1]) * (uc + usn);
                 r = r - ((UFx1 - UFx0) + (UFe1 - UFe0));
-                const T cr1 = ((T(0.5) * (P.h[f] + sZ1[q])) * P.fomn[f]) * (P.v1[f] + P.v1[f + W]);
-                const T cr0 = ((T(0.5) * (P.h[f - 1] + sZ1[q - 1])) * P.fomn[f - 1]) * (P.v1[f - 1] + P.v1[f - 1 + W]);
+                const T cr1 = ((T(0.5) * (hc + sZ1[q])) * ldo(P.fomn, 0, f)) * (ldo(P.v1, 0, f) + ldo(P.v1, W, f));
+                const T cr0 = ((T(0.5) * (hm + sZ1[q - 1])) * ldo(P.fomn, -1, f)) *
+                              (ldo(P.v1, -1, f) + ldo(P.v1, W - 1, f));
                 r = r + T(0.5) * (cr1 + cr0);
                 r = r - ((P.rdrg * uc) * omu) * onu;
-                r = r + (P.sustr[f] * omu) * onu;
-                if (k >= 0 && qj == j) {
-                    P.ru1[f] = r;
+                r = r + (ldo(P.sustr, 0, f) * omu) * onu;
+                if (own) {
+                    sto(P.ru1, f, r);
                     ruc[k] = r;
                 }
-                const T cff = (P.pm[f] + P.pm[f - 1]) * (P.pn[f] + P.pn[f - 1]);
-                const T fac = T(1) / ((P.h[f] + sZS[q]) + (P.h[f - 1] + sZS[q - 1]));
-                const T ub = first ? uc : P.u0[f];
-                us = (ub * (Db(q, f) + Db(q - 1, f - 1)) + (cff * c1dt) * r) * fac;
+                const T cff = own ? cffuc[k] : cffu_at(f);
+                const T fac = T(1) / ((hc + sZS[q]) + (hm + sZS[q - 1]));
+                const T ub = first ? uc : ldo(P.u0, 0, f);
+                us = (ub * (Db(q, 0, f) + Db(q - 1, -1, f)) + (cff * c1dt) * r) * fac;
             }
             // vbar
             if (j >= 2 && j <= M - 1) {
                 const int qi = clampi(i, 1, L - 1);
+                const bool own = k >= 0 && qi == i;
                 const int q = o + (qi - i);
-                const long long f = gidx(qi, j);
-                const T omv = T(2) / (P.pm[f - W] + P.pm[f]);
-                const T onv = T(2) / (P.pn[f - W] + P.pn[f]);
-                const T g0 = gz_pred(q - FW, f - W), g1 = gz_pred(q, f);
-                T r = ((T(0.5) * P.g) * omv) * ((P.h[f - W] + P.h[f]) * (g0 - g1) + (g0 * g0 - g1 * g1));
-                const T vc = P.v1[f], vp = P.v1[f + 1], vm = P.v1[f - 1], vn = P.v1[f + W], vs_ = P.v1[f - W];
+                const uint32_t f = off(qi, j);
+                const T omv = own ? omvc[k] : omv_at(f);
+                const T onv = own ? onvc[k] : onv_at(f);
+                const T g0 = gz_pred(q - FW, -W, f), g1 = gz_pred(q, 0, f);
+                const T hs = ldo(P.h, -W, f), hc = ldo(P.h, 0, f);
+                T r = ((T(0.5) * P.g) * omv) * ((hs + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
+                const T vc = ldo(P.v1, 0, f), vp = ldo(P.v1, 1, f), vm = ldo(P.v1, -1, f);
+                const T vn = ldo(P.v1, W, f), vsn = ldo(P.v1, -W, f);
                 const T VFx1 = T(0.25) * (sDU[q + 1] + sDU[q + 1 - FW]) * (vp + vc);
                 const T VFx0 = T(0.25) * (sDU[q] + sDU[q - FW]) * (vc + vm);
                 const T VFe1 = T(0.25) * (sDV[q] + sDV[q + FW]) * (vc + vn);
-                const T VFe0 = T(0.25) * (sDV[q - FW] + sDV[q]) * (vs_ + vc);
+                const T VFe0 = T(0.25) * (sDV[q - FW] + sDV[q]) * (vsn + vc);
                 r = r - ((VFx1 - VFx0) + (VFe1 - VFe0));
-                const T cr1 = ((T(0.5) * (P.h[f] + sZ1[q])) * P.fomn[f]) * (P.u1[f] + P.u1[f + 1]);
-                const T cr0 = ((T(0.5) * (P.h[f - W] + sZ1[q - FW])) * P.fomn[f - W]) * (P.u1[f - W] + P.u1[f - W + 1]);
+                const T cr1 = ((T(0.5) * (hc + sZ1[q])) * ldo(P.fomn, 0, f)) * (ldo(P.u1, 0, f) + ldo(P.u1, 1, f));
+                const T cr0 = ((T(0.5) * (hs + sZ1[q - FW])) * ldo(P.fomn, -W, f)) *
+                              (ldo(P.u1, -W, f) + ldo(P.u1, 1 - W, f));
                 r = r - T(0.5) * (cr1 + cr0);
                 r = r - ((P.rdrg * vc) * omv) * onv;
-                r = r + (P.svstr[f] * omv) * onv;
-                if (k >= 0 && qi == i) {
-                    P.rv1[f] = r;
+                r = r + (ldo(P.svstr, 0, f) * omv) * onv;
+                if (own) {
+                    sto(P.rv1, f, r);
                     rvc[k] = r;
                 }
-                const T cff = (P.pm[f] + P.pm[f - W]) * (P.pn[f] + P.pn[f - W]);
-                const T fac = T(1) / ((P.h[f] + sZS[q]) + (P.h[f - W] + sZS[q - FW]));
-                const T vb = first ? vc : P.v0[f];
-                vs = (vb * (Db(q, f) + Db(q - FW, f - W)) + (cff * c1dt) * r) * fac;
+                const T cff = own ? cffvc[k] : cffv_at(f);
+                const T fac = T(1) / ((hc + sZS[q]) + (hs + sZS[q - FW]));
+                const T vb = first ? vc : ldo(P.v0, 0, f);
+                vs = (vb * (Db(q, 0, f) + Db(q - FW, -W, f)) + (cff * c1dt) * r) * fac;
             }
         }
         sUS[o] = us;
@@ -264,14 +315,15 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         const int o = b * FW + a;
         T du = T(0), dv = T(0);
         if (inside(i, j)) {
-            const long long f = gidx(i, j);
+            const uint32_t f = off(i, j);
+            const T h0 = ldo(P.h, 0, f);
             if (i >= 1) {
-                const T onu = T(2) / (P.pn[f - 1] + P.pn[f]);
-                du = ((T(0.5) * ((P.h[f - 1] + sZS[o - 1]) + (P.h[f] + sZS[o]))) * sUS[o]) * onu;
+                const T onu = k >= 0 ? onuc[k] : onu_at(f);
+                du = ((T(0.5) * ((ldo(P.h, -1, f) + sZS[o - 1]) + (h0 + sZS[o]))) * sUS[o]) * onu;
             }
             if (j >= 1) {
-                const T omv = T(2) / (P.pm[f - W] + P.pm[f]);
-                dv = ((T(0.5) * ((P.h[f - W] + sZS[o - FW]) + (P.h[f] + sZS[o]))) * sVS[o]) * omv;
+                const T omv = k >= 0 ? omvc[k] : omv_at(f);
+                dv = ((T(0.5) * ((ldo(P.h, -W, f) + sZS[o - FW]) + (h0 + sZS[o]))) * sVS[o]) * omv;
             }
         }
         sDU[o] = du;
@@ -291,21 +343,22 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         T z2 = T(0);
         if (inside(i, j)) {
             const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
+            const bool own = k >= 0 && qi == i && qj == j;
             const int qo = o + (qj - j) * FW + (qi - i);
-            const long long fq = gidx(qi, qj);
+            const uint32_t fq = off(qi, qj);
             const T r = (sDU[qo] - sDU[qo + 1]) + (sDV[qo] - sDV[qo + FW]);
-            const T pmn = P.pm[fq] * P.pn[fq];
-            const T z1q = (k >= 0 && qo == o) ? z1c[k] : P.z1[fq];
+            const T pmn = own ? pmnc[k] : ldo(P.pm, 0, fq) * ldo(P.pn, 0, fq);
+            const T z1q = own ? z1c[k] : ldo(P.z1, 0, fq);
             if (first) z2 = z1q + pmn * (dt * r);
-            else z2 = z1q + pmn * ((a1 * r + a2 * sRZ[qo]) - a3 * P.rz0[fq]);
+            else z2 = z1q + pmn * ((a1 * r + a2 * sRZ[qo]) - a3 * ldo(P.rz0, 0, fq));
             if (k >= 0) {
-                const long long f = gidx(i, j);
-                P.z2[f] = z2;
-                if (qo == o) {
+                sto(P.z2, off(i, j), z2);
+                if (own) {
                     const double w = P.w1;
-                    P.Zt[f] = P.Zt[f] + w * (double)zsc[k];
-                    P.DUa[f] = P.DUa[f] + w * (double)duc[k];
-                    P.DVa[f] = P.DVa[f] + w * (double)dvc[k];
+                    const uint32_t fd = offd(i, j);
+                    sto(P.Zt, fd, ldo(P.Zt, 0, fd) + w * (double)zsc[k]);
+                    sto(P.DUa, fd, ldo(P.DUa, 0, fd) + w * (double)duc[k]);
+                    sto(P.DVa, fd, ldo(P.DVa, 0, fd) + w * (double)dvc[k]);
                 }
             }
         }
@@ -323,69 +376,73 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         const int i = fi0 + a, j = fj0 + b;
         if (!inside(i, j)) return;
         const int o = b * FW + a;
-        const long long fo = gidx(i, j);
+        const uint32_t fo = off(i, j);
         // ubar
         T u2 = T(0);
         if (i >= 2 && i <= L - 1) {
             const int qj = clampi(j, 1, M - 1);
+            const bool own = qj == j;
             const int q = o + (qj - j) * FW;
-            const long long f = gidx(i, qj);
-            const T onu = T(2) / (P.pn[f - 1] + P.pn[f]);
-            const T omu = T(2) / (P.pm[f - 1] + P.pm[f]);
+            const uint32_t f = off(i, qj);
+            const T onu = own ? onuc[k] : onu_at(f);
+            const T omu = own ? omuc[k] : omu_at(f);
             const T g0 = gz_corr(q - 1), g1 = gz_corr(q);
-            T r = ((T(0.5) * P.g) * onu) * ((P.h[f - 1] + P.h[f]) * (g0 - g1) + (g0 * g0 - g1 * g1));
+            const T hm = ldo(P.h, -1, f), hc = ldo(P.h, 0, f);
+            T r = ((T(0.5) * P.g) * onu) * ((hm + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
             const T uc = sUS[q];
             const T UFx1 = T(0.25) * (sDU[q] + sDU[q + 1]) * (uc + sUS[q + 1]);
             const T UFx0 = T(0.25) * (sDU[q - 1] + sDU[q]) * (sUS[q - 1] + uc);
             const T UFe1 = T(0.25) * (sDV[q + FW] + sDV[q + FW - 1]) * (sUS[q + FW] + uc);
             const T UFe0 = T(0.25) * (sDV[q] + sDV[q - 1]) * (uc + sUS[q - FW]);
             r = r - ((UFx1 - UFx0) + (UFe1 - UFe0));
-            const T cr1 = ((T(0.5) * (P.h[f] + sZS[q])) * P.fomn[f]) * (sVS[q] + sVS[q + FW]);
-            const T cr0 = ((T(0.5) * (P.h[f - 1] + sZS[q - 1])) * P.fomn[f - 1]) * (sVS[q - 1] + sVS[q - 1 + FW]);
+            const T cr1 = ((T(0.5) * (hc + sZS[q])) * ldo(P.fomn, 0, f)) * (sVS[q] + sVS[q + FW]);
+            const T cr0 = ((T(0.5) * (hm + sZS[q - 1])) * ldo(P.fomn, -1, f)) * (sVS[q - 1] + sVS[q - 1 + FW]);
             r = r + T(0.5) * (cr1 + cr0);
             r = r - ((P.rdrg * uc) * omu) * onu;
-            r = r + (P.sustr[f] * omu) * onu;
-            const T cff = (P.pm[f] + P.pm[f - 1]) * (P.pn[f] + P.pn[f - 1]);
-            const T fac = T(1) / ((P.h[f] + sZ2[q]) + (P.h[f - 1] + sZ2[q - 1]));
+            r = r + (ldo(P.sustr, 0, f) * omu) * onu;
+            const T cff = own ? cffuc[k] : cffu_at(f);
+            const T fac = T(1) / ((hc + sZ2[q]) + (hm + sZ2[q - 1]));
             T tend;
             if (first) tend = (T(0.5) * dt) * r;
-            else tend = (b1 * r + b2 * (qj == j ? ruc[k] : P.ru1[f])) - b3 * P.ru0[f];
-            const T z1q = qj == j ? z1c[k] : P.z1[f];
-            const T u1q = qj == j ? u1c[k] : P.u1[f];
-            u2 = (u1q * ((P.h[f] + z1q) + (P.h[f - 1] + P.z1[f - 1])) + cff * tend) * fac;
+            else tend = (b1 * r + b2 * (own ? ruc[k] : ldo((const T *)P.ru1, 0, f))) - b3 * ldo(P.ru0, 0, f);
+            const T z1q = own ? z1c[k] : ldo(P.z1, 0, f);
+            const T u1q = own ? u1c[k] : ldo(P.u1, 0, f);
+            u2 = (u1q * ((hc + z1q) + (hm + ldo(P.z1, -1, f))) + cff * tend) * fac;
         }
-        P.u2[fo] = u2;
+        sto(P.u2, fo, u2);
         // vbar
         T v2 = T(0);
         if (j >= 2 && j <= M - 1) {
             const int qi = clampi(i, 1, L - 1);
+            const bool own = qi == i;
             const int q = o + (qi - i);
-            const long long f = gidx(qi, j);
-            const T omv = T(2) / (P.pm[f - W] + P.pm[f]);
-            const T onv = T(2) / (P.pn[f - W] + P.pn[f]);
+            const uint32_t f = off(qi, j);
+            const T omv = own ? omvc[k] : omv_at(f);
+            const T onv = own ? onvc[k] : onv_at(f);
             const T g0 = gz_corr(q - FW), g1 = gz_corr(q);
-            T r = ((T(0.5) * P.g) * omv) * ((P.h[f - W] + P.h[f]) * (g0 - g1) + (g0 * g0 - g1 * g1));
+            const T hs = ldo(P.h, -W, f), hc = ldo(P.h, 0, f);
+            T r = ((T(0.5) * P.g) * omv) * ((hs + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
             const T vc = sVS[q];
             const T VFx1 = T(0.25) * (sDU[q + 1] + sDU[q + 1 - FW]) * (sVS[q + 1] + vc);
             const T VFx0 = T(0.25) * (sDU[q] + sDU[q - FW]) * (vc + sVS[q - 1]);
             const T VFe1 = T(0.25) * (sDV[q] + sDV[q + FW]) * (vc + sVS[q + FW]);
             const T VFe0 = T(0.25) * (sDV[q - FW] + sDV[q]) * (sVS[q - FW] + vc);
             r = r - ((VFx1 - VFx0) + (VFe1 - VFe0));
-            const T cr1 = ((T(0.5) * (P.h[f] + sZS[q])) * P.fomn[f]) * (sUS[q] + sUS[q + 1]);
-            const T cr0 = ((T(0.5) * (P.h[f - W] + sZS[q - FW])) * P.fomn[f - W]) * (sUS[q - FW] + sUS[q - FW + 1]);
+            const T cr1 = ((T(0.5) * (hc + sZS[q])) * ldo(P.fomn, 0, f)) * (sUS[q] + sUS[q + 1]);
+            const T cr0 = ((T(0.5) * (hs + sZS[q - FW])) * ldo(P.fomn, -W, f)) * (sUS[q - FW] + sUS[q - FW + 1]);
             r = r - T(0.5) * (cr1 + cr0);
             r = r - ((P.rdrg * vc) * omv) * onv;
-            r = r + (P.svstr[f] * omv) * onv;
-            const T cff = (P.pm[f] + P.pm[f - W]) * (P.pn[f] + P.pn[f - W]);
-            const T fac = T(1) / ((P.h[f] + sZ2[q]) + (P.h[f - W] + sZ2[q - FW]));
+            r = r + (ldo(P.svstr, 0, f) * omv) * onv;
+            const T cff = own ? cffvc[k] : cffv_at(f);
+            const T fac = T(1) / ((hc + sZ2[q]) + (hs + sZ2[q - FW]));
             T tend;
             if (first) tend = (T(0.5) * dt) * r;
-            else tend = (b1 * r + b2 * (qi == i ? rvc[k] : P.rv1[f])) - b3 * P.rv0[f];
-            const T z1q = qi == i ? z1c[k] : P.z1[f];
-            const T v1q = qi == i ? v1c[k] : P.v1[f];
-            v2 = (v1q * ((P.h[f] + z1q) + (P.h[f - W] + P.z1[f - W])) + cff * tend) * fac;
+            else tend = (b1 * r + b2 * (own ? rvc[k] : ldo((const T *)P.rv1, 0, f))) - b3 * ldo(P.rv0, 0, f);
+            const T z1q = own ? z1c[k] : ldo(P.z1, 0, f);
+            const T v1q = own ? v1c[k] : ldo(P.v1, 0, f);
+            v2 = (v1q * ((hc + z1q) + (hs + ldo(P.z1, -W, f))) + cff * tend) * fac;
         }
-        P.v2[fo] = v2;
+        sto(P.v2, fo, v2);
     });
 }
 
@@ -446,6 +503,8 @@ int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void 
     a.dt = T(dt);
     a.first = first;
     a.w1 = w1;
+    SF_REQUIRE((long long)x->szI * x->szJ < (1LL << 29), "sf_phys_substep_fused: grid of %dx%d points exceeds 2^29",
+               x->szI, x->szJ);
     const long long nb = (long long)a.tiles_i * a.tiles_j;
     SF_REQUIRE(nb < (1LL << 31), "sf_phys_substep_fused: grid too large");
     hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT>), dim3((unsigned)nb), dim3(NT), 0, sf::as_stream(stream), a);
