@@ -97,12 +97,15 @@ struct PhysLds {
     static constexpr int FW = TI + 5, FH = TJ + 5, FN = FW * FH, PLANES = 7;
 };
 
-// Element (f + d) of p, addressed as the uniform base p + d plus a 32-bit byte offset:
-// every neighbour of a point then shares one offset register (global_load saddr form).
+// Element (f + d) of p at byte offset boff of element f: the uniform base plus one
+// 32-bit offset register shared by all neighbours of a point (global_load saddr
+// form); a constant d folds into the instruction's immediate offset, a row stride
+// d = +-W into a shifted scalar base.
 template <typename T>
 __device__ __forceinline__ T ldo(const T *p, long long d, uint32_t boff)
 {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p + d) + boff);
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + (uint64_t)boff +
+                                        d * (long long)sizeof(T));
 }
 template <typename T>
 __device__ __forceinline__ void sto(T *p, uint32_t boff, T v)
@@ -110,9 +113,53 @@ __device__ __forceinline__ void sto(T *p, uint32_t boff, T v)
     *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + boff) = v;
 }
 
-// One output tile (ti, tj) of grid P; sm = PLANES x FN values of LDS.
+// Workgroup barrier that orders LDS only: global loads issued before it stay in
+// flight (the stages exchange data through LDS exclusively).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Map ring index t of region [3-LO, 3+TI+HI) x [3-LO, 3+TJ+HI) minus the core to (a, b).
+template <int TI, int TJ, int LO, int HI>
+__device__ __forceinline__ void ring_point(int t, int &a, int &b)
+{
+    constexpr int W = TI + LO + HI, NBOT = LO * W, WROW = LO + HI, NMID = TJ * WROW;
+    if (t < NBOT) {
+        b = 3 - LO + t / W;
+        a = 3 - LO + t % W;
+    } else if (t < NBOT + NMID) {
+        const int s = t - NBOT;
+        b = 3 + s / WROW;
+        const int c = s % WROW;
+        a = c < LO ? 3 - LO + c : 3 + TI + (c - LO);
+    } else {
+        const int s = t - NBOT - NMID;
+        b = 3 + TJ + s / W;
+        a = 3 - LO + s % W;
+    }
+}
+template <int TI, int TJ, int LO, int HI>
+constexpr int ring_size() { return (TI + LO + HI) * (TJ + LO + HI) - TI * TJ; }
+
+// inputs of the ubar / vbar right-hand side at a u / v point
+template <typename T>
+struct UVIn {
+    T m1, m2, cff;              // u: onu, omu ; v: omv, onv
+    T pmc, pmb, pnc, pnb;       // raw pm, pn at the point and the back neighbour (generic path)
+    T hb, hc;                   // h at the back neighbour (i-1 / j-1) and the point
+    T xb, xc, xf, yf, yb;       // same-component velocity: back, centre, forward, cross+, cross-
+    T fc, fb;                   // fomn at the point and the back neighbour
+    T oc, of, ob, obf;          // other component for Coriolis (see rhs_u / rhs_v)
+    T st;                       // surface stress
+    T x0, z0c, z0b;             // predictor "before" velocity and zeta (t0) at point / back
+};
+
 template <typename T, int TI, int TJ, int NT>
-__device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*sm)[PhysLds<TI, TJ>::FN])
+__device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*sm)[PhysLds<TI, TJ>::FN],
+                                          T *side)
 {
     constexpr int FW = TI + 5;
     constexpr int KC = TI * TJ / NT;
@@ -120,6 +167,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     T *const sZ1 = sm[0], *const sZ2 = sm[0];
     T *const sDU = sm[1], *const sDV = sm[2];
     T *const sZS = sm[3], *const sRZ = sm[4], *const sUS = sm[5], *const sVS = sm[6];
+    // predictor right-hand sides of the ghost-source row (j = 1, M-1) / column (i = 1, L-1)
+    T *const sRUlo = side, *const sRUhi = side + TI, *const sRVlo = side + 2 * TI, *const sRVhi = side + 2 * TI + TJ;
 
     const int szI = P.szI, szJ = P.szJ, L = szI - 1, M = szJ - 1;
     const long long W = szI;
@@ -132,8 +181,13 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     auto clampi = [](int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); };
     auto off = [&](int i, int j) { return (uint32_t)(j * szI + i) * ES; };
     auto offd = [&](int i, int j) { return (uint32_t)(j * szI + i) * 8u; };
+    auto core_ab = [&](int k, int &a, int &b) {
+        const int idx = tid + k * NT;
+        a = 3 + idx % TI;
+        b = 3 + idx / TI;
+    };
 
-    // metric factors of a u point / v point / rho point at byte offset o
+    // metric factors of a u point / v point at byte offset o
     auto onu_at = [&](uint32_t o) { return T(2) / (ldo(P.pn, -1, o) + ldo(P.pn, 0, o)); };
     auto omu_at = [&](uint32_t o) { return T(2) / (ldo(P.pm, -1, o) + ldo(P.pm, 0, o)); };
     auto omv_at = [&](uint32_t o) { return T(2) / (ldo(P.pm, -W, o) + ldo(P.pm, 0, o)); };
@@ -145,305 +199,750 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         return (ldo(P.pm, 0, o) + ldo(P.pm, -W, o)) * (ldo(P.pn, 0, o) + ldo(P.pn, -W, o));
     };
 
-    // core-point registers: state, predictor right-hand sides and metric factors
-    T z1c[KC], u1c[KC], v1c[KC], zsc[KC], ruc[KC], rvc[KC];
-    T onuc[KC], omuc[KC], cffuc[KC], omvc[KC], onvc[KC], cffvc[KC], pmnc[KC];
+    // generic (load-on-demand) inputs of the u / v right-hand side at offset f
+    auto load_u = [&](uint32_t f) {
+        UVIn<T> x;
+        x.pmc = ldo(P.pm, 0, f);
+        x.pmb = ldo(P.pm, -1, f);
+        x.pnc = ldo(P.pn, 0, f);
+        x.pnb = ldo(P.pn, -1, f);
+        x.hb = ldo(P.h, -1, f);
+        x.hc = ldo(P.h, 0, f);
+        x.xb = ldo(P.u1, -1, f);
+        x.xc = ldo(P.u1, 0, f);
+        x.xf = ldo(P.u1, 1, f);
+        x.yf = ldo(P.u1, W, f);
+        x.yb = ldo(P.u1, -W, f);
+        x.fc = ldo(P.fomn, 0, f);
+        x.fb = ldo(P.fomn, -1, f);
+        x.oc = ldo(P.v1, 0, f);
+        x.of = ldo(P.v1, W, f);
+        x.ob = ldo(P.v1, -1, f);
+        x.obf = ldo(P.v1, W - 1, f);
+        x.st = ldo(P.sustr, 0, f);
+        x.x0 = first ? x.xc : ldo(P.u0, 0, f);
+        x.z0c = first ? T(0) : ldo(P.z0, 0, f);
+        x.z0b = first ? T(0) : ldo(P.z0, -1, f);
+        return x;
+    };
+    auto load_v = [&](uint32_t f) {
+        UVIn<T> x;
+        x.pmc = ldo(P.pm, 0, f);
+        x.pmb = ldo(P.pm, -W, f);
+        x.pnc = ldo(P.pn, 0, f);
+        x.pnb = ldo(P.pn, -W, f);
+        x.hb = ldo(P.h, -W, f);
+        x.hc = ldo(P.h, 0, f);
+        x.xb = ldo(P.v1, -W, f);
+        x.xc = ldo(P.v1, 0, f);
+        x.xf = ldo(P.v1, W, f);
+        x.yf = ldo(P.v1, 1, f);
+        x.yb = ldo(P.v1, -1, f);
+        x.fc = ldo(P.fomn, 0, f);
+        x.fb = ldo(P.fomn, -W, f);
+        x.oc = ldo(P.u1, 0, f);
+        x.of = ldo(P.u1, 1, f);
+        x.ob = ldo(P.u1, -W, f);
+        x.obf = ldo(P.u1, 1 - W, f);
+        x.st = ldo(P.svstr, 0, f);
+        x.x0 = first ? x.xc : ldo(P.v0, 0, f);
+        x.z0c = first ? T(0) : ldo(P.z0, 0, f);
+        x.z0b = first ? T(0) : ldo(P.z0, -W, f);
+        return x;
+    };
 
-    // ---- L: z1 over the frame ----------------------------------------------------
-    for_region<TI, TJ, NT, 3, 2>(tid, [&](int k, int a, int b) {
-        const int i = fi0 + a, j = fj0 + b;
-        const T v = inside(i, j) ? ldo(P.z1, 0, off(i, j)) : T(0);
-        sZ1[b * FW + a] = v;
-        if (k >= 0) z1c[k] = v;
-    });
-    __syncthreads();
+    // metric factors of generic inputs (same expressions as onu_at / omu_at / cffu_at ...)
+    auto metrics_u = [&](UVIn<T> &x) {
+        x.m1 = T(2) / (x.pnb + x.pnc);
+        x.m2 = T(2) / (x.pmb + x.pmc);
+        x.cff = (x.pmc + x.pmb) * (x.pnc + x.pnb);
+    };
+    auto metrics_v = [&](UVIn<T> &x) {
+        x.m1 = T(2) / (x.pmb + x.pmc);
+        x.m2 = T(2) / (x.pnb + x.pnc);
+        x.cff = (x.pmc + x.pmb) * (x.pnc + x.pnb);
+    };
 
-    // ---- A: DU1 at u points, DV1 at v points (+ core metric factors) ---------------
-    for_region<TI, TJ, NT, 2, 2>(tid, [&](int k, int a, int b) {
-        const int i = fi0 + a, j = fj0 + b;
-        const int o = b * FW + a;
-        T du = T(0), dv = T(0), u1 = T(0), v1 = T(0);
-        if (inside(i, j)) {
-            const uint32_t f = off(i, j);
-            u1 = ldo(P.u1, 0, f);
-            v1 = ldo(P.v1, 0, f);
-            const T h0 = ldo(P.h, 0, f);
-            if (i >= 1) {
-                const T onu = onu_at(f);
-                du = ((T(0.5) * ((ldo(P.h, -1, f) + sZ1[o - 1]) + (h0 + sZ1[o]))) * u1) * onu;
-                if (k >= 0) {
-                    onuc[k] = onu;
-                    omuc[k] = omu_at(f);
-                    cffuc[k] = cffu_at(f);
-                }
-            }
-            if (j >= 1) {
-                const T omv = omv_at(f);
-                dv = ((T(0.5) * ((ldo(P.h, -W, f) + sZ1[o - FW]) + (h0 + sZ1[o]))) * v1) * omv;
-                if (k >= 0) {
-                    omvc[k] = omv;
-                    onvc[k] = onv_at(f);
-                    cffvc[k] = cffv_at(f);
-                }
-            }
-            if (k >= 0) pmnc[k] = ldo(P.pm, 0, f) * ldo(P.pn, 0, f);
-        }
-        sDU[o] = du;
-        sDV[o] = dv;
-        if (k >= 0) {
-            u1c[k] = u1;
-            v1c[k] = v1;
-        }
-    });
-    __syncthreads();
-
-    // ---- P1: predicted zeta and its right-hand side -----------------------------
-    for_region<TI, TJ, NT, 2, 1>(tid, [&](int k, int a, int b) {
-        const int i = fi0 + a, j = fj0 + b;
-        const int o = b * FW + a;
-        T zs = T(0), r = T(0);
-        if (inside(i, j)) {
-            const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
-            const bool own = qi == i && qj == j;
-            const int qo = o + (qj - j) * FW + (qi - i);
-            const uint32_t fq = off(qi, qj);
-            r = (sDU[qo] - sDU[qo + 1]) + (sDV[qo] - sDV[qo + FW]);
-            const T pmn = (k >= 0 && own) ? pmnc[k] : ldo(P.pm, 0, fq) * ldo(P.pn, 0, fq);
-            if (first) zs = sZ1[qo] + (pmn * dt) * r;
-            else zs = ldo(P.z0, 0, fq) + (pmn * (T(2) * dt)) * r;
-            if (k >= 0 && own) sto(P.rz1, fq, r);
-        }
-        sZS[o] = zs;
-        sRZ[o] = r;
-        if (k >= 0) zsc[k] = zs;
-    });
-    __syncthreads();
-
-    // ---- P2: predicted ubar / vbar ------------------------------------------------
+    // ---- predictor momentum at LDS index q from inputs x (returns rhs in r) ----------
     const T cff4 = T(4.0 / 25.0), cff5 = T(1) - T(2) * cff4;
     const T c1dt = first ? T(0.5) * dt : dt;
-    // pressure surface of the predictor at LDS index o, global neighbour d of offset f
-    auto gz_pred = [&](int o, long long d, uint32_t f) {
-        return first ? T(0.5) * (sZ1[o] + sZS[o]) : cff5 * sZ1[o] + cff4 * (ldo(P.z0, d, f) + sZS[o]);
+    auto gzp = [&](int o, T z0) { return first ? T(0.5) * (sZ1[o] + sZS[o]) : cff5 * sZ1[o] + cff4 * (z0 + sZS[o]); };
+    auto Dbp = [&](int o, T h, T z0) { return first ? h + sZ1[o] : h + z0; };
+    auto pred_u = [&](int q, const UVIn<T> &x, T &r) {
+        const T onu = x.m1, omu = x.m2;
+        const T g0 = gzp(q - 1, x.z0b), g1 = gzp(q, x.z0c);
+        r = ((T(0.5) * P.g) * onu) * ((x.hb + x.hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
+        const T UFx1 = T(0.25) * (sDU[q] + sDU[q + 1]) * (x.xc + x.xf);
+        const T UFx0 = T(0.25) * (sDU[q - 1] + sDU[q]) * (x.xb + x.xc);
+        const T UFe1 = T(0.25) * (sDV[q + FW] + sDV[q + FW - 1]) * (x.yf + x.xc);
+        const T UFe0 = T(0.25) * (sDV[q] + sDV[q - 1]) * (x.xc + x.yb);
+        r = r - ((UFx1 - UFx0) + (UFe1 - UFe0));
+        const T cr1 = ((T(0.5) * (x.hc + sZ1[q])) * x.fc) * (x.oc + x.of);
+        const T cr0 = ((T(0.5) * (x.hb + sZ1[q - 1])) * x.fb) * (x.ob + x.obf);
+        r = r + T(0.5) * (cr1 + cr0);
+        r = r - ((P.rdrg * x.xc) * omu) * onu;
+        r = r + (x.st * omu) * onu;
+        const T fac = T(1) / ((x.hc + sZS[q]) + (x.hb + sZS[q - 1]));
+        return (x.x0 * (Dbp(q, x.hc, x.z0c) + Dbp(q - 1, x.hb, x.z0b)) + (x.cff * c1dt) * r) * fac;
     };
-    // depth of the "before" state of the predictor
-    auto Db = [&](int o, long long d, uint32_t f) {
-        return first ? ldo(P.h, d, f) + sZ1[o] : ldo(P.h, d, f) + ldo(P.z0, d, f);
+    auto pred_v = [&](int q, const UVIn<T> &x, T &r) {
+        const T omv = x.m1, onv = x.m2;
+        const T g0 = gzp(q - FW, x.z0b), g1 = gzp(q, x.z0c);
+        r = ((T(0.5) * P.g) * omv) * ((x.hb + x.hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
+        const T VFx1 = T(0.25) * (sDU[q + 1] + sDU[q + 1 - FW]) * (x.yf + x.xc);
+        const T VFx0 = T(0.25) * (sDU[q] + sDU[q - FW]) * (x.xc + x.yb);
+        const T VFe1 = T(0.25) * (sDV[q] + sDV[q + FW]) * (x.xc + x.xf);
+        const T VFe0 = T(0.25) * (sDV[q - FW] + sDV[q]) * (x.xb + x.xc);
+        r = r - ((VFx1 - VFx0) + (VFe1 - VFe0));
+        const T cr1 = ((T(0.5) * (x.hc + sZ1[q])) * x.fc) * (x.oc + x.of);
+        const T cr0 = ((T(0.5) * (x.hb + sZ1[q - FW])) * x.fb) * (x.ob + x.obf);
+        r = r - T(0.5) * (cr1 + cr0);
+        r = r - ((P.rdrg * x.xc) * omv) * onv;
+        r = r + (x.st * omv) * onv;
+        const T fac = T(1) / ((x.hc + sZS[q]) + (x.hb + sZS[q - FW]));
+        return (x.x0 * (Dbp(q, x.hc, x.z0c) + Dbp(q - FW, x.hb, x.z0b)) + (x.cff * c1dt) * r) * fac;
     };
 
-    for_region<TI, TJ, NT, 1, 1>(tid, [&](int k, int a, int b) {
-        const int i = fi0 + a, j = fj0 + b;
-        const int o = b * FW + a;
-        T us = T(0), vs = T(0);
-        if (inside(i, j)) {
-            // ubar
-            if (i >= 2 && i <= L - 1) {
-                const int qj = clampi(j, 1, M - 1);
-                const bool own = k >= 0 && qj == j;
-                const int q = o + (qj - j) * FW;
-                const uint32_t f = off(i, qj);
-                const T onu = own ? onuc[k] : onu_at(f);
-                const T omu = own ? omuc[k] : omu_at(f);
-                const T g0 = gz_pred(q - 1, -1, f), g1 = gz_pred(q, 0, f);
-                const T hm = ldo(P.h, -1, f), hc = ldo(P.h, 0, f);
-                T r = ((T(0.5) * P.g) * onu) * ((hm + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
-                const T um = ldo(P.u1, -1, f), uc = ldo(P.u1, 0, f), up = ldo(P.u1, 1, f);
-                const T un = ldo(P.u1, W, f), usn = ldo(P.u1, -W, f);
-                const T UFx1 = T(0.25) * (sDU[q] + sDU[q + 1]) * (uc + up);
-                const T UFx0 = T(0.25) * (sDU[q - 1] + sDU[q]) * (um + uc);
-                const T UFe1 = T(0.25) * (sDV[q + FW] + sDV[q + FW - 1]) * (un + uc);
-                const T UFe0 = T(0.25) * (sDV[q] + sDV[q - 1]) * (uc + usn);
-                r = r - ((UFx1 - UFx0) + (UFe1 - UFe0));
-                const T cr1 = ((T(0.5) * (hc + sZ1[q])) * ldo(P.fomn, 0, f)) * (ldo(P.v1, 0, f) + ldo(P.v1, W, f));
-                const T cr0 = ((T(0.5) * (hm + sZ1[q - 1])) * ldo(P.fomn, -1, f)) *
-                              (ldo(P.v1, -1, f) + ldo(P.v1, W - 1, f));
-                r = r + T(0.5) * (cr1 + cr0);
-                r = r - ((P.rdrg * uc) * omu) * onu;
-                r = r + (ldo(P.sustr, 0, f) * omu) * onu;
-                if (own) {
-                    sto(P.ru1, f, r);
-                    ruc[k] = r;
-                }
-                const T cff = own ? cffuc[k] : cffu_at(f);
-                const T fac = T(1) / ((hc + sZS[q]) + (hm + sZS[q - 1]));
-                const T ub = first ? uc : ldo(P.u0, 0, f);
-                us = (ub * (Db(q, 0, f) + Db(q - 1, -1, f)) + (cff * c1dt) * r) * fac;
-            }
-            // vbar
-            if (j >= 2 && j <= M - 1) {
-                const int qi = clampi(i, 1, L - 1);
-                const bool own = k >= 0 && qi == i;
-                const int q = o + (qi - i);
-                const uint32_t f = off(qi, j);
-                const T omv = own ? omvc[k] : omv_at(f);
-                const T onv = own ? onvc[k] : onv_at(f);
-                const T g0 = gz_pred(q - FW, -W, f), g1 = gz_pred(q, 0, f);
-                const T hs = ldo(P.h, -W, f), hc = ldo(P.h, 0, f);
-                T r = ((T(0.5) * P.g) * omv) * ((hs + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
-                const T vc = ldo(P.v1, 0, f), vp = ldo(P.v1, 1, f), vm = ldo(P.v1, -1, f);
-                const T vn = ldo(P.v1, W, f), vsn = ldo(P.v1, -W, f);
-                const T VFx1 = T(0.25) * (sDU[q + 1] + sDU[q + 1 - FW]) * (vp + vc);
-                const T VFx0 = T(0.25) * (sDU[q] + sDU[q - FW]) * (vc + vm);
-                const T VFe1 = T(0.25) * (sDV[q] + sDV[q + FW]) * (vc + vn);
-                const T VFe0 = T(0.25) * (sDV[q - FW] + sDV[q]) * (vsn + vc);
-                r = r - ((VFx1 - VFx0) + (VFe1 - VFe0));
-                const T cr1 = ((T(0.5) * (hc + sZ1[q])) * ldo(P.fomn, 0, f)) * (ldo(P.u1, 0, f) + ldo(P.u1, 1, f));
-                const T cr0 = ((T(0.5) * (hs + sZ1[q - FW])) * ldo(P.fomn, -W, f)) *
-                              (ldo(P.u1, -W, f) + ldo(P.u1, 1 - W, f));
-                r = r - T(0.5) * (cr1 + cr0);
-                r = r - ((P.rdrg * vc) * omv) * onv;
-                r = r + (ldo(P.svstr, 0, f) * omv) * onv;
-                if (own) {
-                    sto(P.rv1, f, r);
-                    rvc[k] = r;
-                }
-                const T cff = own ? cffvc[k] : cffv_at(f);
-                const T fac = T(1) / ((hc + sZS[q]) + (hs + sZS[q - FW]));
-                const T vb = first ? vc : ldo(P.v0, 0, f);
-                vs = (vb * (Db(q, 0, f) + Db(q - FW, -W, f)) + (cff * c1dt) * r) * fac;
-            }
-        }
-        sUS[o] = us;
-        sVS[o] = vs;
-    });
-    __syncthreads();
+    // ============ core-point registers ============
+    bool cin[KC];
+    uint32_t cf[KC];
+    T z1c[KC], u1c[KC], v1c[KC], hc_[KC], hw_[KC], hs_[KC];
+    T pm_[KC], pmw_[KC], pms_[KC], pn_[KC], pnw_[KC], pns_[KC];
+    T onuc[KC], omuc[KC], cffuc[KC], omvc[KC], onvc[KC], cffvc[KC], pmnc[KC];
+    T zsc[KC], ruc[KC], rvc[KC], duc[KC], dvc[KC];
+    // G1 (predictor) and G2 (corrector) prefetch
+    T z0c_[KC], z0w_[KC], z0s_[KC], u0c_[KC], v0c_[KC], fo_[KC], fow_[KC], fos_[KC], su_[KC], sv_[KC];
+    T u1w_[KC], u1e_[KC], u1n_[KC], u1s_[KC], u1es_[KC], v1w_[KC], v1e_[KC], v1n_[KC], v1s_[KC], v1wn_[KC];
+    T rz0_[KC], ru0_[KC], rv0_[KC], z1w_[KC], z1s_[KC];
+    double zt_[KC], dua_[KC], dva_[KC];
 
-    // ---- C0: fluxes DUs, DVs of the predicted state ----------------------------------
-    T duc[KC], dvc[KC];
-    for_region<TI, TJ, NT, 1, 1>(tid, [&](int k, int a, int b) {
+    // fast-path predicates of a core point (its computations use only its own offset)
+    auto rho_int = [&](int i, int j) { return i >= 1 && i <= L - 1 && j >= 1 && j <= M - 1; };
+    auto u_int = [&](int i, int j) { return i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1; };
+    auto v_int = [&](int i, int j) { return i >= 1 && i <= L - 1 && j >= 2 && j <= M - 1; };
+
+    // ---- G0 + L: z1 over the frame; core statics -----------------------------------------
+#pragma unroll
+    for (int k = 0; k < KC; k++) {
+        int a, b;
+        core_ab(k, a, b);
         const int i = fi0 + a, j = fj0 + b;
-        const int o = b * FW + a;
-        T du = T(0), dv = T(0);
-        if (inside(i, j)) {
-            const uint32_t f = off(i, j);
-            const T h0 = ldo(P.h, 0, f);
+        cin[k] = inside(i, j);
+        cf[k] = cin[k] ? off(i, j) : 0u;
+        const uint32_t f = cf[k];
+        z1c[k] = u1c[k] = v1c[k] = hc_[k] = hw_[k] = hs_[k] = T(0);
+        pm_[k] = pmw_[k] = pms_[k] = pn_[k] = pnw_[k] = pns_[k] = T(1);
+        if (cin[k]) {
+            z1c[k] = ldo(P.z1, 0, f);
+            u1c[k] = ldo(P.u1, 0, f);
+            v1c[k] = ldo(P.v1, 0, f);
+            hc_[k] = ldo(P.h, 0, f);
+            pm_[k] = ldo(P.pm, 0, f);
+            pn_[k] = ldo(P.pn, 0, f);
             if (i >= 1) {
-                const T onu = k >= 0 ? onuc[k] : onu_at(f);
-                du = ((T(0.5) * ((ldo(P.h, -1, f) + sZS[o - 1]) + (h0 + sZS[o]))) * sUS[o]) * onu;
+                hw_[k] = ldo(P.h, -1, f);
+                pmw_[k] = ldo(P.pm, -1, f);
+                pnw_[k] = ldo(P.pn, -1, f);
             }
             if (j >= 1) {
-                const T omv = k >= 0 ? omvc[k] : omv_at(f);
-                dv = ((T(0.5) * ((ldo(P.h, -W, f) + sZS[o - FW]) + (h0 + sZS[o]))) * sVS[o]) * omv;
+                hs_[k] = ldo(P.h, -W, f);
+                pms_[k] = ldo(P.pm, -W, f);
+                pns_[k] = ldo(P.pn, -W, f);
             }
         }
-        sDU[o] = du;
-        sDV[o] = dv;
-        if (k >= 0) {
+    }
+    {
+        constexpr int RING = ring_size<TI, TJ, 3, 2>();
+        constexpr int RS = (RING + NT - 1) / NT;
+        T rz[RS];
+        int ro[RS];
+#pragma unroll
+        for (int s = 0; s < RS; s++) {
+            const int t = tid + s * NT;
+            rz[s] = T(0);
+            ro[s] = -1;
+            if (t < RING) {
+                int a, b;
+                ring_point<TI, TJ, 3, 2>(t, a, b);
+                const int i = fi0 + a, j = fj0 + b;
+                ro[s] = b * FW + a;
+                if (inside(i, j)) rz[s] = ldo(P.z1, 0, off(i, j));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            int a, b;
+            core_ab(k, a, b);
+            sZ1[b * FW + a] = z1c[k];
+        }
+#pragma unroll
+        for (int s = 0; s < RS; s++)
+            if (ro[s] >= 0) sZ1[ro[s]] = rz[s];
+    }
+    lds_barrier();
+
+    // ---- A: DU1 at u points, DV1 at v points ---------------------------------------------
+    {
+        constexpr int RING = ring_size<TI, TJ, 2, 2>();
+        constexpr int RS = (RING + NT - 1) / NT;
+        T ru[RS], rv[RS], rh[RS], rhw[RS], rhs[RS], rpm[RS], rpms[RS], rpn[RS], rpnw[RS];
+        int ro[RS], ri[RS], rj[RS];
+#pragma unroll
+        for (int s = 0; s < RS; s++) {
+            const int t = tid + s * NT;
+            ro[s] = -1;
+            ru[s] = rv[s] = rh[s] = rhw[s] = rhs[s] = T(0);
+            rpm[s] = rpms[s] = rpn[s] = rpnw[s] = T(1);
+            ri[s] = rj[s] = -1;
+            if (t < RING) {
+                int a, b;
+                ring_point<TI, TJ, 2, 2>(t, a, b);
+                const int i = fi0 + a, j = fj0 + b;
+                ro[s] = b * FW + a;
+                if (inside(i, j)) {
+                    ri[s] = i;
+                    rj[s] = j;
+                    const uint32_t f = off(i, j);
+                    ru[s] = ldo(P.u1, 0, f);
+                    rv[s] = ldo(P.v1, 0, f);
+                    rh[s] = ldo(P.h, 0, f);
+                    rpm[s] = ldo(P.pm, 0, f);
+                    rpn[s] = ldo(P.pn, 0, f);
+                    if (i >= 1) {
+                        rhw[s] = ldo(P.h, -1, f);
+                        rpnw[s] = ldo(P.pn, -1, f);
+                    }
+                    if (j >= 1) {
+                        rhs[s] = ldo(P.h, -W, f);
+                        rpms[s] = ldo(P.pm, -W, f);
+                    }
+                }
+            }
+        }
+        // core: metric factors, fluxes
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            int a, b;
+            core_ab(k, a, b);
+            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
+            T du = T(0), dv = T(0);
+            onuc[k] = T(2) / (pnw_[k] + pn_[k]);
+            omuc[k] = T(2) / (pmw_[k] + pm_[k]);
+            cffuc[k] = (pm_[k] + pmw_[k]) * (pn_[k] + pnw_[k]);
+            omvc[k] = T(2) / (pms_[k] + pm_[k]);
+            onvc[k] = T(2) / (pns_[k] + pn_[k]);
+            cffvc[k] = (pm_[k] + pms_[k]) * (pn_[k] + pns_[k]);
+            pmnc[k] = pm_[k] * pn_[k];
+            if (cin[k]) {
+                if (i >= 1) du = ((T(0.5) * ((hw_[k] + sZ1[o - 1]) + (hc_[k] + sZ1[o]))) * u1c[k]) * onuc[k];
+                if (j >= 1) dv = ((T(0.5) * ((hs_[k] + sZ1[o - FW]) + (hc_[k] + sZ1[o]))) * v1c[k]) * omvc[k];
+            }
+            sDU[o] = du;
+            sDV[o] = dv;
+        }
+        // ring fluxes
+#pragma unroll
+        for (int s = 0; s < RS; s++) {
+            if (ro[s] < 0) continue;
+            const int o = ro[s];
+            T du = T(0), dv = T(0);
+            if (ri[s] >= 1) {
+                const T onu = T(2) / (rpnw[s] + rpn[s]);
+                du = ((T(0.5) * ((rhw[s] + sZ1[o - 1]) + (rh[s] + sZ1[o]))) * ru[s]) * onu;
+            }
+            if (rj[s] >= 1) {
+                const T omv = T(2) / (rpms[s] + rpm[s]);
+                dv = ((T(0.5) * ((rhs[s] + sZ1[o - FW]) + (rh[s] + sZ1[o]))) * rv[s]) * omv;
+            }
+            sDU[o] = du;
+            sDV[o] = dv;
+        }
+    }
+        // core: issue the predictor inputs (G1)
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            int a, b;
+            core_ab(k, a, b);
+            const int i = fi0 + a, j = fj0 + b;
+            const uint32_t f = cf[k];
+            z0c_[k] = z0w_[k] = z0s_[k] = u0c_[k] = v0c_[k] = T(0);
+            fo_[k] = fow_[k] = fos_[k] = su_[k] = sv_[k] = T(0);
+            u1w_[k] = u1e_[k] = u1n_[k] = u1s_[k] = u1es_[k] = T(0);
+            v1w_[k] = v1e_[k] = v1n_[k] = v1s_[k] = v1wn_[k] = T(0);
+            const bool ui = cin[k] && u_int(i, j), vi = cin[k] && v_int(i, j), ri_ = cin[k] && rho_int(i, j);
+            if (ri_ || ui || vi) {
+                if (!first) z0c_[k] = ldo(P.z0, 0, f);
+                fo_[k] = ldo(P.fomn, 0, f);
+            }
+            if (ui) {
+                if (!first) {
+                    z0w_[k] = ldo(P.z0, -1, f);
+                    u0c_[k] = ldo(P.u0, 0, f);
+                }
+                fow_[k] = ldo(P.fomn, -1, f);
+                su_[k] = ldo(P.sustr, 0, f);
+                u1w_[k] = ldo(P.u1, -1, f);
+                u1e_[k] = ldo(P.u1, 1, f);
+                u1n_[k] = ldo(P.u1, W, f);
+                u1s_[k] = ldo(P.u1, -W, f);
+                v1w_[k] = ldo(P.v1, -1, f);
+                v1wn_[k] = ldo(P.v1, W - 1, f);
+                v1n_[k] = ldo(P.v1, W, f);
+            }
+            if (vi) {
+                if (!first) {
+                    z0s_[k] = ldo(P.z0, -W, f);
+                    v0c_[k] = ldo(P.v0, 0, f);
+                }
+                fos_[k] = ldo(P.fomn, -W, f);
+                sv_[k] = ldo(P.svstr, 0, f);
+                v1e_[k] = ldo(P.v1, 1, f);
+                v1w_[k] = ldo(P.v1, -1, f);
+                v1n_[k] = ldo(P.v1, W, f);
+                v1s_[k] = ldo(P.v1, -W, f);
+                u1e_[k] = ldo(P.u1, 1, f);
+                u1s_[k] = ldo(P.u1, -W, f);
+                u1es_[k] = ldo(P.u1, 1 - W, f);
+            }
+        }
+    lds_barrier();
+
+    // ---- P1: predicted zeta and its right-hand side -----------------------------------
+    {
+        constexpr int RING = ring_size<TI, TJ, 2, 1>();
+        constexpr int RS = (RING + NT - 1) / NT;
+        T rpm[RS], rpn[RS], rz0[RS];
+        int ro[RS], rq[RS];
+#pragma unroll
+        for (int s = 0; s < RS; s++) {
+            const int t = tid + s * NT;
+            ro[s] = -1;
+            rq[s] = 0;
+            rpm[s] = rpn[s] = rz0[s] = T(0);
+            if (t < RING) {
+                int a, b;
+                ring_point<TI, TJ, 2, 1>(t, a, b);
+                const int i = fi0 + a, j = fj0 + b;
+                ro[s] = b * FW + a;
+                if (inside(i, j)) {
+                    const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
+                    rq[s] = ro[s] + (qj - j) * FW + (qi - i);
+                    const uint32_t fq = off(qi, qj);
+                    rpm[s] = ldo(P.pm, 0, fq);
+                    rpn[s] = ldo(P.pn, 0, fq);
+                    if (!first) rz0[s] = ldo(P.z0, 0, fq);
+                } else {
+                    ro[s] = -2 - ro[s];   // outside: store zeros
+                }
+            }
+        }
+        auto zs_of = [&](int qo, T pmn, T z0, T &r) {
+            r = (sDU[qo] - sDU[qo + 1]) + (sDV[qo] - sDV[qo + FW]);
+            if (first) return sZ1[qo] + (pmn * dt) * r;
+            return z0 + (pmn * (T(2) * dt)) * r;
+        };
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            int a, b;
+            core_ab(k, a, b);
+            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
+            T zs = T(0), r = T(0);
+            if (cin[k]) {
+                if (rho_int(i, j)) {
+                    zs = zs_of(o, pmnc[k], z0c_[k], r);
+                    sto(P.rz1, cf[k], r);
+                } else {
+                    const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
+                    const uint32_t fq = off(qi, qj);
+                    const T pmn = ldo(P.pm, 0, fq) * ldo(P.pn, 0, fq);
+                    const T z0 = first ? T(0) : ldo(P.z0, 0, fq);
+                    zs = zs_of(o + (qj - j) * FW + (qi - i), pmn, z0, r);
+                }
+            }
+            sZS[o] = zs;
+            sRZ[o] = r;
+            zsc[k] = zs;
+        }
+#pragma unroll
+        for (int s = 0; s < RS; s++) {
+            if (ro[s] == -1) continue;
+            if (ro[s] < -1) {
+                sZS[-2 - ro[s]] = T(0);
+                sRZ[-2 - ro[s]] = T(0);
+                continue;
+            }
+            T r;
+            const T zs = zs_of(rq[s], rpm[s] * rpn[s], rz0[s], r);
+            sZS[ro[s]] = zs;
+            sRZ[ro[s]] = r;
+        }
+    }
+    lds_barrier();
+
+    // ---- P2: predicted ubar / vbar --------------------------------------------------
+    {
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            int a, b;
+            core_ab(k, a, b);
+            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
+            T us = T(0), vs = T(0);
+            if (cin[k]) {
+                if (u_int(i, j)) {
+                    UVIn<T> x;
+                    x.m1 = onuc[k];
+                    x.m2 = omuc[k];
+                    x.cff = cffuc[k];
+                    x.hb = hw_[k];
+                    x.hc = hc_[k];
+                    x.xb = u1w_[k];
+                    x.xc = u1c[k];
+                    x.xf = u1e_[k];
+                    x.yf = u1n_[k];
+                    x.yb = u1s_[k];
+                    x.fc = fo_[k];
+                    x.fb = fow_[k];
+                    x.oc = v1c[k];
+                    x.of = v1n_[k];
+                    x.ob = v1w_[k];
+                    x.obf = v1wn_[k];
+                    x.st = su_[k];
+                    x.x0 = first ? u1c[k] : u0c_[k];
+                    x.z0c = z0c_[k];
+                    x.z0b = z0w_[k];
+                    T r;
+                    us = pred_u(o, x, r);
+                    sto(P.ru1, cf[k], r);
+                    ruc[k] = r;
+                    if (j == 1) sRUlo[a - 3] = r;
+                    if (j == M - 1) sRUhi[a - 3] = r;
+                } else if (i >= 2 && i <= L - 1) {   // ghost row: source (i, 1) or (i, M-1)
+                    const int qj = clampi(j, 1, M - 1);
+                    T r;
+                    UVIn<T> x = load_u(off(i, qj));
+                    metrics_u(x);
+                    us = pred_u(o + (qj - j) * FW, x, r);
+                }
+                if (v_int(i, j)) {
+                    UVIn<T> x;
+                    x.m1 = omvc[k];
+                    x.m2 = onvc[k];
+                    x.cff = cffvc[k];
+                    x.hb = hs_[k];
+                    x.hc = hc_[k];
+                    x.xb = v1s_[k];
+                    x.xc = v1c[k];
+                    x.xf = v1n_[k];
+                    x.yf = v1e_[k];
+                    x.yb = v1w_[k];
+                    x.fc = fo_[k];
+                    x.fb = fos_[k];
+                    x.oc = u1c[k];
+                    x.of = u1e_[k];
+                    x.ob = u1s_[k];
+                    x.obf = u1es_[k];
+                    x.st = sv_[k];
+                    x.x0 = first ? v1c[k] : v0c_[k];
+                    x.z0c = z0c_[k];
+                    x.z0b = z0s_[k];
+                    T r;
+                    vs = pred_v(o, x, r);
+                    sto(P.rv1, cf[k], r);
+                    rvc[k] = r;
+                    if (i == 1) sRVlo[b - 3] = r;
+                    if (i == L - 1) sRVhi[b - 3] = r;
+                } else if (j >= 2 && j <= M - 1) {   // ghost column: source (1, j) or (L-1, j)
+                    const int qi = clampi(i, 1, L - 1);
+                    T r;
+                    UVIn<T> x = load_v(off(qi, j));
+                    metrics_v(x);
+                    vs = pred_v(o + (qi - i), x, r);
+                }
+            }
+            sUS[o] = us;
+            sVS[o] = vs;
+        }
+        // core: issue the corrector inputs (G2)
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            int a, b;
+            core_ab(k, a, b);
+            const int i = fi0 + a, j = fj0 + b;
+            const uint32_t f = cf[k];
+            rz0_[k] = ru0_[k] = rv0_[k] = z1w_[k] = z1s_[k] = T(0);
+            zt_[k] = dua_[k] = dva_[k] = 0.0;
+            if (cin[k] && rho_int(i, j)) {
+                const uint32_t fd = offd(i, j);
+                if (!first) rz0_[k] = ldo(P.rz0, 0, f);
+                zt_[k] = ldo(P.Zt, 0, fd);
+                dua_[k] = ldo(P.DUa, 0, fd);
+                dva_[k] = ldo(P.DVa, 0, fd);
+            }
+            if (cin[k] && u_int(i, j)) {
+                if (!first) ru0_[k] = ldo(P.ru0, 0, f);
+                z1w_[k] = ldo(P.z1, -1, f);
+            }
+            if (cin[k] && v_int(i, j)) {
+                if (!first) rv0_[k] = ldo(P.rv0, 0, f);
+                z1s_[k] = ldo(P.z1, -W, f);
+            }
+        }
+        // ring: inputs loaded on demand
+        constexpr int RING = ring_size<TI, TJ, 1, 1>();
+#pragma unroll
+        for (int s = 0; s < (RING + NT - 1) / NT; s++) {
+            const int t = tid + s * NT;
+            if (t >= RING) continue;
+            int a, b;
+            ring_point<TI, TJ, 1, 1>(t, a, b);
+            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
+            T us = T(0), vs = T(0), r;
+            if (inside(i, j)) {
+                if (i >= 2 && i <= L - 1) {
+                    const int qj = clampi(j, 1, M - 1);
+                    UVIn<T> x = load_u(off(i, qj));
+                    metrics_u(x);
+                    us = pred_u(o + (qj - j) * FW, x, r);
+                }
+                if (j >= 2 && j <= M - 1) {
+                    const int qi = clampi(i, 1, L - 1);
+                    UVIn<T> x = load_v(off(qi, j));
+                    metrics_v(x);
+                    vs = pred_v(o + (qi - i), x, r);
+                }
+            }
+            sUS[o] = us;
+            sVS[o] = vs;
+        }
+    }
+    lds_barrier();
+
+    // ---- C0: fluxes DUs, DVs of the predicted state ------------------------------------
+    {
+        constexpr int RING = ring_size<TI, TJ, 1, 1>();
+        constexpr int RS = (RING + NT - 1) / NT;
+        T rh[RS], rhw[RS], rhs[RS], rpm[RS], rpms[RS], rpn[RS], rpnw[RS];
+        int ro[RS], ri[RS], rj[RS];
+#pragma unroll
+        for (int s = 0; s < RS; s++) {
+            const int t = tid + s * NT;
+            ro[s] = -1;
+            ri[s] = rj[s] = -1;
+            rh[s] = rhw[s] = rhs[s] = T(0);
+            rpm[s] = rpms[s] = rpn[s] = rpnw[s] = T(1);
+            if (t < RING) {
+                int a, b;
+                ring_point<TI, TJ, 1, 1>(t, a, b);
+                const int i = fi0 + a, j = fj0 + b;
+                ro[s] = b * FW + a;
+                if (inside(i, j)) {
+                    ri[s] = i;
+                    rj[s] = j;
+                    const uint32_t f = off(i, j);
+                    rh[s] = ldo(P.h, 0, f);
+                    rpm[s] = ldo(P.pm, 0, f);
+                    rpn[s] = ldo(P.pn, 0, f);
+                    if (i >= 1) {
+                        rhw[s] = ldo(P.h, -1, f);
+                        rpnw[s] = ldo(P.pn, -1, f);
+                    }
+                    if (j >= 1) {
+                        rhs[s] = ldo(P.h, -W, f);
+                        rpms[s] = ldo(P.pm, -W, f);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            int a, b;
+            core_ab(k, a, b);
+            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
+            T du = T(0), dv = T(0);
+            if (cin[k]) {
+                if (i >= 1) du = ((T(0.5) * ((hw_[k] + sZS[o - 1]) + (hc_[k] + sZS[o]))) * sUS[o]) * onuc[k];
+                if (j >= 1) dv = ((T(0.5) * ((hs_[k] + sZS[o - FW]) + (hc_[k] + sZS[o]))) * sVS[o]) * omvc[k];
+            }
+            sDU[o] = du;
+            sDV[o] = dv;
             duc[k] = du;
             dvc[k] = dv;
         }
-    });
-    __syncthreads();
+#pragma unroll
+        for (int s = 0; s < RS; s++) {
+            if (ro[s] < 0) continue;
+            const int o = ro[s];
+            T du = T(0), dv = T(0);
+            if (ri[s] >= 1) {
+                const T onu = T(2) / (rpnw[s] + rpn[s]);
+                du = ((T(0.5) * ((rhw[s] + sZS[o - 1]) + (rh[s] + sZS[o]))) * sUS[o]) * onu;
+            }
+            if (rj[s] >= 1) {
+                const T omv = T(2) / (rpms[s] + rpm[s]);
+                dv = ((T(0.5) * ((rhs[s] + sZS[o - FW]) + (rh[s] + sZS[o]))) * sVS[o]) * omv;
+            }
+            sDU[o] = du;
+            sDV[o] = dv;
+        }
+    }
+    lds_barrier();
 
     // ---- C1: corrected zeta, time averages ------------------------------------------
     const T a1 = dt * T(5.0) / T(12.0), a2 = dt * T(8.0) / T(12.0), a3 = dt * T(1.0) / T(12.0);
-    for_region<TI, TJ, NT, 1, 0>(tid, [&](int k, int a, int b) {
-        const int i = fi0 + a, j = fj0 + b;
-        const int o = b * FW + a;
-        T z2 = T(0);
-        if (inside(i, j)) {
-            const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
-            const bool own = k >= 0 && qi == i && qj == j;
-            const int qo = o + (qj - j) * FW + (qi - i);
-            const uint32_t fq = off(qi, qj);
+    {
+        auto z2_of = [&](int qo, T pmn, T z1q, T rz0) {
             const T r = (sDU[qo] - sDU[qo + 1]) + (sDV[qo] - sDV[qo + FW]);
-            const T pmn = own ? pmnc[k] : ldo(P.pm, 0, fq) * ldo(P.pn, 0, fq);
-            const T z1q = own ? z1c[k] : ldo(P.z1, 0, fq);
-            if (first) z2 = z1q + pmn * (dt * r);
-            else z2 = z1q + pmn * ((a1 * r + a2 * sRZ[qo]) - a3 * ldo(P.rz0, 0, fq));
-            if (k >= 0) {
-                sto(P.z2, off(i, j), z2);
-                if (own) {
-                    const double w = P.w1;
-                    const uint32_t fd = offd(i, j);
-                    sto(P.Zt, fd, ldo(P.Zt, 0, fd) + w * (double)zsc[k]);
-                    sto(P.DUa, fd, ldo(P.DUa, 0, fd) + w * (double)duc[k]);
-                    sto(P.DVa, fd, ldo(P.DVa, 0, fd) + w * (double)dvc[k]);
+            if (first) return z1q + pmn * (dt * r);
+            return z1q + pmn * ((a1 * r + a2 * sRZ[qo]) - a3 * rz0);
+        };
+        constexpr int RING = ring_size<TI, TJ, 1, 0>();
+        constexpr int RS = (RING + NT - 1) / NT;
+        T rpm[RS], rpn[RS], rz1[RS], rrz0[RS];
+        int ro[RS], rq[RS];
+#pragma unroll
+        for (int s = 0; s < RS; s++) {
+            const int t = tid + s * NT;
+            ro[s] = -1;
+            rq[s] = 0;
+            rpm[s] = rpn[s] = rz1[s] = rrz0[s] = T(0);
+            if (t < RING) {
+                int a, b;
+                ring_point<TI, TJ, 1, 0>(t, a, b);
+                const int i = fi0 + a, j = fj0 + b;
+                ro[s] = b * FW + a;
+                if (inside(i, j)) {
+                    const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
+                    rq[s] = ro[s] + (qj - j) * FW + (qi - i);
+                    const uint32_t fq = off(qi, qj);
+                    rpm[s] = ldo(P.pm, 0, fq);
+                    rpn[s] = ldo(P.pn, 0, fq);
+                    rz1[s] = ldo(P.z1, 0, fq);
+                    if (!first) rrz0[s] = ldo(P.rz0, 0, fq);
+                } else {
+                    ro[s] = -2 - ro[s];
                 }
             }
         }
-        // Z2 reuses the Z1 plane: every reader of Z1 finished before the last barrier
-        sZ2[o] = z2;
-    });
-    __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            int a, b;
+            core_ab(k, a, b);
+            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
+            T z2 = T(0);
+            if (cin[k]) {
+                if (rho_int(i, j)) {
+                    z2 = z2_of(o, pmnc[k], z1c[k], rz0_[k]);
+                    const double w = P.w1;
+                    const uint32_t fd = offd(i, j);
+                    sto(P.Zt, fd, zt_[k] + w * (double)zsc[k]);
+                    sto(P.DUa, fd, dua_[k] + w * (double)duc[k]);
+                    sto(P.DVa, fd, dva_[k] + w * (double)dvc[k]);
+                } else {
+                    const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
+                    const uint32_t fq = off(qi, qj);
+                    const T pmn = ldo(P.pm, 0, fq) * ldo(P.pn, 0, fq);
+                    const T rz0 = first ? T(0) : ldo(P.rz0, 0, fq);
+                    z2 = z2_of(o + (qj - j) * FW + (qi - i), pmn, ldo(P.z1, 0, fq), rz0);
+                }
+                sto(P.z2, cf[k], z2);
+            }
+            // Z2 reuses the Z1 plane: every reader of Z1 finished before the last barrier
+            sZ2[o] = z2;
+        }
+#pragma unroll
+        for (int s = 0; s < RS; s++) {
+            if (ro[s] == -1) continue;
+            if (ro[s] < -1) {
+                sZ2[-2 - ro[s]] = T(0);
+                continue;
+            }
+            sZ2[ro[s]] = z2_of(rq[s], rpm[s] * rpn[s], rz1[s], rrz0[s]);
+        }
+    }
+    lds_barrier();
 
     // ---- C2: corrected ubar / vbar on the core -----------------------------------------
     const T cf4 = T(2.0 / 5.0), cf5 = T(1) - cf4;
     const T b1 = T(0.5) * dt * T(5.0) / T(12.0), b2 = T(0.5) * dt * T(8.0) / T(12.0),
             b3 = T(0.5) * dt * T(1.0) / T(12.0);
-    auto gz_corr = [&](int o) { return cf5 * sZ2[o] + cf4 * sZS[o]; };
-    for_region<TI, TJ, NT, 0, 0>(tid, [&](int k, int a, int b) {
-        const int i = fi0 + a, j = fj0 + b;
-        if (!inside(i, j)) return;
-        const int o = b * FW + a;
-        const uint32_t fo = off(i, j);
-        // ubar
-        T u2 = T(0);
-        if (i >= 2 && i <= L - 1) {
+    auto gzc = [&](int o) { return cf5 * sZ2[o] + cf4 * sZS[o]; };
+    // corrected ubar at LDS index q: metrics m1 = onu, m2 = omu; statics hb/hc, fc/fb, st
+    auto corr_u = [&](int q, T onu, T omu, T cff, T hb, T hc, T fc, T fb, T st, T ru1, T ru0, T z1c_, T z1b,
+                      T u1c_) {
+        const T g0 = gzc(q - 1), g1 = gzc(q);
+        T r = ((T(0.5) * P.g) * onu) * ((hb + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
+        const T uc = sUS[q];
+        const T UFx1 = T(0.25) * (sDU[q] + sDU[q + 1]) * (uc + sUS[q + 1]);
+        const T UFx0 = T(0.25) * (sDU[q - 1] + sDU[q]) * (sUS[q - 1] + uc);
+        const T UFe1 = T(0.25) * (sDV[q + FW] + sDV[q + FW - 1]) * (sUS[q + FW] + uc);
+        const T UFe0 = T(0.25) * (sDV[q] + sDV[q - 1]) * (uc + sUS[q - FW]);
+        r = r - ((UFx1 - UFx0) + (UFe1 - UFe0));
+        const T cr1 = ((T(0.5) * (hc + sZS[q])) * fc) * (sVS[q] + sVS[q + FW]);
+        const T cr0 = ((T(0.5) * (hb + sZS[q - 1])) * fb) * (sVS[q - 1] + sVS[q - 1 + FW]);
+        r = r + T(0.5) * (cr1 + cr0);
+        r = r - ((P.rdrg * uc) * omu) * onu;
+        r = r + (st * omu) * onu;
+        const T fac = T(1) / ((hc + sZ2[q]) + (hb + sZ2[q - 1]));
+        T tend;
+        if (first) tend = (T(0.5) * dt) * r;
+        else tend = (b1 * r + b2 * ru1) - b3 * ru0;
+        return (u1c_ * ((hc + z1c_) + (hb + z1b)) + cff * tend) * fac;
+    };
+    auto corr_v = [&](int q, T omv, T onv, T cff, T hb, T hc, T fc, T fb, T st, T rv1, T rv0, T z1c_, T z1b,
+                      T v1c_) {
+        const T g0 = gzc(q - FW), g1 = gzc(q);
+        T r = ((T(0.5) * P.g) * omv) * ((hb + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
+        const T vc = sVS[q];
+        const T VFx1 = T(0.25) * (sDU[q + 1] + sDU[q + 1 - FW]) * (sVS[q + 1] + vc);
+        const T VFx0 = T(0.25) * (sDU[q] + sDU[q - FW]) * (vc + sVS[q - 1]);
+        const T VFe1 = T(0.25) * (sDV[q] + sDV[q + FW]) * (vc + sVS[q + FW]);
+        const T VFe0 = T(0.25) * (sDV[q - FW] + sDV[q]) * (sVS[q - FW] + vc);
+        r = r - ((VFx1 - VFx0) + (VFe1 - VFe0));
+        const T cr1 = ((T(0.5) * (hc + sZS[q])) * fc) * (sUS[q] + sUS[q + 1]);
+        const T cr0 = ((T(0.5) * (hb + sZS[q - FW])) * fb) * (sUS[q - FW] + sUS[q - FW + 1]);
+        r = r - T(0.5) * (cr1 + cr0);
+        r = r - ((P.rdrg * vc) * omv) * onv;
+        r = r + (st * omv) * onv;
+        const T fac = T(1) / ((hc + sZ2[q]) + (hb + sZ2[q - FW]));
+        T tend;
+        if (first) tend = (T(0.5) * dt) * r;
+        else tend = (b1 * r + b2 * rv1) - b3 * rv0;
+        return (v1c_ * ((hc + z1c_) + (hb + z1b)) + cff * tend) * fac;
+    };
+#pragma unroll
+    for (int k = 0; k < KC; k++) {
+        int a, b;
+        core_ab(k, a, b);
+        const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
+        if (!cin[k]) continue;
+        T u2 = T(0), v2 = T(0);
+        if (u_int(i, j)) {
+            u2 = corr_u(o, onuc[k], omuc[k], cffuc[k], hw_[k], hc_[k], fo_[k], fow_[k], su_[k], ruc[k], ru0_[k],
+                        z1c[k], z1w_[k], u1c[k]);
+        } else if (i >= 2 && i <= L - 1) {   // ghost row j = 0 or M: source row 1 or M-1
             const int qj = clampi(j, 1, M - 1);
-            const bool own = qj == j;
-            const int q = o + (qj - j) * FW;
             const uint32_t f = off(i, qj);
-            const T onu = own ? onuc[k] : onu_at(f);
-            const T omu = own ? omuc[k] : omu_at(f);
-            const T g0 = gz_corr(q - 1), g1 = gz_corr(q);
-            const T hm = ldo(P.h, -1, f), hc = ldo(P.h, 0, f);
-            T r = ((T(0.5) * P.g) * onu) * ((hm + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
-            const T uc = sUS[q];
-            const T UFx1 = T(0.25) * (sDU[q] + sDU[q + 1]) * (uc + sUS[q + 1]);
-            const T UFx0 = T(0.25) * (sDU[q - 1] + sDU[q]) * (sUS[q - 1] + uc);
-            const T UFe1 = T(0.25) * (sDV[q + FW] + sDV[q + FW - 1]) * (sUS[q + FW] + uc);
-            const T UFe0 = T(0.25) * (sDV[q] + sDV[q - 1]) * (uc + sUS[q - FW]);
-            r = r - ((UFx1 - UFx0) + (UFe1 - UFe0));
-            const T cr1 = ((T(0.5) * (hc + sZS[q])) * ldo(P.fomn, 0, f)) * (sVS[q] + sVS[q + FW]);
-            const T cr0 = ((T(0.5) * (hm + sZS[q - 1])) * ldo(P.fomn, -1, f)) * (sVS[q - 1] + sVS[q - 1 + FW]);
-            r = r + T(0.5) * (cr1 + cr0);
-            r = r - ((P.rdrg * uc) * omu) * onu;
-            r = r + (ldo(P.sustr, 0, f) * omu) * onu;
-            const T cff = own ? cffuc[k] : cffu_at(f);
-            const T fac = T(1) / ((hc + sZ2[q]) + (hm + sZ2[q - 1]));
-            T tend;
-            if (first) tend = (T(0.5) * dt) * r;
-            else tend = (b1 * r + b2 * (own ? ruc[k] : ldo((const T *)P.ru1, 0, f))) - b3 * ldo(P.ru0, 0, f);
-            const T z1q = own ? z1c[k] : ldo(P.z1, 0, f);
-            const T u1q = own ? u1c[k] : ldo(P.u1, 0, f);
-            u2 = (u1q * ((hc + z1q) + (hm + ldo(P.z1, -1, f))) + cff * tend) * fac;
+            const T ru1 = qj == 1 ? sRUlo[a - 3] : sRUhi[a - 3];
+            const T ru0 = first ? T(0) : ldo(P.ru0, 0, f);
+            u2 = corr_u(o + (qj - j) * FW, onu_at(f), omu_at(f), cffu_at(f), ldo(P.h, -1, f), ldo(P.h, 0, f),
+                        ldo(P.fomn, 0, f), ldo(P.fomn, -1, f), ldo(P.sustr, 0, f), ru1, ru0, ldo(P.z1, 0, f),
+                        ldo(P.z1, -1, f), ldo(P.u1, 0, f));
         }
-        sto(P.u2, fo, u2);
-        // vbar
-        T v2 = T(0);
-        if (j >= 2 && j <= M - 1) {
+        sto(P.u2, cf[k], u2);
+        if (v_int(i, j)) {
+            v2 = corr_v(o, omvc[k], onvc[k], cffvc[k], hs_[k], hc_[k], fo_[k], fos_[k], sv_[k], rvc[k], rv0_[k],
+                        z1c[k], z1s_[k], v1c[k]);
+        } else if (j >= 2 && j <= M - 1) {   // ghost column i = 0 or L: source column 1 or L-1
             const int qi = clampi(i, 1, L - 1);
-            const bool own = qi == i;
-            const int q = o + (qi - i);
             const uint32_t f = off(qi, j);
-            const T omv = own ? omvc[k] : omv_at(f);
-            const T onv = own ? onvc[k] : onv_at(f);
-            const T g0 = gz_corr(q - FW), g1 = gz_corr(q);
-            const T hs = ldo(P.h, -W, f), hc = ldo(P.h, 0, f);
-            T r = ((T(0.5) * P.g) * omv) * ((hs + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
-            const T vc = sVS[q];
-            const T VFx1 = T(0.25) * (sDU[q + 1] + sDU[q + 1 - FW]) * (sVS[q + 1] + vc);
-            const T VFx0 = T(0.25) * (sDU[q] + sDU[q - FW]) * (vc + sVS[q - 1]);
-            const T VFe1 = T(0.25) * (sDV[q] + sDV[q + FW]) * (vc + sVS[q + FW]);
-            const T VFe0 = T(0.25) * (sDV[q - FW] + sDV[q]) * (sVS[q - FW] + vc);
-            r = r - ((VFx1 - VFx0) + (VFe1 - VFe0));
-            const T cr1 = ((T(0.5) * (hc + sZS[q])) * ldo(P.fomn, 0, f)) * (sUS[q] + sUS[q + 1]);
-            const T cr0 = ((T(0.5) * (hs + sZS[q - FW])) * ldo(P.fomn, -W, f)) * (sUS[q - FW] + sUS[q - FW + 1]);
-            r = r - T(0.5) * (cr1 + cr0);
-            r = r - ((P.rdrg * vc) * omv) * onv;
-            r = r + (ldo(P.svstr, 0, f) * omv) * onv;
-            const T cff = own ? cffvc[k] : cffv_at(f);
-            const T fac = T(1) / ((hc + sZ2[q]) + (hs + sZ2[q - FW]));
-            T tend;
-            if (first) tend = (T(0.5) * dt) * r;
-            else tend = (b1 * r + b2 * (own ? rvc[k] : ldo((const T *)P.rv1, 0, f))) - b3 * ldo(P.rv0, 0, f);
-            const T z1q = own ? z1c[k] : ldo(P.z1, 0, f);
-            const T v1q = own ? v1c[k] : ldo(P.v1, 0, f);
-            v2 = (v1q * ((hc + z1q) + (hs + ldo(P.z1, -W, f))) + cff * tend) * fac;
+            const T rv1 = qi == 1 ? sRVlo[b - 3] : sRVhi[b - 3];
+            const T rv0 = first ? T(0) : ldo(P.rv0, 0, f);
+            v2 = corr_v(o + (qi - i), omv_at(f), onv_at(f), cffv_at(f), ldo(P.h, -W, f), ldo(P.h, 0, f),
+                        ldo(P.fomn, 0, f), ldo(P.fomn, -W, f), ldo(P.svstr, 0, f), rv1, rv0, ldo(P.z1, 0, f),
+                        ldo(P.z1, -W, f), ldo(P.v1, 0, f));
         }
-        sto(P.v2, fo, v2);
-    });
+        sto(P.v2, cf[k], v2);
+    }
 }
 
 // Block id -> tile: each XCD (block id mod 8) walks a contiguous band of tiles so that
@@ -459,8 +958,9 @@ template <typename T, int TI, int TJ, int NT>
 __global__ __launch_bounds__(NT) void k_phys_fused(PFA<T> P)
 {
     __shared__ T sm[PhysLds<TI, TJ>::PLANES][PhysLds<TI, TJ>::FN];
+    __shared__ T side[2 * TI + 2 * TJ];
     const int b = xcd_band(P.tiles_i * P.tiles_j);
-    phys_tile<T, TI, TJ, NT>(P, b % P.tiles_i, b / P.tiles_i, sm);
+    phys_tile<T, TI, TJ, NT>(P, b % P.tiles_i, b / P.tiles_i, sm, side);
 }
 
 template <typename T, int TI, int TJ>
@@ -528,7 +1028,8 @@ int dispatch_tj(const sf_phys_fields *x, double dt, int first, double w1, int tj
 
 extern "C" int sf_phys_fused_default_tile(int32_t precision)
 {
-    return precision == SF_FP64 ? 8 : 16;
+    (void)precision;
+    return 8;
 }
 
 extern "C" int sf_phys_substep_fused(const sf_phys_fields *x, int32_t precision, double dt, int32_t first,
