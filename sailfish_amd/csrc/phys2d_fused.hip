@@ -97,15 +97,15 @@ struct PhysLds {
     static constexpr int FW = TI + 5, FH = TJ + 5, FN = FW * FH, PLANES = 7;
 };
 
-// Element (f + d) of p at byte offset boff of element f: the uniform base plus one
-// 32-bit offset register shared by all neighbours of a point (global_load saddr
-// form); a constant d folds into the instruction's immediate offset, a row stride
-// d = +-W into a shifted scalar base.
+// Element (f + d) of p, given the byte offset boff of element f: the address is the
+// array's scalar base plus a 32-bit vector offset (global_load saddr form).  The
+// neighbour offset boff + d*sizeof(T) is computed in 32-bit arithmetic, so it is one
+// vector add per neighbour and point, shared by every array read at that neighbour.
 template <typename T>
 __device__ __forceinline__ T ldo(const T *p, long long d, uint32_t boff)
 {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + (uint64_t)boff +
-                                        d * (long long)sizeof(T));
+    const uint32_t o = boff + (uint32_t)(d * (long long)sizeof(T));
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + o);
 }
 template <typename T>
 __device__ __forceinline__ void sto(T *p, uint32_t boff, T v)
@@ -157,7 +157,7 @@ struct UVIn {
     T x0, z0c, z0b;             // predictor "before" velocity and zeta (t0) at point / back
 };
 
-template <typename T, int TI, int TJ, int NT>
+template <typename T, int TI, int TJ, int NT, bool FIRST>
 __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*sm)[PhysLds<TI, TJ>::FN],
                                           T *side)
 {
@@ -175,7 +175,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     const int fi0 = ti * TI - P.off_i - 3, fj0 = tj * TJ - P.off_j - 3;   // frame origin
     const int tid = threadIdx.x;
     const T dt = P.dt;
-    const bool first = P.first != 0;
+    constexpr bool first = FIRST;
 
     auto inside = [&](int i, int j) { return i >= 0 && i <= L && j >= 0 && j <= M; };
     auto clampi = [](int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); };
@@ -330,8 +330,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         cin[k] = inside(i, j);
         cf[k] = cin[k] ? off(i, j) : 0u;
         const uint32_t f = cf[k];
-        z1c[k] = u1c[k] = v1c[k] = hc_[k] = hw_[k] = hs_[k] = T(0);
-        pm_[k] = pmw_[k] = pms_[k] = pn_[k] = pnw_[k] = pns_[k] = T(1);
+        // values below are read only under the predicate they are loaded under
+        z1c[k] = T(0);
         if (cin[k]) {
             z1c[k] = ldo(P.z1, 0, f);
             u1c[k] = ldo(P.u1, 0, f);
@@ -391,8 +391,6 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         for (int s = 0; s < RS; s++) {
             const int t = tid + s * NT;
             ro[s] = -1;
-            ru[s] = rv[s] = rh[s] = rhw[s] = rhs[s] = T(0);
-            rpm[s] = rpms[s] = rpn[s] = rpnw[s] = T(1);
             ri[s] = rj[s] = -1;
             if (t < RING) {
                 int a, b;
@@ -426,16 +424,20 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             core_ab(k, a, b);
             const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
             T du = T(0), dv = T(0);
-            onuc[k] = T(2) / (pnw_[k] + pn_[k]);
-            omuc[k] = T(2) / (pmw_[k] + pm_[k]);
-            cffuc[k] = (pm_[k] + pmw_[k]) * (pn_[k] + pnw_[k]);
-            omvc[k] = T(2) / (pms_[k] + pm_[k]);
-            onvc[k] = T(2) / (pns_[k] + pn_[k]);
-            cffvc[k] = (pm_[k] + pms_[k]) * (pn_[k] + pns_[k]);
-            pmnc[k] = pm_[k] * pn_[k];
             if (cin[k]) {
-                if (i >= 1) du = ((T(0.5) * ((hw_[k] + sZ1[o - 1]) + (hc_[k] + sZ1[o]))) * u1c[k]) * onuc[k];
-                if (j >= 1) dv = ((T(0.5) * ((hs_[k] + sZ1[o - FW]) + (hc_[k] + sZ1[o]))) * v1c[k]) * omvc[k];
+                pmnc[k] = pm_[k] * pn_[k];
+                if (i >= 1) {
+                    onuc[k] = T(2) / (pnw_[k] + pn_[k]);
+                    omuc[k] = T(2) / (pmw_[k] + pm_[k]);
+                    cffuc[k] = (pm_[k] + pmw_[k]) * (pn_[k] + pnw_[k]);
+                    du = ((T(0.5) * ((hw_[k] + sZ1[o - 1]) + (hc_[k] + sZ1[o]))) * u1c[k]) * onuc[k];
+                }
+                if (j >= 1) {
+                    omvc[k] = T(2) / (pms_[k] + pm_[k]);
+                    onvc[k] = T(2) / (pns_[k] + pn_[k]);
+                    cffvc[k] = (pm_[k] + pms_[k]) * (pn_[k] + pns_[k]);
+                    dv = ((T(0.5) * ((hs_[k] + sZ1[o - FW]) + (hc_[k] + sZ1[o]))) * v1c[k]) * omvc[k];
+                }
             }
             sDU[o] = du;
             sDV[o] = dv;
@@ -465,10 +467,6 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             core_ab(k, a, b);
             const int i = fi0 + a, j = fj0 + b;
             const uint32_t f = cf[k];
-            z0c_[k] = z0w_[k] = z0s_[k] = u0c_[k] = v0c_[k] = T(0);
-            fo_[k] = fow_[k] = fos_[k] = su_[k] = sv_[k] = T(0);
-            u1w_[k] = u1e_[k] = u1n_[k] = u1s_[k] = u1es_[k] = T(0);
-            v1w_[k] = v1e_[k] = v1n_[k] = v1s_[k] = v1wn_[k] = T(0);
             const bool ui = cin[k] && u_int(i, j), vi = cin[k] && v_int(i, j), ri_ = cin[k] && rho_int(i, j);
             if (ri_ || ui || vi) {
                 if (!first) z0c_[k] = ldo(P.z0, 0, f);
@@ -518,7 +516,6 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const int t = tid + s * NT;
             ro[s] = -1;
             rq[s] = 0;
-            rpm[s] = rpn[s] = rz0[s] = T(0);
             if (t < RING) {
                 int a, b;
                 ring_point<TI, TJ, 2, 1>(t, a, b);
@@ -669,8 +666,6 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             core_ab(k, a, b);
             const int i = fi0 + a, j = fj0 + b;
             const uint32_t f = cf[k];
-            rz0_[k] = ru0_[k] = rv0_[k] = z1w_[k] = z1s_[k] = T(0);
-            zt_[k] = dua_[k] = dva_[k] = 0.0;
             if (cin[k] && rho_int(i, j)) {
                 const uint32_t fd = offd(i, j);
                 if (!first) rz0_[k] = ldo(P.rz0, 0, f);
@@ -728,8 +723,6 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const int t = tid + s * NT;
             ro[s] = -1;
             ri[s] = rj[s] = -1;
-            rh[s] = rhw[s] = rhs[s] = T(0);
-            rpm[s] = rpms[s] = rpn[s] = rpnw[s] = T(1);
             if (t < RING) {
                 int a, b;
                 ring_point<TI, TJ, 1, 1>(t, a, b);
@@ -804,7 +797,6 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const int t = tid + s * NT;
             ro[s] = -1;
             rq[s] = 0;
-            rpm[s] = rpn[s] = rz1[s] = rrz0[s] = T(0);
             if (t < RING) {
                 int a, b;
                 ring_point<TI, TJ, 1, 0>(t, a, b);
@@ -954,19 +946,18 @@ __device__ __forceinline__ int xcd_band(int nblk)
     return b;
 }
 
-template <typename T, int TI, int TJ, int NT>
+template <typename T, int TI, int TJ, int NT, bool FIRST>
 __global__ __launch_bounds__(NT) void k_phys_fused(PFA<T> P)
 {
     __shared__ T sm[PhysLds<TI, TJ>::PLANES][PhysLds<TI, TJ>::FN];
     __shared__ T side[2 * TI + 2 * TJ];
     const int b = xcd_band(P.tiles_i * P.tiles_j);
-    phys_tile<T, TI, TJ, NT>(P, b % P.tiles_i, b / P.tiles_i, sm, side);
+    phys_tile<T, TI, TJ, NT, FIRST>(P, b % P.tiles_i, b / P.tiles_i, sm, side);
 }
 
-template <typename T, int TI, int TJ>
+template <typename T, int TI, int TJ, int NT>
 int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void *stream)
 {
-    constexpr int NT = 256;
     PFA<T> a;
     a.h = (const T *)x->h;
     a.pm = (const T *)x->pm;
@@ -1007,7 +998,12 @@ int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void 
                x->szI, x->szJ);
     const long long nb = (long long)a.tiles_i * a.tiles_j;
     SF_REQUIRE(nb < (1LL << 31), "sf_phys_substep_fused: grid too large");
-    hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT>), dim3((unsigned)nb), dim3(NT), 0, sf::as_stream(stream), a);
+    if (first)
+        hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true>), dim3((unsigned)nb), dim3(NT), 0,
+                           sf::as_stream(stream), a);
+    else
+        hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, false>), dim3((unsigned)nb), dim3(NT), 0,
+                           sf::as_stream(stream), a);
     return sf::check_launch("sf_phys_substep_fused");
 }
 
@@ -1015,9 +1011,9 @@ template <typename T>
 int dispatch_tj(const sf_phys_fields *x, double dt, int first, double w1, int tj, void *stream)
 {
     switch (tj) {
-    case 4: return launch_fused<T, 64, 4>(x, dt, first, w1, stream);
-    case 8: return launch_fused<T, 64, 8>(x, dt, first, w1, stream);
-    case 16: return launch_fused<T, 64, 16>(x, dt, first, w1, stream);
+    case 4: return launch_fused<T, 64, 4, 256>(x, dt, first, w1, stream);
+    case 8: return launch_fused<T, 64, 8, 256>(x, dt, first, w1, stream);
+    case 16: return launch_fused<T, 64, 16, 512>(x, dt, first, w1, stream);
     default: break;
     }
     sf::set_error("sf_phys_substep_fused: unsupported tile height %d (4, 8 or 16)", tj);
