@@ -231,6 +231,85 @@ def run_cpu_faithful(args, budget_s):
                        f"the reference kernels, FP64, {threads} OpenMP threads, {el:.1f} s)")
 
 
+def run_ensemble_gpu(args):
+    """Ensemble workload: 64 members of the basin1 recipe with seeded uneven sizes,
+    winds and noise; batched (one launch per substep, autotuned schedule, replayed
+    from a HIP graph) vs looping the members one launch each."""
+    import torch
+    from sailfish_amd import autotune, configs
+    from sailfish_amd.ensemble import LoopedMembers, Member, PhysEnsemble
+
+    dtype = torch.float64 if args.dtype == "f64" else torch.float32
+    cfgs = configs.ensemble_configs(args.ens_members)
+    members = [Member(c, dtype=dtype) for c in cfgs]
+    ens = PhysEnsemble(members, dtype)
+    pts = ens.points
+    ct = members[0].ct
+    ct.next2DTimeStep()
+    ens.substep(True, ct.cff1())
+    t0 = time.perf_counter()
+    tuned = autotune.tune(ens, nsub=6, reps=2, use_cache=False)
+    tune_s = time.perf_counter() - t0
+    nsub = args.ens_steps - args.ens_steps % 3
+    w = []
+    for _ in range(3):
+        ct.next2DTimeStep()
+        w.append(ct.cff1())
+    ens.capture(w)
+    for _ in range(3):
+        ens.replay()
+    torch.cuda.synchronize()
+    a, b = _events(torch, 1)[0]
+    a.record()
+    for _ in range(nsub // 3):
+        ens.replay()
+    b.record()
+    torch.cuda.synchronize()
+    ms_batched = a.elapsed_time(b) / nsub
+    # the same substeps launched one at a time (no graph)
+    a, b = _events(torch, 1)[0]
+    a.record()
+    for _ in range(30):
+        ens.substep(False, w[0])
+    b.record()
+    torch.cuda.synchronize()
+    ms_direct = a.elapsed_time(b) / 30
+    ens.sync()
+    finite = all(bool(torch.isfinite(m.OCEAN.zeta_t2).all()) for m in members)
+    ens.close()
+    looped = LoopedMembers(members)
+    nl = max(3, min(30, nsub))
+    for _ in range(3):
+        looped.substep(False, w[0])
+    torch.cuda.synchronize()
+    a, b = _events(torch, 1)[0]
+    a.record()
+    for _ in range(nl):
+        looped.substep(False, w[0])
+    b.record()
+    torch.cuda.synchronize()
+    ms_looped = a.elapsed_time(b) / nl
+    bulk = 8 if args.dtype == "f64" else 4
+    bpp = PHYS_BULK * bulk + PHYS_F64 * 8
+    return {
+        "workload": f"{len(members)} basins (basin1 recipe, dx=5 km, dt=20 s), sizes drawn from "
+                    "{256x128, 384x256, 512x256, 768x384, 1024x512} (seed 4), wind U[0.05,0.15] (seed 5)",
+        "points": pts,
+        "substeps_timed": nsub,
+        "schedule": {"tile_j": tuned["tile_j"], "order": tuned["order"], "grid": tuned["grid"]},
+        "autotune_s": tune_s,
+        "batched_graph_ms_per_substep": ms_batched,
+        "batched_graph_cell_steps_per_s": pts / (ms_batched * 1e-3),
+        "batched_direct_ms_per_substep": ms_direct,
+        "looped_ms_per_substep": ms_looped,
+        "looped_cell_steps_per_s": pts / (ms_looped * 1e-3),
+        "speedup_vs_looped": ms_looped / ms_batched,
+        "hbm_frac_batched": bpp * pts / (ms_batched * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "dtype": args.dtype,
+        "finite_after_run": finite,
+    }
+
+
 def calib_copy(args):
     """Plain device-to-device copy of 1 GiB, 5 times: the byte counters of this
     kernel calibrate FETCH_SIZE / WRITE_SIZE (2^30 bytes read and written each)."""
@@ -266,6 +345,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--calib-copy", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-members", type=int, default=64)
+    ap.add_argument("--ens-steps", type=int, default=498)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -327,6 +409,8 @@ def main():
         other = "f32" if args.dtype == "f64" else "f64"
         s2 = run(args, other, min(args.steps, 200), args.warmup)
         out["secondary"] = summary(s2, other, min(args.steps, 200), args.warmup)
+    if not args.no_ensemble and args.mode == "phys":
+        out["ensemble"] = run_ensemble_gpu(args)
     if not args.no_cpu:
         out["cpu_baseline"] = (run_cpu_phys if args.mode == "phys" else run_cpu_faithful)(args, args.cpu_budget)
     print(json.dumps(out))

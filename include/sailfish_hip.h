@@ -255,6 +255,32 @@ int sf_phys_substep_fused(const sf_phys_fields *fields, int32_t precision, doubl
 /* Default tile height of sf_phys_substep_fused for a precision. */
 int sf_phys_fused_default_tile(int32_t precision);
 
+/* ---------------- ensembles of physics-mode basins ---------------------------------- */
+
+/* An ensemble of basins of any sizes advanced together: ONE launch per substep walks a
+ * device list of (member, tile) pairs.  No reference counterpart (the reference runs
+ * one grid per process, mod_cppkernels.h:154-171); each member's result equals its
+ * solo sf_phys_substep_fused run bit for bit.  members[k].zeta[s] etc. are the three
+ * time slots of member k with slot s = level t_s at creation; the library owns the
+ * rotation from then on (sf_ens_substep performs cycleTimes2D before each substep:
+ * level t_k lives in slot (k + rot) % 3, rot = sf_ens_rotation()).  scratch[] unused.
+ * dt[k]: substep of member k.  The ensemble does not own member memory. */
+typedef struct sf_ens sf_ens;
+int sf_ens_create(const sf_phys_fields *members, const double *dt, int32_t n, int32_t precision,
+                  sf_ens **out);
+int sf_ens_destroy(sf_ens *ens);
+/* Schedule: tile height (4, 8, 16), tile order (0 strided over blocks, 1 contiguous
+ * chunk per block with XCD banding) and grid size (0 = one block per tile). */
+int sf_ens_configure(sf_ens *ens, int32_t tile_j, int32_t order, int32_t grid);
+int sf_ens_num_tiles(const sf_ens *ens);
+int sf_ens_rotation(const sf_ens *ens);
+/* One substep of every member (first != 0: Euler start), cff1 = average weight. */
+int sf_ens_substep(sf_ens *ens, int32_t first, double cff1, void *stream);
+/* Capture nsub (multiple of 3) non-first substeps with weights cff1[0..nsub) into a
+ * HIP graph; sf_ens_replay launches it on a stream.  Reconfiguring drops the graph. */
+int sf_ens_capture(sf_ens *ens, int32_t nsub, const double *cff1, void *stream);
+int sf_ens_replay(sf_ens *ens, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,11 @@ Gaussian (reflect padding, truncated at 4 sigma) written here.
           from white noise (seed 2) smoothed sigma=20 cells mapped to 200..4000 m,
           double-gyre tau_x = 0.1 cos(2 pi y/Ly), zeta noise N(0, 1 cm) (seed 3)
           smoothed sigma=5 cells, dt=5 s
+  ensemble_configs
+          members of the basin1 recipe (dx=5 km, dt=20 s) with seeded sizes from
+          {256x128, 384x256, 512x256, 768x384, 1024x512} (seed 4), wind amplitude
+          U[0.05, 0.15] N/m^2 (seed 5) and their own free-surface noise (seed 1000+k,
+          1 cm smoothed sigma=5 cells) on top of the 0.1 m Gaussian bump
 """
 from __future__ import annotations
 
@@ -93,3 +98,27 @@ def random_smooth(szI: int, szJ: int, seed: int = 0) -> dict:
                 pm=1.0 / dx, pn=1.0 / dy, f=1e-4 + 1e-5 * sm(3.0),
                 zeta0=0.05 * sm(1.0), ubar0=0.1 * sm(1.0), vbar0=0.1 * sm(1.0),
                 sustr=1e-4 * sm(4.0), svstr=1e-4 * sm(4.0), dtfast=2.0, nsteps=10, rdrg=3e-4)
+
+
+ENSEMBLE_SIZES = ((256, 128), (384, 256), (512, 256), (768, 384), (1024, 512))
+
+
+def ensemble_member(szI: int, szJ: int, amp: float, seed: int, dx: float = 5e3) -> dict:
+    """One ensemble member: the basin1 recipe at any size with wind amplitude ``amp``
+    and its own seeded free-surface noise added to the Gaussian bump."""
+    cfg = basin1(szI, szJ, dx=dx)
+    _X, Y, _Lx, Ly = _coords(szI, szJ, dx, dx)
+    cfg["sustr"] = amp * np.sin(np.pi * Y / Ly) / RHO0
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cfg["zeta0"] = cfg["zeta0"] + gaussian_smooth(0.01 * rng.standard_normal((szJ, szI)), 5.0)
+    cfg["name"] = f"member{seed}_{szI}x{szJ}"
+    return cfg
+
+
+def ensemble_configs(n: int = 64, sizes=ENSEMBLE_SIZES, seed_size: int = 4, seed_amp: int = 5) -> list:
+    """The 64-member ensemble workload (uneven sizes, per-member wind and noise)."""
+    rs = np.random.Generator(np.random.PCG64(seed_size))
+    ra = np.random.Generator(np.random.PCG64(seed_amp))
+    pick = rs.integers(0, len(sizes), size=n)
+    amps = ra.uniform(0.05, 0.15, size=n)
+    return [ensemble_member(sizes[p][0], sizes[p][1], float(a), 1000 + k) for k, (p, a) in enumerate(zip(pick, amps))]

@@ -35,6 +35,8 @@
 // form bit for bit.
 #include "sf_internal.h"
 
+#include <vector>
+
 namespace {
 
 template <typename T>
@@ -101,16 +103,20 @@ struct PhysLds {
 // array's scalar base plus a 32-bit vector offset (global_load saddr form).  The
 // neighbour offset boff + d*sizeof(T) is computed in 32-bit arithmetic, so it is one
 // vector add per neighbour and point, shared by every array read at that neighbour.
+// All field pointers point to global memory; the explicit address space keeps the
+// global_load/global_store forms also for pointers read from a member table.
+#define SF_GLOBAL __attribute__((address_space(1)))
+#define SF_CONST __attribute__((address_space(4)))
 template <typename T>
 __device__ __forceinline__ T ldo(const T *p, long long d, uint32_t boff)
 {
     const uint32_t o = boff + (uint32_t)(d * (long long)sizeof(T));
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + o);
+    return *(const SF_GLOBAL T *)((const SF_GLOBAL char *)p + o);
 }
 template <typename T>
 __device__ __forceinline__ void sto(T *p, uint32_t boff, T v)
 {
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + boff) = v;
+    *(SF_GLOBAL T *)((SF_GLOBAL char *)p + boff) = v;
 }
 
 // Workgroup barrier that orders LDS only: global loads issued before it stay in
@@ -1020,6 +1026,56 @@ int dispatch_tj(const sf_phys_fields *x, double dt, int first, double w1, int tj
     return SF_ERR_ARG;
 }
 
+
+// ---------------------------------------------------------------------------------
+// Ensembles: many basins of uneven size advance in ONE launch per substep.  The
+// launch walks a device-side list of (member, tile) pairs; each workgroup takes
+// tiles from the list in one of two orders and runs the same per-tile function as
+// the single-grid kernel, so a member's result equals its solo run bit for bit.
+// ---------------------------------------------------------------------------------
+// The member table holds, for every member, its kernel arguments for each of the
+// three time rotations (index 3*member + rot): no runtime indexing of slot pointers
+// inside the kernel.
+// order 0: block b takes list entries b, b + G, b + 2G, ...
+// order 1: block b (after the XCD band remap) takes a contiguous chunk of the list
+template <typename T, int TI, int TJ, int NT, bool FIRST>
+__global__ __launch_bounds__(NT) void k_phys_ensemble(const PFA<T> *__restrict__ members,
+                                                      const int2 *__restrict__ list, int ntiles, int order,
+                                                      int rot, double w1)
+{
+    __shared__ T sm[PhysLds<TI, TJ>::PLANES][PhysLds<TI, TJ>::FN];
+    __shared__ T side[2 * TI + 2 * TJ];
+    const int G = gridDim.x;
+    int t0, t1, step;
+    if (order == 0) {
+        t0 = blockIdx.x;
+        t1 = ntiles;
+        step = G;
+    } else {
+        const int b = xcd_band(G);
+        t0 = (int)((long long)ntiles * b / G);
+        t1 = (int)((long long)ntiles * (b + 1) / G);
+        step = 1;
+    }
+    // the tables are read-only for the whole launch: scalar loads through the constant
+    // address space keep every member pointer in scalar registers
+    const SF_CONST int *clist = (const SF_CONST int *)list;
+    const SF_CONST PFA<T> *cmem = (const SF_CONST PFA<T> *)members;
+    for (int t = t0; t < t1; t += step) {
+        const int ex = clist[2 * t], ey = clist[2 * t + 1];
+        PFA<T> P;
+        {
+            static_assert(sizeof(PFA<T>) % 4 == 0, "argument block of whole words");
+            const SF_CONST uint32_t *src = (const SF_CONST uint32_t *)(cmem + 3 * ex + rot);
+            uint32_t *dst = reinterpret_cast<uint32_t *>(&P);
+#pragma unroll
+            for (int w = 0; w < (int)(sizeof(PFA<T>) / 4); w++) dst[w] = src[w];
+        }
+        P.w1 = w1;
+        phys_tile<T, TI, TJ, NT, FIRST>(P, ey % P.tiles_i, ey / P.tiles_i, sm, side);
+        lds_barrier();   // the next tile overwrites the LDS planes
+    }
+}
 }  // namespace
 
 extern "C" int sf_phys_fused_default_tile(int32_t precision)
@@ -1044,4 +1100,261 @@ extern "C" int sf_phys_substep_fused(const sf_phys_fields *x, int32_t precision,
     if (precision == SF_FP32) return dispatch_tj<float>(x, dt, first, cff1, tj, stream);
     sf::set_error("sf_phys_substep_fused: unknown precision %d", precision);
     return SF_ERR_ARG;
+}
+
+
+// ---------------------------------------------------------------------------------
+// Ensemble C ABI
+// ---------------------------------------------------------------------------------
+struct sf_ens {
+    int n, precision, device;
+    int tile_j, order, grid;
+    int ntiles, rot;
+    void *members;              // device PFA<T>[3 n]: arguments of member k for rotation r at 3k + r
+    int2 *list;                 // device (member, tile) list for tile_j
+    std::vector<sf_phys_fields> host;
+    std::vector<double> dt;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    int graph_nsub;
+};
+
+namespace {
+
+template <int TJ>
+int ens_tiles(const sf_phys_fields &x, int &ti, int &tj, int &oi, int &oj)
+{
+    constexpr int TI = 64;
+    oi = (x.szI % TI == 1) ? 1 : 0;
+    oj = (x.szJ % TJ == 1) ? 1 : 0;
+    ti = (x.szI + oi + TI - 1) / TI;
+    tj = (x.szJ + oj + TJ - 1) / TJ;
+    return ti * tj;
+}
+
+template <typename T>
+int ens_build(sf_ens *e)
+{
+    std::vector<PFA<T>> hm(3 * e->n);
+    std::vector<int2> list;
+    for (int k = 0; k < e->n; k++) {
+        const sf_phys_fields &x = e->host[k];
+        int ti = 0, tj = 0, oi = 0, oj = 0, nt = 0;
+        switch (e->tile_j) {
+        case 4: nt = ens_tiles<4>(x, ti, tj, oi, oj); break;
+        case 8: nt = ens_tiles<8>(x, ti, tj, oi, oj); break;
+        default: nt = ens_tiles<16>(x, ti, tj, oi, oj); break;
+        }
+        for (int rot = 0; rot < 3; rot++) {
+            // level t_l lives in slot (l + rot) % 3
+            const int s0 = rot % 3, s1 = (rot + 1) % 3, s2 = (rot + 2) % 3;
+            PFA<T> &a = hm[3 * k + rot];
+            a.h = (const T *)x.h;
+            a.pm = (const T *)x.pm;
+            a.pn = (const T *)x.pn;
+            a.fomn = (const T *)x.fomn;
+            a.sustr = (const T *)x.sustr;
+            a.svstr = (const T *)x.svstr;
+            a.z0 = (const T *)x.zeta[s0];
+            a.z1 = (const T *)x.zeta[s1];
+            a.z2 = (T *)x.zeta[s2];
+            a.u0 = (const T *)x.ubar[s0];
+            a.u1 = (const T *)x.ubar[s1];
+            a.u2 = (T *)x.ubar[s2];
+            a.v0 = (const T *)x.vbar[s0];
+            a.v1 = (const T *)x.vbar[s1];
+            a.v2 = (T *)x.vbar[s2];
+            a.rz0 = (const T *)x.rzeta[s0];
+            a.rz1 = (T *)x.rzeta[s1];
+            a.ru0 = (const T *)x.rubar[s0];
+            a.ru1 = (T *)x.rubar[s1];
+            a.rv0 = (const T *)x.rvbar[s0];
+            a.rv1 = (T *)x.rvbar[s1];
+            a.Zt = x.Zt_avg1;
+            a.DUa = x.DU_avg1;
+            a.DVa = x.DV_avg1;
+            a.szI = x.szI;
+            a.szJ = x.szJ;
+            a.tiles_i = ti;
+            a.tiles_j = tj;
+            a.off_i = oi;
+            a.off_j = oj;
+            a.g = T(x.g);
+            a.rdrg = T(x.rdrg);
+            a.dt = T(e->dt[k]);
+            a.first = 0;
+            a.w1 = 0.0;
+        }
+        for (int t = 0; t < nt; t++) list.push_back(make_int2(k, t));
+    }
+    e->ntiles = (int)list.size();
+    hipError_t r = hipFree(e->members);
+    if (r == hipSuccess) r = hipFree(e->list);
+    e->members = nullptr;
+    e->list = nullptr;
+    if (r == hipSuccess) r = hipMalloc(&e->members, sizeof(PFA<T>) * hm.size());
+    if (r == hipSuccess) r = hipMalloc((void **)&e->list, sizeof(int2) * list.size());
+    if (r == hipSuccess) r = hipMemcpy(e->members, hm.data(), sizeof(PFA<T>) * hm.size(), hipMemcpyHostToDevice);
+    if (r == hipSuccess) r = hipMemcpy(e->list, list.data(), sizeof(int2) * list.size(), hipMemcpyHostToDevice);
+    if (r != hipSuccess) return sf::hip_fail(r, "sf_ens: device tables");
+    return SF_OK;
+}
+
+template <typename T, int TJ>
+void ens_launch_tj(sf_ens *e, int first, double w1, int rot, hipStream_t s)
+{
+    constexpr int TI = 64, NT = TJ == 16 ? 512 : 256;
+    const int G = e->grid > 0 ? (e->grid < e->ntiles ? e->grid : e->ntiles) : e->ntiles;
+    const PFA<T> *m = (const PFA<T> *)e->members;
+    if (first)
+        hipLaunchKernelGGL((k_phys_ensemble<T, TI, TJ, NT, true>), dim3(G), dim3(NT), 0, s, m, e->list, e->ntiles,
+                           e->order, rot, w1);
+    else
+        hipLaunchKernelGGL((k_phys_ensemble<T, TI, TJ, NT, false>), dim3(G), dim3(NT), 0, s, m, e->list, e->ntiles,
+                           e->order, rot, w1);
+}
+
+template <typename T>
+void ens_launch(sf_ens *e, int first, double w1, int rot, hipStream_t s)
+{
+    switch (e->tile_j) {
+    case 4: ens_launch_tj<T, 4>(e, first, w1, rot, s); break;
+    case 8: ens_launch_tj<T, 8>(e, first, w1, rot, s); break;
+    default: ens_launch_tj<T, 16>(e, first, w1, rot, s); break;
+    }
+}
+
+int ens_rebuild(sf_ens *e)
+{
+    return e->precision == SF_FP64 ? ens_build<double>(e) : ens_build<float>(e);
+}
+
+void ens_drop_graph(sf_ens *e)
+{
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+    e->graph_nsub = 0;
+}
+
+}  // namespace
+
+extern "C" int sf_ens_create(const sf_phys_fields *members, const double *dt, int32_t n, int32_t precision,
+                             sf_ens **out)
+{
+    SF_REQUIRE(out && members && dt && n > 0, "sf_ens_create: bad arguments");
+    SF_REQUIRE(precision == SF_FP64 || precision == SF_FP32, "sf_ens_create: unknown precision %d", precision);
+    *out = nullptr;
+    for (int k = 0; k < n; k++) {
+        const sf_phys_fields &x = members[k];
+        SF_REQUIRE(x.szI >= 4 && x.szJ >= 4, "sf_ens_create: member %d grid %dx%d too small", k, x.szI, x.szJ);
+        SF_REQUIRE((long long)x.szI * x.szJ < (1LL << 29), "sf_ens_create: member %d exceeds 2^29 points", k);
+        SF_REQUIRE(x.h && x.pm && x.pn && x.fomn && x.sustr && x.svstr, "sf_ens_create: member %d null grid field", k);
+        for (int s = 0; s < 3; s++)
+            SF_REQUIRE(x.zeta[s] && x.ubar[s] && x.vbar[s] && x.rzeta[s] && x.rubar[s] && x.rvbar[s],
+                       "sf_ens_create: member %d null time slot %d", k, s);
+        SF_REQUIRE(x.Zt_avg1 && x.DU_avg1 && x.DV_avg1, "sf_ens_create: member %d null average", k);
+    }
+    sf_ens *e = new sf_ens();
+    e->n = n;
+    e->precision = precision;
+    (void)hipGetDevice(&e->device);
+    e->tile_j = 8;
+    e->order = 1;
+    e->grid = 0;
+    e->rot = 0;
+    e->members = nullptr;
+    e->list = nullptr;
+    e->graph = nullptr;
+    e->exec = nullptr;
+    e->graph_nsub = 0;
+    e->host.assign(members, members + n);
+    e->dt.assign(dt, dt + n);
+    const int rc = ens_rebuild(e);
+    if (rc != SF_OK) {
+        (void)hipFree(e->members);
+        (void)hipFree(e->list);
+        delete e;
+        return rc;
+    }
+    *out = e;
+    return SF_OK;
+}
+
+extern "C" int sf_ens_destroy(sf_ens *e)
+{
+    if (!e) return SF_OK;
+    ens_drop_graph(e);
+    (void)hipFree(e->members);
+    (void)hipFree(e->list);
+    delete e;
+    return SF_OK;
+}
+
+extern "C" int sf_ens_configure(sf_ens *e, int32_t tile_j, int32_t order, int32_t grid)
+{
+    SF_REQUIRE(e, "sf_ens_configure: null ensemble");
+    SF_REQUIRE(tile_j == 4 || tile_j == 8 || tile_j == 16, "sf_ens_configure: tile height %d (4, 8 or 16)", tile_j);
+    SF_REQUIRE(order == 0 || order == 1, "sf_ens_configure: order %d (0 strided, 1 chunked)", order);
+    SF_REQUIRE(grid >= 0, "sf_ens_configure: grid %d", grid);
+    ens_drop_graph(e);
+    e->order = order;
+    e->grid = grid;
+    if (tile_j != e->tile_j) {
+        e->tile_j = tile_j;
+        return ens_rebuild(e);
+    }
+    return SF_OK;
+}
+
+extern "C" int sf_ens_num_tiles(const sf_ens *e) { return e ? e->ntiles : -1; }
+
+extern "C" int sf_ens_rotation(const sf_ens *e) { return e ? e->rot : -1; }
+
+extern "C" int sf_ens_substep(sf_ens *e, int32_t first, double cff1, void *stream)
+{
+    SF_REQUIRE(e, "sf_ens_substep: null ensemble");
+    e->rot = (e->rot + 1) % 3;   // cycleTimes2D of every member
+    if (e->precision == SF_FP64) ens_launch<double>(e, first, cff1, e->rot, sf::as_stream(stream));
+    else ens_launch<float>(e, first, cff1, e->rot, sf::as_stream(stream));
+    return sf::check_launch("sf_ens_substep");
+}
+
+extern "C" int sf_ens_capture(sf_ens *e, int32_t nsub, const double *cff1, void *stream)
+{
+    SF_REQUIRE(e && cff1, "sf_ens_capture: null argument");
+    SF_REQUIRE(nsub > 0 && nsub % 3 == 0, "sf_ens_capture: nsub %d must be a positive multiple of 3", nsub);
+    ens_drop_graph(e);
+    hipStream_t cs = nullptr;
+    hipError_t r = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (r != hipSuccess) return sf::hip_fail(r, "sf_ens_capture: stream");
+    r = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (r == hipSuccess) {
+        int rot = e->rot;
+        for (int k = 0; k < nsub; k++) {
+            rot = (rot + 1) % 3;
+            if (e->precision == SF_FP64) ens_launch<double>(e, 0, cff1[k], rot, cs);
+            else ens_launch<float>(e, 0, cff1[k], rot, cs);
+        }
+        r = hipStreamEndCapture(cs, &e->graph);
+    }
+    if (r == hipSuccess) r = hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    if (r != hipSuccess) {
+        ens_drop_graph(e);
+        return sf::hip_fail(r, "sf_ens_capture");
+    }
+    e->graph_nsub = nsub;
+    (void)stream;
+    return SF_OK;
+}
+
+extern "C" int sf_ens_replay(sf_ens *e, void *stream)
+{
+    SF_REQUIRE(e && e->exec, "sf_ens_replay: no captured graph");
+    hipError_t r = hipGraphLaunch(e->exec, sf::as_stream(stream));
+    if (r != hipSuccess) return sf::hip_fail(r, "sf_ens_replay");
+    // nsub is a multiple of 3: the rotation is unchanged
+    return SF_OK;
 }
