@@ -281,6 +281,37 @@ int sf_ens_substep(sf_ens *ens, int32_t first, double cff1, void *stream);
 int sf_ens_capture(sf_ens *ens, int32_t nsub, const double *cff1, void *stream);
 int sf_ens_replay(sf_ens *ens, void *stream);
 
+/* ---------------- learned subgrid momentum closure ---------------------------------- */
+
+/* The closure network (45 -> 64 GELU -> 64 GELU -> 2) trained and evaluated on matrix
+ * cores (BF16 operands, FP32 accumulation).  No reference counterpart (the north-star
+ * closure is new); it enters the physics substep as an extra momentum stress through
+ * the sustr/svstr fields (obsolete/step2d_LF_AM3.h.py:2595-2606 is where ROMS adds its
+ * surface stress).  Parameter vector layout: W1[48][64], b1[64], W2[64][64], b2[64],
+ * W3[64][16], b3[16] (padding rows/columns zero). */
+typedef struct sf_mlp sf_mlp;
+int sf_mlp_num_params(void);
+/* max_blocks: the largest number of workgroups of one training step. */
+int sf_mlp_create(int32_t max_blocks, sf_mlp **out);
+int sf_mlp_destroy(sf_mlp *mlp);
+int sf_mlp_set_params(sf_mlp *mlp, const float *host_params);
+int sf_mlp_get_params(sf_mlp *mlp, float *host_params);
+/* gradient and loss of the last sf_mlp_train_step (host copies) */
+int sf_mlp_get_grad(sf_mlp *mlp, float *host_grad, float *host_loss);
+int sf_mlp_set_norm(sf_mlp *mlp, const float *mu45, const float *sd45, const float *ynorm4);
+/* One step on the batch idx[0 .. nblocks*ntile*64) of X (BF16 bits [n][48]) and Y
+ * ([n][2]): forward + backward + deterministic gradient reduction, then Adam when
+ * adam != 0 (step counts from 1). */
+int sf_mlp_train_step(sf_mlp *mlp, const uint16_t *X, const float *Y, const int32_t *idx, int32_t nblocks,
+                      int32_t ntile, int32_t adam, float lr, float beta1, float beta2, float eps, int32_t step,
+                      void *stream);
+int sf_mlp_forward(sf_mlp *mlp, const uint16_t *X, int32_t n, float *Y, void *stream);
+/* Raw features [S2][45] of the t1 state of a physics-mode field set (f: Coriolis). */
+int sf_closure_features(const sf_phys_fields *fields, int32_t precision, const void *f, float *out, void *stream);
+/* sustr_eff = sustr + R_u * D_u, svstr_eff = svstr + R_v * D_v from the t1 state. */
+int sf_closure_stress(sf_mlp *mlp, const sf_phys_fields *fields, int32_t precision, const void *f,
+                      void *sustr_eff, void *svstr_eff, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,374 @@
+"""Learned subgrid momentum closure for the coarse physics-mode model.
+
+Pipeline (all state on the GPU; the network runs on the project's BF16-MFMA kernels
+in csrc/mlp.hip):
+
+1. fine runs of the basin1 recipe at 4x the coarse resolution (``fine_config``);
+2. ``coarse_grain``: 4x4 block means of zeta, face means of ubar/vbar and h, then
+   the closed-basin boundary rules of the coarse grid;
+3. samples: features = 3x3 stencils of (ubar, vbar, zeta, h, f) of the coarse-grained
+   state at t (sf_closure_features); targets = tendency residual
+   (CG(fine)(t + dt_c) - coarse_step(CG(fine)(t))) / dt_c of ubar and vbar, with the
+   coarse step a forward/backward Euler substep of the plain coarse model;
+4. training with Adam on normalised features/targets (sf_mlp_train_step);
+5. coarse runs with the closure: before every substep, sf_closure_stress turns the
+   network output R into the extra momentum stress R * D added to the wind stress of
+   that substep; the plain coarse scheme is the same run without it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _lib, configs
+from .model2d import build_model
+from .phys2d import PhysStepper, sf_phys_fields
+
+P = C.c_void_p
+NIN, IN, NOUT = 45, 48, 2
+
+
+def _bind(lib):
+    if getattr(lib, "_sf_mlp_bound", False):
+        return lib
+    f = C.c_float
+    lib.sf_mlp_num_params.restype = C.c_int32
+    sig = {
+        "sf_mlp_create": [C.c_int32, C.POINTER(P)],
+        "sf_mlp_destroy": [P],
+        "sf_mlp_set_params": [P, C.POINTER(f)],
+        "sf_mlp_get_params": [P, C.POINTER(f)],
+        "sf_mlp_get_grad": [P, C.POINTER(f), C.POINTER(f)],
+        "sf_mlp_set_norm": [P, C.POINTER(f), C.POINTER(f), C.POINTER(f)],
+        "sf_mlp_train_step": [P, P, P, P, C.c_int32, C.c_int32, C.c_int32, f, f, f, f, C.c_int32, P],
+        "sf_mlp_forward": [P, P, C.c_int32, P, P],
+        "sf_closure_features": [C.POINTER(sf_phys_fields), C.c_int32, P, P, P],
+        "sf_closure_stress": [P, C.POINTER(sf_phys_fields), C.c_int32, P, P, P, P],
+    }
+    for name, args in sig.items():
+        getattr(lib, name).argtypes = args
+        getattr(lib, name).restype = C.c_int32
+    lib._sf_mlp_bound = True
+    return lib
+
+
+def to_bf16_bits(x: torch.Tensor) -> torch.Tensor:
+    """FP32 -> BF16 (round to nearest even) as int16 bit patterns."""
+    return x.to(torch.float32).to(torch.bfloat16).view(torch.int16).contiguous()
+
+
+def _fp(a: np.ndarray):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class ClosureMLP:
+    """Device network: FP32 master parameters, Adam moments, BF16 copy, normalisation."""
+
+    def __init__(self, max_blocks: int = 1024, params: np.ndarray | None = None, seed: int = 7):
+        _lib.require_gpu()
+        self.lib = _bind(_lib.load())
+        self.nparam = int(self.lib.sf_mlp_num_params())
+        h = P()
+        _lib.check(self.lib.sf_mlp_create(int(max_blocks), C.byref(h)), "sf_mlp_create")
+        self.handle = h
+        self.max_blocks = int(max_blocks)
+        if params is None:
+            params = init_params(seed)
+        self.set_params(params)
+        self.mu = np.zeros(NIN, np.float32)
+        self.sd = np.ones(NIN, np.float32)
+        self.ynorm = np.array([0, 1, 0, 1], np.float32)
+        self.step_count = 0
+
+    def close(self):
+        if self.handle:
+            self.lib.sf_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: np.ndarray):
+        a, p = _fp(params)
+        assert a.size == self.nparam
+        _lib.check(self.lib.sf_mlp_set_params(self.handle, p), "sf_mlp_set_params")
+        self.step_count = 0
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.nparam, np.float32)
+        _lib.check(self.lib.sf_mlp_get_params(self.handle, out.ctypes.data_as(C.POINTER(C.c_float))),
+                   "sf_mlp_get_params")
+        return out
+
+    def get_grad(self):
+        g = np.zeros(self.nparam, np.float32)
+        loss = np.zeros(1, np.float32)
+        _lib.check(self.lib.sf_mlp_get_grad(self.handle, g.ctypes.data_as(C.POINTER(C.c_float)),
+                                            loss.ctypes.data_as(C.POINTER(C.c_float))), "sf_mlp_get_grad")
+        return g, float(loss[0])
+
+    def set_norm(self, mu, sd, ynorm):
+        self.mu, self.sd, self.ynorm = (np.asarray(mu, np.float32), np.asarray(sd, np.float32),
+                                        np.asarray(ynorm, np.float32))
+        a, pa = _fp(self.mu)
+        b, pb = _fp(self.sd)
+        c, pc = _fp(self.ynorm)
+        _lib.check(self.lib.sf_mlp_set_norm(self.handle, pa, pb, pc), "sf_mlp_set_norm")
+
+    def train_step(self, Xb: torch.Tensor, Y: torch.Tensor, idx: torch.Tensor, adam: bool = True, lr: float = 1e-3,
+                   beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8, ntile: int = 1, stream=None):
+        """One step on the samples idx (a multiple of 64 * ntile)."""
+        n = idx.numel()
+        assert n % (64 * ntile) == 0
+        nblocks = n // (64 * ntile)
+        if adam:
+            self.step_count += 1
+        rc = self.lib.sf_mlp_train_step(self.handle, Xb.data_ptr(), Y.data_ptr(), idx.data_ptr(), nblocks, ntile,
+                                        int(adam), lr, beta1, beta2, eps, max(self.step_count, 1),
+                                        _lib.stream_handle(stream))
+        _lib.check(rc, "sf_mlp_train_step")
+
+    def forward(self, Xb: torch.Tensor) -> torch.Tensor:
+        n = Xb.shape[0]
+        Y = torch.empty((n, NOUT), dtype=torch.float32, device=Xb.device)
+        _lib.check(self.lib.sf_mlp_forward(self.handle, Xb.data_ptr(), n, Y.data_ptr(), None), "sf_mlp_forward")
+        return Y
+
+
+def init_params(seed: int = 7) -> np.ndarray:
+    """He-style initial parameters in the kernel layout (padding zero)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    W1 = np.zeros((IN, 64))
+    W1[:NIN] = rng.standard_normal((NIN, 64)) * math.sqrt(2.0 / NIN)
+    W2 = rng.standard_normal((64, 64)) * math.sqrt(2.0 / 64)
+    W3 = np.zeros((64, 16))
+    W3[:, :NOUT] = rng.standard_normal((64, NOUT)) * math.sqrt(1.0 / 64)
+    return np.concatenate([W1.ravel(), np.zeros(64), W2.ravel(), np.zeros(64), W3.ravel(), np.zeros(16)])
+
+
+# ------------------------------------------------------------------------------------
+# grids and coarse-graining
+# ------------------------------------------------------------------------------------
+def fine_config(coarse_szI: int = 512, coarse_szJ: int = 256, ratio: int = 4, dx_coarse: float = 1250.0,
+                amp: float = 0.1, bump: float = 0.1, center=(0.5, 0.5), dt_coarse: float = 5.0) -> dict:
+    """The basin1 recipe at ``ratio`` times the coarse resolution: the fine interior is
+    exactly ``ratio`` x the coarse interior (szI_f = ratio*(szI_c-2) + 2)."""
+    szI = ratio * (coarse_szI - 2) + 2
+    szJ = ratio * (coarse_szJ - 2) + 2
+    dx = dx_coarse / ratio
+    cfg = configs.basin1(szI, szJ, dx=dx)
+    X, Y, Lx, Ly = configs._coords(szI, szJ, dx, dx)
+    cfg["sustr"] = amp * np.sin(np.pi * Y / Ly) / configs.RHO0
+    r2 = (X - center[0] * Lx) ** 2 + (Y - center[1] * Ly) ** 2
+    cfg["zeta0"] = bump * np.exp(-r2 / (2.0 * 50e3 ** 2))
+    cfg["dtfast"] = dt_coarse / ratio
+    cfg["ratio"] = ratio
+    return cfg
+
+
+def _closed_bc(z, u, v):
+    """Closed-basin boundary values (free-slip ghosts, zero normal flow), in place,
+    in the order of the physics oracle's boundary routine."""
+    z[1:-1, 0] = z[1:-1, 1]
+    z[1:-1, -1] = z[1:-1, -2]
+    z[0, :] = z[1, :]
+    z[-1, :] = z[-2, :]
+    u[:, 0] = 0.0
+    u[:, 1] = 0.0
+    u[:, -1] = 0.0
+    u[0, 2:-1] = u[1, 2:-1]
+    u[-1, 2:-1] = u[-2, 2:-1]
+    v[0, :] = 0.0
+    v[1, :] = 0.0
+    v[-1, :] = 0.0
+    v[2:-1, 0] = v[2:-1, 1]
+    v[2:-1, -1] = v[2:-1, -2]
+
+
+def coarse_grain(z, u, v, r: int = 4):
+    """Fine (szJ_f, szI_f) fields -> coarse ((szJ_f-2)/r + 2, (szI_f-2)/r + 2):
+    zeta by r x r block means of the interior cells, ubar by means over the r fine u
+    points of each coarse west face, vbar over each coarse south face."""
+    Jf, If = z.shape
+    Jc, Ic = (Jf - 2) // r + 2, (If - 2) // r + 2
+    zc = torch.zeros((Jc, Ic), dtype=z.dtype, device=z.device)
+    uc = torch.zeros_like(zc)
+    vc = torch.zeros_like(zc)
+    zi = z[1:-1, 1:-1].reshape(Jc - 2, r, Ic - 2, r)
+    zc[1:-1, 1:-1] = zi.mean(dim=(1, 3))
+    # u faces: fine column 1 + r*(I-1), rows of the coarse cell; coarse columns 1 .. Ic-1
+    uf = u[1:-1, 1::r][:, :Ic - 1]                       # (Jf-2, Ic-1)
+    uc[1:-1, 1:] = uf.reshape(Jc - 2, r, Ic - 1).mean(dim=1)
+    vf = v[1::r, 1:-1][:Jc - 1, :]                       # (Jc-1, If-2)
+    vc[1:, 1:-1] = vf.reshape(Jc - 1, Ic - 2, r).mean(dim=2)
+    _closed_bc(zc, uc, vc)
+    return zc, uc, vc
+
+
+def coarse_config(fcfg: dict) -> dict:
+    """Coarse grid of a fine configuration: block-mean bathymetry, face-mean wind."""
+    r = fcfg["ratio"]
+    hf = torch.from_numpy(fcfg["h"])
+    Jf, If = hf.shape
+    Jc, Ic = (Jf - 2) // r + 2, (If - 2) // r + 2
+    hc = torch.zeros((Jc, Ic), dtype=torch.float64)
+    hc[1:-1, 1:-1] = hf[1:-1, 1:-1].reshape(Jc - 2, r, Ic - 2, r).mean(dim=(1, 3))
+    hc[1:-1, 0] = hc[1:-1, 1]
+    hc[1:-1, -1] = hc[1:-1, -2]
+    hc[0, :] = hc[1, :]
+    hc[-1, :] = hc[-2, :]
+    sf = torch.from_numpy(fcfg["sustr"])
+    sc = torch.zeros_like(hc)
+    sc[1:-1, 1:] = sf[1:-1, 1::r][:, :Ic - 1].reshape(Jc - 2, r, Ic - 1).mean(dim=1)
+    sc[0, :] = sc[1, :]
+    sc[-1, :] = sc[-2, :]
+    dx = fcfg["dx"] * r
+    cfg = dict(name="coarse", szI=Ic, szJ=Jc, dx=dx, dy=dx, h=hc.numpy(), f=np.full((Jc, Ic), float(fcfg["f"].flat[0])),
+               pm=np.full((Jc, Ic), 1.0 / dx), pn=np.full((Jc, Ic), 1.0 / dx), sustr=sc.numpy(),
+               svstr=np.zeros((Jc, Ic)), dtfast=fcfg["dtfast"] * r, rdrg=fcfg["rdrg"])
+    cfg["zeta0"] = np.zeros((Jc, Ic))
+    return cfg
+
+
+# ------------------------------------------------------------------------------------
+# models
+# ------------------------------------------------------------------------------------
+class Model:
+    """A physics-mode basin: CompTimes, GRID, OCEAN, fused stepper, Coriolis field."""
+
+    def __init__(self, cfg: dict, dtype=torch.float64):
+        self.cfg = cfg
+        self.ct, self.GRID, self.OCEAN, _B = build_model(cfg, dtype=dtype, ndtfast=30)
+        self.stepper = PhysStepper(self.GRID, self.OCEAN, rdrg=cfg.get("rdrg", 3e-4), fused=True)
+        self.dtype = dtype
+        self.shape = (self.GRID.szJ, self.GRID.szI)
+        self.f = torch.from_numpy(np.ascontiguousarray(cfg["f"], dtype=np.float64).ravel()).to("cuda", dtype)
+        self.n = 0
+
+    def set_state(self, z, u, v):
+        """All three time slots := (z, u, v); rhs history and averages cleared; the next
+        substep is an Euler start."""
+        for k in range(3):
+            self.OCEAN.zeta[k].copy_(z.reshape(self.shape))
+            self.OCEAN.ubar[k].copy_(u.reshape(self.shape))
+            self.OCEAN.vbar[k].copy_(v.reshape(self.shape))
+        for name in ("rzeta", "rubar", "rvbar"):
+            getattr(self.OCEAN, name).zero_()
+        self.n = 0
+
+    def state(self):
+        O = self.OCEAN
+        return (O.zeta_t2.reshape(self.shape), O.ubar_t2.reshape(self.shape), O.vbar_t2.reshape(self.shape))
+
+    def step(self, closure: "ClosureMLP | None" = None, su=None, sv=None):
+        self.ct.next2DTimeStep()
+        self.OCEAN.cycleTimes2D()
+        st = self.stepper
+        a = st.fields(self.OCEAN)
+        if closure is not None:
+            rc = closure.lib.sf_closure_stress(closure.handle, C.byref(a), st.precision, self.f.data_ptr(),
+                                               su.data_ptr(), sv.data_ptr(), None)
+            _lib.check(rc, "sf_closure_stress")
+            a.sustr, a.svstr = su.data_ptr(), sv.data_ptr()
+        rc = st.lib.sf_phys_substep_fused(C.byref(a), st.precision, float(self.ct.dtfast), int(self.n == 0),
+                                          self.ct.cff1(), st.tile_j, None)
+        _lib.check(rc, "sf_phys_substep_fused")
+        self.n += 1
+
+    def features(self) -> torch.Tensor:
+        """Raw features [S2, 45] of the current t1 state (call after cycleTimes2D)."""
+        st = self.stepper
+        S2 = self.shape[0] * self.shape[1]
+        out = torch.empty((S2, NIN), dtype=torch.float32, device="cuda")
+        lib = _bind(st.lib)
+        a = st.fields(self.OCEAN)
+        _lib.check(lib.sf_closure_features(C.byref(a), st.precision, self.f.data_ptr(), out.data_ptr(), None),
+                   "sf_closure_features")
+        return out
+
+
+def closure_buffers(model: Model):
+    S2 = model.shape[0] * model.shape[1]
+    return (torch.empty(S2, dtype=model.dtype, device="cuda"), torch.empty(S2, dtype=model.dtype, device="cuda"))
+
+
+def sample_mask(shape) -> torch.Tensor:
+    """Points where both the u and the v target exist: 2 <= i <= L-1, 2 <= j <= M-1."""
+    J, I = shape
+    m = torch.zeros(shape, dtype=torch.bool, device="cuda")
+    m[2:J - 1, 2:I - 1] = True
+    return m.reshape(-1)
+
+
+def make_samples(fine: Model, coarse: Model, nsnap: int, spacing: int, r: int = 4):
+    """Run the fine model and return (features [n,45], targets [n,2]) from nsnap
+    snapshots ``spacing`` coarse steps apart (targets: tendency residuals in m/s^2)."""
+    feats, targs = [], []
+    mask = sample_mask(coarse.shape)
+    dtc = coarse.ct.dtfast
+    for s in range(nsnap):
+        for _ in range(spacing * r - r):
+            fine.step()
+        z0, u0, v0 = (x.clone() for x in fine.state())
+        for _ in range(r):
+            fine.step()
+        z1, u1, v1 = fine.state()
+        zc0, uc0, vc0 = coarse_grain(z0, u0, v0, r)
+        zc1, uc1, vc1 = coarse_grain(z1, u1, v1, r)
+        coarse.set_state(zc0, uc0, vc0)
+        F = coarse.features()          # every time slot holds CG(t)
+        coarse.step()                  # one plain Euler substep from CG(t)
+        zp, up, vp = coarse.state()
+        Ru = ((uc1 - up) / dtc).reshape(-1)
+        Rv = ((vc1 - vp) / dtc).reshape(-1)
+        feats.append(F[mask])
+        targs.append(torch.stack([Ru[mask], Rv[mask]], dim=1).float())
+    return torch.cat(feats), torch.cat(targs)
+
+
+def normalise(F: torch.Tensor, Y: torch.Tensor):
+    mu = F.mean(0)
+    sd = F.std(0)
+    sd = torch.where(sd > 0, sd, torch.ones_like(sd))
+    ym = Y.mean(0)
+    ys = Y.std(0)
+    ys = torch.where(ys > 0, ys, torch.ones_like(ys))
+    return mu, sd, ym, ys
+
+
+def pack_inputs(F: torch.Tensor, mu, sd) -> torch.Tensor:
+    """Normalised BF16 inputs [n, 48] (padding columns zero)."""
+    n = F.shape[0]
+    Xn = torch.zeros((n, IN), dtype=torch.float32, device=F.device)
+    Xn[:, :NIN] = (F - mu) / sd
+    return to_bf16_bits(Xn)
+
+
+def train(mlp: ClosureMLP, Xb: torch.Tensor, Yn: torch.Tensor, epochs: int, batch: int = 4096, lr: float = 1e-3,
+          seed: int = 6, log_every: int = 0):
+    """Adam over shuffled mini-batches (seeded); returns the per-epoch mean loss."""
+    n = Xb.shape[0]
+    nb = n // batch
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    losses = []
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g).to(torch.int32).cuda()
+        acc = 0.0
+        for b in range(nb):
+            idx = perm[b * batch:(b + 1) * batch]
+            mlp.train_step(Xb, Yn, idx, adam=True, lr=lr)
+            if log_every and b % log_every == 0:
+                acc += mlp.get_grad()[1]
+        losses.append(mlp.get_grad()[1])
+    return losses
+
+
+def rmse(a: torch.Tensor, b: torch.Tensor) -> float:
+    return float(torch.sqrt(torch.mean((a[1:-1, 1:-1].double() - b[1:-1, 1:-1].double()) ** 2)))

@@ -1,0 +1,199 @@
+"""GPU checks of the closure network kernels (csrc/mlp.hip) against the FP64 CPU MLP
+(oracle/mlp_ref.py): forward and gradient with the kernels' BF16 operand rounding
+emulated in FP64 (tight: pins the MFMA lane layout and the backward algebra), the
+same against the unrounded FP64 network (loose: BF16 training fidelity), one Adam
+step, a training run that must reduce the loss, and the closure feature/stress
+kernels against numpy."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mlp_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(seed=3):
+    th = R.init_params(seed)
+    p = R.unpack(th)
+    rng = np.random.default_rng(seed)
+    p["b1"] = 0.1 * rng.standard_normal(64)
+    p["b2"] = 0.1 * rng.standard_normal(64)
+    p["b3"][:2] = 0.1 * rng.standard_normal(2)
+    return R.pack(p).astype(np.float32).astype(np.float64)
+
+
+def _data(n, seed=4):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((n, R.IN))
+    X[:, :R.NIN] = rng.standard_normal((n, R.NIN))
+    Y = rng.standard_normal((n, R.NOUT))
+    return R.bf16_round(X.astype(np.float32)), Y.astype(np.float32).astype(np.float64)
+
+
+def _dev(X, Y):
+    import torch
+    from sailfish_amd.closure import to_bf16_bits
+    Xb = to_bf16_bits(torch.from_numpy(X).float().cuda())
+    Yd = torch.from_numpy(Y).float().cuda().contiguous()
+    return Xb, Yd
+
+
+def test_forward_matches_emulated_cpu():
+    from sailfish_amd.closure import ClosureMLP
+    print("[case] mlp forward", file=sys.stderr, flush=True)
+    th = _params()
+    X, Y = _data(300)
+    Xb, _ = _dev(X, Y)
+    mlp = ClosureMLP(params=th)
+    out = mlp.forward(Xb).cpu().numpy().astype(np.float64)
+    ref, _ = R.forward_bf16(th, X)
+    scale = np.abs(ref).max()
+    e = np.abs(out - ref) / scale
+    err, p99 = e.max(), np.percentile(e, 99)
+    print(f"forward rel err max {err:.2e} p99 {p99:.2e}", file=sys.stderr)
+    # FP32 GELU may put a hidden activation on the other side of a BF16 rounding
+    # midpoint than the FP64 emulation (one BF16 ulp on that activation): the bound on
+    # the maximum allows a few such flips, the percentile bound does not
+    assert err < 2e-3 and p99 < 1e-4
+    full, _ = R.forward(th, X)
+    assert np.abs(out - full).max() / scale < 3e-2
+    mlp.close()
+
+
+@pytest.mark.parametrize("ntile,nblk", [(1, 4), (3, 2)])
+def test_gradient_matches_emulated_cpu(ntile, nblk):
+    import torch
+    from sailfish_amd.closure import ClosureMLP
+    print(f"[case] mlp grad ntile={ntile} nblk={nblk}", file=sys.stderr, flush=True)
+    th = _params(5)
+    n = 64 * ntile * nblk
+    X, Y = _data(n + 50, seed=8)
+    Xb, Yd = _dev(X, Y)
+    rng = np.random.default_rng(1)
+    idx_np = rng.permutation(n + 50)[:n].astype(np.int32)
+    idx = torch.from_numpy(idx_np).cuda()
+    mlp = ClosureMLP(params=th)
+    mlp.train_step(Xb, Yd, idx, adam=False, ntile=ntile)
+    g, loss = mlp.get_grad()
+    ge, le = R.grad_bf16(th, X[idx_np], Y[idx_np])
+    gf, lf = R.grad(th, X[idx_np], Y[idx_np])
+    G, GE, GF = R.unpack(g.astype(np.float64)), R.unpack(ge), R.unpack(gf)
+    for name in GE:
+        e = np.linalg.norm(G[name] - GE[name]) / max(np.linalg.norm(GE[name]), 1e-30)
+        f = np.linalg.norm(G[name] - GF[name]) / max(np.linalg.norm(GF[name]), 1e-30)
+        print(f"grad {name}: rel err vs emulated {e:.2e}, vs fp64 {f:.2e}", file=sys.stderr)
+        assert e < 1e-3, name
+        assert f < 5e-2, name
+    assert abs(loss - le) / le < 1e-4
+    mlp.close()
+
+
+def test_adam_step_matches_cpu():
+    import torch
+    from sailfish_amd.closure import ClosureMLP
+    print("[case] mlp adam", file=sys.stderr, flush=True)
+    th = _params(6)
+    X, Y = _data(256, seed=2)
+    Xb, Yd = _dev(X, Y)
+    idx = torch.arange(256, dtype=torch.int32).cuda()
+    mlp = ClosureMLP(params=th)
+    mlp.train_step(Xb, Yd, idx, adam=False)
+    g, _ = mlp.get_grad()
+    mlp.train_step(Xb, Yd, idx, adam=True, lr=1e-3)
+    new = mlp.get_params().astype(np.float64)
+    ref, _, _ = R.adam(th, g.astype(np.float64), np.zeros_like(th), np.zeros_like(th), 1, lr=1e-3)
+    assert np.abs(new - ref).max() < 1e-6
+    mlp.close()
+
+
+def test_training_reduces_loss():
+    import torch
+    from sailfish_amd.closure import ClosureMLP, to_bf16_bits, train
+    print("[case] mlp training", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(0)
+    n = 64 * 256
+    F = rng.standard_normal((n, R.NIN)).astype(np.float32)
+    Yt = np.stack([np.tanh(F[:, 0] + 0.5 * F[:, 3] * F[:, 7]), np.sin(F[:, 11]) - 0.3 * F[:, 20]], 1)
+    X = np.zeros((n, R.IN), np.float32)
+    X[:, :R.NIN] = F
+    Xb = to_bf16_bits(torch.from_numpy(X).cuda())
+    Yd = torch.from_numpy(Yt.astype(np.float32)).cuda().contiguous()
+    mlp = ClosureMLP(seed=3)
+    losses = train(mlp, Xb, Yd, epochs=8, batch=2048, lr=3e-3)
+    print("losses", losses, file=sys.stderr)
+    assert losses[-1] < 0.25 * losses[0]
+    mlp.close()
+
+
+def _coarse_model():
+    import torch
+    from sailfish_amd import configs
+    from sailfish_amd.closure import Model
+    cfg = configs.random_smooth(37, 21, seed=5)
+    m = Model(cfg, dtype=torch.float64)
+    return m
+
+
+def test_closure_features_match_numpy():
+    import torch
+    print("[case] closure features", file=sys.stderr, flush=True)
+    m = _coarse_model()
+    F = m.features().cpu().numpy()
+    J, I = m.shape
+    O = m.OCEAN
+    flds = [O.ubar_t1, O.vbar_t1, O.zeta_t1, torch.from_numpy(m.GRID.host["h"].ravel()).cuda(), m.f]
+    flds = [x.reshape(J, I).double().cpu().numpy() for x in flds]
+    ref = np.zeros((J * I, 45), np.float32)
+    for j in range(1, J - 1):
+        for i in range(1, I - 1):
+            k = 0
+            for a in flds:
+                for dj in (-1, 0, 1):
+                    for di in (-1, 0, 1):
+                        ref[j * I + i, k] = np.float32(a[j + dj, i + di])
+                        k += 1
+    assert np.array_equal(F, ref)
+
+
+def test_closure_stress_constant_output():
+    """Network with zero last layer and bias (c_u, c_v): the closure stress is exactly
+    sustr + c_u * D_u at interior u points (likewise v), the stress elsewhere."""
+    import ctypes as C
+    import torch
+    from sailfish_amd import _lib
+    from sailfish_amd.closure import ClosureMLP, closure_buffers
+    print("[case] closure stress", file=sys.stderr, flush=True)
+    m = _coarse_model()
+    th = R.unpack(_params(2))
+    th["W3"][:] = 0.0
+    th["b3"][:] = 0.0
+    th["b3"][0], th["b3"][1] = 0.25, -0.5
+    mlp = ClosureMLP(params=R.pack(th))
+    mlp.set_norm(np.zeros(45), np.ones(45), np.array([1e-6, 2.0, -1e-6, 4.0]))
+    su, sv = closure_buffers(m)
+    a = m.stepper.fields(m.OCEAN)
+    rc = mlp.lib.sf_closure_stress(mlp.handle, C.byref(a), m.stepper.precision, m.f.data_ptr(), su.data_ptr(),
+                                   sv.data_ptr(), None)
+    _lib.check(rc, "sf_closure_stress")
+    torch.cuda.synchronize()
+    J, I = m.shape
+    h = m.GRID.host["h"]
+    z = m.OCEAN.zeta_t1.reshape(J, I).cpu().numpy()
+    s0 = m.stepper.static["sustr"].reshape(J, I).cpu().numpy()
+    t0 = m.stepper.static["svstr"].reshape(J, I).cpu().numpy()
+    Ru = np.float32(np.float32(0.25) * np.float32(2.0) + np.float32(1e-6))
+    Rv = np.float32(np.float32(-0.5) * np.float32(4.0) + np.float32(-1e-6))
+    eu, ev = s0.copy(), t0.copy()
+    for j in range(1, J - 1):
+        for i in range(2, I - 1):
+            eu[j, i] = s0[j, i] + float(Ru) * (0.5 * ((h[j, i - 1] + z[j, i - 1]) + (h[j, i] + z[j, i])))
+    for j in range(2, J - 1):
+        for i in range(1, I - 1):
+            ev[j, i] = t0[j, i] + float(Rv) * (0.5 * ((h[j - 1, i] + z[j - 1, i]) + (h[j, i] + z[j, i])))
+    assert np.array_equal(su.reshape(J, I).cpu().numpy(), eu)
+    assert np.array_equal(sv.reshape(J, I).cpu().numpy(), ev)
+    mlp.close()
