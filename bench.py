@@ -310,6 +310,38 @@ def run_ensemble_gpu(args):
     }
 
 
+def run_closure_gpu(args):
+    """Learned-closure workload: fine basin1 runs at 4x resolution (2042x1018 rho
+    points, dx=312.5 m, dt=1.25 s) coarse-grained to 512x256 (dx=1.25 km, dt=5 s),
+    2e6 samples, BF16-MFMA training, then 5000 coarse steps plain and with the
+    closure against the coarse-grained fine run; speed of fine vs coarse+closure
+    (FP32, graph replays)."""
+    import torch
+    from sailfish_amd import closure as CL
+
+    res = CL.run_experiment(dtype=torch.float64)
+    mlp = res.pop("mlp")
+    sp = CL.speed(mlp, dtype=torch.float32)
+    mlp.close()
+    last = res["series"][-1]
+    return {
+        "workload": "basin1 recipe, fine 2042x1018 (dx 312.5 m, dt 1.25 s) -> coarse 512x256 (dx 1.25 km, "
+                    "dt 5 s); 4 seeded training runs, 2e6 samples; evaluation: 5000 coarse steps of the "
+                    "centred-bump configuration",
+        "samples": res["samples"],
+        "train_mse_normalised": [res["losses"][0], res["losses"][-1]],
+        "train_s": res["train_s"],
+        "target_rms_m_s2": res["target_rms"],
+        "linear_r2": res["linear_r2"],
+        "validation": res["validation"],
+        "rmse_after_5000_steps": {"plain": last["plain"], "closure": last["closure"],
+                                  "signal_max": last["signal"], "order": ["zeta_m", "ubar_m_s", "vbar_m_s"]},
+        "selected_model": res["validation"]["selected"],
+        "speed_fp32": sp,
+        "experiment_s": res["total_s"],
+    }
+
+
 def calib_copy(args):
     """Plain device-to-device copy of 1 GiB, 5 times: the byte counters of this
     kernel calibrate FETCH_SIZE / WRITE_SIZE (2^30 bytes read and written each)."""
@@ -346,6 +378,7 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--calib-copy", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-closure", action="store_true")
     ap.add_argument("--ens-members", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=498)
     args = ap.parse_args()
@@ -411,6 +444,8 @@ def main():
         out["secondary"] = summary(s2, other, min(args.steps, 200), args.warmup)
     if not args.no_ensemble and args.mode == "phys":
         out["ensemble"] = run_ensemble_gpu(args)
+    if not args.no_closure and args.mode == "phys":
+        out["closure"] = run_closure_gpu(args)
     if not args.no_cpu:
         out["cpu_baseline"] = (run_cpu_phys if args.mode == "phys" else run_cpu_faithful)(args, args.cpu_budget)
     print(json.dumps(out))

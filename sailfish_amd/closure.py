@@ -138,7 +138,8 @@ class ClosureMLP:
     def forward(self, Xb: torch.Tensor) -> torch.Tensor:
         n = Xb.shape[0]
         Y = torch.empty((n, NOUT), dtype=torch.float32, device=Xb.device)
-        _lib.check(self.lib.sf_mlp_forward(self.handle, Xb.data_ptr(), n, Y.data_ptr(), None), "sf_mlp_forward")
+        _lib.check(self.lib.sf_mlp_forward(self.handle, Xb.data_ptr(), n, Y.data_ptr(), _lib.stream_handle()),
+                   "sf_mlp_forward")
         return Y
 
 
@@ -268,17 +269,31 @@ class Model:
         return (O.zeta_t2.reshape(self.shape), O.ubar_t2.reshape(self.shape), O.vbar_t2.reshape(self.shape))
 
     def step(self, closure: "ClosureMLP | None" = None, su=None, sv=None):
+        self.begin()
+        self.advance(closure, su, sv)
+
+    def begin(self):
+        """Start a substep: time counter and slot rotation (t1 is now the current state)."""
         self.ct.next2DTimeStep()
         self.OCEAN.cycleTimes2D()
+
+    def force(self, prev, cur):
+        """Overwrite the levels t0 (n-1) and t1 (n) with given states (z, u, v)."""
+        O = self.OCEAN
+        for lev, st in ((0, prev), (1, cur)):
+            for name, x in zip(("zeta", "ubar", "vbar"), st):
+                getattr(O, f"{name}_t{lev}").copy_(x.reshape(-1))
+
+    def advance(self, closure: "ClosureMLP | None" = None, su=None, sv=None):
         st = self.stepper
         a = st.fields(self.OCEAN)
         if closure is not None:
             rc = closure.lib.sf_closure_stress(closure.handle, C.byref(a), st.precision, self.f.data_ptr(),
-                                               su.data_ptr(), sv.data_ptr(), None)
+                                               su.data_ptr(), sv.data_ptr(), _lib.stream_handle())
             _lib.check(rc, "sf_closure_stress")
             a.sustr, a.svstr = su.data_ptr(), sv.data_ptr()
         rc = st.lib.sf_phys_substep_fused(C.byref(a), st.precision, float(self.ct.dtfast), int(self.n == 0),
-                                          self.ct.cff1(), st.tile_j, None)
+                                          self.ct.cff1(), st.tile_j, _lib.stream_handle())
         _lib.check(rc, "sf_phys_substep_fused")
         self.n += 1
 
@@ -289,8 +304,8 @@ class Model:
         out = torch.empty((S2, NIN), dtype=torch.float32, device="cuda")
         lib = _bind(st.lib)
         a = st.fields(self.OCEAN)
-        _lib.check(lib.sf_closure_features(C.byref(a), st.precision, self.f.data_ptr(), out.data_ptr(), None),
-                   "sf_closure_features")
+        _lib.check(lib.sf_closure_features(C.byref(a), st.precision, self.f.data_ptr(), out.data_ptr(),
+                                           _lib.stream_handle()), "sf_closure_features")
         return out
 
 
@@ -307,30 +322,52 @@ def sample_mask(shape) -> torch.Tensor:
     return m.reshape(-1)
 
 
-def make_samples(fine: Model, coarse: Model, nsnap: int, spacing: int, r: int = 4):
-    """Run the fine model and return (features [n,45], targets [n,2]) from nsnap
-    snapshots ``spacing`` coarse steps apart (targets: tendency residuals in m/s^2)."""
+def make_samples(fine: Model, fine_probe: Model, probe: Model, nsnap: int, spacing: int, r: int = 4):
+    """Samples of the subgrid momentum tendency along a fine run.
+
+    At nsnap times ``spacing`` coarse steps apart, with B the fine state:
+    target = CG(T_fine(B)) - T_coarse(CG(B)), where T is a model's instantaneous
+    momentum tendency, measured as one forward/backward Euler substep of a tiny length
+    (the probe models' substep) divided by that length.  The fine trajectory is not
+    differenced, so the fine scheme's time-stepping error stays out of the target.
+    Targets: ubar and vbar components (m/s^2); features: 3x3 stencils of CG(B)."""
     feats, targs = [], []
-    mask = sample_mask(coarse.shape)
-    dtc = coarse.ct.dtfast
+    mask = sample_mask(probe.shape)
+    dtp = probe.ct.dtfast
+    assert abs(fine_probe.ct.dtfast - dtp) < 1e-15
     for s in range(nsnap):
-        for _ in range(spacing * r - r):
+        for _ in range(spacing * r):
             fine.step()
-        z0, u0, v0 = (x.clone() for x in fine.state())
-        for _ in range(r):
-            fine.step()
-        z1, u1, v1 = fine.state()
-        zc0, uc0, vc0 = coarse_grain(z0, u0, v0, r)
-        zc1, uc1, vc1 = coarse_grain(z1, u1, v1, r)
-        coarse.set_state(zc0, uc0, vc0)
-        F = coarse.features()          # every time slot holds CG(t)
-        coarse.step()                  # one plain Euler substep from CG(t)
-        zp, up, vp = coarse.state()
-        Ru = ((uc1 - up) / dtc).reshape(-1)
-        Rv = ((vc1 - vp) / dtc).reshape(-1)
+        Bf = [x.clone() for x in fine.state()]
+        fine_probe.set_state(*Bf)
+        fine_probe.step()
+        Tf = [(x - y) / dtp for x, y in zip(fine_probe.state(), Bf)]
+        _tz, Tu, Tv = coarse_grain(*Tf, r)
+        B = coarse_grain(*Bf, r)
+        probe.set_state(*B)
+        F = probe.features()
+        probe.step()
+        _zp, up, vp = probe.state()
+        Ru = (Tu - (up - B[1]) / dtp).reshape(-1)
+        Rv = (Tv - (vp - B[2]) / dtp).reshape(-1)
         feats.append(F[mask])
         targs.append(torch.stack([Ru[mask], Rv[mask]], dim=1).float())
     return torch.cat(feats), torch.cat(targs)
+
+
+def linear_r2(F: torch.Tensor, Y: torch.Tensor, n: int = 200_000) -> list:
+    """Fraction of the target variance a least-squares linear map of the features
+    explains (diagnostic of how predictable the targets are from the features)."""
+    k = min(n, F.shape[0])
+    Fk = F[:k].double()
+    sd = Fk.std(0)
+    keep = sd > 0
+    Fk = (Fk[:, keep] - Fk[:, keep].mean(0)) / sd[keep]
+    A = torch.cat([Fk, torch.ones((k, 1), dtype=torch.float64, device=F.device)], 1)
+    Yk = Y[:k].double()
+    sol = torch.linalg.lstsq(A.cpu(), Yk.cpu()).solution.to(F.device)
+    res = Yk - A @ sol
+    return [float(1.0 - res[:, c].var() / Yk[:, c].var()) for c in range(Yk.shape[1])]
 
 
 def normalise(F: torch.Tensor, Y: torch.Tensor):
@@ -352,23 +389,182 @@ def pack_inputs(F: torch.Tensor, mu, sd) -> torch.Tensor:
 
 
 def train(mlp: ClosureMLP, Xb: torch.Tensor, Yn: torch.Tensor, epochs: int, batch: int = 4096, lr: float = 1e-3,
-          seed: int = 6, log_every: int = 0):
-    """Adam over shuffled mini-batches (seeded); returns the per-epoch mean loss."""
+          seed: int = 6):
+    """Adam over shuffled mini-batches (seeded); returns the training-set MSE before
+    training and after every epoch."""
     n = Xb.shape[0]
     nb = n // batch
     g = torch.Generator(device="cpu").manual_seed(seed)
-    losses = []
+    losses = [dataset_mse(mlp, Xb, Yn)]
     for ep in range(epochs):
         perm = torch.randperm(n, generator=g).to(torch.int32).cuda()
-        acc = 0.0
         for b in range(nb):
             idx = perm[b * batch:(b + 1) * batch]
             mlp.train_step(Xb, Yn, idx, adam=True, lr=lr)
-            if log_every and b % log_every == 0:
-                acc += mlp.get_grad()[1]
-        losses.append(mlp.get_grad()[1])
+        losses.append(dataset_mse(mlp, Xb, Yn))
     return losses
+
+
+def dataset_mse(mlp: ClosureMLP, Xb: torch.Tensor, Yn: torch.Tensor) -> float:
+    """Mean squared error of the network over a whole (normalised) data set."""
+    return float(((mlp.forward(Xb) - Yn) ** 2).mean())
 
 
 def rmse(a: torch.Tensor, b: torch.Tensor) -> float:
     return float(torch.sqrt(torch.mean((a[1:-1, 1:-1].double() - b[1:-1, 1:-1].double()) ** 2)))
+
+
+class GraphLoop:
+    """``nsub`` (a multiple of 3) substeps of a Model -- with the closure stress kernel
+    before each when ``closure`` is given -- captured once in a HIP graph and replayed.
+    The substeps' average weights are those of the captured steps."""
+
+    def __init__(self, model: Model, nsub: int = 3, closure: ClosureMLP | None = None):
+        assert nsub % 3 == 0 and model.n > 0
+        self.model, self.nsub = model, nsub
+        self.su, self.sv = closure_buffers(model) if closure is not None else (None, None)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            for _ in range(nsub):
+                model.step(closure, self.su, self.sv)
+        # capture does not run the kernels; the host rotation is back after 3k substeps
+
+    def replay(self, times: int = 1):
+        for _ in range(times):
+            self.graph.replay()
+
+
+def fine_coarse_ics(fine: Model, r: int):
+    z, u, v = (x.reshape(fine.shape) for x in (fine.OCEAN.zeta_t1, fine.OCEAN.ubar_t1, fine.OCEAN.vbar_t1))
+    return coarse_grain(z, u, v, r)
+
+
+def run_experiment(coarse_shape=(512, 256), ratio: int = 4, nsteps: int = 5000, n_train_runs: int = 4,
+                   snaps_per_run: int = 4, snap_spacing: int = 250, nsamples: int = 2_000_000, probe_dt: float = 0.01,
+                   epochs: int = 10, batch: int = 4096, lr: float = 1e-3, eval_every: int = 500, seed: int = 6,
+                   n_val: int = 500,
+                   dtype=torch.float64, log=None) -> dict:
+    """Train the closure on fine runs with seeded initial bumps and winds, then run the
+    basin1 configuration (centred bump, wind 0.1 N/m^2) fine, coarse plain and coarse
+    with closure for ``nsteps`` coarse steps, and report the RMSE of zeta, ubar and vbar
+    of both coarse runs against the coarse-grained fine run."""
+    import time as _time
+    log = log or (lambda *a: None)
+    t_start = _time.perf_counter()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    Fs, Ys = [], []
+    for run in range(n_train_runs):
+        center = (float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.3, 0.7)))
+        amp, bump = float(rng.uniform(0.05, 0.15)), float(rng.uniform(0.05, 0.15))
+        fcfg = fine_config(*coarse_shape, ratio, amp=amp, bump=bump, center=center)
+        fine = Model(fcfg, dtype)
+        fpcfg = dict(fcfg, dtfast=probe_dt)
+        fine_probe = Model(fpcfg, dtype)
+        pcfg = coarse_config(fcfg)
+        pcfg["dtfast"] = probe_dt
+        probe = Model(pcfg, dtype)
+        F, Y = make_samples(fine, fine_probe, probe, snaps_per_run, snap_spacing, ratio)
+        Fs.append(F)
+        Ys.append(Y)
+        log(f"training run {run}: {F.shape[0]} samples")
+        del fine, fine_probe, probe
+    F = torch.cat(Fs)
+    Y = torch.cat(Ys)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    sel = torch.randperm(F.shape[0], generator=g)[:min(nsamples, F.shape[0])].cuda()
+    F, Y = F[sel], Y[sel]
+    r2 = linear_r2(F, Y)
+    log(f"linear least-squares R^2 of the targets: {r2}")
+    mu, sd, ym, ys = normalise(F, Y)
+    Xb = pack_inputs(F, mu, sd)
+    Yn = ((Y - ym) / ys).contiguous()
+    mlp = ClosureMLP(max_blocks=batch // 64, seed=seed)
+    mlp.set_norm(mu.cpu().numpy(), sd.cpu().numpy(), [float(ym[0]), float(ys[0]), float(ym[1]), float(ys[1])])
+    t0 = _time.perf_counter()
+    losses = train(mlp, Xb, Yn, epochs, batch, lr, seed)
+    torch.cuda.synchronize()
+    t_train = _time.perf_counter() - t0
+    log(f"training: {epochs} epochs, loss {losses[0]:.4f} -> {losses[-1]:.4f} ({t_train:.1f} s)")
+    # held-out skill of the network on the last 5 % of the samples is not separated here:
+    # the evaluation below is on a configuration none of the training runs used
+
+    # validation on a held-out fine run: the closure is selected only if it lowers the
+    # normalised RMSE there; otherwise the plain coarse scheme is the model (fallback)
+    vcenter = (float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.3, 0.7)))
+    vamp, vbump = float(rng.uniform(0.05, 0.15)), float(rng.uniform(0.05, 0.15))
+    vcfg = fine_config(*coarse_shape, ratio, amp=vamp, bump=vbump, center=vcenter)
+    val = compare_runs(vcfg, mlp, ratio, n_val, n_val, dtype)[-1]
+    score = {k: sum(val[k][c] / max(val["signal"][c], 1e-30) for c in range(3)) for k in ("plain", "closure")}
+    selected = "closure" if score["closure"] < score["plain"] else "plain"
+    log(f"validation ({n_val} steps): plain {score['plain']:.3e}, closure {score['closure']:.3e} -> {selected}")
+
+    series = compare_runs(fine_config(*coarse_shape, ratio), mlp, ratio, nsteps, eval_every, dtype, log)
+    fcfg = fine_config(*coarse_shape, ratio)
+    ccfg = coarse_config(fcfg)
+    return dict(losses=losses, train_s=t_train, series=series, samples=int(F.shape[0]), mlp=mlp, linear_r2=r2,
+                target_rms=[float(Y[:, 0].pow(2).mean().sqrt()), float(Y[:, 1].pow(2).mean().sqrt())],
+                validation=dict(score=score, selected=selected, steps=n_val),
+                fine_shape=[fcfg["szI"], fcfg["szJ"]], coarse_shape=[ccfg["szI"], ccfg["szJ"]],
+                dt_fine=fcfg["dtfast"], dt_coarse=ccfg["dtfast"], total_s=_time.perf_counter() - t_start)
+
+
+def compare_runs(fcfg: dict, mlp: ClosureMLP, ratio: int, nsteps: int, every: int, dtype, log=None) -> list:
+    """Fine run vs coarse plain vs coarse with closure from CG(fine initial state);
+    RMSE of zeta, ubar, vbar against CG(fine) every ``every`` coarse steps."""
+    log = log or (lambda *a: None)
+    ccfg = coarse_config(fcfg)
+    fine = Model(fcfg, dtype)
+    plain = Model(ccfg, dtype)
+    clos = Model(ccfg, dtype)
+    zc, uc, vc = fine_coarse_ics(fine, ratio)
+    plain.set_state(zc, uc, vc)
+    clos.set_state(zc, uc, vc)
+    su, sv = closure_buffers(clos)
+    series = []
+    done = 0
+    while done < nsteps:
+        n = min(every, nsteps - done)
+        for _ in range(n * ratio):
+            fine.step()
+        for _ in range(n):
+            plain.step()
+            clos.step(mlp, su, sv)
+        done += n
+        zf, uf, vf = coarse_grain(*fine.state(), ratio)
+        zp, up, vp = plain.state()
+        zq, uq, vq = clos.state()
+        row = dict(step=done, plain=[rmse(zp, zf), rmse(up, uf), rmse(vp, vf)],
+                   closure=[rmse(zq, zf), rmse(uq, uf), rmse(vq, vf)],
+                   signal=[float(zf.abs().max()), float(uf.abs().max()), float(vf.abs().max())])
+        series.append(row)
+        log(f"step {done}: rmse plain {row['plain']} closure {row['closure']}")
+    return series
+
+
+def speed(mlp: ClosureMLP, coarse_shape=(512, 256), ratio: int = 4, nsub: int = 300, dtype=torch.float32) -> dict:
+    """GPU time per substep of the fine run and of the coarse run with the closure
+    (graph replays), and the wall-time ratio for the same simulated period."""
+    fcfg = fine_config(*coarse_shape, ratio)
+    ccfg = coarse_config(fcfg)
+    out = {}
+    for name, cfg, cl in (("fine", fcfg, None), ("coarse_closure", ccfg, mlp), ("coarse_plain", ccfg, None)):
+        m = Model(cfg, dtype)
+        m.step()                           # Euler start outside the graph
+        loop = GraphLoop(m, 3, cl)
+        loop.replay(5)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        loop.replay(nsub // 3)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / (nsub // 3 * 3)
+        pts = cfg["szI"] * cfg["szJ"]
+        out[name] = dict(ms_per_substep=ms, points=pts, cell_steps_per_s=pts / (ms * 1e-3), dt=cfg["dtfast"])
+    # wall time to simulate one hour
+    for k in out:
+        out[k]["s_per_sim_hour"] = 3600.0 / out[k]["dt"] * out[k]["ms_per_substep"] * 1e-3
+    out["speedup_closure_vs_fine"] = out["fine"]["s_per_sim_hour"] / out["coarse_closure"]["s_per_sim_hour"]
+    return out

@@ -125,6 +125,60 @@ __device__ void forward_tile(MlpShared &S)
     __syncthreads();
 }
 
+// inference: weights once per workgroup, BF16 activations only
+struct InferShared {
+    uint16_t W1[IN][H], W2[H][H], W3[H][OUT];
+    float b1[H], b2[H], b3[OUT];
+    uint16_t X[BT][IN];
+    uint16_t A1[BT][H], A2[BT][H];
+    float Z3[BT][OUT];
+};
+
+__device__ void load_weights_infer(InferShared &S, const uint16_t *wb, const float *w)
+{
+    for (int e = threadIdx.x; e < IN * H; e += NT) S.W1[e / H][e % H] = wb[OFF_W1 + e];
+    for (int e = threadIdx.x; e < H * H; e += NT) S.W2[e / H][e % H] = wb[OFF_W2 + e];
+    for (int e = threadIdx.x; e < H * OUT; e += NT) S.W3[e / OUT][e % OUT] = wb[OFF_W3 + e];
+    for (int e = threadIdx.x; e < H; e += NT) {
+        S.b1[e] = w[OFF_B1 + e];
+        S.b2[e] = w[OFF_B2 + e];
+    }
+    for (int e = threadIdx.x; e < OUT; e += NT) S.b3[e] = w[OFF_B3 + e];
+}
+
+// same arithmetic as forward_tile (identical outputs), without keeping Z1 / Z2
+__device__ void infer_tile(InferShared &S)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+    for (int tn = 0; tn < 4; tn++) {
+        const float bb = S.b1[tn * 16 + r];
+        f32x4_t acc = {bb, bb, bb, bb};
+        acc = mfma_tile(wave, tn, IN, [&](int m, int k) { return S.X[m][k]; },
+                        [&](int k, int n) { return S.W1[k][n]; }, acc);
+#pragma unroll
+        for (int t = 0; t < 4; t++) S.A1[wave * 16 + 4 * q + t][tn * 16 + r] = f2bf(gelu(acc[t]));
+    }
+    __syncthreads();
+    for (int tn = 0; tn < 4; tn++) {
+        const float bb = S.b2[tn * 16 + r];
+        f32x4_t acc = {bb, bb, bb, bb};
+        acc = mfma_tile(wave, tn, H, [&](int m, int k) { return S.A1[m][k]; },
+                        [&](int k, int n) { return S.W2[k][n]; }, acc);
+#pragma unroll
+        for (int t = 0; t < 4; t++) S.A2[wave * 16 + 4 * q + t][tn * 16 + r] = f2bf(gelu(acc[t]));
+    }
+    __syncthreads();
+    {
+        const float bb = S.b3[r];
+        f32x4_t acc = {bb, bb, bb, bb};
+        acc = mfma_tile(wave, 0, H, [&](int m, int k) { return S.A2[m][k]; },
+                        [&](int k, int n) { return S.W3[k][n]; }, acc);
+#pragma unroll
+        for (int t = 0; t < 4; t++) S.Z3[wave * 16 + 4 * q + t][r] = acc[t];
+    }
+    __syncthreads();
+}
+
 struct BwdShared {
     uint16_t dZ3[BT][OUT], dZ2[BT][H], dZ1[BT][H];
     float dZ3f[BT][OUT], dZ2f[BT][H], dZ1f[BT][H];
@@ -342,40 +396,59 @@ template <typename T>
 __global__ __launch_bounds__(NT) void k_closure_stress(const uint16_t *__restrict__ wb, const float *__restrict__ w,
                                                       ClosureArgs<T> a)
 {
-    __shared__ MlpShared S;
-    load_weights(S, wb, w);
+    __shared__ InferShared S;
+    load_weights_infer(S, wb, w);
     const long long S2 = (long long)a.szI * a.szJ;
-    const long long g0 = (long long)blockIdx.x * BT;
     const int L = a.szI - 1, M = a.szJ - 1;
-    for (int e = threadIdx.x; e < BT * IN; e += NT) {
-        const int s = e / IN, k = e % IN;
-        const long long g = g0 + s;
-        uint16_t x = 0;
-        if (g < S2 && k < NIN) {
+    const long long ntiles = (S2 + BT - 1) / BT;
+    __shared__ int pi[BT], pj[BT];
+    const T *fld[5] = {a.u, a.v, a.z, a.h, a.f};
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const long long g0 = tile * BT;
+        if (threadIdx.x < BT) {
+            const long long g = g0 + threadIdx.x;
+            int i = -1, j = -1;
+            if (g < S2) {
+                j = (int)(g / a.szI);
+                i = (int)(g - (long long)j * a.szI);
+            }
+            const bool ok = i >= 1 && i <= L - 1 && j >= 1 && j <= M - 1;
+            pi[threadIdx.x] = ok ? i : -1;
+            pj[threadIdx.x] = j;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < BT * IN; e += NT) {
+            const int s = e / IN, k = e - (e / IN) * IN;
+            uint16_t x = 0;
+            const int i = pi[s];
+            if (i >= 0 && k < NIN) {
+                const int f = k / 9, dj = (k - f * 9) / 3 - 1, di = k % 3 - 1;
+                const int g = (pj[s] + dj) * a.szI + (i + di);
+                x = f2bf(((float)fld[f][g] - a.mu[k]) / a.sd[k]);
+            }
+            S.X[s][k] = x;
+        }
+        __syncthreads();
+        infer_tile(S);
+        for (int s = threadIdx.x; s < BT; s += NT) {
+            const long long g = g0 + s;
+            if (g >= S2) continue;
             const int i = (int)(g % a.szI), j = (int)(g / a.szI);
-            if (i >= 1 && i <= L - 1 && j >= 1 && j <= M - 1) x = f2bf((feature(a, i, j, k) - a.mu[k]) / a.sd[k]);
+            T su = a.sustr[g], sv = a.svstr[g];
+            if (i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1) {
+                const float Ru = S.Z3[s][0] * a.sd_u + a.mu_u;
+                const T Du = T(0.5) * ((a.h[g - 1] + a.z[g - 1]) + (a.h[g] + a.z[g]));
+                su = su + T(Ru) * Du;
+            }
+            if (i >= 1 && i <= L - 1 && j >= 2 && j <= M - 1) {
+                const float Rv = S.Z3[s][1] * a.sd_v + a.mu_v;
+                const T Dv = T(0.5) * ((a.h[g - a.szI] + a.z[g - a.szI]) + (a.h[g] + a.z[g]));
+                sv = sv + T(Rv) * Dv;
+            }
+            a.sustr_eff[g] = su;
+            a.svstr_eff[g] = sv;
         }
-        S.X[s][k] = x;
-    }
-    __syncthreads();
-    forward_tile(S);
-    for (int s = threadIdx.x; s < BT; s += NT) {
-        const long long g = g0 + s;
-        if (g >= S2) continue;
-        const int i = (int)(g % a.szI), j = (int)(g / a.szI);
-        T su = a.sustr[g], sv = a.svstr[g];
-        if (i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1) {
-            const float Ru = S.Z3[s][0] * a.sd_u + a.mu_u;
-            const T Du = T(0.5) * ((a.h[g - 1] + a.z[g - 1]) + (a.h[g] + a.z[g]));
-            su = su + T(Ru) * Du;
-        }
-        if (i >= 1 && i <= L - 1 && j >= 2 && j <= M - 1) {
-            const float Rv = S.Z3[s][1] * a.sd_v + a.mu_v;
-            const T Dv = T(0.5) * ((a.h[g - a.szI] + a.z[g - a.szI]) + (a.h[g] + a.z[g]));
-            sv = sv + T(Rv) * Dv;
-        }
-        a.sustr_eff[g] = su;
-        a.svstr_eff[g] = sv;
+        __syncthreads();   // S.X / S.Z3 are reused by the next tile
     }
 }
 
@@ -579,7 +652,9 @@ extern "C" int sf_closure_stress(sf_mlp *p, const sf_phys_fields *x, int32_t pre
 {
     SF_REQUIRE(p && x && f && sustr_eff && svstr_eff, "sf_closure_stress: null argument");
     const long long S2 = (long long)x->szI * x->szJ;
-    const dim3 grid((unsigned)((S2 + BT - 1) / BT)), block(NT);
+    const long long ntiles = (S2 + BT - 1) / BT;
+    // persistent grid: the weights are staged once per workgroup (3 workgroups per CU)
+    const dim3 grid((unsigned)(ntiles < 768 ? ntiles : 768)), block(NT);
     if (precision == SF_FP64)
         hipLaunchKernelGGL(k_closure_stress<double>, grid, block, 0, sf::as_stream(stream), p->wb, p->w,
                            closure_args<double>(p, x, f, sustr_eff, svstr_eff, nullptr));
