@@ -342,6 +342,70 @@ def run_closure_gpu(args):
     }
 
 
+def run_3d_gpu(args):
+    """Baroclinic 3D kernels (faithful reference discretisation, FP64) on a 1024 x 512
+    basin1-recipe grid with 32 s-levels: rhs3d (Coriolis, horizontal and vertical
+    advection), set_maxflux and omega, each timed with HIP events on the launch
+    stream.  A 3D cell-step = one (i, j, k) rho cell x one baroclinic step."""
+    import numpy as np
+    import torch
+    from sailfish_amd import configs
+    from sailfish_amd import mod_operators as ops
+    from sailfish_amd.main3d import initial_depths
+    from sailfish_amd.model2d import build_model3d
+
+    szI, szJ, N = args.szI3, args.szJ3, args.N3
+    cfg = configs.basin1(szI, szJ, dx=5e3)
+    rng = np.random.Generator(np.random.PCG64(3))
+    base = configs.gaussian_smooth(rng.standard_normal((szJ, szI)), 8.0)
+    cfg["u0"] = 0.05 * base[None] * np.ones((N + 1, 1, 1))
+    cfg["v0"] = 0.03 * base[None] * np.ones((N + 1, 1, 1))
+    ct, GRID, OCEAN, BOUNDARY = build_model3d(cfg, N=N, ndtfast=40, dt=300.0, fix_q5=True)
+    initial_depths(GRID, OCEAN)
+    c = ops.ctx_of(GRID)
+    BC = BOUNDARY.device_maps()["bcIdxFieldIdx2"]
+    Huon, Hvom, W = OCEAN.Huon.reshape(-1), OCEAN.Hvom.reshape(-1), OCEAN.W.reshape(-1)
+    Hz, z_w = GRID.Hz.reshape(-1), GRID.z_w.reshape(-1)
+    S2 = GRID.szI * GRID.szJ
+    grsz, bksz = (S2 // 512 + 1,), (512,)
+    kern = {
+        "set_maxflux": lambda: ops.set_maxflux(None, None, (OCEAN.u_t2, OCEAN.v_t2, Huon, Hvom, Hz), ctx=c),
+        "omega": lambda: ops.omega(None, None, (W, OCEAN.u_t2, OCEAN.v_t2, Huon, Hvom, z_w, BC), ctx=c),
+        "addCoriolis": lambda: ops.addCoriolis(grsz, bksz, (GRID.fomn.reshape(-1), OCEAN.u_t2, OCEAN.v_t2,
+                                                            OCEAN.ru_t2, OCEAN.rv_t2, Hz), ctx=c),
+        "horizontalAdvection": lambda: ops.horizontalAdvection(grsz, bksz, (OCEAN.u_t2, OCEAN.v_t2, Huon, Hvom,
+                                                                            OCEAN.ru_t2, OCEAN.rv_t2, BC), ctx=c),
+        "verticalAdvection": lambda: ops.verticalAdvection(grsz, bksz, (OCEAN.u_t2, OCEAN.v_t2, W, OCEAN.ru_t2,
+                                                                        OCEAN.rv_t2, BC), ctx=c),
+    }
+    # algorithmic FP64 bytes per 3D cell (fields read and/or written once)
+    fields = {"set_maxflux": 5, "omega": 4, "addCoriolis": 6, "horizontalAdvection": 6, "verticalAdvection": 5}
+    for f in kern.values():
+        f()
+    torch.cuda.synchronize()
+    reps = 20
+    out = {}
+    cells = S2 * (N + 1)
+    for name, f in kern.items():
+        a, b = _events(torch, 1)[0]
+        a.record()
+        for _ in range(reps):
+            f()
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        out[name] = {"ms": ms, "GB_s": fields[name] * 8 * cells / (ms * 1e-3) / 1e9}
+    tot = sum(v["ms"] for v in out.values())
+    cells_int = S2 * N
+    return {
+        "workload": f"{szI}x{szJ}x{N} s-levels, basin1 recipe (dx 5 km), FP64, reference (faithful) 3D kernels",
+        "kernels": out,
+        "ms_rhs3d_maxflux_omega": tot,
+        "cell_steps_per_s_3d": cells_int / (tot * 1e-3),
+        "hbm_frac_3d": sum(fields[k] * 8 * cells for k in kern) / (tot * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+
+
 def calib_copy(args):
     """Plain device-to-device copy of 1 GiB, 5 times: the byte counters of this
     kernel calibrate FETCH_SIZE / WRITE_SIZE (2^30 bytes read and written each)."""
@@ -379,6 +443,10 @@ def main():
     ap.add_argument("--calib-copy", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-closure", action="store_true")
+    ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--szI3", type=int, default=1024)
+    ap.add_argument("--szJ3", type=int, default=512)
+    ap.add_argument("--N3", type=int, default=32)
     ap.add_argument("--ens-members", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=498)
     args = ap.parse_args()
@@ -446,6 +514,8 @@ def main():
         out["ensemble"] = run_ensemble_gpu(args)
     if not args.no_closure and args.mode == "phys":
         out["closure"] = run_closure_gpu(args)
+    if not args.no_3d and args.mode == "phys":
+        out["baroclinic_3d"] = run_3d_gpu(args)
     if not args.no_cpu:
         out["cpu_baseline"] = (run_cpu_phys if args.mode == "phys" else run_cpu_faithful)(args, args.cpu_budget)
     print(json.dumps(out))

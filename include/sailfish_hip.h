@@ -281,6 +281,23 @@ int sf_ens_substep(sf_ens *ens, int32_t first, double cff1, void *stream);
 int sf_ens_capture(sf_ens *ens, int32_t nsub, const double *cff1, void *stream);
 int sf_ens_replay(sf_ens *ens, void *stream);
 
+/* ---------------- open-boundary volume conservation --------------------------------- */
+
+#define SF_OBC_WEST 1
+#define SF_OBC_EAST 2
+#define SF_OBC_SOUTH 4
+#define SF_OBC_NORTH 8
+
+/* obc_volcons + conserve_mass (nonlinear/obc_volcons.py:1-72): FP64 sums of the open
+ * segments' cross-section area and net volume flux of the state (zeta, ubar, vbar) in
+ * `precision`; out[0] = area, out[1] = flux, out[2] = ubar_xs = flux / area (device,
+ * FP64).  apply != 0: the boundary velocities are corrected by -/+ ubar_xs (west,
+ * south / east, north).  work: sf_obc_volcons_work_size(szI, szJ) doubles. */
+int sf_obc_volcons_work_size(int32_t szI, int32_t szJ);
+int sf_obc_volcons(const void *zeta, const void *h, const void *pm, const void *pn, void *ubar, void *vbar,
+                   int32_t szI, int32_t szJ, int32_t precision, int32_t flags, int32_t apply, double *work,
+                   double *out, void *stream);
+
 /* ---------------- learned subgrid momentum closure ---------------------------------- */
 
 /* The closure network (45 -> 64 GELU -> 64 GELU -> 2) trained and evaluated on matrix

@@ -73,3 +73,19 @@ class OPhys:
             k = min(max(iif - 1, 0), len(weight1) - 1)
             self.substep(dt, start_first and n == 0, float(weight1[k]))
         return iif
+
+
+def obc_volcons(szI, szJ, z, h, pm, pn, u, v, flags, apply=True):
+    """sfo_obc_volcons: returns (area, flux, ubar_xs); corrects u, v in place when apply."""
+    L = lib()
+    if not getattr(L, "_volcons_bound", False):
+        L.sfo_obc_volcons.argtypes = [C.c_int, C.c_int] + [P] * 6 + [C.c_int, C.c_int, P]
+        L.sfo_obc_volcons.restype = None
+        L._volcons_bound = True
+    out = np.zeros(3)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (z, h, pm, pn)]
+    for a in (u, v):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    L.sfo_obc_volcons(int(szI), int(szJ), *[O.ptr(a) for a in arrs], O.ptr(u), O.ptr(v), int(flags), int(apply),
+                      O.ptr(out))
+    return float(out[0]), float(out[1]), float(out[2])

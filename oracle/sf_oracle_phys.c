@@ -241,3 +241,49 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
         }
     sfo_phys_bc(p, z2, u2, v2);
 }
+
+/* obc_volcons + conserve_mass (nonlinear/obc_volcons.py:1-72) in FP64: sequential sums
+ * over the open segments in the order west, east, south, north (points ascending).
+ * flags: 1 west, 2 east, 4 south, 8 north.  out[0] area, out[1] flux, out[2] ubar_xs.
+ * apply != 0 corrects the boundary velocities in place. */
+void sfo_obc_volcons(int szI, int szJ, const double *z, const double *h, const double *pm, const double *pn,
+                     double *u, double *v, int flags, int apply, double *out)
+{
+    const int W = szI, L = szI - 1, M = szJ - 1;
+    double area = 0.0, flux = 0.0;
+    for (int seg = 0; seg < 4; seg++) {
+        if (!(flags & (1 << seg))) continue;
+        const int n = seg < 2 ? M - 1 : L - 1;
+        for (int b = 0; b < n; b++) {
+            double cff, fl;
+            if (seg < 2) {
+                const int j = 1 + b, i = seg == 0 ? 1 : L;
+                const ptrdiff_t f = (ptrdiff_t)j * W + i;
+                const double onu = 2.0 / (pn[f - 1] + pn[f]);
+                cff = (0.5 * ((z[f - 1] + h[f - 1]) + (z[f] + h[f]))) * onu;
+                fl = seg == 0 ? cff * u[f] : -(cff * u[f]);
+            } else {
+                const int i = 1 + b, j = seg == 2 ? 1 : M;
+                const ptrdiff_t f = (ptrdiff_t)j * W + i;
+                const double omv = 2.0 / (pm[f - W] + pm[f]);
+                cff = (0.5 * ((z[f - W] + h[f - W]) + (z[f] + h[f]))) * omv;
+                fl = seg == 2 ? cff * v[f] : -(cff * v[f]);
+            }
+            area += cff;
+            flux += fl;
+        }
+    }
+    out[0] = area;
+    out[1] = flux;
+    out[2] = area > 0.0 ? flux / area : 0.0;
+    if (!apply) return;
+    const double xs = out[2];
+    if (flags & 1)
+        for (int j = 1; j < M; j++) u[(ptrdiff_t)j * W + 1] -= xs;
+    if (flags & 2)
+        for (int j = 1; j < M; j++) u[(ptrdiff_t)j * W + L] += xs;
+    if (flags & 4)
+        for (int i = 1; i < L; i++) v[(ptrdiff_t)W + i] -= xs;
+    if (flags & 8)
+        for (int i = 1; i < L; i++) v[(ptrdiff_t)M * W + i] += xs;
+}

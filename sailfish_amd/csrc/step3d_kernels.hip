@@ -19,7 +19,20 @@ struct A {
     long long n;
 };
 
-__device__ __forceinline__ double r3(A a, long long idx) { return (idx < 0 || idx >= a.n) ? 0.0 : a.p[idx]; }
+// out-of-range reads are 0.0; the load itself always uses a clamped (valid) index, so it
+// is unconditional and identical reads of one expression tree merge into one load
+__device__ __forceinline__ double r3(A a, long long idx)
+{
+    const long long c = idx < 0 ? 0 : (idx >= a.n ? a.n - 1 : idx);
+    const double x = a.p[c];
+    return (idx < 0 || idx >= a.n) ? 0.0 : x;
+}
+template <bool CHK>
+__device__ __forceinline__ double rd(A a, long long idx)
+{
+    if constexpr (CHK) return r3(a, idx);
+    return a.p[idx];
+}
 
 struct G3 {
     int szI, szJ, N;
@@ -45,15 +58,17 @@ __device__ __forceinline__ bool isInnerCell(const G3 &g, long long f)
     if (f >= g.S2) return;
 
 // ---- addCoriolis (rhs_kernels.cpp:6-35) -------------------------------------------
+template <bool CHK>
 __device__ __forceinline__ double cor_UF(const G3 &g, A fomn, A v, A Hz, long long K, long long f)
 {
     const long long b = K * g.S2 + f;
-    return (r3(Hz, b) * r3(fomn, f)) * ((r3(v, b) + r3(v, b + g.szI)) * 0.5);
+    return (rd<CHK>(Hz, b) * rd<CHK>(fomn, f)) * ((rd<CHK>(v, b) + rd<CHK>(v, b + g.szI)) * 0.5);
 }
+template <bool CHK>
 __device__ __forceinline__ double cor_VF(const G3 &g, A fomn, A u, A Hz, long long K, long long f)
 {
     const long long b = K * g.S2 + f;
-    return (r3(Hz, b) * r3(fomn, f)) * ((r3(u, b) + r3(u, b + 1)) * 0.5);
+    return (rd<CHK>(Hz, b) * rd<CHK>(fomn, f)) * ((rd<CHK>(u, b) + rd<CHK>(u, b + 1)) * 0.5);
 }
 
 __global__ __launch_bounds__(256) void k_addCoriolis(G3 g, A fomn, A u, A v, double *ru, double *rv, A Hz)
@@ -62,8 +77,16 @@ __global__ __launch_bounds__(256) void k_addCoriolis(G3 g, A fomn, A u, A v, dou
     if (!isRNode(g, f)) return;
     for (long long K = 0; K < g.N; K++) {
         const long long b = K * g.S2 + f;
-        const double rtou = (cor_UF(g, fomn, v, Hz, K, f) + cor_UF(g, fomn, v, Hz, K, f - 1)) * 0.5;
-        const double rtov = (cor_VF(g, fomn, u, Hz, K, f) + cor_VF(g, fomn, u, Hz, K, f - g.szI)) * 0.5;
+        double rtou, rtov;
+        // rho nodes have i, j >= 1: the reads at b-1 and b-szI exist; the forward ones
+        // (b+1, b+szI) are checked against the array sizes once per level
+        if (b + g.szI < v.n && b + 1 < u.n && b < Hz.n && f < fomn.n) {
+            rtou = (cor_UF<false>(g, fomn, v, Hz, K, f) + cor_UF<false>(g, fomn, v, Hz, K, f - 1)) * 0.5;
+            rtov = (cor_VF<false>(g, fomn, u, Hz, K, f) + cor_VF<false>(g, fomn, u, Hz, K, f - g.szI)) * 0.5;
+        } else {
+            rtou = (cor_UF<true>(g, fomn, v, Hz, K, f) + cor_UF<true>(g, fomn, v, Hz, K, f - 1)) * 0.5;
+            rtov = (cor_VF<true>(g, fomn, u, Hz, K, f) + cor_VF<true>(g, fomn, u, Hz, K, f - g.szI)) * 0.5;
+        }
         ru[b] = ru[b] + rtou;
         rv[b] = rv[b] - rtov;
     }
@@ -75,63 +98,83 @@ struct H {
     long long W;
 };
 
-#define AT(a, o) r3((a), base + (o))
+#define AT(a, o) rd<CHK>((a), base + (o))
+template <bool CHK>
 __device__ __forceinline__ double dxx(A a, long long base, long long o)
 {
     return AT(a, o - 1) - 2.0 * AT(a, o) + AT(a, o + 1);
 }
+template <bool CHK>
 __device__ __forceinline__ double dee(A a, long long base, long long o, long long W)
 {
     return AT(a, o - W) - 2.0 * AT(a, o) + AT(a, o + W);
 }
+template <bool CHK>
 __device__ __forceinline__ double h_uR(const H &h, long long base, long long o) { return (AT(h.u, o) + AT(h.u, o + 1)) * 0.5; }
+template <bool CHK>
 __device__ __forceinline__ double h_uP(const H &h, long long base, long long o) { return (AT(h.u, o) + AT(h.u, o - 1)) * 0.5; }
+template <bool CHK>
 __device__ __forceinline__ double h_vR(const H &h, long long base, long long o) { return (AT(h.v, o) + AT(h.v, o + h.W)) * 0.5; }
+template <bool CHK>
 __device__ __forceinline__ double h_vP(const H &h, long long base, long long o) { return (AT(h.v, o) + AT(h.v, o - h.W)) * 0.5; }
+template <bool CHK>
 __device__ __forceinline__ double h_HuR(const H &h, long long base, long long o) { return (AT(h.Hu, o) + AT(h.Hu, o + 1)) * 0.5; }
+template <bool CHK>
 __device__ __forceinline__ double h_HuP(const H &h, long long base, long long o) { return (AT(h.Hu, o) + AT(h.Hu, o - 1)) * 0.5; }
+template <bool CHK>
 __device__ __forceinline__ double h_HvP(const H &h, long long base, long long o) { return (AT(h.Hv, o) + AT(h.Hv, o - h.W)) * 0.5; }
+template <bool CHK>
 __device__ __forceinline__ double h_HvR(const H &h, long long base, long long o) { return (AT(h.Hv, o) + AT(h.Hv, o + h.W)) * 0.5; }
+template <bool CHK>
 __device__ __forceinline__ double h_uxxR(const H &h, long long base, long long o)
 {
-    return (h_uR(h, base, o) + h_uR(h, base, o + 1) >= 0) ? dxx(h.u, base, o) : dxx(h.u, base, o + 1);
+    return (h_uR<CHK>(h, base, o) + h_uR<CHK>(h, base, o + 1) >= 0) ? dxx<CHK>(h.u, base, o) : dxx<CHK>(h.u, base, o + 1);
 }
+template <bool CHK>
 __device__ __forceinline__ double h_ueeP(const H &h, long long base, long long o)
 {
-    return (h_HvP(h, base, o) + h_HvP(h, base, o - h.W) >= 0) ? dee(h.u, base, o, h.W) : dee(h.u, base, o - h.W, h.W);
+    return (h_HvP<CHK>(h, base, o) + h_HvP<CHK>(h, base, o - h.W) >= 0) ? dee<CHK>(h.u, base, o, h.W) : dee<CHK>(h.u, base, o - h.W, h.W);
 }
+template <bool CHK>
 __device__ __forceinline__ double h_HuxxR(const H &h, long long base, long long o)
 {
-    return (dxx(h.Hu, base, o) + dxx(h.Hu, base, o + 1)) * 0.5;
+    return (dxx<CHK>(h.Hu, base, o) + dxx<CHK>(h.Hu, base, o + 1)) * 0.5;
 }
+template <bool CHK>
 __device__ __forceinline__ double h_HueeP(const H &h, long long base, long long o)
 {
-    return (dee(h.Hu, base, o, h.W) + dee(h.Hu, base, o - 1, h.W)) * 0.5;
+    return (dee<CHK>(h.Hu, base, o, h.W) + dee<CHK>(h.Hu, base, o - 1, h.W)) * 0.5;
 }
+template <bool CHK>
 __device__ __forceinline__ double h_HvxxP(const H &h, long long base, long long o)
 {
-    return (dxx(h.Hv, base, o) + dxx(h.Hv, base, o - h.W)) * 0.5;
+    return (dxx<CHK>(h.Hv, base, o) + dxx<CHK>(h.Hv, base, o - h.W)) * 0.5;
 }
+template <bool CHK>
 __device__ __forceinline__ double h_HveeR(const H &h, long long base, long long o)
 {
-    return (dee(h.Hv, base, o, h.W) + dee(h.Hv, base, o + h.W, h.W)) * 0.5;
+    return (dee<CHK>(h.Hv, base, o, h.W) + dee<CHK>(h.Hv, base, o + h.W, h.W)) * 0.5;
 }
 constexpr double Gadv = -0.5;
+template <bool CHK>
 __device__ __forceinline__ double UFxR(const H &h, long long b, long long o)
 {
-    return h_uR(h, b, o) + (Gadv * h_uxxR(h, b, o)) * (h_HuR(h, b, o) + Gadv * h_HuxxR(h, b, o));
+    return h_uR<CHK>(h, b, o) + (Gadv * h_uxxR<CHK>(h, b, o)) * (h_HuR<CHK>(h, b, o) + Gadv * h_HuxxR<CHK>(h, b, o));
 }
+template <bool CHK>
 __device__ __forceinline__ double UFeP(const H &h, long long b, long long o)
 {
-    return h_uP(h, b, o) + (Gadv * h_ueeP(h, b, o)) * (h_HvP(h, b, o) + Gadv * h_HvxxP(h, b, o));
+    return h_uP<CHK>(h, b, o) + (Gadv * h_ueeP<CHK>(h, b, o)) * (h_HvP<CHK>(h, b, o) + Gadv * h_HvxxP<CHK>(h, b, o));
 }
+template <bool CHK>
 __device__ __forceinline__ double VFxP(const H &h, long long b, long long o)
 {
-    return h_vP(h, b, o) + (Gadv * h_ueeP(h, b, o)) * (h_HuP(h, b, o) + Gadv * h_HueeP(h, b, o));
+    return h_vP<CHK>(h, b, o) + (Gadv * h_ueeP<CHK>(h, b, o)) * (h_HuP<CHK>(h, b, o) + Gadv * h_HueeP<CHK>(h, b, o));
 }
+template <bool CHK>
 __device__ __forceinline__ double VFeR(const H &h, long long b, long long o)
 {
-    return h_vR(h, b, o) + (Gadv * h_uxxR(h, b, o)) * (h_HvR(h, b, o) + Gadv * h_HveeR(h, b, o));
+    return h_vR<CHK>(h, b, o) + (Gadv * h_uxxR<CHK>(h, b, o)) * (h_HvR<CHK>(h, b, o) + Gadv * h_HveeR<CHK>(h, b, o));
 }
 #undef AT
 
@@ -139,10 +182,17 @@ __global__ __launch_bounds__(256) void k_horizontalAdvection(G3 g, H h, double *
 {
     COLUMN_INDEX
     const long long W = g.szI;
+    const long long n = h.u.n;   // u, v, Huon, Hvom share one size
     for (long long K = 0; K < g.N; K++) {
         const long long b = K * g.S2 + f;
-        const double du = (UFxR(h, b, 0) - UFxR(h, b, -1)) + (UFeP(h, b, 0) - UFeP(h, b, -W));
-        const double dv = (VFxP(h, b, 0) - VFxP(h, b, -1)) + (VFeR(h, b, 0) - VFeR(h, b, -W));
+        double du, dv;
+        if (b - 4 * W >= 0 && b + 4 * W < n) {   // every read of the tree is in range
+            du = (UFxR<false>(h, b, 0) - UFxR<false>(h, b, -1)) + (UFeP<false>(h, b, 0) - UFeP<false>(h, b, -W));
+            dv = (VFxP<false>(h, b, 0) - VFxP<false>(h, b, -1)) + (VFeR<false>(h, b, 0) - VFeR<false>(h, b, -W));
+        } else {
+            du = (UFxR<true>(h, b, 0) - UFxR<true>(h, b, -1)) + (UFeP<true>(h, b, 0) - UFeP<true>(h, b, -W));
+            dv = (VFxP<true>(h, b, 0) - VFxP<true>(h, b, -1)) + (VFeR<true>(h, b, 0) - VFeR<true>(h, b, -W));
+        }
         ru[b] = ru[b] - du;
         rv[b] = rv[b] - dv;
     }
@@ -158,15 +208,17 @@ __global__ void k_hadv_tail(long long S2, long long end, double *ru, double *rv)
 }
 
 // ---- verticalAdvection (rhs_kernels.cpp:135-197) ----------------------------------
+template <bool CHK = true>
 __device__ __forceinline__ double v4(A a, long long S2, long long K, long long f)
 {
-    return (9.0 / 16.0) * (r3(a, K * S2 + f) + r3(a, (K + 1) * S2 + f)) -
-           (1.0 / 16.0) * (r3(a, (K - 1) * S2 + f) + r3(a, (K + 2) * S2 + f));
+    return (9.0 / 16.0) * (rd<CHK>(a, K * S2 + f) + rd<CHK>(a, (K + 1) * S2 + f)) -
+           (1.0 / 16.0) * (rd<CHK>(a, (K - 1) * S2 + f) + rd<CHK>(a, (K + 2) * S2 + f));
 }
+template <bool CHK = true>
 __device__ __forceinline__ double h4(A a, long long S2, long long K, long long f, long long d)
 {
     const long long b = K * S2 + f;
-    return (9.0 / 16.0) * (r3(a, b) + r3(a, b + d)) - (1.0 / 16.0) * (r3(a, b - d) + r3(a, b + 2 * d));
+    return (9.0 / 16.0) * (rd<CHK>(a, b) + rd<CHK>(a, b + d)) - (1.0 / 16.0) * (rd<CHK>(a, b - d) + rd<CHK>(a, b + 2 * d));
 }
 
 __global__ __launch_bounds__(256) void k_verticalAdvection(G3 g, A U, A V, A Wa, double *ru, double *rv,
@@ -177,9 +229,19 @@ __global__ __launch_bounds__(256) void k_verticalAdvection(G3 g, A U, A V, A Wa,
     const long long S2 = g.S2, N = g.N, W = g.szI;
 #define FU(k) (v4(U, S2, (k), f) * h4(Wa, S2, (k), f, 1))
 #define FV(k) (v4(V, S2, (k), f) * h4(Wa, S2, (k), f, W))
-    for (long long K = 2; K < N - 2; K++) {
-        ru[K * S2 + f] = ru[K * S2 + f] - (FU(K + 1) - FU(K));
-        rv[K * S2 + f] = rv[K * S2 + f] - (FV(K + 1) - FV(K));
+    // interior levels: every read is in range (inner cell, levels K-1 .. K+3 <= N-1);
+    // the flux at the upper face of level K is the lower-face flux of level K+1
+    if (N - 2 > 2) {
+        double fu = v4<false>(U, S2, 2, f) * h4<false>(Wa, S2, 2, f, 1);
+        double fv = v4<false>(V, S2, 2, f) * h4<false>(Wa, S2, 2, f, W);
+        for (long long K = 2; K < N - 2; K++) {
+            const double fu1 = v4<false>(U, S2, K + 1, f) * h4<false>(Wa, S2, K + 1, f, 1);
+            const double fv1 = v4<false>(V, S2, K + 1, f) * h4<false>(Wa, S2, K + 1, f, W);
+            ru[K * S2 + f] = ru[K * S2 + f] - (fu1 - fu);
+            rv[K * S2 + f] = rv[K * S2 + f] - (fv1 - fv);
+            fu = fu1;
+            fv = fv1;
+        }
     }
     rv[1 * S2 + f] -= (9.0 / 16.0) * (FV(1) + FV(2)) - (1.0 / 16.0) * (FV(1) + FV(3));
     const double u0 = r3(U, f), u1 = r3(U, S2 + f), u2 = r3(U, 2 * S2 + f);
