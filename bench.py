@@ -57,6 +57,22 @@ def _events(torch, n):
     return [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
 
 
+def measured_traffic(dtype_name):
+    """HBM bytes per launch of k_phys_fused from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE record (profiles/k_phys_fused_traffic.json), used only while the kernel
+    source is the one that was profiled (same SHA-1); otherwise None."""
+    import hashlib
+    rec_path = ROOT / "profiles" / "k_phys_fused_traffic.json"
+    src = ROOT / "sailfish_amd" / "csrc" / "phys2d_fused.hip"
+    try:
+        rec = json.loads(rec_path.read_text())
+        if hashlib.sha1(src.read_bytes()).hexdigest() != rec["source_sha1"]:
+            return None
+        return int(rec[dtype_name]["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def run_phys_gpu(args, dtype_name, steps, warmup):
     import torch
     from sailfish_amd import configs
@@ -498,7 +514,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": r["achieved"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.dtype) if args.mode == "phys" and args.szI == 2048
+            and args.szJ == 1024 and args.tile_j in (0, 8) else None,
             "kernel": r["kernel"],
             "kernel_ms": r["kern_ms"],
             "bytes_per_point": r["bpp"],
