@@ -83,7 +83,7 @@ def run_phys_gpu(args, dtype_name, steps, warmup):
     cfg = configs.basin2(args.szI, args.szJ, dx=args.dx)
     ct, GRID, OCEAN, _B = build_model(cfg, dtype=dtype, ndtfast=30)
     st = PhysStepper(GRID, OCEAN, rdrg=cfg["rdrg"], fused=True, tile_j=args.tile_j)
-    tile_j = args.tile_j or st.lib.sf_phys_fused_default_tile(st.precision)
+    tile_j = args.tile_j or st.lib.sf_phys_fused_default_tile(st.precision, GRID.szI, GRID.szJ)
     S2 = GRID.szI * GRID.szJ
     n = [0]
 

@@ -252,8 +252,9 @@ int sf_phys_substep(const sf_phys_fields *fields, int32_t precision, double dt, 
 int sf_phys_substep_fused(const sf_phys_fields *fields, int32_t precision, double dt, int32_t first,
                           double cff1, int32_t tile_j, void *stream);
 
-/* Default tile height of sf_phys_substep_fused for a precision. */
-int sf_phys_fused_default_tile(int32_t precision);
+/* Default tile height of sf_phys_substep_fused for a precision and grid size (8, or 4
+ * when 64 x 8 tiles would give fewer than 1024 workgroups). */
+int sf_phys_fused_default_tile(int32_t precision, int32_t szI, int32_t szJ);
 
 /* ---------------- ensembles of physics-mode basins ---------------------------------- */
 

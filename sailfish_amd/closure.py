@@ -380,11 +380,19 @@ def normalise(F: torch.Tensor, Y: torch.Tensor):
     return mu, sd, ym, ys
 
 
+def recip_sd(sd) -> np.ndarray:
+    """1 / sd rounded to FP32 (the scale sf_mlp_set_norm stores for the inference kernel)."""
+    sd = sd.detach().cpu().numpy() if isinstance(sd, torch.Tensor) else np.asarray(sd)
+    return np.float32(1.0) / sd.astype(np.float32)
+
+
 def pack_inputs(F: torch.Tensor, mu, sd) -> torch.Tensor:
-    """Normalised BF16 inputs [n, 48] (padding columns zero)."""
+    """Normalised BF16 inputs [n, 48] (padding columns zero): (F - mu) * (1 / sd), the
+    arithmetic of the closure inference kernel."""
     n = F.shape[0]
     Xn = torch.zeros((n, IN), dtype=torch.float32, device=F.device)
-    Xn[:, :NIN] = (F - mu) / sd
+    mu = torch.as_tensor(mu, dtype=torch.float32, device=F.device)
+    Xn[:, :NIN] = (F - mu) * torch.from_numpy(recip_sd(sd)).to(F.device)
     return to_bf16_bits(Xn)
 
 

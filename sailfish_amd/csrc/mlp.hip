@@ -14,6 +14,10 @@
 //   A (16 x 16, row-major M x K): lane holds A[r][4q .. 4q+3]
 //   B (16 x 16, K x N):           lane holds B[4q .. 4q+3][r]
 //   D (16 x 16):                  lane holds D[4q + t][r], t = 0..3
+// GELU is x * Phi(x) with Phi from the erfc approximation of Abramowitz & Stegun
+// (1964) eq. 7.1.26 (|error| <= 1.5e-7); in FP32 with the hardware reciprocal and
+// exponential, |GELU error| <= 4.3e-7 and |GELU' error| <= 3e-7 over the real line.
+// BF16 rounding is round-to-nearest-even (v_cvt_pk_bf16_f32).
 #include "sf_internal.h"
 
 #include <math.h>
@@ -29,19 +33,38 @@ constexpr int OFF_W1 = 0, OFF_B1 = IN * H, OFF_W2 = OFF_B1 + H, OFF_B2 = OFF_W2 
 constexpr int BT = 64;    // samples per tile
 constexpr int NT = 256;   // threads per workgroup (4 waves)
 
-// FP32 -> BF16, round to nearest even (finite inputs)
-__device__ __forceinline__ uint16_t f2bf(float x)
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+// two FP32 -> packed BF16 pair (lo in bits 0..15), round to nearest even
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi)
 {
-    uint32_t u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t){lo, hi}, bf16x2_t));
 }
+__device__ __forceinline__ uint16_t f2bf(float x) { return (uint16_t)pack_bf16x2(x, 0.0f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// Phi(x) (standard normal CDF) and exp(-x^2 / 2).  erfc(y) ~ (a1 t + .. + a5 t^5) exp(-y^2)
+// with t = 1 / (1 + p y), y = |x| / sqrt 2; the factor 1/2 of Phi = erfc / 2 is folded
+// into the coefficients.
+struct PhiExp {
+    float phi, e;
+};
+__device__ __forceinline__ PhiExp phi_exp(float x)
+{
+    constexpr float p = 0.3275911f * 0.70710678118654752f;
+    constexpr float c1 = 0.5f * 0.254829592f, c2 = 0.5f * -0.284496736f, c3 = 0.5f * 1.421413741f,
+                    c4 = 0.5f * -1.453152027f, c5 = 0.5f * 1.061405429f;
+    const float t = __builtin_amdgcn_rcpf(fmaf(p, fabsf(x), 1.0f));
+    const float e = __builtin_amdgcn_exp2f((x * x) * -0.72134752044448170f);   // 2^(-x^2 log2(e) / 2)
+    const float h = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, c5, c4), c3), c2), c1) * e;   // Phi(-|x|)
+    return {x >= 0.0f ? 1.0f - h : h, e};
+}
+__device__ __forceinline__ float gelu(float x) { return x * phi_exp(x).phi; }
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * expf(-0.5f * x * x) * 0.39894228040143268f;
+    const PhiExp pe = phi_exp(x);
+    return pe.phi + x * pe.e * 0.39894228040143268f;
 }
 
 // One 16x16 output tile (tm, tn) of A(M x K) * B(K x N) accumulated onto acc; A(m, k)
@@ -125,58 +148,29 @@ __device__ void forward_tile(MlpShared &S)
     __syncthreads();
 }
 
-// inference: weights once per workgroup, BF16 activations only
-struct InferShared {
-    uint16_t W1[IN][H], W2[H][H], W3[H][OUT];
-    float b1[H], b2[H], b3[OUT];
-    uint16_t X[BT][IN];
-    uint16_t A1[BT][H], A2[BT][H];
-    float Z3[BT][OUT];
-};
+// Inference operand layout.  The inference kernel evaluates the transposed products
+// Z^T = W^T X^T: the A operand is a weight tile of W^T, the B operand the activations
+// (lane r, q: sample r, units 4q..4q+3), and the D layout of one layer is then the B
+// layout of the next.  Tile order: layer 1 (tn, kb) at tn * 3 + kb, layer 2 at
+// 12 + tn * 4 + kb, layer 3 at 28 + kb; lane r + 16 q of tile (tn, kb) holds
+// W[16 kb + 4 q + t][16 tn + r], t = 0..3, as 4 BF16 values.
+constexpr int NWT = 32;                     // weight tiles
+constexpr int NWPK = NWT * 64 * 4;          // BF16 values in the packed layout
 
-__device__ void load_weights_infer(InferShared &S, const uint16_t *wb, const float *w)
+// slot of parameter p in the packed layout, -1 for biases
+__device__ __forceinline__ int wpk_slot(int p)
 {
-    for (int e = threadIdx.x; e < IN * H; e += NT) S.W1[e / H][e % H] = wb[OFF_W1 + e];
-    for (int e = threadIdx.x; e < H * H; e += NT) S.W2[e / H][e % H] = wb[OFF_W2 + e];
-    for (int e = threadIdx.x; e < H * OUT; e += NT) S.W3[e / OUT][e % OUT] = wb[OFF_W3 + e];
-    for (int e = threadIdx.x; e < H; e += NT) {
-        S.b1[e] = w[OFF_B1 + e];
-        S.b2[e] = w[OFF_B2 + e];
+    int k, n, tile;
+    if (p < OFF_B1) {
+        k = p / H, n = p % H, tile = (n >> 4) * 3 + (k >> 4);
+    } else if (p >= OFF_W2 && p < OFF_B2) {
+        k = (p - OFF_W2) / H, n = (p - OFF_W2) % H, tile = 12 + (n >> 4) * 4 + (k >> 4);
+    } else if (p >= OFF_W3 && p < OFF_B3) {
+        k = (p - OFF_W3) / OUT, n = (p - OFF_W3) % OUT, tile = 28 + (k >> 4);
+    } else {
+        return -1;
     }
-    for (int e = threadIdx.x; e < OUT; e += NT) S.b3[e] = w[OFF_B3 + e];
-}
-
-// same arithmetic as forward_tile (identical outputs), without keeping Z1 / Z2
-__device__ void infer_tile(InferShared &S)
-{
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-    for (int tn = 0; tn < 4; tn++) {
-        const float bb = S.b1[tn * 16 + r];
-        f32x4_t acc = {bb, bb, bb, bb};
-        acc = mfma_tile(wave, tn, IN, [&](int m, int k) { return S.X[m][k]; },
-                        [&](int k, int n) { return S.W1[k][n]; }, acc);
-#pragma unroll
-        for (int t = 0; t < 4; t++) S.A1[wave * 16 + 4 * q + t][tn * 16 + r] = f2bf(gelu(acc[t]));
-    }
-    __syncthreads();
-    for (int tn = 0; tn < 4; tn++) {
-        const float bb = S.b2[tn * 16 + r];
-        f32x4_t acc = {bb, bb, bb, bb};
-        acc = mfma_tile(wave, tn, H, [&](int m, int k) { return S.A1[m][k]; },
-                        [&](int k, int n) { return S.W2[k][n]; }, acc);
-#pragma unroll
-        for (int t = 0; t < 4; t++) S.A2[wave * 16 + 4 * q + t][tn * 16 + r] = f2bf(gelu(acc[t]));
-    }
-    __syncthreads();
-    {
-        const float bb = S.b3[r];
-        f32x4_t acc = {bb, bb, bb, bb};
-        acc = mfma_tile(wave, 0, H, [&](int m, int k) { return S.A2[m][k]; },
-                        [&](int k, int n) { return S.W3[k][n]; }, acc);
-#pragma unroll
-        for (int t = 0; t < 4; t++) S.Z3[wave * 16 + 4 * q + t][r] = acc[t];
-    }
-    __syncthreads();
+    return (tile * 64 + (n & 15) + 16 * ((k & 15) >> 2)) * 4 + (k & 3);
 }
 
 struct BwdShared {
@@ -312,7 +306,8 @@ __global__ __launch_bounds__(256) void k_mlp_reduce_adam(const float *__restrict
                                                         const float *__restrict__ partial_loss, int nblk,
                                                         float *__restrict__ grad, float *__restrict__ w,
                                                         float *__restrict__ m, float *__restrict__ v,
-                                                        uint16_t *__restrict__ wb, int adam, float lr,
+                                                        uint16_t *__restrict__ wb, uint16_t *__restrict__ wpk,
+                                                        int adam, float lr,
                                                         float beta1, float beta2, float eps, float bc1, float bc2,
                                                         float inv_count, float *__restrict__ loss_out)
 {
@@ -335,6 +330,8 @@ __global__ __launch_bounds__(256) void k_mlp_reduce_adam(const float *__restrict
     const float nw = w[p] - lr * mh / (sqrtf(vh) + eps);
     w[p] = nw;
     wb[p] = f2bf(nw);
+    const int slot = wpk_slot(p);
+    if (slot >= 0) wpk[slot] = wb[p];
 }
 
 // outputs Y[n][2] of BF16 inputs X[n][48]
@@ -363,7 +360,6 @@ struct ClosureArgs {
     const T *u, *v, *z, *h, *f, *sustr, *svstr;
     T *sustr_eff, *svstr_eff;
     float *feat_out;            // optional raw features [S2][45]
-    const float *mu, *sd;       // input normalisation [45]
     float mu_u, sd_u, mu_v, sd_v;   // output de-normalisation
     int szI, szJ;
 };
@@ -391,64 +387,122 @@ __global__ __launch_bounds__(256) void k_closure_features(ClosureArgs<T> a)
 }
 
 // closure stress: stress_eff = stress + R * D at interior u / v points, where R is the
-// network's (de-normalised) tendency residual and D the depth at the velocity point
-template <typename T>
-__global__ __launch_bounds__(NT) void k_closure_stress(const uint16_t *__restrict__ wb, const float *__restrict__ w,
-                                                      ClosureArgs<T> a)
+// network's (de-normalised) tendency residual and D the depth at the velocity point.
+// Each wave owns chunks of 64 consecutive grid points (lane = point while gathering the
+// features and applying the stress) and keeps the network in registers; features are
+// normalised as (x - mu) * rsd, rsd = 1 / sd rounded to FP32, and rounded to BF16.
+// norm = {mu[45], rsd[45]}; wpk: the packed BF16 weights (NWPK values).
+constexpr int XS = 26;   // feature-tile row stride in BF16 pairs (104 B: conflict-free 8-B row writes)
+
+__device__ __forceinline__ void wave_lds_sync()
 {
-    __shared__ InferShared S;
-    load_weights_infer(S, wb, w);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ bf16x4_t gelu_bf16x4(f32x4_t z)
+{
+    const uint32_t lo = pack_bf16x2(gelu(z[0]), gelu(z[1])), hi = pack_bf16x2(gelu(z[2]), gelu(z[3]));
+    return __builtin_bit_cast(bf16x4_t, (uint64_t)lo | ((uint64_t)hi << 32));
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_closure_stress(const uint64_t *__restrict__ wpk, const float *__restrict__ w,
+                                                      const float *__restrict__ norm, ClosureArgs<T> a)
+{
+    __shared__ uint64_t Wl[NWT * 64];
+    __shared__ __attribute__((aligned(16))) uint32_t Xl[NT / 64][64 * XS];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+    for (int e = threadIdx.x; e < NWT * 64; e += NT) Wl[e] = wpk[e];
+    __syncthreads();
+    bf16x4_t Wt[NWT];
+#pragma unroll
+    for (int t = 0; t < NWT; t++) Wt[t] = __builtin_bit_cast(bf16x4_t, Wl[t * 64 + lane]);
+    f32x4_t B1[4], B2[4];
+#pragma unroll
+    for (int tn = 0; tn < 4; tn++) {
+        B1[tn] = *(const f32x4_t *)(w + OFF_B1 + 16 * tn + 4 * q);
+        B2[tn] = *(const f32x4_t *)(w + OFF_B2 + 16 * tn + 4 * q);
+    }
+    const f32x4_t B3 = *(const f32x4_t *)(w + OFF_B3 + 4 * q);
+    uint32_t *X = Xl[wave];
+
     const long long S2 = (long long)a.szI * a.szJ;
-    const int L = a.szI - 1, M = a.szJ - 1;
-    const long long ntiles = (S2 + BT - 1) / BT;
-    __shared__ int pi[BT], pj[BT];
+    const int szI = a.szI, L = a.szI - 1, M = a.szJ - 1;
+    const long long nchunk = (S2 + 63) / 64;
     const T *fld[5] = {a.u, a.v, a.z, a.h, a.f};
-    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const long long g0 = tile * BT;
-        if (threadIdx.x < BT) {
-            const long long g = g0 + threadIdx.x;
-            int i = -1, j = -1;
-            if (g < S2) {
-                j = (int)(g / a.szI);
-                i = (int)(g - (long long)j * a.szI);
-            }
-            const bool ok = i >= 1 && i <= L - 1 && j >= 1 && j <= M - 1;
-            pi[threadIdx.x] = ok ? i : -1;
-            pj[threadIdx.x] = j;
+    for (long long c = (long long)blockIdx.x * (NT / 64) + wave; c < nchunk; c += (long long)gridDim.x * (NT / 64)) {
+        const long long g = c * 64 + lane;
+        const int j = (int)(g / szI), i = (int)(g - (long long)j * szI);
+        const bool in = g < S2 && i >= 1 && i <= L - 1 && j >= 1 && j <= M - 1;
+        const int gs = in ? (int)g : szI + 1;   // other lanes read an interior point; their outputs are unused
+        // features of point gs -> row `lane` of the wave's tile
+        float xv[IN];
+#pragma unroll
+        for (int k = 0; k < NIN; k++) {
+            const int f = k / 9, dj = (k % 9) / 3 - 1, di = k % 3 - 1;
+            xv[k] = ((float)fld[f][gs + dj * szI + di] - norm[k]) * norm[NIN + k];
         }
-        __syncthreads();
-        for (int e = threadIdx.x; e < BT * IN; e += NT) {
-            const int s = e / IN, k = e - (e / IN) * IN;
-            uint16_t x = 0;
-            const int i = pi[s];
-            if (i >= 0 && k < NIN) {
-                const int f = k / 9, dj = (k - f * 9) / 3 - 1, di = k % 3 - 1;
-                const int g = (pj[s] + dj) * a.szI + (i + di);
-                x = f2bf(((float)fld[f][g] - a.mu[k]) / a.sd[k]);
+        xv[45] = xv[46] = xv[47] = 0.0f;
+        uint64_t *row = (uint64_t *)(X + lane * XS);
+#pragma unroll
+        for (int e = 0; e < IN / 4; e++)
+            row[e] = (uint64_t)pack_bf16x2(xv[4 * e], xv[4 * e + 1]) |
+                     ((uint64_t)pack_bf16x2(xv[4 * e + 2], xv[4 * e + 3]) << 32);
+        wave_lds_sync();
+        float Ru[4], Rv[4];
+#pragma unroll
+        for (int gq = 0; gq < 4; gq++) {
+            const uint64_t *xr = (const uint64_t *)(X + (16 * gq + r) * XS);
+            bf16x4_t xb[3], a1[4], a2[4];
+#pragma unroll
+            for (int kb = 0; kb < 3; kb++) xb[kb] = __builtin_bit_cast(bf16x4_t, xr[4 * kb + q]);
+#pragma unroll
+            for (int tn = 0; tn < 4; tn++) {
+                f32x4_t acc = B1[tn];
+#pragma unroll
+                for (int kb = 0; kb < 3; kb++)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wt[tn * 3 + kb], xb[kb], acc, 0, 0, 0);
+                a1[tn] = gelu_bf16x4(acc);
             }
-            S.X[s][k] = x;
+#pragma unroll
+            for (int tn = 0; tn < 4; tn++) {
+                f32x4_t acc = B2[tn];
+#pragma unroll
+                for (int kb = 0; kb < 4; kb++)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wt[12 + tn * 4 + kb], a1[kb], acc, 0, 0, 0);
+                a2[tn] = gelu_bf16x4(acc);
+            }
+            f32x4_t acc = B3;
+#pragma unroll
+            for (int kb = 0; kb < 4; kb++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wt[28 + kb], a2[kb], acc, 0, 0, 0);
+            Ru[gq] = acc[0];   // lanes q = 0: outputs 0 and 1 of sample r of group gq
+            Rv[gq] = acc[1];
         }
-        __syncthreads();
-        infer_tile(S);
-        for (int s = threadIdx.x; s < BT; s += NT) {
-            const long long g = g0 + s;
-            if (g >= S2) continue;
-            const int i = (int)(g % a.szI), j = (int)(g / a.szI);
+        wave_lds_sync();   // the tile is rewritten by the next chunk
+        float zu = 0.0f, zv = 0.0f;
+#pragma unroll
+        for (int gq = 0; gq < 4; gq++) {
+            const float tu = __shfl(Ru[gq], r), tv = __shfl(Rv[gq], r);
+            if (q == gq) zu = tu, zv = tv;
+        }
+        if (g < S2) {
             T su = a.sustr[g], sv = a.svstr[g];
             if (i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1) {
-                const float Ru = S.Z3[s][0] * a.sd_u + a.mu_u;
+                const float Rs = zu * a.sd_u + a.mu_u;
                 const T Du = T(0.5) * ((a.h[g - 1] + a.z[g - 1]) + (a.h[g] + a.z[g]));
-                su = su + T(Ru) * Du;
+                su = su + T(Rs) * Du;
             }
             if (i >= 1 && i <= L - 1 && j >= 2 && j <= M - 1) {
-                const float Rv = S.Z3[s][1] * a.sd_v + a.mu_v;
-                const T Dv = T(0.5) * ((a.h[g - a.szI] + a.z[g - a.szI]) + (a.h[g] + a.z[g]));
-                sv = sv + T(Rv) * Dv;
+                const float Rs = zv * a.sd_v + a.mu_v;
+                const T Dv = T(0.5) * ((a.h[g - szI] + a.z[g - szI]) + (a.h[g] + a.z[g]));
+                sv = sv + T(Rs) * Dv;
             }
             a.sustr_eff[g] = su;
             a.svstr_eff[g] = sv;
         }
-        __syncthreads();   // S.X / S.Z3 are reused by the next tile
     }
 }
 
@@ -459,8 +513,9 @@ __global__ __launch_bounds__(NT) void k_closure_stress(const uint16_t *__restric
 // ------------------------------------------------------------------------------------
 struct sf_mlp {
     float *w, *m, *v, *grad, *partial, *ploss, *loss;
-    uint16_t *wb;
-    float *norm;                // mu[45], sd[45], mu_u, sd_u, mu_v, sd_v
+    uint16_t *wb;               // BF16 parameters, parameter order
+    uint16_t *wpk;              // BF16 weights, inference operand order (NWPK)
+    float *norm;                // mu[45], 1 / sd[45] (FP32)
     float ynorm[4];
     int max_blocks;
 };
@@ -481,6 +536,7 @@ extern "C" int sf_mlp_create(int32_t max_blocks, sf_mlp **out)
     if (r == hipSuccess) r = hipMalloc((void **)&p->ploss, sizeof(float) * (size_t)max_blocks);
     if (r == hipSuccess) r = hipMalloc((void **)&p->loss, sizeof(float));
     if (r == hipSuccess) r = hipMalloc((void **)&p->wb, sizeof(uint16_t) * NPARAM);
+    if (r == hipSuccess) r = hipMalloc((void **)&p->wpk, sizeof(uint16_t) * NWPK);
     if (r == hipSuccess) r = hipMalloc((void **)&p->norm, sizeof(float) * (2 * NIN));
     if (r != hipSuccess) {
         (void)hipFree(p->w);
@@ -491,6 +547,7 @@ extern "C" int sf_mlp_create(int32_t max_blocks, sf_mlp **out)
         (void)hipFree(p->ploss);
         (void)hipFree(p->loss);
         (void)hipFree(p->wb);
+        (void)hipFree(p->wpk);
         (void)hipFree(p->norm);
         delete p;
         return sf::hip_fail(r, "sf_mlp_create");
@@ -510,16 +567,20 @@ extern "C" int sf_mlp_destroy(sf_mlp *p)
     (void)hipFree(p->ploss);
     (void)hipFree(p->loss);
     (void)hipFree(p->wb);
+    (void)hipFree(p->wpk);
     (void)hipFree(p->norm);
     delete p;
     return SF_OK;
 }
 
 namespace {
-__global__ void k_to_bf16(const float *__restrict__ w, uint16_t *__restrict__ wb, int n)
+__global__ void k_to_bf16(const float *__restrict__ w, uint16_t *__restrict__ wb, uint16_t *__restrict__ wpk)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) wb[p] = f2bf(w[p]);
+    if (p >= NPARAM) return;
+    wb[p] = f2bf(w[p]);
+    const int slot = wpk_slot(p);
+    if (slot >= 0) wpk[slot] = wb[p];
 }
 }  // namespace
 
@@ -532,7 +593,7 @@ extern "C" int sf_mlp_set_params(sf_mlp *p, const float *host)
     if (r == hipSuccess) r = hipMemset(p->m, 0, sizeof(float) * NPARAM);
     if (r == hipSuccess) r = hipMemset(p->v, 0, sizeof(float) * NPARAM);
     if (r != hipSuccess) return sf::hip_fail(r, "sf_mlp_set_params");
-    hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0, nullptr, p->w, p->wb, NPARAM);
+    hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0, nullptr, p->w, p->wb, p->wpk);
     r = hipDeviceSynchronize();
     if (r != hipSuccess) return sf::hip_fail(r, "sf_mlp_set_params");
     return sf::check_launch("sf_mlp_set_params");
@@ -555,12 +616,18 @@ extern "C" int sf_mlp_get_grad(sf_mlp *p, float *host, float *loss)
     return SF_OK;
 }
 
-/* Input normalisation mu[45], sd[45] and output de-normalisation (mu_u, sd_u, mu_v, sd_v). */
+/* Input normalisation mu[45], sd[45] (features enter the network as (x - mu) * rsd with
+ * rsd = 1 / sd rounded to FP32) and output de-normalisation (mu_u, sd_u, mu_v, sd_v). */
 extern "C" int sf_mlp_set_norm(sf_mlp *p, const float *mu, const float *sd, const float *ynorm)
 {
     SF_REQUIRE(p && mu && sd && ynorm, "sf_mlp_set_norm: null argument");
-    hipError_t r = hipMemcpy(p->norm, mu, sizeof(float) * NIN, hipMemcpyHostToDevice);
-    if (r == hipSuccess) r = hipMemcpy(p->norm + NIN, sd, sizeof(float) * NIN, hipMemcpyHostToDevice);
+    float host[2 * NIN];
+    for (int k = 0; k < NIN; k++) {
+        SF_REQUIRE(sd[k] != 0.0f, "sf_mlp_set_norm: sd[%d] is zero", k);
+        host[k] = mu[k];
+        host[NIN + k] = 1.0f / sd[k];
+    }
+    hipError_t r = hipMemcpy(p->norm, host, sizeof(float) * 2 * NIN, hipMemcpyHostToDevice);
     if (r != hipSuccess) return sf::hip_fail(r, "sf_mlp_set_norm");
     for (int k = 0; k < 4; k++) p->ynorm[k] = ynorm[k];
     return SF_OK;
@@ -585,7 +652,7 @@ extern "C" int sf_mlp_train_step(sf_mlp *p, const uint16_t *X, const float *Y, c
     const float bc1 = adam ? 1.0f - powf(beta1, (float)step) : 1.0f;
     const float bc2 = adam ? 1.0f - powf(beta2, (float)step) : 1.0f;
     hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, s, p->partial, p->ploss,
-                       nblocks, p->grad, p->w, p->m, p->v, p->wb, adam, lr, beta1, beta2, eps, bc1, bc2,
+                       nblocks, p->grad, p->w, p->m, p->v, p->wb, p->wpk, adam, lr, beta1, beta2, eps, bc1, bc2,
                        1.0f / (float)(batch * NOUT), p->loss);
     return sf::check_launch("sf_mlp_train_step");
 }
@@ -615,8 +682,6 @@ ClosureArgs<T> closure_args(sf_mlp *p, const sf_phys_fields *x, const void *f, v
     a.sustr_eff = (T *)sustr_eff;
     a.svstr_eff = (T *)svstr_eff;
     a.feat_out = feat;
-    a.mu = p ? p->norm : nullptr;
-    a.sd = p ? p->norm + NIN : nullptr;
     a.mu_u = p ? p->ynorm[0] : 0.f;
     a.sd_u = p ? p->ynorm[1] : 0.f;
     a.mu_v = p ? p->ynorm[2] : 0.f;
@@ -652,14 +717,18 @@ extern "C" int sf_closure_stress(sf_mlp *p, const sf_phys_fields *x, int32_t pre
 {
     SF_REQUIRE(p && x && f && sustr_eff && svstr_eff, "sf_closure_stress: null argument");
     const long long S2 = (long long)x->szI * x->szJ;
-    const long long ntiles = (S2 + BT - 1) / BT;
-    // persistent grid: the weights are staged once per workgroup (3 workgroups per CU)
-    const dim3 grid((unsigned)(ntiles < 768 ? ntiles : 768)), block(NT);
+    SF_REQUIRE(x->szI >= 3 && x->szJ >= 3 && S2 < (1LL << 30), "sf_closure_stress: grid %d x %d", x->szI, x->szJ);
+    const long long nchunk = (S2 + 63) / 64;
+    // persistent grid of 4-wave workgroups, one 64-point chunk per wave per pass; the
+    // weights are staged through LDS once per workgroup
+    const long long nblk = (nchunk + 3) / 4;
+    const dim3 grid((unsigned)(nblk < 1024 ? nblk : 1024)), block(NT);
+    const uint64_t *wpk = (const uint64_t *)p->wpk;
     if (precision == SF_FP64)
-        hipLaunchKernelGGL(k_closure_stress<double>, grid, block, 0, sf::as_stream(stream), p->wb, p->w,
+        hipLaunchKernelGGL(k_closure_stress<double>, grid, block, 0, sf::as_stream(stream), wpk, p->w, p->norm,
                            closure_args<double>(p, x, f, sustr_eff, svstr_eff, nullptr));
     else
-        hipLaunchKernelGGL(k_closure_stress<float>, grid, block, 0, sf::as_stream(stream), p->wb, p->w,
+        hipLaunchKernelGGL(k_closure_stress<float>, grid, block, 0, sf::as_stream(stream), wpk, p->w, p->norm,
                            closure_args<float>(p, x, f, sustr_eff, svstr_eff, nullptr));
     return sf::check_launch("sf_closure_stress");
 }

@@ -1078,10 +1078,13 @@ __global__ __launch_bounds__(NT) void k_phys_ensemble(const PFA<T> *__restrict__
 }
 }  // namespace
 
-extern "C" int sf_phys_fused_default_tile(int32_t precision)
+extern "C" int sf_phys_fused_default_tile(int32_t precision, int32_t szI, int32_t szJ)
 {
     (void)precision;
-    return 8;
+    // 64 x 8 tiles, unless that leaves fewer than 4 workgroups per CU (256 CUs): small
+    // grids run faster on 64 x 4 tiles (twice the workgroups to hide latency)
+    const long long tiles8 = (long long)((szI + 63) / 64) * ((szJ + 7) / 8);
+    return tiles8 < 1024 ? 4 : 8;
 }
 
 extern "C" int sf_phys_substep_fused(const sf_phys_fields *x, int32_t precision, double dt, int32_t first,
@@ -1095,7 +1098,7 @@ extern "C" int sf_phys_substep_fused(const sf_phys_fields *x, int32_t precision,
     SF_REQUIRE(x->rzeta[0] && x->rzeta[1] && x->rubar[0] && x->rubar[1] && x->rvbar[0] && x->rvbar[1],
                "sf_phys_substep_fused: null rhs slot");
     SF_REQUIRE(x->Zt_avg1 && x->DU_avg1 && x->DV_avg1, "sf_phys_substep_fused: null average");
-    const int tj = tile_j > 0 ? tile_j : sf_phys_fused_default_tile(precision);
+    const int tj = tile_j > 0 ? tile_j : sf_phys_fused_default_tile(precision, x->szI, x->szJ);
     if (precision == SF_FP64) return dispatch_tj<double>(x, dt, first, cff1, tj, stream);
     if (precision == SF_FP32) return dispatch_tj<float>(x, dt, first, cff1, tj, stream);
     sf::set_error("sf_phys_substep_fused: unknown precision %d", precision);

@@ -33,7 +33,7 @@ def _bind():
     lib.sf_phys_substep_fused.argtypes = [C.POINTER(sf_phys_fields), C.c_int32, C.c_double, C.c_int32, C.c_double,
                                           C.c_int32, P]
     lib.sf_phys_substep_fused.restype = C.c_int32
-    lib.sf_phys_fused_default_tile.argtypes = [C.c_int32]
+    lib.sf_phys_fused_default_tile.argtypes = [C.c_int32, C.c_int32, C.c_int32]
     lib.sf_phys_fused_default_tile.restype = C.c_int32
     return lib
 

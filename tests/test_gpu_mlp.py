@@ -197,3 +197,51 @@ def test_closure_stress_constant_output():
     assert np.array_equal(su.reshape(J, I).cpu().numpy(), eu)
     assert np.array_equal(sv.reshape(J, I).cpu().numpy(), ev)
     mlp.close()
+
+
+@pytest.mark.parametrize("shape,dtype", [((37, 21), "f64"), ((700, 420), "f32")])
+def test_closure_stress_matches_network_forward(shape, dtype):
+    """Random network, features normalised by their own statistics, zero wind stress
+    and identity output scaling: the closure stress equals R * D, with R the network
+    forward kernel's output on the packed features of the same state, bit for bit
+    (same MFMA sums, GELU and BF16 rounding).  The 700 x 420 grid has more 64-point
+    chunks than the persistent grid has waves, so waves take several chunks."""
+    import ctypes as C
+    import torch
+    from sailfish_amd import _lib, configs
+    from sailfish_amd.closure import ClosureMLP, Model, closure_buffers, pack_inputs
+    print(f"[case] closure stress vs forward {shape} {dtype}", file=sys.stderr, flush=True)
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    m = Model(configs.random_smooth(*shape, seed=9), dtype=tdt)
+    st = m.stepper
+    st.static["sustr"].zero_()
+    st.static["svstr"].zero_()
+    F = m.features()
+    mu, sd = F.mean(0), F.std(0)
+    sd = torch.where(sd > 0, sd, torch.ones_like(sd))
+    mlp = ClosureMLP(params=_params(11))
+    mlp.set_norm(mu.cpu().numpy(), sd.cpu().numpy(), np.array([0.0, 1.0, 0.0, 1.0]))
+    Y = mlp.forward(pack_inputs(F, mu, sd)).cpu().numpy()
+    su, sv = closure_buffers(m)
+    a = st.fields(m.OCEAN)
+    _lib.check(mlp.lib.sf_closure_stress(mlp.handle, C.byref(a), st.precision, m.f.data_ptr(), su.data_ptr(),
+                                         sv.data_ptr(), None), "sf_closure_stress")
+    torch.cuda.synchronize()
+    J, I = m.shape
+    npdt = np.float64 if dtype == "f64" else np.float32
+    h = st.static["h"].reshape(J, I).cpu().numpy()
+    z = m.OCEAN.zeta_t1.reshape(J, I).cpu().numpy()
+    Ru = Y[:, 0].reshape(J, I).astype(npdt)
+    Rv = Y[:, 1].reshape(J, I).astype(npdt)
+    half = npdt(0.5)
+    eu = np.zeros((J, I), npdt)
+    ev = np.zeros((J, I), npdt)
+    Du = half * ((h[1:-1, 1:-2] + z[1:-1, 1:-2]) + (h[1:-1, 2:-1] + z[1:-1, 2:-1]))
+    eu[1:-1, 2:-1] = Ru[1:-1, 2:-1] * Du
+    Dv = half * ((h[1:-2, 1:-1] + z[1:-2, 1:-1]) + (h[2:-1, 1:-1] + z[2:-1, 1:-1]))
+    ev[2:-1, 1:-1] = Rv[2:-1, 1:-1] * Dv
+    gu, gv = su.reshape(J, I).cpu().numpy(), sv.reshape(J, I).cpu().numpy()
+    assert np.abs(Y).max() > 0.1
+    assert np.array_equal(gu, eu), np.abs(gu - eu).max()
+    assert np.array_equal(gv, ev), np.abs(gv - ev).max()
+    mlp.close()
