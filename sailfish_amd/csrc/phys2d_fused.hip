@@ -194,10 +194,10 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     };
 
     // metric factors of a u point / v point at byte offset o
-    auto onu_at = [&](uint32_t o) { return T(2) / (ldo(P.pn, -1, o) + ldo(P.pn, 0, o)); };
-    auto omu_at = [&](uint32_t o) { return T(2) / (ldo(P.pm, -1, o) + ldo(P.pm, 0, o)); };
-    auto omv_at = [&](uint32_t o) { return T(2) / (ldo(P.pm, -W, o) + ldo(P.pm, 0, o)); };
-    auto onv_at = [&](uint32_t o) { return T(2) / (ldo(P.pn, -W, o) + ldo(P.pn, 0, o)); };
+    auto onu_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(P.pn, -1, o) + ldo(P.pn, 0, o)); };
+    auto omu_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(P.pm, -1, o) + ldo(P.pm, 0, o)); };
+    auto omv_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(P.pm, -W, o) + ldo(P.pm, 0, o)); };
+    auto onv_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(P.pn, -W, o) + ldo(P.pn, 0, o)); };
     auto cffu_at = [&](uint32_t o) {
         return (ldo(P.pm, 0, o) + ldo(P.pm, -1, o)) * (ldo(P.pn, 0, o) + ldo(P.pn, -1, o));
     };
@@ -259,13 +259,13 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
 
     // metric factors of generic inputs (same expressions as onu_at / omu_at / cffu_at ...)
     auto metrics_u = [&](UVIn<T> &x) {
-        x.m1 = T(2) / (x.pnb + x.pnc);
-        x.m2 = T(2) / (x.pmb + x.pmc);
+        x.m1 = T(2) * sf::recip(x.pnb + x.pnc);
+        x.m2 = T(2) * sf::recip(x.pmb + x.pmc);
         x.cff = (x.pmc + x.pmb) * (x.pnc + x.pnb);
     };
     auto metrics_v = [&](UVIn<T> &x) {
-        x.m1 = T(2) / (x.pmb + x.pmc);
-        x.m2 = T(2) / (x.pnb + x.pnc);
+        x.m1 = T(2) * sf::recip(x.pmb + x.pmc);
+        x.m2 = T(2) * sf::recip(x.pnb + x.pnc);
         x.cff = (x.pmc + x.pmb) * (x.pnc + x.pnb);
     };
 
@@ -288,7 +288,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         r = r + T(0.5) * (cr1 + cr0);
         r = r - ((P.rdrg * x.xc) * omu) * onu;
         r = r + (x.st * omu) * onu;
-        const T fac = T(1) / ((x.hc + sZS[q]) + (x.hb + sZS[q - 1]));
+        const T fac = sf::recip((x.hc + sZS[q]) + (x.hb + sZS[q - 1]));
         return (x.x0 * (Dbp(q, x.hc, x.z0c) + Dbp(q - 1, x.hb, x.z0b)) + (x.cff * c1dt) * r) * fac;
     };
     auto pred_v = [&](int q, const UVIn<T> &x, T &r) {
@@ -305,7 +305,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         r = r - T(0.5) * (cr1 + cr0);
         r = r - ((P.rdrg * x.xc) * omv) * onv;
         r = r + (x.st * omv) * onv;
-        const T fac = T(1) / ((x.hc + sZS[q]) + (x.hb + sZS[q - FW]));
+        const T fac = sf::recip((x.hc + sZS[q]) + (x.hb + sZS[q - FW]));
         return (x.x0 * (Dbp(q, x.hc, x.z0c) + Dbp(q - FW, x.hb, x.z0b)) + (x.cff * c1dt) * r) * fac;
     };
 
@@ -433,14 +433,14 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             if (cin[k]) {
                 pmnc[k] = pm_[k] * pn_[k];
                 if (i >= 1) {
-                    onuc[k] = T(2) / (pnw_[k] + pn_[k]);
-                    omuc[k] = T(2) / (pmw_[k] + pm_[k]);
+                    onuc[k] = T(2) * sf::recip(pnw_[k] + pn_[k]);
+                    omuc[k] = T(2) * sf::recip(pmw_[k] + pm_[k]);
                     cffuc[k] = (pm_[k] + pmw_[k]) * (pn_[k] + pnw_[k]);
                     du = ((T(0.5) * ((hw_[k] + sZ1[o - 1]) + (hc_[k] + sZ1[o]))) * u1c[k]) * onuc[k];
                 }
                 if (j >= 1) {
-                    omvc[k] = T(2) / (pms_[k] + pm_[k]);
-                    onvc[k] = T(2) / (pns_[k] + pn_[k]);
+                    omvc[k] = T(2) * sf::recip(pms_[k] + pm_[k]);
+                    onvc[k] = T(2) * sf::recip(pns_[k] + pn_[k]);
                     cffvc[k] = (pm_[k] + pms_[k]) * (pn_[k] + pns_[k]);
                     dv = ((T(0.5) * ((hs_[k] + sZ1[o - FW]) + (hc_[k] + sZ1[o]))) * v1c[k]) * omvc[k];
                 }
@@ -455,11 +455,11 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const int o = ro[s];
             T du = T(0), dv = T(0);
             if (ri[s] >= 1) {
-                const T onu = T(2) / (rpnw[s] + rpn[s]);
+                const T onu = T(2) * sf::recip(rpnw[s] + rpn[s]);
                 du = ((T(0.5) * ((rhw[s] + sZ1[o - 1]) + (rh[s] + sZ1[o]))) * ru[s]) * onu;
             }
             if (rj[s] >= 1) {
-                const T omv = T(2) / (rpms[s] + rpm[s]);
+                const T omv = T(2) * sf::recip(rpms[s] + rpm[s]);
                 dv = ((T(0.5) * ((rhs[s] + sZ1[o - FW]) + (rh[s] + sZ1[o]))) * rv[s]) * omv;
             }
             sDU[o] = du;
@@ -773,11 +773,11 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const int o = ro[s];
             T du = T(0), dv = T(0);
             if (ri[s] >= 1) {
-                const T onu = T(2) / (rpnw[s] + rpn[s]);
+                const T onu = T(2) * sf::recip(rpnw[s] + rpn[s]);
                 du = ((T(0.5) * ((rhw[s] + sZS[o - 1]) + (rh[s] + sZS[o]))) * sUS[o]) * onu;
             }
             if (rj[s] >= 1) {
-                const T omv = T(2) / (rpms[s] + rpm[s]);
+                const T omv = T(2) * sf::recip(rpms[s] + rpm[s]);
                 dv = ((T(0.5) * ((rhs[s] + sZS[o - FW]) + (rh[s] + sZS[o]))) * sVS[o]) * omv;
             }
             sDU[o] = du;
@@ -880,7 +880,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         r = r + T(0.5) * (cr1 + cr0);
         r = r - ((P.rdrg * uc) * omu) * onu;
         r = r + (st * omu) * onu;
-        const T fac = T(1) / ((hc + sZ2[q]) + (hb + sZ2[q - 1]));
+        const T fac = sf::recip((hc + sZ2[q]) + (hb + sZ2[q - 1]));
         T tend;
         if (first) tend = (T(0.5) * dt) * r;
         else tend = (b1 * r + b2 * ru1) - b3 * ru0;
@@ -901,7 +901,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         r = r - T(0.5) * (cr1 + cr0);
         r = r - ((P.rdrg * vc) * omv) * onv;
         r = r + (st * omv) * onv;
-        const T fac = T(1) / ((hc + sZ2[q]) + (hb + sZ2[q - FW]));
+        const T fac = sf::recip((hc + sZ2[q]) + (hb + sZ2[q - FW]));
         T tend;
         if (first) tend = (T(0.5) * dt) * r;
         else tend = (b1 * r + b2 * rv1) - b3 * rv0;

@@ -37,7 +37,7 @@ __device__ __forceinline__ T Dat(const PF<T> &p, const T *z, long long f) { retu
 template <typename T>
 __device__ __forceinline__ T DUon(const PF<T> &p, const T *z, const T *u, long long f)
 {
-    const T onu = T(2) / (p.pn[f - 1] + p.pn[f]);
+    const T onu = T(2) * sf::recip(p.pn[f - 1] + p.pn[f]);
     return ((T(0.5) * (Dat(p, z, f - 1) + Dat(p, z, f))) * u[f]) * onu;
 }
 
@@ -45,7 +45,7 @@ template <typename T>
 __device__ __forceinline__ T DVom(const PF<T> &p, const T *z, const T *v, long long f)
 {
     const long long W = p.szI;
-    const T omv = T(2) / (p.pm[f - W] + p.pm[f]);
+    const T omv = T(2) * sf::recip(p.pm[f - W] + p.pm[f]);
     return ((T(0.5) * (Dat(p, z, f - W) + Dat(p, z, f))) * v[f]) * omv;
 }
 
@@ -61,8 +61,8 @@ __device__ T rhs_u(const PF<T> &p, State3<T> s, const T *gz, long long f)
 {
     const long long W = p.szI;
     const T *z = s.z, *u = s.u, *v = s.v;
-    const T onu = T(2) / (p.pn[f - 1] + p.pn[f]);
-    const T omu = T(2) / (p.pm[f - 1] + p.pm[f]);
+    const T onu = T(2) * sf::recip(p.pn[f - 1] + p.pn[f]);
+    const T omu = T(2) * sf::recip(p.pm[f - 1] + p.pm[f]);
     const T g0 = gz[f - 1], g1 = gz[f];
     T r = ((T(0.5) * p.g) * onu) * ((p.h[f - 1] + p.h[f]) * (g0 - g1) + (g0 * g0 - g1 * g1));
     const T UFx1 = T(0.25) * (DUon(p, z, u, f) + DUon(p, z, u, f + 1)) * (u[f] + u[f + 1]);
@@ -83,8 +83,8 @@ __device__ T rhs_v(const PF<T> &p, State3<T> s, const T *gz, long long f)
 {
     const long long W = p.szI;
     const T *z = s.z, *u = s.u, *v = s.v;
-    const T omv = T(2) / (p.pm[f - W] + p.pm[f]);
-    const T onv = T(2) / (p.pn[f - W] + p.pn[f]);
+    const T omv = T(2) * sf::recip(p.pm[f - W] + p.pm[f]);
+    const T onv = T(2) * sf::recip(p.pn[f - W] + p.pn[f]);
     const T g0 = gz[f - W], g1 = gz[f];
     T r = ((T(0.5) * p.g) * omv) * ((p.h[f - W] + p.h[f]) * (g0 - g1) + (g0 * g0 - g1 * g1));
     const T VFx1 = T(0.25) * (DUon(p, z, u, f + 1) + DUon(p, z, u, f + 1 - W)) * (v[f + 1] + v[f]);
@@ -155,7 +155,7 @@ __device__ T pred_u(const Sub<T> &a, long long f, bool store_rhs)
     const T r = rhs_u(a.p, s1, a.gz, f);
     if (store_rhs) a.ru1[f] = r;
     const T cff = (a.p.pm[f] + a.p.pm[f - 1]) * (a.p.pn[f] + a.p.pn[f - 1]);
-    const T fac = T(1) / (Dat(a.p, (const T *)a.zs, f) + Dat(a.p, (const T *)a.zs, f - 1));
+    const T fac = sf::recip(Dat(a.p, (const T *)a.zs, f) + Dat(a.p, (const T *)a.zs, f - 1));
     const T *zb = a.first ? a.z1 : a.z0;
     const T *ub = a.first ? a.u1 : a.u0;
     const T c1 = a.first ? T(0.5) * a.dt : a.dt;
@@ -171,7 +171,7 @@ __device__ T pred_v(const Sub<T> &a, long long f, bool store_rhs)
     const T r = rhs_v(a.p, s1, a.gz, f);
     if (store_rhs) a.rv1[f] = r;
     const T cff = (a.p.pm[f] + a.p.pm[f - W]) * (a.p.pn[f] + a.p.pn[f - W]);
-    const T fac = T(1) / (Dat(a.p, (const T *)a.zs, f) + Dat(a.p, (const T *)a.zs, f - W));
+    const T fac = sf::recip(Dat(a.p, (const T *)a.zs, f) + Dat(a.p, (const T *)a.zs, f - W));
     const T *zb = a.first ? a.z1 : a.z0;
     const T *vb = a.first ? a.v1 : a.v0;
     const T c1 = a.first ? T(0.5) * a.dt : a.dt;
@@ -244,7 +244,7 @@ __device__ T corr_u(const Sub<T> &a, long long f)
     State3<T> ss{a.zs, a.us, a.vs};
     const T r = rhs_u(a.p, ss, a.gz, f);
     const T cff = (a.p.pm[f] + a.p.pm[f - 1]) * (a.p.pn[f] + a.p.pn[f - 1]);
-    const T fac = T(1) / (Dat(a.p, (const T *)a.z2, f) + Dat(a.p, (const T *)a.z2, f - 1));
+    const T fac = sf::recip(Dat(a.p, (const T *)a.z2, f) + Dat(a.p, (const T *)a.z2, f - 1));
     const T b1 = T(0.5) * a.dt * T(5.0) / T(12.0), b2 = T(0.5) * a.dt * T(8.0) / T(12.0),
             b3 = T(0.5) * a.dt * T(1.0) / T(12.0);
     const T tend = a.first ? (T(0.5) * a.dt) * r : (b1 * r + b2 * a.ru1[f]) - b3 * a.ru0[f];
@@ -258,7 +258,7 @@ __device__ T corr_v(const Sub<T> &a, long long f)
     State3<T> ss{a.zs, a.us, a.vs};
     const T r = rhs_v(a.p, ss, a.gz, f);
     const T cff = (a.p.pm[f] + a.p.pm[f - W]) * (a.p.pn[f] + a.p.pn[f - W]);
-    const T fac = T(1) / (Dat(a.p, (const T *)a.z2, f) + Dat(a.p, (const T *)a.z2, f - W));
+    const T fac = sf::recip(Dat(a.p, (const T *)a.z2, f) + Dat(a.p, (const T *)a.z2, f - W));
     const T b1 = T(0.5) * a.dt * T(5.0) / T(12.0), b2 = T(0.5) * a.dt * T(8.0) / T(12.0),
             b3 = T(0.5) * a.dt * T(1.0) / T(12.0);
     const T tend = a.first ? (T(0.5) * a.dt) * r : (b1 * r + b2 * a.rv1[f]) - b3 * a.rv0[f];
