@@ -96,7 +96,7 @@ __device__ __forceinline__ void for_region(int tid, F &&fn)
 
 template <int TI, int TJ>
 struct PhysLds {
-    static constexpr int FW = TI + 5, FH = TJ + 5, FN = FW * FH, PLANES = 7;
+    static constexpr int FW = TI + 5, FH = TJ + 5, FN = FW * FH, PLANES = 9;
 };
 
 // Element (f + d) of p, given the byte offset boff of element f: the address is the
@@ -173,6 +173,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     T *const sZ1 = sm[0], *const sZ2 = sm[0];
     T *const sDU = sm[1], *const sDV = sm[2];
     T *const sZS = sm[3], *const sRZ = sm[4], *const sUS = sm[5], *const sVS = sm[6];
+    T *const sU1 = sm[7], *const sV1 = sm[8];   // ubar, vbar at t1 over the frame (read-only after L)
     // predictor right-hand sides of the ghost-source row (j = 1, M-1) / column (i = 1, L-1)
     T *const sRUlo = side, *const sRUhi = side + TI, *const sRVlo = side + 2 * TI, *const sRVhi = side + 2 * TI + TJ;
 
@@ -206,7 +207,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     };
 
     // generic (load-on-demand) inputs of the u / v right-hand side at offset f
-    auto load_u = [&](uint32_t f) {
+    auto load_u = [&](uint32_t f, int q) {
         UVIn<T> x;
         x.pmc = ldo(P.pm, 0, f);
         x.pmb = ldo(P.pm, -1, f);
@@ -214,24 +215,24 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         x.pnb = ldo(P.pn, -1, f);
         x.hb = ldo(P.h, -1, f);
         x.hc = ldo(P.h, 0, f);
-        x.xb = ldo(P.u1, -1, f);
-        x.xc = ldo(P.u1, 0, f);
-        x.xf = ldo(P.u1, 1, f);
-        x.yf = ldo(P.u1, W, f);
-        x.yb = ldo(P.u1, -W, f);
+        x.xb = sU1[q - 1];
+        x.xc = sU1[q];
+        x.xf = sU1[q + 1];
+        x.yf = sU1[q + FW];
+        x.yb = sU1[q - FW];
         x.fc = ldo(P.fomn, 0, f);
         x.fb = ldo(P.fomn, -1, f);
-        x.oc = ldo(P.v1, 0, f);
-        x.of = ldo(P.v1, W, f);
-        x.ob = ldo(P.v1, -1, f);
-        x.obf = ldo(P.v1, W - 1, f);
+        x.oc = sV1[q];
+        x.of = sV1[q + FW];
+        x.ob = sV1[q - 1];
+        x.obf = sV1[q + FW - 1];
         x.st = ldo(P.sustr, 0, f);
         x.x0 = first ? x.xc : ldo(P.u0, 0, f);
         x.z0c = first ? T(0) : ldo(P.z0, 0, f);
         x.z0b = first ? T(0) : ldo(P.z0, -1, f);
         return x;
     };
-    auto load_v = [&](uint32_t f) {
+    auto load_v = [&](uint32_t f, int q) {
         UVIn<T> x;
         x.pmc = ldo(P.pm, 0, f);
         x.pmb = ldo(P.pm, -W, f);
@@ -239,17 +240,17 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         x.pnb = ldo(P.pn, -W, f);
         x.hb = ldo(P.h, -W, f);
         x.hc = ldo(P.h, 0, f);
-        x.xb = ldo(P.v1, -W, f);
-        x.xc = ldo(P.v1, 0, f);
-        x.xf = ldo(P.v1, W, f);
-        x.yf = ldo(P.v1, 1, f);
-        x.yb = ldo(P.v1, -1, f);
+        x.xb = sV1[q - FW];
+        x.xc = sV1[q];
+        x.xf = sV1[q + FW];
+        x.yf = sV1[q + 1];
+        x.yb = sV1[q - 1];
         x.fc = ldo(P.fomn, 0, f);
         x.fb = ldo(P.fomn, -W, f);
-        x.oc = ldo(P.u1, 0, f);
-        x.of = ldo(P.u1, 1, f);
-        x.ob = ldo(P.u1, -W, f);
-        x.obf = ldo(P.u1, 1 - W, f);
+        x.oc = sU1[q];
+        x.of = sU1[q + 1];
+        x.ob = sU1[q - FW];
+        x.obf = sU1[q + 1 - FW];
         x.st = ldo(P.svstr, 0, f);
         x.x0 = first ? x.xc : ldo(P.v0, 0, f);
         x.z0c = first ? T(0) : ldo(P.z0, 0, f);
@@ -318,7 +319,6 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     T zsc[KC], ruc[KC], rvc[KC], duc[KC], dvc[KC];
     // G1 (predictor) and G2 (corrector) prefetch
     T z0c_[KC], z0w_[KC], z0s_[KC], u0c_[KC], v0c_[KC], fo_[KC], fow_[KC], fos_[KC], su_[KC], sv_[KC];
-    T u1w_[KC], u1e_[KC], u1n_[KC], u1s_[KC], u1es_[KC], v1w_[KC], v1e_[KC], v1n_[KC], v1s_[KC], v1wn_[KC];
     T rz0_[KC], ru0_[KC], rv0_[KC], z1w_[KC], z1s_[KC];
     double zt_[KC], dua_[KC], dva_[KC];
 
@@ -360,30 +360,42 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     {
         constexpr int RING = ring_size<TI, TJ, 3, 2>();
         constexpr int RS = (RING + NT - 1) / NT;
-        T rz[RS];
+        T rz[RS], ru[RS], rv[RS];
         int ro[RS];
 #pragma unroll
         for (int s = 0; s < RS; s++) {
             const int t = tid + s * NT;
-            rz[s] = T(0);
+            rz[s] = ru[s] = rv[s] = T(0);
             ro[s] = -1;
             if (t < RING) {
                 int a, b;
                 ring_point<TI, TJ, 3, 2>(t, a, b);
                 const int i = fi0 + a, j = fj0 + b;
                 ro[s] = b * FW + a;
-                if (inside(i, j)) rz[s] = ldo(P.z1, 0, off(i, j));
+                if (inside(i, j)) {
+                    const uint32_t f = off(i, j);
+                    rz[s] = ldo(P.z1, 0, f);
+                    ru[s] = ldo(P.u1, 0, f);
+                    rv[s] = ldo(P.v1, 0, f);
+                }
             }
         }
 #pragma unroll
         for (int k = 0; k < KC; k++) {
             int a, b;
             core_ab(k, a, b);
-            sZ1[b * FW + a] = z1c[k];
+            const int o = b * FW + a;
+            sZ1[o] = z1c[k];
+            sU1[o] = cin[k] ? u1c[k] : T(0);
+            sV1[o] = cin[k] ? v1c[k] : T(0);
         }
 #pragma unroll
         for (int s = 0; s < RS; s++)
-            if (ro[s] >= 0) sZ1[ro[s]] = rz[s];
+            if (ro[s] >= 0) {
+                sZ1[ro[s]] = rz[s];
+                sU1[ro[s]] = ru[s];
+                sV1[ro[s]] = rv[s];
+            }
     }
     lds_barrier();
 
@@ -391,7 +403,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     {
         constexpr int RING = ring_size<TI, TJ, 2, 2>();
         constexpr int RS = (RING + NT - 1) / NT;
-        T ru[RS], rv[RS], rh[RS], rhw[RS], rhs[RS], rpm[RS], rpms[RS], rpn[RS], rpnw[RS];
+        T rh[RS], rhw[RS], rhs[RS], rpm[RS], rpms[RS], rpn[RS], rpnw[RS];
         int ro[RS], ri[RS], rj[RS];
 #pragma unroll
         for (int s = 0; s < RS; s++) {
@@ -407,8 +419,6 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     ri[s] = i;
                     rj[s] = j;
                     const uint32_t f = off(i, j);
-                    ru[s] = ldo(P.u1, 0, f);
-                    rv[s] = ldo(P.v1, 0, f);
                     rh[s] = ldo(P.h, 0, f);
                     rpm[s] = ldo(P.pm, 0, f);
                     rpn[s] = ldo(P.pn, 0, f);
@@ -456,11 +466,11 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             T du = T(0), dv = T(0);
             if (ri[s] >= 1) {
                 const T onu = T(2) * sf::recip(rpnw[s] + rpn[s]);
-                du = ((T(0.5) * ((rhw[s] + sZ1[o - 1]) + (rh[s] + sZ1[o]))) * ru[s]) * onu;
+                du = ((T(0.5) * ((rhw[s] + sZ1[o - 1]) + (rh[s] + sZ1[o]))) * sU1[o]) * onu;
             }
             if (rj[s] >= 1) {
                 const T omv = T(2) * sf::recip(rpms[s] + rpm[s]);
-                dv = ((T(0.5) * ((rhs[s] + sZ1[o - FW]) + (rh[s] + sZ1[o]))) * rv[s]) * omv;
+                dv = ((T(0.5) * ((rhs[s] + sZ1[o - FW]) + (rh[s] + sZ1[o]))) * sV1[o]) * omv;
             }
             sDU[o] = du;
             sDV[o] = dv;
@@ -485,13 +495,6 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 }
                 fow_[k] = ldo(P.fomn, -1, f);
                 su_[k] = ldo(P.sustr, 0, f);
-                u1w_[k] = ldo(P.u1, -1, f);
-                u1e_[k] = ldo(P.u1, 1, f);
-                u1n_[k] = ldo(P.u1, W, f);
-                u1s_[k] = ldo(P.u1, -W, f);
-                v1w_[k] = ldo(P.v1, -1, f);
-                v1wn_[k] = ldo(P.v1, W - 1, f);
-                v1n_[k] = ldo(P.v1, W, f);
             }
             if (vi) {
                 if (!first) {
@@ -500,13 +503,6 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 }
                 fos_[k] = ldo(P.fomn, -W, f);
                 sv_[k] = ldo(P.svstr, 0, f);
-                v1e_[k] = ldo(P.v1, 1, f);
-                v1w_[k] = ldo(P.v1, -1, f);
-                v1n_[k] = ldo(P.v1, W, f);
-                v1s_[k] = ldo(P.v1, -W, f);
-                u1e_[k] = ldo(P.u1, 1, f);
-                u1s_[k] = ldo(P.u1, -W, f);
-                u1es_[k] = ldo(P.u1, 1 - W, f);
             }
         }
     lds_barrier();
@@ -598,17 +594,17 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     x.cff = cffuc[k];
                     x.hb = hw_[k];
                     x.hc = hc_[k];
-                    x.xb = u1w_[k];
+                    x.xb = sU1[o - 1];
                     x.xc = u1c[k];
-                    x.xf = u1e_[k];
-                    x.yf = u1n_[k];
-                    x.yb = u1s_[k];
+                    x.xf = sU1[o + 1];
+                    x.yf = sU1[o + FW];
+                    x.yb = sU1[o - FW];
                     x.fc = fo_[k];
                     x.fb = fow_[k];
                     x.oc = v1c[k];
-                    x.of = v1n_[k];
-                    x.ob = v1w_[k];
-                    x.obf = v1wn_[k];
+                    x.of = sV1[o + FW];
+                    x.ob = sV1[o - 1];
+                    x.obf = sV1[o + FW - 1];
                     x.st = su_[k];
                     x.x0 = first ? u1c[k] : u0c_[k];
                     x.z0c = z0c_[k];
@@ -622,7 +618,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 } else if (i >= 2 && i <= L - 1) {   // ghost row: source (i, 1) or (i, M-1)
                     const int qj = clampi(j, 1, M - 1);
                     T r;
-                    UVIn<T> x = load_u(off(i, qj));
+                    UVIn<T> x = load_u(off(i, qj), o + (qj - j) * FW);
                     metrics_u(x);
                     us = pred_u(o + (qj - j) * FW, x, r);
                 }
@@ -633,17 +629,17 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     x.cff = cffvc[k];
                     x.hb = hs_[k];
                     x.hc = hc_[k];
-                    x.xb = v1s_[k];
+                    x.xb = sV1[o - FW];
                     x.xc = v1c[k];
-                    x.xf = v1n_[k];
-                    x.yf = v1e_[k];
-                    x.yb = v1w_[k];
+                    x.xf = sV1[o + FW];
+                    x.yf = sV1[o + 1];
+                    x.yb = sV1[o - 1];
                     x.fc = fo_[k];
                     x.fb = fos_[k];
                     x.oc = u1c[k];
-                    x.of = u1e_[k];
-                    x.ob = u1s_[k];
-                    x.obf = u1es_[k];
+                    x.of = sU1[o + 1];
+                    x.ob = sU1[o - FW];
+                    x.obf = sU1[o + 1 - FW];
                     x.st = sv_[k];
                     x.x0 = first ? v1c[k] : v0c_[k];
                     x.z0c = z0c_[k];
@@ -657,7 +653,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 } else if (j >= 2 && j <= M - 1) {   // ghost column: source (1, j) or (L-1, j)
                     const int qi = clampi(i, 1, L - 1);
                     T r;
-                    UVIn<T> x = load_v(off(qi, j));
+                    UVIn<T> x = load_v(off(qi, j), o + (qi - i));
                     metrics_v(x);
                     vs = pred_v(o + (qi - i), x, r);
                 }
@@ -701,13 +697,13 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             if (inside(i, j)) {
                 if (i >= 2 && i <= L - 1) {
                     const int qj = clampi(j, 1, M - 1);
-                    UVIn<T> x = load_u(off(i, qj));
+                    UVIn<T> x = load_u(off(i, qj), o + (qj - j) * FW);
                     metrics_u(x);
                     us = pred_u(o + (qj - j) * FW, x, r);
                 }
                 if (j >= 2 && j <= M - 1) {
                     const int qi = clampi(i, 1, L - 1);
-                    UVIn<T> x = load_v(off(qi, j));
+                    UVIn<T> x = load_v(off(qi, j), o + (qi - i));
                     metrics_v(x);
                     vs = pred_v(o + (qi - i), x, r);
                 }
@@ -924,7 +920,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const T ru0 = first ? T(0) : ldo(P.ru0, 0, f);
             u2 = corr_u(o + (qj - j) * FW, onu_at(f), omu_at(f), cffu_at(f), ldo(P.h, -1, f), ldo(P.h, 0, f),
                         ldo(P.fomn, 0, f), ldo(P.fomn, -1, f), ldo(P.sustr, 0, f), ru1, ru0, ldo(P.z1, 0, f),
-                        ldo(P.z1, -1, f), ldo(P.u1, 0, f));
+                        ldo(P.z1, -1, f), sU1[o + (qj - j) * FW]);
         }
         sto(P.u2, cf[k], u2);
         if (v_int(i, j)) {
@@ -937,7 +933,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const T rv0 = first ? T(0) : ldo(P.rv0, 0, f);
             v2 = corr_v(o + (qi - i), omv_at(f), onv_at(f), cffv_at(f), ldo(P.h, -W, f), ldo(P.h, 0, f),
                         ldo(P.fomn, 0, f), ldo(P.fomn, -W, f), ldo(P.svstr, 0, f), rv1, rv0, ldo(P.z1, 0, f),
-                        ldo(P.z1, -W, f), ldo(P.v1, 0, f));
+                        ldo(P.z1, -W, f), sV1[o + (qi - i)]);
         }
         sto(P.v2, cf[k], v2);
     }
