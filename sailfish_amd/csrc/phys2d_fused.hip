@@ -96,7 +96,7 @@ __device__ __forceinline__ void for_region(int tid, F &&fn)
 
 template <int TI, int TJ>
 struct PhysLds {
-    static constexpr int FW = TI + 5, FH = TJ + 5, FN = FW * FH, PLANES = 9;
+    static constexpr int FW = TI + 5, FH = TJ + 5, FN = FW * FH, PLANES = 11;
 };
 
 // Element (f + d) of p, given the byte offset boff of element f: the address is the
@@ -174,6 +174,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     T *const sDU = sm[1], *const sDV = sm[2];
     T *const sZS = sm[3], *const sRZ = sm[4], *const sUS = sm[5], *const sVS = sm[6];
     T *const sU1 = sm[7], *const sV1 = sm[8];   // ubar, vbar at t1 over the frame (read-only after L)
+    T *const sPM = sm[9], *const sPN = sm[10];  // pm, pn over the frame (read-only after L)
     // predictor right-hand sides of the ghost-source row (j = 1, M-1) / column (i = 1, L-1)
     T *const sRUlo = side, *const sRUhi = side + TI, *const sRVlo = side + 2 * TI, *const sRVhi = side + 2 * TI + TJ;
 
@@ -209,10 +210,10 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     // generic (load-on-demand) inputs of the u / v right-hand side at offset f
     auto load_u = [&](uint32_t f, int q) {
         UVIn<T> x;
-        x.pmc = ldo(P.pm, 0, f);
-        x.pmb = ldo(P.pm, -1, f);
-        x.pnc = ldo(P.pn, 0, f);
-        x.pnb = ldo(P.pn, -1, f);
+        x.pmc = sPM[q];
+        x.pmb = sPM[q - 1];
+        x.pnc = sPN[q];
+        x.pnb = sPN[q - 1];
         x.hb = ldo(P.h, -1, f);
         x.hc = ldo(P.h, 0, f);
         x.xb = sU1[q - 1];
@@ -234,10 +235,10 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     };
     auto load_v = [&](uint32_t f, int q) {
         UVIn<T> x;
-        x.pmc = ldo(P.pm, 0, f);
-        x.pmb = ldo(P.pm, -W, f);
-        x.pnc = ldo(P.pn, 0, f);
-        x.pnb = ldo(P.pn, -W, f);
+        x.pmc = sPM[q];
+        x.pmb = sPM[q - FW];
+        x.pnc = sPN[q];
+        x.pnb = sPN[q - FW];
         x.hb = ldo(P.h, -W, f);
         x.hc = ldo(P.h, 0, f);
         x.xb = sV1[q - FW];
@@ -314,7 +315,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     bool cin[KC];
     uint32_t cf[KC];
     T z1c[KC], u1c[KC], v1c[KC], hc_[KC], hw_[KC], hs_[KC];
-    T pm_[KC], pmw_[KC], pms_[KC], pn_[KC], pnw_[KC], pns_[KC];
+    T pm_[KC], pn_[KC];
     T onuc[KC], omuc[KC], cffuc[KC], omvc[KC], onvc[KC], cffvc[KC], pmnc[KC];
     T zsc[KC], ruc[KC], rvc[KC], duc[KC], dvc[KC];
     // G1 (predictor) and G2 (corrector) prefetch
@@ -345,27 +346,19 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             hc_[k] = ldo(P.h, 0, f);
             pm_[k] = ldo(P.pm, 0, f);
             pn_[k] = ldo(P.pn, 0, f);
-            if (i >= 1) {
-                hw_[k] = ldo(P.h, -1, f);
-                pmw_[k] = ldo(P.pm, -1, f);
-                pnw_[k] = ldo(P.pn, -1, f);
-            }
-            if (j >= 1) {
-                hs_[k] = ldo(P.h, -W, f);
-                pms_[k] = ldo(P.pm, -W, f);
-                pns_[k] = ldo(P.pn, -W, f);
-            }
+            if (i >= 1) hw_[k] = ldo(P.h, -1, f);
+            if (j >= 1) hs_[k] = ldo(P.h, -W, f);
         }
     }
     {
         constexpr int RING = ring_size<TI, TJ, 3, 2>();
         constexpr int RS = (RING + NT - 1) / NT;
-        T rz[RS], ru[RS], rv[RS];
+        T rz[RS], ru[RS], rv[RS], rm[RS], rn[RS];
         int ro[RS];
 #pragma unroll
         for (int s = 0; s < RS; s++) {
             const int t = tid + s * NT;
-            rz[s] = ru[s] = rv[s] = T(0);
+            rz[s] = ru[s] = rv[s] = rm[s] = rn[s] = T(0);
             ro[s] = -1;
             if (t < RING) {
                 int a, b;
@@ -377,6 +370,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     rz[s] = ldo(P.z1, 0, f);
                     ru[s] = ldo(P.u1, 0, f);
                     rv[s] = ldo(P.v1, 0, f);
+                    rm[s] = ldo(P.pm, 0, f);
+                    rn[s] = ldo(P.pn, 0, f);
                 }
             }
         }
@@ -388,6 +383,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             sZ1[o] = z1c[k];
             sU1[o] = cin[k] ? u1c[k] : T(0);
             sV1[o] = cin[k] ? v1c[k] : T(0);
+            sPM[o] = cin[k] ? pm_[k] : T(0);
+            sPN[o] = cin[k] ? pn_[k] : T(0);
         }
 #pragma unroll
         for (int s = 0; s < RS; s++)
@@ -395,6 +392,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 sZ1[ro[s]] = rz[s];
                 sU1[ro[s]] = ru[s];
                 sV1[ro[s]] = rv[s];
+                sPM[ro[s]] = rm[s];
+                sPN[ro[s]] = rn[s];
             }
     }
     lds_barrier();
@@ -403,7 +402,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     {
         constexpr int RING = ring_size<TI, TJ, 2, 2>();
         constexpr int RS = (RING + NT - 1) / NT;
-        T rh[RS], rhw[RS], rhs[RS], rpm[RS], rpms[RS], rpn[RS], rpnw[RS];
+        T rh[RS], rhw[RS], rhs[RS];
         int ro[RS], ri[RS], rj[RS];
 #pragma unroll
         for (int s = 0; s < RS; s++) {
@@ -420,16 +419,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     rj[s] = j;
                     const uint32_t f = off(i, j);
                     rh[s] = ldo(P.h, 0, f);
-                    rpm[s] = ldo(P.pm, 0, f);
-                    rpn[s] = ldo(P.pn, 0, f);
-                    if (i >= 1) {
-                        rhw[s] = ldo(P.h, -1, f);
-                        rpnw[s] = ldo(P.pn, -1, f);
-                    }
-                    if (j >= 1) {
-                        rhs[s] = ldo(P.h, -W, f);
-                        rpms[s] = ldo(P.pm, -W, f);
-                    }
+                    if (i >= 1) rhw[s] = ldo(P.h, -1, f);
+                    if (j >= 1) rhs[s] = ldo(P.h, -W, f);
                 }
             }
         }
@@ -443,15 +434,17 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             if (cin[k]) {
                 pmnc[k] = pm_[k] * pn_[k];
                 if (i >= 1) {
-                    onuc[k] = T(2) * sf::recip(pnw_[k] + pn_[k]);
-                    omuc[k] = T(2) * sf::recip(pmw_[k] + pm_[k]);
-                    cffuc[k] = (pm_[k] + pmw_[k]) * (pn_[k] + pnw_[k]);
+                    const T pmw = sPM[o - 1], pnw = sPN[o - 1];
+                    onuc[k] = T(2) * sf::recip(pnw + pn_[k]);
+                    omuc[k] = T(2) * sf::recip(pmw + pm_[k]);
+                    cffuc[k] = (pm_[k] + pmw) * (pn_[k] + pnw);
                     du = ((T(0.5) * ((hw_[k] + sZ1[o - 1]) + (hc_[k] + sZ1[o]))) * u1c[k]) * onuc[k];
                 }
                 if (j >= 1) {
-                    omvc[k] = T(2) * sf::recip(pms_[k] + pm_[k]);
-                    onvc[k] = T(2) * sf::recip(pns_[k] + pn_[k]);
-                    cffvc[k] = (pm_[k] + pms_[k]) * (pn_[k] + pns_[k]);
+                    const T pms = sPM[o - FW], pns = sPN[o - FW];
+                    omvc[k] = T(2) * sf::recip(pms + pm_[k]);
+                    onvc[k] = T(2) * sf::recip(pns + pn_[k]);
+                    cffvc[k] = (pm_[k] + pms) * (pn_[k] + pns);
                     dv = ((T(0.5) * ((hs_[k] + sZ1[o - FW]) + (hc_[k] + sZ1[o]))) * v1c[k]) * omvc[k];
                 }
             }
@@ -465,11 +458,11 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const int o = ro[s];
             T du = T(0), dv = T(0);
             if (ri[s] >= 1) {
-                const T onu = T(2) * sf::recip(rpnw[s] + rpn[s]);
+                const T onu = T(2) * sf::recip(sPN[o - 1] + sPN[o]);
                 du = ((T(0.5) * ((rhw[s] + sZ1[o - 1]) + (rh[s] + sZ1[o]))) * sU1[o]) * onu;
             }
             if (rj[s] >= 1) {
-                const T omv = T(2) * sf::recip(rpms[s] + rpm[s]);
+                const T omv = T(2) * sf::recip(sPM[o - FW] + sPM[o]);
                 dv = ((T(0.5) * ((rhs[s] + sZ1[o - FW]) + (rh[s] + sZ1[o]))) * sV1[o]) * omv;
             }
             sDU[o] = du;
@@ -511,7 +504,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     {
         constexpr int RING = ring_size<TI, TJ, 2, 1>();
         constexpr int RS = (RING + NT - 1) / NT;
-        T rpm[RS], rpn[RS], rz0[RS];
+        T rz0[RS];
         int ro[RS], rq[RS];
 #pragma unroll
         for (int s = 0; s < RS; s++) {
@@ -526,10 +519,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 if (inside(i, j)) {
                     const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
                     rq[s] = ro[s] + (qj - j) * FW + (qi - i);
-                    const uint32_t fq = off(qi, qj);
-                    rpm[s] = ldo(P.pm, 0, fq);
-                    rpn[s] = ldo(P.pn, 0, fq);
-                    if (!first) rz0[s] = ldo(P.z0, 0, fq);
+                    if (!first) rz0[s] = ldo(P.z0, 0, off(qi, qj));
                 } else {
                     ro[s] = -2 - ro[s];   // outside: store zeros
                 }
@@ -553,9 +543,10 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 } else {
                     const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
                     const uint32_t fq = off(qi, qj);
-                    const T pmn = ldo(P.pm, 0, fq) * ldo(P.pn, 0, fq);
+                    const int oq = o + (qj - j) * FW + (qi - i);
+                    const T pmn = sPM[oq] * sPN[oq];
                     const T z0 = first ? T(0) : ldo(P.z0, 0, fq);
-                    zs = zs_of(o + (qj - j) * FW + (qi - i), pmn, z0, r);
+                    zs = zs_of(oq, pmn, z0, r);
                 }
             }
             sZS[o] = zs;
@@ -571,7 +562,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 continue;
             }
             T r;
-            const T zs = zs_of(rq[s], rpm[s] * rpn[s], rz0[s], r);
+            const T zs = zs_of(rq[s], sPM[rq[s]] * sPN[rq[s]], rz0[s], r);
             sZS[ro[s]] = zs;
             sRZ[ro[s]] = r;
         }
@@ -718,7 +709,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     {
         constexpr int RING = ring_size<TI, TJ, 1, 1>();
         constexpr int RS = (RING + NT - 1) / NT;
-        T rh[RS], rhw[RS], rhs[RS], rpm[RS], rpms[RS], rpn[RS], rpnw[RS];
+        T rh[RS], rhw[RS], rhs[RS];
         int ro[RS], ri[RS], rj[RS];
 #pragma unroll
         for (int s = 0; s < RS; s++) {
@@ -735,16 +726,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     rj[s] = j;
                     const uint32_t f = off(i, j);
                     rh[s] = ldo(P.h, 0, f);
-                    rpm[s] = ldo(P.pm, 0, f);
-                    rpn[s] = ldo(P.pn, 0, f);
-                    if (i >= 1) {
-                        rhw[s] = ldo(P.h, -1, f);
-                        rpnw[s] = ldo(P.pn, -1, f);
-                    }
-                    if (j >= 1) {
-                        rhs[s] = ldo(P.h, -W, f);
-                        rpms[s] = ldo(P.pm, -W, f);
-                    }
+                    if (i >= 1) rhw[s] = ldo(P.h, -1, f);
+                    if (j >= 1) rhs[s] = ldo(P.h, -W, f);
                 }
             }
         }
@@ -769,11 +752,11 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const int o = ro[s];
             T du = T(0), dv = T(0);
             if (ri[s] >= 1) {
-                const T onu = T(2) * sf::recip(rpnw[s] + rpn[s]);
+                const T onu = T(2) * sf::recip(sPN[o - 1] + sPN[o]);
                 du = ((T(0.5) * ((rhw[s] + sZS[o - 1]) + (rh[s] + sZS[o]))) * sUS[o]) * onu;
             }
             if (rj[s] >= 1) {
-                const T omv = T(2) * sf::recip(rpms[s] + rpm[s]);
+                const T omv = T(2) * sf::recip(sPM[o - FW] + sPM[o]);
                 dv = ((T(0.5) * ((rhs[s] + sZS[o - FW]) + (rh[s] + sZS[o]))) * sVS[o]) * omv;
             }
             sDU[o] = du;
@@ -792,7 +775,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         };
         constexpr int RING = ring_size<TI, TJ, 1, 0>();
         constexpr int RS = (RING + NT - 1) / NT;
-        T rpm[RS], rpn[RS], rz1[RS], rrz0[RS];
+        T rz1[RS], rrz0[RS];
         int ro[RS], rq[RS];
 #pragma unroll
         for (int s = 0; s < RS; s++) {
@@ -808,8 +791,6 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
                     rq[s] = ro[s] + (qj - j) * FW + (qi - i);
                     const uint32_t fq = off(qi, qj);
-                    rpm[s] = ldo(P.pm, 0, fq);
-                    rpn[s] = ldo(P.pn, 0, fq);
                     rz1[s] = ldo(P.z1, 0, fq);
                     if (!first) rrz0[s] = ldo(P.rz0, 0, fq);
                 } else {
@@ -834,9 +815,10 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 } else {
                     const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
                     const uint32_t fq = off(qi, qj);
-                    const T pmn = ldo(P.pm, 0, fq) * ldo(P.pn, 0, fq);
+                    const int oq = o + (qj - j) * FW + (qi - i);
+                    const T pmn = sPM[oq] * sPN[oq];
                     const T rz0 = first ? T(0) : ldo(P.rz0, 0, fq);
-                    z2 = z2_of(o + (qj - j) * FW + (qi - i), pmn, ldo(P.z1, 0, fq), rz0);
+                    z2 = z2_of(oq, pmn, ldo(P.z1, 0, fq), rz0);
                 }
                 sto(P.z2, cf[k], z2);
             }
@@ -850,7 +832,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 sZ2[-2 - ro[s]] = T(0);
                 continue;
             }
-            sZ2[ro[s]] = z2_of(rq[s], rpm[s] * rpn[s], rz1[s], rrz0[s]);
+            sZ2[ro[s]] = z2_of(rq[s], sPM[rq[s]] * sPN[rq[s]], rz1[s], rrz0[s]);
         }
     }
     lds_barrier();
