@@ -657,7 +657,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         for (int k = 0; k < KC; k++) {
             int a, b;
             core_ab(k, a, b);
-            const int i = fi0 + a, j = fj0 + b;
+            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
             const uint32_t f = cf[k];
             if (cin[k] && rho_int(i, j)) {
                 const uint32_t fd = offd(i, j);
@@ -668,11 +668,11 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             }
             if (cin[k] && u_int(i, j)) {
                 if (!first) ru0_[k] = ldo(P.ru0, 0, f);
-                z1w_[k] = ldo(P.z1, -1, f);
+                z1w_[k] = sZ1[o - 1];   // the Z1 plane holds z1 until C1
             }
             if (cin[k] && v_int(i, j)) {
                 if (!first) rv0_[k] = ldo(P.rv0, 0, f);
-                z1s_[k] = ldo(P.z1, -W, f);
+                z1s_[k] = sZ1[o - FW];
             }
         }
         // ring: inputs loaded on demand
