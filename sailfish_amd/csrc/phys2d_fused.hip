@@ -1,1060 +1,9 @@
-// phys2d_fused.hip -- the physics-mode barotropic substep (ROMS LF-AM3 in 2D form,
-// obsolete/step2d_LF_AM3.h.py:917-2780) as ONE launch per substep.
-//
-// Each workgroup owns an output tile of TI x TJ points (all fields: zeta, ubar,
-// vbar, their right-hand sides and the FP64 time averages of the tile's points).
-// It loads a frame of (TI+5) x (TJ+5) points (3 to the low side, 2 to the high
-// side) and evaluates the four stages of the substep on shrinking regions, so that
-// no intermediate state ever goes to HBM:
-//
-//   stage  region (frame coords, core = [3, 3+T))   writes (LDS plane)
-//   L      [0, T+5)    z1                               Z1
-//   A      [1, T+5)    DU1 = D*u*on_u, DV1 = D*v*om_v   DU, DV
-//   P1     [1, T+4)    predicted zeta zs, its RHS rz    ZS, RZ
-//   P2     [2, T+4)    predicted ubar us, vbar vs       US, VS
-//   C0     [2, T+4)    DUs, DVs of the predicted state  DU, DV (reused)
-//   C1     [2, T+3)    corrected zeta z2 (+ averages)   Z1 (reused as Z2)
-//   C2     [3, T+3)    corrected ubar, vbar             -
-//
-// Seven planes of (TI+5)(TJ+5) values.  The core points come first in every stage
-// with the same thread mapping, so a thread keeps its core points' values (z1, u1,
-// v1, zs, the predictor right-hand sides) in registers from stage to stage.  Fields
-// that are only read (h, pm, pn, fomn, the stresses, the old time level, the
-// history right-hand sides) are read from global memory where they are used; after
-// their first touch those reads hit L1/L2.
-//
-// Closed-basin boundary: a ghost rho point takes the value of its clamped interior
-// source, computed by the same thread from the same stencil; u = 0 at i <= 1 and
-// i = L, v = 0 at j <= 1 and j = M; the remaining ghost rows/columns of u and v
-// copy their inward neighbour.  The tile origin is shifted by one point when
-// sz % T == 1 so that the source of every ghost u/v point is a core point of the
-// same tile (its predictor right-hand side was written by the same workgroup).
-//
-// Expression order is the one of phys2d_kernels.hip; with contraction off the FP64
-// instance equals the oracle bit for bit and each precision equals the four-launch
-// form bit for bit.
-#include "sf_internal.h"
+// phys2d_fused.hip -- C ABI of the fused physics substep and of ensembles, and the
+// FP64 instances of their kernels (contraction off: bitwise parity with the CPU
+// restatement).  The kernels are in phys2d_fused_impl.h; the FP32 instances are
+// compiled in phys2d_fused_f32.hip.
+#include "phys2d_fused_impl.h"
 
-#include <vector>
-
-namespace {
-
-template <typename T>
-struct PFA {
-    const T *h, *pm, *pn, *fomn, *sustr, *svstr;
-    const T *z0, *z1, *u0, *u1, *v0, *v1;
-    T *z2, *u2, *v2;
-    const T *rz0, *ru0, *rv0;
-    T *rz1, *ru1, *rv1;
-    double *Zt, *DUa, *DVa;
-    int szI, szJ, tiles_i, tiles_j, off_i, off_j;
-    T g, rdrg, dt;
-    int first;
-    double w1;
-};
-
-// Visit the points of region [3-LO, 3+TI+HI) x [3-LO, 3+TJ+HI) of the frame: the
-// TI*TJ core points first (slot k >= 0, point tid + k*NT in row-major core order),
-// then the ring around the core (slot -1).
-template <int TI, int TJ, int NT, int LO, int HI, typename F>
-__device__ __forceinline__ void for_region(int tid, F &&fn)
-{
-    constexpr int CORE = TI * TJ;
-    static_assert(CORE % NT == 0, "core must be a multiple of the block");
-#pragma unroll
-    for (int k = 0; k < CORE / NT; k++) {
-        const int idx = tid + k * NT;
-        fn(k, 3 + idx % TI, 3 + idx / TI);
-    }
-    constexpr int W = TI + LO + HI, H = TJ + LO + HI;
-    constexpr int RING = W * H - CORE;
-    if constexpr (RING > 0) {
-        constexpr int NBOT = LO * W, WROW = LO + HI, NMID = TJ * WROW;
-#pragma unroll
-        for (int k = 0; k < (RING + NT - 1) / NT; k++) {
-            const int t = tid + k * NT;
-            if (t < RING) {
-                int a, b;
-                if (t < NBOT) {
-                    b = 3 - LO + t / W;
-                    a = 3 - LO + t % W;
-                } else if (t < NBOT + NMID) {
-                    const int s = t - NBOT;
-                    b = 3 + s / WROW;
-                    const int c = s % WROW;
-                    a = c < LO ? 3 - LO + c : 3 + TI + (c - LO);
-                } else {
-                    const int s = t - NBOT - NMID;
-                    b = 3 + TJ + s / W;
-                    a = 3 - LO + s % W;
-                }
-                fn(-1, a, b);
-            }
-        }
-    }
-}
-
-template <int TI, int TJ>
-struct PhysLds {
-    static constexpr int FW = TI + 5, FH = TJ + 5, FN = FW * FH, PLANES = 11;
-};
-
-// Element (f + d) of p, given the byte offset boff of element f: the address is the
-// array's scalar base plus a 32-bit vector offset (global_load saddr form).  The
-// neighbour offset boff + d*sizeof(T) is computed in 32-bit arithmetic, so it is one
-// vector add per neighbour and point, shared by every array read at that neighbour.
-// All field pointers point to global memory; the explicit address space keeps the
-// global_load/global_store forms also for pointers read from a member table.
-#define SF_GLOBAL __attribute__((address_space(1)))
-#define SF_CONST __attribute__((address_space(4)))
-template <typename T>
-__device__ __forceinline__ T ldo(const T *p, long long d, uint32_t boff)
-{
-    const uint32_t o = boff + (uint32_t)(d * (long long)sizeof(T));
-    return *(const SF_GLOBAL T *)((const SF_GLOBAL char *)p + o);
-}
-template <typename T>
-__device__ __forceinline__ void sto(T *p, uint32_t boff, T v)
-{
-    *(SF_GLOBAL T *)((SF_GLOBAL char *)p + boff) = v;
-}
-
-// Workgroup barrier that orders LDS only: global loads issued before it stay in
-// flight (the stages exchange data through LDS exclusively).
-__device__ __forceinline__ void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// Map ring index t of region [3-LO, 3+TI+HI) x [3-LO, 3+TJ+HI) minus the core to (a, b).
-template <int TI, int TJ, int LO, int HI>
-__device__ __forceinline__ void ring_point(int t, int &a, int &b)
-{
-    constexpr int W = TI + LO + HI, NBOT = LO * W, WROW = LO + HI, NMID = TJ * WROW;
-    if (t < NBOT) {
-        b = 3 - LO + t / W;
-        a = 3 - LO + t % W;
-    } else if (t < NBOT + NMID) {
-        const int s = t - NBOT;
-        b = 3 + s / WROW;
-        const int c = s % WROW;
-        a = c < LO ? 3 - LO + c : 3 + TI + (c - LO);
-    } else {
-        const int s = t - NBOT - NMID;
-        b = 3 + TJ + s / W;
-        a = 3 - LO + s % W;
-    }
-}
-template <int TI, int TJ, int LO, int HI>
-constexpr int ring_size() { return (TI + LO + HI) * (TJ + LO + HI) - TI * TJ; }
-
-// inputs of the ubar / vbar right-hand side at a u / v point
-template <typename T>
-struct UVIn {
-    T m1, m2, cff;              // u: onu, omu ; v: omv, onv
-    T pmc, pmb, pnc, pnb;       // raw pm, pn at the point and the back neighbour (generic path)
-    T hb, hc;                   // h at the back neighbour (i-1 / j-1) and the point
-    T xb, xc, xf, yf, yb;       // same-component velocity: back, centre, forward, cross+, cross-
-    T fc, fb;                   // fomn at the point and the back neighbour
-    T oc, of, ob, obf;          // other component for Coriolis (see rhs_u / rhs_v)
-    T st;                       // surface stress
-    T x0, z0c, z0b;             // predictor "before" velocity and zeta (t0) at point / back
-};
-
-template <typename T, int TI, int TJ, int NT, bool FIRST>
-__device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*sm)[PhysLds<TI, TJ>::FN],
-                                          T *side)
-{
-    constexpr int FW = TI + 5;
-    constexpr int KC = TI * TJ / NT;
-    constexpr uint32_t ES = sizeof(T);
-    T *const sZ1 = sm[0], *const sZ2 = sm[0];
-    T *const sDU = sm[1], *const sDV = sm[2];
-    T *const sZS = sm[3], *const sRZ = sm[4], *const sUS = sm[5], *const sVS = sm[6];
-    T *const sU1 = sm[7], *const sV1 = sm[8];   // ubar, vbar at t1 over the frame (read-only after L)
-    T *const sPM = sm[9], *const sPN = sm[10];  // pm, pn over the frame (read-only after L)
-    // predictor right-hand sides of the ghost-source row (j = 1, M-1) / column (i = 1, L-1)
-    T *const sRUlo = side, *const sRUhi = side + TI, *const sRVlo = side + 2 * TI, *const sRVhi = side + 2 * TI + TJ;
-
-    const int szI = P.szI, szJ = P.szJ, L = szI - 1, M = szJ - 1;
-    const long long W = szI;
-    const int fi0 = ti * TI - P.off_i - 3, fj0 = tj * TJ - P.off_j - 3;   // frame origin
-    const int tid = threadIdx.x;
-    const T dt = P.dt;
-    constexpr bool first = FIRST;
-
-    auto inside = [&](int i, int j) { return i >= 0 && i <= L && j >= 0 && j <= M; };
-    auto clampi = [](int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); };
-    auto off = [&](int i, int j) { return (uint32_t)(j * szI + i) * ES; };
-    auto offd = [&](int i, int j) { return (uint32_t)(j * szI + i) * 8u; };
-    auto core_ab = [&](int k, int &a, int &b) {
-        const int idx = tid + k * NT;
-        a = 3 + idx % TI;
-        b = 3 + idx / TI;
-    };
-
-    // metric factors of a u point / v point at byte offset o
-    auto onu_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(P.pn, -1, o) + ldo(P.pn, 0, o)); };
-    auto omu_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(P.pm, -1, o) + ldo(P.pm, 0, o)); };
-    auto omv_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(P.pm, -W, o) + ldo(P.pm, 0, o)); };
-    auto onv_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(P.pn, -W, o) + ldo(P.pn, 0, o)); };
-    auto cffu_at = [&](uint32_t o) {
-        return (ldo(P.pm, 0, o) + ldo(P.pm, -1, o)) * (ldo(P.pn, 0, o) + ldo(P.pn, -1, o));
-    };
-    auto cffv_at = [&](uint32_t o) {
-        return (ldo(P.pm, 0, o) + ldo(P.pm, -W, o)) * (ldo(P.pn, 0, o) + ldo(P.pn, -W, o));
-    };
-
-    // generic (load-on-demand) inputs of the u / v right-hand side at offset f
-    auto load_u = [&](uint32_t f, int q) {
-        UVIn<T> x;
-        x.pmc = sPM[q];
-        x.pmb = sPM[q - 1];
-        x.pnc = sPN[q];
-        x.pnb = sPN[q - 1];
-        x.hb = ldo(P.h, -1, f);
-        x.hc = ldo(P.h, 0, f);
-        x.xb = sU1[q - 1];
-        x.xc = sU1[q];
-        x.xf = sU1[q + 1];
-        x.yf = sU1[q + FW];
-        x.yb = sU1[q - FW];
-        x.fc = ldo(P.fomn, 0, f);
-        x.fb = ldo(P.fomn, -1, f);
-        x.oc = sV1[q];
-        x.of = sV1[q + FW];
-        x.ob = sV1[q - 1];
-        x.obf = sV1[q + FW - 1];
-        x.st = ldo(P.sustr, 0, f);
-        x.x0 = first ? x.xc : ldo(P.u0, 0, f);
-        x.z0c = first ? T(0) : ldo(P.z0, 0, f);
-        x.z0b = first ? T(0) : ldo(P.z0, -1, f);
-        return x;
-    };
-    auto load_v = [&](uint32_t f, int q) {
-        UVIn<T> x;
-        x.pmc = sPM[q];
-        x.pmb = sPM[q - FW];
-        x.pnc = sPN[q];
-        x.pnb = sPN[q - FW];
-        x.hb = ldo(P.h, -W, f);
-        x.hc = ldo(P.h, 0, f);
-        x.xb = sV1[q - FW];
-        x.xc = sV1[q];
-        x.xf = sV1[q + FW];
-        x.yf = sV1[q + 1];
-        x.yb = sV1[q - 1];
-        x.fc = ldo(P.fomn, 0, f);
-        x.fb = ldo(P.fomn, -W, f);
-        x.oc = sU1[q];
-        x.of = sU1[q + 1];
-        x.ob = sU1[q - FW];
-        x.obf = sU1[q + 1 - FW];
-        x.st = ldo(P.svstr, 0, f);
-        x.x0 = first ? x.xc : ldo(P.v0, 0, f);
-        x.z0c = first ? T(0) : ldo(P.z0, 0, f);
-        x.z0b = first ? T(0) : ldo(P.z0, -W, f);
-        return x;
-    };
-
-    // metric factors of generic inputs (same expressions as onu_at / omu_at / cffu_at ...)
-    auto metrics_u = [&](UVIn<T> &x) {
-        x.m1 = T(2) * sf::recip(x.pnb + x.pnc);
-        x.m2 = T(2) * sf::recip(x.pmb + x.pmc);
-        x.cff = (x.pmc + x.pmb) * (x.pnc + x.pnb);
-    };
-    auto metrics_v = [&](UVIn<T> &x) {
-        x.m1 = T(2) * sf::recip(x.pmb + x.pmc);
-        x.m2 = T(2) * sf::recip(x.pnb + x.pnc);
-        x.cff = (x.pmc + x.pmb) * (x.pnc + x.pnb);
-    };
-
-    // ---- predictor momentum at LDS index q from inputs x (returns rhs in r) ----------
-    const T cff4 = T(4.0 / 25.0), cff5 = T(1) - T(2) * cff4;
-    const T c1dt = first ? T(0.5) * dt : dt;
-    auto gzp = [&](int o, T z0) { return first ? T(0.5) * (sZ1[o] + sZS[o]) : cff5 * sZ1[o] + cff4 * (z0 + sZS[o]); };
-    auto Dbp = [&](int o, T h, T z0) { return first ? h + sZ1[o] : h + z0; };
-    auto pred_u = [&](int q, const UVIn<T> &x, T &r) {
-        const T onu = x.m1, omu = x.m2;
-        const T g0 = gzp(q - 1, x.z0b), g1 = gzp(q, x.z0c);
-        r = ((T(0.5) * P.g) * onu) * ((x.hb + x.hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
-        const T UFx1 = T(0.25) * (sDU[q] + sDU[q + 1]) * (x.xc + x.xf);
-        const T UFx0 = T(0.25) * (sDU[q - 1] + sDU[q]) * (x.xb + x.xc);
-        const T UFe1 = T(0.25) * (sDV[q + FW] + sDV[q + FW - 1]) * (x.yf + x.xc);
-        const T UFe0 = T(0.25) * (sDV[q] + sDV[q - 1]) * (x.xc + x.yb);
-        r = r - ((UFx1 - UFx0) + (UFe1 - UFe0));
-        const T cr1 = ((T(0.5) * (x.hc + sZ1[q])) * x.fc) * (x.oc + x.of);
-        const T cr0 = ((T(0.5) * (x.hb + sZ1[q - 1])) * x.fb) * (x.ob + x.obf);
-        r = r + T(0.5) * (cr1 + cr0);
-        r = r - ((P.rdrg * x.xc) * omu) * onu;
-        r = r + (x.st * omu) * onu;
-        const T fac = sf::recip((x.hc + sZS[q]) + (x.hb + sZS[q - 1]));
-        return (x.x0 * (Dbp(q, x.hc, x.z0c) + Dbp(q - 1, x.hb, x.z0b)) + (x.cff * c1dt) * r) * fac;
-    };
-    auto pred_v = [&](int q, const UVIn<T> &x, T &r) {
-        const T omv = x.m1, onv = x.m2;
-        const T g0 = gzp(q - FW, x.z0b), g1 = gzp(q, x.z0c);
-        r = ((T(0.5) * P.g) * omv) * ((x.hb + x.hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
-        const T VFx1 = T(0.25) * (sDU[q + 1] + sDU[q + 1 - FW]) * (x.yf + x.xc);
-        const T VFx0 = T(0.25) * (sDU[q] + sDU[q - FW]) * (x.xc + x.yb);
-        const T VFe1 = T(0.25) * (sDV[q] + sDV[q + FW]) * (x.xc + x.xf);
-        const T VFe0 = T(0.25) * (sDV[q - FW] + sDV[q]) * (x.xb + x.xc);
-        r = r - ((VFx1 - VFx0) + (VFe1 - VFe0));
-        const T cr1 = ((T(0.5) * (x.hc + sZ1[q])) * x.fc) * (x.oc + x.of);
-        const T cr0 = ((T(0.5) * (x.hb + sZ1[q - FW])) * x.fb) * (x.ob + x.obf);
-        r = r - T(0.5) * (cr1 + cr0);
-        r = r - ((P.rdrg * x.xc) * omv) * onv;
-        r = r + (x.st * omv) * onv;
-        const T fac = sf::recip((x.hc + sZS[q]) + (x.hb + sZS[q - FW]));
-        return (x.x0 * (Dbp(q, x.hc, x.z0c) + Dbp(q - FW, x.hb, x.z0b)) + (x.cff * c1dt) * r) * fac;
-    };
-
-    // ============ core-point registers ============
-    bool cin[KC];
-    uint32_t cf[KC];
-    T z1c[KC], u1c[KC], v1c[KC], hc_[KC], hw_[KC], hs_[KC];
-    T pm_[KC], pn_[KC];
-    T onuc[KC], omuc[KC], cffuc[KC], omvc[KC], onvc[KC], cffvc[KC], pmnc[KC];
-    T zsc[KC], ruc[KC], rvc[KC], duc[KC], dvc[KC];
-    // G1 (predictor) and G2 (corrector) prefetch
-    T z0c_[KC], z0w_[KC], z0s_[KC], u0c_[KC], v0c_[KC], fo_[KC], fow_[KC], fos_[KC], su_[KC], sv_[KC];
-    T rz0_[KC], ru0_[KC], rv0_[KC], z1w_[KC], z1s_[KC];
-    double zt_[KC], dua_[KC], dva_[KC];
-
-    // fast-path predicates of a core point (its computations use only its own offset)
-    auto rho_int = [&](int i, int j) { return i >= 1 && i <= L - 1 && j >= 1 && j <= M - 1; };
-    auto u_int = [&](int i, int j) { return i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1; };
-    auto v_int = [&](int i, int j) { return i >= 1 && i <= L - 1 && j >= 2 && j <= M - 1; };
-
-    // ---- G0 + L: z1 over the frame; core statics -----------------------------------------
-#pragma unroll
-    for (int k = 0; k < KC; k++) {
-        int a, b;
-        core_ab(k, a, b);
-        const int i = fi0 + a, j = fj0 + b;
-        cin[k] = inside(i, j);
-        cf[k] = cin[k] ? off(i, j) : 0u;
-        const uint32_t f = cf[k];
-        // values below are read only under the predicate they are loaded under
-        z1c[k] = T(0);
-        if (cin[k]) {
-            z1c[k] = ldo(P.z1, 0, f);
-            u1c[k] = ldo(P.u1, 0, f);
-            v1c[k] = ldo(P.v1, 0, f);
-            hc_[k] = ldo(P.h, 0, f);
-            pm_[k] = ldo(P.pm, 0, f);
-            pn_[k] = ldo(P.pn, 0, f);
-            if (i >= 1) hw_[k] = ldo(P.h, -1, f);
-            if (j >= 1) hs_[k] = ldo(P.h, -W, f);
-        }
-    }
-    {
-        constexpr int RING = ring_size<TI, TJ, 3, 2>();
-        constexpr int RS = (RING + NT - 1) / NT;
-        T rz[RS], ru[RS], rv[RS], rm[RS], rn[RS];
-        int ro[RS];
-#pragma unroll
-        for (int s = 0; s < RS; s++) {
-            const int t = tid + s * NT;
-            rz[s] = ru[s] = rv[s] = rm[s] = rn[s] = T(0);
-            ro[s] = -1;
-            if (t < RING) {
-                int a, b;
-                ring_point<TI, TJ, 3, 2>(t, a, b);
-                const int i = fi0 + a, j = fj0 + b;
-                ro[s] = b * FW + a;
-                if (inside(i, j)) {
-                    const uint32_t f = off(i, j);
-                    rz[s] = ldo(P.z1, 0, f);
-                    ru[s] = ldo(P.u1, 0, f);
-                    rv[s] = ldo(P.v1, 0, f);
-                    rm[s] = ldo(P.pm, 0, f);
-                    rn[s] = ldo(P.pn, 0, f);
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < KC; k++) {
-            int a, b;
-            core_ab(k, a, b);
-            const int o = b * FW + a;
-            sZ1[o] = z1c[k];
-            sU1[o] = cin[k] ? u1c[k] : T(0);
-            sV1[o] = cin[k] ? v1c[k] : T(0);
-            sPM[o] = cin[k] ? pm_[k] : T(0);
-            sPN[o] = cin[k] ? pn_[k] : T(0);
-        }
-#pragma unroll
-        for (int s = 0; s < RS; s++)
-            if (ro[s] >= 0) {
-                sZ1[ro[s]] = rz[s];
-                sU1[ro[s]] = ru[s];
-                sV1[ro[s]] = rv[s];
-                sPM[ro[s]] = rm[s];
-                sPN[ro[s]] = rn[s];
-            }
-    }
-    lds_barrier();
-
-    // ---- A: DU1 at u points, DV1 at v points ---------------------------------------------
-    {
-        constexpr int RING = ring_size<TI, TJ, 2, 2>();
-        constexpr int RS = (RING + NT - 1) / NT;
-        T rh[RS], rhw[RS], rhs[RS];
-        int ro[RS], ri[RS], rj[RS];
-#pragma unroll
-        for (int s = 0; s < RS; s++) {
-            const int t = tid + s * NT;
-            ro[s] = -1;
-            ri[s] = rj[s] = -1;
-            if (t < RING) {
-                int a, b;
-                ring_point<TI, TJ, 2, 2>(t, a, b);
-                const int i = fi0 + a, j = fj0 + b;
-                ro[s] = b * FW + a;
-                if (inside(i, j)) {
-                    ri[s] = i;
-                    rj[s] = j;
-                    const uint32_t f = off(i, j);
-                    rh[s] = ldo(P.h, 0, f);
-                    if (i >= 1) rhw[s] = ldo(P.h, -1, f);
-                    if (j >= 1) rhs[s] = ldo(P.h, -W, f);
-                }
-            }
-        }
-        // core: metric factors, fluxes
-#pragma unroll
-        for (int k = 0; k < KC; k++) {
-            int a, b;
-            core_ab(k, a, b);
-            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
-            T du = T(0), dv = T(0);
-            if (cin[k]) {
-                pmnc[k] = pm_[k] * pn_[k];
-                if (i >= 1) {
-                    const T pmw = sPM[o - 1], pnw = sPN[o - 1];
-                    onuc[k] = T(2) * sf::recip(pnw + pn_[k]);
-                    omuc[k] = T(2) * sf::recip(pmw + pm_[k]);
-                    cffuc[k] = (pm_[k] + pmw) * (pn_[k] + pnw);
-                    du = ((T(0.5) * ((hw_[k] + sZ1[o - 1]) + (hc_[k] + sZ1[o]))) * u1c[k]) * onuc[k];
-                }
-                if (j >= 1) {
-                    const T pms = sPM[o - FW], pns = sPN[o - FW];
-                    omvc[k] = T(2) * sf::recip(pms + pm_[k]);
-                    onvc[k] = T(2) * sf::recip(pns + pn_[k]);
-                    cffvc[k] = (pm_[k] + pms) * (pn_[k] + pns);
-                    dv = ((T(0.5) * ((hs_[k] + sZ1[o - FW]) + (hc_[k] + sZ1[o]))) * v1c[k]) * omvc[k];
-                }
-            }
-            sDU[o] = du;
-            sDV[o] = dv;
-        }
-        // ring fluxes
-#pragma unroll
-        for (int s = 0; s < RS; s++) {
-            if (ro[s] < 0) continue;
-            const int o = ro[s];
-            T du = T(0), dv = T(0);
-            if (ri[s] >= 1) {
-                const T onu = T(2) * sf::recip(sPN[o - 1] + sPN[o]);
-                du = ((T(0.5) * ((rhw[s] + sZ1[o - 1]) + (rh[s] + sZ1[o]))) * sU1[o]) * onu;
-            }
-            if (rj[s] >= 1) {
-                const T omv = T(2) * sf::recip(sPM[o - FW] + sPM[o]);
-                dv = ((T(0.5) * ((rhs[s] + sZ1[o - FW]) + (rh[s] + sZ1[o]))) * sV1[o]) * omv;
-            }
-            sDU[o] = du;
-            sDV[o] = dv;
-        }
-    }
-        // core: issue the predictor inputs (G1)
-#pragma unroll
-        for (int k = 0; k < KC; k++) {
-            int a, b;
-            core_ab(k, a, b);
-            const int i = fi0 + a, j = fj0 + b;
-            const uint32_t f = cf[k];
-            const bool ui = cin[k] && u_int(i, j), vi = cin[k] && v_int(i, j), ri_ = cin[k] && rho_int(i, j);
-            if (ri_ || ui || vi) {
-                if (!first) z0c_[k] = ldo(P.z0, 0, f);
-                fo_[k] = ldo(P.fomn, 0, f);
-            }
-            if (ui) {
-                if (!first) {
-                    z0w_[k] = ldo(P.z0, -1, f);
-                    u0c_[k] = ldo(P.u0, 0, f);
-                }
-                fow_[k] = ldo(P.fomn, -1, f);
-                su_[k] = ldo(P.sustr, 0, f);
-            }
-            if (vi) {
-                if (!first) {
-                    z0s_[k] = ldo(P.z0, -W, f);
-                    v0c_[k] = ldo(P.v0, 0, f);
-                }
-                fos_[k] = ldo(P.fomn, -W, f);
-                sv_[k] = ldo(P.svstr, 0, f);
-            }
-        }
-    lds_barrier();
-
-    // ---- P1: predicted zeta and its right-hand side -----------------------------------
-    {
-        constexpr int RING = ring_size<TI, TJ, 2, 1>();
-        constexpr int RS = (RING + NT - 1) / NT;
-        T rz0[RS];
-        int ro[RS], rq[RS];
-#pragma unroll
-        for (int s = 0; s < RS; s++) {
-            const int t = tid + s * NT;
-            ro[s] = -1;
-            rq[s] = 0;
-            if (t < RING) {
-                int a, b;
-                ring_point<TI, TJ, 2, 1>(t, a, b);
-                const int i = fi0 + a, j = fj0 + b;
-                ro[s] = b * FW + a;
-                if (inside(i, j)) {
-                    const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
-                    rq[s] = ro[s] + (qj - j) * FW + (qi - i);
-                    if (!first) rz0[s] = ldo(P.z0, 0, off(qi, qj));
-                } else {
-                    ro[s] = -2 - ro[s];   // outside: store zeros
-                }
-            }
-        }
-        auto zs_of = [&](int qo, T pmn, T z0, T &r) {
-            r = (sDU[qo] - sDU[qo + 1]) + (sDV[qo] - sDV[qo + FW]);
-            if (first) return sZ1[qo] + (pmn * dt) * r;
-            return z0 + (pmn * (T(2) * dt)) * r;
-        };
-#pragma unroll
-        for (int k = 0; k < KC; k++) {
-            int a, b;
-            core_ab(k, a, b);
-            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
-            T zs = T(0), r = T(0);
-            if (cin[k]) {
-                if (rho_int(i, j)) {
-                    zs = zs_of(o, pmnc[k], z0c_[k], r);
-                    sto(P.rz1, cf[k], r);
-                } else {
-                    const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
-                    const uint32_t fq = off(qi, qj);
-                    const int oq = o + (qj - j) * FW + (qi - i);
-                    const T pmn = sPM[oq] * sPN[oq];
-                    const T z0 = first ? T(0) : ldo(P.z0, 0, fq);
-                    zs = zs_of(oq, pmn, z0, r);
-                }
-            }
-            sZS[o] = zs;
-            sRZ[o] = r;
-            zsc[k] = zs;
-        }
-#pragma unroll
-        for (int s = 0; s < RS; s++) {
-            if (ro[s] == -1) continue;
-            if (ro[s] < -1) {
-                sZS[-2 - ro[s]] = T(0);
-                sRZ[-2 - ro[s]] = T(0);
-                continue;
-            }
-            T r;
-            const T zs = zs_of(rq[s], sPM[rq[s]] * sPN[rq[s]], rz0[s], r);
-            sZS[ro[s]] = zs;
-            sRZ[ro[s]] = r;
-        }
-    }
-    lds_barrier();
-
-    // ---- P2: predicted ubar / vbar --------------------------------------------------
-    {
-#pragma unroll
-        for (int k = 0; k < KC; k++) {
-            int a, b;
-            core_ab(k, a, b);
-            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
-            T us = T(0), vs = T(0);
-            if (cin[k]) {
-                if (u_int(i, j)) {
-                    UVIn<T> x;
-                    x.m1 = onuc[k];
-                    x.m2 = omuc[k];
-                    x.cff = cffuc[k];
-                    x.hb = hw_[k];
-                    x.hc = hc_[k];
-                    x.xb = sU1[o - 1];
-                    x.xc = u1c[k];
-                    x.xf = sU1[o + 1];
-                    x.yf = sU1[o + FW];
-                    x.yb = sU1[o - FW];
-                    x.fc = fo_[k];
-                    x.fb = fow_[k];
-                    x.oc = v1c[k];
-                    x.of = sV1[o + FW];
-                    x.ob = sV1[o - 1];
-                    x.obf = sV1[o + FW - 1];
-                    x.st = su_[k];
-                    x.x0 = first ? u1c[k] : u0c_[k];
-                    x.z0c = z0c_[k];
-                    x.z0b = z0w_[k];
-                    T r;
-                    us = pred_u(o, x, r);
-                    sto(P.ru1, cf[k], r);
-                    ruc[k] = r;
-                    if (j == 1) sRUlo[a - 3] = r;
-                    if (j == M - 1) sRUhi[a - 3] = r;
-                } else if (i >= 2 && i <= L - 1) {   // ghost row: source (i, 1) or (i, M-1)
-                    const int qj = clampi(j, 1, M - 1);
-                    T r;
-                    UVIn<T> x = load_u(off(i, qj), o + (qj - j) * FW);
-                    metrics_u(x);
-                    us = pred_u(o + (qj - j) * FW, x, r);
-                }
-                if (v_int(i, j)) {
-                    UVIn<T> x;
-                    x.m1 = omvc[k];
-                    x.m2 = onvc[k];
-                    x.cff = cffvc[k];
-                    x.hb = hs_[k];
-                    x.hc = hc_[k];
-                    x.xb = sV1[o - FW];
-                    x.xc = v1c[k];
-                    x.xf = sV1[o + FW];
-                    x.yf = sV1[o + 1];
-                    x.yb = sV1[o - 1];
-                    x.fc = fo_[k];
-                    x.fb = fos_[k];
-                    x.oc = u1c[k];
-                    x.of = sU1[o + 1];
-                    x.ob = sU1[o - FW];
-                    x.obf = sU1[o + 1 - FW];
-                    x.st = sv_[k];
-                    x.x0 = first ? v1c[k] : v0c_[k];
-                    x.z0c = z0c_[k];
-                    x.z0b = z0s_[k];
-                    T r;
-                    vs = pred_v(o, x, r);
-                    sto(P.rv1, cf[k], r);
-                    rvc[k] = r;
-                    if (i == 1) sRVlo[b - 3] = r;
-                    if (i == L - 1) sRVhi[b - 3] = r;
-                } else if (j >= 2 && j <= M - 1) {   // ghost column: source (1, j) or (L-1, j)
-                    const int qi = clampi(i, 1, L - 1);
-                    T r;
-                    UVIn<T> x = load_v(off(qi, j), o + (qi - i));
-                    metrics_v(x);
-                    vs = pred_v(o + (qi - i), x, r);
-                }
-            }
-            sUS[o] = us;
-            sVS[o] = vs;
-        }
-        // core: issue the corrector inputs (G2)
-#pragma unroll
-        for (int k = 0; k < KC; k++) {
-            int a, b;
-            core_ab(k, a, b);
-            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
-            const uint32_t f = cf[k];
-            if (cin[k] && rho_int(i, j)) {
-                const uint32_t fd = offd(i, j);
-                if (!first) rz0_[k] = ldo(P.rz0, 0, f);
-                zt_[k] = ldo(P.Zt, 0, fd);
-                dua_[k] = ldo(P.DUa, 0, fd);
-                dva_[k] = ldo(P.DVa, 0, fd);
-            }
-            if (cin[k] && u_int(i, j)) {
-                if (!first) ru0_[k] = ldo(P.ru0, 0, f);
-                z1w_[k] = sZ1[o - 1];   // the Z1 plane holds z1 until C1
-            }
-            if (cin[k] && v_int(i, j)) {
-                if (!first) rv0_[k] = ldo(P.rv0, 0, f);
-                z1s_[k] = sZ1[o - FW];
-            }
-        }
-        // ring: inputs loaded on demand
-        constexpr int RING = ring_size<TI, TJ, 1, 1>();
-#pragma unroll
-        for (int s = 0; s < (RING + NT - 1) / NT; s++) {
-            const int t = tid + s * NT;
-            if (t >= RING) continue;
-            int a, b;
-            ring_point<TI, TJ, 1, 1>(t, a, b);
-            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
-            T us = T(0), vs = T(0), r;
-            if (inside(i, j)) {
-                if (i >= 2 && i <= L - 1) {
-                    const int qj = clampi(j, 1, M - 1);
-                    UVIn<T> x = load_u(off(i, qj), o + (qj - j) * FW);
-                    metrics_u(x);
-                    us = pred_u(o + (qj - j) * FW, x, r);
-                }
-                if (j >= 2 && j <= M - 1) {
-                    const int qi = clampi(i, 1, L - 1);
-                    UVIn<T> x = load_v(off(qi, j), o + (qi - i));
-                    metrics_v(x);
-                    vs = pred_v(o + (qi - i), x, r);
-                }
-            }
-            sUS[o] = us;
-            sVS[o] = vs;
-        }
-    }
-    lds_barrier();
-
-    // ---- C0: fluxes DUs, DVs of the predicted state ------------------------------------
-    {
-        constexpr int RING = ring_size<TI, TJ, 1, 1>();
-        constexpr int RS = (RING + NT - 1) / NT;
-        T rh[RS], rhw[RS], rhs[RS];
-        int ro[RS], ri[RS], rj[RS];
-#pragma unroll
-        for (int s = 0; s < RS; s++) {
-            const int t = tid + s * NT;
-            ro[s] = -1;
-            ri[s] = rj[s] = -1;
-            if (t < RING) {
-                int a, b;
-                ring_point<TI, TJ, 1, 1>(t, a, b);
-                const int i = fi0 + a, j = fj0 + b;
-                ro[s] = b * FW + a;
-                if (inside(i, j)) {
-                    ri[s] = i;
-                    rj[s] = j;
-                    const uint32_t f = off(i, j);
-                    rh[s] = ldo(P.h, 0, f);
-                    if (i >= 1) rhw[s] = ldo(P.h, -1, f);
-                    if (j >= 1) rhs[s] = ldo(P.h, -W, f);
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < KC; k++) {
-            int a, b;
-            core_ab(k, a, b);
-            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
-            T du = T(0), dv = T(0);
-            if (cin[k]) {
-                if (i >= 1) du = ((T(0.5) * ((hw_[k] + sZS[o - 1]) + (hc_[k] + sZS[o]))) * sUS[o]) * onuc[k];
-                if (j >= 1) dv = ((T(0.5) * ((hs_[k] + sZS[o - FW]) + (hc_[k] + sZS[o]))) * sVS[o]) * omvc[k];
-            }
-            sDU[o] = du;
-            sDV[o] = dv;
-            duc[k] = du;
-            dvc[k] = dv;
-        }
-#pragma unroll
-        for (int s = 0; s < RS; s++) {
-            if (ro[s] < 0) continue;
-            const int o = ro[s];
-            T du = T(0), dv = T(0);
-            if (ri[s] >= 1) {
-                const T onu = T(2) * sf::recip(sPN[o - 1] + sPN[o]);
-                du = ((T(0.5) * ((rhw[s] + sZS[o - 1]) + (rh[s] + sZS[o]))) * sUS[o]) * onu;
-            }
-            if (rj[s] >= 1) {
-                const T omv = T(2) * sf::recip(sPM[o - FW] + sPM[o]);
-                dv = ((T(0.5) * ((rhs[s] + sZS[o - FW]) + (rh[s] + sZS[o]))) * sVS[o]) * omv;
-            }
-            sDU[o] = du;
-            sDV[o] = dv;
-        }
-    }
-    lds_barrier();
-
-    // ---- C1: corrected zeta, time averages ------------------------------------------
-    const T a1 = dt * T(5.0) / T(12.0), a2 = dt * T(8.0) / T(12.0), a3 = dt * T(1.0) / T(12.0);
-    {
-        auto z2_of = [&](int qo, T pmn, T z1q, T rz0) {
-            const T r = (sDU[qo] - sDU[qo + 1]) + (sDV[qo] - sDV[qo + FW]);
-            if (first) return z1q + pmn * (dt * r);
-            return z1q + pmn * ((a1 * r + a2 * sRZ[qo]) - a3 * rz0);
-        };
-        constexpr int RING = ring_size<TI, TJ, 1, 0>();
-        constexpr int RS = (RING + NT - 1) / NT;
-        T rz1[RS], rrz0[RS];
-        int ro[RS], rq[RS];
-#pragma unroll
-        for (int s = 0; s < RS; s++) {
-            const int t = tid + s * NT;
-            ro[s] = -1;
-            rq[s] = 0;
-            if (t < RING) {
-                int a, b;
-                ring_point<TI, TJ, 1, 0>(t, a, b);
-                const int i = fi0 + a, j = fj0 + b;
-                ro[s] = b * FW + a;
-                if (inside(i, j)) {
-                    const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
-                    rq[s] = ro[s] + (qj - j) * FW + (qi - i);
-                    const uint32_t fq = off(qi, qj);
-                    rz1[s] = ldo(P.z1, 0, fq);
-                    if (!first) rrz0[s] = ldo(P.rz0, 0, fq);
-                } else {
-                    ro[s] = -2 - ro[s];
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < KC; k++) {
-            int a, b;
-            core_ab(k, a, b);
-            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
-            T z2 = T(0);
-            if (cin[k]) {
-                if (rho_int(i, j)) {
-                    z2 = z2_of(o, pmnc[k], z1c[k], rz0_[k]);
-                    const double w = P.w1;
-                    const uint32_t fd = offd(i, j);
-                    sto(P.Zt, fd, zt_[k] + w * (double)zsc[k]);
-                    sto(P.DUa, fd, dua_[k] + w * (double)duc[k]);
-                    sto(P.DVa, fd, dva_[k] + w * (double)dvc[k]);
-                } else {
-                    const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
-                    const uint32_t fq = off(qi, qj);
-                    const int oq = o + (qj - j) * FW + (qi - i);
-                    const T pmn = sPM[oq] * sPN[oq];
-                    const T rz0 = first ? T(0) : ldo(P.rz0, 0, fq);
-                    z2 = z2_of(oq, pmn, ldo(P.z1, 0, fq), rz0);
-                }
-                sto(P.z2, cf[k], z2);
-            }
-            // Z2 reuses the Z1 plane: every reader of Z1 finished before the last barrier
-            sZ2[o] = z2;
-        }
-#pragma unroll
-        for (int s = 0; s < RS; s++) {
-            if (ro[s] == -1) continue;
-            if (ro[s] < -1) {
-                sZ2[-2 - ro[s]] = T(0);
-                continue;
-            }
-            sZ2[ro[s]] = z2_of(rq[s], sPM[rq[s]] * sPN[rq[s]], rz1[s], rrz0[s]);
-        }
-    }
-    lds_barrier();
-
-    // ---- C2: corrected ubar / vbar on the core -----------------------------------------
-    const T cf4 = T(2.0 / 5.0), cf5 = T(1) - cf4;
-    const T b1 = T(0.5) * dt * T(5.0) / T(12.0), b2 = T(0.5) * dt * T(8.0) / T(12.0),
-            b3 = T(0.5) * dt * T(1.0) / T(12.0);
-    auto gzc = [&](int o) { return cf5 * sZ2[o] + cf4 * sZS[o]; };
-    // corrected ubar at LDS index q: metrics m1 = onu, m2 = omu; statics hb/hc, fc/fb, st
-    auto corr_u = [&](int q, T onu, T omu, T cff, T hb, T hc, T fc, T fb, T st, T ru1, T ru0, T z1c_, T z1b,
-                      T u1c_) {
-        const T g0 = gzc(q - 1), g1 = gzc(q);
-        T r = ((T(0.5) * P.g) * onu) * ((hb + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
-        const T uc = sUS[q];
-        const T UFx1 = T(0.25) * (sDU[q] + sDU[q + 1]) * (uc + sUS[q + 1]);
-        const T UFx0 = T(0.25) * (sDU[q - 1] + sDU[q]) * (sUS[q - 1] + uc);
-        const T UFe1 = T(0.25) * (sDV[q + FW] + sDV[q + FW - 1]) * (sUS[q + FW] + uc);
-        const T UFe0 = T(0.25) * (sDV[q] + sDV[q - 1]) * (uc + sUS[q - FW]);
-        r = r - ((UFx1 - UFx0) + (UFe1 - UFe0));
-        const T cr1 = ((T(0.5) * (hc + sZS[q])) * fc) * (sVS[q] + sVS[q + FW]);
-        const T cr0 = ((T(0.5) * (hb + sZS[q - 1])) * fb) * (sVS[q - 1] + sVS[q - 1 + FW]);
-        r = r + T(0.5) * (cr1 + cr0);
-        r = r - ((P.rdrg * uc) * omu) * onu;
-        r = r + (st * omu) * onu;
-        const T fac = sf::recip((hc + sZ2[q]) + (hb + sZ2[q - 1]));
-        T tend;
-        if (first) tend = (T(0.5) * dt) * r;
-        else tend = (b1 * r + b2 * ru1) - b3 * ru0;
-        return (u1c_ * ((hc + z1c_) + (hb + z1b)) + cff * tend) * fac;
-    };
-    auto corr_v = [&](int q, T omv, T onv, T cff, T hb, T hc, T fc, T fb, T st, T rv1, T rv0, T z1c_, T z1b,
-                      T v1c_) {
-        const T g0 = gzc(q - FW), g1 = gzc(q);
-        T r = ((T(0.5) * P.g) * omv) * ((hb + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
-        const T vc = sVS[q];
-        const T VFx1 = T(0.25) * (sDU[q + 1] + sDU[q + 1 - FW]) * (sVS[q + 1] + vc);
-        const T VFx0 = T(0.25) * (sDU[q] + sDU[q - FW]) * (vc + sVS[q - 1]);
-        const T VFe1 = T(0.25) * (sDV[q] + sDV[q + FW]) * (vc + sVS[q + FW]);
-        const T VFe0 = T(0.25) * (sDV[q - FW] + sDV[q]) * (sVS[q - FW] + vc);
-        r = r - ((VFx1 - VFx0) + (VFe1 - VFe0));
-        const T cr1 = ((T(0.5) * (hc + sZS[q])) * fc) * (sUS[q] + sUS[q + 1]);
-        const T cr0 = ((T(0.5) * (hb + sZS[q - FW])) * fb) * (sUS[q - FW] + sUS[q - FW + 1]);
-        r = r - T(0.5) * (cr1 + cr0);
-        r = r - ((P.rdrg * vc) * omv) * onv;
-        r = r + (st * omv) * onv;
-        const T fac = sf::recip((hc + sZ2[q]) + (hb + sZ2[q - FW]));
-        T tend;
-        if (first) tend = (T(0.5) * dt) * r;
-        else tend = (b1 * r + b2 * rv1) - b3 * rv0;
-        return (v1c_ * ((hc + z1c_) + (hb + z1b)) + cff * tend) * fac;
-    };
-#pragma unroll
-    for (int k = 0; k < KC; k++) {
-        int a, b;
-        core_ab(k, a, b);
-        const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
-        if (!cin[k]) continue;
-        T u2 = T(0), v2 = T(0);
-        if (u_int(i, j)) {
-            u2 = corr_u(o, onuc[k], omuc[k], cffuc[k], hw_[k], hc_[k], fo_[k], fow_[k], su_[k], ruc[k], ru0_[k],
-                        z1c[k], z1w_[k], u1c[k]);
-        } else if (i >= 2 && i <= L - 1) {   // ghost row j = 0 or M: source row 1 or M-1
-            const int qj = clampi(j, 1, M - 1);
-            const uint32_t f = off(i, qj);
-            const T ru1 = qj == 1 ? sRUlo[a - 3] : sRUhi[a - 3];
-            const T ru0 = first ? T(0) : ldo(P.ru0, 0, f);
-            u2 = corr_u(o + (qj - j) * FW, onu_at(f), omu_at(f), cffu_at(f), ldo(P.h, -1, f), ldo(P.h, 0, f),
-                        ldo(P.fomn, 0, f), ldo(P.fomn, -1, f), ldo(P.sustr, 0, f), ru1, ru0, ldo(P.z1, 0, f),
-                        ldo(P.z1, -1, f), sU1[o + (qj - j) * FW]);
-        }
-        sto(P.u2, cf[k], u2);
-        if (v_int(i, j)) {
-            v2 = corr_v(o, omvc[k], onvc[k], cffvc[k], hs_[k], hc_[k], fo_[k], fos_[k], sv_[k], rvc[k], rv0_[k],
-                        z1c[k], z1s_[k], v1c[k]);
-        } else if (j >= 2 && j <= M - 1) {   // ghost column i = 0 or L: source column 1 or L-1
-            const int qi = clampi(i, 1, L - 1);
-            const uint32_t f = off(qi, j);
-            const T rv1 = qi == 1 ? sRVlo[b - 3] : sRVhi[b - 3];
-            const T rv0 = first ? T(0) : ldo(P.rv0, 0, f);
-            v2 = corr_v(o + (qi - i), omv_at(f), onv_at(f), cffv_at(f), ldo(P.h, -W, f), ldo(P.h, 0, f),
-                        ldo(P.fomn, 0, f), ldo(P.fomn, -W, f), ldo(P.svstr, 0, f), rv1, rv0, ldo(P.z1, 0, f),
-                        ldo(P.z1, -W, f), sV1[o + (qi - i)]);
-        }
-        sto(P.v2, cf[k], v2);
-    }
-}
-
-// Block id -> tile: each XCD (block id mod 8) walks a contiguous band of tiles so that
-// neighbouring tiles share that XCD's L2.
-__device__ __forceinline__ int xcd_band(int nblk)
-{
-    int b = blockIdx.x;
-    if ((nblk & 7) == 0) b = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
-    return b;
-}
-
-template <typename T, int TI, int TJ, int NT, bool FIRST>
-__global__ __launch_bounds__(NT) void k_phys_fused(PFA<T> P)
-{
-    __shared__ T sm[PhysLds<TI, TJ>::PLANES][PhysLds<TI, TJ>::FN];
-    __shared__ T side[2 * TI + 2 * TJ];
-    const int b = xcd_band(P.tiles_i * P.tiles_j);
-    phys_tile<T, TI, TJ, NT, FIRST>(P, b % P.tiles_i, b / P.tiles_i, sm, side);
-}
-
-template <typename T, int TI, int TJ, int NT>
-int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void *stream)
-{
-    PFA<T> a;
-    a.h = (const T *)x->h;
-    a.pm = (const T *)x->pm;
-    a.pn = (const T *)x->pn;
-    a.fomn = (const T *)x->fomn;
-    a.sustr = (const T *)x->sustr;
-    a.svstr = (const T *)x->svstr;
-    a.z0 = (const T *)x->zeta[0];
-    a.z1 = (const T *)x->zeta[1];
-    a.z2 = (T *)x->zeta[2];
-    a.u0 = (const T *)x->ubar[0];
-    a.u1 = (const T *)x->ubar[1];
-    a.u2 = (T *)x->ubar[2];
-    a.v0 = (const T *)x->vbar[0];
-    a.v1 = (const T *)x->vbar[1];
-    a.v2 = (T *)x->vbar[2];
-    a.rz0 = (const T *)x->rzeta[0];
-    a.rz1 = (T *)x->rzeta[1];
-    a.ru0 = (const T *)x->rubar[0];
-    a.ru1 = (T *)x->rubar[1];
-    a.rv0 = (const T *)x->rvbar[0];
-    a.rv1 = (T *)x->rvbar[1];
-    a.Zt = x->Zt_avg1;
-    a.DUa = x->DU_avg1;
-    a.DVa = x->DV_avg1;
-    a.szI = x->szI;
-    a.szJ = x->szJ;
-    a.off_i = (x->szI % TI == 1) ? 1 : 0;
-    a.off_j = (x->szJ % TJ == 1) ? 1 : 0;
-    a.tiles_i = (x->szI + a.off_i + TI - 1) / TI;
-    a.tiles_j = (x->szJ + a.off_j + TJ - 1) / TJ;
-    a.g = T(x->g);
-    a.rdrg = T(x->rdrg);
-    a.dt = T(dt);
-    a.first = first;
-    a.w1 = w1;
-    SF_REQUIRE((long long)x->szI * x->szJ < (1LL << 29), "sf_phys_substep_fused: grid of %dx%d points exceeds 2^29",
-               x->szI, x->szJ);
-    const long long nb = (long long)a.tiles_i * a.tiles_j;
-    SF_REQUIRE(nb < (1LL << 31), "sf_phys_substep_fused: grid too large");
-    if (first)
-        hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true>), dim3((unsigned)nb), dim3(NT), 0,
-                           sf::as_stream(stream), a);
-    else
-        hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, false>), dim3((unsigned)nb), dim3(NT), 0,
-                           sf::as_stream(stream), a);
-    return sf::check_launch("sf_phys_substep_fused");
-}
-
-template <typename T>
-int dispatch_tj(const sf_phys_fields *x, double dt, int first, double w1, int tj, void *stream)
-{
-    switch (tj) {
-    case 4: return launch_fused<T, 64, 4, 256>(x, dt, first, w1, stream);
-    case 8: return launch_fused<T, 64, 8, 256>(x, dt, first, w1, stream);
-    case 16: return launch_fused<T, 64, 16, 512>(x, dt, first, w1, stream);
-    default: break;
-    }
-    sf::set_error("sf_phys_substep_fused: unsupported tile height %d (4, 8 or 16)", tj);
-    return SF_ERR_ARG;
-}
-
-
-// ---------------------------------------------------------------------------------
-// Ensembles: many basins of uneven size advance in ONE launch per substep.  The
-// launch walks a device-side list of (member, tile) pairs; each workgroup takes
-// tiles from the list in one of two orders and runs the same per-tile function as
-// the single-grid kernel, so a member's result equals its solo run bit for bit.
-// ---------------------------------------------------------------------------------
-// The member table holds, for every member, its kernel arguments for each of the
-// three time rotations (index 3*member + rot): no runtime indexing of slot pointers
-// inside the kernel.
-// order 0: block b takes list entries b, b + G, b + 2G, ...
-// order 1: block b (after the XCD band remap) takes a contiguous chunk of the list
-template <typename T, int TI, int TJ, int NT, bool FIRST>
-__global__ __launch_bounds__(NT) void k_phys_ensemble(const PFA<T> *__restrict__ members,
-                                                      const int2 *__restrict__ list, int ntiles, int order,
-                                                      int rot, double w1)
-{
-    __shared__ T sm[PhysLds<TI, TJ>::PLANES][PhysLds<TI, TJ>::FN];
-    __shared__ T side[2 * TI + 2 * TJ];
-    const int G = gridDim.x;
-    int t0, t1, step;
-    if (order == 0) {
-        t0 = blockIdx.x;
-        t1 = ntiles;
-        step = G;
-    } else {
-        const int b = xcd_band(G);
-        t0 = (int)((long long)ntiles * b / G);
-        t1 = (int)((long long)ntiles * (b + 1) / G);
-        step = 1;
-    }
-    // the tables are read-only for the whole launch: scalar loads through the constant
-    // address space keep every member pointer in scalar registers
-    const SF_CONST int *clist = (const SF_CONST int *)list;
-    const SF_CONST PFA<T> *cmem = (const SF_CONST PFA<T> *)members;
-    for (int t = t0; t < t1; t += step) {
-        const int ex = clist[2 * t], ey = clist[2 * t + 1];
-        PFA<T> P;
-        {
-            static_assert(sizeof(PFA<T>) % 4 == 0, "argument block of whole words");
-            const SF_CONST uint32_t *src = (const SF_CONST uint32_t *)(cmem + 3 * ex + rot);
-            uint32_t *dst = reinterpret_cast<uint32_t *>(&P);
-#pragma unroll
-            for (int w = 0; w < (int)(sizeof(PFA<T>) / 4); w++) dst[w] = src[w];
-        }
-        P.w1 = w1;
-        phys_tile<T, TI, TJ, NT, FIRST>(P, ey % P.tiles_i, ey / P.tiles_i, sm, side);
-        lds_barrier();   // the next tile overwrites the LDS planes
-    }
-}
-}  // namespace
 
 extern "C" int sf_phys_fused_default_tile(int32_t precision, int32_t szI, int32_t szJ)
 {
@@ -1078,132 +27,12 @@ extern "C" int sf_phys_substep_fused(const sf_phys_fields *x, int32_t precision,
     SF_REQUIRE(x->Zt_avg1 && x->DU_avg1 && x->DV_avg1, "sf_phys_substep_fused: null average");
     const int tj = tile_j > 0 ? tile_j : sf_phys_fused_default_tile(precision, x->szI, x->szJ);
     if (precision == SF_FP64) return dispatch_tj<double>(x, dt, first, cff1, tj, stream);
-    if (precision == SF_FP32) return dispatch_tj<float>(x, dt, first, cff1, tj, stream);
+    if (precision == SF_FP32) return sf_phys_fused_dispatch_f32(x, dt, first, cff1, tj, stream);
     sf::set_error("sf_phys_substep_fused: unknown precision %d", precision);
     return SF_ERR_ARG;
 }
 
-
-// ---------------------------------------------------------------------------------
-// Ensemble C ABI
-// ---------------------------------------------------------------------------------
-struct sf_ens {
-    int n, precision, device;
-    int tile_j, order, grid;
-    int ntiles, rot;
-    void *members;              // device PFA<T>[3 n]: arguments of member k for rotation r at 3k + r
-    int2 *list;                 // device (member, tile) list for tile_j
-    std::vector<sf_phys_fields> host;
-    std::vector<double> dt;
-    hipGraph_t graph;
-    hipGraphExec_t exec;
-    int graph_nsub;
-};
-
 namespace {
-
-template <int TJ>
-int ens_tiles(const sf_phys_fields &x, int &ti, int &tj, int &oi, int &oj)
-{
-    constexpr int TI = 64;
-    oi = (x.szI % TI == 1) ? 1 : 0;
-    oj = (x.szJ % TJ == 1) ? 1 : 0;
-    ti = (x.szI + oi + TI - 1) / TI;
-    tj = (x.szJ + oj + TJ - 1) / TJ;
-    return ti * tj;
-}
-
-template <typename T>
-int ens_build(sf_ens *e)
-{
-    std::vector<PFA<T>> hm(3 * e->n);
-    std::vector<int2> list;
-    for (int k = 0; k < e->n; k++) {
-        const sf_phys_fields &x = e->host[k];
-        int ti = 0, tj = 0, oi = 0, oj = 0, nt = 0;
-        switch (e->tile_j) {
-        case 4: nt = ens_tiles<4>(x, ti, tj, oi, oj); break;
-        case 8: nt = ens_tiles<8>(x, ti, tj, oi, oj); break;
-        default: nt = ens_tiles<16>(x, ti, tj, oi, oj); break;
-        }
-        for (int rot = 0; rot < 3; rot++) {
-            // level t_l lives in slot (l + rot) % 3
-            const int s0 = rot % 3, s1 = (rot + 1) % 3, s2 = (rot + 2) % 3;
-            PFA<T> &a = hm[3 * k + rot];
-            a.h = (const T *)x.h;
-            a.pm = (const T *)x.pm;
-            a.pn = (const T *)x.pn;
-            a.fomn = (const T *)x.fomn;
-            a.sustr = (const T *)x.sustr;
-            a.svstr = (const T *)x.svstr;
-            a.z0 = (const T *)x.zeta[s0];
-            a.z1 = (const T *)x.zeta[s1];
-            a.z2 = (T *)x.zeta[s2];
-            a.u0 = (const T *)x.ubar[s0];
-            a.u1 = (const T *)x.ubar[s1];
-            a.u2 = (T *)x.ubar[s2];
-            a.v0 = (const T *)x.vbar[s0];
-            a.v1 = (const T *)x.vbar[s1];
-            a.v2 = (T *)x.vbar[s2];
-            a.rz0 = (const T *)x.rzeta[s0];
-            a.rz1 = (T *)x.rzeta[s1];
-            a.ru0 = (const T *)x.rubar[s0];
-            a.ru1 = (T *)x.rubar[s1];
-            a.rv0 = (const T *)x.rvbar[s0];
-            a.rv1 = (T *)x.rvbar[s1];
-            a.Zt = x.Zt_avg1;
-            a.DUa = x.DU_avg1;
-            a.DVa = x.DV_avg1;
-            a.szI = x.szI;
-            a.szJ = x.szJ;
-            a.tiles_i = ti;
-            a.tiles_j = tj;
-            a.off_i = oi;
-            a.off_j = oj;
-            a.g = T(x.g);
-            a.rdrg = T(x.rdrg);
-            a.dt = T(e->dt[k]);
-            a.first = 0;
-            a.w1 = 0.0;
-        }
-        for (int t = 0; t < nt; t++) list.push_back(make_int2(k, t));
-    }
-    e->ntiles = (int)list.size();
-    hipError_t r = hipFree(e->members);
-    if (r == hipSuccess) r = hipFree(e->list);
-    e->members = nullptr;
-    e->list = nullptr;
-    if (r == hipSuccess) r = hipMalloc(&e->members, sizeof(PFA<T>) * hm.size());
-    if (r == hipSuccess) r = hipMalloc((void **)&e->list, sizeof(int2) * list.size());
-    if (r == hipSuccess) r = hipMemcpy(e->members, hm.data(), sizeof(PFA<T>) * hm.size(), hipMemcpyHostToDevice);
-    if (r == hipSuccess) r = hipMemcpy(e->list, list.data(), sizeof(int2) * list.size(), hipMemcpyHostToDevice);
-    if (r != hipSuccess) return sf::hip_fail(r, "sf_ens: device tables");
-    return SF_OK;
-}
-
-template <typename T, int TJ>
-void ens_launch_tj(sf_ens *e, int first, double w1, int rot, hipStream_t s)
-{
-    constexpr int TI = 64, NT = TJ == 16 ? 512 : 256;
-    const int G = e->grid > 0 ? (e->grid < e->ntiles ? e->grid : e->ntiles) : e->ntiles;
-    const PFA<T> *m = (const PFA<T> *)e->members;
-    if (first)
-        hipLaunchKernelGGL((k_phys_ensemble<T, TI, TJ, NT, true>), dim3(G), dim3(NT), 0, s, m, e->list, e->ntiles,
-                           e->order, rot, w1);
-    else
-        hipLaunchKernelGGL((k_phys_ensemble<T, TI, TJ, NT, false>), dim3(G), dim3(NT), 0, s, m, e->list, e->ntiles,
-                           e->order, rot, w1);
-}
-
-template <typename T>
-void ens_launch(sf_ens *e, int first, double w1, int rot, hipStream_t s)
-{
-    switch (e->tile_j) {
-    case 4: ens_launch_tj<T, 4>(e, first, w1, rot, s); break;
-    case 8: ens_launch_tj<T, 8>(e, first, w1, rot, s); break;
-    default: ens_launch_tj<T, 16>(e, first, w1, rot, s); break;
-    }
-}
 
 int ens_rebuild(sf_ens *e)
 {
@@ -1298,7 +127,7 @@ extern "C" int sf_ens_substep(sf_ens *e, int32_t first, double cff1, void *strea
     SF_REQUIRE(e, "sf_ens_substep: null ensemble");
     e->rot = (e->rot + 1) % 3;   // cycleTimes2D of every member
     if (e->precision == SF_FP64) ens_launch<double>(e, first, cff1, e->rot, sf::as_stream(stream));
-    else ens_launch<float>(e, first, cff1, e->rot, sf::as_stream(stream));
+    else sf_ens_launch_f32(e, first, cff1, e->rot, sf::as_stream(stream));
     return sf::check_launch("sf_ens_substep");
 }
 
@@ -1316,7 +145,7 @@ extern "C" int sf_ens_capture(sf_ens *e, int32_t nsub, const double *cff1, void 
         for (int k = 0; k < nsub; k++) {
             rot = (rot + 1) % 3;
             if (e->precision == SF_FP64) ens_launch<double>(e, 0, cff1[k], rot, cs);
-            else ens_launch<float>(e, 0, cff1[k], rot, cs);
+            else sf_ens_launch_f32(e, 0, cff1[k], rot, cs);
         }
         r = hipStreamEndCapture(cs, &e->graph);
     }

@@ -1,7 +1,8 @@
 """GPU parity of the physics-mode substep against the CPU oracle: the four-launch
 form (sf_phys_substep) and the fused one-launch form (sf_phys_substep_fused, every
-tile height) equal the oracle bit for bit in FP64; in FP32 the fused form equals
-the four-launch form bit for bit and both stay within a stated tolerance of FP64."""
+tile height) equal the oracle bit for bit in FP64.  In FP32 the fused form (compiled
+with fused multiply-adds) agrees with the four-launch form (separately rounded
+products) to a few FP32 roundings, and stays within a stated tolerance of FP64."""
 from __future__ import annotations
 
 import sys
@@ -81,21 +82,32 @@ def test_phys_fp64_bitwise(szI, szJ, mode):
 
 @pytest.mark.parametrize("szI,szJ", [(65, 17), (130, 66)])
 @pytest.mark.parametrize("tj", [4, 8, 16])
-def test_phys_fp32_fused_equals_unfused(szI, szJ, tj):
+def test_phys_fp32_fused_matches_unfused(szI, szJ, tj):
+    """10 substeps: every field of the fused FP32 form within 2e-6 of the field's
+    magnitude of the four-launch FP32 form (the two differ only in the rounding of
+    multiply-adds, ~1e-7 relative per operation)."""
     import torch
     print(f"[case] phys fp32 fused{tj} vs unfused {szI}x{szJ}", file=sys.stderr, flush=True)
     cfg = configs.random_smooth(szI, szJ, seed=7)
     _, _, A = _gpu_run(cfg, 10, torch.float32, True, tj)
     _, _, B = _gpu_run(cfg, 10, torch.float32, False)
     bad = []
+
+    def close(x, y, name):
+        x, y = x.astype(np.float64), y.astype(np.float64)
+        scale = max(np.abs(y).max(), 1e-30)
+        err = np.abs(x - y).max() / scale
+        print(f"  {name}: max rel diff {err:.2e}", file=sys.stderr)
+        return err <= 2e-6
+
     for name in FIELDS + RHS:
         for k in range(3 if name in FIELDS else 2):
             x = getattr(A, f"{name}_t{k}").cpu().numpy()
             y = getattr(B, f"{name}_t{k}").cpu().numpy()
-            if not bitwise_equal(x, y):
+            if not close(x, y, f"{name}_t{k}"):
                 bad.append((name, k))
     for name in AVG:
-        if not bitwise_equal(getattr(A, name).cpu().numpy(), getattr(B, name).cpu().numpy()):
+        if not close(getattr(A, name).cpu().numpy(), getattr(B, name).cpu().numpy(), name):
             bad.append(name)
     assert not bad, bad
 
