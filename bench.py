@@ -59,16 +59,26 @@ def _events(torch, n):
     return [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
 
 
+PHYS_KERNEL_SOURCES = ("sf_internal.h", "phys2d_fused_impl.h", "phys2d_fused.hip", "phys2d_fused_f32.hip")
+
+
+def kernel_source_sha1() -> str:
+    """SHA-1 over the sources of k_phys_fused (in PHYS_KERNEL_SOURCES order)."""
+    import hashlib
+    h = hashlib.sha1()
+    for name in PHYS_KERNEL_SOURCES:
+        h.update((ROOT / "sailfish_amd" / "csrc" / name).read_bytes())
+    return h.hexdigest()
+
+
 def measured_traffic(dtype_name):
     """HBM bytes per launch of k_phys_fused from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE record (profiles/k_phys_fused_traffic.json), used only while the kernel
     source is the one that was profiled (same SHA-1); otherwise None."""
-    import hashlib
     rec_path = ROOT / "profiles" / "k_phys_fused_traffic.json"
-    src = ROOT / "sailfish_amd" / "csrc" / "phys2d_fused.hip"
     try:
         rec = json.loads(rec_path.read_text())
-        if hashlib.sha1(src.read_bytes()).hexdigest() != rec["source_sha1"]:
+        if kernel_source_sha1() != rec["source_sha1"]:
             return None
         return int(rec[dtype_name]["hbm_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
@@ -356,7 +366,7 @@ def run_closure_gpu(args):
     import torch
     from sailfish_amd import closure as CL
 
-    res = CL.run_experiment(dtype=torch.float64)
+    res = CL.run_experiment(dtype=torch.float64, log=lambda *a: print("[closure]", *a, file=sys.stderr, flush=True))
     mlp = res.pop("mlp")
     sp = CL.speed(mlp, dtype=torch.float32)
     mlp.close()
@@ -496,8 +506,14 @@ def main():
         calib_copy(args)
         return
 
+    t_start = time.perf_counter()
+
+    def progress(what):
+        print(f"[bench] {what} done ({time.perf_counter() - t_start:.0f} s)", file=sys.stderr, flush=True)
+
     run = run_phys_gpu if args.mode == "phys" else run_faithful_gpu
     r = run(args, args.dtype, args.steps, args.warmup)
+    progress(f"{args.mode} {args.dtype}")
     value = r["S2"] / (r["ms_per_step"] * 1e-3)
     if args.mode == "phys":
         workload = (f"ROMS 2D LF-AM3 barotropic substep (predictor+corrector, C2 advection, Coriolis, "
@@ -548,14 +564,19 @@ def main():
         other = "f32" if args.dtype == "f64" else "f64"
         s2 = run(args, other, min(args.steps, 200), args.warmup)
         out["secondary"] = summary(s2, other, min(args.steps, 200), args.warmup)
+        progress(f"secondary {other}")
     if not args.no_ensemble and args.mode == "phys":
         out["ensemble"] = run_ensemble_gpu(args)
+        progress("ensemble")
     if not args.no_closure and args.mode == "phys":
         out["closure"] = run_closure_gpu(args)
+        progress("closure")
     if not args.no_3d and args.mode == "phys":
         out["baroclinic_3d"] = run_3d_gpu(args)
+        progress("3d")
     if not args.no_cpu:
         out["cpu_baseline"] = (run_cpu_phys if args.mode == "phys" else run_cpu_faithful)(args, args.cpu_budget)
+        progress("cpu baseline")
     print(json.dumps(out))
 
 
