@@ -97,104 +97,146 @@ struct H {
     A u, v, Hu, Hv;
     long long W;
 };
+// the same four fields staged in LDS for one level of a tile: element at flat offset
+// idx of the tile view (row stride W = tile width + 4)
+struct AL {
+    const double *p;
+};
+template <bool CHK>
+__device__ __forceinline__ double rd(AL a, long long idx)
+{
+    return a.p[idx];
+}
+struct HL {
+    AL u, v, Hu, Hv;
+    long long W;
+};
 
 #define AT(a, o) rd<CHK>((a), base + (o))
-template <bool CHK>
-__device__ __forceinline__ double dxx(A a, long long base, long long o)
+template <bool CHK, class AA>
+__device__ __forceinline__ double dxx(AA a, long long base, long long o)
 {
     return AT(a, o - 1) - 2.0 * AT(a, o) + AT(a, o + 1);
 }
-template <bool CHK>
-__device__ __forceinline__ double dee(A a, long long base, long long o, long long W)
+template <bool CHK, class AA>
+__device__ __forceinline__ double dee(AA a, long long base, long long o, long long W)
 {
     return AT(a, o - W) - 2.0 * AT(a, o) + AT(a, o + W);
 }
-template <bool CHK>
-__device__ __forceinline__ double h_uR(const H &h, long long base, long long o) { return (AT(h.u, o) + AT(h.u, o + 1)) * 0.5; }
-template <bool CHK>
-__device__ __forceinline__ double h_uP(const H &h, long long base, long long o) { return (AT(h.u, o) + AT(h.u, o - 1)) * 0.5; }
-template <bool CHK>
-__device__ __forceinline__ double h_vR(const H &h, long long base, long long o) { return (AT(h.v, o) + AT(h.v, o + h.W)) * 0.5; }
-template <bool CHK>
-__device__ __forceinline__ double h_vP(const H &h, long long base, long long o) { return (AT(h.v, o) + AT(h.v, o - h.W)) * 0.5; }
-template <bool CHK>
-__device__ __forceinline__ double h_HuR(const H &h, long long base, long long o) { return (AT(h.Hu, o) + AT(h.Hu, o + 1)) * 0.5; }
-template <bool CHK>
-__device__ __forceinline__ double h_HuP(const H &h, long long base, long long o) { return (AT(h.Hu, o) + AT(h.Hu, o - 1)) * 0.5; }
-template <bool CHK>
-__device__ __forceinline__ double h_HvP(const H &h, long long base, long long o) { return (AT(h.Hv, o) + AT(h.Hv, o - h.W)) * 0.5; }
-template <bool CHK>
-__device__ __forceinline__ double h_HvR(const H &h, long long base, long long o) { return (AT(h.Hv, o) + AT(h.Hv, o + h.W)) * 0.5; }
-template <bool CHK>
-__device__ __forceinline__ double h_uxxR(const H &h, long long base, long long o)
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_uR(const HH &h, long long base, long long o) { return (AT(h.u, o) + AT(h.u, o + 1)) * 0.5; }
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_uP(const HH &h, long long base, long long o) { return (AT(h.u, o) + AT(h.u, o - 1)) * 0.5; }
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_vR(const HH &h, long long base, long long o) { return (AT(h.v, o) + AT(h.v, o + h.W)) * 0.5; }
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_vP(const HH &h, long long base, long long o) { return (AT(h.v, o) + AT(h.v, o - h.W)) * 0.5; }
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_HuR(const HH &h, long long base, long long o) { return (AT(h.Hu, o) + AT(h.Hu, o + 1)) * 0.5; }
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_HuP(const HH &h, long long base, long long o) { return (AT(h.Hu, o) + AT(h.Hu, o - 1)) * 0.5; }
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_HvP(const HH &h, long long base, long long o) { return (AT(h.Hv, o) + AT(h.Hv, o - h.W)) * 0.5; }
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_HvR(const HH &h, long long base, long long o) { return (AT(h.Hv, o) + AT(h.Hv, o + h.W)) * 0.5; }
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_uxxR(const HH &h, long long base, long long o)
 {
     return (h_uR<CHK>(h, base, o) + h_uR<CHK>(h, base, o + 1) >= 0) ? dxx<CHK>(h.u, base, o) : dxx<CHK>(h.u, base, o + 1);
 }
-template <bool CHK>
-__device__ __forceinline__ double h_ueeP(const H &h, long long base, long long o)
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_ueeP(const HH &h, long long base, long long o)
 {
     return (h_HvP<CHK>(h, base, o) + h_HvP<CHK>(h, base, o - h.W) >= 0) ? dee<CHK>(h.u, base, o, h.W) : dee<CHK>(h.u, base, o - h.W, h.W);
 }
-template <bool CHK>
-__device__ __forceinline__ double h_HuxxR(const H &h, long long base, long long o)
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_HuxxR(const HH &h, long long base, long long o)
 {
     return (dxx<CHK>(h.Hu, base, o) + dxx<CHK>(h.Hu, base, o + 1)) * 0.5;
 }
-template <bool CHK>
-__device__ __forceinline__ double h_HueeP(const H &h, long long base, long long o)
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_HueeP(const HH &h, long long base, long long o)
 {
     return (dee<CHK>(h.Hu, base, o, h.W) + dee<CHK>(h.Hu, base, o - 1, h.W)) * 0.5;
 }
-template <bool CHK>
-__device__ __forceinline__ double h_HvxxP(const H &h, long long base, long long o)
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_HvxxP(const HH &h, long long base, long long o)
 {
     return (dxx<CHK>(h.Hv, base, o) + dxx<CHK>(h.Hv, base, o - h.W)) * 0.5;
 }
-template <bool CHK>
-__device__ __forceinline__ double h_HveeR(const H &h, long long base, long long o)
+template <bool CHK, class HH>
+__device__ __forceinline__ double h_HveeR(const HH &h, long long base, long long o)
 {
     return (dee<CHK>(h.Hv, base, o, h.W) + dee<CHK>(h.Hv, base, o + h.W, h.W)) * 0.5;
 }
 constexpr double Gadv = -0.5;
-template <bool CHK>
-__device__ __forceinline__ double UFxR(const H &h, long long b, long long o)
+template <bool CHK, class HH>
+__device__ __forceinline__ double UFxR(const HH &h, long long b, long long o)
 {
     return h_uR<CHK>(h, b, o) + (Gadv * h_uxxR<CHK>(h, b, o)) * (h_HuR<CHK>(h, b, o) + Gadv * h_HuxxR<CHK>(h, b, o));
 }
-template <bool CHK>
-__device__ __forceinline__ double UFeP(const H &h, long long b, long long o)
+template <bool CHK, class HH>
+__device__ __forceinline__ double UFeP(const HH &h, long long b, long long o)
 {
     return h_uP<CHK>(h, b, o) + (Gadv * h_ueeP<CHK>(h, b, o)) * (h_HvP<CHK>(h, b, o) + Gadv * h_HvxxP<CHK>(h, b, o));
 }
-template <bool CHK>
-__device__ __forceinline__ double VFxP(const H &h, long long b, long long o)
+template <bool CHK, class HH>
+__device__ __forceinline__ double VFxP(const HH &h, long long b, long long o)
 {
     return h_vP<CHK>(h, b, o) + (Gadv * h_ueeP<CHK>(h, b, o)) * (h_HuP<CHK>(h, b, o) + Gadv * h_HueeP<CHK>(h, b, o));
 }
-template <bool CHK>
-__device__ __forceinline__ double VFeR(const H &h, long long b, long long o)
+template <bool CHK, class HH>
+__device__ __forceinline__ double VFeR(const HH &h, long long b, long long o)
 {
     return h_vR<CHK>(h, b, o) + (Gadv * h_uxxR<CHK>(h, b, o)) * (h_HvR<CHK>(h, b, o) + Gadv * h_HveeR<CHK>(h, b, o));
 }
 #undef AT
 
-__global__ __launch_bounds__(256) void k_horizontalAdvection(G3 g, H h, double *ru, double *rv)
+// Tiled form: a workgroup owns HTI x HTJ water columns and walks the levels; at each
+// level it stages u, v, Huon, Hvom over the tile and its halo (3 rows below, 2 above,
+// 2 columns each side: the reach of the expression tree) in LDS, so each value is
+// read from global memory once per tile instead of once per use.  The staged rows are
+// flat index ranges, so offsets that wrap across rows or levels (the reference's flat
+// Eval arithmetic) and reads outside the arrays (0.0) are reproduced exactly; the
+// arithmetic is the reference's expression tree in its operand order.
+constexpr int HTI = 64, HTJ = 8, HNT = HTI * HTJ, HLW = HTI + 4, HLH = HTJ + 5, HLN = HLW * HLH;
+
+__global__ __launch_bounds__(HNT) void k_horizontalAdvection_tiled(G3 g, H h, double *ru, double *rv, int tiles_i)
 {
-    COLUMN_INDEX
-    const long long W = g.szI;
-    const long long n = h.u.n;   // u, v, Huon, Hvom share one size
+    __shared__ double sl[4][HLN];
+    const int ti = blockIdx.x % tiles_i, tj = blockIdx.x / tiles_i;
+    const int i0 = ti * HTI, j0 = tj * HTJ;
+    const int c = threadIdx.x % HTI, r = threadIdx.x / HTI;
+    const int i = i0 + c, j = j0 + r;
+    const bool act = i < g.szI && j < g.szJ;
+    const long long W = g.szI, n = h.u.n;
+    const HL hl{AL{sl[0]}, AL{sl[1]}, AL{sl[2]}, AL{sl[3]}, HLW};
+    const long long lb = (long long)(r + 3) * HLW + (c + 2);   // the column's element in the tile view
+    const long long row0 = (long long)(j0 - 3) * W + (i0 - 2);  // flat index of tile-view element 0, level 0
     for (long long K = 0; K < g.N; K++) {
-        const long long b = K * g.S2 + f;
-        double du, dv;
-        if (b - 4 * W >= 0 && b + 4 * W < n) {   // every read of the tree is in range
-            du = (UFxR<false>(h, b, 0) - UFxR<false>(h, b, -1)) + (UFeP<false>(h, b, 0) - UFeP<false>(h, b, -W));
-            dv = (VFxP<false>(h, b, 0) - VFxP<false>(h, b, -1)) + (VFeR<false>(h, b, 0) - VFeR<false>(h, b, -W));
-        } else {
-            du = (UFxR<true>(h, b, 0) - UFxR<true>(h, b, -1)) + (UFeP<true>(h, b, 0) - UFeP<true>(h, b, -W));
-            dv = (VFxP<true>(h, b, 0) - VFxP<true>(h, b, -1)) + (VFeR<true>(h, b, 0) - VFeR<true>(h, b, -W));
+        const long long kb = K * g.S2;
+        for (int e = threadIdx.x; e < HLN; e += HNT) {
+            const int rr = e / HLW, cc = e - rr * HLW;
+            const long long gi = kb + row0 + (long long)rr * W + cc;
+            const bool ok = gi >= 0 && gi < n;
+            const long long gc = ok ? gi : 0;
+            const double a0 = h.u.p[gc], a1 = h.v.p[gc], a2 = h.Hu.p[gc], a3 = h.Hv.p[gc];
+            sl[0][e] = ok ? a0 : 0.0;
+            sl[1][e] = ok ? a1 : 0.0;
+            sl[2][e] = ok ? a2 : 0.0;
+            sl[3][e] = ok ? a3 : 0.0;
         }
-        ru[b] = ru[b] - du;
-        rv[b] = rv[b] - dv;
+        __syncthreads();
+        if (act) {
+            const double du = (UFxR<false>(hl, lb, 0) - UFxR<false>(hl, lb, -1)) +
+                              (UFeP<false>(hl, lb, 0) - UFeP<false>(hl, lb, -HLW));
+            const double dv = (VFxP<false>(hl, lb, 0) - VFxP<false>(hl, lb, -1)) +
+                              (VFeR<false>(hl, lb, 0) - VFeR<false>(hl, lb, -HLW));
+            const long long b = kb + (long long)j * W + i;
+            ru[b] = ru[b] - du;
+            rv[b] = rv[b] - dv;
+        }
+        __syncthreads();   // the next level overwrites the tile
     }
 }
 
@@ -231,16 +273,24 @@ __global__ __launch_bounds__(256) void k_verticalAdvection(G3 g, A U, A V, A Wa,
 #define FV(k) (v4(V, S2, (k), f) * h4(Wa, S2, (k), f, W))
     // interior levels: every read is in range (inner cell, levels K-1 .. K+3 <= N-1);
     // the flux at the upper face of level K is the lower-face flux of level K+1
+    // (v4 at level K reads levels K-1 .. K+2: a sliding window of four values per
+    // field, one new load per level)
     if (N - 2 > 2) {
-        double fu = v4<false>(U, S2, 2, f) * h4<false>(Wa, S2, 2, f, 1);
-        double fv = v4<false>(V, S2, 2, f) * h4<false>(Wa, S2, 2, f, W);
+        const double *Up = U.p + f, *Vp = V.p + f;
+        double ua = Up[S2], ub = Up[2 * S2], uc = Up[3 * S2], ud = Up[4 * S2];
+        double va = Vp[S2], vb = Vp[2 * S2], vc = Vp[3 * S2], vd = Vp[4 * S2];
+        double fu = ((9.0 / 16.0) * (ub + uc) - (1.0 / 16.0) * (ua + ud)) * h4<false>(Wa, S2, 2, f, 1);
+        double fv = ((9.0 / 16.0) * (vb + vc) - (1.0 / 16.0) * (va + vd)) * h4<false>(Wa, S2, 2, f, W);
         for (long long K = 2; K < N - 2; K++) {
-            const double fu1 = v4<false>(U, S2, K + 1, f) * h4<false>(Wa, S2, K + 1, f, 1);
-            const double fv1 = v4<false>(V, S2, K + 1, f) * h4<false>(Wa, S2, K + 1, f, W);
+            const double ue = Up[(K + 3) * S2], ve = Vp[(K + 3) * S2];
+            const double fu1 = ((9.0 / 16.0) * (uc + ud) - (1.0 / 16.0) * (ub + ue)) * h4<false>(Wa, S2, K + 1, f, 1);
+            const double fv1 = ((9.0 / 16.0) * (vc + vd) - (1.0 / 16.0) * (vb + ve)) * h4<false>(Wa, S2, K + 1, f, W);
             ru[K * S2 + f] = ru[K * S2 + f] - (fu1 - fu);
             rv[K * S2 + f] = rv[K * S2 + f] - (fv1 - fv);
             fu = fu1;
             fv = fv1;
+            ub = uc, uc = ud, ud = ue;
+            vb = vc, vc = vd, vd = ve;
         }
     }
     rv[1 * S2 + f] -= (9.0 / 16.0) * (FV(1) + FV(2)) - (1.0 / 16.0) * (FV(1) + FV(3));
@@ -488,8 +538,9 @@ extern "C" int sf_horizontal_advection(sf_ctx *c, const double *u, const double 
     SF_REQUIRE(u && v && Huon && Hvom && ru && rv, "sf_horizontal_advection: null field");
     const long long n3 = (c->szK + 1LL) * c->S2;
     H h{A{u, n3}, A{v, n3}, A{Huon, n3}, A{Hvom, n3}, c->szI};
-    hipLaunchKernelGGL(k_horizontalAdvection, dim3(sf::grid1d(c->S2)), dim3(sf::kBlock1D), 0, sf::as_stream(stream),
-                       grid3(c), h, ru, rv);
+    const int tiles_i = (c->szI + HTI - 1) / HTI, tiles_j = (c->szJ + HTJ - 1) / HTJ;
+    hipLaunchKernelGGL(k_horizontalAdvection_tiled, dim3((unsigned)(tiles_i * tiles_j)), dim3(HNT), 0,
+                       sf::as_stream(stream), grid3(c), h, ru, rv, tiles_i);
     int rc = sf::check_launch("sf_horizontal_advection");
     if (rc != SF_OK) return rc;
     const long long end = launch_threads < n3 ? (long long)launch_threads : n3;

@@ -36,7 +36,8 @@ def _oracle3(cfg, GRID, ct, N, fused_fix_q5=True):
     return t, g, s, b, m
 
 
-@pytest.mark.parametrize("szI,szJ,N,fused", [(24, 14, 6, False), (40, 20, 8, True), (33, 17, 5, False)])
+@pytest.mark.parametrize("szI,szJ,N,fused", [(24, 14, 6, False), (40, 20, 8, True), (33, 17, 5, False),
+                                            (130, 21, 4, False)])
 def test_main3d_bitwise(szI, szJ, N, fused):
     import torch
     from oracle import oracle as O
