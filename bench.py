@@ -22,7 +22,10 @@ around 99 back-to-back launches (the per-launch argument blocks of the three
 time-slot rotations are built once, so the host only issues the launches).
 ``cpu_baseline`` times the CPU oracle (C restatement, OpenMP) on a bounded number
 of substeps of the same workload.  ``secondary`` repeats the GPU measurement in the
-other precision.  ``--calib-copy`` instead runs a plain device copy of known size
+other precision.  ``roofline.copy_GBs`` is the bandwidth of a plain 1 GiB device copy
+on the same GPU (the practical streaming ceiling); ``large_grid`` repeats the FP32
+kernel timing on a 4096 x 4096 basin whose working set is far beyond the 256 MB
+last-level cache.  ``--calib-copy`` instead runs a plain device copy of known size
 (the calibration run for the HBM byte counters).
 """
 from __future__ import annotations
@@ -155,6 +158,68 @@ def run_phys_gpu(args, dtype_name, steps, warmup):
     return dict(cfg=cfg, S2=S2, ms_per_step=ms_per_step, wall=wall, kern_ms=kern_ms, bpp=bpp,
                 achieved=achieved, finite=finite, zmax=zmax, nk=nk, kernel="k_phys_fused",
                 entry="sf_phys_substep_fused", tile=[64, tile_j], substeps=n[0] + nk)
+
+
+def run_large_grid_gpu(args):
+    """The FP32 physics substep on a grid whose working set (2.2 GB) is far beyond the
+    256 MB last-level cache: 4096 x 4096 basin2_analytic, 99 back-to-back launches."""
+    import ctypes as C
+    import torch
+    from sailfish_amd import _lib, configs
+    from sailfish_amd.model2d import build_model
+    from sailfish_amd.phys2d import PhysStepper
+
+    n = args.large
+    cfg = configs.basin2_analytic(n, n, dx=args.dx)
+    ct, GRID, OCEAN, _B = build_model(cfg, dtype=torch.float32, ndtfast=30)
+    st = PhysStepper(GRID, OCEAN, rdrg=cfg["rdrg"], fused=True)
+    tile_j = st.lib.sf_phys_fused_default_tile(st.precision, GRID.szI, GRID.szJ)
+    for k in range(3):
+        ct.next2DTimeStep()
+        OCEAN.cycleTimes2D()
+        st.step(ct, OCEAN, first=(k == 0))
+    blocks = []
+    for _ in range(3):
+        OCEAN.cycleTimes2D()
+        blocks.append(st.fields(OCEAN))
+    fn, sh = st.lib.sf_phys_substep_fused, _lib.stream_handle()
+    nk, rc = 99, 0
+    torch.cuda.synchronize()
+    a, b = _events(torch, 1)[0]
+    a.record()
+    for k in range(nk):
+        ct.next2DTimeStep()
+        rc |= fn(C.byref(blocks[k % 3]), st.precision, float(ct.dtfast), 0, ct.cff1(), tile_j, sh)
+    b.record()
+    torch.cuda.synchronize()
+    _lib.check(rc, "sf_phys_substep_fused")
+    ms = a.elapsed_time(b) / nk
+    S2 = GRID.szI * GRID.szJ
+    bpp = PHYS_BULK * 4 + PHYS_F64 * 8
+    return {"workload": f"basin2_analytic {n}x{n} (FP32 state, FP64 averages)", "points": S2, "kernel_ms": ms,
+            "cell_steps_per_s": S2 / (ms * 1e-3), "achieved_GBs": bpp * S2 / (ms * 1e-3) / 1e9,
+            "frac": bpp * S2 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "finite_after_run": bool(torch.isfinite(OCEAN.zeta_t2).all())}
+
+
+def copy_bandwidth():
+    """Device-to-device copy of 1 GiB (read + write) with torch: the bandwidth a plain
+    streaming kernel reaches on this GPU."""
+    import torch
+    n = 1 << 30
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = _events(torch, 1)[0]
+    a.record()
+    for _ in range(10):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / 10
+    del src, dst
+    return 2 * n / (ms * 1e-3) / 1e9
 
 
 def run_faithful_gpu(args, dtype_name, steps, warmup):
@@ -491,6 +556,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-closure", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--large", type=int, default=4096, help="side of the beyond-cache grid (0: skip)")
     ap.add_argument("--szI3", type=int, default=1024)
     ap.add_argument("--szJ3", type=int, default=512)
     ap.add_argument("--N3", type=int, default=32)
@@ -560,6 +626,10 @@ def main():
             "launches_timed": r["nk"],
         },
     }
+    out["roofline"]["copy_GBs"] = copy_bandwidth()
+    if args.large and args.mode == "phys":
+        out["large_grid"] = run_large_grid_gpu(args)
+        progress("large grid")
     if not args.no_secondary:
         other = "f32" if args.dtype == "f64" else "f64"
         s2 = run(args, other, min(args.steps, 200), args.warmup)

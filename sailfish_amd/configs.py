@@ -86,6 +86,21 @@ def basin2(szI: int = 2048, szJ: int = 1024, dx: float = 2e3, seed_h: int = 2,
                 zeta0=zeta0, dtfast=5.0, nsteps=1000, rdrg=3e-4)
 
 
+def basin2_analytic(szI: int = 4096, szJ: int = 4096, dx: float = 2e3) -> dict:
+    """The basin2 recipe (beta plane, cos wind, bathymetry in [200, 4000] m, small
+    free-surface bumps) with analytic fields instead of smoothed noise, so grids far
+    larger than the 256 MB last-level cache set up in seconds (throughput runs)."""
+    X, Y, Lx, Ly = _coords(szI, szJ, dx, dx)
+    s = np.sin(6 * np.pi * X / Lx) * np.cos(4 * np.pi * Y / Ly)
+    h = 200.0 + (0.5 + 0.5 * s) * (4000.0 - 200.0)
+    f = 1e-4 + 2e-11 * Y
+    zeta0 = 0.01 * np.sin(10 * np.pi * X / Lx) * np.sin(8 * np.pi * Y / Ly)
+    return dict(name="basin2_analytic", szI=szI, szJ=szJ, dx=dx, dy=dx, h=h, f=f,
+                pm=np.full_like(h, 1.0 / dx), pn=np.full_like(h, 1.0 / dx),
+                sustr=0.1 * np.cos(2 * np.pi * Y / Ly) / RHO0, svstr=np.zeros_like(h),
+                zeta0=zeta0, dtfast=5.0, nsteps=1000, rdrg=3e-4)
+
+
 def random_smooth(szI: int, szJ: int, seed: int = 0) -> dict:
     """Small non-square test basin with smooth random bathymetry, metrics that vary in
     space and a random free-surface and velocity state (for parity tests)."""
