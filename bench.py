@@ -457,8 +457,8 @@ def run_closure_gpu(args):
 def run_3d_gpu(args):
     """Baroclinic 3D kernels (faithful reference discretisation, FP64) on a 1024 x 512
     basin1-recipe grid with 32 s-levels: rhs3d (Coriolis, horizontal and vertical
-    advection), set_maxflux and omega, each timed with HIP events on the launch
-    stream.  A 3D cell-step = one (i, j, k) rho cell x one baroclinic step."""
+    advection) as one fused launch and as the three one-to-one kernels, set_maxflux and
+    omega, each timed with HIP events on the launch stream.  A 3D cell-step = one (i, j, k) rho cell x one baroclinic step."""
     import numpy as np
     import torch
     from sailfish_amd import configs
@@ -489,9 +489,14 @@ def run_3d_gpu(args):
                                                                             OCEAN.ru_t2, OCEAN.rv_t2, BC), ctx=c),
         "verticalAdvection": lambda: ops.verticalAdvection(grsz, bksz, (OCEAN.u_t2, OCEAN.v_t2, W, OCEAN.ru_t2,
                                                                         OCEAN.rv_t2, BC), ctx=c),
+        "rhs3d_fused": lambda: ops.rhs3d_fused((GRID.fomn.reshape(-1), OCEAN.u_t2, OCEAN.v_t2, Huon, Hvom, W, Hz,
+                                                OCEAN.ru_t2, OCEAN.rv_t2, BC), launch_threads=(S2 // 512 + 1) * 512,
+                                               ctx=c),
     }
-    # algorithmic FP64 bytes per 3D cell (fields read and/or written once)
-    fields = {"set_maxflux": 5, "omega": 4, "addCoriolis": 6, "horizontalAdvection": 6, "verticalAdvection": 5}
+    # algorithmic FP64 bytes per 3D cell (fields read and/or written once); the fused
+    # rhs3d reads u, v, Huon, Hvom, W, Hz and writes ru, rv
+    fields = {"set_maxflux": 5, "omega": 4, "addCoriolis": 6, "horizontalAdvection": 6, "verticalAdvection": 5,
+              "rhs3d_fused": 8}
     for f in kern.values():
         f()
     torch.cuda.synchronize()
@@ -507,15 +512,57 @@ def run_3d_gpu(args):
         torch.cuda.synchronize()
         ms = a.elapsed_time(b) / reps
         out[name] = {"ms": ms, "GB_s": fields[name] * 8 * cells / (ms * 1e-3) / 1e9}
-    tot = sum(v["ms"] for v in out.values())
+    step = ("rhs3d_fused", "set_maxflux", "omega")      # the product path of one baroclinic step
+    tot = sum(out[k]["ms"] for k in step)
     cells_int = S2 * N
-    return {
+    res = {
         "workload": f"{szI}x{szJ}x{N} s-levels, basin1 recipe (dx 5 km), FP64, reference (faithful) 3D kernels",
         "kernels": out,
         "ms_rhs3d_maxflux_omega": tot,
         "cell_steps_per_s_3d": cells_int / (tot * 1e-3),
-        "hbm_frac_3d": sum(fields[k] * 8 * cells for k in kern) / (tot * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "hbm_frac_3d": sum(fields[k] * 8 * cells for k in step) / (tot * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "note": "rhs3d_fused (one launch) is the product path; addCoriolis / horizontalAdvection / "
+                "verticalAdvection are the one-to-one reference kernels, timed for comparison",
     }
+    if not args.no_cpu:
+        host = {k: x.detach().cpu().numpy().astype(np.float64).ravel() for k, x in
+                dict(u=OCEAN.u_t2, v=OCEAN.v_t2, Huon=Huon, Hvom=Hvom, W=W, Hz=Hz, z_w=z_w).items()}
+        res["cpu_baseline"] = run_cpu_3d(GRID, BOUNDARY, host, N, args.cpu_budget / 3)
+    return res
+
+
+def run_cpu_3d(GRID, BOUNDARY, host, N, budget_s):
+    """The same 3D kernels (rhs3d with the reference's launch sizes, set_maxflux, omega)
+    in the oracle's C restatement (FP64, OpenMP) on the same grid and state."""
+    import numpy as np
+    from oracle import oracle as O
+    from oracle import oracle3d as O3
+
+    threads = cpu_threads()
+    O.set_threads(threads)
+    hh = GRID.host
+    g2 = O.OGrid(GRID.szI, GRID.szJ, 1, dict(h=hh["h"], pm=hh["pm"], pn=hh["pn"], on_u=hh["on_u"],
+                                           om_v=hh["om_v"]))
+    m = O3.OModel3(g2, N, hh["fomn"], hh["sc_r"], hh["sc_w"], hh["Cs_r"], hh["Cs_w"], GRID.hc, GRID.Vtransform,
+                   BOUNDARY.zetaBC.bcIdxFieldIdx2, BOUNDARY.uvelBC.bcIdxFieldType,
+                   BOUNDARY.vvelBC.bcIdxFieldType)
+    m.u[2][:], m.v[2][:] = host["u"], host["v"]
+    m.Huon[:], m.Hvom[:], m.W[:], m.z_w[:] = host["Huon"], host["Hvom"], host["W"], host["z_w"]
+    m.Hz[:] = host["Hz"][:m.Hz.size]
+    L = O3.lib()
+    n, t0 = 0, time.perf_counter()
+    while True:
+        m.rhs3d()
+        L.sfo_set_maxflux(m.cp, O3.ptr(m.u[2]), O3.ptr(m.v[2]), O3.ptr(m.Huon), O3.ptr(m.Hvom), O3.ptr(m.Hz))
+        L.sfo_omega(m.cp, O3.ptr(m.W), O3.ptr(m.Huon), O3.ptr(m.Hvom), O3.ptr(m.z_w), O3.ptr(m.BC))
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 1000:
+            break
+    cells = GRID.szI * GRID.szJ * N
+    return dict(value=cells * n / el, unit="3D cell-steps/s", cores=threads, kind="port",
+                sample=f"{n} x (rhs3d + set_maxflux + omega) on {GRID.szI}x{GRID.szJ}x{N} (oracle C restatement, "
+                       f"FP64, {threads} OpenMP threads, {el:.1f} s)")
 
 
 def calib_copy(args):

@@ -135,6 +135,15 @@ int sf_horizontal_advection(sf_ctx *ctx, const double *u, const double *v, const
 int sf_vertical_advection(sf_ctx *ctx, const double *u, const double *v, const double *W,
                           double *ru, double *rv, const int32_t *BC, void *stream);
 
+/* rhs3d as ONE launch (nonlinear/rhs.py:38-71 with rhs_kernels.cpp:6-197): ru, rv
+ * := 0 over all szK+1 levels, then addCoriolis (cor != 0), horizontalAdvection with
+ * the zeros of its threads past S2 (launch_threads as for sf_horizontal_advection) and
+ * verticalAdvection (adv != 0), each element rounded in the same sequence as by the
+ * four separate launches, so the result equals them bit for bit. */
+int sf_rhs3d(sf_ctx *ctx, const double *fomn, const double *u, const double *v, const double *Huon,
+             const double *Hvom, const double *W, const double *Hz, const int32_t *BC, double *ru, double *rv,
+             int64_t launch_threads, int32_t cor, int32_t adv, void *stream);
+
 /* set_maxflux (nonlinear/step3D_uv.kernels.cpp:372-405). */
 int sf_set_maxflux(sf_ctx *ctx, const double *u, const double *v, double *Huon, double *Hvom,
                    const double *Hz, void *stream);

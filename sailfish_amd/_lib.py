@@ -56,6 +56,7 @@ _SIGS = {
     "sf_add_coriolis": [P, P, P, P, P, P, P, P],
     "sf_horizontal_advection": [P, P, P, P, P, P, P, I64, P],
     "sf_vertical_advection": [P, P, P, P, P, P, P, P],
+    "sf_rhs3d": [P, P, P, P, P, P, P, P, P, P, P, I64, I32, I32, P],
     "sf_set_maxflux": [P, P, P, P, P, P, P],
     "sf_omega": [P, P, P, P, P, P, P],
     "sf_set_depth": [P, I32, P, P, P, P, F64, P, P, P, P, P, P],

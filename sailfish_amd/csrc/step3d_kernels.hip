@@ -312,6 +312,129 @@ __global__ __launch_bounds__(256) void k_verticalAdvection(G3 g, A U, A V, A Wa,
 #undef FV
 }
 
+// ---- rhs3d in one launch (nonlinear/rhs.py:38-71) ---------------------------------
+// ru, rv := 0; addCoriolis; horizontalAdvection (+ the zeros its threads past S2
+// store); verticalAdvection -- evaluated per column and level in that order, so every
+// element goes through the same sequence of roundings as with the four launches:
+//   r = 0;  r = r + rtou (rho nodes);  r = r - du;  r = 0 (flat index in the tail
+//   range);  r = r - (fu(K+1) - fu(K)) (inner cells, levels 2..N-3)
+// and the special levels 0, 1, N-2 and the level copies of verticalAdvection follow
+// per column after the level loop.  u, v, Huon, Hvom of each level are staged in LDS
+// over the tile and its halo (as in k_horizontalAdvection_tiled); W and the vertical
+// stencils are read from global memory, u/v along the column through a sliding window.
+template <int TJ>
+__global__ __launch_bounds__(HTI * TJ) void k_rhs3d_fused(G3 g, H h, A fomn, A Hz, A Wa, const int32_t *BC,
+                                                          double *ru, double *rv, long long tail_end, int cor,
+                                                          int adv, int tiles_i)
+{
+    constexpr int NT = HTI * TJ, LN = HLW * (TJ + 5);
+    __shared__ double sl[4][LN];
+    const int ti = blockIdx.x % tiles_i, tj = blockIdx.x / tiles_i;
+    const int i0 = ti * HTI, j0 = tj * TJ;
+    const int c = threadIdx.x % HTI, r = threadIdx.x / HTI;
+    const int i = i0 + c, j = j0 + r;
+    const bool act = i < g.szI && j < g.szJ;
+    const long long W = g.szI, S2 = g.S2, N = g.N, n = h.u.n;
+    const long long f = act ? (long long)j * W + i : 0;
+    const HL hl{AL{sl[0]}, AL{sl[1]}, AL{sl[2]}, AL{sl[3]}, HLW};
+    const long long lb = (long long)(r + 3) * HLW + (c + 2);
+    const long long row0 = (long long)(j0 - 3) * W + (i0 - 2);
+    const bool rnode = act && cor && isRNode(g, f);
+    const bool vadv = act && adv && BC[f] < 0 && isInnerCell(g, f);
+    const bool vint = vadv && N - 2 > 2;   // interior levels 2 .. N-3 handled in the level loop
+    double fo_c = 0.0, fo_w = 0.0, fo_s = 0.0;
+    if (rnode) {
+        fo_c = fomn.p[f];
+        fo_w = fomn.p[f - 1];
+        fo_s = fomn.p[f - W];
+    }
+    // vertical-flux window of the column (v4 at level K reads levels K-1 .. K+2)
+    double ub = 0.0, uc = 0.0, ud = 0.0, vb = 0.0, vc = 0.0, vd = 0.0, fu = 0.0, fv = 0.0;
+    if (vint) {
+        const double *Up = h.u.p + f, *Vp = h.v.p + f;
+        const double ua = Up[S2], va = Vp[S2];
+        ub = Up[2 * S2], uc = Up[3 * S2], ud = Up[4 * S2];
+        vb = Vp[2 * S2], vc = Vp[3 * S2], vd = Vp[4 * S2];
+        fu = ((9.0 / 16.0) * (ub + uc) - (1.0 / 16.0) * (ua + ud)) * h4<false>(Wa, S2, 2, f, 1);
+        fv = ((9.0 / 16.0) * (vb + vc) - (1.0 / 16.0) * (va + vd)) * h4<false>(Wa, S2, 2, f, W);
+    }
+    for (long long K = 0; K < N; K++) {
+        const long long kb = K * S2;
+        for (int e = threadIdx.x; e < LN; e += NT) {
+            const int rr = e / HLW, cc = e - rr * HLW;
+            const long long gi = kb + row0 + (long long)rr * W + cc;
+            const bool ok = gi >= 0 && gi < n;
+            const long long gc = ok ? gi : 0;
+            const double a0 = h.u.p[gc], a1 = h.v.p[gc], a2 = h.Hu.p[gc], a3 = h.Hv.p[gc];
+            sl[0][e] = ok ? a0 : 0.0;
+            sl[1][e] = ok ? a1 : 0.0;
+            sl[2][e] = ok ? a2 : 0.0;
+            sl[3][e] = ok ? a3 : 0.0;
+        }
+        __syncthreads();
+        if (act) {
+            const long long b = kb + f;
+            double xu = 0.0, xv = 0.0;
+            if (rnode) {
+                const double *su = sl[0], *sv = sl[1];
+                const double hzc = Hz.p[b], hzw = Hz.p[b - 1], hzs = Hz.p[b - W];
+                const double UF0 = (hzc * fo_c) * ((sv[lb] + sv[lb + HLW]) * 0.5);
+                const double UF1 = (hzw * fo_w) * ((sv[lb - 1] + sv[lb - 1 + HLW]) * 0.5);
+                const double VF0 = (hzc * fo_c) * ((su[lb] + su[lb + 1]) * 0.5);
+                const double VF1 = (hzs * fo_s) * ((su[lb - HLW] + su[lb - HLW + 1]) * 0.5);
+                xu = xu + (UF0 + UF1) * 0.5;
+                xv = xv - (VF0 + VF1) * 0.5;
+            }
+            const double du = (UFxR<false>(hl, lb, 0) - UFxR<false>(hl, lb, -1)) +
+                              (UFeP<false>(hl, lb, 0) - UFeP<false>(hl, lb, -HLW));
+            const double dv = (VFxP<false>(hl, lb, 0) - VFxP<false>(hl, lb, -1)) +
+                              (VFeR<false>(hl, lb, 0) - VFeR<false>(hl, lb, -HLW));
+            xu = xu - du;
+            xv = xv - dv;
+            if (K >= 1 && b < tail_end) xu = xv = 0.0;
+            if (vint && K >= 2 && K < N - 2) {
+                const double ue = h.u.p[(K + 3) * S2 + f], ve = h.v.p[(K + 3) * S2 + f];
+                const double fu1 = ((9.0 / 16.0) * (uc + ud) - (1.0 / 16.0) * (ub + ue)) * h4<false>(Wa, S2, K + 1, f, 1);
+                const double fv1 = ((9.0 / 16.0) * (vc + vd) - (1.0 / 16.0) * (vb + ve)) * h4<false>(Wa, S2, K + 1, f, W);
+                xu = xu - (fu1 - fu);
+                xv = xv - (fv1 - fv);
+                fu = fu1;
+                fv = fv1;
+                ub = uc, uc = ud, ud = ue;
+                vb = vc, vc = vd, vd = ve;
+            }
+            ru[b] = xu;
+            rv[b] = xv;
+        }
+        __syncthreads();   // the next level overwrites the tile
+    }
+    if (!act) return;
+    ru[N * S2 + f] = 0.0;
+    rv[N * S2 + f] = 0.0;
+    if (!vadv) return;
+    // special levels and copies of verticalAdvection (same order as k_verticalAdvection)
+    const A U = h.u, V = h.v;
+#define FU(k) (v4(U, S2, (k), f) * h4(Wa, S2, (k), f, 1))
+#define FV(k) (v4(V, S2, (k), f) * h4(Wa, S2, (k), f, W))
+    rv[1 * S2 + f] -= (9.0 / 16.0) * (FV(1) + FV(2)) - (1.0 / 16.0) * (FV(1) + FV(3));
+    const double u0 = r3(U, f), u1 = r3(U, S2 + f), u2 = r3(U, 2 * S2 + f);
+    const double uuu = ((9.0 / 16.0) * (u0 + u1) - (1.0 / 16.0) * (0.0 + u2)) * h4(Wa, S2, 1, f, 1);
+    const double uuu2 = ((9.0 / 16.0) * (0.0 + u0) - (1.0 / 16.0) * (0.0 + u1)) * h4(Wa, S2, 0, f, 1);
+    ru[1 * S2 + f] -= (FU(2) - uuu);
+    ru[f] -= (uuu - uuu2);
+    ru[(N - 2) * S2 + f] -= (9.0 / 16.0) * (FU(N - 1) + FU(N - 2)) - (1.0 / 16.0) * (FU(N - 1) + FU(N - 3));
+    rv[(N - 2) * S2 + f] -= (9.0 / 16.0) * (FV(N - 1) + FV(N - 2)) - (1.0 / 16.0) * (FV(N - 1) + FV(N - 3));
+    rv[f] = rv[S2 + f];
+    ru[(N - 2) * S2 + f] = ru[(N - 3) * S2 + f];
+    rv[(N - 2) * S2 + f] = rv[(N - 3) * S2 + f];
+    ru[(N - 1) * S2 + f] = ru[(N - 2) * S2 + f];
+    rv[(N - 1) * S2 + f] = rv[(N - 2) * S2 + f];
+    ru[N * S2 + f] = ru[(N - 2) * S2 + f];
+    rv[N * S2 + f] = rv[(N - 2) * S2 + f];
+#undef FU
+#undef FV
+}
+
 // ---- set_maxflux (step3D_uv.kernels.cpp:372-405) ----------------------------------
 __global__ __launch_bounds__(256) void k_set_maxflux(G3 g, const double *u, const double *v, double *Huon,
                                                      double *Hvom, const double *Hz)
@@ -559,6 +682,23 @@ extern "C" int sf_vertical_advection(sf_ctx *c, const double *u, const double *v
     SF_REQUIRE(u && v && W && ru && rv && BC, "sf_vertical_advection: null field");
     const long long n3 = (c->szK + 1LL) * c->S2;
     LAUNCH_COLUMNS(k_verticalAdvection, c, stream, grid3(c), A{u, n3}, A{v, n3}, A{W, n3}, ru, rv, BC);
+}
+
+extern "C" int sf_rhs3d(sf_ctx *c, const double *fomn, const double *u, const double *v, const double *Huon,
+                        const double *Hvom, const double *W, const double *Hz, const int32_t *BC, double *ru,
+                        double *rv, int64_t launch_threads, int32_t cor, int32_t adv, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(fomn && u && v && Huon && Hvom && W && Hz && BC && ru && rv, "sf_rhs3d: null field");
+    const long long n3 = (c->szK + 1LL) * c->S2;
+    H h{A{u, n3}, A{v, n3}, A{Huon, n3}, A{Hvom, n3}, c->szI};
+    const long long end = launch_threads < n3 ? (long long)launch_threads : n3;
+    constexpr int TJ = 4;   // 256-thread workgroups: three per CU at this kernel's register count
+    const int tiles_i = (c->szI + HTI - 1) / HTI, tiles_j = (c->szJ + TJ - 1) / TJ;
+    hipLaunchKernelGGL(k_rhs3d_fused<TJ>, dim3((unsigned)(tiles_i * tiles_j)), dim3(HTI * TJ), 0,
+                       sf::as_stream(stream), grid3(c), h, A{fomn, c->S2}, A{Hz, (long long)c->szK * c->S2},
+                       A{W, n3}, BC, ru, rv, end, cor ? 1 : 0, adv ? 1 : 0, tiles_i);
+    return sf::check_launch("sf_rhs3d");
 }
 
 extern "C" int sf_set_maxflux(sf_ctx *c, const double *u, const double *v, double *Huon, double *Hvom,

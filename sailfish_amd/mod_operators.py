@@ -205,6 +205,16 @@ def horizontalAdvection(grid, block, args, ctx=None):
               nt, _stream())
 
 
+def rhs3d_fused(args, launch_threads: int, cor: bool = True, adv: bool = True, ctx=None):
+    """nonlinear/rhs.py:38-71 as one launch: ru, rv := 0, addCoriolis (cor),
+    horizontalAdvection with the reference's launch of ``launch_threads`` threads,
+    verticalAdvection (adv).  args = (fomn, u, v, Huon, Hvom, W, Hz, ru, rv, BC)."""
+    fomn, u, v, Hu, Hv, W, Hz, ru, rv, BC = args
+    c = ctx or ctx_of()
+    _lib.call("sf_rhs3d", c.handle, _f64(fomn), _f64(u), _f64(v), _f64(Hu), _f64(Hv), _f64(W), _f64(Hz), _ptr(BC),
+              _f64(ru), _f64(rv), int(launch_threads), int(bool(cor)), int(bool(adv)), _stream())
+
+
 def verticalAdvection(grid, block, args, ctx=None):
     """rhs_kernels.cpp:135-197; args = (u, v, W, ru, rv, BC)."""
     u, v, W, ru, rv, BC = args

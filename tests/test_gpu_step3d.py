@@ -80,3 +80,39 @@ def test_main3d_bitwise(szI, szJ, N, fused):
     bad2 = compare_states(prod, orc, skip=skip)
     assert not bad and not bad2, (bad, bad2)
     assert np.all(np.isfinite(OCEAN.u.detach().cpu().numpy()))
+
+
+@pytest.mark.parametrize("szI,szJ,N,cor,adv", [(24, 14, 6, True, True), (130, 75, 9, True, True),
+                                              (33, 17, 4, False, True), (70, 19, 5, True, False),
+                                              (20, 10, 7, True, True)])
+def test_rhs3d_fused_equals_four_launches(szI, szJ, N, cor, adv):
+    """sf_rhs3d against zero + addCoriolis + horizontalAdvection (with its tail zeros) +
+    verticalAdvection, bit for bit, on random smooth 3D fields (the 20x10 grid has
+    S2 < 512, so the tail zeros reach past level 1)."""
+    import torch
+    from sailfish_amd import mod_operators as ops
+    from sailfish_amd import rhs as R
+    from sailfish_amd.model2d import build_model3d
+
+    print(f"[case] rhs3d fused {szI}x{szJ}x{N} cor={cor} adv={adv}", file=sys.stderr, flush=True)
+    cfg = _cfg3(szI, szJ, N, seed=szI * 7 + N)
+    ct, GRID, OCEAN, BOUNDARY = build_model3d(cfg, N=N, ndtfast=4, dt=8.0, fix_q5=True)
+    rng = np.random.default_rng(szJ)
+    for t in (OCEAN.Huon, OCEAN.Hvom, OCEAN.W):
+        t.copy_(torch.from_numpy(rng.standard_normal(t.numel())).reshape(t.shape).cuda())
+    for t in (OCEAN.u_t2, OCEAN.v_t2):
+        t.copy_(torch.from_numpy(0.1 * rng.standard_normal(t.numel())).cuda())
+    old = (R.UV_COR, R.UV_ADV)
+    try:
+        R.UV_COR, R.UV_ADV = cor, adv
+        R.rhs3d(GRID, OCEAN, BOUNDARY, fused=False)
+        ref_u, ref_v = OCEAN.ru_t2.clone(), OCEAN.rv_t2.clone()
+        OCEAN.ru_t2.fill_(7.0)
+        OCEAN.rv_t2.fill_(-7.0)
+        R.rhs3d(GRID, OCEAN, BOUNDARY, fused=True)
+    finally:
+        R.UV_COR, R.UV_ADV = old
+    torch.cuda.synchronize()
+    assert bitwise_equal(OCEAN.ru_t2.cpu().numpy(), ref_u.cpu().numpy())
+    assert bitwise_equal(OCEAN.rv_t2.cpu().numpy(), ref_v.cpu().numpy())
+    assert np.abs(ref_u.cpu().numpy()).max() > 0
