@@ -682,6 +682,16 @@ def main():
         s2 = run(args, other, min(args.steps, 200), args.warmup)
         out["secondary"] = summary(s2, other, min(args.steps, 200), args.warmup)
         progress(f"secondary {other}")
+    if not args.no_secondary and args.mode == "phys":
+        # the reference discretisation (faithful mode, FP64, bitwise equal to the oracle) on
+        # the same grid; 70 substeps keep it finite with the column-0/row-0 mask variant
+        fa = run_faithful_gpu(args, "f64", 60, 10)
+        out["faithful"] = summary(fa, "f64", 60, 10)
+        out["faithful"]["workload"] = ("reference step2d discretisation (nonlinear/step2d.py), fused tile kernel, "
+                                       f"{args.szI}x{args.szJ}, FP64")
+        if not args.no_cpu:
+            out["faithful"]["cpu_baseline"] = run_cpu_faithful(args, args.cpu_budget / 3)
+        progress("faithful f64")
     if not args.no_ensemble and args.mode == "phys":
         out["ensemble"] = run_ensemble_gpu(args)
         progress("ensemble")
