@@ -1060,6 +1060,31 @@ __global__ __launch_bounds__(NT) void k_phys_ensemble(const PFA<T> *__restrict__
         lds_barrier();   // the next tile overwrites the LDS planes
     }
 }
+
+// One workgroup per list entry (grid = number of tiles): no tile loop, so the tile
+// code is register-allocated as in the single-basin kernel.  order 0 takes entry b,
+// order 1 the XCD-banded entry of block b.
+template <typename T, int TI, int TJ, int NT, bool FIRST>
+__global__ __launch_bounds__(NT) void k_phys_ensemble1(const PFA<T> *__restrict__ members,
+                                                       const int2 *__restrict__ list, int order, int rot,
+                                                       double w1)
+{
+    __shared__ T sm[PhysLds<TI, TJ>::PLANES][PhysLds<TI, TJ>::FN];
+    __shared__ T side[2 * TI + 2 * TJ];
+    const int t = order == 0 ? (int)blockIdx.x : xcd_band(gridDim.x);
+    const SF_CONST int *clist = (const SF_CONST int *)list;
+    const SF_CONST PFA<T> *cmem = (const SF_CONST PFA<T> *)members;
+    const int ex = clist[2 * t], ey = clist[2 * t + 1];
+    PFA<T> P;
+    {
+        const SF_CONST uint32_t *src = (const SF_CONST uint32_t *)(cmem + 3 * ex + rot);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(&P);
+#pragma unroll
+        for (int w = 0; w < (int)(sizeof(PFA<T>) / 4); w++) dst[w] = src[w];
+    }
+    P.w1 = w1;
+    phys_tile<T, TI, TJ, NT, FIRST>(P, ey % P.tiles_i, ey / P.tiles_i, sm, side);
+}
 }  // namespace
 
 
@@ -1166,6 +1191,15 @@ void ens_launch_tj(sf_ens *e, int first, double w1, int rot, hipStream_t s)
     constexpr int TI = 64, NT = TJ == 16 ? 512 : 256;
     const int G = e->grid > 0 ? (e->grid < e->ntiles ? e->grid : e->ntiles) : e->ntiles;
     const PFA<T> *m = (const PFA<T> *)e->members;
+    if (G == e->ntiles) {   // one workgroup per tile
+        if (first)
+            hipLaunchKernelGGL((k_phys_ensemble1<T, TI, TJ, NT, true>), dim3(G), dim3(NT), 0, s, m, e->list,
+                               e->order, rot, w1);
+        else
+            hipLaunchKernelGGL((k_phys_ensemble1<T, TI, TJ, NT, false>), dim3(G), dim3(NT), 0, s, m, e->list,
+                               e->order, rot, w1);
+        return;
+    }
     if (first)
         hipLaunchKernelGGL((k_phys_ensemble<T, TI, TJ, NT, true>), dim3(G), dim3(NT), 0, s, m, e->list, e->ntiles,
                            e->order, rot, w1);
