@@ -130,3 +130,34 @@ def test_phys_fp32_error_vs_fp64():
     ea = np.abs(OCEAN.Zt_avg1.cpu().numpy() - o.st.Zt_avg1).max() / max(np.abs(o.st.Zt_avg1).max(), 1e-30)
     print(f"fp32 phys: rel zeta {ez:.2e}, rel ubar {eu:.2e}, rel Zt_avg1 {ea:.2e}", file=sys.stderr)
     assert ez <= 2e-4 and eu <= 2e-3 and ea <= 1e-4
+
+
+def test_phys_full_size_conservation_and_fp32():
+    """The benchmark workload itself (2048 x 1024 closed basin, basin2 recipe), 30
+    substeps: in FP64 the free-surface volume sum(zeta / (pm pn)) over the interior is
+    conserved to 1e-12 of sum(|zeta| / (pm pn)) (closed walls, flux-form continuity);
+    the FP32 run stays within 1e-4 of max|zeta| and 1e-3 of max|ubar| of the FP64 run;
+    both stay finite."""
+    import torch
+    print("[case] phys full size 2048x1024", file=sys.stderr, flush=True)
+    cfg = configs.basin2(2048, 1024)
+    area = (1.0 / (cfg["pm"] * cfg["pn"]))[1:-1, 1:-1]
+
+    def volume(z):
+        return float((z.reshape(1024, 2048)[1:-1, 1:-1] * area).sum()), float((np.abs(z.reshape(1024, 2048)[1:-1, 1:-1]) * area).sum())
+
+    _, _, O64 = _gpu_run(cfg, 1, torch.float64, True)
+    V0, S0 = volume(O64.zeta_t2.cpu().numpy())
+    _, _, O64 = _gpu_run(cfg, 30, torch.float64, True)
+    _, _, O32 = _gpu_run(cfg, 30, torch.float32, True)
+    z64 = O64.zeta_t2.cpu().numpy()
+    V1, _ = volume(z64)
+    print(f"volume drift {abs(V1 - V0) / S0:.2e}", file=sys.stderr)
+    assert abs(V1 - V0) <= 1e-12 * S0
+    z32 = O32.zeta_t2.cpu().double().numpy()
+    u64, u32 = O64.ubar_t2.cpu().numpy(), O32.ubar_t2.cpu().double().numpy()
+    assert np.isfinite(z64).all() and np.isfinite(z32).all()
+    ez = np.abs(z32 - z64).max() / np.abs(z64).max()
+    eu = np.abs(u32 - u64).max() / np.abs(u64).max()
+    print(f"fp32 vs fp64 at full size: zeta {ez:.2e}, ubar {eu:.2e}", file=sys.stderr)
+    assert ez <= 1e-4 and eu <= 1e-3

@@ -132,3 +132,52 @@ def test_fused_maps_rejections():
     assert not ok and "column-0" in msg
     ok, msg = check_maps(40, 20, 20, Boundary(_G(40, 20), None))
     assert ok, msg
+
+
+def test_c_abi_rejects_bad_arguments_without_a_gpu():
+    """Argument checks of the C ABI return SF_ERR_ARG with a message before any HIP
+    call: null handles and fields, unknown precision, unsupported tile heights, grids
+    too small, empty ensembles."""
+    from sailfish_amd import _lib
+    from sailfish_amd.phys2d import sf_phys_fields
+    lib = _lib.load()
+    lib.sf_last_error_string.restype = C.c_char_p
+    P = C.c_void_p
+    err = lambda: lib.sf_last_error_string().decode()
+
+    f = lib.sf_phys_substep_fused
+    f.argtypes = [C.POINTER(sf_phys_fields), C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_int32, P]
+    f.restype = C.c_int32
+    assert f(None, 0, 1.0, 0, 0.5, 8, None) == -1 and "null" in err()
+    a = sf_phys_fields()
+    a.szI, a.szJ = 2, 64
+    assert f(C.byref(a), 0, 1.0, 0, 0.5, 8, None) == -1 and "too small" in err()
+    # every pointer set to a dummy non-null address: the tile and precision checks
+    # reject the call before anything is launched or dereferenced
+    a.szI, a.szJ = 128, 64
+    for name, ct in sf_phys_fields._fields_:
+        if ct is C.c_void_p:
+            setattr(a, name, 0x1000)
+        elif getattr(ct, "_type_", None) is C.c_void_p:
+            fld = getattr(a, name)
+            for k in range(len(fld)):
+                fld[k] = 0x1000
+    assert f(C.byref(a), _lib.SF_FP64, 1.0, 0, 0.5, 5, None) == -1 and "tile height" in err()
+    assert f(C.byref(a), 7, 1.0, 0, 0.5, 8, None) == -1 and "precision" in err()
+
+    e = lib.sf_ens_create
+    e.argtypes = [C.POINTER(sf_phys_fields), C.POINTER(C.c_double), C.c_int32, C.c_int32, C.POINTER(P)]
+    e.restype = C.c_int32
+    out = P()
+    dt = (C.c_double * 1)(1.0)
+    assert e(C.byref(a), dt, 0, 0, C.byref(out)) == -1
+    assert e(C.byref(a), dt, 1, 9, C.byref(out)) == -1 and "precision" in err()
+    lib.sf_ens_configure.argtypes = [P, C.c_int32, C.c_int32, C.c_int32]
+    assert lib.sf_ens_configure(None, 8, 0, 0) == -1 and "null" in err()
+
+    lib.sf_mlp_create.argtypes = [C.c_int32, C.POINTER(P)]
+    assert lib.sf_mlp_create(0, C.byref(out)) == -1
+    lib.sf_closure_stress.argtypes = [P, P, C.c_int32, P, P, P, P]
+    assert lib.sf_closure_stress(None, None, 0, None, None, None, None) == -1 and "null" in err()
+    lib.sf_rhs3d.argtypes = [P] * 11 + [C.c_int64, C.c_int32, C.c_int32, P]
+    assert lib.sf_rhs3d(*([None] * 11), 0, 1, 1, None) == -1 and "null context" in err()
