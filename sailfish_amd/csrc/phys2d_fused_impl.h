@@ -119,10 +119,21 @@ __device__ __forceinline__ T ldo(const T *p, long long d, uint32_t boff)
     const uint32_t o = boff + (uint32_t)(d * (long long)sizeof(T));
     return *(const SF_GLOBAL T *)((const SF_GLOBAL char *)p + o);
 }
+// the FP64 time averages are read once per point; in the FP64 instances a streaming
+// (non-temporal) load measured faster, in the FP32 instances a plain one
+template <typename T>
+__device__ __forceinline__ double ldavg(const double *p, uint32_t boff)
+{
+    if constexpr (sizeof(T) == 8)
+        return __builtin_nontemporal_load((const SF_GLOBAL double *)((const SF_GLOBAL char *)p + boff));
+    return ldo(p, 0, boff);
+}
 template <typename T>
 __device__ __forceinline__ void sto(T *p, uint32_t boff, T v)
 {
-    *(SF_GLOBAL T *)((SF_GLOBAL char *)p + boff) = v;
+    // every output of the substep is written once and read by no other workgroup of
+    // this launch: streaming (non-temporal) stores keep L2 for the halo reads
+    __builtin_nontemporal_store(v, (SF_GLOBAL T *)((SF_GLOBAL char *)p + boff));
 }
 
 // Workgroup barrier that orders LDS only: global loads issued before it stay in
@@ -668,9 +679,9 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             if (cin[k] && rho_int(i, j)) {
                 const uint32_t fd = offd(i, j);
                 if (!first) rz0_[k] = ldo(P.rz0, 0, f);
-                zt_[k] = ldo(P.Zt, 0, fd);
-                dua_[k] = ldo(P.DUa, 0, fd);
-                dva_[k] = ldo(P.DVa, 0, fd);
+                zt_[k] = ldavg<T>(P.Zt, fd);
+                dua_[k] = ldavg<T>(P.DUa, fd);
+                dva_[k] = ldavg<T>(P.DVa, fd);
             }
             if (cin[k] && u_int(i, j)) {
                 if (!first) ru0_[k] = ldo(P.ru0, 0, f);
