@@ -436,6 +436,8 @@ __global__ __launch_bounds__(HTI * TJ) void k_rhs3d_fused(G3 g, H h, A fomn, A H
 }
 
 // ---- set_maxflux (step3D_uv.kernels.cpp:372-405) ----------------------------------
+// the fluxes are stored with streaming (non-temporal) stores: no thread of the launch
+// reads them back
 __global__ __launch_bounds__(256) void k_set_maxflux(G3 g, const double *u, const double *v, double *Huon,
                                                      double *Hvom, const double *Hz)
 {
@@ -443,12 +445,12 @@ __global__ __launch_bounds__(256) void k_set_maxflux(G3 g, const double *u, cons
     if (isUNode(g, f))
         for (long long K = 0; K < g.N; K++) {
             const long long b = K * g.S2 + f;
-            Huon[b] = (((Hz[b] + Hz[b - 1]) * 0.5) * u[b]) * g.on_u[f];
+            __builtin_nontemporal_store((((Hz[b] + Hz[b - 1]) * 0.5) * u[b]) * g.on_u[f], Huon + b);
         }
     if (isVNode(g, f))
         for (long long K = 0; K < g.N; K++) {
             const long long b = K * g.S2 + f;
-            Hvom[b] = (((Hz[b] + Hz[b - g.szI]) * 0.5) * v[b]) * g.om_v[f];
+            __builtin_nontemporal_store((((Hz[b] + Hz[b - g.szI]) * 0.5) * v[b]) * g.om_v[f], Hvom + b);
         }
 }
 
