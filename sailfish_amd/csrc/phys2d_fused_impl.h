@@ -56,6 +56,8 @@ struct PFA {
     int szI, szJ, tiles_i, tiles_j, off_i, off_j;
     T g, rdrg, dt;
     int first;
+    int nts;   // streaming (non-temporal) stores and average loads: set when one substep's
+               // bytes exceed what the 256 MB last-level cache keeps for the next substep
     double w1;
 };
 
@@ -119,21 +121,23 @@ __device__ __forceinline__ T ldo(const T *p, long long d, uint32_t boff)
     const uint32_t o = boff + (uint32_t)(d * (long long)sizeof(T));
     return *(const SF_GLOBAL T *)((const SF_GLOBAL char *)p + o);
 }
-// the FP64 time averages are read once per point; in the FP64 instances a streaming
-// (non-temporal) load measured faster, in the FP32 instances a plain one
-template <typename T>
+// The outputs of a substep are read by no other workgroup of the launch, and the FP64
+// averages are read once per point.  When a substep moves more bytes than the last-level
+// cache keeps between launches, streaming (non-temporal) stores and average loads keep L2
+// for the halo reads; below that size plain ones let the next substep find its inputs in
+// that cache.  NTS is a template parameter: a runtime choice between the two store forms
+// would be merged into one plain store by the compiler.
+template <bool NTS>
 __device__ __forceinline__ double ldavg(const double *p, uint32_t boff)
 {
-    if constexpr (sizeof(T) == 8)
-        return __builtin_nontemporal_load((const SF_GLOBAL double *)((const SF_GLOBAL char *)p + boff));
+    if constexpr (NTS) return __builtin_nontemporal_load((const SF_GLOBAL double *)((const SF_GLOBAL char *)p + boff));
     return ldo(p, 0, boff);
 }
-template <typename T>
+template <bool NTS, typename T>
 __device__ __forceinline__ void sto(T *p, uint32_t boff, T v)
 {
-    // every output of the substep is written once and read by no other workgroup of
-    // this launch: streaming (non-temporal) stores keep L2 for the halo reads
-    __builtin_nontemporal_store(v, (SF_GLOBAL T *)((SF_GLOBAL char *)p + boff));
+    if constexpr (NTS) __builtin_nontemporal_store(v, (SF_GLOBAL T *)((SF_GLOBAL char *)p + boff));
+    else *(SF_GLOBAL T *)((SF_GLOBAL char *)p + boff) = v;
 }
 
 // Workgroup barrier that orders LDS only: global loads issued before it stay in
@@ -180,7 +184,7 @@ struct UVIn {
     T x0, z0c, z0b;             // predictor "before" velocity and zeta (t0) at point / back
 };
 
-template <typename T, int TI, int TJ, int NT, bool FIRST>
+template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS>
 __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*sm)[PhysLds<TI, TJ>::FN],
                                           T *side)
 {
@@ -556,7 +560,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             if (cin[k]) {
                 if (rho_int(i, j)) {
                     zs = zs_of(o, pmnc[k], z0c_[k], r);
-                    sto(P.rz1, cf[k], r);
+                    sto<NTS>(P.rz1, cf[k], r);
                 } else {
                     const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
                     const uint32_t fq = off(qi, qj);
@@ -619,7 +623,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     x.z0b = z0w_[k];
                     T r;
                     us = pred_u(o, x, r);
-                    sto(P.ru1, cf[k], r);
+                    sto<NTS>(P.ru1, cf[k], r);
                     ruc[k] = r;
                     if (j == 1) sRUlo[a - 3] = r;
                     if (j == M - 1) sRUhi[a - 3] = r;
@@ -654,7 +658,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     x.z0b = z0s_[k];
                     T r;
                     vs = pred_v(o, x, r);
-                    sto(P.rv1, cf[k], r);
+                    sto<NTS>(P.rv1, cf[k], r);
                     rvc[k] = r;
                     if (i == 1) sRVlo[b - 3] = r;
                     if (i == L - 1) sRVhi[b - 3] = r;
@@ -679,9 +683,9 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             if (cin[k] && rho_int(i, j)) {
                 const uint32_t fd = offd(i, j);
                 if (!first) rz0_[k] = ldo(P.rz0, 0, f);
-                zt_[k] = ldavg<T>(P.Zt, fd);
-                dua_[k] = ldavg<T>(P.DUa, fd);
-                dva_[k] = ldavg<T>(P.DVa, fd);
+                zt_[k] = ldavg<NTS>(P.Zt, fd);
+                dua_[k] = ldavg<NTS>(P.DUa, fd);
+                dva_[k] = ldavg<NTS>(P.DVa, fd);
             }
             if (cin[k] && u_int(i, j)) {
                 if (!first) ru0_[k] = ldo(P.ru0, 0, f);
@@ -826,9 +830,9 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     z2 = z2_of(o, pmnc[k], z1c[k], rz0_[k]);
                     const double w = P.w1;
                     const uint32_t fd = offd(i, j);
-                    sto(P.Zt, fd, zt_[k] + w * (double)zsc[k]);
-                    sto(P.DUa, fd, dua_[k] + w * (double)duc[k]);
-                    sto(P.DVa, fd, dva_[k] + w * (double)dvc[k]);
+                    sto<NTS>(P.Zt, fd, zt_[k] + w * (double)zsc[k]);
+                    sto<NTS>(P.DUa, fd, dua_[k] + w * (double)duc[k]);
+                    sto<NTS>(P.DVa, fd, dva_[k] + w * (double)dvc[k]);
                 } else {
                     const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
                     const uint32_t fq = off(qi, qj);
@@ -837,7 +841,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     const T rz0 = first ? T(0) : ldo(P.rz0, 0, fq);
                     z2 = z2_of(oq, pmn, ldo(P.z1, 0, fq), rz0);
                 }
-                sto(P.z2, cf[k], z2);
+                sto<NTS>(P.z2, cf[k], z2);
             }
             // Z2 reuses the Z1 plane: every reader of Z1 finished before the last barrier
             sZ2[o] = z2;
@@ -921,7 +925,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                         ldo(P.fomn, 0, f), ldo(P.fomn, -1, f), ldo(P.sustr, 0, f), ru1, ru0, ldo(P.z1, 0, f),
                         ldo(P.z1, -1, f), sU1[o + (qj - j) * FW]);
         }
-        sto(P.u2, cf[k], u2);
+        sto<NTS>(P.u2, cf[k], u2);
         if (v_int(i, j)) {
             v2 = corr_v(o, omvc[k], onvc[k], cffvc[k], hs_[k], hc_[k], fo_[k], fos_[k], sv_[k], rvc[k], rv0_[k],
                         z1c[k], z1s_[k], v1c[k]);
@@ -934,7 +938,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                         ldo(P.fomn, 0, f), ldo(P.fomn, -W, f), ldo(P.svstr, 0, f), rv1, rv0, ldo(P.z1, 0, f),
                         ldo(P.z1, -W, f), sV1[o + (qi - i)]);
         }
-        sto(P.v2, cf[k], v2);
+        sto<NTS>(P.v2, cf[k], v2);
     }
 }
 
@@ -947,13 +951,13 @@ __device__ __forceinline__ int xcd_band(int nblk)
     return b;
 }
 
-template <typename T, int TI, int TJ, int NT, bool FIRST>
+template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS>
 __global__ __launch_bounds__(NT) void k_phys_fused(PFA<T> P)
 {
     __shared__ T sm[PhysLds<TI, TJ>::PLANES][PhysLds<TI, TJ>::FN];
     __shared__ T side[2 * TI + 2 * TJ];
     const int b = xcd_band(P.tiles_i * P.tiles_j);
-    phys_tile<T, TI, TJ, NT, FIRST>(P, b % P.tiles_i, b / P.tiles_i, sm, side);
+    phys_tile<T, TI, TJ, NT, FIRST, NTS>(P, b % P.tiles_i, b / P.tiles_i, sm, side);
 }
 
 template <typename T, int TI, int TJ, int NT>
@@ -994,17 +998,18 @@ int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void 
     a.rdrg = T(x->rdrg);
     a.dt = T(dt);
     a.first = first;
+    a.nts = (double)x->szI * x->szJ * (3.0 * sizeof(T) * 7 + 8.0 * 6) > 384.0 * (1 << 20) ? 1 : 0;
     a.w1 = w1;
     SF_REQUIRE((long long)x->szI * x->szJ < (1LL << 29), "sf_phys_substep_fused: grid of %dx%d points exceeds 2^29",
                x->szI, x->szJ);
     const long long nb = (long long)a.tiles_i * a.tiles_j;
     SF_REQUIRE(nb < (1LL << 31), "sf_phys_substep_fused: grid too large");
-    if (first)
-        hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true>), dim3((unsigned)nb), dim3(NT), 0,
-                           sf::as_stream(stream), a);
-    else
-        hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, false>), dim3((unsigned)nb), dim3(NT), 0,
-                           sf::as_stream(stream), a);
+    const dim3 grid((unsigned)nb), block(NT);
+    hipStream_t s = sf::as_stream(stream);
+    if (first && a.nts) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true, true>), grid, block, 0, s, a);
+    else if (first) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true, false>), grid, block, 0, s, a);
+    else if (a.nts) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, false, false>), grid, block, 0, s, a);
     return sf::check_launch("sf_phys_substep_fused");
 }
 
@@ -1033,7 +1038,7 @@ int dispatch_tj(const sf_phys_fields *x, double dt, int first, double w1, int tj
 // inside the kernel.
 // order 0: block b takes list entries b, b + G, b + 2G, ...
 // order 1: block b (after the XCD band remap) takes a contiguous chunk of the list
-template <typename T, int TI, int TJ, int NT, bool FIRST>
+template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS>
 __global__ __launch_bounds__(NT) void k_phys_ensemble(const PFA<T> *__restrict__ members,
                                                       const int2 *__restrict__ list, int ntiles, int order,
                                                       int rot, double w1)
@@ -1067,7 +1072,7 @@ __global__ __launch_bounds__(NT) void k_phys_ensemble(const PFA<T> *__restrict__
             for (int w = 0; w < (int)(sizeof(PFA<T>) / 4); w++) dst[w] = src[w];
         }
         P.w1 = w1;
-        phys_tile<T, TI, TJ, NT, FIRST>(P, ey % P.tiles_i, ey / P.tiles_i, sm, side);
+        phys_tile<T, TI, TJ, NT, FIRST, NTS>(P, ey % P.tiles_i, ey / P.tiles_i, sm, side);
         lds_barrier();   // the next tile overwrites the LDS planes
     }
 }
@@ -1075,7 +1080,7 @@ __global__ __launch_bounds__(NT) void k_phys_ensemble(const PFA<T> *__restrict__
 // One workgroup per list entry (grid = number of tiles): no tile loop, so the tile
 // code is register-allocated as in the single-basin kernel.  order 0 takes entry b,
 // order 1 the XCD-banded entry of block b.
-template <typename T, int TI, int TJ, int NT, bool FIRST>
+template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS>
 __global__ __launch_bounds__(NT) void k_phys_ensemble1(const PFA<T> *__restrict__ members,
                                                        const int2 *__restrict__ list, int order, int rot,
                                                        double w1)
@@ -1094,7 +1099,7 @@ __global__ __launch_bounds__(NT) void k_phys_ensemble1(const PFA<T> *__restrict_
         for (int w = 0; w < (int)(sizeof(PFA<T>) / 4); w++) dst[w] = src[w];
     }
     P.w1 = w1;
-    phys_tile<T, TI, TJ, NT, FIRST>(P, ey % P.tiles_i, ey / P.tiles_i, sm, side);
+    phys_tile<T, TI, TJ, NT, FIRST, NTS>(P, ey % P.tiles_i, ey / P.tiles_i, sm, side);
 }
 }  // namespace
 
@@ -1106,6 +1111,7 @@ struct sf_ens {
     int n, precision, device;
     int tile_j, order, grid;
     int ntiles, rot;
+    int nts;                    // streaming stores (see PFA::nts), set by ens_build
     void *members;              // device PFA<T>[3 n]: arguments of member k for rotation r at 3k + r
     int2 *list;                 // device (member, tile) list for tile_j
     std::vector<sf_phys_fields> host;
@@ -1133,6 +1139,11 @@ int ens_build(sf_ens *e)
 {
     std::vector<PFA<T>> hm(3 * e->n);
     std::vector<int2> list;
+    double bytes = 0.0;   // one substep of the whole ensemble (see PFA::nts)
+    for (int k = 0; k < e->n; k++)
+        bytes += (double)e->host[k].szI * e->host[k].szJ * (3.0 * sizeof(T) * 7 + 8.0 * 6);
+    const int nts = bytes > 384.0 * (1 << 20) ? 1 : 0;
+    e->nts = nts;
     for (int k = 0; k < e->n; k++) {
         const sf_phys_fields &x = e->host[k];
         int ti = 0, tj = 0, oi = 0, oj = 0, nt = 0;
@@ -1179,6 +1190,7 @@ int ens_build(sf_ens *e)
             a.rdrg = T(x.rdrg);
             a.dt = T(e->dt[k]);
             a.first = 0;
+            a.nts = nts;
             a.w1 = 0.0;
         }
         for (int t = 0; t < nt; t++) list.push_back(make_int2(k, t));
@@ -1203,20 +1215,32 @@ void ens_launch_tj(sf_ens *e, int first, double w1, int rot, hipStream_t s)
     const int G = e->grid > 0 ? (e->grid < e->ntiles ? e->grid : e->ntiles) : e->ntiles;
     const PFA<T> *m = (const PFA<T> *)e->members;
     if (G == e->ntiles) {   // one workgroup per tile
-        if (first)
-            hipLaunchKernelGGL((k_phys_ensemble1<T, TI, TJ, NT, true>), dim3(G), dim3(NT), 0, s, m, e->list,
+        if (first && e->nts)
+            hipLaunchKernelGGL((k_phys_ensemble1<T, TI, TJ, NT, true, true>), dim3(G), dim3(NT), 0, s, m, e->list,
+                               e->order, rot, w1);
+        else if (first)
+            hipLaunchKernelGGL((k_phys_ensemble1<T, TI, TJ, NT, true, false>), dim3(G), dim3(NT), 0, s, m, e->list,
+                               e->order, rot, w1);
+        else if (e->nts)
+            hipLaunchKernelGGL((k_phys_ensemble1<T, TI, TJ, NT, false, true>), dim3(G), dim3(NT), 0, s, m, e->list,
                                e->order, rot, w1);
         else
-            hipLaunchKernelGGL((k_phys_ensemble1<T, TI, TJ, NT, false>), dim3(G), dim3(NT), 0, s, m, e->list,
-                               e->order, rot, w1);
+            hipLaunchKernelGGL((k_phys_ensemble1<T, TI, TJ, NT, false, false>), dim3(G), dim3(NT), 0, s, m,
+                               e->list, e->order, rot, w1);
         return;
     }
-    if (first)
-        hipLaunchKernelGGL((k_phys_ensemble<T, TI, TJ, NT, true>), dim3(G), dim3(NT), 0, s, m, e->list, e->ntiles,
-                           e->order, rot, w1);
+    if (first && e->nts)
+        hipLaunchKernelGGL((k_phys_ensemble<T, TI, TJ, NT, true, true>), dim3(G), dim3(NT), 0, s, m, e->list,
+                           e->ntiles, e->order, rot, w1);
+    else if (first)
+        hipLaunchKernelGGL((k_phys_ensemble<T, TI, TJ, NT, true, false>), dim3(G), dim3(NT), 0, s, m, e->list,
+                           e->ntiles, e->order, rot, w1);
+    else if (e->nts)
+        hipLaunchKernelGGL((k_phys_ensemble<T, TI, TJ, NT, false, true>), dim3(G), dim3(NT), 0, s, m, e->list,
+                           e->ntiles, e->order, rot, w1);
     else
-        hipLaunchKernelGGL((k_phys_ensemble<T, TI, TJ, NT, false>), dim3(G), dim3(NT), 0, s, m, e->list, e->ntiles,
-                           e->order, rot, w1);
+        hipLaunchKernelGGL((k_phys_ensemble<T, TI, TJ, NT, false, false>), dim3(G), dim3(NT), 0, s, m, e->list,
+                           e->ntiles, e->order, rot, w1);
 }
 
 template <typename T>
