@@ -333,7 +333,9 @@ int sf_mlp_train_step(sf_mlp *mlp, const uint16_t *X, const float *Y, const int3
                       int32_t ntile, int32_t adam, float lr, float beta1, float beta2, float eps, int32_t step,
                       void *stream);
 int sf_mlp_forward(sf_mlp *mlp, const uint16_t *X, int32_t n, float *Y, void *stream);
-/* Raw features [S2][45] of the t1 state of a physics-mode field set (f: Coriolis). */
+/* Raw features [S2][45] of the t1 state of a physics-mode field set (f: Coriolis): per
+ * field (ubar, vbar, zeta, h, f) the 3x3 stencil at the point, the centre as its value and
+ * the 8 neighbours as differences neighbour - centre (state precision, rounded to FP32). */
 int sf_closure_features(const sf_phys_fields *fields, int32_t precision, const void *f, float *out, void *stream);
 /* sustr_eff = sustr + R_u * D_u, svstr_eff = svstr + R_v * D_v from the t1 state. */
 int sf_closure_stress(sf_mlp *mlp, const sf_phys_fields *fields, int32_t precision, const void *f,

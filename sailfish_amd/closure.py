@@ -6,10 +6,11 @@ in csrc/mlp.hip):
 1. fine runs of the basin1 recipe at 4x the coarse resolution (``fine_config``);
 2. ``coarse_grain``: 4x4 block means of zeta, face means of ubar/vbar and h, then
    the closed-basin boundary rules of the coarse grid;
-3. samples: features = 3x3 stencils of (ubar, vbar, zeta, h, f) of the coarse-grained
-   state at t (sf_closure_features); targets = tendency residual
-   (CG(fine)(t + dt_c) - coarse_step(CG(fine)(t))) / dt_c of ubar and vbar, with the
-   coarse step a forward/backward Euler substep of the plain coarse model;
+3. samples (``make_samples``): features = 3x3 stencils of (ubar, vbar, zeta, h, f) of
+   the coarse-grained state, as centre value and neighbour - centre differences
+   (sf_closure_features); targets = subgrid momentum tendency CG(T_fine(B)) -
+   T_coarse(CG(B)), each instantaneous tendency measured by a tiny forward/backward
+   Euler probe substep;
 4. training with Adam on normalised features/targets (sf_mlp_train_step);
 5. coarse runs with the closure: before every substep, sf_closure_stress turns the
    network output R into the extra momentum stress R * D added to the wind stress of
@@ -330,7 +331,8 @@ def make_samples(fine: Model, fine_probe: Model, probe: Model, nsnap: int, spaci
     momentum tendency, measured as one forward/backward Euler substep of a tiny length
     (the probe models' substep) divided by that length.  The fine trajectory is not
     differenced, so the fine scheme's time-stepping error stays out of the target.
-    Targets: ubar and vbar components (m/s^2); features: 3x3 stencils of CG(B)."""
+    Targets: ubar and vbar components (m/s^2); features: 3x3 stencils of CG(B) (centre
+    value and neighbour - centre differences)."""
     feats, targs = [], []
     mask = sample_mask(probe.shape)
     dtp = probe.ct.dtfast
@@ -361,19 +363,26 @@ def linear_r2(F: torch.Tensor, Y: torch.Tensor, n: int = 200_000) -> list:
     k = min(n, F.shape[0])
     Fk = F[:k].double()
     sd = Fk.std(0)
-    keep = sd > 0
+    keep = sd > 1e-5 * Fk.abs().mean(0) + 1e-30   # constant features (up to FP32 rounding) dropped
     Fk = (Fk[:, keep] - Fk[:, keep].mean(0)) / sd[keep]
-    A = torch.cat([Fk, torch.ones((k, 1), dtype=torch.float64, device=F.device)], 1)
-    Yk = Y[:k].double()
-    sol = torch.linalg.lstsq(A.cpu(), Yk.cpu()).solution.to(F.device)
+    A = torch.cat([Fk, torch.ones((k, 1), dtype=torch.float64, device=F.device)], 1).cpu()
+    Yk = Y[:k].double().cpu()
+    # normal equations solved through the pseudo-inverse of A^T A (eigenvalues below
+    # 1e-12 of the largest dropped): an in-sample fit with an intercept, so R^2 >= 0
+    ev, V = torch.linalg.eigh(A.T @ A)
+    inv = torch.where(ev > 1e-12 * ev.max(), 1.0 / ev, torch.zeros_like(ev))
+    sol = V @ (inv[:, None] * (V.T @ (A.T @ Yk)))
     res = Yk - A @ sol
     return [float(1.0 - res[:, c].var() / Yk[:, c].var()) for c in range(Yk.shape[1])]
 
 
 def normalise(F: torch.Tensor, Y: torch.Tensor):
+    """Per-feature mean and spread; a feature constant up to FP32 rounding (spread below
+    1e-5 of its magnitude, e.g. the x differences of a beta-plane f) gets spread 1, so
+    its rounding noise is not amplified into an input."""
     mu = F.mean(0)
     sd = F.std(0)
-    sd = torch.where(sd > 0, sd, torch.ones_like(sd))
+    sd = torch.where(sd > 1e-5 * F.abs().mean(0) + 1e-30, sd, torch.ones_like(sd))
     ym = Y.mean(0)
     ys = Y.std(0)
     ys = torch.where(ys > 0, ys, torch.ones_like(ys))
@@ -397,18 +406,30 @@ def pack_inputs(F: torch.Tensor, mu, sd) -> torch.Tensor:
 
 
 def train(mlp: ClosureMLP, Xb: torch.Tensor, Yn: torch.Tensor, epochs: int, batch: int = 4096, lr: float = 1e-3,
-          seed: int = 6):
-    """Adam over shuffled mini-batches (seeded); returns the training-set MSE before
-    training and after every epoch."""
+          seed: int = 6, lr_final: float | None = None, noise: float = 0.0):
+    """Adam over shuffled mini-batches (seeded); the learning rate follows a cosine from
+    ``lr`` to ``lr_final`` over the epochs (constant when lr_final is None).  With
+    ``noise`` > 0 every epoch trains on the normalised inputs plus Gaussian noise of
+    that standard deviation (input-noise regularisation, which keeps the network's
+    response smooth for states off the training trajectories).  Returns the
+    training-set MSE (clean inputs) before training and after every epoch."""
     n = Xb.shape[0]
     nb = n // batch
     g = torch.Generator(device="cpu").manual_seed(seed)
+    gn = torch.Generator(device=Xb.device).manual_seed(seed + 1)
     losses = [dataset_mse(mlp, Xb, Yn)]
     for ep in range(epochs):
+        lr_ep = lr if lr_final is None else lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * ep / epochs))
+        Xe = Xb
+        if noise > 0.0:
+            Xn = Xb.view(torch.bfloat16).float()
+            Xn[:, :NIN] += noise * torch.randn((n, NIN), generator=gn, device=Xb.device)
+            Xe = to_bf16_bits(Xn)
+            del Xn
         perm = torch.randperm(n, generator=g).to(torch.int32).cuda()
         for b in range(nb):
             idx = perm[b * batch:(b + 1) * batch]
-            mlp.train_step(Xb, Yn, idx, adam=True, lr=lr)
+            mlp.train_step(Xe, Yn, idx, adam=True, lr=lr_ep)
         losses.append(dataset_mse(mlp, Xb, Yn))
     return losses
 
@@ -450,8 +471,9 @@ def fine_coarse_ics(fine: Model, r: int):
 
 def run_experiment(coarse_shape=(512, 256), ratio: int = 4, nsteps: int = 5000, n_train_runs: int = 4,
                    snaps_per_run: int = 4, snap_spacing: int = 250, nsamples: int = 2_000_000, probe_dt: float = 0.01,
-                   epochs: int = 10, batch: int = 4096, lr: float = 1e-3, eval_every: int = 500, seed: int = 6,
-                   n_val: int = 500,
+                   epochs: int = 40, batch: int = 4096, lr: float = 2e-3, eval_every: int = 500, seed: int = 6,
+                   n_val: int | None = None, gains=(1.0, 0.5, 0.25, 0.1), lr_final: float | None = 1e-5,
+                   noise: float = 1.0,
                    dtype=torch.float64, log=None) -> dict:
     """Train the closure on fine runs with seeded initial bumps and winds, then run the
     basin1 configuration (centred bump, wind 0.1 N/m^2) fine, coarse plain and coarse
@@ -490,7 +512,7 @@ def run_experiment(coarse_shape=(512, 256), ratio: int = 4, nsteps: int = 5000, 
     mlp = ClosureMLP(max_blocks=batch // 64, seed=seed)
     mlp.set_norm(mu.cpu().numpy(), sd.cpu().numpy(), [float(ym[0]), float(ys[0]), float(ym[1]), float(ys[1])])
     t0 = _time.perf_counter()
-    losses = train(mlp, Xb, Yn, epochs, batch, lr, seed)
+    losses = train(mlp, Xb, Yn, epochs, batch, lr, seed, lr_final, noise)
     torch.cuda.synchronize()
     t_train = _time.perf_counter() - t0
     log(f"training: {epochs} epochs, loss {losses[0]:.4f} -> {losses[-1]:.4f} ({t_train:.1f} s)")
@@ -502,33 +524,67 @@ def run_experiment(coarse_shape=(512, 256), ratio: int = 4, nsteps: int = 5000, 
     vcenter = (float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.3, 0.7)))
     vamp, vbump = float(rng.uniform(0.05, 0.15)), float(rng.uniform(0.05, 0.15))
     vcfg = fine_config(*coarse_shape, ratio, amp=vamp, bump=vbump, center=vcenter)
-    val = compare_runs(vcfg, mlp, ratio, n_val, n_val, dtype)[-1]
-    score = {k: sum(val[k][c] / max(val["signal"][c], 1e-30) for c in range(3)) for k in ("plain", "closure")}
-    selected = "closure" if score["closure"] < score["plain"] else "plain"
-    log(f"validation ({n_val} steps): plain {score['plain']:.3e}, closure {score['closure']:.3e} -> {selected}")
+    n_val = nsteps if n_val is None else n_val
+    # the closure's output is also scaled by a gain chosen on the validation run (a
+    # network that fits the tendency a priori can still drift when coupled)
+    base = [float(ym[0]), float(ys[0]), float(ym[1]), float(ys[1])]
+    params = mlp.get_params()
+    variants = {}
+    for gain in gains:
+        variants[gain] = ClosureMLP(max_blocks=1024, params=params)
+        variants[gain].set_norm(mu.cpu().numpy(), sd.cpu().numpy(), [gain * x for x in base])
+    vseries = compare_runs(vcfg, None, ratio, n_val, max(n_val // 10, 1), dtype, variants=variants)
+    gain_scores = {g: validation_score(vseries, g) for g in gains}
+    plain_score = validation_score(vseries, "plain")
+    for g in gains:
+        log(f"validation gain {g}: plain {plain_score:.3e}, closure {gain_scores[g]:.3e}")
+    for v in variants.values():
+        v.close()
+    # a run that diverged (non-finite score) ranks last
+    best = min(gain_scores, key=lambda g: gain_scores[g] if math.isfinite(gain_scores[g]) else math.inf)
+    score = dict(plain=plain_score, closure=gain_scores[best])
+    selected = "closure" if math.isfinite(score["closure"]) and score["closure"] < score["plain"] else "plain"
+    mlp.set_norm(mu.cpu().numpy(), sd.cpu().numpy(), [best * x for x in base])
+    log(f"validation ({n_val} steps, mean over time): plain {score['plain']:.3e}, closure (gain {best}) {score['closure']:.3e}"
+        f" -> {selected}")
 
     series = compare_runs(fine_config(*coarse_shape, ratio), mlp, ratio, nsteps, eval_every, dtype, log)
     fcfg = fine_config(*coarse_shape, ratio)
     ccfg = coarse_config(fcfg)
     return dict(losses=losses, train_s=t_train, series=series, samples=int(F.shape[0]), mlp=mlp, linear_r2=r2,
                 target_rms=[float(Y[:, 0].pow(2).mean().sqrt()), float(Y[:, 1].pow(2).mean().sqrt())],
-                validation=dict(score=score, selected=selected, steps=n_val),
+                validation=dict(score=score, selected=selected, steps=n_val, gain=best,
+                                gain_scores={str(k): v for k, v in gain_scores.items()}),
                 fine_shape=[fcfg["szI"], fcfg["szJ"]], coarse_shape=[ccfg["szI"], ccfg["szJ"]],
                 dt_fine=fcfg["dtfast"], dt_coarse=ccfg["dtfast"], total_s=_time.perf_counter() - t_start)
 
 
-def compare_runs(fcfg: dict, mlp: ClosureMLP, ratio: int, nsteps: int, every: int, dtype, log=None) -> list:
+def validation_score(series: list, key="plain") -> float:
+    """Mean over the checkpoints of a compare_runs series of the RMSE of zeta, ubar and
+    vbar, each divided by that field's signal maximum, summed over the three fields.
+    ``key`` is "plain", "closure" or the name of one of the variants."""
+    def err(r):
+        return r[key] if key in ("plain", "closure") else r["variants"][key]
+    return sum(err(r)[c] / max(r["signal"][c], 1e-30) for r in series for c in range(3)) / len(series)
+
+
+def compare_runs(fcfg: dict, mlp: ClosureMLP | None, ratio: int, nsteps: int, every: int, dtype, log=None,
+                 variants: dict | None = None) -> list:
     """Fine run vs coarse plain vs coarse with closure from CG(fine initial state);
-    RMSE of zeta, ubar, vbar against CG(fine) every ``every`` coarse steps."""
+    RMSE of zeta, ubar, vbar against CG(fine) every ``every`` coarse steps.
+    ``variants`` (name -> network) adds one more coarse run per network beside them,
+    reported under row["variants"][name]."""
     log = log or (lambda *a: None)
     ccfg = coarse_config(fcfg)
     fine = Model(fcfg, dtype)
     plain = Model(ccfg, dtype)
-    clos = Model(ccfg, dtype)
     zc, uc, vc = fine_coarse_ics(fine, ratio)
     plain.set_state(zc, uc, vc)
-    clos.set_state(zc, uc, vc)
-    su, sv = closure_buffers(clos)
+    runs = {}
+    for name, net in ([("closure", mlp)] if mlp is not None else []) + list((variants or {}).items()):
+        m = Model(ccfg, dtype)
+        m.set_state(zc, uc, vc)
+        runs[name] = (m, net, *closure_buffers(m))
     series = []
     done = 0
     while done < nsteps:
@@ -537,16 +593,23 @@ def compare_runs(fcfg: dict, mlp: ClosureMLP, ratio: int, nsteps: int, every: in
             fine.step()
         for _ in range(n):
             plain.step()
-            clos.step(mlp, su, sv)
+            for m, net, su, sv in runs.values():
+                m.step(net, su, sv)
         done += n
         zf, uf, vf = coarse_grain(*fine.state(), ratio)
         zp, up, vp = plain.state()
-        zq, uq, vq = clos.state()
         row = dict(step=done, plain=[rmse(zp, zf), rmse(up, uf), rmse(vp, vf)],
-                   closure=[rmse(zq, zf), rmse(uq, uf), rmse(vq, vf)],
                    signal=[float(zf.abs().max()), float(uf.abs().max()), float(vf.abs().max())])
+        errs = {}
+        for name, (m, *_rest) in runs.items():
+            zq, uq, vq = m.state()
+            errs[name] = [rmse(zq, zf), rmse(uq, uf), rmse(vq, vf)]
+        if mlp is not None:
+            row["closure"] = errs.pop("closure")
+        if variants:
+            row["variants"] = errs
         series.append(row)
-        log(f"step {done}: rmse plain {row['plain']} closure {row['closure']}")
+        log(f"step {done}: rmse plain {row['plain']} closure {row.get('closure')}")
     return series
 
 

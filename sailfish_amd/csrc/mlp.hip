@@ -1,6 +1,7 @@
 // mlp.hip -- the learned subgrid momentum closure on gfx950 matrix cores.
 //
-// Network: 45 inputs (3x3 stencils of ubar, vbar, zeta, h, f around a coarse point)
+// Network: 45 inputs (per field ubar, vbar, zeta, h, f: the value at a coarse point and the
+// 8 differences neighbour - point of its 3x3 stencil)
 // -> 64 GELU -> 64 GELU -> 2 outputs (ubar and vbar tendency residuals).  Padded
 // shapes: input 48, output 16.  Parameter vector (FP32 master copy, 8336 values):
 //   W1[48][64], b1[64], W2[64][64], b2[64], W3[64][16], b3[16]
@@ -354,7 +355,12 @@ __global__ __launch_bounds__(NT) void k_mlp_forward(const uint16_t *__restrict__
 }
 
 // ---- closure features and stress ------------------------------------------------------
-// feature index: field * 9 + (dj + 1) * 3 + (di + 1), fields ubar, vbar, zeta, h, f
+// feature index: field * 9 + (dj + 1) * 3 + (di + 1), fields ubar, vbar, zeta, h, f.  The
+// centre entry (dj = di = 0) is the field value, the others the difference neighbour -
+// centre, formed in the state precision and then rounded to FP32: after the per-feature
+// normalisation the stencil differences (the derivatives a subgrid closure needs) keep
+// their resolution through the BF16 inputs, which the raw neighbour values (equal to
+// the centre value in their leading digits) would lose.
 template <typename T>
 struct ClosureArgs {
     const T *u, *v, *z, *h, *f, *sustr, *svstr;
@@ -368,9 +374,9 @@ template <typename T>
 __device__ __forceinline__ float feature(const ClosureArgs<T> &a, int i, int j, int k)
 {
     const int fld = k / 9, dj = (k % 9) / 3 - 1, di = k % 3 - 1;
-    const long long g = (long long)(j + dj) * a.szI + (i + di);
+    const long long gc = (long long)j * a.szI + i, g = gc + (long long)dj * a.szI + di;
     const T *p = fld == 0 ? a.u : fld == 1 ? a.v : fld == 2 ? a.z : fld == 3 ? a.h : a.f;
-    return (float)p[g];
+    return (dj == 0 && di == 0) ? (float)p[gc] : (float)(p[g] - p[gc]);
 }
 
 // raw features of every point with 1 <= i <= L-1, 1 <= j <= M-1 (others zero)
@@ -442,7 +448,9 @@ __global__ __launch_bounds__(NT) void k_closure_stress(const uint64_t *__restric
 #pragma unroll
         for (int k = 0; k < NIN; k++) {
             const int f = k / 9, dj = (k % 9) / 3 - 1, di = k % 3 - 1;
-            xv[k] = ((float)fld[f][gs + dj * szI + di] - norm[k]) * norm[NIN + k];
+            const T c = fld[f][gs];
+            const float x = (dj == 0 && di == 0) ? (float)c : (float)(fld[f][gs + dj * szI + di] - c);
+            xv[k] = (x - norm[k]) * norm[NIN + k];
         }
         xv[45] = xv[46] = xv[47] = 0.0f;
         uint64_t *row = (uint64_t *)(X + lane * XS);

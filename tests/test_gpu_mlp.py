@@ -154,7 +154,8 @@ def test_closure_features_match_numpy():
             for a in flds:
                 for dj in (-1, 0, 1):
                     for di in (-1, 0, 1):
-                        ref[j * I + i, k] = np.float32(a[j + dj, i + di])
+                        c = a[j, i]
+                        ref[j * I + i, k] = np.float32(c if dj == di == 0 else a[j + dj, i + di] - c)
                         k += 1
     assert np.array_equal(F, ref)
 
