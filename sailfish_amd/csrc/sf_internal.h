@@ -46,6 +46,19 @@ __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x
 
 constexpr int kBlock1D = 256;
 
+// compute units of a device, queried once per device (256 on MI355X)
+inline int cu_count(int device)
+{
+    static int cache[64];
+    if (device < 0 || device >= 64) return 256;
+    if (cache[device] == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || v <= 0) v = 256;
+        cache[device] = v;
+    }
+    return cache[device];
+}
+
 inline unsigned grid1d(long long n, int block = kBlock1D)
 {
     return (unsigned)((n + block - 1) / block);

@@ -322,26 +322,39 @@ __global__ __launch_bounds__(256) void k_verticalAdvection(G3 g, A U, A V, A Wa,
 // per column after the level loop.  u, v, Huon, Hvom of each level are staged in LDS
 // over the tile and its halo (as in k_horizontalAdvection_tiled); W and the vertical
 // stencils are read from global memory, u/v along the column through a sliding window.
+//
+// The levels are split into nchunk chunks of kchunk levels (the last one takes the
+// remainder and has at least 3 levels), one workgroup per tile and chunk.  A chunk seeds
+// the vertical-flux window at its first interior level with the same expression the
+// sliding window evaluates there, so the split changes no result.  The first chunk
+// (levels 0, 1) does the bottom special levels, the last chunk (levels N-3 .. N) the
+// top ones; with more than one chunk (N >= 6) the two touch disjoint levels.
+//
+// The tile is double-buffered in LDS (one barrier per level), and the kernel is held to
+// 128 VGPRs (4 waves per SIMD: two 512-thread workgroups per CU) -- 138 VGPRs and 3
+// waves per SIMD otherwise, 25 % slower on 1024 x 512 x 32.
 template <int TJ>
-__global__ __launch_bounds__(HTI * TJ) void k_rhs3d_fused(G3 g, H h, A fomn, A Hz, A Wa, const int32_t *BC,
-                                                          double *ru, double *rv, long long tail_end, int cor,
-                                                          int adv, int tiles_i)
+__global__ __launch_bounds__(HTI * TJ) __attribute__((amdgpu_waves_per_eu(4)))
+void k_rhs3d_fused(G3 g, H h, A fomn, A Hz, A Wa, const int32_t *BC, double *ru, double *rv, long long tail_end,
+                   int cor, int adv, int tiles_i, int ntiles, int kchunk, int nchunk)
 {
     constexpr int NT = HTI * TJ, LN = HLW * (TJ + 5);
-    __shared__ double sl[4][LN];
-    const int ti = blockIdx.x % tiles_i, tj = blockIdx.x / tiles_i;
+    __shared__ double sl[2][4][LN];
+    const int tile = blockIdx.x % ntiles, kc = blockIdx.x / ntiles;
+    const int ti = tile % tiles_i, tj = tile / tiles_i;
     const int i0 = ti * HTI, j0 = tj * TJ;
     const int c = threadIdx.x % HTI, r = threadIdx.x / HTI;
     const int i = i0 + c, j = j0 + r;
     const bool act = i < g.szI && j < g.szJ;
     const long long W = g.szI, S2 = g.S2, N = g.N, n = h.u.n;
     const long long f = act ? (long long)j * W + i : 0;
-    const HL hl{AL{sl[0]}, AL{sl[1]}, AL{sl[2]}, AL{sl[3]}, HLW};
     const long long lb = (long long)(r + 3) * HLW + (c + 2);
     const long long row0 = (long long)(j0 - 3) * W + (i0 - 2);
     const bool rnode = act && cor && isRNode(g, f);
     const bool vadv = act && adv && BC[f] < 0 && isInnerCell(g, f);
     const bool vint = vadv && N - 2 > 2;   // interior levels 2 .. N-3 handled in the level loop
+    const long long k0 = (long long)kc * kchunk, k1 = kc == nchunk - 1 ? N : k0 + kchunk;
+    const long long ks = k0 > 2 ? k0 : 2;   // first interior level of the chunk
     double fo_c = 0.0, fo_w = 0.0, fo_s = 0.0;
     if (rnode) {
         fo_c = fomn.p[f];
@@ -350,33 +363,36 @@ __global__ __launch_bounds__(HTI * TJ) void k_rhs3d_fused(G3 g, H h, A fomn, A H
     }
     // vertical-flux window of the column (v4 at level K reads levels K-1 .. K+2)
     double ub = 0.0, uc = 0.0, ud = 0.0, vb = 0.0, vc = 0.0, vd = 0.0, fu = 0.0, fv = 0.0;
-    if (vint) {
+    if (vint && ks < k1 && ks < N - 2) {
         const double *Up = h.u.p + f, *Vp = h.v.p + f;
-        const double ua = Up[S2], va = Vp[S2];
-        ub = Up[2 * S2], uc = Up[3 * S2], ud = Up[4 * S2];
-        vb = Vp[2 * S2], vc = Vp[3 * S2], vd = Vp[4 * S2];
-        fu = ((9.0 / 16.0) * (ub + uc) - (1.0 / 16.0) * (ua + ud)) * h4<false>(Wa, S2, 2, f, 1);
-        fv = ((9.0 / 16.0) * (vb + vc) - (1.0 / 16.0) * (va + vd)) * h4<false>(Wa, S2, 2, f, W);
+        const double ua = Up[(ks - 1) * S2], va = Vp[(ks - 1) * S2];
+        ub = Up[ks * S2], uc = Up[(ks + 1) * S2], ud = Up[(ks + 2) * S2];
+        vb = Vp[ks * S2], vc = Vp[(ks + 1) * S2], vd = Vp[(ks + 2) * S2];
+        fu = ((9.0 / 16.0) * (ub + uc) - (1.0 / 16.0) * (ua + ud)) * h4<false>(Wa, S2, ks, f, 1);
+        fv = ((9.0 / 16.0) * (vb + vc) - (1.0 / 16.0) * (va + vd)) * h4<false>(Wa, S2, ks, f, W);
     }
-    for (long long K = 0; K < N; K++) {
-        const long long kb = K * S2;
+    for (long long K = k0; K < k1; K++) {
+        const long long kb = K * S2, b = kb + f;
+        // level K writes one tile buffer while stragglers of level K-1 may still read
+        // the other; the barrier of level K orders the reuse two levels on
+        double(*sb)[LN] = sl[(K - k0) & 1];
+        const HL hl{AL{sb[0]}, AL{sb[1]}, AL{sb[2]}, AL{sb[3]}, HLW};
         for (int e = threadIdx.x; e < LN; e += NT) {
             const int rr = e / HLW, cc = e - rr * HLW;
             const long long gi = kb + row0 + (long long)rr * W + cc;
             const bool ok = gi >= 0 && gi < n;
             const long long gc = ok ? gi : 0;
             const double a0 = h.u.p[gc], a1 = h.v.p[gc], a2 = h.Hu.p[gc], a3 = h.Hv.p[gc];
-            sl[0][e] = ok ? a0 : 0.0;
-            sl[1][e] = ok ? a1 : 0.0;
-            sl[2][e] = ok ? a2 : 0.0;
-            sl[3][e] = ok ? a3 : 0.0;
+            sb[0][e] = ok ? a0 : 0.0;
+            sb[1][e] = ok ? a1 : 0.0;
+            sb[2][e] = ok ? a2 : 0.0;
+            sb[3][e] = ok ? a3 : 0.0;
         }
         __syncthreads();
         if (act) {
-            const long long b = kb + f;
             double xu = 0.0, xv = 0.0;
             if (rnode) {
-                const double *su = sl[0], *sv = sl[1];
+                const double *su = sb[0], *sv = sb[1];
                 const double hzc = Hz.p[b], hzw = Hz.p[b - 1], hzs = Hz.p[b - W];
                 const double UF0 = (hzc * fo_c) * ((sv[lb] + sv[lb + HLW]) * 0.5);
                 const double UF1 = (hzw * fo_w) * ((sv[lb - 1] + sv[lb - 1 + HLW]) * 0.5);
@@ -406,25 +422,32 @@ __global__ __launch_bounds__(HTI * TJ) void k_rhs3d_fused(G3 g, H h, A fomn, A H
             ru[b] = xu;
             rv[b] = xv;
         }
-        __syncthreads();   // the next level overwrites the tile
     }
     if (!act) return;
-    ru[N * S2 + f] = 0.0;
-    rv[N * S2 + f] = 0.0;
+    const bool bottom = kc == 0, top = kc == nchunk - 1;
+    if (top) {
+        ru[N * S2 + f] = 0.0;
+        rv[N * S2 + f] = 0.0;
+    }
     if (!vadv) return;
     // special levels and copies of verticalAdvection (same order as k_verticalAdvection)
     const A U = h.u, V = h.v;
 #define FU(k) (v4(U, S2, (k), f) * h4(Wa, S2, (k), f, 1))
 #define FV(k) (v4(V, S2, (k), f) * h4(Wa, S2, (k), f, W))
-    rv[1 * S2 + f] -= (9.0 / 16.0) * (FV(1) + FV(2)) - (1.0 / 16.0) * (FV(1) + FV(3));
-    const double u0 = r3(U, f), u1 = r3(U, S2 + f), u2 = r3(U, 2 * S2 + f);
-    const double uuu = ((9.0 / 16.0) * (u0 + u1) - (1.0 / 16.0) * (0.0 + u2)) * h4(Wa, S2, 1, f, 1);
-    const double uuu2 = ((9.0 / 16.0) * (0.0 + u0) - (1.0 / 16.0) * (0.0 + u1)) * h4(Wa, S2, 0, f, 1);
-    ru[1 * S2 + f] -= (FU(2) - uuu);
-    ru[f] -= (uuu - uuu2);
-    ru[(N - 2) * S2 + f] -= (9.0 / 16.0) * (FU(N - 1) + FU(N - 2)) - (1.0 / 16.0) * (FU(N - 1) + FU(N - 3));
-    rv[(N - 2) * S2 + f] -= (9.0 / 16.0) * (FV(N - 1) + FV(N - 2)) - (1.0 / 16.0) * (FV(N - 1) + FV(N - 3));
-    rv[f] = rv[S2 + f];
+    if (bottom) {
+        rv[1 * S2 + f] -= (9.0 / 16.0) * (FV(1) + FV(2)) - (1.0 / 16.0) * (FV(1) + FV(3));
+        const double u0 = r3(U, f), u1 = r3(U, S2 + f), u2 = r3(U, 2 * S2 + f);
+        const double uuu = ((9.0 / 16.0) * (u0 + u1) - (1.0 / 16.0) * (0.0 + u2)) * h4(Wa, S2, 1, f, 1);
+        const double uuu2 = ((9.0 / 16.0) * (0.0 + u0) - (1.0 / 16.0) * (0.0 + u1)) * h4(Wa, S2, 0, f, 1);
+        ru[1 * S2 + f] -= (FU(2) - uuu);
+        ru[f] -= (uuu - uuu2);
+    }
+    if (top) {
+        ru[(N - 2) * S2 + f] -= (9.0 / 16.0) * (FU(N - 1) + FU(N - 2)) - (1.0 / 16.0) * (FU(N - 1) + FU(N - 3));
+        rv[(N - 2) * S2 + f] -= (9.0 / 16.0) * (FV(N - 1) + FV(N - 2)) - (1.0 / 16.0) * (FV(N - 1) + FV(N - 3));
+    }
+    if (bottom) rv[f] = rv[S2 + f];
+    if (!top) return;
     ru[(N - 2) * S2 + f] = ru[(N - 3) * S2 + f];
     rv[(N - 2) * S2 + f] = rv[(N - 3) * S2 + f];
     ru[(N - 1) * S2 + f] = ru[(N - 2) * S2 + f];
@@ -695,11 +718,19 @@ extern "C" int sf_rhs3d(sf_ctx *c, const double *fomn, const double *u, const do
     const long long n3 = (c->szK + 1LL) * c->S2;
     H h{A{u, n3}, A{v, n3}, A{Huon, n3}, A{Hvom, n3}, c->szI};
     const long long end = launch_threads < n3 ? (long long)launch_threads : n3;
-    constexpr int TJ = 4;   // 256-thread workgroups: three per CU at this kernel's register count
-    const int tiles_i = (c->szI + HTI - 1) / HTI, tiles_j = (c->szJ + TJ - 1) / TJ;
-    hipLaunchKernelGGL(k_rhs3d_fused<TJ>, dim3((unsigned)(tiles_i * tiles_j)), dim3(HTI * TJ), 0,
+    constexpr int TJ = 8;   // 512-thread workgroups, 64 x 8 columns
+    const int tiles_i = (c->szI + HTI - 1) / HTI, tiles_j = (c->szJ + TJ - 1) / TJ, ntiles = tiles_i * tiles_j;
+    // level chunks: the fewest that give every resident workgroup slot (two per CU) a
+    // workgroup, with at least 3 levels per chunk (the bottom and top special levels
+    // each stay within one chunk); the last chunk takes the remainder.  1024 x 512 x 32
+    // runs unsplit, 512 x 256 x 32 in two chunks (101 us; 120 us in four).
+    const long long slots = 2LL * sf::cu_count(c->device);
+    int nchunk = 1;
+    while ((long long)ntiles * nchunk < slots && c->szK / (nchunk + 1) >= 3) nchunk++;
+    const int kchunk = c->szK / nchunk;
+    hipLaunchKernelGGL(k_rhs3d_fused<TJ>, dim3((unsigned)ntiles * (unsigned)nchunk), dim3(HTI * TJ), 0,
                        sf::as_stream(stream), grid3(c), h, A{fomn, c->S2}, A{Hz, (long long)c->szK * c->S2},
-                       A{W, n3}, BC, ru, rv, end, cor ? 1 : 0, adv ? 1 : 0, tiles_i);
+                       A{W, n3}, BC, ru, rv, end, cor ? 1 : 0, adv ? 1 : 0, tiles_i, ntiles, kchunk, nchunk);
     return sf::check_launch("sf_rhs3d");
 }
 

@@ -37,7 +37,7 @@ def _oracle3(cfg, GRID, ct, N, fused_fix_q5=True):
 
 
 @pytest.mark.parametrize("szI,szJ,N,fused", [(24, 14, 6, False), (40, 20, 8, True), (33, 17, 5, False),
-                                            (130, 21, 4, False)])
+                                            (130, 21, 4, False), (36, 18, 17, True)])
 def test_main3d_bitwise(szI, szJ, N, fused):
     import torch
     from oracle import oracle as O
@@ -84,11 +84,13 @@ def test_main3d_bitwise(szI, szJ, N, fused):
 
 @pytest.mark.parametrize("szI,szJ,N,cor,adv", [(24, 14, 6, True, True), (130, 75, 9, True, True),
                                               (33, 17, 4, False, True), (70, 19, 5, True, False),
-                                              (20, 10, 7, True, True)])
+                                              (20, 10, 7, True, True), (70, 23, 16, True, True),
+                                              (66, 19, 27, True, True), (20, 10, 17, True, True)])
 def test_rhs3d_fused_equals_four_launches(szI, szJ, N, cor, adv):
     """sf_rhs3d against zero + addCoriolis + horizontalAdvection (with its tail zeros) +
     verticalAdvection, bit for bit, on random smooth 3D fields (the 20x10 grid has
-    S2 < 512, so the tail zeros reach past level 1)."""
+    S2 < 512, so the tail zeros reach past level 1; 16 and more levels are split into
+    level chunks, one workgroup each)."""
     import torch
     from sailfish_amd import mod_operators as ops
     from sailfish_amd import rhs as R
