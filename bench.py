@@ -515,6 +515,30 @@ def run_3d_gpu(args):
     step = ("rhs3d_fused", "set_maxflux", "omega")      # the product path of one baroclinic step
     tot = sum(out[k]["ms"] for k in step)
     cells_int = S2 * N
+    # FP32 bulk form of the same three kernels on the same state rounded to FP32
+    u32, v32, Hz32, zw32 = (x.float() for x in (OCEAN.u_t2, OCEAN.v_t2, Hz, z_w))
+    Hu32, Hv32, W32 = (torch.zeros(cells, dtype=torch.float32, device=u32.device) for _ in range(3))
+    ru32, rv32 = torch.zeros_like(u32), torch.zeros_like(u32)
+    kern32 = {
+        "set_maxflux": lambda: ops.set_maxflux(None, None, (u32, v32, Hu32, Hv32, Hz32), ctx=c),
+        "omega": lambda: ops.omega(None, None, (W32, u32, v32, Hu32, Hv32, zw32, BC), ctx=c),
+        "rhs3d_fused": lambda: ops.rhs3d_fused((GRID.fomn.reshape(-1), u32, v32, Hu32, Hv32, W32, Hz32, ru32, rv32,
+                                                BC), launch_threads=(S2 // 512 + 1) * 512, ctx=c),
+    }
+    for f in kern32.values():
+        f()
+    torch.cuda.synchronize()
+    out32 = {}
+    for name, f in kern32.items():
+        a, b = _events(torch, 1)[0]
+        a.record()
+        for _ in range(reps):
+            f()
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        out32[name] = {"ms": ms, "GB_s": fields[name] * 4 * cells / (ms * 1e-3) / 1e9}
+    tot32 = sum(out32[k]["ms"] for k in step)
     res = {
         "workload": f"{szI}x{szJ}x{N} s-levels, basin1 recipe (dx 5 km), FP64, reference (faithful) 3D kernels",
         "kernels": out,
@@ -523,6 +547,9 @@ def run_3d_gpu(args):
         "hbm_frac_3d": sum(fields[k] * 8 * cells for k in step) / (tot * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "note": "rhs3d_fused (one launch) is the product path; addCoriolis / horizontalAdvection / "
                 "verticalAdvection are the one-to-one reference kernels, timed for comparison",
+        "fp32": {"kernels": out32, "ms_rhs3d_maxflux_omega": tot32, "cell_steps_per_s_3d": cells_int / (tot32 * 1e-3),
+                 "hbm_frac_3d": sum(fields[k] * 4 * cells for k in step) / (tot32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                 "note": "FP32 bulk fields (2D metrics FP64, omega column sums FP64); bytes at 4 B per field"},
     }
     if not args.no_cpu:
         host = {k: x.detach().cpu().numpy().astype(np.float64).ravel() for k, x in

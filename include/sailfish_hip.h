@@ -152,6 +152,20 @@ int sf_set_maxflux(sf_ctx *ctx, const double *u, const double *v, double *Huon, 
 int sf_omega(sf_ctx *ctx, double *W, const double *Huon, const double *Hvom, const double *z_w,
              const int32_t *BC, void *stream);
 
+/* FP32 bulk forms of the three kernels above (the same call sites:
+ * nonlinear/rhs.py:38-71, nonlinear/main3d.py:141,321 and step3D_uv.kernels.cpp:62-165,
+ * 372-405): the 3D fields in FP32, the 2D grid metrics (fomn and the context's on_u,
+ * om_v) in FP64, stencil arithmetic in FP32.  sf_omega_f32 forms the column sums and the
+ * surface correction in FP64 and needs szK <= 64.  Error against the FP64 forms is
+ * stated with the tests that check it. */
+int sf_rhs3d_f32(sf_ctx *ctx, const double *fomn, const float *u, const float *v, const float *Huon,
+                 const float *Hvom, const float *W, const float *Hz, const int32_t *BC, float *ru, float *rv,
+                 int64_t launch_threads, int32_t cor, int32_t adv, void *stream);
+int sf_set_maxflux_f32(sf_ctx *ctx, const float *u, const float *v, float *Huon, float *Hvom, const float *Hz,
+                       void *stream);
+int sf_omega_f32(sf_ctx *ctx, float *W, const float *Huon, const float *Hvom, const float *z_w,
+                 const int32_t *BC, void *stream);
+
 /* set_depth (nonlinear/step3D_uv.kernels.cpp:225-291). */
 int sf_set_depth(sf_ctx *ctx, int32_t Vtransform, const double *Zt_avg1, double *z_w, double *z_r,
                  const double *h, double hc, double *Hz, const double *sc_r, const double *sc_w,

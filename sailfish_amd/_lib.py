@@ -59,6 +59,9 @@ _SIGS = {
     "sf_rhs3d": [P, P, P, P, P, P, P, P, P, P, P, I64, I32, I32, P],
     "sf_set_maxflux": [P, P, P, P, P, P, P],
     "sf_omega": [P, P, P, P, P, P, P],
+    "sf_rhs3d_f32": [P, P, P, P, P, P, P, P, P, P, P, I64, I32, I32, P],
+    "sf_set_maxflux_f32": [P, P, P, P, P, P, P],
+    "sf_omega_f32": [P, P, P, P, P, P, P],
     "sf_set_depth": [P, I32, P, P, P, P, F64, P, P, P, P, P, P],
     "sf_set_zeta": [P, P, P, P, P],
     "sf_set_lateral_uv_bcs": [P, F64, P, P, P, P, P],

@@ -14,21 +14,28 @@
 
 namespace {
 
-struct A {
-    const double *p;
+// a 3D (or 2D) field in global memory: pointer and element count; value type double
+// (reference precision) or float (FP32 bulk form of the product-path kernels)
+template <class T>
+struct AG {
+    const T *p;
     long long n;
+    using value_type = T;
 };
+using A = AG<double>;
+using AF = AG<float>;
 
-// out-of-range reads are 0.0; the load itself always uses a clamped (valid) index, so it
+// out-of-range reads are 0; the load itself always uses a clamped (valid) index, so it
 // is unconditional and identical reads of one expression tree merge into one load
-__device__ __forceinline__ double r3(A a, long long idx)
+template <class T>
+__device__ __forceinline__ T r3(AG<T> a, long long idx)
 {
     const long long c = idx < 0 ? 0 : (idx >= a.n ? a.n - 1 : idx);
-    const double x = a.p[c];
-    return (idx < 0 || idx >= a.n) ? 0.0 : x;
+    const T x = a.p[c];
+    return (idx < 0 || idx >= a.n) ? T(0) : x;
 }
-template <bool CHK>
-__device__ __forceinline__ double rd(A a, long long idx)
+template <bool CHK, class T>
+__device__ __forceinline__ T rd(AG<T> a, long long idx)
 {
     if constexpr (CHK) return r3(a, idx);
     return a.p[idx];
@@ -93,102 +100,112 @@ __global__ __launch_bounds__(256) void k_addCoriolis(G3 g, A fomn, A u, A v, dou
 }
 
 // ---- horizontalAdvection (rhs_kernels.cpp:70-132) ---------------------------------
-struct H {
-    A u, v, Hu, Hv;
+template <class T>
+struct HG {
+    AG<T> u, v, Hu, Hv;
     long long W;
+    using value_type = T;
 };
+using H = HG<double>;
 // the same four fields staged in LDS for one level of a tile: element at flat offset
 // idx of the tile view (row stride W = tile width + 4)
-struct AL {
-    const double *p;
+template <class T>
+struct ALT {
+    const T *p;
+    using value_type = T;
 };
-template <bool CHK>
-__device__ __forceinline__ double rd(AL a, long long idx)
+template <bool CHK, class T>
+__device__ __forceinline__ T rd(ALT<T> a, long long idx)
 {
     return a.p[idx];
 }
-struct HL {
-    AL u, v, Hu, Hv;
+template <class T>
+struct HLT {
+    ALT<T> u, v, Hu, Hv;
     long long W;
+    using value_type = T;
 };
+using AL = ALT<double>;
+using HL = HLT<double>;
 
 #define AT(a, o) rd<CHK>((a), base + (o))
-template <bool CHK, class AA>
-__device__ __forceinline__ double dxx(AA a, long long base, long long o)
+// V: the value type of the accessor; literals are converted to it (exact in both types)
+template <bool CHK, class AA, class V = typename AA::value_type>
+__device__ __forceinline__ V dxx(AA a, long long base, long long o)
 {
-    return AT(a, o - 1) - 2.0 * AT(a, o) + AT(a, o + 1);
+    return AT(a, o - 1) - V(2) * AT(a, o) + AT(a, o + 1);
 }
-template <bool CHK, class AA>
-__device__ __forceinline__ double dee(AA a, long long base, long long o, long long W)
+template <bool CHK, class AA, class V = typename AA::value_type>
+__device__ __forceinline__ V dee(AA a, long long base, long long o, long long W)
 {
-    return AT(a, o - W) - 2.0 * AT(a, o) + AT(a, o + W);
+    return AT(a, o - W) - V(2) * AT(a, o) + AT(a, o + W);
 }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_uR(const HH &h, long long base, long long o) { return (AT(h.u, o) + AT(h.u, o + 1)) * 0.5; }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_uP(const HH &h, long long base, long long o) { return (AT(h.u, o) + AT(h.u, o - 1)) * 0.5; }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_vR(const HH &h, long long base, long long o) { return (AT(h.v, o) + AT(h.v, o + h.W)) * 0.5; }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_vP(const HH &h, long long base, long long o) { return (AT(h.v, o) + AT(h.v, o - h.W)) * 0.5; }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_HuR(const HH &h, long long base, long long o) { return (AT(h.Hu, o) + AT(h.Hu, o + 1)) * 0.5; }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_HuP(const HH &h, long long base, long long o) { return (AT(h.Hu, o) + AT(h.Hu, o - 1)) * 0.5; }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_HvP(const HH &h, long long base, long long o) { return (AT(h.Hv, o) + AT(h.Hv, o - h.W)) * 0.5; }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_HvR(const HH &h, long long base, long long o) { return (AT(h.Hv, o) + AT(h.Hv, o + h.W)) * 0.5; }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_uxxR(const HH &h, long long base, long long o)
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_uR(const HH &h, long long base, long long o) { return (AT(h.u, o) + AT(h.u, o + 1)) * V(0.5); }
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_uP(const HH &h, long long base, long long o) { return (AT(h.u, o) + AT(h.u, o - 1)) * V(0.5); }
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_vR(const HH &h, long long base, long long o) { return (AT(h.v, o) + AT(h.v, o + h.W)) * V(0.5); }
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_vP(const HH &h, long long base, long long o) { return (AT(h.v, o) + AT(h.v, o - h.W)) * V(0.5); }
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_HuR(const HH &h, long long base, long long o) { return (AT(h.Hu, o) + AT(h.Hu, o + 1)) * V(0.5); }
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_HuP(const HH &h, long long base, long long o) { return (AT(h.Hu, o) + AT(h.Hu, o - 1)) * V(0.5); }
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_HvP(const HH &h, long long base, long long o) { return (AT(h.Hv, o) + AT(h.Hv, o - h.W)) * V(0.5); }
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_HvR(const HH &h, long long base, long long o) { return (AT(h.Hv, o) + AT(h.Hv, o + h.W)) * V(0.5); }
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_uxxR(const HH &h, long long base, long long o)
 {
     return (h_uR<CHK>(h, base, o) + h_uR<CHK>(h, base, o + 1) >= 0) ? dxx<CHK>(h.u, base, o) : dxx<CHK>(h.u, base, o + 1);
 }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_ueeP(const HH &h, long long base, long long o)
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_ueeP(const HH &h, long long base, long long o)
 {
     return (h_HvP<CHK>(h, base, o) + h_HvP<CHK>(h, base, o - h.W) >= 0) ? dee<CHK>(h.u, base, o, h.W) : dee<CHK>(h.u, base, o - h.W, h.W);
 }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_HuxxR(const HH &h, long long base, long long o)
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_HuxxR(const HH &h, long long base, long long o)
 {
-    return (dxx<CHK>(h.Hu, base, o) + dxx<CHK>(h.Hu, base, o + 1)) * 0.5;
+    return (dxx<CHK>(h.Hu, base, o) + dxx<CHK>(h.Hu, base, o + 1)) * V(0.5);
 }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_HueeP(const HH &h, long long base, long long o)
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_HueeP(const HH &h, long long base, long long o)
 {
-    return (dee<CHK>(h.Hu, base, o, h.W) + dee<CHK>(h.Hu, base, o - 1, h.W)) * 0.5;
+    return (dee<CHK>(h.Hu, base, o, h.W) + dee<CHK>(h.Hu, base, o - 1, h.W)) * V(0.5);
 }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_HvxxP(const HH &h, long long base, long long o)
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_HvxxP(const HH &h, long long base, long long o)
 {
-    return (dxx<CHK>(h.Hv, base, o) + dxx<CHK>(h.Hv, base, o - h.W)) * 0.5;
+    return (dxx<CHK>(h.Hv, base, o) + dxx<CHK>(h.Hv, base, o - h.W)) * V(0.5);
 }
-template <bool CHK, class HH>
-__device__ __forceinline__ double h_HveeR(const HH &h, long long base, long long o)
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V h_HveeR(const HH &h, long long base, long long o)
 {
-    return (dee<CHK>(h.Hv, base, o, h.W) + dee<CHK>(h.Hv, base, o + h.W, h.W)) * 0.5;
+    return (dee<CHK>(h.Hv, base, o, h.W) + dee<CHK>(h.Hv, base, o + h.W, h.W)) * V(0.5);
 }
 constexpr double Gadv = -0.5;
-template <bool CHK, class HH>
-__device__ __forceinline__ double UFxR(const HH &h, long long b, long long o)
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V UFxR(const HH &h, long long b, long long o)
 {
-    return h_uR<CHK>(h, b, o) + (Gadv * h_uxxR<CHK>(h, b, o)) * (h_HuR<CHK>(h, b, o) + Gadv * h_HuxxR<CHK>(h, b, o));
+    return h_uR<CHK>(h, b, o) + (V(Gadv) * h_uxxR<CHK>(h, b, o)) * (h_HuR<CHK>(h, b, o) + V(Gadv) * h_HuxxR<CHK>(h, b, o));
 }
-template <bool CHK, class HH>
-__device__ __forceinline__ double UFeP(const HH &h, long long b, long long o)
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V UFeP(const HH &h, long long b, long long o)
 {
-    return h_uP<CHK>(h, b, o) + (Gadv * h_ueeP<CHK>(h, b, o)) * (h_HvP<CHK>(h, b, o) + Gadv * h_HvxxP<CHK>(h, b, o));
+    return h_uP<CHK>(h, b, o) + (V(Gadv) * h_ueeP<CHK>(h, b, o)) * (h_HvP<CHK>(h, b, o) + V(Gadv) * h_HvxxP<CHK>(h, b, o));
 }
-template <bool CHK, class HH>
-__device__ __forceinline__ double VFxP(const HH &h, long long b, long long o)
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V VFxP(const HH &h, long long b, long long o)
 {
-    return h_vP<CHK>(h, b, o) + (Gadv * h_ueeP<CHK>(h, b, o)) * (h_HuP<CHK>(h, b, o) + Gadv * h_HueeP<CHK>(h, b, o));
+    return h_vP<CHK>(h, b, o) + (V(Gadv) * h_ueeP<CHK>(h, b, o)) * (h_HuP<CHK>(h, b, o) + V(Gadv) * h_HueeP<CHK>(h, b, o));
 }
-template <bool CHK, class HH>
-__device__ __forceinline__ double VFeR(const HH &h, long long b, long long o)
+template <bool CHK, class HH, class V = typename HH::value_type>
+__device__ __forceinline__ V VFeR(const HH &h, long long b, long long o)
 {
-    return h_vR<CHK>(h, b, o) + (Gadv * h_uxxR<CHK>(h, b, o)) * (h_HvR<CHK>(h, b, o) + Gadv * h_HveeR<CHK>(h, b, o));
+    return h_vR<CHK>(h, b, o) + (V(Gadv) * h_uxxR<CHK>(h, b, o)) * (h_HvR<CHK>(h, b, o) + V(Gadv) * h_HveeR<CHK>(h, b, o));
 }
 #undef AT
 
@@ -250,17 +267,17 @@ __global__ void k_hadv_tail(long long S2, long long end, double *ru, double *rv)
 }
 
 // ---- verticalAdvection (rhs_kernels.cpp:135-197) ----------------------------------
-template <bool CHK = true>
-__device__ __forceinline__ double v4(A a, long long S2, long long K, long long f)
+template <bool CHK = true, class T>
+__device__ __forceinline__ T v4(AG<T> a, long long S2, long long K, long long f)
 {
-    return (9.0 / 16.0) * (rd<CHK>(a, K * S2 + f) + rd<CHK>(a, (K + 1) * S2 + f)) -
-           (1.0 / 16.0) * (rd<CHK>(a, (K - 1) * S2 + f) + rd<CHK>(a, (K + 2) * S2 + f));
+    return T(9.0 / 16.0) * (rd<CHK>(a, K * S2 + f) + rd<CHK>(a, (K + 1) * S2 + f)) -
+           T(1.0 / 16.0) * (rd<CHK>(a, (K - 1) * S2 + f) + rd<CHK>(a, (K + 2) * S2 + f));
 }
-template <bool CHK = true>
-__device__ __forceinline__ double h4(A a, long long S2, long long K, long long f, long long d)
+template <bool CHK = true, class T>
+__device__ __forceinline__ T h4(AG<T> a, long long S2, long long K, long long f, long long d)
 {
     const long long b = K * S2 + f;
-    return (9.0 / 16.0) * (rd<CHK>(a, b) + rd<CHK>(a, b + d)) - (1.0 / 16.0) * (rd<CHK>(a, b - d) + rd<CHK>(a, b + 2 * d));
+    return T(9.0 / 16.0) * (rd<CHK>(a, b) + rd<CHK>(a, b + d)) - T(1.0 / 16.0) * (rd<CHK>(a, b - d) + rd<CHK>(a, b + 2 * d));
 }
 
 __global__ __launch_bounds__(256) void k_verticalAdvection(G3 g, A U, A V, A Wa, double *ru, double *rv,
@@ -333,13 +350,13 @@ __global__ __launch_bounds__(256) void k_verticalAdvection(G3 g, A U, A V, A Wa,
 // The tile is double-buffered in LDS (one barrier per level), and the kernel is held to
 // 128 VGPRs (4 waves per SIMD: two 512-thread workgroups per CU) -- 138 VGPRs and 3
 // waves per SIMD otherwise, 25 % slower on 1024 x 512 x 32.
-template <int TJ>
+template <int TJ, class T>
 __global__ __launch_bounds__(HTI * TJ) __attribute__((amdgpu_waves_per_eu(4)))
-void k_rhs3d_fused(G3 g, H h, A fomn, A Hz, A Wa, const int32_t *BC, double *ru, double *rv, long long tail_end,
+void k_rhs3d_fused(G3 g, HG<T> h, A fomn, AG<T> Hz, AG<T> Wa, const int32_t *BC, T *ru, T *rv, long long tail_end,
                    int cor, int adv, int tiles_i, int ntiles, int kchunk, int nchunk)
 {
     constexpr int NT = HTI * TJ, LN = HLW * (TJ + 5);
-    __shared__ double sl[2][4][LN];
+    __shared__ T sl[2][4][LN];
     const int tile = blockIdx.x % ntiles, kc = blockIdx.x / ntiles;
     const int ti = tile % tiles_i, tj = tile / tiles_i;
     const int i0 = ti * HTI, j0 = tj * TJ;
@@ -355,63 +372,63 @@ void k_rhs3d_fused(G3 g, H h, A fomn, A Hz, A Wa, const int32_t *BC, double *ru,
     const bool vint = vadv && N - 2 > 2;   // interior levels 2 .. N-3 handled in the level loop
     const long long k0 = (long long)kc * kchunk, k1 = kc == nchunk - 1 ? N : k0 + kchunk;
     const long long ks = k0 > 2 ? k0 : 2;   // first interior level of the chunk
-    double fo_c = 0.0, fo_w = 0.0, fo_s = 0.0;
+    T fo_c = T(0), fo_w = T(0), fo_s = T(0);
     if (rnode) {
-        fo_c = fomn.p[f];
-        fo_w = fomn.p[f - 1];
-        fo_s = fomn.p[f - W];
+        fo_c = T(fomn.p[f]);
+        fo_w = T(fomn.p[f - 1]);
+        fo_s = T(fomn.p[f - W]);
     }
     // vertical-flux window of the column (v4 at level K reads levels K-1 .. K+2)
-    double ub = 0.0, uc = 0.0, ud = 0.0, vb = 0.0, vc = 0.0, vd = 0.0, fu = 0.0, fv = 0.0;
+    T ub = T(0), uc = T(0), ud = T(0), vb = T(0), vc = T(0), vd = T(0), fu = T(0), fv = T(0);
     if (vint && ks < k1 && ks < N - 2) {
-        const double *Up = h.u.p + f, *Vp = h.v.p + f;
-        const double ua = Up[(ks - 1) * S2], va = Vp[(ks - 1) * S2];
+        const T *Up = h.u.p + f, *Vp = h.v.p + f;
+        const T ua = Up[(ks - 1) * S2], va = Vp[(ks - 1) * S2];
         ub = Up[ks * S2], uc = Up[(ks + 1) * S2], ud = Up[(ks + 2) * S2];
         vb = Vp[ks * S2], vc = Vp[(ks + 1) * S2], vd = Vp[(ks + 2) * S2];
-        fu = ((9.0 / 16.0) * (ub + uc) - (1.0 / 16.0) * (ua + ud)) * h4<false>(Wa, S2, ks, f, 1);
-        fv = ((9.0 / 16.0) * (vb + vc) - (1.0 / 16.0) * (va + vd)) * h4<false>(Wa, S2, ks, f, W);
+        fu = (T(9.0 / 16.0) * (ub + uc) - T(1.0 / 16.0) * (ua + ud)) * h4<false>(Wa, S2, ks, f, 1);
+        fv = (T(9.0 / 16.0) * (vb + vc) - T(1.0 / 16.0) * (va + vd)) * h4<false>(Wa, S2, ks, f, W);
     }
     for (long long K = k0; K < k1; K++) {
         const long long kb = K * S2, b = kb + f;
         // level K writes one tile buffer while stragglers of level K-1 may still read
         // the other; the barrier of level K orders the reuse two levels on
-        double(*sb)[LN] = sl[(K - k0) & 1];
-        const HL hl{AL{sb[0]}, AL{sb[1]}, AL{sb[2]}, AL{sb[3]}, HLW};
+        T(*sb)[LN] = sl[(K - k0) & 1];
+        const HLT<T> hl{ALT<T>{sb[0]}, ALT<T>{sb[1]}, ALT<T>{sb[2]}, ALT<T>{sb[3]}, HLW};
         for (int e = threadIdx.x; e < LN; e += NT) {
             const int rr = e / HLW, cc = e - rr * HLW;
             const long long gi = kb + row0 + (long long)rr * W + cc;
             const bool ok = gi >= 0 && gi < n;
             const long long gc = ok ? gi : 0;
-            const double a0 = h.u.p[gc], a1 = h.v.p[gc], a2 = h.Hu.p[gc], a3 = h.Hv.p[gc];
-            sb[0][e] = ok ? a0 : 0.0;
-            sb[1][e] = ok ? a1 : 0.0;
-            sb[2][e] = ok ? a2 : 0.0;
-            sb[3][e] = ok ? a3 : 0.0;
+            const T a0 = h.u.p[gc], a1 = h.v.p[gc], a2 = h.Hu.p[gc], a3 = h.Hv.p[gc];
+            sb[0][e] = ok ? a0 : T(0);
+            sb[1][e] = ok ? a1 : T(0);
+            sb[2][e] = ok ? a2 : T(0);
+            sb[3][e] = ok ? a3 : T(0);
         }
         __syncthreads();
         if (act) {
-            double xu = 0.0, xv = 0.0;
+            T xu = T(0), xv = T(0);
             if (rnode) {
-                const double *su = sb[0], *sv = sb[1];
-                const double hzc = Hz.p[b], hzw = Hz.p[b - 1], hzs = Hz.p[b - W];
-                const double UF0 = (hzc * fo_c) * ((sv[lb] + sv[lb + HLW]) * 0.5);
-                const double UF1 = (hzw * fo_w) * ((sv[lb - 1] + sv[lb - 1 + HLW]) * 0.5);
-                const double VF0 = (hzc * fo_c) * ((su[lb] + su[lb + 1]) * 0.5);
-                const double VF1 = (hzs * fo_s) * ((su[lb - HLW] + su[lb - HLW + 1]) * 0.5);
-                xu = xu + (UF0 + UF1) * 0.5;
-                xv = xv - (VF0 + VF1) * 0.5;
+                const T *su = sb[0], *sv = sb[1];
+                const T hzc = Hz.p[b], hzw = Hz.p[b - 1], hzs = Hz.p[b - W];
+                const T UF0 = (hzc * fo_c) * ((sv[lb] + sv[lb + HLW]) * T(0.5));
+                const T UF1 = (hzw * fo_w) * ((sv[lb - 1] + sv[lb - 1 + HLW]) * T(0.5));
+                const T VF0 = (hzc * fo_c) * ((su[lb] + su[lb + 1]) * T(0.5));
+                const T VF1 = (hzs * fo_s) * ((su[lb - HLW] + su[lb - HLW + 1]) * T(0.5));
+                xu = xu + (UF0 + UF1) * T(0.5);
+                xv = xv - (VF0 + VF1) * T(0.5);
             }
-            const double du = (UFxR<false>(hl, lb, 0) - UFxR<false>(hl, lb, -1)) +
+            const T du = (UFxR<false>(hl, lb, 0) - UFxR<false>(hl, lb, -1)) +
                               (UFeP<false>(hl, lb, 0) - UFeP<false>(hl, lb, -HLW));
-            const double dv = (VFxP<false>(hl, lb, 0) - VFxP<false>(hl, lb, -1)) +
+            const T dv = (VFxP<false>(hl, lb, 0) - VFxP<false>(hl, lb, -1)) +
                               (VFeR<false>(hl, lb, 0) - VFeR<false>(hl, lb, -HLW));
             xu = xu - du;
             xv = xv - dv;
-            if (K >= 1 && b < tail_end) xu = xv = 0.0;
+            if (K >= 1 && b < tail_end) xu = xv = T(0);
             if (vint && K >= 2 && K < N - 2) {
-                const double ue = h.u.p[(K + 3) * S2 + f], ve = h.v.p[(K + 3) * S2 + f];
-                const double fu1 = ((9.0 / 16.0) * (uc + ud) - (1.0 / 16.0) * (ub + ue)) * h4<false>(Wa, S2, K + 1, f, 1);
-                const double fv1 = ((9.0 / 16.0) * (vc + vd) - (1.0 / 16.0) * (vb + ve)) * h4<false>(Wa, S2, K + 1, f, W);
+                const T ue = h.u.p[(K + 3) * S2 + f], ve = h.v.p[(K + 3) * S2 + f];
+                const T fu1 = (T(9.0 / 16.0) * (uc + ud) - T(1.0 / 16.0) * (ub + ue)) * h4<false>(Wa, S2, K + 1, f, 1);
+                const T fv1 = (T(9.0 / 16.0) * (vc + vd) - T(1.0 / 16.0) * (vb + ve)) * h4<false>(Wa, S2, K + 1, f, W);
                 xu = xu - (fu1 - fu);
                 xv = xv - (fv1 - fv);
                 fu = fu1;
@@ -426,25 +443,25 @@ void k_rhs3d_fused(G3 g, H h, A fomn, A Hz, A Wa, const int32_t *BC, double *ru,
     if (!act) return;
     const bool bottom = kc == 0, top = kc == nchunk - 1;
     if (top) {
-        ru[N * S2 + f] = 0.0;
-        rv[N * S2 + f] = 0.0;
+        ru[N * S2 + f] = T(0);
+        rv[N * S2 + f] = T(0);
     }
     if (!vadv) return;
     // special levels and copies of verticalAdvection (same order as k_verticalAdvection)
-    const A U = h.u, V = h.v;
+    const AG<T> U = h.u, V = h.v;
 #define FU(k) (v4(U, S2, (k), f) * h4(Wa, S2, (k), f, 1))
 #define FV(k) (v4(V, S2, (k), f) * h4(Wa, S2, (k), f, W))
     if (bottom) {
-        rv[1 * S2 + f] -= (9.0 / 16.0) * (FV(1) + FV(2)) - (1.0 / 16.0) * (FV(1) + FV(3));
-        const double u0 = r3(U, f), u1 = r3(U, S2 + f), u2 = r3(U, 2 * S2 + f);
-        const double uuu = ((9.0 / 16.0) * (u0 + u1) - (1.0 / 16.0) * (0.0 + u2)) * h4(Wa, S2, 1, f, 1);
-        const double uuu2 = ((9.0 / 16.0) * (0.0 + u0) - (1.0 / 16.0) * (0.0 + u1)) * h4(Wa, S2, 0, f, 1);
+        rv[1 * S2 + f] -= T(9.0 / 16.0) * (FV(1) + FV(2)) - T(1.0 / 16.0) * (FV(1) + FV(3));
+        const T u0 = r3(U, f), u1 = r3(U, S2 + f), u2 = r3(U, 2 * S2 + f);
+        const T uuu = (T(9.0 / 16.0) * (u0 + u1) - T(1.0 / 16.0) * (T(0) + u2)) * h4(Wa, S2, 1, f, 1);
+        const T uuu2 = (T(9.0 / 16.0) * (T(0) + u0) - T(1.0 / 16.0) * (T(0) + u1)) * h4(Wa, S2, 0, f, 1);
         ru[1 * S2 + f] -= (FU(2) - uuu);
         ru[f] -= (uuu - uuu2);
     }
     if (top) {
-        ru[(N - 2) * S2 + f] -= (9.0 / 16.0) * (FU(N - 1) + FU(N - 2)) - (1.0 / 16.0) * (FU(N - 1) + FU(N - 3));
-        rv[(N - 2) * S2 + f] -= (9.0 / 16.0) * (FV(N - 1) + FV(N - 2)) - (1.0 / 16.0) * (FV(N - 1) + FV(N - 3));
+        ru[(N - 2) * S2 + f] -= T(9.0 / 16.0) * (FU(N - 1) + FU(N - 2)) - T(1.0 / 16.0) * (FU(N - 1) + FU(N - 3));
+        rv[(N - 2) * S2 + f] -= T(9.0 / 16.0) * (FV(N - 1) + FV(N - 2)) - T(1.0 / 16.0) * (FV(N - 1) + FV(N - 3));
     }
     if (bottom) rv[f] = rv[S2 + f];
     if (!top) return;
@@ -461,18 +478,18 @@ void k_rhs3d_fused(G3 g, H h, A fomn, A Hz, A Wa, const int32_t *BC, double *ru,
 // ---- set_maxflux (step3D_uv.kernels.cpp:372-405) ----------------------------------
 // the fluxes are stored with streaming (non-temporal) stores: no thread of the launch
 // reads them back.  Both fluxes of a level are formed together, so Hz is read once.
-__global__ __launch_bounds__(256) void k_set_maxflux(G3 g, const double *u, const double *v, double *Huon,
-                                                     double *Hvom, const double *Hz)
+template <class T>
+__global__ __launch_bounds__(256) void k_set_maxflux(G3 g, const T *u, const T *v, T *Huon, T *Hvom, const T *Hz)
 {
     COLUMN_INDEX
     const bool un = isUNode(g, f), vn = isVNode(g, f);
     if (!un && !vn) return;
-    const double onu = un ? g.on_u[f] : 0.0, omv = vn ? g.om_v[f] : 0.0;
+    const T onu = un ? T(g.on_u[f]) : T(0), omv = vn ? T(g.om_v[f]) : T(0);
     for (long long K = 0; K < g.N; K++) {
         const long long b = K * g.S2 + f;
-        const double hz = Hz[b];
-        if (un) __builtin_nontemporal_store((((hz + Hz[b - 1]) * 0.5) * u[b]) * onu, Huon + b);
-        if (vn) __builtin_nontemporal_store((((hz + Hz[b - g.szI]) * 0.5) * v[b]) * omv, Hvom + b);
+        const T hz = Hz[b];
+        if (un) __builtin_nontemporal_store((((hz + Hz[b - 1]) * T(0.5)) * u[b]) * onu, Huon + b);
+        if (vn) __builtin_nontemporal_store((((hz + Hz[b - g.szI]) * T(0.5)) * v[b]) * omv, Hvom + b);
     }
 }
 
@@ -502,11 +519,12 @@ __global__ __launch_bounds__(256) void k_omega(G3 g, double *W, A Hu, A Hv, cons
 
 // omega with the column's partial sums kept in registers (N <= NMAX): W is written once
 // per level instead of written, read back and rewritten.  The values are those of
-// k_omega: the same partial sums, then W = w - (Ws * (z_w - zb)) / h.  (1024 x 512 x 32:
+// k_omega: the same partial sums, then W = w - (Ws * (z_w - zb)) / h.  The sums and the
+// correction are FP64 for either value type T of the fields.  (1024 x 512 x 32:
 // 138 us against 149 us for k_omega; recomputing the partial sums in a second pass
 // instead of holding them took 195 us.)
-template <int NMAX>
-__global__ __launch_bounds__(256) void k_omega_reg(G3 g, double *W, A Hu, A Hv, const double *z_w,
+template <int NMAX, class T>
+__global__ __launch_bounds__(256) void k_omega_reg(G3 g, T *W, AG<T> Hu, AG<T> Hv, const T *z_w,
                                                    const int32_t *BC)
 {
     COLUMN_INDEX
@@ -515,27 +533,27 @@ __global__ __launch_bounds__(256) void k_omega_reg(G3 g, double *W, A Hu, A Hv, 
     const int N = g.N;
     double wk[NMAX];
     double w = 0.0;
-    W[f] = w;
+    W[f] = T(0);
 #pragma unroll
     for (int K = 1; K < NMAX; K++) {
         wk[K] = 0.0;
         if (K < N) {
             const long long b = K * S2 + f;
-            w = w - ((r3(Hu, b + 1) - r3(Hu, b)) + (r3(Hv, b + g.szI) - r3(Hv, b)));
+            w = w - (((double)r3(Hu, b + 1) - (double)r3(Hu, b)) + ((double)r3(Hv, b + g.szI) - (double)r3(Hv, b)));
             wk[K] = w;
         }
     }
-    const double zb = z_w[f];
-    const double h = z_w[(long long)(N - 1) * S2 + f] - zb;
+    const double zb = (double)z_w[f];
+    const double h = (double)z_w[(long long)(N - 1) * S2 + f] - zb;
     const double Ws = w;   // N > 1
 #pragma unroll
     for (int K = 1; K < NMAX; K++) {
         if (K < N) {
             const long long b = K * S2 + f;
-            W[b] = wk[K] - (Ws * (z_w[b] - zb)) / h;
+            W[b] = T(wk[K] - (Ws * ((double)z_w[b] - zb)) / h);
         }
     }
-    W[(long long)N * S2 + f] = 0.0;
+    W[(long long)N * S2 + f] = T(0);
 }
 
 // ---- set_depth / set_zeta (step3D_uv.kernels.cpp:225-310) -------------------------
@@ -746,29 +764,47 @@ extern "C" int sf_vertical_advection(sf_ctx *c, const double *u, const double *v
     LAUNCH_COLUMNS(k_verticalAdvection, c, stream, grid3(c), A{u, n3}, A{v, n3}, A{W, n3}, ru, rv, BC);
 }
 
+template <class T>
+static int rhs3d_launch(sf_ctx *c, const double *fomn, const T *u, const T *v, const T *Huon, const T *Hvom,
+                        const T *W, const T *Hz, const int32_t *BC, T *ru, T *rv, int64_t launch_threads,
+                        int32_t cor, int32_t adv, void *stream)
+{
+    const long long n3 = (c->szK + 1LL) * c->S2;
+    HG<T> h{AG<T>{u, n3}, AG<T>{v, n3}, AG<T>{Huon, n3}, AG<T>{Hvom, n3}, c->szI};
+    const long long end = launch_threads < n3 ? (long long)launch_threads : n3;
+    constexpr int TJ = 8;   // 512-thread workgroups, 64 x 8 columns
+    const int tiles_i = (c->szI + HTI - 1) / HTI, tiles_j = (c->szJ + TJ - 1) / TJ, ntiles = tiles_i * tiles_j;
+    // level chunks: the fewest that give every resident workgroup slot (two per CU in
+    // FP64, three in FP32) a
+    // workgroup, with at least 3 levels per chunk (the bottom and top special levels
+    // each stay within one chunk); the last chunk takes the remainder.  1024 x 512 x 32
+    // runs unsplit, 512 x 256 x 32 in two chunks (101 us; 120 us in four).
+    const long long slots = (sizeof(T) == 4 ? 3LL : 2LL) * sf::cu_count(c->device);   // FP32: 69 VGPRs
+    int nchunk = 1;
+    while ((long long)ntiles * nchunk < slots && c->szK / (nchunk + 1) >= 3) nchunk++;
+    const int kchunk = c->szK / nchunk;
+    hipLaunchKernelGGL((k_rhs3d_fused<TJ, T>), dim3((unsigned)ntiles * (unsigned)nchunk), dim3(HTI * TJ), 0,
+                       sf::as_stream(stream), grid3(c), h, A{fomn, c->S2}, AG<T>{Hz, (long long)c->szK * c->S2},
+                       AG<T>{W, n3}, BC, ru, rv, end, cor ? 1 : 0, adv ? 1 : 0, tiles_i, ntiles, kchunk, nchunk);
+    return sf::check_launch("sf_rhs3d");
+}
+
 extern "C" int sf_rhs3d(sf_ctx *c, const double *fomn, const double *u, const double *v, const double *Huon,
                         const double *Hvom, const double *W, const double *Hz, const int32_t *BC, double *ru,
                         double *rv, int64_t launch_threads, int32_t cor, int32_t adv, void *stream)
 {
     CTX3(c);
     SF_REQUIRE(fomn && u && v && Huon && Hvom && W && Hz && BC && ru && rv, "sf_rhs3d: null field");
-    const long long n3 = (c->szK + 1LL) * c->S2;
-    H h{A{u, n3}, A{v, n3}, A{Huon, n3}, A{Hvom, n3}, c->szI};
-    const long long end = launch_threads < n3 ? (long long)launch_threads : n3;
-    constexpr int TJ = 8;   // 512-thread workgroups, 64 x 8 columns
-    const int tiles_i = (c->szI + HTI - 1) / HTI, tiles_j = (c->szJ + TJ - 1) / TJ, ntiles = tiles_i * tiles_j;
-    // level chunks: the fewest that give every resident workgroup slot (two per CU) a
-    // workgroup, with at least 3 levels per chunk (the bottom and top special levels
-    // each stay within one chunk); the last chunk takes the remainder.  1024 x 512 x 32
-    // runs unsplit, 512 x 256 x 32 in two chunks (101 us; 120 us in four).
-    const long long slots = 2LL * sf::cu_count(c->device);
-    int nchunk = 1;
-    while ((long long)ntiles * nchunk < slots && c->szK / (nchunk + 1) >= 3) nchunk++;
-    const int kchunk = c->szK / nchunk;
-    hipLaunchKernelGGL(k_rhs3d_fused<TJ>, dim3((unsigned)ntiles * (unsigned)nchunk), dim3(HTI * TJ), 0,
-                       sf::as_stream(stream), grid3(c), h, A{fomn, c->S2}, A{Hz, (long long)c->szK * c->S2},
-                       A{W, n3}, BC, ru, rv, end, cor ? 1 : 0, adv ? 1 : 0, tiles_i, ntiles, kchunk, nchunk);
-    return sf::check_launch("sf_rhs3d");
+    return rhs3d_launch<double>(c, fomn, u, v, Huon, Hvom, W, Hz, BC, ru, rv, launch_threads, cor, adv, stream);
+}
+
+extern "C" int sf_rhs3d_f32(sf_ctx *c, const double *fomn, const float *u, const float *v, const float *Huon,
+                            const float *Hvom, const float *W, const float *Hz, const int32_t *BC, float *ru,
+                            float *rv, int64_t launch_threads, int32_t cor, int32_t adv, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(fomn && u && v && Huon && Hvom && W && Hz && BC && ru && rv, "sf_rhs3d_f32: null field");
+    return rhs3d_launch<float>(c, fomn, u, v, Huon, Hvom, W, Hz, BC, ru, rv, launch_threads, cor, adv, stream);
 }
 
 extern "C" int sf_set_maxflux(sf_ctx *c, const double *u, const double *v, double *Huon, double *Hvom,
@@ -776,7 +812,30 @@ extern "C" int sf_set_maxflux(sf_ctx *c, const double *u, const double *v, doubl
 {
     CTX3(c);
     SF_REQUIRE(u && v && Huon && Hvom && Hz, "sf_set_maxflux: null field");
-    LAUNCH_COLUMNS(k_set_maxflux, c, stream, grid3(c), u, v, Huon, Hvom, Hz);
+    LAUNCH_COLUMNS(k_set_maxflux<double>, c, stream, grid3(c), u, v, Huon, Hvom, Hz);
+}
+
+extern "C" int sf_set_maxflux_f32(sf_ctx *c, const float *u, const float *v, float *Huon, float *Hvom,
+                                  const float *Hz, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(u && v && Huon && Hvom && Hz, "sf_set_maxflux_f32: null field");
+    LAUNCH_COLUMNS(k_set_maxflux<float>, c, stream, grid3(c), u, v, Huon, Hvom, Hz);
+}
+
+template <class T>
+static int omega_launch(sf_ctx *c, T *W, const T *Huon, const T *Hvom, const T *z_w, const int32_t *BC,
+                        void *stream)
+{
+    const long long n3 = (c->szK + 1LL) * c->S2;
+    const AG<T> hu{Huon, n3}, hv{Hvom, n3};
+    if (c->szK <= 16) {
+        LAUNCH_COLUMNS((k_omega_reg<16, T>), c, stream, grid3(c), W, hu, hv, z_w, BC);
+    }
+    if (c->szK <= 32) {
+        LAUNCH_COLUMNS((k_omega_reg<32, T>), c, stream, grid3(c), W, hu, hv, z_w, BC);
+    }
+    LAUNCH_COLUMNS((k_omega_reg<64, T>), c, stream, grid3(c), W, hu, hv, z_w, BC);
 }
 
 extern "C" int sf_omega(sf_ctx *c, double *W, const double *Huon, const double *Hvom, const double *z_w,
@@ -784,17 +843,18 @@ extern "C" int sf_omega(sf_ctx *c, double *W, const double *Huon, const double *
 {
     CTX3(c);
     SF_REQUIRE(W && Huon && Hvom && z_w && BC, "sf_omega: null field");
+    if (c->szK <= 64) return omega_launch<double>(c, W, Huon, Hvom, z_w, BC, stream);
     const long long n3 = (c->szK + 1LL) * c->S2;
-    if (c->szK <= 16) {
-        LAUNCH_COLUMNS(k_omega_reg<16>, c, stream, grid3(c), W, A{Huon, n3}, A{Hvom, n3}, z_w, BC);
-    }
-    if (c->szK <= 32) {
-        LAUNCH_COLUMNS(k_omega_reg<32>, c, stream, grid3(c), W, A{Huon, n3}, A{Hvom, n3}, z_w, BC);
-    }
-    if (c->szK <= 64) {
-        LAUNCH_COLUMNS(k_omega_reg<64>, c, stream, grid3(c), W, A{Huon, n3}, A{Hvom, n3}, z_w, BC);
-    }
     LAUNCH_COLUMNS(k_omega, c, stream, grid3(c), W, A{Huon, n3}, A{Hvom, n3}, z_w, BC);
+}
+
+extern "C" int sf_omega_f32(sf_ctx *c, float *W, const float *Huon, const float *Hvom, const float *z_w,
+                            const int32_t *BC, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(W && Huon && Hvom && z_w && BC, "sf_omega_f32: null field");
+    SF_REQUIRE(c->szK <= 64, "sf_omega_f32: at most 64 levels (N=%d)", c->szK);
+    return omega_launch<float>(c, W, Huon, Hvom, z_w, BC, stream);
 }
 
 extern "C" int sf_set_depth(sf_ctx *c, int32_t Vtransform, const double *Zt_avg1, double *z_w, double *z_r,

@@ -39,6 +39,17 @@ def _f64(t):
     return _ptr(t)
 
 
+def _bulk(name: str, fields) -> tuple[str, list]:
+    """Entry point for the precision of the 3D bulk fields: ``name`` when all are FP64,
+    ``name + "_f32"`` when all are FP32 (the FP32 bulk form); mixed precisions raise."""
+    dts = {t.dtype for t in fields}
+    if dts == {torch.float64}:
+        return name, [_ptr(t) for t in fields]
+    if dts == {torch.float32}:
+        return name + "_f32", [_ptr(t) for t in fields]
+    raise _lib.SailfishError(f"{name}: 3D fields must all be FP64 or all FP32, got {sorted(map(str, dts))}")
+
+
 class Context:
     """Owns one sf_ctx (freed with the object)."""
 
@@ -208,11 +219,14 @@ def horizontalAdvection(grid, block, args, ctx=None):
 def rhs3d_fused(args, launch_threads: int, cor: bool = True, adv: bool = True, ctx=None):
     """nonlinear/rhs.py:38-71 as one launch: ru, rv := 0, addCoriolis (cor),
     horizontalAdvection with the reference's launch of ``launch_threads`` threads,
-    verticalAdvection (adv).  args = (fomn, u, v, Huon, Hvom, W, Hz, ru, rv, BC)."""
+    verticalAdvection (adv).  args = (fomn, u, v, Huon, Hvom, W, Hz, ru, rv, BC); the 3D
+    fields all FP64 (bitwise the reference's arithmetic) or all FP32 (FP32 bulk form),
+    fomn FP64 in both."""
     fomn, u, v, Hu, Hv, W, Hz, ru, rv, BC = args
     c = ctx or ctx_of()
-    _lib.call("sf_rhs3d", c.handle, _f64(fomn), _f64(u), _f64(v), _f64(Hu), _f64(Hv), _f64(W), _f64(Hz), _ptr(BC),
-              _f64(ru), _f64(rv), int(launch_threads), int(bool(cor)), int(bool(adv)), _stream())
+    name, p = _bulk("sf_rhs3d", (u, v, Hu, Hv, W, Hz, ru, rv))
+    _lib.call(name, c.handle, _f64(fomn), *p[:6], _ptr(BC), *p[6:], int(launch_threads), int(bool(cor)),
+              int(bool(adv)), _stream())
 
 
 def verticalAdvection(grid, block, args, ctx=None):
@@ -224,17 +238,20 @@ def verticalAdvection(grid, block, args, ctx=None):
 
 
 def set_maxflux(grid, block, args, ctx=None):
-    """step3D_uv.kernels.cpp:372-405; args = (u, v, Huon, Hvom, Hz)."""
+    """step3D_uv.kernels.cpp:372-405; args = (u, v, Huon, Hvom, Hz), all FP64 or all FP32."""
     u, v, Hu, Hv, Hz = args
     c = ctx or ctx_of()
-    _lib.call("sf_set_maxflux", c.handle, _f64(u), _f64(v), _f64(Hu), _f64(Hv), _f64(Hz), _stream())
+    name, p = _bulk("sf_set_maxflux", (u, v, Hu, Hv, Hz))
+    _lib.call(name, c.handle, *p, _stream())
 
 
 def omega(grid, block, args, ctx=None):
-    """step3D_uv.kernels.cpp:62-165; args = (W, u, v, Huon, Hvom, z_w, BC)."""
+    """step3D_uv.kernels.cpp:62-165; args = (W, u, v, Huon, Hvom, z_w, BC); W, Huon, Hvom,
+    z_w all FP64 or all FP32 (column sums in FP64 either way)."""
     W, _u, _v, Hu, Hv, z_w, BC = args
     c = ctx or ctx_of()
-    _lib.call("sf_omega", c.handle, _f64(W), _f64(Hu), _f64(Hv), _f64(z_w), _ptr(BC), _stream())
+    name, p = _bulk("sf_omega", (W, Hu, Hv, z_w))
+    _lib.call(name, c.handle, *p, _ptr(BC), _stream())
 
 
 def set_depth(grid, block, args, ctx=None):

@@ -118,3 +118,48 @@ def test_rhs3d_fused_equals_four_launches(szI, szJ, N, cor, adv):
     assert bitwise_equal(OCEAN.ru_t2.cpu().numpy(), ref_u.cpu().numpy())
     assert bitwise_equal(OCEAN.rv_t2.cpu().numpy(), ref_v.cpu().numpy())
     assert np.abs(ref_u.cpu().numpy()).max() > 0
+
+
+@pytest.mark.parametrize("szI,szJ,N", [(130, 75, 20), (260, 140, 32)])
+def test_fp32_bulk_3d_kernels_against_fp64(szI, szJ, N):
+    """FP32 bulk forms of set_maxflux -> omega -> rhs3d, chained as in one baroclinic
+    step, against the FP64 forms (equal to the oracle bit for bit) on the same state
+    rounded once to FP32.  Error of each output, as max |FP32 - FP64| over max |FP64|:
+    Huon/Hvom <= 5e-7, W (omega) and ru/rv <= 1e-6 (measured 0.6-1.4e-7 on both grids)."""
+    import torch
+    from sailfish_amd import mod_operators as ops
+    from sailfish_amd.main3d import initial_depths
+    from sailfish_amd.model2d import build_model3d
+
+    print(f"[case] fp32 bulk 3D {szI}x{szJ}x{N}", file=sys.stderr, flush=True)
+    cfg = _cfg3(szI, szJ, N, seed=szI + N)
+    ct, GRID, OCEAN, BOUNDARY = build_model3d(cfg, N=N, ndtfast=4, dt=8.0, fix_q5=True)
+    initial_depths(GRID, OCEAN)
+    c = ops.ctx_of(GRID)
+    BC = BOUNDARY.device_maps()["bcIdxFieldIdx2"]
+    S2 = GRID.szI * GRID.szJ
+    lt = (S2 // 512 + 1) * 512
+    fomn = GRID.fomn.reshape(-1)
+
+    def chain(u, v, Hz, z_w):
+        Huon, Hvom, W = (torch.full(((N + 1) * S2,), 9.0, dtype=u.dtype, device=u.device) for _ in range(3))
+        ru, rv = (torch.full_like(u, 9.0) for _ in range(2))
+        ops.set_maxflux(None, None, (u, v, Huon, Hvom, Hz), ctx=c)
+        ops.omega(None, None, (W, u, v, Huon, Hvom, z_w, BC), ctx=c)
+        ops.rhs3d_fused((fomn, u, v, Huon, Hvom, W, Hz, ru, rv, BC), launch_threads=lt, ctx=c)
+        torch.cuda.synchronize()
+        return dict(Huon=Huon, Hvom=Hvom, W=W, ru=ru, rv=rv)
+
+    u64, v64 = OCEAN.u_t2.clone(), OCEAN.v_t2.clone()
+    Hz64, zw64 = GRID.Hz.reshape(-1).clone(), GRID.z_w.reshape(-1).clone()
+    # the FP64 state rounded once to FP32; the FP64 reference runs on those same values
+    u32, v32, Hz32, zw32 = (x.float() for x in (u64, v64, Hz64, zw64))
+    ref = chain(u32.double(), v32.double(), Hz32.double(), zw32.double())
+    got = chain(u32, v32, Hz32, zw32)
+    tol = dict(Huon=5e-7, Hvom=5e-7, W=1e-6, ru=1e-6, rv=1e-6)
+    for k, t in tol.items():
+        r, g = ref[k].cpu().numpy(), got[k].double().cpu().numpy()
+        scale = np.abs(r).max()
+        err = np.abs(g - r).max() / scale
+        print(f"  {k}: max |FP32 - FP64| / max |FP64| = {err:.2e} (max |FP64| {scale:.3e})", file=sys.stderr)
+        assert scale > 0 and err <= t, (k, err)
