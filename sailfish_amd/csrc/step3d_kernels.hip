@@ -347,11 +347,12 @@ __global__ __launch_bounds__(256) void k_verticalAdvection(G3 g, A U, A V, A Wa,
 // (levels 0, 1) does the bottom special levels, the last chunk (levels N-3 .. N) the
 // top ones; with more than one chunk (N >= 6) the two touch disjoint levels.
 //
-// The tile is double-buffered in LDS (one barrier per level), and the kernel is held to
-// 128 VGPRs (4 waves per SIMD: two 512-thread workgroups per CU) -- 138 VGPRs and 3
-// waves per SIMD otherwise, 25 % slower on 1024 x 512 x 32.
-template <int TJ, class T>
-__global__ __launch_bounds__(HTI * TJ) __attribute__((amdgpu_waves_per_eu(4)))
+// The tile is double-buffered in LDS (one barrier per level).  WPE is the wave count
+// per SIMD the register allocation is held to: FP64 instances at 4 (128 VGPRs, two
+// 512-thread workgroups per CU; 138 VGPRs and 3 waves otherwise, 25 % slower on
+// 1024 x 512 x 32); FP32 at 4 (69 VGPRs) or 8 (see rhs3d_launch).
+template <int TJ, class T, int WPE>
+__global__ __launch_bounds__(HTI * TJ) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_rhs3d_fused(G3 g, HG<T> h, A fomn, AG<T> Hz, AG<T> Wa, const int32_t *BC, T *ru, T *rv, long long tail_end,
                    int cor, int adv, int tiles_i, int ntiles, int kchunk, int nchunk)
 {
@@ -774,18 +775,32 @@ static int rhs3d_launch(sf_ctx *c, const double *fomn, const T *u, const T *v, c
     const long long end = launch_threads < n3 ? (long long)launch_threads : n3;
     constexpr int TJ = 8;   // 512-thread workgroups, 64 x 8 columns
     const int tiles_i = (c->szI + HTI - 1) / HTI, tiles_j = (c->szJ + TJ - 1) / TJ, ntiles = tiles_i * tiles_j;
+    const long long cus = sf::cu_count(c->device);
+    // FP32 instances need 69 VGPRs (7 waves per SIMD: three workgroups per CU).  Held to
+    // 64 VGPRs (8 waves, four workgroups, 11 VGPRs spilled) they are faster on grids of
+    // at least four tiles per CU (1024 x 512 x 32: 273 us against 312 us) and slower on
+    // smaller ones (512 x 256 x 32: 72 us against 64 us with level chunks).
+    const bool wide32 = sizeof(T) == 4 && ntiles >= 4 * cus;
     // level chunks: the fewest that give every resident workgroup slot (two per CU in
-    // FP64, three in FP32) a
-    // workgroup, with at least 3 levels per chunk (the bottom and top special levels
-    // each stay within one chunk); the last chunk takes the remainder.  1024 x 512 x 32
-    // runs unsplit, 512 x 256 x 32 in two chunks (101 us; 120 us in four).
-    const long long slots = (sizeof(T) == 4 ? 3LL : 2LL) * sf::cu_count(c->device);   // FP32: 69 VGPRs
+    // FP64, three in FP32) a workgroup, with at least 3 levels per chunk (the bottom and
+    // top special levels each stay within one chunk); the last chunk takes the remainder.
+    // FP64 1024 x 512 x 32 runs unsplit, 512 x 256 x 32 in two chunks (101 us; 120 us in
+    // four).
+    const long long slots = (sizeof(T) == 4 ? 3 : 2) * cus;
     int nchunk = 1;
-    while ((long long)ntiles * nchunk < slots && c->szK / (nchunk + 1) >= 3) nchunk++;
+    while (!wide32 && (long long)ntiles * nchunk < slots && c->szK / (nchunk + 1) >= 3) nchunk++;
     const int kchunk = c->szK / nchunk;
-    hipLaunchKernelGGL((k_rhs3d_fused<TJ, T>), dim3((unsigned)ntiles * (unsigned)nchunk), dim3(HTI * TJ), 0,
-                       sf::as_stream(stream), grid3(c), h, A{fomn, c->S2}, AG<T>{Hz, (long long)c->szK * c->S2},
-                       AG<T>{W, n3}, BC, ru, rv, end, cor ? 1 : 0, adv ? 1 : 0, tiles_i, ntiles, kchunk, nchunk);
+    const dim3 grid((unsigned)ntiles * (unsigned)nchunk), block(HTI * TJ);
+    const AG<T> hz{Hz, (long long)c->szK * c->S2}, wa{W, n3};
+    if (wide32) {
+        hipLaunchKernelGGL((k_rhs3d_fused<TJ, T, 8>), grid, block, 0, sf::as_stream(stream), grid3(c), h,
+                           A{fomn, c->S2}, hz, wa, BC, ru, rv, end, cor ? 1 : 0, adv ? 1 : 0, tiles_i, ntiles,
+                           kchunk, nchunk);
+    } else {
+        hipLaunchKernelGGL((k_rhs3d_fused<TJ, T, 4>), grid, block, 0, sf::as_stream(stream), grid3(c), h,
+                           A{fomn, c->S2}, hz, wa, BC, ru, rv, end, cor ? 1 : 0, adv ? 1 : 0, tiles_i, ntiles,
+                           kchunk, nchunk);
+    }
     return sf::check_launch("sf_rhs3d");
 }
 
