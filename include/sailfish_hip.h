@@ -180,13 +180,23 @@ int sf_set_lateral_uv_bcs(sf_ctx *ctx, double t, double *u, double *v, const int
                           const int32_t *vType, void *stream);
 
 /* step3d_UV (nonlinear/step3D_uv.kernels.cpp:646-707) with its scratch arrays SD, D,
- * RHS ((szK+1)*S2 doubles each, nonlinear/main3d.py:48-58). */
+ * RHS ((szK+1)*S2 doubles each, nonlinear/main3d.py:48-58).  sf_step3d_uv_f32 (declared
+ * below) is its FP32 bulk form for the same call site (main3d.py:304-306). */
 int sf_step3d_uv(sf_ctx *ctx, double *u, double *v, const double *ru, const double *rv,
                  double *ubar_t1, double *vbar_t1, double *ubar_t2, double *vbar_t2,
                  const double *Hz, const double *Akv, const double *z_r, const double *DU_avg1,
                  const double *DV_avg1, const double *tmpU, const double *tmpV, int32_t iic,
                  int32_t ntfirst, double lambda, double Dt, double *SD, double *D, double *RHS,
                  void *stream);
+
+/* FP32 bulk form of sf_step3d_uv: u, v, ru, rv, the 2D ubar/vbar, Hz, Akv, z_r, tmpU,
+ * tmpV and the scratch in FP32; DU_avg1/DV_avg1 (barotropic averages) in FP64; the
+ * column sums (depth means, barotropic velocities) formed in FP64. */
+int sf_step3d_uv_f32(sf_ctx *ctx, float *u, float *v, const float *ru, const float *rv, float *ubar_t1,
+                     float *vbar_t1, float *ubar_t2, float *vbar_t2, const float *Hz, const float *Akv,
+                     const float *z_r, const double *DU_avg1, const double *DV_avg1, const float *tmpU,
+                     const float *tmpV, int32_t iic, int32_t ntfirst, double lambda, double Dt, float *SD, float *D,
+                     float *RHS, void *stream);
 
 /* ---------------- fused barotropic substep --------------------------------------- */
 

@@ -62,6 +62,7 @@ _SIGS = {
     "sf_rhs3d_f32": [P, P, P, P, P, P, P, P, P, P, P, I64, I32, I32, P],
     "sf_set_maxflux_f32": [P, P, P, P, P, P, P],
     "sf_omega_f32": [P, P, P, P, P, P, P],
+    "sf_step3d_uv_f32": [P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I32, I32, F64, F64, P, P, P, P],
     "sf_set_depth": [P, I32, P, P, P, P, F64, P, P, P, P, P, P],
     "sf_set_zeta": [P, P, P, P, P],
     "sf_set_lateral_uv_bcs": [P, F64, P, P, P, P, P],

@@ -611,41 +611,47 @@ __global__ __launch_bounds__(256) void k_lateral_uv(G3 g, double t, double *u, d
 }
 
 // ---- step3d_UV (step3D_uv.kernels.cpp:646-707) ------------------------------------
+// T: the 3D fields, the 2D barotropic velocities and the tridiagonal scratch (double:
+// the reference's arithmetic, bit for bit; float: FP32 bulk form).  The grid metrics and
+// DU_avg1/DV_avg1 are FP64, and the column sums (depth means, barotropic velocities) are
+// formed in FP64 for either T.
+template <class T>
 __device__ void build_matrix(const G3 &g, bool vdir, long long f, double cff, double Dt, double lambda,
-                             double *SD, double *D, double *RHS, const double *Hz, const double *Akv,
-                             const double *z_r, const double *u, const double *ru)
+                             T *SD, T *D, T *RHS, const T *Hz, const T *Akv, const T *z_r, const T *u, const T *ru)
 {
     const long long S2 = g.S2, N = g.N;
     const long long d = vdir ? g.szI : 1;
     if (vdir ? !isVNode(g, f) : !isUNode(g, f)) return;
-#define HZU(K) ((Hz[(K) * S2 + f] + Hz[(K) * S2 + f - d]) * 0.5)
-#define AKVU(K) ((Akv[(K) * S2 + f] + Akv[(K) * S2 + f - d]) * 0.5)
-#define DZ(K) (((z_r[(K) * S2 + f] - z_r[((K) - 1) * S2 + f]) + (z_r[(K) * S2 + f - d] - z_r[((K) - 1) * S2 + f - d])) * 0.5)
-    const double pmU = (g.pm[f] + g.pm[f - d]) * 0.5;
-    const double pnU = (g.pn[f] + g.pn[f - d]) * 0.5;
-    const double cdt_mn = (cff * Dt) * (pmU * pnU);
-    SD[(N - 1) * S2 + f] = 0.0;
+#define HZU(K) ((Hz[(K) * S2 + f] + Hz[(K) * S2 + f - d]) * T(0.5))
+#define AKVU(K) ((Akv[(K) * S2 + f] + Akv[(K) * S2 + f - d]) * T(0.5))
+#define DZ(K) (((z_r[(K) * S2 + f] - z_r[((K) - 1) * S2 + f]) + (z_r[(K) * S2 + f - d] - z_r[((K) - 1) * S2 + f - d])) * T(0.5))
+    const T pmU = (T(g.pm[f]) + T(g.pm[f - d])) * T(0.5);
+    const T pnU = (T(g.pn[f]) + T(g.pn[f - d])) * T(0.5);
+    const T cdt_mn = T(cff * Dt) * (pmU * pnU);
+    const T mldt = T((-lambda) * Dt);
+    SD[(N - 1) * S2 + f] = T(0);
     D[f] = HZU(0);
-    RHS[f] = 0.0;
-    for (long long K = 0; K < N - 1; K++) SD[K * S2 + f] = (((-lambda) * Dt) * AKVU(K)) / DZ(K + 1);
+    RHS[f] = T(0);
+    for (long long K = 0; K < N - 1; K++) SD[K * S2 + f] = (mldt * AKVU(K)) / DZ(K + 1);
     for (long long K = 1; K < N; K++) {
         RHS[K * S2 + f] = HZU(K) * (u[K * S2 + f] + cdt_mn * ru[K * S2 + f]);
         D[K * S2 + f] = (HZU(K) - SD[K * S2 + f]) - SD[(K - 1) * S2 + f];
     }
     D[(N - 1) * S2 + f] = HZU(N - 1) - SD[(N - 2) * S2 + f];
-    SD[f] = (((-lambda) * Dt) * AKVU(0)) / DZ(1);
-    D[f] = HZU(0) - 2.0 * SD[f];
+    SD[f] = (mldt * AKVU(0)) / DZ(1);
+    D[f] = HZU(0) - T(2) * SD[f];
 #undef HZU
 #undef AKVU
 #undef DZ
 }
 
-__device__ void solve_tri(const G3 &g, long long f, double *SD, double *D, double *RHS, double *x)
+template <class T>
+__device__ void solve_tri(const G3 &g, long long f, T *SD, T *D, T *RHS, T *x)
 {
     const long long S2 = g.S2, N = g.N;
     if (!isUNode(g, f)) return;
     for (long long k = 1; k < N; k++) {
-        const double w = SD[(k - 1) * S2 + f] / D[(k - 1) * S2 + f];
+        const T w = SD[(k - 1) * S2 + f] / D[(k - 1) * S2 + f];
         D[k * S2 + f] -= w * SD[(k - 1) * S2 + f];
         RHS[k * S2 + f] -= w * RHS[(k - 1) * S2 + f];
     }
@@ -654,31 +660,36 @@ __device__ void solve_tri(const G3 &g, long long f, double *SD, double *D, doubl
         x[k * S2 + f] = (RHS[k * S2 + f] - SD[k * S2 + f] * x[(k + 1) * S2 + f]) / D[k * S2 + f];
 }
 
-__device__ void correct_mean(const G3 &g, bool vdir, long long f, double *x, const double *Hz, const double *Davg,
-                             const double *metric, const double *xold)
+template <class T>
+__device__ void correct_mean(const G3 &g, bool vdir, long long f, T *x, const T *Hz, const double *Davg,
+                             const double *metric, const T *xold)
 {
     const long long S2 = g.S2, N = g.N;
     const long long d = vdir ? g.szI : 1;
     if (vdir ? !isVNode(g, f) : !isUNode(g, f)) return;
     double Hs = 0.0, Hx = 0.0;
     for (long long K = 0; K < N; K++) {
-        const double hu = (Hz[K * S2 + f] + Hz[K * S2 + f - d]) * 0.5;
-        Hs += hu;
-        Hx += xold[K * S2 + f] * hu;
+        const T hu = (Hz[K * S2 + f] + Hz[K * S2 + f - d]) * T(0.5);
+        Hs += (double)hu;
+        Hx += (double)xold[K * S2 + f] * (double)hu;
     }
     const double diff = (Hx - Davg[f] / metric[f]) / Hs;
-    for (long long K = 0; K < N; K++) x[K * S2 + f] = x[K * S2 + f] - diff;
+    for (long long K = 0; K < N; K++) x[K * S2 + f] = T((double)x[K * S2 + f] - diff);
 }
 
+template <class T>
 struct UVArgs {
-    double *u, *v;
-    const double *ru, *rv;
-    double *ubar_t1, *vbar_t1, *ubar_t2, *vbar_t2;
-    const double *Hz, *Akv, *z_r, *DU_avg1, *DV_avg1, *tmpU, *tmpV;
-    double *SD, *D, *RHS;
+    T *u, *v;
+    const T *ru, *rv;
+    T *ubar_t1, *vbar_t1, *ubar_t2, *vbar_t2;
+    const T *Hz, *Akv, *z_r;
+    const double *DU_avg1, *DV_avg1;
+    const T *tmpU, *tmpV;
+    T *SD, *D, *RHS;
 };
 
-__global__ __launch_bounds__(256) void k_step3d_uv(G3 g, UVArgs a, double cff, double lambda, double Dt)
+template <class T>
+__global__ __launch_bounds__(256) void k_step3d_uv(G3 g, UVArgs<T> a, double cff, double lambda, double Dt)
 {
     COLUMN_INDEX
     const long long S2 = g.S2, N = g.N;
@@ -692,21 +703,21 @@ __global__ __launch_bounds__(256) void k_step3d_uv(G3 g, UVArgs a, double cff, d
     double HU = 0.0, HV = 0.0, HuU = 0.0, HvV = 0.0;
     for (long long K = 0; K < N; K++) {
         const long long b = K * S2 + f;
-        const double hzu = (a.Hz[b] + a.Hz[b - 1]) * 0.5;
-        const double hzv = (a.Hz[b] + a.Hz[b - g.szI]) * 0.5;
-        HU += hzu;
-        HV += hzv;
-        HuU += hzu * a.u[b];
-        HvV += hzv * a.v[b];
+        const T hzu = (a.Hz[b] + a.Hz[b - 1]) * T(0.5);
+        const T hzv = (a.Hz[b] + a.Hz[b - g.szI]) * T(0.5);
+        HU += (double)hzu;
+        HV += (double)hzv;
+        HuU += (double)hzu * (double)a.u[b];
+        HvV += (double)hzv * (double)a.v[b];
     }
     const double uUp = (HuU - a.DU_avg1[f] / g.on_u[f]) / HU;
     const double vUp = (HvV - a.DV_avg1[f] / g.om_v[f]) / HV;
     const double ub = (a.DU_avg1[f] / g.on_u[f]) / HU + uUp;
     const double vb = (a.DV_avg1[f] / g.om_v[f]) / HV + vUp;
-    a.ubar_t2[f] = ub;
-    a.vbar_t2[f] = vb;
-    a.ubar_t1[f] = ub;
-    a.vbar_t1[f] = vb;
+    a.ubar_t2[f] = T(ub);
+    a.vbar_t2[f] = T(vb);
+    a.ubar_t1[f] = T(ub);
+    a.vbar_t1[f] = T(vb);
 }
 
 G3 grid3(const sf_ctx *c)
@@ -910,6 +921,26 @@ extern "C" int sf_step3d_uv(sf_ctx *c, double *u, double *v, const double *ru, c
     if (iic == ntfirst) cff = 1.0;
     else if (iic == ntfirst + 1) cff = 3.0 / 2.0;
     else cff = 23.0 / 12.0;
-    UVArgs a{u, v, ru, rv, ubar_t1, vbar_t1, ubar_t2, vbar_t2, Hz, Akv, z_r, DU_avg1, DV_avg1, tmpU, tmpV, SD, D, RHS};
-    LAUNCH_COLUMNS(k_step3d_uv, c, stream, grid3(c), a, cff, lambda, Dt);
+    UVArgs<double> a{u, v, ru, rv, ubar_t1, vbar_t1, ubar_t2, vbar_t2, Hz, Akv, z_r, DU_avg1, DV_avg1, tmpU, tmpV,
+                     SD, D, RHS};
+    LAUNCH_COLUMNS(k_step3d_uv<double>, c, stream, grid3(c), a, cff, lambda, Dt);
+}
+
+extern "C" int sf_step3d_uv_f32(sf_ctx *c, float *u, float *v, const float *ru, const float *rv, float *ubar_t1,
+                                float *vbar_t1, float *ubar_t2, float *vbar_t2, const float *Hz, const float *Akv,
+                                const float *z_r, const double *DU_avg1, const double *DV_avg1, const float *tmpU,
+                                const float *tmpV, int32_t iic, int32_t ntfirst, double lambda, double Dt, float *SD,
+                                float *D, float *RHS, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(u && v && ru && rv && ubar_t1 && vbar_t1 && ubar_t2 && vbar_t2 && Hz && Akv && z_r && DU_avg1 &&
+                   DV_avg1 && tmpU && tmpV && SD && D && RHS,
+               "sf_step3d_uv_f32: null field");
+    double cff;
+    if (iic == ntfirst) cff = 1.0;
+    else if (iic == ntfirst + 1) cff = 3.0 / 2.0;
+    else cff = 23.0 / 12.0;
+    UVArgs<float> a{u, v, ru, rv, ubar_t1, vbar_t1, ubar_t2, vbar_t2, Hz, Akv, z_r, DU_avg1, DV_avg1, tmpU, tmpV,
+                    SD, D, RHS};
+    LAUNCH_COLUMNS(k_step3d_uv<float>, c, stream, grid3(c), a, cff, lambda, Dt);
 }

@@ -301,6 +301,24 @@ def step3d_UV(grid, block, args, ctx=None):
     if id(c) not in _velEq:
         raise _lib.SailfishError("setVerticalVelEq has not been called for this grid")
     SD, D, RHS = _velEq[id(c)]
-    _lib.call("sf_step3d_uv", c.handle, _f64(u), _f64(v), _f64(ru), _f64(rv), _f64(ub1), _f64(vb1), _f64(ub2),
-              _f64(vb2), _f64(Hz), _f64(AKv), _f64(z_r), _f64(DU), _f64(DV), _f64(tmpU), _f64(tmpV), int(iic),
-              int(ntfirst), float(lam), float(dt), _f64(SD), _f64(D), _f64(RHS), _stream())
+    if u.dtype == torch.float32:
+        # FP32 bulk form: 3D fields, ubar/vbar and the scratch in FP32, DU/DV_avg1 in FP64
+        SD, D, RHS = _velEq32(c, SD)
+    name, p = _bulk("sf_step3d_uv", (u, v, ru, rv, ub1, vb1, ub2, vb2, Hz, AKv, z_r))
+    q = [_ptr(x) for x in (tmpU, tmpV)] if name.endswith("_f32") else [_f64(tmpU), _f64(tmpV)]
+    if name.endswith("_f32") and {tmpU.dtype, tmpV.dtype} != {torch.float32}:
+        raise _lib.SailfishError("step3d_UV: tmpU/tmpV must be FP32 with FP32 fields")
+    _lib.call(name, c.handle, *p, _f64(DU), _f64(DV), *q, int(iic), int(ntfirst), float(lam), float(dt),
+              _ptr(SD), _ptr(D), _ptr(RHS), _stream())
+
+
+_velEq32_cache: dict = {}
+
+
+def _velEq32(c, like):
+    """FP32 scratch of the FP32 step3d_UV (same shape as the FP64 eqSD/eqD/eqRHS)."""
+    key = id(c)
+    if key not in _velEq32_cache or _velEq32_cache[key][0].numel() != like.numel():
+        _velEq32_cache[key] = tuple(torch.zeros(like.numel(), dtype=torch.float32, device=like.device)
+                                    for _ in range(3))
+    return _velEq32_cache[key]
