@@ -998,7 +998,7 @@ int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void 
     a.rdrg = T(x->rdrg);
     a.dt = T(dt);
     a.first = first;
-    a.nts = (double)x->szI * x->szJ * (3.0 * sizeof(T) * 7 + 8.0 * 6) > 384.0 * (1 << 20) ? 1 : 0;
+    a.nts = (double)x->szI * x->szJ * (3.0 * sizeof(T) * 7 + 8.0 * 6) > sf::kStreamingBytes ? 1 : 0;
     a.w1 = w1;
     SF_REQUIRE((long long)x->szI * x->szJ < (1LL << 29), "sf_phys_substep_fused: grid of %dx%d points exceeds 2^29",
                x->szI, x->szJ);
@@ -1142,7 +1142,7 @@ int ens_build(sf_ens *e)
     double bytes = 0.0;   // one substep of the whole ensemble (see PFA::nts)
     for (int k = 0; k < e->n; k++)
         bytes += (double)e->host[k].szI * e->host[k].szJ * (3.0 * sizeof(T) * 7 + 8.0 * 6);
-    const int nts = bytes > 384.0 * (1 << 20) ? 1 : 0;
+    const int nts = bytes > sf::kStreamingBytes ? 1 : 0;
     e->nts = nts;
     for (int k = 0; k < e->n; k++) {
         const sf_phys_fields &x = e->host[k];

@@ -46,6 +46,13 @@ __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x
 
 constexpr int kBlock1D = 256;
 
+// Bytes one substep must move before the fused 2D substeps write their outputs with
+// streaming (non-temporal) stores: beyond the 256 MB last-level cache the next substep
+// cannot re-read them from it, and streaming keeps L2 for the halo reads.  Measured on
+// 2048 x 1024: plain stores faster at 264 MiB (FP32 physics), streaming faster at 352 MiB
+// (FP64 reference discretisation) and 432 MiB (FP64 physics).
+constexpr double kStreamingBytes = 320.0 * (1 << 20);
+
 // compute units of a device, queried once per device (256 on MI355X)
 inline int cu_count(int device)
 {

@@ -96,7 +96,22 @@ __device__ __forceinline__ bool is_masked(long long f, int stride)
     return (f % stride == 0) || (f / stride == 0);
 }
 
-template <typename T>
+// NTS: outputs written with streaming (non-temporal) stores and the FP64 averages read
+// with streaming loads (chosen by the launcher from the bytes one substep moves)
+template <bool NTS, typename V>
+__device__ __forceinline__ void stv(V *p, V v)
+{
+    if constexpr (NTS) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+template <bool NTS>
+__device__ __forceinline__ double ldavg(const double *p)
+{
+    if constexpr (NTS) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
+template <typename T, bool NTS>
 __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
 {
     // six LDS planes on the R2 frame, reused as their contents die:
@@ -158,9 +173,9 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
                 rz0[k] = F.rzeta_t0[f];
                 rz2[k] = F.rzeta_t2[f];
                 if (interior(i, j)) {
-                    zt[k] = F.Zt_avg1[f];
-                    du[k] = F.DU_avg1[f];
-                    dv[k] = F.DV_avg1[f];
+                    zt[k] = ldavg<NTS>(F.Zt_avg1 + f);
+                    du[k] = ldavg<NTS>(F.DU_avg1 + f);
+                    dv[k] = ldavg<NTS>(F.DV_avg1 + f);
                 }
             }
         }
@@ -289,7 +304,7 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
                     const T hv = (sH[r] + sH[r - W2]) * T(0.5);
                     ru = (gc * onu[k]) * (hu * (gz - gzl) + T(0.5) * (gz * gz - gzl * gzl));
                     rv = (gc * omv[k]) * (hv * (gz - gzd) + T(0.5) * (gz * gz - gzd * gzd));
-                    if (inT) F.rubar_t1[f] = ru;
+                    if (inT) stv<NTS>(F.rubar_t1 + f, ru);
                 } else {
                     ru = F.rubar_t1[f];   // never written at non-interior points
                     rv = F.rvbar_t1[f];
@@ -305,7 +320,7 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
                     const T gz = sGZ[r];
                     const T gzl = sGZ[r - 1];
                     const T hu = (sH[r] + sH[r - 1]) * T(0.5);
-                    F.rubar_t1[f] = (gc * onu[k]) * (hu * (gz - gzl) + T(0.5) * (gz * gz - gzl * gzl));
+                    stv<NTS>(F.rubar_t1 + f, (gc * onu[k]) * (hu * (gz - gzl) + T(0.5) * (gz * gz - gzl * gzl)));
                 }
             }
             if (band(i, j)) {
@@ -326,9 +341,9 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
             px = DU * onu[k];
             py = DV * omv[k];
             if (inT && interior(i, j)) {
-                F.Zt_avg1[f] = zt[k] + cff1 * (double)zp[k];
-                F.DU_avg1[f] = du[k] + (cff1 * (double)DU) * (double)onu[k];
-                F.DV_avg1[f] = dv[k] + (cff1 * (double)DV) * (double)omv[k];
+                stv<NTS>(F.Zt_avg1 + f, zt[k] + cff1 * (double)zp[k]);
+                stv<NTS>(F.DU_avg1 + f, du[k] + (cff1 * (double)DU) * (double)onu[k]);
+                stv<NTS>(F.DV_avg1 + f, dv[k] + (cff1 * (double)DV) * (double)omv[k]);
             }
         }
         sPX[r] = px;
@@ -349,7 +364,7 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
         T rz;
         if (interior(i, j)) {
             rz = (((sPX[r + 1] - sPX[r]) + (sPY[r + W2] - sPY[r])) * pmr[k]) * pnr[k];
-            F.rzeta_t1[f] = rz;
+            stv<NTS>(F.rzeta_t1 + f, rz);
         } else {
             rz = F.rzeta_t1[f];
         }
@@ -372,16 +387,16 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
             const int s = F.bc_zeta[f];
             if (s >= 0) z = sZc[lidx(s % szI - gi0, s / szI - gj0)];
         }
-        F.zeta_t2[f] = z;
+        stv<NTS>(F.zeta_t2 + f, z);
         if (!is_masked(f, P.q5_stride)) {
-            F.ubar_t2[f] = up[k];
-            F.vbar_t2[f] = P.zero_vbar ? T(0) : vp[k];
+            stv<NTS>(F.ubar_t2 + f, up[k]);
+            stv<NTS>(F.vbar_t2 + f, P.zero_vbar ? T(0) : vp[k]);
         } else if (P.fold_masked) {
             // stride == szI: the masked points are column 0 and row 0, which no
             // other workgroup reads, so they are finished here (as k_step2d_masked)
-            F.ubar_t2[f] = up[k];
-            F.vbar_t2[f] = P.zero_vbar ? T(0) : vp[k];
-            F.ubar_t1[f] = am3(ub1[k], Dt, F.rubar_t1[f], F.rubar_t0[f], vb1[k]);
+            stv<NTS>(F.ubar_t2 + f, up[k]);
+            stv<NTS>(F.vbar_t2 + f, P.zero_vbar ? T(0) : vp[k]);
+            stv<NTS>(F.ubar_t1 + f, am3(ub1[k], Dt, F.rubar_t1[f], F.rubar_t0[f], vb1[k]));
         }
     }
 }
@@ -466,7 +481,13 @@ int launch(sf_ctx *c, const sf_step2d_fields *a, double dtfast, double cff1, dou
     P.zero_vbar = (flags & SF_SUBSTEP_ZERO_VBAR) ? 1 : 0;
     P.fold_masked = (c->q5_stride == c->szI) ? 1 : 0;
     if (parts & 1) {
-        hipLaunchKernelGGL(k_step2d_fused<T>, dim3(P.tiles_i * P.tiles_j), dim3(NT), 0, s, F, P);
+        // 11 bulk fields read and 5 written, 3 FP64 averages read and written
+        const bool nts = (double)c->S2 * (16.0 * sizeof(T) + 6.0 * 8) > sf::kStreamingBytes;
+        if (nts) {
+            hipLaunchKernelGGL((k_step2d_fused<T, true>), dim3(P.tiles_i * P.tiles_j), dim3(NT), 0, s, F, P);
+        } else {
+            hipLaunchKernelGGL((k_step2d_fused<T, false>), dim3(P.tiles_i * P.tiles_j), dim3(NT), 0, s, F, P);
+        }
         int rc = sf::check_launch("sf_step2d_substep (tiles)");
         if (rc != SF_OK) return rc;
     }
