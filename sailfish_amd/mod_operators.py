@@ -301,8 +301,9 @@ def step3d_UV(grid, block, args, ctx=None):
     if id(c) not in _velEq:
         raise _lib.SailfishError("setVerticalVelEq has not been called for this grid")
     SD, D, RHS = _velEq[id(c)]
-    if u.dtype == torch.float32:
-        # FP32 bulk form: 3D fields, ubar/vbar and the scratch in FP32, DU/DV_avg1 in FP64
+    if u.dtype == torch.float32 and SD.dtype != torch.float32:
+        # FP32 bulk form: 3D fields, ubar/vbar and the scratch in FP32, DU/DV_avg1 in FP64;
+        # FP64 scratch registered by setVerticalVelEq is replaced by FP32 scratch of its size
         SD, D, RHS = _velEq32(c, SD)
     name, p = _bulk("sf_step3d_uv", (u, v, ru, rv, ub1, vb1, ub2, vb2, Hz, AKv, z_r))
     q = [_ptr(x) for x in (tmpU, tmpV)] if name.endswith("_f32") else [_f64(tmpU), _f64(tmpV)]
@@ -316,9 +317,10 @@ _velEq32_cache: dict = {}
 
 
 def _velEq32(c, like):
-    """FP32 scratch of the FP32 step3d_UV (same shape as the FP64 eqSD/eqD/eqRHS)."""
+    """FP32 scratch of the FP32 step3d_UV (same shape and device as the FP64 eqSD/eqD/eqRHS)."""
     key = id(c)
-    if key not in _velEq32_cache or _velEq32_cache[key][0].numel() != like.numel():
+    if (key not in _velEq32_cache or _velEq32_cache[key][0].numel() != like.numel()
+            or _velEq32_cache[key][0].device != like.device):
         _velEq32_cache[key] = tuple(torch.zeros(like.numel(), dtype=torch.float32, device=like.device)
                                     for _ in range(3))
     return _velEq32_cache[key]

@@ -78,3 +78,39 @@ def compare_states(prod, orc, names=STATE2D + ("Zt_avg1", "DU_avg1", "DV_avg1"),
                     nmis = int(np.sum((x.view(np.uint64) != y.view(np.uint64)) & ~(np.isnan(x) & np.isnan(y))))
                 bad.append((n, k, nmis, float(np.nanmax(d)) if np.isfinite(d).any() else float("nan")))
     return bad
+
+
+def masked_points(S2, stride):
+    f = np.arange(S2)
+    return (f % stride == 0) | (f // stride == 0)
+
+
+def compare_fused(prod, orc, szI, szJ, stride):
+    """Mismatches of the fused substep's state against the oracle on every array the path
+    reads: all of zeta, vbar, rzeta, the FP64 averages, ubar t2 and rubar, and ubar t0/t1
+    at the masked points (stride: the computeMomentumPred mask stride).  The fused kernel
+    does not write what no kernel reads (rvbar, the dead AM3 increment of ubar t0/t1 at
+    unmasked points)."""
+    bad = []
+    assert prod["slots"] == orc["slots"]
+    sl = prod["slots"]
+    masked = masked_points(szI * szJ, stride)
+    for n in ("zeta", "vbar", "rzeta"):
+        for k in range(3):
+            if not bitwise_equal(prod[n][k], orc[n][k]):
+                bad.append((n, k))
+    for n in ("Zt_avg1", "DU_avg1", "DV_avg1"):
+        if not bitwise_equal(prod[n], orc[n]):
+            bad.append((n, -1))
+    t2 = sl["ubar"][2]
+    if not bitwise_equal(prod["ubar"][t2], orc["ubar"][t2]):
+        bad.append(("ubar", "t2"))
+    for lev in (0, 1):
+        s = sl["ubar"][lev]
+        if not bitwise_equal(prod["ubar"][s][masked], orc["ubar"][s][masked]):
+            bad.append(("ubar", f"t{lev}@masked"))
+    for lev in (0, 1):
+        s = sl["rubar"][lev]
+        if not bitwise_equal(prod["rubar"][s], orc["rubar"][s]):
+            bad.append(("rubar", f"t{lev}"))
+    return bad

@@ -13,40 +13,9 @@ import numpy as np
 import pytest
 
 from sailfish_amd import configs
-from tests.helpers2d import bitwise_equal, oracle_arrays, oracle_model, product_arrays
+from tests.helpers2d import compare_fused, oracle_arrays, oracle_model, product_arrays
 
 pytestmark = pytest.mark.gpu
-
-
-def _masked(S2, stride):
-    f = np.arange(S2)
-    return (f % stride == 0) | (f // stride == 0)
-
-
-def compare_fused(prod, orc, szI, szJ, stride):
-    bad = []
-    assert prod["slots"] == orc["slots"]
-    sl = prod["slots"]
-    masked = _masked(szI * szJ, stride)
-    for n in ("zeta", "vbar", "rzeta"):
-        for k in range(3):
-            if not bitwise_equal(prod[n][k], orc[n][k]):
-                bad.append((n, k))
-    for n in ("Zt_avg1", "DU_avg1", "DV_avg1"):
-        if not bitwise_equal(prod[n], orc[n]):
-            bad.append((n, -1))
-    t2 = sl["ubar"][2]
-    if not bitwise_equal(prod["ubar"][t2], orc["ubar"][t2]):
-        bad.append(("ubar", "t2"))
-    for lev in (0, 1):
-        s = sl["ubar"][lev]
-        if not bitwise_equal(prod["ubar"][s][masked], orc["ubar"][s][masked]):
-            bad.append(("ubar", f"t{lev}@masked"))
-    for lev in (0, 1):
-        s = sl["rubar"][lev]
-        if not bitwise_equal(prod["rubar"][s], orc["rubar"][s]):
-            bad.append(("rubar", f"t{lev}"))
-    return bad
 
 
 def _run(cfg, nsteps, mode, fix_q5, dtype="f64", lbc=None):
