@@ -357,6 +357,43 @@ def make_samples(fine: Model, fine_probe: Model, probe: Model, nsnap: int, spaci
     return torch.cat(feats), torch.cat(targs)
 
 
+def make_samples_step(fine: Model, coarse: Model, nsnap: int, spacing: int, r: int = 4):
+    """One-step-error samples along a fine run (the correction a coarse step needs).
+
+    At nsnap times ``spacing`` coarse steps apart, with C(k) the coarse-grained fine state
+    k coarse steps on: the coarse model takes three steps whose two input levels are
+    forced to (C(k-1), C(k)), k = 1, 2, 3 (an Euler start, then two LF-AM3 steps whose
+    right-hand-side history comes from the forced states); X is the result of the last.
+    Target = (C(4) - X) / dt for ubar and vbar (m/s^2): the extra momentum tendency that
+    would have made that coarse step land on the coarse-grained fine state.  Features:
+    3x3 stencils of C(3), as in make_samples."""
+    feats, targs = [], []
+    mask = sample_mask(coarse.shape)
+    dtc = coarse.ct.dtfast
+    for s in range(nsnap):
+        for _ in range((spacing - 4) * r):
+            fine.step()
+        C = []
+        for q in range(5):
+            if q:
+                for _ in range(r):
+                    fine.step()
+            C.append([x.clone() for x in coarse_grain(*fine.state(), r)])
+        coarse.set_state(*C[0])
+        for k in (1, 2, 3):
+            coarse.begin()
+            coarse.force(C[k - 1], C[k])
+            if k == 3:
+                F = coarse.features()
+            coarse.advance()
+        _z, uX, vX = coarse.state()
+        Ru = ((C[4][1] - uX) / dtc).reshape(-1)
+        Rv = ((C[4][2] - vX) / dtc).reshape(-1)
+        feats.append(F[mask])
+        targs.append(torch.stack([Ru[mask], Rv[mask]], dim=1).float())
+    return torch.cat(feats), torch.cat(targs)
+
+
 def linear_r2(F: torch.Tensor, Y: torch.Tensor, n: int = 200_000) -> list:
     """Fraction of the target variance a least-squares linear map of the features
     explains (diagnostic of how predictable the targets are from the features)."""
@@ -473,7 +510,7 @@ def run_experiment(coarse_shape=(512, 256), ratio: int = 4, nsteps: int = 5000, 
                    snaps_per_run: int = 4, snap_spacing: int = 250, nsamples: int = 2_000_000, probe_dt: float = 0.01,
                    epochs: int = 40, batch: int = 4096, lr: float = 2e-3, eval_every: int = 500, seed: int = 6,
                    n_val: int | None = None, gains=(1.0, 0.5, 0.25, 0.1), lr_final: float | None = 1e-5,
-                   noise: float = 1.0,
+                   noise: float = 1.0, target: str = "probe",
                    dtype=torch.float64, log=None) -> dict:
     """Train the closure on fine runs with seeded initial bumps and winds, then run the
     basin1 configuration (centred bump, wind 0.1 N/m^2) fine, coarse plain and coarse
@@ -489,16 +526,22 @@ def run_experiment(coarse_shape=(512, 256), ratio: int = 4, nsteps: int = 5000, 
         amp, bump = float(rng.uniform(0.05, 0.15)), float(rng.uniform(0.05, 0.15))
         fcfg = fine_config(*coarse_shape, ratio, amp=amp, bump=bump, center=center)
         fine = Model(fcfg, dtype)
-        fpcfg = dict(fcfg, dtfast=probe_dt)
-        fine_probe = Model(fpcfg, dtype)
-        pcfg = coarse_config(fcfg)
-        pcfg["dtfast"] = probe_dt
-        probe = Model(pcfg, dtype)
-        F, Y = make_samples(fine, fine_probe, probe, snaps_per_run, snap_spacing, ratio)
+        if target == "step":
+            coarse = Model(coarse_config(fcfg), dtype)
+            F, Y = make_samples_step(fine, coarse, snaps_per_run, snap_spacing, ratio)
+            del coarse
+        else:
+            fpcfg = dict(fcfg, dtfast=probe_dt)
+            fine_probe = Model(fpcfg, dtype)
+            pcfg = coarse_config(fcfg)
+            pcfg["dtfast"] = probe_dt
+            probe = Model(pcfg, dtype)
+            F, Y = make_samples(fine, fine_probe, probe, snaps_per_run, snap_spacing, ratio)
+            del fine_probe, probe
         Fs.append(F)
         Ys.append(Y)
         log(f"training run {run}: {F.shape[0]} samples")
-        del fine, fine_probe, probe
+        del fine
     F = torch.cat(Fs)
     Y = torch.cat(Ys)
     g = torch.Generator(device="cpu").manual_seed(seed)

@@ -11,13 +11,14 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_small_closure_experiment_runs_and_learns():
+@pytest.mark.parametrize("target", ["probe", "step"])
+def test_small_closure_experiment_runs_and_learns(target):
     import torch
     from sailfish_amd.closure import run_experiment
-    print("[case] closure experiment small", file=sys.stderr, flush=True)
+    print(f"[case] closure experiment small, {target} targets", file=sys.stderr, flush=True)
     res = run_experiment(coarse_shape=(66, 34), nsteps=120, n_train_runs=2, snaps_per_run=3, snap_spacing=12,
                          nsamples=10_000, epochs=8, batch=1024, lr=2e-3, eval_every=60, dtype=torch.float64,
-                         n_val=60, log=lambda *a: print(*a, file=sys.stderr))
+                         n_val=60, target=target, log=lambda *a: print(*a, file=sys.stderr))
     assert res["samples"] > 5000
     # training-set MSE of the normalised targets: training must bring it below the
     # untrained network's and below predicting the mean (MSE 1)
