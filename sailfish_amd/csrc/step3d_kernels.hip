@@ -348,9 +348,9 @@ __global__ __launch_bounds__(256) void k_verticalAdvection(G3 g, A U, A V, A Wa,
 // top ones; with more than one chunk (N >= 6) the two touch disjoint levels.
 //
 // The tile is double-buffered in LDS (one barrier per level).  WPE is the wave count
-// per SIMD the register allocation is held to: FP64 instances at 4 (128 VGPRs, two
-// 512-thread workgroups per CU; 138 VGPRs and 3 waves otherwise, 25 % slower on
-// 1024 x 512 x 32); FP32 at 4 (69 VGPRs) or 8 (see rhs3d_launch).
+// per SIMD the compiler targets: 4 for FP64 (at most 128 VGPRs, two 512-thread
+// workgroups per CU; 138 VGPRs and 3 waves otherwise, 25 % slower on 1024 x 512 x 32),
+// 4 or 8 for FP32 (see rhs3d_launch).
 template <int TJ, class T, int WPE>
 __global__ __launch_bounds__(HTI * TJ) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_rhs3d_fused(G3 g, HG<T> h, A fomn, AG<T> Hz, AG<T> Wa, const int32_t *BC, T *ru, T *rv, long long tail_end,
@@ -358,6 +358,15 @@ void k_rhs3d_fused(G3 g, HG<T> h, A fomn, AG<T> Hz, AG<T> Wa, const int32_t *BC,
 {
     constexpr int NT = HTI * TJ, LN = HLW * (TJ + 5);
     __shared__ T sl[2][4][LN];
+    // FP32: the face fluxes of the level are each evaluated once, into LDS -- x-faces
+    // c' = -1 .. HTI-1 of every row (UFxR, VFxP; column c'+1) and e-faces of rows
+    // r' = -1 .. TJ-1 (UFeP, VFeR; row r'+1) -- instead of once by each cell beside the
+    // face (50 instead of 64-69 VGPRs; 1024 x 512 x 32: 222 us against 273 us).  FP64
+    // evaluates them per cell: its occupancy is bound by LDS either way (two workgroups
+    // per CU), and the extra barrier made it slower (448 us against 443 us).
+    constexpr bool FACES = sizeof(T) == 4;
+    __shared__ T fxu[FACES ? TJ : 1][HTI + 1], fxv[FACES ? TJ : 1][HTI + 1];
+    __shared__ T feu[FACES ? TJ + 1 : 1][HTI], fev[FACES ? TJ + 1 : 1][HTI];
     const int tile = blockIdx.x % ntiles, kc = blockIdx.x / ntiles;
     const int ti = tile % tiles_i, tj = tile / tiles_i;
     const int i0 = ti * HTI, j0 = tj * TJ;
@@ -407,6 +416,30 @@ void k_rhs3d_fused(G3 g, HG<T> h, A fomn, AG<T> Hz, AG<T> Wa, const int32_t *BC,
             sb[3][e] = ok ? a3 : T(0);
         }
         __syncthreads();
+        if constexpr (FACES) {
+            // wave r = tile row r, lane c = column c.  Round 1: x-face c' = c-1 of row r;
+            // round 2: e-face of row r; round 3: the e-faces of row -1 (wave 0) and the
+            // last x-face c' = HTI-1 of every row (wave 1).  Each is the expression the
+            // cells on either side of the face evaluate (UFxR(0) of the cell on its left
+            // is UFxR(-1) of the cell on its right), at a tile position they read.
+            {
+                const long long p = (long long)(r + 3) * HLW + (c + 1);
+                fxu[r][c] = UFxR<false>(hl, p, 0);
+                fxv[r][c] = VFxP<false>(hl, p, 0);
+            }
+            feu[r + 1][c] = UFeP<false>(hl, lb, 0);
+            fev[r + 1][c] = VFeR<false>(hl, lb, 0);
+            if (r == 0) {
+                const long long p = 2LL * HLW + (c + 2);
+                feu[0][c] = UFeP<false>(hl, p, 0);
+                fev[0][c] = VFeR<false>(hl, p, 0);
+            } else if (r == 1 && c < TJ) {
+                const long long p = (long long)(c + 3) * HLW + (HTI + 1);
+                fxu[c][HTI] = UFxR<false>(hl, p, 0);
+                fxv[c][HTI] = VFxP<false>(hl, p, 0);
+            }
+            __syncthreads();
+        }
         if (act) {
             T xu = T(0), xv = T(0);
             if (rnode) {
@@ -419,10 +452,16 @@ void k_rhs3d_fused(G3 g, HG<T> h, A fomn, AG<T> Hz, AG<T> Wa, const int32_t *BC,
                 xu = xu + (UF0 + UF1) * T(0.5);
                 xv = xv - (VF0 + VF1) * T(0.5);
             }
-            const T du = (UFxR<false>(hl, lb, 0) - UFxR<false>(hl, lb, -1)) +
-                              (UFeP<false>(hl, lb, 0) - UFeP<false>(hl, lb, -HLW));
-            const T dv = (VFxP<false>(hl, lb, 0) - VFxP<false>(hl, lb, -1)) +
-                              (VFeR<false>(hl, lb, 0) - VFeR<false>(hl, lb, -HLW));
+            T du, dv;
+            if constexpr (FACES) {
+                du = (fxu[r][c + 1] - fxu[r][c]) + (feu[r + 1][c] - feu[r][c]);
+                dv = (fxv[r][c + 1] - fxv[r][c]) + (fev[r + 1][c] - fev[r][c]);
+            } else {
+                du = (UFxR<false>(hl, lb, 0) - UFxR<false>(hl, lb, -1)) +
+                     (UFeP<false>(hl, lb, 0) - UFeP<false>(hl, lb, -HLW));
+                dv = (VFxP<false>(hl, lb, 0) - VFxP<false>(hl, lb, -1)) +
+                     (VFeR<false>(hl, lb, 0) - VFeR<false>(hl, lb, -HLW));
+            }
             xu = xu - du;
             xv = xv - dv;
             if (K >= 1 && b < tail_end) xu = xv = T(0);
@@ -787,10 +826,11 @@ static int rhs3d_launch(sf_ctx *c, const double *fomn, const T *u, const T *v, c
     constexpr int TJ = 8;   // 512-thread workgroups, 64 x 8 columns
     const int tiles_i = (c->szI + HTI - 1) / HTI, tiles_j = (c->szJ + TJ - 1) / TJ, ntiles = tiles_i * tiles_j;
     const long long cus = sf::cu_count(c->device);
-    // FP32 instances need 69 VGPRs (7 waves per SIMD: three workgroups per CU).  Held to
-    // 64 VGPRs (8 waves, four workgroups, 11 VGPRs spilled) they are faster on grids of
-    // at least four tiles per CU (1024 x 512 x 32: 273 us against 312 us) and slower on
-    // smaller ones (512 x 256 x 32: 72 us against 64 us with level chunks).
+    // FP32: the instance compiled for 8 waves per SIMD is faster on grids of at least four
+    // tiles per CU (1024 x 512 x 32: 222-228 us against 324 us for the 4-wave instance,
+    // both 50 VGPRs and 37 KB LDS -- the wave target changes the compiler's schedule, not
+    // only its register limit); smaller grids run the 4-wave instance with level chunks
+    // (512 x 256 x 32: 65 us in three chunks, 73 us in four).
     const bool wide32 = sizeof(T) == 4 && ntiles >= 4 * cus;
     // level chunks: the fewest that give every resident workgroup slot (two per CU in
     // FP64, three in FP32) a workgroup, with at least 3 levels per chunk (the bottom and
