@@ -684,7 +684,7 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=498)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
               "(the barotropic path would shard as row slabs with a 3-row halo exchange per substep)",
               file=sys.stderr)
         sys.exit(2)

@@ -335,7 +335,7 @@ int sf_obc_volcons(const void *zeta, const void *h, const void *pm, const void *
 /* ---------------- learned subgrid momentum closure ---------------------------------- */
 
 /* The closure network (45 -> 64 GELU -> 64 GELU -> 2) trained and evaluated on matrix
- * cores (BF16 operands, FP32 accumulation).  No reference counterpart (the north-star
+ * cores (BF16 operands, FP32 accumulation).  No reference counterpart (the learned
  * closure is new); it enters the physics substep as an extra momentum stress through
  * the sustr/svstr fields (obsolete/step2d_LF_AM3.h.py:2595-2606 is where ROMS adds its
  * surface stress).  Parameter vector layout: W1[48][64], b1[64], W2[64][64], b2[64],

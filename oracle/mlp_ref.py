@@ -1,6 +1,6 @@
 """FP64 CPU restatement of the closure MLP (45 -> 64 -> 64 -> 2, GELU) with its
 forward pass, backward pass (MSE loss) and Adam step.  TEST INFRASTRUCTURE ONLY:
-it is the checker of the GPU kernels in sailfish_amd/csrc/mlp.hip and is pinned by
+it is the reference for the GPU kernels in sailfish_amd/csrc/mlp.hip and is pinned by
 central finite differences in tests/test_mlp_ref.py.
 
 Parameter vector layout (shared with the GPU kernels, 8336 FP32 values):
@@ -109,7 +109,7 @@ def bf16_round(x) -> np.ndarray:
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-# ---- the GPU kernels' rounding, emulated in FP64 (checker of the MFMA kernels) ------
+# ---- the GPU kernels' rounding, emulated in FP64 (reference for the MFMA kernels) ----
 def forward_bf16(theta, X):
     """Forward pass with the GPU's operand rounding: BF16 inputs, weights and hidden
     activations, exact products and sums, FP32-level biases."""

@@ -417,8 +417,8 @@ void k_rhs3d_fused(G3 g, HG<T> h, A fomn, AG<T> Hz, AG<T> Wa, const int32_t *BC,
         }
         __syncthreads();
         if constexpr (FACES) {
-            // wave r = tile row r, lane c = column c.  Round 1: x-face c' = c-1 of row r;
-            // round 2: e-face of row r; round 3: the e-faces of row -1 (wave 0) and the
+            // wave r = tile row r, lane c = column c.  Pass 1: x-face c' = c-1 of row r;
+            // pass 2: e-face of row r; pass 3: the e-faces of row -1 (wave 0) and the
             // last x-face c' = HTI-1 of every row (wave 1).  Each is the expression the
             // cells on either side of the face evaluate (UFxR(0) of the cell on its left
             // is UFxR(-1) of the cell on its right), at a tile position they read.
