@@ -347,26 +347,27 @@ __global__ __launch_bounds__(256) void k_verticalAdvection(G3 g, A U, A V, A Wa,
 // (levels 0, 1) does the bottom special levels, the last chunk (levels N-3 .. N) the
 // top ones; with more than one chunk (N >= 6) the two touch disjoint levels.
 //
-// The tile is double-buffered in LDS (one barrier per level).  WPE is the wave count
-// per SIMD the compiler targets: 4 for FP64 (at most 128 VGPRs, two 512-thread
-// workgroups per CU; 138 VGPRs and 3 waves otherwise, 25 % slower on 1024 x 512 x 32),
-// 4 or 8 for FP32 (see rhs3d_launch).
+// Each level: stage the tile, barrier, face fluxes (and, in FP64, the Coriolis terms),
+// barrier, cells.  FP32 double-buffers the tile; FP64 uses one buffer (see below).  WPE
+// is the wave count per SIMD the compiler targets: 6 for FP64 (at most 80 VGPRs), 4 or 8
+// for FP32 (see rhs3d_launch).
 template <int TJ, class T, int WPE>
 __global__ __launch_bounds__(HTI * TJ) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_rhs3d_fused(G3 g, HG<T> h, A fomn, AG<T> Hz, AG<T> Wa, const int32_t *BC, T *ru, T *rv, long long tail_end,
                    int cor, int adv, int tiles_i, int ntiles, int kchunk, int nchunk)
 {
     constexpr int NT = HTI * TJ, LN = HLW * (TJ + 5);
-    __shared__ T sl[2][4][LN];
-    // FP32: the face fluxes of the level are each evaluated once, into LDS -- x-faces
+    // The face fluxes of the level are each evaluated once, into LDS -- x-faces
     // c' = -1 .. HTI-1 of every row (UFxR, VFxP; column c'+1) and e-faces of rows
     // r' = -1 .. TJ-1 (UFeP, VFeR; row r'+1) -- instead of once by each cell beside the
-    // face (50 instead of 64-69 VGPRs; 1024 x 512 x 32: 222 us against 273 us).  FP64
-    // evaluates them per cell: its occupancy is bound by LDS either way (two workgroups
-    // per CU), and the extra barrier made it slower (448 us against 443 us).
-    constexpr bool FACES = sizeof(T) == 4;
-    __shared__ T fxu[FACES ? TJ : 1][HTI + 1], fxv[FACES ? TJ : 1][HTI + 1];
-    __shared__ T feu[FACES ? TJ + 1 : 1][HTI], fev[FACES ? TJ + 1 : 1][HTI];
+    // face.  FP32 (50 VGPRs instead of 64-69): 1024 x 512 x 32 222 us against 273 us.
+    // FP64 also forms the Coriolis terms before the second barrier, so no cell reads the
+    // tile after it and one tile buffer suffices: 46 KB LDS and 80 VGPRs, three
+    // workgroups per CU instead of two (1024 x 512 x 32: 436 us against 443 us per cell
+    // with two buffers; 512 x 256 x 32: 96 us against 104 us).
+    constexpr bool SB = sizeof(T) == 8;   // single tile buffer
+    __shared__ T sl[SB ? 1 : 2][4][LN];
+    __shared__ T fxu[TJ][HTI + 1], fxv[TJ][HTI + 1], feu[TJ + 1][HTI], fev[TJ + 1][HTI];
     const int tile = blockIdx.x % ntiles, kc = blockIdx.x / ntiles;
     const int ti = tile % tiles_i, tj = tile / tiles_i;
     const int i0 = ti * HTI, j0 = tj * TJ;
@@ -400,9 +401,11 @@ void k_rhs3d_fused(G3 g, HG<T> h, A fomn, AG<T> Hz, AG<T> Wa, const int32_t *BC,
     }
     for (long long K = k0; K < k1; K++) {
         const long long kb = K * S2, b = kb + f;
-        // level K writes one tile buffer while stragglers of level K-1 may still read
-        // the other; the barrier of level K orders the reuse two levels on
-        T(*sb)[LN] = sl[(K - k0) & 1];
+        // FP32: level K writes one tile buffer while stragglers of level K-1 may still read
+        // the other (their Coriolis terms); the barrier of level K orders the reuse two
+        // levels on.  FP64 reads the tile only before the level's second barrier, so one
+        // buffer serves every level.
+        T(*sb)[LN] = sl[SB ? 0 : (int)((K - k0) & 1)];
         const HLT<T> hl{ALT<T>{sb[0]}, ALT<T>{sb[1]}, ALT<T>{sb[2]}, ALT<T>{sb[3]}, HLW};
         for (int e = threadIdx.x; e < LN; e += NT) {
             const int rr = e / HLW, cc = e - rr * HLW;
@@ -416,33 +419,47 @@ void k_rhs3d_fused(G3 g, HG<T> h, A fomn, AG<T> Hz, AG<T> Wa, const int32_t *BC,
             sb[3][e] = ok ? a3 : T(0);
         }
         __syncthreads();
-        if constexpr (FACES) {
-            // wave r = tile row r, lane c = column c.  Pass 1: x-face c' = c-1 of row r;
-            // pass 2: e-face of row r; pass 3: the e-faces of row -1 (wave 0) and the
-            // last x-face c' = HTI-1 of every row (wave 1).  Each is the expression the
-            // cells on either side of the face evaluate (UFxR(0) of the cell on its left
-            // is UFxR(-1) of the cell on its right), at a tile position they read.
-            {
-                const long long p = (long long)(r + 3) * HLW + (c + 1);
-                fxu[r][c] = UFxR<false>(hl, p, 0);
-                fxv[r][c] = VFxP<false>(hl, p, 0);
+        T cu = T(0), cv = T(0);   // SB: the cell's Coriolis terms, formed before the tile is reused
+        if constexpr (SB) {
+            if (act && rnode) {
+                const T *su = sb[0], *sv = sb[1];
+                const T hzc = Hz.p[b], hzw = Hz.p[b - 1], hzs = Hz.p[b - W];
+                const T UF0 = (hzc * fo_c) * ((sv[lb] + sv[lb + HLW]) * T(0.5));
+                const T UF1 = (hzw * fo_w) * ((sv[lb - 1] + sv[lb - 1 + HLW]) * T(0.5));
+                const T VF0 = (hzc * fo_c) * ((su[lb] + su[lb + 1]) * T(0.5));
+                const T VF1 = (hzs * fo_s) * ((su[lb - HLW] + su[lb - HLW + 1]) * T(0.5));
+                cu = (UF0 + UF1) * T(0.5);
+                cv = (VF0 + VF1) * T(0.5);
             }
-            feu[r + 1][c] = UFeP<false>(hl, lb, 0);
-            fev[r + 1][c] = VFeR<false>(hl, lb, 0);
-            if (r == 0) {
-                const long long p = 2LL * HLW + (c + 2);
-                feu[0][c] = UFeP<false>(hl, p, 0);
-                fev[0][c] = VFeR<false>(hl, p, 0);
-            } else if (r == 1 && c < TJ) {
-                const long long p = (long long)(c + 3) * HLW + (HTI + 1);
-                fxu[c][HTI] = UFxR<false>(hl, p, 0);
-                fxv[c][HTI] = VFxP<false>(hl, p, 0);
-            }
-            __syncthreads();
         }
+        // wave r = tile row r, lane c = column c.  Pass 1: x-face c' = c-1 of row r;
+        // pass 2: e-face of row r; pass 3: the e-faces of row -1 (wave 0) and the
+        // last x-face c' = HTI-1 of every row (wave 1).  Each is the expression the
+        // cells on either side of the face evaluate (UFxR(0) of the cell on its left
+        // is UFxR(-1) of the cell on its right), at a tile position they read.
+        {
+            const long long p = (long long)(r + 3) * HLW + (c + 1);
+            fxu[r][c] = UFxR<false>(hl, p, 0);
+            fxv[r][c] = VFxP<false>(hl, p, 0);
+        }
+        feu[r + 1][c] = UFeP<false>(hl, lb, 0);
+        fev[r + 1][c] = VFeR<false>(hl, lb, 0);
+        if (r == 0) {
+            const long long p = 2LL * HLW + (c + 2);
+            feu[0][c] = UFeP<false>(hl, p, 0);
+            fev[0][c] = VFeR<false>(hl, p, 0);
+        } else if (r == 1 && c < TJ) {
+            const long long p = (long long)(c + 3) * HLW + (HTI + 1);
+            fxu[c][HTI] = UFxR<false>(hl, p, 0);
+            fxv[c][HTI] = VFxP<false>(hl, p, 0);
+        }
+        __syncthreads();
         if (act) {
             T xu = T(0), xv = T(0);
-            if (rnode) {
+            if (SB && rnode) {
+                xu = xu + cu;
+                xv = xv - cv;
+            } else if (rnode) {
                 const T *su = sb[0], *sv = sb[1];
                 const T hzc = Hz.p[b], hzw = Hz.p[b - 1], hzs = Hz.p[b - W];
                 const T UF0 = (hzc * fo_c) * ((sv[lb] + sv[lb + HLW]) * T(0.5));
@@ -452,16 +469,8 @@ void k_rhs3d_fused(G3 g, HG<T> h, A fomn, AG<T> Hz, AG<T> Wa, const int32_t *BC,
                 xu = xu + (UF0 + UF1) * T(0.5);
                 xv = xv - (VF0 + VF1) * T(0.5);
             }
-            T du, dv;
-            if constexpr (FACES) {
-                du = (fxu[r][c + 1] - fxu[r][c]) + (feu[r + 1][c] - feu[r][c]);
-                dv = (fxv[r][c + 1] - fxv[r][c]) + (fev[r + 1][c] - fev[r][c]);
-            } else {
-                du = (UFxR<false>(hl, lb, 0) - UFxR<false>(hl, lb, -1)) +
-                     (UFeP<false>(hl, lb, 0) - UFeP<false>(hl, lb, -HLW));
-                dv = (VFxP<false>(hl, lb, 0) - VFxP<false>(hl, lb, -1)) +
-                     (VFeR<false>(hl, lb, 0) - VFeR<false>(hl, lb, -HLW));
-            }
+            const T du = (fxu[r][c + 1] - fxu[r][c]) + (feu[r + 1][c] - feu[r][c]);
+            const T dv = (fxv[r][c + 1] - fxv[r][c]) + (fev[r + 1][c] - fev[r][c]);
             xu = xu - du;
             xv = xv - dv;
             if (K >= 1 && b < tail_end) xu = xv = T(0);
@@ -832,12 +841,11 @@ static int rhs3d_launch(sf_ctx *c, const double *fomn, const T *u, const T *v, c
     // only its register limit); smaller grids run the 4-wave instance with level chunks
     // (512 x 256 x 32: 65 us in three chunks, 73 us in four).
     const bool wide32 = sizeof(T) == 4 && ntiles >= 4 * cus;
-    // level chunks: the fewest that give every resident workgroup slot (two per CU in
-    // FP64, three in FP32) a workgroup, with at least 3 levels per chunk (the bottom and
-    // top special levels each stay within one chunk); the last chunk takes the remainder.
-    // FP64 1024 x 512 x 32 runs unsplit, 512 x 256 x 32 in two chunks (101 us; 120 us in
-    // four).
-    const long long slots = (sizeof(T) == 4 ? 3 : 2) * cus;
+    // level chunks: the fewest that give every resident workgroup slot (three per CU) a
+    // workgroup, with at least 3 levels per chunk (the bottom and top special levels each
+    // stay within one chunk); the last chunk takes the remainder.  1024 x 512 x 32 runs
+    // unsplit.
+    const long long slots = 3 * cus;
     int nchunk = 1;
     while (!wide32 && (long long)ntiles * nchunk < slots && c->szK / (nchunk + 1) >= 3) nchunk++;
     const int kchunk = c->szK / nchunk;
@@ -848,7 +856,8 @@ static int rhs3d_launch(sf_ctx *c, const double *fomn, const T *u, const T *v, c
                            A{fomn, c->S2}, hz, wa, BC, ru, rv, end, cor ? 1 : 0, adv ? 1 : 0, tiles_i, ntiles,
                            kchunk, nchunk);
     } else {
-        hipLaunchKernelGGL((k_rhs3d_fused<TJ, T, 4>), grid, block, 0, sf::as_stream(stream), grid3(c), h,
+        constexpr int WPE = sizeof(T) == 8 ? 6 : 4;   // FP64: at most 80 VGPRs, three workgroups per CU
+        hipLaunchKernelGGL((k_rhs3d_fused<TJ, T, WPE>), grid, block, 0, sf::as_stream(stream), grid3(c), h,
                            A{fomn, c->S2}, hz, wa, BC, ru, rv, end, cor ? 1 : 0, adv ? 1 : 0, tiles_i, ntiles,
                            kchunk, nchunk);
     }
