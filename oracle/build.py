@@ -21,16 +21,25 @@ CFLAGS = ["-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast
           "-fopenmp", "-Wall", "-Wno-unused-function"]
 
 
-def build(verbose: bool = True) -> Path:
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = ["gcc", *CFLAGS, *map(str, SOURCES), "-o", str(tmp), "-lm"]
+def build(verbose: bool = True, sources=None, out=None, opt=None, extra=()) -> Path:
+    """Compile ``sources`` (default: oracle/sf_oracle*.c) into ``out`` (default:
+    oracle/liboracle.so).  ``opt`` replaces the optimisation flag and ``extra`` adds
+    flags: the mutation test builds its variants at -O1 to save time, the CPU-baseline
+    leg of bench.py builds with -O3 -march=native on the machine it times.
+    -ffp-contract=off and IEEE semantics hold in every variant, so all of them compute
+    the same bits."""
+    sources = list(sources) if sources is not None else SOURCES
+    out = Path(out) if out is not None else LIB
+    flags = [opt if (f == "-O2" and opt) else f for f in CFLAGS] + list(extra)
+    tmp = out.with_suffix(".so.tmp")
+    cmd = ["gcc", *flags, *map(str, sources), "-o", str(tmp), "-lm"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{res.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print(f"[oracle.build] built {LIB}", file=sys.stderr)
-    return LIB
+        print(f"[oracle.build] built {out}", file=sys.stderr)
+    return out
 
 
 if __name__ == "__main__":

@@ -12,8 +12,10 @@ BC lists are applied entry by entry in list order (serial schedule of mod_BC.c).
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import math
+import os
 from pathlib import Path
 
 import numpy as np
@@ -31,36 +33,78 @@ class sfo_grid(C.Structure):
                 ("on_u", P), ("om_v", P), ("pn", P), ("pm", P), ("q5_stride", C.c_int)]
 
 
+_G = C.POINTER(sfo_grid)
+_SIGS2D = {
+    "sfo_computeZetaRHS": ([_G, P, P, P, P, P, P, P, C.c_int, F64, P], None),
+    "sfo_computeZetaPred": ([_G, F64, P, P, P, P], None),
+    "sfo_computeMomentumRHSPred": ([_G, P, P, P, P, P, F64, F64], None),
+    "sfo_computeMomentumRHSCorr": ([_G, P, P, P, P, P, P, F64], None),
+    "sfo_computeMomentumPred": ([_G, F64, P, P, P, P, P, P, P, P, P], None),
+    "sfo_AdamsMoultonCorr3rd": ([_G, F64, P, P, P, P], None),
+    "sfo_AdamsMoultonCorr3rd2": ([_G, F64, P, P, P, P, P, P, P, P, P, P, P], None),
+    "sfo_setBC": ([P, P, C.c_int, F64], None),
+    "sfo_copyBC": ([P, P, P, C.c_int], None),
+    "sfo_num_threads": ([], C.c_int),
+    "sfo_set_num_threads": ([C.c_int], None),
+}
+
+
 _lib = None
+_lib_path = None
+
+# ctypes signatures of every exported oracle function; the 3D and physics drivers add
+# theirs at import (declare()), and every load of a library binds all of them
+_SIGS: dict = {}
+
+
+def declare(sigs: dict):
+    """Register {name: (argtypes, restype)}; binds them on the loaded library too."""
+    _SIGS.update(sigs)
+    if _lib is not None:
+        _bind(_lib)
+
+
+def _bind(L):
+    for name, (at, rt) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.argtypes = at
+        fn.restype = rt
+
+
+def _load(path):
+    L = C.CDLL(str(path))
+    _bind(L)
+    return L
 
 
 def lib():
-    global _lib
+    """The loaded oracle library: oracle/liboracle.so (built on first use), or the file
+    named by the environment variable SF_ORACLE_LIB."""
+    global _lib, _lib_path
     if _lib is None:
-        if not LIB_PATH.exists():
+        env = os.environ.get("SF_ORACLE_LIB")
+        path = Path(env) if env else LIB_PATH
+        if not env and not path.exists():
             from . import build
             build.build(verbose=False)
-        L = C.CDLL(str(LIB_PATH))
-        G = C.POINTER(sfo_grid)
-        sig = {
-            "sfo_computeZetaRHS": [G, P, P, P, P, P, P, P, C.c_int, F64, P],
-            "sfo_computeZetaPred": [G, F64, P, P, P, P],
-            "sfo_computeMomentumRHSPred": [G, P, P, P, P, P, F64, F64],
-            "sfo_computeMomentumRHSCorr": [G, P, P, P, P, P, P, F64],
-            "sfo_computeMomentumPred": [G, F64, P, P, P, P, P, P, P, P, P],
-            "sfo_AdamsMoultonCorr3rd": [G, F64, P, P, P, P],
-            "sfo_AdamsMoultonCorr3rd2": [G, F64, P, P, P, P, P, P, P, P, P, P, P],
-            "sfo_setBC": [P, P, C.c_int, F64],
-            "sfo_copyBC": [P, P, P, C.c_int],
-            "sfo_num_threads": [],
-            "sfo_set_num_threads": [C.c_int],
-        }
-        for name, at in sig.items():
-            fn = getattr(L, name)
-            fn.argtypes = at
-            fn.restype = C.c_int if name == "sfo_num_threads" else None
-        _lib = L
+        _lib, _lib_path = _load(path), path
     return _lib
+
+
+@contextlib.contextmanager
+def use_library(path):
+    """Route every oracle call to another build of the oracle sources (a mutation
+    test builds deliberately wrong variants and checks that the pins reject them)."""
+    global _lib, _lib_path
+    saved = (_lib, _lib_path)
+    _lib, _lib_path = _load(path), Path(path)
+    try:
+        yield _lib
+    finally:
+        _lib, _lib_path = saved
+
+
+declare(_SIGS2D)
 
 
 def ptr(a: np.ndarray) -> int:

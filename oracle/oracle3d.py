@@ -21,32 +21,23 @@ class sfo_grid3(C.Structure):
                 ("on_u", P), ("om_v", P), ("pn", P), ("pm", P)]
 
 
-_bound = False
+_G3 = C.POINTER(sfo_grid3)
+O.declare({
+    "sfo_addCoriolis": ([_G3, P, P, P, P, P, P], None),
+    "sfo_horizontalAdvection": ([_G3, P, P, P, P, P, P, I32], None),
+    "sfo_verticalAdvection": ([_G3, P, P, P, P, P, P], None),
+    "sfo_set_maxflux": ([_G3, P, P, P, P, P], None),
+    "sfo_omega": ([_G3, P, P, P, P, P], None),
+    "sfo_set_depth": ([_G3, C.c_int, P, P, P, P, F64, P, P, P, P, P], None),
+    "sfo_set_zeta": ([_G3, P, P, P], None),
+    "sfo_setLateralUVBCs": ([_G3, F64, P, P, P, P, P, P], C.c_int),
+    "sfo_step3d_UV": ([_G3, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, C.c_int, C.c_int, F64, F64,
+                       P, P, P], None),
+})
 
 
 def lib():
-    global _bound
-    L = O.lib()
-    if not _bound:
-        G = C.POINTER(sfo_grid3)
-        sig = {
-            "sfo_addCoriolis": ([G, P, P, P, P, P, P], None),
-            "sfo_horizontalAdvection": ([G, P, P, P, P, P, P, I32], None),
-            "sfo_verticalAdvection": ([G, P, P, P, P, P, P], None),
-            "sfo_set_maxflux": ([G, P, P, P, P, P], None),
-            "sfo_omega": ([G, P, P, P, P, P], None),
-            "sfo_set_depth": ([G, C.c_int, P, P, P, P, F64, P, P, P, P, P], None),
-            "sfo_set_zeta": ([G, P, P, P], None),
-            "sfo_setLateralUVBCs": ([G, F64, P, P, P, P, P, P], C.c_int),
-            "sfo_step3d_UV": ([G, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, C.c_int, C.c_int, F64, F64,
-                               P, P, P], None),
-        }
-        for n, (at, rt) in sig.items():
-            fn = getattr(L, n)
-            fn.argtypes = at
-            fn.restype = rt
-        _bound = True
-    return L
+    return O.lib()
 
 
 ptr = O.ptr

@@ -17,17 +17,18 @@ class sfo_phys(C.Structure):
                 ("sustr", P), ("svstr", P), ("g", F64), ("rdrg", F64)]
 
 
-_bound = False
+_PP = C.POINTER(sfo_phys)
+O.declare({
+    "sfo_phys_substep": ([_PP, F64, C.c_int, F64] + [P] * 22, None),
+    "sfo_phys_rhs_zeta": ([_PP, P, P, P, C.c_int, C.c_int], F64),
+    "sfo_phys_rhs_u": ([_PP, P, P, P, P, C.c_int, C.c_int], F64),
+    "sfo_phys_rhs_v": ([_PP, P, P, P, P, C.c_int, C.c_int], F64),
+    "sfo_obc_volcons": ([C.c_int, C.c_int] + [P] * 6 + [C.c_int, C.c_int, P], None),
+})
 
 
 def lib():
-    global _bound
-    L = O.lib()
-    if not _bound:
-        L.sfo_phys_substep.argtypes = [C.POINTER(sfo_phys), F64, C.c_int, F64] + [P] * 22
-        L.sfo_phys_substep.restype = None
-        _bound = True
-    return L
+    return O.lib()
 
 
 class OPhys:
@@ -78,10 +79,6 @@ class OPhys:
 def obc_volcons(szI, szJ, z, h, pm, pn, u, v, flags, apply=True):
     """sfo_obc_volcons: returns (area, flux, ubar_xs); corrects u, v in place when apply."""
     L = lib()
-    if not getattr(L, "_volcons_bound", False):
-        L.sfo_obc_volcons.argtypes = [C.c_int, C.c_int] + [P] * 6 + [C.c_int, C.c_int, P]
-        L.sfo_obc_volcons.restype = None
-        L._volcons_bound = True
     out = np.zeros(3)
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (z, h, pm, pn)]
     for a in (u, v):

@@ -20,8 +20,20 @@
  *   - BC copy lists (mod_BC.c:16-23) race when a source is also a destination; the
  *     schedule fixed here is the serial one in list order (one legal outcome).
  *
- * Parity pins for every function: tests/test_oracle2d.py (closed forms, telescoping
- * budget identity, brute-force cross-check, golden fixtures in tests/golden/).
+ * Pins (tests/test_oracle2d.py; tests/test_oracle_mutations.py plants mistakes in
+ * this file and checks that one of them fails):
+ *   sfo_computeZetaRHS        test_zeta_rhs_linear_flux_closed_form,
+ *                             test_zeta_rhs_telescoping_budget,
+ *                             test_zeta_rhs_corrector_accumulates_averages
+ *   sfo_computeZetaPred       test_zeta_pred_closed_form_dt_and_slots
+ *   sfo_computeMomentumRHSPred test_momentum_rhs_pgf_closed_form,
+ *                             test_momentum_rhs_pred_weight_and_slots
+ *   sfo_computeMomentumRHSCorr test_momentum_rhs_corr_weight_and_slots
+ *   sfo_computeMomentumPred   test_momentum_pred_closed_form, test_momentum_pred_mask_patterns
+ *   sfo_AdamsMoultonCorr3rd   test_am3_constant_rhs_and_reference_coefficients
+ *   sfo_AdamsMoultonCorr3rd2  test_am3_depth_weighted_closed_form
+ *   sfo_setBC / sfo_copyBC    the hand-enumerated boundary sets of tests/test_host.py and
+ *                             test_barotropic_loop_averages_of_rest_state
  */
 #include <math.h>
 #include <stddef.h>

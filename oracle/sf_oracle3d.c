@@ -4,6 +4,22 @@
  *
  * TEST INFRASTRUCTURE ONLY (see sf_oracle2d.c).
  *
+ * Pins (tests/test_oracle3d.py; tests/test_oracle_mutations.py plants mistakes here):
+ *   sfo_addCoriolis          test_coriolis_uniform_flow
+ *   sfo_horizontalAdvection  test_horizontal_advection_linear_u,
+ *                            test_horizontal_advection_xi_curvature_closed_form,
+ *                            test_horizontal_advection_eta_curvature_upwind_closed_form,
+ *                            test_horizontal_advection_tail_zeroing
+ *   sfo_verticalAdvection    test_vertical_advection_linear_profile,
+ *                            test_vertical_advection_special_levels_closed_form
+ *   sfo_set_maxflux          test_set_maxflux_closed_form
+ *   sfo_omega                test_omega_divergence_free_is_zero_and_linear_flux
+ *   sfo_set_depth            test_set_depth_rest_state_closed_form,
+ *                            test_set_depth_vtransform1_and_free_surface
+ *   sfo_set_zeta             test_set_zeta_closed_form
+ *   sfo_setLateralUVBCs      test_lateral_uv_bcs_clamp_value
+ *   sfo_step3d_UV            test_step3d_uv_tridiagonal_against_dense_solve (AB3 factors)
+ *
  * 3D fields are flat arrays with level stride S2 = szI*szJ; a stencil offset
  * (dk, dj, di) from column f at level K reads index (K+dk)*S2 + f + dj*szI + di with
  * no wrapping rules beyond the flat arithmetic itself (the reference's Eval,

@@ -20,6 +20,20 @@
  *
  * TEST INFRASTRUCTURE ONLY (see sf_oracle2d.c).  Operand order below is the order
  * the GPU kernels use; built with -ffp-contract=off.
+ *
+ * Pins (tests/test_oracle_phys.py, tests/test_oracle_volcons.py;
+ * tests/test_oracle_mutations.py plants mistakes here):
+ *   sfo_phys_rhs_zeta   test_rhs_zeta_divergence_closed_form
+ *   sfo_phys_rhs_u/_v   test_rhs_pressure_gradient_closed_form, test_rhs_coriolis_closed_form,
+ *                       test_rhs_wind_and_linear_drag_closed_form, test_rhs_c2_advection_closed_form
+ *   sfo_phys_substep    test_leapfrog_predictor_of_a_divergent_flow,
+ *                       test_predictor_pressure_surface_weights,
+ *                       test_time_stepping_constant_wind_is_exact,
+ *                       test_time_stepping_linear_drag_decay, test_averages_of_uniform_flow,
+ *                       test_seiche_period, test_coriolis_acceleration_first_step,
+ *                       test_volume_conserved_closed_basin, test_mirror_symmetry_without_rotation
+ *   sfo_phys_bc         the closed-basin tests above (walls, ghosts)
+ *   sfo_obc_volcons     tests/test_oracle_volcons.py (closed forms)
  */
 #include <math.h>
 #include <stddef.h>

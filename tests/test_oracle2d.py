@@ -211,3 +211,200 @@ def test_buffers_skipped_by_fused_kernel_are_dead(fix_q5):
     assert np.array_equal(s1.v("ubar", 2), s2.v("ubar", 2))
     for n_ in ("Zt_avg1", "DU_avg1", "DV_avg1"):
         assert np.array_equal(getattr(s1, n_), getattr(s2, n_))
+
+
+# ------------------------------------------------------------------------------------
+# Closed-form pins of each kernel's arithmetic, slot wiring and metric factors.  Every
+# expected value below is derived by hand from the reference kernel text on fields
+# for which the stencils have exact closed forms (uniform or linear); dx != dy so that
+# a swapped metric (pm for pn, on_u for om_v) changes the result.
+# ------------------------------------------------------------------------------------
+
+def _call_zeta_rhs(g, zeta, ubar, vbar, avgs, is_pred, cff1, res):
+    O.lib().sfo_computeZetaRHS(g.cp, P(zeta), P(g.h), P(ubar), P(vbar), P(avgs[1]), P(avgs[2]), P(avgs[0]),
+                               int(is_pred), float(cff1), P(res))
+
+
+def test_zeta_rhs_corrector_accumulates_averages():
+    """Uniform zeta = Z, ubar = U, vbar = V, h = H (mod_cppkernels.cpp:116-125): the
+    corrector adds w*Z to Zt_avg1, w*U*(Z+H)*on_u = w*U*D*dy to DU_avg1 and w*V*D*dx
+    to DV_avg1 on interior points; the predictor leaves the averages alone; the
+    divergence of the uniform fluxes is 0."""
+    szI, szJ, dx, dy, H = 9, 7, 1e3, 1.5e3, 40.0
+    g = grid(szI, szJ, dx, dy, h=np.full((szJ, szI), H))
+    n = szI * szJ
+    Z, U, V, w = 0.3, 0.7, -0.4, 0.125
+    zeta, ubar, vbar = np.full(n, Z), np.full(n, U), np.full(n, V)
+    a0 = [np.full(n, 2.0), np.full(n, -3.0), np.full(n, 5.0)]      # Zt, DU, DV
+    avgs = [a.copy() for a in a0]
+    res = np.full(n, np.nan)
+    _call_zeta_rhs(g, zeta, ubar, vbar, avgs, True, w, res)
+    assert all(np.array_equal(a, b) for a, b in zip(avgs, a0))
+    _call_zeta_rhs(g, zeta, ubar, vbar, avgs, False, w, res)
+    inner = np.zeros((szJ, szI), bool)
+    inner[1:-1, 1:-1] = True
+    inner = inner.ravel()
+    D = Z + H
+    assert np.allclose(avgs[0][inner], 2.0 + w * Z, rtol=1e-15)
+    assert np.allclose(avgs[1][inner], -3.0 + w * U * D * dy, rtol=1e-14)
+    assert np.allclose(avgs[2][inner], 5.0 + w * V * D * dx, rtol=1e-14)
+    assert all(np.array_equal(a[~inner], b[~inner]) for a, b in zip(avgs, a0))
+    assert np.all(np.abs(res[inner]) <= 1e-12 * U * D * dy / (dx * dy))
+
+
+def test_zeta_pred_closed_form_dt_and_slots():
+    """computeZetaPred (cpp:133-150): zeta_t2 = zeta_t0 + Dt*rzeta_t1 (Dt, not 2*Dt, Q2)
+    on points with i != 0 and j != 0; zeta_t1 is not read; the rest is untouched."""
+    szI, szJ = 11, 6
+    g = grid(szI, szJ)
+    rng = np.random.default_rng(4)
+    n = szI * szJ
+    z0, z1 = rng.standard_normal(n), rng.standard_normal(n) + 5.0
+    r = np.full(n, 0.25)
+    z2 = np.full(n, np.nan)
+    Dt = 3.0
+    O.lib().sfo_computeZetaPred(g.cp, Dt, P(z0), P(z1), P(z2), P(r))
+    f = np.arange(n)
+    w = (f % szI != 0) & (f // szI != 0)
+    assert np.allclose(z2[w], z0[w] + Dt * 0.25, rtol=1e-15, atol=1e-15)
+    assert np.all(np.isnan(z2[~w]))
+
+
+def _linear_surface(szI, szJ, dx, dy, a, b):
+    I, J = np.meshgrid(np.arange(szI, dtype=float), np.arange(szJ, dtype=float))
+    return a * I * dx + b * J * dy
+
+
+def _pgf_expected(szI, szJ, dx, dy, H, a, b, gc=G):
+    """Pressure-gradient right-hand side of the reference (cpp:39-40, Q1 sign +) for a
+    linear surface gz = a*x + b*y and uniform h: at u point (j, i),
+    g*on_u*(H*DX(gz) + 0.5*DX(gz^2)) with DX(gz) = a*dx and DX(gz^2) = a*dx*(gz(i)+gz(i-1));
+    on_u = dy, om_v = dx away from row/column 0."""
+    gz = _linear_surface(szI, szJ, dx, dy, a, b)
+    ru = np.zeros((szJ, szI))
+    rv = np.zeros((szJ, szI))
+    ru[:, 1:] = gc * dy * (H * a * dx + 0.5 * a * dx * (gz[:, 1:] + gz[:, :-1]))
+    rv[1:, :] = gc * dx * (H * b * dy + 0.5 * b * dy * (gz[1:, :] + gz[:-1, :]))
+    return ru, rv
+
+
+def test_momentum_rhs_pred_weight_and_slots():
+    """computeMomentumRHSPred (cpp:12-42): gz = (1-w)*zeta_t2 + w*zeta_t0.  zeta_t2 and
+    zeta_t0 are different linear surfaces and w = 0.4, so the slot that takes each
+    weight is pinned; the xi and eta components follow the closed form above."""
+    szI, szJ, dx, dy, H, w = 12, 9, 1e3, 2e3, 30.0, 0.4
+    g = grid(szI, szJ, dx, dy, h=np.full((szJ, szI), H))
+    (a2, b2), (a0, b0) = (2e-5, -1e-5), (-1e-5, 4e-5)
+    z2 = np.ascontiguousarray(_linear_surface(szI, szJ, dx, dy, a2, b2).ravel())
+    z0 = np.ascontiguousarray(_linear_surface(szI, szJ, dx, dy, a0, b0).ravel())
+    ru, rv = np.full(szI * szJ, np.nan), np.full(szI * szJ, np.nan)
+    O.lib().sfo_computeMomentumRHSPred(g.cp, P(g.h), P(ru), P(rv), P(z0), P(z2), G, w)
+    ea, eb = (1 - w) * a2 + w * a0, (1 - w) * b2 + w * b0
+    wu, wv = _pgf_expected(szI, szJ, dx, dy, H, ea, eb)
+    sl = (slice(1, -1), slice(1, -1))
+    assert np.allclose(ru.reshape(szJ, szI)[sl], wu[sl], rtol=1e-11, atol=0)
+    assert np.allclose(rv.reshape(szJ, szI)[sl], wv[sl], rtol=1e-11, atol=0)
+
+
+def test_momentum_rhs_corr_weight_and_slots():
+    """computeMomentumRHSCorr (cpp:45-77): gz = (3/5)*zeta_t2 + (2/5)*zeta_t1; zeta_t0 is
+    not read (it holds a huge field here)."""
+    szI, szJ, dx, dy, H = 12, 9, 1e3, 2e3, 30.0
+    g = grid(szI, szJ, dx, dy, h=np.full((szJ, szI), H))
+    (a2, b2), (a1, b1) = (2e-5, -1e-5), (-1e-5, 5e-5)
+    z2 = np.ascontiguousarray(_linear_surface(szI, szJ, dx, dy, a2, b2).ravel())
+    z1 = np.ascontiguousarray(_linear_surface(szI, szJ, dx, dy, a1, b1).ravel())
+    z0 = np.full(szI * szJ, 1e6)
+    ru, rv = np.full(szI * szJ, np.nan), np.full(szI * szJ, np.nan)
+    O.lib().sfo_computeMomentumRHSCorr(g.cp, P(g.h), P(ru), P(rv), P(z0), P(z1), P(z2), G)
+    ea, eb = 0.6 * a2 + 0.4 * a1, 0.6 * b2 + 0.4 * b1
+    wu, wv = _pgf_expected(szI, szJ, dx, dy, H, ea, eb)
+    sl = (slice(1, -1), slice(1, -1))
+    assert np.allclose(ru.reshape(szJ, szI)[sl], wu[sl], rtol=1e-11, atol=0)
+    assert np.allclose(rv.reshape(szJ, szI)[sl], wv[sl], rtol=1e-11, atol=0)
+
+
+def test_momentum_pred_closed_form():
+    """computeMomentumPred (cpp:155-186) with uniform depths D1 = H + Z1 (zeta_t1) and
+    D2 = H + Z2 (zeta_t2): u_t2 = (u_t1*D1 + Dt*rhsu/(dx*dy))/D2, the same for v, on
+    the written points (column-0/row-0 mask)."""
+    szI, szJ, dx, dy, H = 10, 8, 1e3, 2.5e3, 20.0
+    g = grid(szI, szJ, dx, dy, h=np.full((szJ, szI), H), fix_q5=True)
+    rng = np.random.default_rng(5)
+    n = szI * szJ
+    Z1, Z2, Dt = 0.5, -0.25, 4.0
+    z1, z2 = np.full(n, Z1), np.full(n, Z2)
+    u1, v1 = rng.standard_normal(n), rng.standard_normal(n)
+    ru, rv = 1e3 * rng.standard_normal(n), 1e3 * rng.standard_normal(n)
+    u2, v2 = np.full(n, np.nan), np.full(n, np.nan)
+    O.lib().sfo_computeMomentumPred(g.cp, Dt, P(u1), P(u2), P(v1), P(v2), P(ru), P(rv), P(g.h), P(z1), P(z2))
+    f = np.arange(n)
+    w = (f % szI != 0) & (f // szI != 0)
+    D1, D2 = H + Z1, H + Z2
+    assert np.allclose(u2[w], (u1[w] * D1 + Dt * ru[w] / (dx * dy)) / D2, rtol=1e-13)
+    assert np.allclose(v2[w], (v1[w] * D1 + Dt * rv[w] / (dx * dy)) / D2, rtol=1e-13)
+    assert np.all(np.isnan(u2[~w]))
+
+
+def test_am3_depth_weighted_closed_form():
+    """AdamsMoultonCorr3rd2 (cpp:213-251), the intended corrector: with uniform depths
+    u = (u*D1 + Dt/(dx*dy)*(5/12 r2 + 8/12 r1 - 1/12 r0))/D2; distinct r slots pin
+    their weights (checked where every neighbour of the stencil is inside the array)."""
+    szI, szJ, dx, dy, H = 9, 7, 1e3, 3e3, 25.0
+    g = grid(szI, szJ, dx, dy, h=np.full((szJ, szI), H))
+    n = szI * szJ
+    Z1, Z2, Dt = 0.4, 0.1, 2.0
+    z1, z2 = np.full(n, Z1), np.full(n, Z2)
+    u, v = np.full(n, 0.3), np.full(n, -0.2)
+    ru = [np.full(n, x) for x in (1200.0, 2400.0, 3600.0)]
+    rv = [np.full(n, x) for x in (-600.0, 1800.0, 4800.0)]
+    O.lib().sfo_AdamsMoultonCorr3rd2(g.cp, Dt, P(u), P(v), P(ru[0]), P(ru[1]), P(ru[2]), P(rv[0]), P(rv[1]),
+                                     P(rv[2]), P(g.h), P(z1), P(z2))
+    D1, D2 = H + Z1, H + Z2
+    eu = (0.3 * D1 + Dt / (dx * dy) * (5 / 12 * 3600 + 8 / 12 * 2400 - 1 / 12 * 1200)) / D2
+    ev = (-0.2 * D1 + Dt / (dx * dy) * (5 / 12 * 4800 + 8 / 12 * 1800 + 1 / 12 * 600)) / D2
+    f = np.arange(n)
+    ok = f >= szI
+    assert np.allclose(u[ok], eu, rtol=1e-14) and np.allclose(v[ok], ev, rtol=1e-14)
+
+
+def test_barotropic_loop_averages_of_rest_state():
+    """main3d.py:166-189 on a basin at rest with uniform zeta = Z: every substep keeps
+    the state, so after the nfast+1 substeps Zt_avg1 = Z * sum(weight1[iif-1]) = Z
+    (weights sum to 1 over the loop) and DU_avg1 = DV_avg1 = 0."""
+    szI, szJ = 12, 9
+    g = grid(szI, szJ, fix_q5=True)
+    ct = CompTimes(240.0, 30)
+    set_weights(ct)
+    s = O.OState(g)
+    for k in range(3):
+        s.zeta[k][:] = 0.75
+    s.Zt_avg1[:] = 123.0
+    b = O.OBoundary(g)
+    t = O.OTimes(ct.dtfast, ct.weight1, ct.nfast)
+    O.barotropic_loop(t, g, s, b)
+    inner = np.zeros((szJ, szI), bool)
+    inner[1:-1, 1:-1] = True
+    inner = inner.ravel()
+    assert np.allclose(s.Zt_avg1[inner], 0.75 * np.sum(ct.weight1[:ct.nfast + 1]), rtol=1e-14)
+    assert abs(np.sum(ct.weight1[:ct.nfast + 1]) - 1.0) < 1e-14
+    assert np.all(s.DU_avg1 == 0) and np.all(s.DV_avg1 == 0)
+
+
+def test_set_weights_equal_reference_fixture():
+    """The host weights equal the reference's own set_weights output
+    (tests/golden/set_weights.json, written by tests/golden/make_set_weights_fixture.py
+    from nonlinear/set_weights.py) bit for bit, for six splitting ratios: this pins the
+    exact values, including the range(nfast-1, 1, -1) shift loops of :148-156."""
+    import json
+    from pathlib import Path
+    fx = json.loads((Path(__file__).resolve().parent / "golden" / "set_weights.json").read_text())
+    assert len(fx["cases"]) >= 6
+    for case in fx["cases"]:
+        ct = CompTimes(300.0, case["ndtfast"])
+        w1, w2, nf = set_weights(ct)
+        n = len(case["weight1"])
+        assert nf == case["nfast"], case["ndtfast"]
+        assert np.array_equal(np.asarray(w1[:n]), np.asarray(case["weight1"])), case["ndtfast"]
+        assert np.array_equal(np.asarray(w2[:n]), np.asarray(case["weight2"])), case["ndtfast"]
+        assert np.all(np.asarray(w1[n:]) == 0) and np.all(np.asarray(w2[n:]) == 0)

@@ -176,10 +176,12 @@ def test_vertical_advection_linear_profile():
     assert np.array_equal(rv[0], rv[1])
 
 
-def test_step3d_uv_tridiagonal_against_dense_solve():
+@pytest.mark.parametrize("iic,cff", [(0, 1.0), (1, 1.5), (5, 23.0 / 12.0)])
+def test_step3d_uv_tridiagonal_against_dense_solve(iic, cff):
     """One column of step3d_UV against numpy: assemble the reference's matrix
     (createVertViscousOpMatrix, :568-639) and solve densely; the Thomas sweep
-    (:520-565) plus the depth-mean correction (:475-517) must agree to 1e-12."""
+    (:520-565) plus the depth-mean correction (:475-517) must agree to 1e-12.  The
+    time-step factor is 1, 3/2, 23/12 at iic = ntfirst, ntfirst+1, later (:652-660)."""
     rng = np.random.default_rng(3)
     szI, szJ, N = 8, 7, 9
     g, M, m = model(szI, szJ, N, h=80 + 40 * rng.random((7, 8)))
@@ -196,10 +198,10 @@ def test_step3d_uv_tridiagonal_against_dense_solve():
     M.AKv[:] = 1e-2 * (1 + rng.random(n3))
     s.DU_avg1[:] = 50 * rng.standard_normal(S2)
     s.DV_avg1[:] = 50 * rng.standard_normal(S2)
-    dt, lam, cff = 300.0, 1.0, 23.0 / 12.0
+    dt, lam = 300.0, 1.0
     O3.lib().sfo_step3d_UV(M.cp, P(M.u[2]), P(M.v[2]), P(M.ru[2]), P(M.rv[2]), P(s.ubar[1]), P(s.vbar[1]),
                            P(s.ubar[2]), P(s.vbar[2]), P(M.Hz), P(M.AKv), P(M.z_r), P(s.DU_avg1), P(s.DV_avg1),
-                           P(tmpU), P(tmpV), 5, 0, lam, dt, P(M.eqSD), P(M.eqD), P(M.eqRHS))
+                           P(tmpU), P(tmpV), iic, 0, lam, dt, P(M.eqSD), P(M.eqD), P(M.eqRHS))
     f = 3 * szI + 4      # an interior U and V node
     Hz = M.Hz.reshape(N, S2)[:, [f, f - 1]]
     HzU = 0.5 * (Hz[:, 0] + Hz[:, 1])
@@ -227,3 +229,131 @@ def test_step3d_uv_tridiagonal_against_dense_solve():
     HuU = (HzU * got).sum()
     ub = s.DU_avg1[f] / g.f["on_u"][f] / Hsum + (HuU - s.DU_avg1[f] / g.f["on_u"][f]) / Hsum
     assert abs(s.ubar[2][f] - ub) <= 1e-12 * max(1, abs(ub)) and s.ubar[1][f] == s.ubar[2][f]
+
+
+# ------------------------------------------------------------------------------------
+# Closed forms of the curvature (upwind) terms, the special levels of the vertical
+# advection, and the lateral boundary values, derived by hand from the reference text
+# (nonlinear/rhs_kernels.cpp:70-197, modules/mod_cppkernels.h:608-702 and :915-960,
+# nonlinear/step3D_uv.kernels.cpp:294-368).
+# ------------------------------------------------------------------------------------
+
+def _hadv(szI, szJ, N, u2d, Hu, Hv, ru0=0.5, rv0=-0.25):
+    g, M, _ = model(szI=szI, szJ=szJ, N=N)
+    S2 = g.S2
+    M.u[2][:] = np.tile(np.asarray(u2d, float).ravel(), N + 1)
+    M.v[2][:] = 0.0
+    M.Huon[:] = Hu
+    M.Hvom[:] = Hv
+    ru, rv = np.full((N + 1) * S2, ru0), np.full((N + 1) * S2, rv0)
+    O3.lib().sfo_horizontalAdvection(M.cp, P(M.u[2]), P(M.v[2]), P(M.Huon), P(M.Hvom), P(ru), P(rv), S2)
+    return ru.reshape(N + 1, szJ, szI), rv.reshape(N + 1, szJ, szI)
+
+
+def test_horizontal_advection_xi_curvature_closed_form():
+    """u = a*i^4*(1 + e*j), v = 0, uniform Huon = Hu, Hvom = Hv.  uR > 0 selects
+    DXX(u)(i) = a*g*(12 i^2 + 2) (upwindUtoR, h:608-625), so with Gadv = -1/2:
+      DX(UFxR) = a*g*4*(i^3 + i) - 6*a*g*Hu*(2i - 1)
+      DE(UFeP) = DE(uP) = a*e*(i^4 + (i-1)^4)/2          (uP = UtoP(u), Q10)
+      DE(VFeR) = -(1/2)*Hv*a*e*(12 i^2 + 2)               (u curvature in VFeR, Q11)
+    ru = ru0 - DX(UFxR) - DE(UFeP), rv = rv0 - DE(VFeR) on interior levels and points."""
+    szI, szJ, N, a, e, Hu, Hv = 16, 12, 4, 1e-4, 0.1, 3.0, -2.0
+    I, J = np.meshgrid(np.arange(szI, dtype=float), np.arange(szJ, dtype=float))
+    gj = 1 + e * J
+    ru, rv = _hadv(szI, szJ, N, a * I ** 4 * gj, Hu, Hv)
+    want_u = 0.5 - (a * gj * 4 * (I ** 3 + I) - 6 * a * gj * Hu * (2 * I - 1) + a * e * (I ** 4 + (I - 1) ** 4) / 2)
+    want_v = -0.25 + 0.5 * Hv * a * e * (12 * I ** 2 + 2)
+    sl = (slice(0, N), slice(3, -3), slice(3, -3))
+    assert np.allclose(ru[sl], np.broadcast_to(want_u, ru.shape)[sl], rtol=1e-10, atol=1e-12)
+    assert np.allclose(rv[sl], np.broadcast_to(want_v, rv.shape)[sl], rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("Hv", [2.0, -2.0])
+def test_horizontal_advection_eta_curvature_upwind_closed_form(Hv):
+    """u = a*j^4*(1 + d*i), v = 0: DXX(u) = 0, DEE(u) = a*G(i)*(12 j^2 + 2), G = 1 + d*i.
+    upwindUtoP (h:660-677) takes DEE(u) at j when Hvom >= 0, at j-1 otherwise: s = j or
+    j - 1.  Then
+      DX(UFxR) = a*j^4*d
+      DE(UFeP) = a*(1 + d*(i - 1/2))*(4j^3 - 6j^2 + 4j - 1) - (1/2)*Hv*a*G(i)*12*(s^2 - (s-1)^2)
+    and VFxP = -(1/2)*uetaP*Hu (the u curvature in VFxP, Q11) gives
+      DX(VFxP) = -(1/2)*Hu*a*d*(12 s^2 + 2)."""
+    szI, szJ, N, a, d, Hu = 14, 16, 4, 1e-4, 0.1, 3.0
+    I, J = np.meshgrid(np.arange(szI, dtype=float), np.arange(szJ, dtype=float))
+    G = 1 + d * I
+    ru, rv = _hadv(szI, szJ, N, a * J ** 4 * G, Hu, Hv)
+    s = J if Hv >= 0 else J - 1
+    dcurv = 12 * (s ** 2 - (s - 1) ** 2)          # DE of 12 s^2
+    de_uf = a * (1 + d * (I - 0.5)) * (4 * J ** 3 - 6 * J ** 2 + 4 * J - 1) - 0.5 * Hv * a * G * dcurv
+    want_u = 0.5 - (a * J ** 4 * d + de_uf)
+    want_v = -0.25 + 0.5 * Hu * a * d * (12 * s ** 2 + 2)
+    sl = (slice(0, N), slice(3, -3), slice(3, -3))
+    assert np.allclose(ru[sl], np.broadcast_to(want_u, ru.shape)[sl], rtol=1e-10, atol=1e-12)
+    assert np.allclose(rv[sl], np.broadcast_to(want_v, rv.shape)[sl], rtol=1e-10, atol=1e-12)
+
+
+def test_vertical_advection_special_levels_closed_form():
+    """u = b*(K + c), v = bv*(K + cv), W uniform: F(K) = W*b*(K + 1/2 + c) for K >= 1.
+    The ad-hoc levels of rhs_kernels.cpp:169-194 then give
+      ru(1) = ru0 - W*b*(33 - c)/16      (F(2) - uuu, uuu = W*b*(17c + 7)/16)
+      ru(0) = ru0 - W*b*(9c + 8)/16      (uuu - uuu2, uuu2 = W*b*(8c - 1)/16)
+      rv(1) = rv0 - W*bv*(16cv + 31)/16, rv(0) = rv(1)
+    and levels N-2, N-1, N copy level N-3 (the N-2 update is overwritten)."""
+    g, M, _ = model(szI=16, szJ=12, N=10)
+    N, S2 = M.N, g.S2
+    b, c, bv, cv, Wc, ru0, rv0 = 0.02, 1.5, -0.03, 0.75, 0.5, 0.125, -0.375
+    lev = np.repeat(np.arange(N + 1, dtype=float), S2)
+    M.u[2][:] = b * (lev + c)
+    M.v[2][:] = bv * (lev + cv)
+    M.W[:] = Wc
+    ru, rv = np.full((N + 1) * S2, ru0), np.full((N + 1) * S2, rv0)
+    O3.lib().sfo_verticalAdvection(M.cp, P(M.u[2]), P(M.v[2]), P(M.W), P(ru), P(rv), P(M.BC))
+    ru = ru.reshape(N + 1, 12, 16)[:, 2:-2, 2:-2]
+    rv = rv.reshape(N + 1, 12, 16)[:, 2:-2, 2:-2]
+    assert np.allclose(ru[1], ru0 - Wc * b * (33 - c) / 16, rtol=1e-13)
+    assert np.allclose(ru[0], ru0 - Wc * b * (9 * c + 8) / 16, rtol=1e-13)
+    assert np.allclose(ru[2:N - 2], ru0 - Wc * b, rtol=1e-13)
+    assert np.allclose(rv[1], rv0 - Wc * bv * (16 * cv + 31) / 16, rtol=1e-13)
+    assert np.array_equal(rv[0], rv[1])
+    for K in (N - 2, N - 1, N):
+        assert np.array_equal(ru[K], ru[N - 3]) and np.array_equal(rv[K], rv[N - 3])
+
+
+def test_set_zeta_closed_form():
+    """set_zeta (:294-310): zeta_t1 = zeta_t2 = Zt_avg1 on rho nodes (i > 0, j > 0);
+    row 0 and column 0 keep their values."""
+    g, M, _ = model(szI=9, szJ=7, N=4)
+    S2 = g.S2
+    rng = np.random.default_rng(8)
+    Zt = rng.standard_normal(S2)
+    z1, z2 = np.full(S2, np.nan), np.full(S2, np.nan)
+    O3.lib().sfo_set_zeta(M.cp, P(z1), P(z2), P(Zt))
+    f = np.arange(S2)
+    r = (f % 9 > 0) & (f // 9 > 0)
+    assert np.array_equal(z1[r], Zt[r]) and np.array_equal(z2[r], Zt[r])
+    assert np.all(np.isnan(z1[~r])) and np.all(np.isnan(z2[~r]))
+
+
+def test_lateral_uv_bcs_clamp_value():
+    """setLateralUVBCs (:314-368): clamped u boundary points take 0.05*sin(2e-4*t) at
+    every level 0..N-1 (t = pi/(2*2e-4) gives 0.05, t = 1000 gives 0.05*sin(0.2));
+    closed v points take 0; other points are untouched."""
+    g, M, _ = model(szI=9, szJ=7, N=4)
+    N, S2 = M.N, g.S2
+    uType = np.zeros(S2, np.int32)
+    vType = np.zeros(S2, np.int32)
+    cu = np.array([9 * 2 + 1, 9 * 3 + 1, 9 * 4 + 1])
+    cv = np.array([9 * 1 + 3, 9 * 1 + 4])
+    uType[cu] = (1 << 3) + 1            # bcClamped (mod_boundary.py:28-57)
+    vType[cv] = 1 << 4                  # bcClosed
+    BC = -np.ones(S2, np.int32)
+    for t, val in ((np.pi / (2 * 2e-4), 0.05), (1000.0, 0.05 * np.sin(0.2))):
+        u = np.full((N + 1) * S2, 7.0)
+        v = np.full((N + 1) * S2, 7.0)
+        rc = O3.lib().sfo_setLateralUVBCs(M.cp, t, P(u), P(v), P(BC), P(BC), P(uType), P(vType))
+        assert rc == 0
+        u, v = u.reshape(N + 1, S2), v.reshape(N + 1, S2)
+        assert np.allclose(u[:N, cu], val, rtol=1e-15) and np.all(u[N, cu] == 7.0)
+        assert np.all(v[:N, cv] == 0.0)
+        mask = np.ones(S2, bool)
+        mask[cu] = False
+        assert np.all(u[:, mask] == 7.0)

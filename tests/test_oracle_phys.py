@@ -89,3 +89,246 @@ def test_coriolis_acceleration_first_step():
     u = o.st.v("ubar", 2).reshape(szJ, szI)[10:30, 10:30]
     assert np.allclose(v, -f0 * U * dt, rtol=1e-6)
     assert np.allclose(u, U, rtol=1e-12)
+
+
+# ------------------------------------------------------------------------------------
+# Closed forms of each term of the ROMS right-hand sides (obsolete/step2d_LF_AM3.h.py)
+# and of the time stepping, on fields for which the C-grid operators are exact.
+# dx != dy throughout, so a swapped metric factor changes every expected value.
+# ------------------------------------------------------------------------------------
+
+import ctypes as C  # noqa: E402
+
+from oracle import oracle as O  # noqa: E402
+
+SZI, SZJ, DX, DY, H0 = 14, 12, 1e3, 2e3, 30.0
+
+
+class _Fields:
+    """Static fields + a sfo_phys struct for direct calls of the rhs functions."""
+
+    def __init__(self, f0=0.0, sustr=0.0, svstr=0.0, rdrg=0.0, g=10.0):
+        S2 = SZI * SZJ
+        self.a = {"h": np.full(S2, H0), "pm": np.full(S2, 1 / DX), "pn": np.full(S2, 1 / DY),
+                  "fomn": np.full(S2, f0 * DX * DY), "sustr": np.full(S2, sustr), "svstr": np.full(S2, svstr)}
+        p = O.ptr
+        self.c = OP.sfo_phys(SZI, SZJ, p(self.a["h"]), p(self.a["pm"]), p(self.a["pn"]), p(self.a["fomn"]),
+                             p(self.a["sustr"]), p(self.a["svstr"]), g, rdrg)
+
+    def rhs(self, which, z, u, v, gz=None):
+        L = OP.lib()
+        p = O.ptr
+        out = np.full((SZJ, SZI), np.nan)
+        for j in range(2, SZJ - 2):
+            for i in range(2, SZI - 2):
+                if which == "zeta":
+                    out[j, i] = L.sfo_phys_rhs_zeta(C.byref(self.c), p(z), p(u), p(v), j, i)
+                elif which == "u":
+                    out[j, i] = L.sfo_phys_rhs_u(C.byref(self.c), p(z), p(u), p(v), p(gz), j, i)
+                else:
+                    out[j, i] = L.sfo_phys_rhs_v(C.byref(self.c), p(z), p(u), p(v), p(gz), j, i)
+        return out[2:-2, 2:-2]
+
+
+def _ij():
+    I, J = np.meshgrid(np.arange(SZI, dtype=float), np.arange(SZJ, dtype=float))
+    return I, J
+
+
+def _flat(a):
+    return np.ascontiguousarray(np.asarray(a, float).ravel())
+
+
+def test_rhs_zeta_divergence_closed_form():
+    """Continuity (:917-1014): rzeta = -(div of D*u*on_u, D*v*om_v), in m^3/s.  u = a*i,
+    v = b*j, flat surface: rzeta = -H*(a*dy + b*dx)."""
+    F = _Fields()
+    I, J = _ij()
+    a, b = 0.02, -0.03
+    z0 = np.zeros(SZI * SZJ)
+    r = F.rhs("zeta", z0, _flat(a * I), _flat(b * J))
+    assert np.allclose(r, -H0 * (a * DY + b * DX), rtol=1e-12)
+
+
+def test_rhs_pressure_gradient_closed_form():
+    """PGF (:1107-1237): at u point i, 0.5*g*on_u*((h+h)*(gz(i-1)-gz(i)) + gz(i-1)^2 -
+    gz(i)^2) = -g*dy*dx*a*(H + gz_u) for gz = a*x + b*y, gz_u its value at the u point
+    (ROMS sign: the surface slope decelerates the flow); likewise for v."""
+    F = _Fields()
+    I, J = _ij()
+    a, b = 3e-6, -2e-6
+    gz = a * I * DX + b * J * DY
+    z0, u0 = np.zeros(SZI * SZJ), np.zeros(SZI * SZJ)
+    ru = F.rhs("u", z0, u0, u0, _flat(gz))
+    rv = F.rhs("v", z0, u0, u0, _flat(gz))
+    gzu = (a * (I - 0.5) * DX + b * J * DY)[2:-2, 2:-2]
+    gzv = (a * I * DX + b * (J - 0.5) * DY)[2:-2, 2:-2]
+    assert np.allclose(ru, -10.0 * DY * DX * a * (H0 + gzu), rtol=1e-10)
+    assert np.allclose(rv, -10.0 * DX * DY * b * (H0 + gzv), rtol=1e-10)
+
+
+def test_rhs_coriolis_closed_form():
+    """Coriolis (:1503-1530) on an f-plane with uniform flow: the u equation gains
+    +f*D*V*dx*dy, the v equation -f*D*U*dx*dy."""
+    f0, U, V = 1e-4, 0.2, -0.3
+    F = _Fields(f0=f0)
+    S2 = SZI * SZJ
+    z0 = np.zeros(S2)
+    ru = F.rhs("u", z0, np.zeros(S2), np.full(S2, V), z0)
+    rv = F.rhs("v", z0, np.full(S2, U), np.zeros(S2), z0)
+    assert np.allclose(ru, f0 * H0 * V * DX * DY, rtol=1e-13)
+    assert np.allclose(rv, -f0 * H0 * U * DX * DY, rtol=1e-13)
+
+
+def test_rhs_wind_and_linear_drag_closed_form():
+    """Surface stress (:2595-2606) and linear bottom drag (:2241-2251,
+    nonlinear/set_vbc.py:8-13) with uniform flow: rhs_u = (sustr - rdrg*U)*om_u*on_u
+    = (sustr - rdrg*U)*dx*dy, rhs_v = (svstr - rdrg*V)*dx*dy."""
+    S, Sv, R, U, V = 1e-4, -2e-4, 3e-4, 0.5, 0.25
+    F = _Fields(sustr=S, svstr=Sv, rdrg=R)
+    S2 = SZI * SZJ
+    z0 = np.zeros(S2)
+    ru = F.rhs("u", z0, np.full(S2, U), np.full(S2, V), z0)
+    rv = F.rhs("v", z0, np.full(S2, U), np.full(S2, V), z0)
+    assert np.allclose(ru, (S - R * U) * DX * DY, rtol=1e-12)
+    assert np.allclose(rv, (Sv - R * V) * DX * DY, rtol=1e-12)
+
+
+def test_rhs_c2_advection_closed_form():
+    """C2 momentum advection (:1244-1290) at uniform depth is exact for these fields:
+    u = a*x_u alone gives -d(D u u)/dx * dx*dy = -2*H*a^2*x_u*dx*dy; v = b*y_v alone gives
+    -2*H*b^2*y_v*dx*dy; the shear u = c*y with uniform V gives -H*V*c*dx*dy and no v
+    tendency."""
+    F = _Fields()
+    I, J = _ij()
+    S2 = SZI * SZJ
+    z0 = np.zeros(S2)
+    a, b, c, V = 1e-3, -2e-3, 1e-6, 0.4
+    xu = (I - 1) * DX             # u point i sits at x = (i - 1)*dx
+    ru = F.rhs("u", z0, _flat(a * xu), np.zeros(S2), z0)
+    assert np.allclose(ru, (-2 * H0 * a * a * xu * DX * DY)[2:-2, 2:-2], rtol=1e-10)
+    yv = (J - 1) * DY
+    rv = F.rhs("v", z0, np.zeros(S2), _flat(b * yv), z0)
+    assert np.allclose(rv, (-2 * H0 * b * b * yv * DX * DY)[2:-2, 2:-2], rtol=1e-10)
+    ru = F.rhs("u", z0, _flat(c * J * DY), np.full(S2, V), z0)
+    rv = F.rhs("v", z0, _flat(c * J * DY), np.full(S2, V), z0)
+    assert np.allclose(ru, -H0 * V * c * DX * DY, rtol=1e-10)
+    # v tendency of the shear: VFx = 0.25*(DUon+DUon)*(V+V) is uniform along xi -> 0
+    assert np.all(np.abs(rv) <= 1e-12 * H0 * V * c * DX * DY * SZJ)
+
+
+def _flat_run(nsub, dt, cfg_extra, rdrg=0.0, w1=None):
+    szI, szJ = 96, 96
+    cfg = flat_basin(szI, szJ, DX, H0)
+    cfg["pn"] = np.full((szJ, szI), 1 / DY)
+    cfg["zeta0"] = np.zeros((szJ, szI))
+    cfg.update(cfg_extra)
+    o = oracle_for(cfg, rdrg=rdrg)
+    weights = np.zeros(257) if w1 is None else w1
+    o.run(nsub, dt, weights)
+    return o
+
+
+def test_time_stepping_constant_wind_is_exact():
+    """A uniform wind on a flat basin at rest: away from the walls (further than the
+    stencil reach of the substeps taken) the forcing is constant in time, and the
+    forward/backward-Euler start and the LF-AM3 steps integrate a constant tendency
+    exactly: u = sustr*t/D, v = svstr*t/D."""
+    S, Sv, dt, n = 1e-4, -5e-5, 20.0, 12
+    o = _flat_run(n, dt, {"sustr": np.full((96, 96), S), "svstr": np.full((96, 96), Sv)})
+    u = o.st.v("ubar", 2).reshape(96, 96)[40:56, 40:56]
+    v = o.st.v("vbar", 2).reshape(96, 96)[40:56, 40:56]
+    assert np.allclose(u, S * n * dt / H0, rtol=1e-12)
+    assert np.allclose(v, Sv * n * dt / H0, rtol=1e-12)
+
+
+def test_time_stepping_linear_drag_decay():
+    """Uniform flow on a flat basin under linear drag only decays as exp(-rdrg*t/D)
+    away from the walls.  At k*dt = rdrg*dt/D = 1e-3 the Euler start step errs by
+    (k*dt)^2/2 = 5e-7 relative; the LF-AM3 steps after it are 3rd order and add less
+    than 1e-9 over 40 substeps (a wrong AM3 weight adds ~1e-4 per step, a flipped drag
+    sign grows the flow)."""
+    R, dt, U, V = 3e-4 * 100, 1.0, 0.3, -0.2
+    err = {}
+    for n in (2, 40):
+        o = _flat_run(n, dt, {"ubar0": np.full((96, 96), U), "vbar0": np.full((96, 96), V)}, rdrg=R)
+        u = o.st.v("ubar", 2).reshape(96, 96)[44:52, 44:52]
+        v = o.st.v("vbar", 2).reshape(96, 96)[44:52, 44:52]
+        dec = np.exp(-R * n * dt / H0)
+        err[n] = (u / (U * dec) - 1, v / (V * dec) - 1)
+        for e in err[n]:
+            assert np.all(np.abs(e - 5.0e-7) < 1e-8), (n, e.max(), e.min())
+    assert np.all(np.abs(err[40][0] - err[2][0]) < 1e-9)
+
+
+def test_averages_of_uniform_flow():
+    """The barotropic averages accumulate weight1[iif-1] times the predicted state:
+    uniform flow (U, V) over a flat surface Z at depth H stays uniform away from the
+    walls, so Zt_avg1 = Z*sum(w), DU_avg1 = sum(w)*(H+Z)*U*on_u = sum(w)*(H+Z)*U*dy and
+    DV_avg1 = sum(w)*(H+Z)*V*dx there."""
+    U, V, Z, n = 0.2, -0.1, 0.5, 6
+    w1 = np.zeros(257)
+    w1[:n] = [0.1, 0.3, 0.05, 0.2, 0.15, 0.2]
+    o = _flat_run(n, 5.0, {"ubar0": np.full((96, 96), U), "vbar0": np.full((96, 96), V),
+                           "zeta0": np.full((96, 96), Z)}, w1=w1)
+    sw = w1.sum()
+    sl = (slice(40, 56), slice(40, 56))
+    assert np.allclose(o.st.Zt_avg1.reshape(96, 96)[sl], Z * sw, rtol=1e-13)
+    assert np.allclose(o.st.DU_avg1.reshape(96, 96)[sl], sw * (H0 + Z) * U * DY, rtol=1e-12)
+    assert np.allclose(o.st.DV_avg1.reshape(96, 96)[sl], sw * (H0 + Z) * V * DX, rtol=1e-12)
+
+
+def _one_substep(z0, z1, u0, u1, dt, w=0.0, f0=0.0):
+    """One LF-AM3 (non-first) substep from given levels n-1 (z0, u0) and n (z1, u1),
+    v = 0, on a flat basin of depth H0 with dx != dy; returns the oracle driver (its
+    scratch holds the predicted state zs, us)."""
+    szI, szJ = 48, 40
+    cfg = flat_basin(szI, szJ, DX, H0, f0)
+    cfg["pn"] = np.full((szJ, szI), 1 / DY)
+    cfg["zeta0"] = np.zeros((szJ, szI))
+    o = oracle_for(cfg)
+    o.st.cycleTimes2D()
+    for name, lev, val in (("zeta", 0, z0), ("zeta", 1, z1), ("ubar", 0, u0), ("ubar", 1, u1)):
+        o.st.v(name, lev)[:] = np.broadcast_to(val, (szJ, szI)).ravel()
+    o.substep(dt, False, w)
+    return o, szI, szJ
+
+
+def test_leapfrog_predictor_of_a_divergent_flow():
+    """Predictor continuity (:917-1014) is a leapfrog over 2*dt: with u = a*x (uniform
+    divergence a) on a flat surface Z1 at level n and Z0 at level n-1, the predicted
+    surface is zs = Z0 - 2*dt*a*(H + Z1), the stored right-hand side is
+    rzeta = -(H + Z1)*a*dx*dy, and the averages take w*zs (the predicted state, as the
+    live corrector accumulates it)."""
+    a, Z0, Z1, dt, w = 2e-5, 0.3, 0.5, 10.0, 0.25
+    I = np.arange(48, dtype=float)[None, :]
+    xu = (I - 1) * DX
+    o, szI, szJ = _one_substep(Z0, Z1, a * xu, a * xu, dt, w=w)
+    sl = (slice(8, -8), slice(8, -8))
+    zs = o.scr[0].reshape(szJ, szI)[sl]
+    rz = o.st.v("rzeta", 1).reshape(szJ, szI)[sl]
+    assert np.allclose(zs, Z0 - 2 * dt * a * (H0 + Z1), rtol=1e-13)
+    assert np.allclose(rz, -(H0 + Z1) * a * DX * DY, rtol=1e-12)
+    assert np.allclose(o.st.Zt_avg1.reshape(szJ, szI)[sl], w * (Z0 - 2 * dt * a * (H0 + Z1)), rtol=1e-13)
+
+
+def test_predictor_pressure_surface_weights():
+    """Predictor momentum (:2623-2780) at rest: the surface of the pressure gradient is
+    gz = (1 - 2*cff4)*zeta(n) + cff4*(zeta(n-1) + zs) with cff4 = 4/25.  At rest the
+    predicted surface is zs = zeta(n-1), so for linear surfaces of slopes a1 (level n)
+    and a0 (level n-1) gz has slope 17/25*a1 + 8/25*a0, and the predicted velocity is
+    us = 4*dt*r/(dx*dy*(D(zs, i) + D(zs, i-1))) with r = -g*dx*dy*slope*(H + gz_u)."""
+    a1, a0, dt = 2e-6, -3e-6, 10.0
+    I = np.arange(48, dtype=float)[None, :]
+    x = (I - 0.5) * DX
+    z1, z0 = a1 * x, a0 * x
+    o, szI, szJ = _one_substep(np.broadcast_to(z0, (40, 48)), np.broadcast_to(z1, (40, 48)), 0.0, 0.0, dt)
+    sl = (slice(8, -8), slice(8, -8))
+    slope = 17 / 25 * a1 + 8 / 25 * a0
+    xu = (I - 1) * DX
+    gz_u = slope * xu
+    r = -10.0 * DX * DY * slope * (H0 + gz_u)
+    zs_u2 = 2 * H0 + a0 * (x + (x - DX))             # D(zs, i) + D(zs, i-1), zs = z0
+    us = 4 * dt * r / (DX * DY * zs_u2)
+    got = o.scr[1].reshape(szJ, szI)[sl]
+    assert np.allclose(got, np.broadcast_to(us, (40, 48))[sl], rtol=1e-10)
