@@ -6,25 +6,29 @@ seeded Gaussian-filtered bathymetry mapped to 200..4000 m, beta plane
 f = 1e-4 + 2e-11 y, double-gyre wind tau_x = 0.1 cos(2 pi y / Ly), seeded 1 cm
 free-surface noise, linear drag 3e-4 m/s, dt = 5 s; the ROMS 2D LF-AM3 substep
 (predictor + corrector with C2 advection, Coriolis, stresses, drag) as ONE fused
-launch (sf_phys_substep_fused).  Default precision: FP32 state and stencil
-arithmetic with the time averages in FP64 (the production precision of this
-path); ``--dtype f64`` runs everything in FP64 (the form that matches the FP64
-oracle bit for bit).  The other precision is measured too (``secondary``).  A "step" is one barotropic substep
-over the whole grid; inputs are resident in HBM before timing starts.
+launch (sf_phys_substep_fused).  Default precision: FP64 throughout, the reference's
+precision (the form that matches the FP64 oracle bit for bit); ``--dtype f32`` runs
+FP32 state and stencil arithmetic with the time averages in FP64.  The other
+precision is measured too (``secondary``).  A "step" is one barotropic substep over
+the whole grid; inputs are resident in HBM before timing starts.
 
 Mode ``faithful``: the reference discretisation of nonlinear/step2d.py
 (sf_step2d_substep, fused tile kernel) on the same grid, with the column-0/row-0
 mask variant of computeMomentumPred.
 
-Prints one JSON line.  The timed region is one replay of a HIP graph that holds the
-K substep launches, each with its own arguments (slot rotation and averaging weight);
-``ms_per_step`` is its time per substep (HIP events on the replay stream) and the same
-K launches issued one by one are reported as ``config.ms_per_step_stream_launches``.
-``roofline`` is the dominant kernel: algorithmic bytes per launch over its mean launch
-time, which is the timed region's time per substep (the launches run back to back;
-``--no-graph`` times K stream launches and then 99 back-to-back launches instead).
-``cpu_baseline`` times the CPU oracle (C restatement, OpenMP) on a bounded number
-of substeps of the same workload.  ``secondary`` repeats the GPU measurement in the
+Prints one JSON line.  The timed region is a HIP graph that holds the K substep
+launches, each with its own arguments (slot rotation and averaging weight), one graph
+per starting rotation so that consecutive replays continue the rotation; after at
+least 150 ms of untimed replays, five replays are timed with HIP events on the replay
+stream and ``ms_per_step`` is the median replay over K.  The same K launches issued one
+by one are reported as ``config.ms_per_step_stream_launches``.  ``roofline`` is the
+dominant kernel: algorithmic bytes per launch (the 29 fields of the substep) over its
+mean launch time, which is the timed region's time per substep (the launches run back
+to back; ``--no-graph`` times K stream launches and then 99 back-to-back launches
+instead).
+``cpu_baseline`` times the CPU oracle (C restatement, built with -O3 -march=native
+on the machine that runs the benchmark) on a bounded number of substeps of the same
+workload, on all the threads the process may use and on one.  ``secondary`` repeats the GPU measurement in the
 other precision.  ``roofline.copy_GBs`` is the bandwidth of a plain 1 GiB device copy
 on the same GPU (the practical streaming ceiling); ``large_grid`` repeats the FP32
 kernel timing on a 4096 x 4096 basin whose working set is far beyond the 256 MB
@@ -54,11 +58,20 @@ METRIC = "cell-steps/s + % HBM roofline; closure-coarse vs fine run at equal zet
 #   writes: zeta/ubar/vbar at t2, rzeta/rubar/rvbar at t1            (6 bulk)
 #           Zt_avg1, DU_avg1, DV_avg1                                (3 FP64)
 PHYS_BULK, PHYS_F64 = 21, 6
+# The contract count of the same substep adds on_u and om_v (29 fields: 232 B FP64,
+# 140 B FP32 state + FP64 averages); this kernel derives those two metric factors from
+# pm and pn instead of reading them.  The roofline uses the contract count; the
+# kernel's own read set (216 B / 132 B) is reported beside it.
+PHYS_BULK_CONTRACT = 23
+# timed region: REPS replays of K substeps after at least WARM_MS of untimed replays
+REPS, WARM_MS = 5, 150.0
 # reference-discretisation substep (k_step2d_fused):
 #   reads : zeta_t1, zeta_t0, h, ubar_t1, vbar_t1, on_u, om_v, pm, pn,
 #           rzeta_t0, rzeta_t2 (11 bulk), Zt/DU/DV_avg1 (3 FP64)
-#   writes: zeta_t2, ubar_t2, vbar_t2, rzeta_t1, rubar_t1 (5 bulk), averages (3 FP64)
-FAITHFUL_BULK, FAITHFUL_F64 = 16, 6
+#   writes: zeta_t2, ubar_t2, vbar_t2, rzeta_t1 (4 bulk), averages (3 FP64)
+# (the contract count, 168 B FP64; the kernel also writes rubar_t1, an observable array
+# of the reference's predictor, which is not counted)
+FAITHFUL_BULK, FAITHFUL_F64 = 15, 6
 
 
 def _events(torch, n):
@@ -156,53 +169,83 @@ def run_phys_gpu(args, dtype_name, steps, warmup):
         kern_ms = a.elapsed_time(b) / nk
         finite = bool(torch.isfinite(OCEAN.zeta_t2).all()) and bool(torch.isfinite(OCEAN.ubar_t2).all())
         bulk = 8 if dtype_name == "f64" else 4
-        bpp = PHYS_BULK * bulk + PHYS_F64 * 8
+        bpp = PHYS_BULK_CONTRACT * bulk + PHYS_F64 * 8
         return dict(cfg=cfg, S2=S2, ms_per_step=ms_per_step, wall=wall, kern_ms=kern_ms, bpp=bpp,
+                    bpp_read=PHYS_BULK * bulk + PHYS_F64 * 8,
                     achieved=bpp * S2 / (kern_ms * 1e-3) / 1e9, finite=finite,
                     zmax=float(OCEAN.zeta_t2.abs().max()), nk=nk, kernel="k_phys_fused",
                     entry="sf_phys_substep_fused", tile=[64, tile_j], substeps=n[0] + nk)
 
-    # the timed region: K substeps captured once in a HIP graph, each node with its own
-    # arguments (slot rotation and averaging weight of that substep, exactly as the
-    # loop above issues them); one untimed replay, then one timed replay
-    graph = torch.cuda.CUDAGraph()
+    # the timed region: K substeps captured in HIP graphs, each node with its own
+    # arguments (slot rotation and averaging weight of that substep).  Three graphs,
+    # one per starting rotation, so that every replay continues the rotation where the
+    # previous one left it, whatever K is: graph r holds substeps r, r+1, ... r+K-1 of
+    # the rotation sequence.  Untimed replays warm the clocks (at least WARM_MS), then
+    # REPS timed replays of exactly K substeps each; ms_per_step is the median replay
+    # over K (HIP events on the replay stream).
+    graphs = []
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
+    w_seq = []
+    for _ in range(steps):
+        ct.next2DTimeStep()
+        w_seq.append(ct.cff1())
     with torch.cuda.stream(side):
-        with torch.cuda.graph(graph, stream=side):
-            shc = _lib.stream_handle()
-            for k in range(steps):
-                ct.next2DTimeStep()
-                rc |= fn(C.byref(blocks[(done + k) % 3]), prec, dt, 0, ct.cff1(), tile_j, shc)
+        for r in range(3):
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph, stream=side):
+                shc = _lib.stream_handle()
+                for k in range(steps):
+                    rc |= fn(C.byref(blocks[(r + k) % 3]), prec, dt, 0, w_seq[k], tile_j, shc)
+            graphs.append(graph)
     torch.cuda.current_stream().wait_stream(side)
     _lib.check(rc, "sf_phys_substep_fused (capture)")
-    graph.replay()
+    pos = [done]
+
+    def replay():
+        graphs[pos[0] % 3].replay()
+        pos[0] += steps
+
     torch.cuda.synchronize()
+    t_w = time.perf_counter()
+    nwarm = 0
+    while nwarm < 2 or time.perf_counter() - t_w < WARM_MS * 1e-3:
+        replay()
+        nwarm += 1
+        if nwarm % 4 == 0:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    rep_ms = []
     t0 = time.perf_counter()
-    ev0, ev1 = _events(torch, 1)[0]
-    ev0.record()
-    graph.replay()
-    ev1.record()
-    torch.cuda.synchronize()
+    for _ in range(REPS):
+        ev0, ev1 = _events(torch, 1)[0]
+        ev0.record()
+        replay()
+        ev1.record()
+        torch.cuda.synchronize()
+        rep_ms.append(ev0.elapsed_time(ev1))
     wall = time.perf_counter() - t0
-    ms_per_step = ev0.elapsed_time(ev1) / steps
-    done += 2 * steps
+    rep_ms.sort()
+    ms_per_step = rep_ms[len(rep_ms) // 2] / steps
+    done = pos[0]
     for _ in range(done % 3):
         OCEAN.cycleTimes2D()
     n[0] += done
-    del graph
+    del graphs
     # the graph replays the K launches back to back: the timed region's time per step is
     # the kernel's average launch duration (the kernel trace of the same replay agrees)
-    kern_ms, nk = ms_per_step, 0
+    kern_ms, nk = ms_per_step, steps
     finite = bool(torch.isfinite(OCEAN.zeta_t2).all()) and bool(torch.isfinite(OCEAN.ubar_t2).all())
     zmax = float(OCEAN.zeta_t2.abs().max())
     bulk = 8 if dtype_name == "f64" else 4
     bpp = PHYS_BULK * bulk + PHYS_F64 * 8
-    achieved = bpp * S2 / (kern_ms * 1e-3) / 1e9
+    bpp_contract = PHYS_BULK_CONTRACT * bulk + PHYS_F64 * 8
+    achieved = bpp_contract * S2 / (kern_ms * 1e-3) / 1e9
     return dict(cfg=cfg, S2=S2, ms_per_step=ms_per_step, ms_per_step_launched=ms_per_step_launched, wall=wall,
-                kern_ms=kern_ms, bpp=bpp,
-                achieved=achieved, finite=finite, zmax=zmax, nk=steps, kernel="k_phys_fused",
-                entry="sf_phys_substep_fused", tile=[64, tile_j], substeps=n[0])
+                kern_ms=kern_ms, bpp=bpp_contract, bpp_read=bpp,
+                achieved=achieved, finite=finite, zmax=zmax, nk=nk, kernel="k_phys_fused",
+                entry="sf_phys_substep_fused", tile=[64, tile_j], substeps=n[0],
+                replays_ms=rep_ms, warm_replays=nwarm)
 
 
 def run_large_grid_gpu(args):
@@ -240,7 +283,7 @@ def run_large_grid_gpu(args):
     _lib.check(rc, "sf_phys_substep_fused")
     ms = a.elapsed_time(b) / nk
     S2 = GRID.szI * GRID.szJ
-    bpp = PHYS_BULK * 4 + PHYS_F64 * 8
+    bpp = PHYS_BULK_CONTRACT * 4 + PHYS_F64 * 8
     return {"workload": f"basin2_analytic {n}x{n} (FP32 state, FP64 averages)", "points": S2, "kernel_ms": ms,
             "cell_steps_per_s": S2 / (ms * 1e-3), "achieved_GBs": bpp * S2 / (ms * 1e-3) / 1e9,
             "frac": bpp * S2 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -318,12 +361,72 @@ def run_faithful_gpu(args, dtype_name, steps, warmup):
 
 
 def cpu_threads():
-    return int(os.environ.get("SF_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    """Threads of the CPU baseline: the cores this process may run on, capped by
+    OMP_NUM_THREADS when the environment sets it (the CPU share of the machine)."""
+    env = int(os.environ.get("SF_CPU_THREADS", "0") or 0)
+    if env:
+        return env
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(n, omp) if omp > 0 else n
+
+
+_NATIVE = {}
+
+
+def native_oracle():
+    """The oracle's C restatement built for the machine that times it
+    (gcc -O3 -march=native -ffp-contract=off -fopenmp), once per process, into a
+    temporary directory; returns the use_library context of that build."""
+    import tempfile
+    from oracle import build, oracle as O
+    if "path" not in _NATIVE:
+        d = Path(tempfile.mkdtemp(prefix="sf_oracle_native_"))
+        _NATIVE["path"] = build.build(verbose=False, out=d / "liboracle_native.so", opt="-O3",
+                                      extra=["-march=native"])
+    return O.use_library(_NATIVE["path"])
+
+
+def _timed_runs(step, budget_s, runs=3):
+    """One untimed warm-up step, then ``runs`` runs of n steps each (n sized so that
+    the runs take about budget_s together); returns (median s per step, n)."""
+    t0 = time.perf_counter()
+    step()
+    t1 = max(time.perf_counter() - t0, 1e-6)
+    n = max(1, int(budget_s / runs / t1))
+    per = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        for _ in range(n):
+            step()
+        per.append((time.perf_counter() - t0) / n)
+    per.sort()
+    return per[len(per) // 2], n
+
+
+def _cpu_baseline(step_factory, units, unit, what, budget_s):
+    """Median-of-3 rate of a CPU step on all threads and on one thread."""
+    from oracle import oracle as O
+    threads = cpu_threads()
+    out = {}
+    with native_oracle():
+        for nt, share in ((threads, 0.4), (1, 0.6)):
+            O.set_threads(nt)
+            step = step_factory()
+            sec, n = _timed_runs(step, budget_s * share)
+            out[nt] = (units / sec, n)
+        O.set_threads(threads)
+    v_all, n_all = out[threads]
+    v_one, n_one = out[1]
+    return dict(value=v_all, unit=unit, cores=threads, kind="port",
+                value_1core=v_one, nproc=os.cpu_count(),
+                build="gcc -O3 -march=native -ffp-contract=off -fopenmp (built on this machine)",
+                sample=f"{what}: median of 3 runs of {n_all} steps on {threads} OpenMP threads and of {n_one} "
+                       f"steps on 1 thread, each after one warm-up step")
 
 
 def run_cpu_phys(args, budget_s):
     """Physics-mode oracle (oracle/sf_oracle_phys.c, FP64, OpenMP) on the same grid."""
-    from oracle import oracle as O
     from oracle import oracle_phys as OP
     from sailfish_amd import configs
     from sailfish_amd.mod_comptimes import CompTimes
@@ -331,25 +434,21 @@ def run_cpu_phys(args, budget_s):
     from sailfish_amd.set_weights import set_weights
 
     cfg = configs.basin2(args.szI, args.szJ, dx=args.dx)
-    threads = cpu_threads()
-    O.set_threads(threads)
     ct = CompTimes(cfg["dtfast"] * 30, 30)
     set_weights(ct)
     fomn = update_metrics(cfg["pm"], cfg["pn"], cfg["f"])["fomn"]
     o = OP.OPhys(cfg, fomn, rdrg=cfg["rdrg"])
-    iif = o.run(1, cfg["dtfast"], ct.weight1)            # first (Euler) substep, untimed
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        iif = o.run(1, cfg["dtfast"], ct.weight1, start_first=False, iif0=iif)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or n >= 100000:
-            break
+    st = {"iif": o.run(1, cfg["dtfast"], ct.weight1)}      # first (Euler) substep
+
+    def factory():
+        def step():
+            st["iif"] = o.run(1, cfg["dtfast"], ct.weight1, start_first=False, iif0=st["iif"])
+        return step
+
     S2 = cfg["szI"] * cfg["szJ"]
-    return dict(value=S2 * n / el, unit="cell-steps/s", cores=threads, kind="port",
-                sample=f"{n} physics substeps of the {cfg['szI']}x{cfg['szJ']} basin (oracle C restatement, "
-                       f"FP64, {threads} OpenMP threads, {el:.1f} s)")
+    return _cpu_baseline(factory, S2, "cell-steps/s",
+                         f"physics substeps of the {cfg['szI']}x{cfg['szJ']} basin (oracle C restatement, FP64)",
+                         budget_s)
 
 
 def run_cpu_faithful(args, budget_s):
@@ -361,8 +460,6 @@ def run_cpu_faithful(args, budget_s):
     from sailfish_amd.set_weights import set_weights
 
     cfg = configs.basin2(args.szI, args.szJ, dx=args.dx)
-    threads = cpu_threads()
-    O.set_threads(threads)
     ct = CompTimes(cfg["dtfast"] * 30, 30)
     set_weights(ct)
     m = update_metrics(cfg["pm"], cfg["pn"], cfg["f"])
@@ -373,19 +470,16 @@ def run_cpu_faithful(args, budget_s):
         s.zeta[k][:] = cfg["zeta0"].ravel()
     b = O.OBoundary(g)
     t = O.OTimes(cfg["dtfast"], ct.weight1, ct.nfast)
-    O.main2d(t, g, s, b, 1)
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        O.main2d(t, g, s, b, 1)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or n >= 100000:
-            break
+
+    def factory():
+        def step():
+            O.main2d(t, g, s, b, 1)
+        return step
+
     S2 = cfg["szI"] * cfg["szJ"]
-    return dict(value=S2 * n / el, unit="cell-steps/s", cores=threads, kind="port",
-                sample=f"{n} substeps of the {cfg['szI']}x{cfg['szJ']} basin (oracle C restatement of "
-                       f"the reference kernels, FP64, {threads} OpenMP threads, {el:.1f} s)")
+    return _cpu_baseline(factory, S2, "cell-steps/s",
+                         f"substeps of the {cfg['szI']}x{cfg['szJ']} basin (oracle C restatement of the reference "
+                         "kernels, FP64)", budget_s)
 
 
 def run_ensemble_gpu(args):
@@ -396,7 +490,9 @@ def run_ensemble_gpu(args):
     from sailfish_amd import autotune, configs
     from sailfish_amd.ensemble import LoopedMembers, Member, PhysEnsemble
 
-    dtype = torch.float64 if args.dtype == "f64" else torch.float32
+    # the ensemble workload is specified with FP32 state and FP64 accumulators
+    ens_dtype = "f32"
+    dtype = torch.float32
     cfgs = configs.ensemble_configs(args.ens_members)
     members = [Member(c, dtype=dtype) for c in cfgs]
     ens = PhysEnsemble(members, dtype)
@@ -446,8 +542,8 @@ def run_ensemble_gpu(args):
     b.record()
     torch.cuda.synchronize()
     ms_looped = a.elapsed_time(b) / nl
-    bulk = 8 if args.dtype == "f64" else 4
-    bpp = PHYS_BULK * bulk + PHYS_F64 * 8
+    bulk = 8 if ens_dtype == "f64" else 4
+    bpp = PHYS_BULK_CONTRACT * bulk + PHYS_F64 * 8
     return {
         "workload": f"{len(members)} basins (basin1 recipe, dx=5 km, dt=20 s), sizes drawn from "
                     "{256x128, 384x256, 512x256, 768x384, 1024x512} (seed 4), wind U[0.05,0.15] (seed 5)",
@@ -462,7 +558,7 @@ def run_ensemble_gpu(args):
         "looped_cell_steps_per_s": pts / (ms_looped * 1e-3),
         "speedup_vs_looped": ms_looped / ms_batched,
         "hbm_frac_batched": bpp * pts / (ms_batched * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "dtype": args.dtype,
+        "dtype": ens_dtype,
         "finite_after_run": finite,
     }
 
@@ -610,8 +706,6 @@ def run_cpu_3d(GRID, BOUNDARY, host, N, budget_s):
     from oracle import oracle as O
     from oracle import oracle3d as O3
 
-    threads = cpu_threads()
-    O.set_threads(threads)
     hh = GRID.host
     g2 = O.OGrid(GRID.szI, GRID.szJ, 1, dict(h=hh["h"], pm=hh["pm"], pn=hh["pn"], on_u=hh["on_u"],
                                            om_v=hh["om_v"]))
@@ -621,20 +715,19 @@ def run_cpu_3d(GRID, BOUNDARY, host, N, budget_s):
     m.u[2][:], m.v[2][:] = host["u"], host["v"]
     m.Huon[:], m.Hvom[:], m.W[:], m.z_w[:] = host["Huon"], host["Hvom"], host["W"], host["z_w"]
     m.Hz[:] = host["Hz"][:m.Hz.size]
-    L = O3.lib()
-    n, t0 = 0, time.perf_counter()
-    while True:
-        m.rhs3d()
-        L.sfo_set_maxflux(m.cp, O3.ptr(m.u[2]), O3.ptr(m.v[2]), O3.ptr(m.Huon), O3.ptr(m.Hvom), O3.ptr(m.Hz))
-        L.sfo_omega(m.cp, O3.ptr(m.W), O3.ptr(m.Huon), O3.ptr(m.Hvom), O3.ptr(m.z_w), O3.ptr(m.BC))
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or n >= 1000:
-            break
+
+    def factory():
+        def step():
+            L = O3.lib()
+            m.rhs3d()
+            L.sfo_set_maxflux(m.cp, O3.ptr(m.u[2]), O3.ptr(m.v[2]), O3.ptr(m.Huon), O3.ptr(m.Hvom), O3.ptr(m.Hz))
+            L.sfo_omega(m.cp, O3.ptr(m.W), O3.ptr(m.Huon), O3.ptr(m.Hvom), O3.ptr(m.z_w), O3.ptr(m.BC))
+        return step
+
     cells = GRID.szI * GRID.szJ * N
-    return dict(value=cells * n / el, unit="3D cell-steps/s", cores=threads, kind="port",
-                sample=f"{n} x (rhs3d + set_maxflux + omega) on {GRID.szI}x{GRID.szJ}x{N} (oracle C restatement, "
-                       f"FP64, {threads} OpenMP threads, {el:.1f} s)")
+    return _cpu_baseline(factory, cells, "3D cell-steps/s",
+                         f"rhs3d + set_maxflux + omega on {GRID.szI}x{GRID.szJ}x{N} (oracle C restatement, FP64)",
+                         budget_s)
 
 
 def calib_copy(args):
@@ -663,7 +756,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--mode", choices=["phys", "faithful"], default="phys")
-    ap.add_argument("--dtype", choices=["f64", "f32"], default="f32")
+    ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     ap.add_argument("--tile-j", type=int, default=0)
     ap.add_argument("--szI", type=int, default=2048)
     ap.add_argument("--szJ", type=int, default=1024)
@@ -700,6 +793,8 @@ def main():
     run = run_phys_gpu if args.mode == "phys" else run_faithful_gpu
     r = run(args, args.dtype, args.steps, args.warmup)
     progress(f"{args.mode} {args.dtype}")
+    if not r["finite"]:
+        raise SystemExit("bench.py: the headline state is not finite after the timed run")
     value = r["S2"] / (r["ms_per_step"] * 1e-3)
     if args.mode == "phys":
         workload = (f"ROMS 2D LF-AM3 barotropic substep (predictor+corrector, C2 advection, Coriolis, "
@@ -730,8 +825,10 @@ def main():
             "finite_after_run": r["finite"],
             "max_abs_zeta_after_run": r["zmax"],
             "substeps_run": r["substeps"],
-            "timed_as": ("one replay of a HIP graph holding the K substep launches, each with its own "
-                         "arguments" if r.get("ms_per_step_launched") else "K stream launches"),
+            "timed_as": (f"median of {REPS} replays of a HIP graph holding the K substep launches, each with "
+                         f"its own arguments, after >= {WARM_MS:g} ms of untimed replays"
+                         if r.get("ms_per_step_launched") else "K stream launches"),
+            "replay_ms": r.get("replays_ms"),
             "ms_per_step_stream_launches": r.get("ms_per_step_launched"),
         },
         "roofline": {
@@ -745,6 +842,9 @@ def main():
             "kernel": r["kernel"],
             "kernel_ms": r["kern_ms"],
             "bytes_per_point": r["bpp"],
+            "bytes_per_point_kernel_reads": r.get("bpp_read"),
+            "frac_at_kernel_reads": (r.get("bpp_read") or r["bpp"]) * r["S2"] / (r["kern_ms"] * 1e-3) / 1e9
+            / HBM_PEAK_GBS,
             "points_per_launch": r["S2"],
             "launches_timed": r["nk"],
         },
@@ -760,9 +860,11 @@ def main():
         progress(f"secondary {other}")
     if not args.no_secondary and args.mode == "phys":
         # the reference discretisation (faithful mode, FP64, bitwise equal to the oracle) on
-        # the same grid; 70 substeps keep it finite with the column-0/row-0 mask variant
-        fa = run_faithful_gpu(args, "f64", 60, 10)
-        out["faithful"] = summary(fa, "f64", 60, 10)
+        # the same grid, with the column-0/row-0 mask variant
+        # warm-up 5 + 40 timed + 40 single-launch substeps: the reference discretisation
+        # stays finite for ~120 substeps at dt = 5 s on this grid
+        fa = run_faithful_gpu(args, "f64", 40, 5)
+        out["faithful"] = summary(fa, "f64", 40, 5)
         out["faithful"]["workload"] = ("reference step2d discretisation (nonlinear/step2d.py), fused tile kernel, "
                                        f"{args.szI}x{args.szJ}, FP64")
         if not args.no_cpu:

@@ -201,7 +201,7 @@ int sf_step3d_uv_f32(sf_ctx *ctx, float *u, float *v, const float *ru, const flo
 /* ---------------- fused barotropic substep --------------------------------------- */
 
 /* Pointers of one substep, after the caller's time-level rotation (cycleTimes2D,
- * modules/mod_ocean.py:284-318): index k of each 3-slot array is time level t_k.
+ * modules/mod_ocean.py:120-155): index k of each 3-slot array is time level t_k.
  * Bulk fields have the precision passed to sf_step2d_substep (double or float);
  * the averages Zt/DU/DV_avg1 are always double.  Slots the substep never touches
  * may be null: ubar[0], vbar[0], rubar[2], rvbar[0], rvbar[2]. */

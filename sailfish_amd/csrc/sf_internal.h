@@ -18,10 +18,9 @@ struct sf_ctx {
     int device;
     int q5_stride;            // mask stride of computeMomentumPred (szJ, or szI when fixed)
     // dense boundary-code fields for the fused substep (one int32 per rho point),
-    // allocated by sf_ctx_set_bc_codes; see step2d_fused.hip
+    // allocated by sf_ctx_set_bc_maps; see step2d_fused.hip
     int32_t *bc_zeta, *bc_ubar, *bc_vbar;
     int bc_ready;
-    int bc_ubar_has_clamp, bc_vbar_has_clamp;
 };
 
 namespace sf {

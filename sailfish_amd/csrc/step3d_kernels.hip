@@ -4,7 +4,7 @@
 // setLateralUVBCs / step3d_UV (nonlinear/step3D_uv.kernels.cpp).
 //
 // Layout: 3D fields are flat with level stride S2 = szI*szJ (k slowest, xi
-// fastest, modules/mod_ocean.py:229-240).  One thread per water column (j,i), the
+// fastest, modules/mod_ocean.py:69-71).  One thread per water column (j,i), the
 // level loop inside the thread; consecutive threads take consecutive i, so every
 // level of every field is read with unit stride across a wavefront.  Stencil
 // offsets are flat (the reference's Eval arithmetic, mod_cppkernels.h:337-340);

@@ -2,7 +2,7 @@
 nonlinear/main3d.py:166-189).
 
 The reference's main2d never ends (its step counter ``iic`` is not advanced,
-main2d.py:147-151); here the caller gives the number of substeps.
+main2d.py:38-83, mod_comptimes.py:69-71); here the caller gives the number of substeps.
 """
 from __future__ import annotations
 

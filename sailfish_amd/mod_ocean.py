@@ -40,7 +40,7 @@ class T_OCEAN:
             self.W = z2((N + 1, M + 1, L + 1))
             self.Huon = z2((N + 1, M + 1, L + 1))
             self.Hvom = z2((N + 1, M + 1, L + 1))
-            self.AKv = 0.001 + z2((N + 1, M + 1, L + 1))   # mod_ocean.py:255
+            self.AKv = 0.001 + z2((N + 1, M + 1, L + 1))   # mod_ocean.py:91
             self.AK = 0.00001
             self._views3d()
 
@@ -64,12 +64,12 @@ class T_OCEAN:
         setattr(obj, f"{name}_t2", t0)
 
     def cycleTimes2D(self):
-        """mod_ocean.py:284-318"""
+        """mod_ocean.py:120-155"""
         for name in _ROT2D:
             self._rotate(self, name)
 
     def cycleTimes3D(self):
-        """mod_ocean.py:321-345"""
+        """mod_ocean.py:157-191"""
         for name in _ROT3D:
             self._rotate(self, name)
 
