@@ -179,6 +179,17 @@ int sf_set_zeta(sf_ctx *ctx, double *zeta_t1, double *zeta_t2, const double *Zt_
 int sf_set_lateral_uv_bcs(sf_ctx *ctx, double t, double *u, double *v, const int32_t *uType,
                           const int32_t *vType, void *stream);
 
+/* FP32 bulk forms of set_depth, set_zeta and setLateralUVBCs (same call sites,
+ * nonlinear/main3d.py:62-63,151,293,309): z_w, z_r, Hz, zeta and u, v in FP32; Zt_avg1,
+ * h and the s-coordinate curves in FP64.  Every depth is formed in FP64 as by
+ * sf_set_depth (Hz as the FP64 difference of the FP64 z_w) and rounded once. */
+int sf_set_depth_f32(sf_ctx *ctx, int32_t Vtransform, const double *Zt_avg1, float *z_w, float *z_r,
+                     const double *h, double hc, float *Hz, const double *sc_r, const double *sc_w,
+                     const double *Cs_r, const double *Cs_w, void *stream);
+int sf_set_zeta_f32(sf_ctx *ctx, float *zeta_t1, float *zeta_t2, const double *Zt_avg1, void *stream);
+int sf_set_lateral_uv_bcs_f32(sf_ctx *ctx, double t, float *u, float *v, const int32_t *uType,
+                              const int32_t *vType, void *stream);
+
 /* step3d_UV (nonlinear/step3D_uv.kernels.cpp:646-707) with its scratch arrays SD, D,
  * RHS ((szK+1)*S2 doubles each, nonlinear/main3d.py:48-58).  sf_step3d_uv_f32 (declared
  * below) is its FP32 bulk form for the same call site (main3d.py:304-306). */

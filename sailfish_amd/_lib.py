@@ -66,6 +66,9 @@ _SIGS = {
     "sf_set_depth": [P, I32, P, P, P, P, F64, P, P, P, P, P, P],
     "sf_set_zeta": [P, P, P, P, P],
     "sf_set_lateral_uv_bcs": [P, F64, P, P, P, P, P],
+    "sf_set_depth_f32": [P, I32, P, P, P, P, F64, P, P, P, P, P, P],
+    "sf_set_zeta_f32": [P, P, P, P, P],
+    "sf_set_lateral_uv_bcs_f32": [P, F64, P, P, P, P, P],
     "sf_step3d_uv": [P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I32, I32, F64, F64, P, P, P, P],
 }
 _RESTYPES = {"sf_last_error_string": C.c_char_p}

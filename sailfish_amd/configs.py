@@ -137,3 +137,16 @@ def ensemble_configs(n: int = 64, sizes=ENSEMBLE_SIZES, seed_size: int = 4, seed
     pick = rs.integers(0, len(sizes), size=n)
     amps = ra.uniform(0.05, 0.15, size=n)
     return [ensemble_member(sizes[p][0], sizes[p][1], float(a), 1000 + k) for k, (p, a) in enumerate(zip(pick, amps))]
+
+
+def basin3(szI: int = 1024, szJ: int = 512, N: int = 32, dx: float = 5e3) -> dict:
+    """The 3D mode-split workload: the basin1 recipe (depth, f-plane, wind, Gaussian
+    free-surface bump, fluid at rest) on 1024 x 512 rho points with N = 32 s-levels
+    (Vtransform 2, Vstretching 4, theta_s 5, theta_b 0.4, hc 50 m), 40 barotropic
+    substeps per baroclinic step of 300 s (dtfast 7.5 s).  The temperature profile
+    T = 4 + 16 exp(z/500) with S = 35 and a linear equation of state only matters with
+    a baroclinic pressure gradient (see the ``baroclinic`` module)."""
+    cfg = basin1(szI, szJ, dx=dx)
+    cfg.update(name=f"basin3_{szI}x{szJ}x{N}", N=N, ndtfast=40, dt=300.0, dtfast=7.5, nsteps=200,
+               theta_s=5.0, theta_b=0.4, hc=50.0)
+    return cfg

@@ -606,56 +606,62 @@ __global__ __launch_bounds__(256) void k_omega_reg(G3 g, T *W, AG<T> Hu, AG<T> H
 }
 
 // ---- set_depth / set_zeta (step3D_uv.kernels.cpp:225-310) -------------------------
-__global__ __launch_bounds__(256) void k_set_depth(G3 g, int Vtransform, const double *Zt, double *z_w,
-                                                   double *z_r, const double *h, double hc, double *Hz,
-                                                   const double *sc_r, const double *sc_w, const double *Cs_r,
-                                                   const double *Cs_w)
+// T: the depth arrays z_w, z_r, Hz (double: the reference's arithmetic; float: the FP32
+// bulk form, each value formed in FP64 as for double and rounded once).
+template <class T>
+__global__ __launch_bounds__(256) void k_set_depth(G3 g, int Vtransform, const double *Zt, T *z_w, T *z_r,
+                                                   const double *h, double hc, T *Hz, const double *sc_r,
+                                                   const double *sc_w, const double *Cs_r, const double *Cs_w)
 {
     COLUMN_INDEX
     const long long S2 = g.S2, N = g.N;
     const double zt = Zt[f], hh = h[f];
-    if (Vtransform == 1) {
-        for (long long K = 0; K < N; K++) {
+    double zlow = 0.0;     // z_w of level K-1 (FP64) for Hz
+    for (long long K = 0; K < N; K++) {
+        double zw, zr;
+        if (Vtransform == 1) {
             const double z_r0 = hc * (sc_r[K] - Cs_r[K]) + Cs_r[K] * hh;
             const double z_w0 = hc * (sc_w[K] - Cs_w[K]) + Cs_w[K] * hh;
-            z_w[K * S2 + f] = z_w0 + zt * (1.0 + z_w0 / hh);
-            z_r[K * S2 + f] = z_r0 + zt * (1.0 + z_r0 / hh);
-        }
-        z_w[N * S2 + f] = zt;
-    } else if (Vtransform == 2) {
-        z_w[N * S2 + f] = 0.0;
-        for (long long K = 0; K < N; K++) {
+            zw = z_w0 + zt * (1.0 + z_w0 / hh);
+            zr = z_r0 + zt * (1.0 + z_r0 / hh);
+        } else {
             const double cr = (hc * sc_r[K] + Cs_r[K] * hh) / (hc + hh);
             const double cw = (hc * sc_w[K] + Cs_w[K] * hh) / (hc + hh);
-            z_w[K * S2 + f] = zt + (zt + hh) * cw;
-            z_r[K * S2 + f] = zt + (zt + hh) * cr;
+            zw = zt + (zt + hh) * cw;
+            zr = zt + (zt + hh) * cr;
         }
-        z_w[N * S2 + f] = zt;
+        z_w[K * S2 + f] = T(zw);
+        z_r[K * S2 + f] = T(zr);
+        if (K > 0) Hz[(K - 1) * S2 + f] = T(zw - zlow);
+        zlow = zw;
     }
-    for (long long K = 0; K < N; K++) Hz[K * S2 + f] = z_w[(K + 1) * S2 + f] - z_w[K * S2 + f];
+    z_w[N * S2 + f] = T(zt);
+    Hz[(N - 1) * S2 + f] = T(zt - zlow);
 }
 
-__global__ __launch_bounds__(256) void k_set_zeta(G3 g, double *zeta1, double *zeta2, const double *Zt)
+template <class T>
+__global__ __launch_bounds__(256) void k_set_zeta(G3 g, T *zeta1, T *zeta2, const double *Zt)
 {
     COLUMN_INDEX
     if (!isRNode(g, f)) return;
-    zeta2[f] = Zt[f];
-    zeta1[f] = Zt[f];
+    zeta2[f] = T(Zt[f]);
+    zeta1[f] = T(Zt[f]);
 }
 
 // ---- setLateralUVBCs (step3D_uv.kernels.cpp:314-368) ------------------------------
-__global__ __launch_bounds__(256) void k_lateral_uv(G3 g, double t, double *u, double *v, const int32_t *uType,
+template <class T>
+__global__ __launch_bounds__(256) void k_lateral_uv(G3 g, double t, T *u, T *v, const int32_t *uType,
                                                     const int32_t *vType)
 {
     COLUMN_INDEX
     const int bu = uType[f], bv = vType[f];
     if (bu <= 0 && bv <= 0) return;
     if (bu > 0 && !(bu & (1 << 4)) && (bu & (1 << 3))) {
-        const double val = 0.05 * sin(t * 0.0002);
+        const T val = T(0.05 * sin(t * 0.0002));
         for (long long k = 0; k < g.N; k++) u[f + k * g.S2] = val;
     }
     if (bv > 0 && ((bv & (1 << 4)) || (bv & (1 << 3))))
-        for (long long k = 0; k < g.N; k++) v[f + k * g.S2] = 0.0;
+        for (long long k = 0; k < g.N; k++) v[f + k * g.S2] = T(0);
 }
 
 // ---- step3d_UV (step3D_uv.kernels.cpp:646-707) ------------------------------------
@@ -939,21 +945,45 @@ extern "C" int sf_set_depth(sf_ctx *c, int32_t Vtransform, const double *Zt_avg1
     CTX3(c);
     SF_REQUIRE(Zt_avg1 && z_w && z_r && h && Hz && sc_r && sc_w && Cs_r && Cs_w, "sf_set_depth: null field");
     SF_REQUIRE(Vtransform == 1 || Vtransform == 2, "sf_set_depth: Vtransform=%d", Vtransform);
-    LAUNCH_COLUMNS(k_set_depth, c, stream, grid3(c), Vtransform, Zt_avg1, z_w, z_r, h, hc, Hz, sc_r, sc_w, Cs_r,
-                   Cs_w);
+    LAUNCH_COLUMNS(k_set_depth<double>, c, stream, grid3(c), Vtransform, Zt_avg1, z_w, z_r, h, hc, Hz, sc_r, sc_w,
+                   Cs_r, Cs_w);
+}
+
+extern "C" int sf_set_depth_f32(sf_ctx *c, int32_t Vtransform, const double *Zt_avg1, float *z_w, float *z_r,
+                                const double *h, double hc, float *Hz, const double *sc_r, const double *sc_w,
+                                const double *Cs_r, const double *Cs_w, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(Zt_avg1 && z_w && z_r && h && Hz && sc_r && sc_w && Cs_r && Cs_w, "sf_set_depth_f32: null field");
+    SF_REQUIRE(Vtransform == 1 || Vtransform == 2, "sf_set_depth_f32: Vtransform=%d", Vtransform);
+    LAUNCH_COLUMNS(k_set_depth<float>, c, stream, grid3(c), Vtransform, Zt_avg1, z_w, z_r, h, hc, Hz, sc_r, sc_w,
+                   Cs_r, Cs_w);
 }
 
 extern "C" int sf_set_zeta(sf_ctx *c, double *zeta_t1, double *zeta_t2, const double *Zt_avg1, void *stream)
 {
     SF_REQUIRE(c && zeta_t1 && zeta_t2 && Zt_avg1, "sf_set_zeta: null argument");
-    LAUNCH_COLUMNS(k_set_zeta, c, stream, grid3(c), zeta_t1, zeta_t2, Zt_avg1);
+    LAUNCH_COLUMNS(k_set_zeta<double>, c, stream, grid3(c), zeta_t1, zeta_t2, Zt_avg1);
+}
+
+extern "C" int sf_set_zeta_f32(sf_ctx *c, float *zeta_t1, float *zeta_t2, const double *Zt_avg1, void *stream)
+{
+    SF_REQUIRE(c && zeta_t1 && zeta_t2 && Zt_avg1, "sf_set_zeta_f32: null argument");
+    LAUNCH_COLUMNS(k_set_zeta<float>, c, stream, grid3(c), zeta_t1, zeta_t2, Zt_avg1);
 }
 
 extern "C" int sf_set_lateral_uv_bcs(sf_ctx *c, double t, double *u, double *v, const int32_t *uType,
                                      const int32_t *vType, void *stream)
 {
     SF_REQUIRE(c && u && v && uType && vType, "sf_set_lateral_uv_bcs: null argument");
-    LAUNCH_COLUMNS(k_lateral_uv, c, stream, grid3(c), t, u, v, uType, vType);
+    LAUNCH_COLUMNS(k_lateral_uv<double>, c, stream, grid3(c), t, u, v, uType, vType);
+}
+
+extern "C" int sf_set_lateral_uv_bcs_f32(sf_ctx *c, double t, float *u, float *v, const int32_t *uType,
+                                         const int32_t *vType, void *stream)
+{
+    SF_REQUIRE(c && u && v && uType && vType, "sf_set_lateral_uv_bcs_f32: null argument");
+    LAUNCH_COLUMNS(k_lateral_uv<float>, c, stream, grid3(c), t, u, v, uType, vType);
 }
 
 extern "C" int sf_step3d_uv(sf_ctx *c, double *u, double *v, const double *ru, const double *rv, double *ubar_t1,

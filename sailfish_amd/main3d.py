@@ -1,9 +1,10 @@
-"""Baroclinic driver (nonlinear/main3d.py:25-330) without plotting or NetCDF output.
+"""Baroclinic driver (nonlinear/main3d.py:25-330) without plotting.
 
 Per baroclinic step: set_maxflux, omega, set_zeta, rhs3d, the nfast+1 barotropic
-substeps (main3d.py:166-189), set_depth, step3d_UV on copies tmpU/tmpV,
-setLateralUVBCs, omega, nextTimeStep.  ``fused=True`` runs each barotropic substep
-as one fused kernel (same state).
+substeps (main3d.py:166-189), the NetCDF history record when one is due
+(main3d.py:202-216; ``history`` is an ``io_netcdf.HistoryWriter``), set_depth,
+step3d_UV on copies tmpU/tmpV, setLateralUVBCs, omega, nextTimeStep.  ``fused=True``
+runs each barotropic substep as one fused kernel (same state).
 """
 from __future__ import annotations
 
@@ -15,10 +16,11 @@ from .rhs import rhs3d
 
 
 class Main3dState:
-    """The driver's scratch: tridiagonal SD/D/RHS and tmpU/tmpV (main3d.py:48-52)."""
+    """The driver's scratch: tridiagonal SD/D/RHS and tmpU/tmpV (main3d.py:48-52), in the
+    precision of the 3D fields (FP64, or FP32 for the FP32 bulk form)."""
 
     def __init__(self, GRID, OCEAN):
-        z = lambda shape: torch.zeros(shape, dtype=torch.float64, device=GRID.device)
+        z = lambda shape: torch.zeros(shape, dtype=OCEAN.u.dtype, device=GRID.device)
         self.eqSD = z(GRID.shape3DW)
         self.eqD = z(GRID.shape3DW)
         self.eqRHS = z(GRID.shape3DW)
@@ -33,7 +35,7 @@ def initial_depths(GRID, OCEAN):
                                GRID.Cs_w), ctx=ops.ctx_of(GRID))
 
 
-def baroclinic_step(compTimes, GRID, OCEAN, BOUNDARY, st: Main3dState, fused: bool = False):
+def baroclinic_step(compTimes, GRID, OCEAN, BOUNDARY, st: Main3dState, fused: bool = False, history=None):
     """One pass of the main3d.py:87-325 loop body."""
     c = ops.ctx_of(GRID)
     dm = BOUNDARY.device_maps()
@@ -45,6 +47,8 @@ def baroclinic_step(compTimes, GRID, OCEAN, BOUNDARY, st: Main3dState, fused: bo
     ops.set_zeta(None, None, (OCEAN.zeta_t1, OCEAN.zeta_t2, OCEAN.Zt_avg1), ctx=c)
     rhs3d(GRID, OCEAN, BOUNDARY)
     barotropic_loop(compTimes, GRID, OCEAN, BOUNDARY, fused=fused)
+    if history is not None:
+        history.maybe_write(compTimes, OCEAN)
     initial_depths(GRID, OCEAN)
     st.tmpU.copy_(OCEAN.u_t2)
     st.tmpV.copy_(OCEAN.v_t2)
@@ -59,9 +63,10 @@ def baroclinic_step(compTimes, GRID, OCEAN, BOUNDARY, st: Main3dState, fused: bo
     compTimes.nextTimeStep()
 
 
-def main3d(compTimes, GRID, OCEAN, BOUNDARY, nsteps: int | None = None, fused: bool = False):
+def main3d(compTimes, GRID, OCEAN, BOUNDARY, nsteps: int | None = None, fused: bool = False, history=None):
     """main3d.py:25-330: initial set_depth, then baroclinic steps until
-    isFinalTimeStep (or ``nsteps`` steps)."""
+    isFinalTimeStep (or ``nsteps`` steps).  A run resumed from a restart
+    (``io_netcdf.read_restart``) continues exactly where the written run was."""
     st = Main3dState(GRID, OCEAN)
     ops.setVerticalVelEq(None, None, (st.eqSD, st.eqD, st.eqRHS), ctx=ops.ctx_of(GRID))
     initial_depths(GRID, OCEAN)
@@ -69,6 +74,6 @@ def main3d(compTimes, GRID, OCEAN, BOUNDARY, nsteps: int | None = None, fused: b
     while not compTimes.isFinalTimeStep():
         if nsteps is not None and n >= nsteps:
             break
-        baroclinic_step(compTimes, GRID, OCEAN, BOUNDARY, st, fused=fused)
+        baroclinic_step(compTimes, GRID, OCEAN, BOUNDARY, st, fused=fused, history=history)
         n += 1
     return st

@@ -95,15 +95,18 @@ class Grid:
             setattr(self, k, torch.from_numpy(np.ascontiguousarray(v)).to(self.device))
         return self
 
-    def set_vertical(self, sc_r, sc_w, Cs_r, Cs_w, hc=None):
+    def set_vertical(self, sc_r, sc_w, Cs_r, Cs_w, hc=None, dtype=None):
+        """s-coordinate curves (FP64) and the depth arrays z_r, z_w, Hz in ``dtype``
+        (FP64 by default; FP32 for the FP32 bulk form)."""
         import torch
+        dtype = torch.float64 if dtype is None else dtype
         if hc is not None:
             self.hc = float(hc)
         for k, v in (("sc_r", sc_r), ("sc_w", sc_w), ("Cs_r", Cs_r), ("Cs_w", Cs_w)):
             v = np.asarray(v, dtype=np.float64)
             self.host[k] = v
             setattr(self, k, torch.from_numpy(v.copy()).to(self.device))
-        self.z_r = torch.zeros((self.N,) + self.shape2D, dtype=torch.float64, device=self.device)
-        self.z_w = torch.zeros((self.N + 1,) + self.shape2D, dtype=torch.float64, device=self.device)
-        self.Hz = torch.zeros(self.shape3D, dtype=torch.float64, device=self.device)
+        self.z_r = torch.zeros((self.N,) + self.shape2D, dtype=dtype, device=self.device)
+        self.z_w = torch.zeros((self.N + 1,) + self.shape2D, dtype=dtype, device=self.device)
+        self.Hz = torch.zeros(self.shape3D, dtype=dtype, device=self.device)
         return self

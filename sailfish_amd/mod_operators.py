@@ -256,18 +256,21 @@ def omega(grid, block, args, ctx=None):
 
 def set_depth(grid, block, args, ctx=None):
     """step3D_uv.kernels.cpp:225-291; args = (Vtransform, Zt_avg1, z_w, z_r, h, hc, Hz,
-    sc_r, sc_w, Cs_r, Cs_w)."""
+    sc_r, sc_w, Cs_r, Cs_w); z_w, z_r, Hz all FP64 or all FP32 (FP32 bulk form)."""
     Vt, Zt, z_w, z_r, h, hc, Hz, sc_r, sc_w, Cs_r, Cs_w = args
     c = ctx or ctx_of()
-    _lib.call("sf_set_depth", c.handle, int(Vt), _f64(Zt), _f64(z_w), _f64(z_r), _f64(h), float(hc), _f64(Hz),
+    name, (pw, pr, ph) = _bulk("sf_set_depth", (z_w, z_r, Hz))
+    _lib.call(name, c.handle, int(Vt), _f64(Zt), pw, pr, _f64(h), float(hc), ph,
               _f64(sc_r), _f64(sc_w), _f64(Cs_r), _f64(Cs_w), _stream())
 
 
 def set_zeta(grid, block, args, ctx=None):
-    """step3D_uv.kernels.cpp:294-310; args = (zeta_t1, zeta_t2, Zt_avg1)."""
+    """step3D_uv.kernels.cpp:294-310; args = (zeta_t1, zeta_t2, Zt_avg1); zeta FP64 or
+    FP32, Zt_avg1 FP64."""
     z1, z2, Zt = args
     c = ctx or ctx_of()
-    _lib.call("sf_set_zeta", c.handle, _f64(z1), _f64(z2), _f64(Zt), _stream())
+    name, p = _bulk("sf_set_zeta", (z1, z2))
+    _lib.call(name, c.handle, *p, _f64(Zt), _stream())
 
 
 def setLateralUVBCs(grid, block, args, ctx=None):
@@ -279,7 +282,8 @@ def setLateralUVBCs(grid, block, args, ctx=None):
     if grad_only.any():
         raise _lib.SailfishError("setLateralUVBCs: gradient v boundaries index the 2D index field at 3D "
                                  "offsets in the reference (step3D_uv.kernels.cpp:363); not supported")
-    _lib.call("sf_set_lateral_uv_bcs", c.handle, float(t), _f64(u), _f64(v), _ptr(ut), _ptr(vt), _stream())
+    name, p = _bulk("sf_set_lateral_uv_bcs", (u, v))
+    _lib.call(name, c.handle, float(t), *p, _ptr(ut), _ptr(vt), _stream())
 
 
 _velEq = {}

@@ -47,9 +47,14 @@ def init_state(OCEAN, cfg: dict):
 
 def build_model3d(cfg: dict, *, N: int, device="cuda", ndtfast: int = 30, dt: float | None = None,
                   lbc=None, fix_q5: bool = True, Vtransform: int = 2, Vstretching: int = 4,
-                  theta_s: float = 5.0, theta_b: float = 0.4, Tcline: float = 50.0):
+                  theta_s: float = 5.0, theta_b: float = 0.4, Tcline: float = 50.0,
+                  dtype=torch.float64):
     """A 3D model as tmp/testProgram.py:36-75 assembles it: grid + s-coordinates,
-    initial depths at zeta = 0, AKv = 2e-3 + 8e-3*exp(z_w/150)."""
+    initial depths at zeta = 0, AKv = 2e-3 + 8e-3*exp(z_w/150).
+
+    ``dtype`` is the precision of the bulk state (2D and 3D fields, the depths z_w, z_r,
+    Hz); torch.float32 gives the FP32 bulk form, whose barotropic averages Zt/DU/DV_avg1
+    and grid metrics stay FP64."""
     from .mod_grid import Grid
     from .set_coords import s_coordinates
     szI, szJ = int(cfg["szI"]), int(cfg["szJ"])
@@ -59,19 +64,23 @@ def build_model3d(cfg: dict, *, N: int, device="cuda", ndtfast: int = 30, dt: fl
     GRID.set_fields(h=cfg["h"], pm=cfg["pm"], pn=cfg["pn"], f=cfg["f"])
     sc_r, sc_w, Cs_r, Cs_w = s_coordinates(N, Vstretching, theta_s, theta_b)
     hc = min(GRID.hmin, Tcline) if Vtransform == 1 else Tcline
-    GRID.set_vertical(sc_r, sc_w, Cs_r, Cs_w, hc=hc)
+    GRID.set_vertical(sc_r, sc_w, Cs_r, Cs_w, hc=hc, dtype=dtype)
     dt = float(dt) if dt is not None else cfg["dtfast"] * ndtfast
     compTimes = CompTimes(dt=dt, ndtfast=ndtfast)
     set_weights(compTimes)
-    OCEAN = T_OCEAN(GRID, dtype=torch.float64, with3d=True)
+    OCEAN = T_OCEAN(GRID, dtype=dtype, with3d=True)
     init_state(OCEAN, cfg)
+    # main3d's set_zeta copies Zt_avg1 into zeta t1/t2 before the first barotropic loop
+    # (main3d.py:151) and the first set_depth reads it (main3d.py:62-63): the initial free
+    # surface starts there too
+    OCEAN.Zt_avg1.copy_(torch.from_numpy(np.asarray(cfg["zeta0"], dtype=np.float64).reshape(-1)).to(device))
     BOUNDARY = Boundary(GRID, lbc, device=device)
     mod_operators.initModule(GRID)
     z_w0 = rest_depths(GRID)
-    OCEAN.AKv.copy_(torch.from_numpy(2.0e-3 + 8.0e-3 * np.exp(z_w0 / 150.0)).to(device))
+    OCEAN.AKv.copy_(torch.from_numpy(2.0e-3 + 8.0e-3 * np.exp(z_w0 / 150.0)).to(device, dtype))
     for name in ("u0", "v0"):
         if name in cfg:
-            a = torch.from_numpy(np.asarray(cfg[name], dtype=np.float64)).to(device)
+            a = torch.from_numpy(np.asarray(cfg[name], dtype=np.float64)).to(device, dtype)
             getattr(OCEAN, name[0]).copy_(a.expand_as(getattr(OCEAN, name[0])))
     return compTimes, GRID, OCEAN, BOUNDARY
 

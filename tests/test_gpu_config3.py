@@ -1,0 +1,102 @@
+"""The 3D mode-split workload at its full size: 1024 x 512 rho points, 32 s-levels,
+40 barotropic substeps per baroclinic step of 300 s (configs.basin3), two baroclinic
+steps through the drop-in main3d driver.
+
+- FP64: every array the step writes is bitwise equal to the oracle's.
+- FP32 bulk form (FP32 2D and 3D fields and depths, FP64 barotropic averages and
+  metrics) against the FP64 run: the error of zeta, ubar, vbar, u, v and omega (W) after
+  the two steps, as max |FP32 - FP64| over max |FP64| of each field, is within the
+  bounds written below."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from sailfish_amd import configs
+from tests.helpers2d import bitwise_equal, compare_states, oracle_arrays, product_arrays
+from tests.test_gpu_step3d import _oracle3
+
+pytestmark = pytest.mark.gpu
+
+NSTEPS = 2
+
+# FP32 bulk form against FP64 after two baroclinic steps (82 barotropic substeps):
+# max |difference| / max |FP64 field|
+FP32_BOUNDS = {"zeta": 2e-4, "ubar": 2e-3, "vbar": 2e-3, "u": 2e-3, "v": 2e-3, "W": 5e-3}
+
+
+def _run(cfg, dtype):
+    from sailfish_amd.main3d import main3d
+    from sailfish_amd.model2d import build_model3d
+    ct, GRID, OCEAN, BOUNDARY = build_model3d(cfg, N=cfg["N"], ndtfast=cfg["ndtfast"], dt=cfg["dt"],
+                                              fix_q5=True, dtype=dtype)
+    main3d(ct, GRID, OCEAN, BOUNDARY, nsteps=NSTEPS, fused=True)
+    return ct, GRID, OCEAN, BOUNDARY
+
+
+def _fields(OCEAN, S2, N):
+    get = lambda t: t.detach().cpu().numpy().astype(np.float64)
+    out = {"zeta": get(OCEAN.zeta_t2), "ubar": get(OCEAN.ubar_t2), "vbar": get(OCEAN.vbar_t2),
+           "u": get(OCEAN.u_t2).reshape(N + 1, S2)[:N], "v": get(OCEAN.v_t2).reshape(N + 1, S2)[:N],
+           "W": get(OCEAN.W).reshape(N + 1, S2)}
+    return out
+
+
+def test_config3_fp64_bitwise_against_oracle():
+    import torch
+    cfg = configs.basin3()
+    N = cfg["N"]
+    print(f"[case] config3 FP64 {cfg['szI']}x{cfg['szJ']}x{N}, {NSTEPS} baroclinic steps", file=sys.stderr,
+          flush=True)
+    ct, GRID, OCEAN, BOUNDARY = _run(cfg, torch.float64)
+    torch.cuda.synchronize()
+    GRID._bcfields = (BOUNDARY.zetaBC.bcIdxFieldIdx2, BOUNDARY.uvelBC.bcIdxFieldType,
+                      BOUNDARY.vvelBC.bcIdxFieldType)
+    t, g, s, b, m = _oracle3(cfg, GRID, ct, N)
+    m.AKv[:] = OCEAN.AKv.detach().cpu().numpy().ravel()
+    m.set_depth(s)
+    time = 0.0
+    for iic in range(NSTEPS):
+        t.time2D = time
+        m.baroclinic_step(t, s, b, iic, 0, ct.dt, time)
+        time += ct.dt
+    bad = []
+    for name in ("u", "v", "ru", "rv"):
+        if not bitwise_equal(getattr(OCEAN, name)[2].detach().cpu().numpy().ravel(), getattr(m, name)[2]):
+            bad.append(name)
+    for name in ("W", "Huon", "Hvom"):
+        if not bitwise_equal(getattr(OCEAN, name).detach().cpu().numpy().ravel(), getattr(m, name)):
+            bad.append(name)
+    for name, arr in (("z_w", GRID.z_w), ("z_r", GRID.z_r), ("Hz", GRID.Hz)):
+        if not bitwise_equal(arr.detach().cpu().numpy().ravel(), getattr(m, name)):
+            bad.append(name)
+    prod, orc = product_arrays(OCEAN), oracle_arrays(s)
+    # the fused substep leaves rvbar and the corrector's rubar slot unwritten and keeps
+    # the dead AM3 increment of ubar only at masked points
+    skip = {("rvbar", k) for k in range(3)} | {("rubar", k) for k in range(3)} | \
+           {("ubar", prod["slots"]["ubar"][0]), ("ubar", prod["slots"]["ubar"][1])}
+    bad2 = compare_states(prod, orc, skip=skip)
+    assert not bad and not bad2, (bad, bad2)
+    u = OCEAN.u_t2.detach().cpu().numpy()
+    assert np.all(np.isfinite(u)) and np.abs(u).max() > 1e-3     # the bump has set the fluid moving
+
+
+def test_config3_fp32_bulk_against_fp64():
+    import torch
+    cfg = configs.basin3()
+    N, S2 = cfg["N"], cfg["szI"] * cfg["szJ"]
+    print(f"[case] config3 FP32 vs FP64 {cfg['szI']}x{cfg['szJ']}x{N}", file=sys.stderr, flush=True)
+    *_, O64, _B = _run(cfg, torch.float64)
+    ref = _fields(O64, S2, N)
+    del O64, _B
+    *_, O32, _B = _run(cfg, torch.float32)
+    assert O32.u.dtype == torch.float32 and O32.DU_avg1.dtype == torch.float64
+    got = _fields(O32, S2, N)
+    errs = {k: float(np.abs(got[k] - ref[k]).max() / np.abs(ref[k]).max()) for k in ref}
+    print("[config3 fp32 error vs fp64] " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), file=sys.stderr,
+          flush=True)
+    assert all(np.isfinite(got[k]).all() for k in got)
+    over = {k: v for k, v in errs.items() if not v <= FP32_BOUNDS[k]}
+    assert not over, (over, errs)

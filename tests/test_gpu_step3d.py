@@ -27,6 +27,7 @@ def _cfg3(szI, szJ, N, seed):
 def _oracle3(cfg, GRID, ct, N, fused_fix_q5=True):
     from oracle import oracle3d as O3
     t, g, s, b = oracle_model(cfg, ct.weight1, ct.nfast, fix_q5=fused_fix_q5)
+    s.Zt_avg1[:] = np.asarray(cfg["zeta0"], float).ravel()     # as build_model3d starts it
     BOUNDARY_fields = GRID._bcfields
     m = O3.OModel3(g, N, GRID.host["fomn"], GRID.host["sc_r"], GRID.host["sc_w"], GRID.host["Cs_r"],
                    GRID.host["Cs_w"], GRID.hc, GRID.Vtransform, *BOUNDARY_fields)
