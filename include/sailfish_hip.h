@@ -300,6 +300,55 @@ int sf_phys_substep_fused(const sf_phys_fields *fields, int32_t precision, doubl
  * when 64 x 8 tiles would give fewer than 1024 workgroups). */
 int sf_phys_fused_default_tile(int32_t precision, int32_t szI, int32_t szJ);
 
+/* ---------------- coupled split-explicit step (ROMS 2D/3D coupling) ---------------- */
+
+/* Forcing of the barotropic mode by the baroclinic step: rufrc = sum over the szK levels
+ * of ru + ((sustr - rdrg * u(level 0)) * om_u) * on_u on the interior u points (i = 2 ..
+ * szI-2, j = 1 .. szJ-2), rvfrc likewise on the interior v points (j = 2 .. szJ-2,
+ * i = 1 .. szI-2), 0 elsewhere; FP64 sums of FP64 or FP32 (precision) 3D fields.  The
+ * reference computes ru, rv here but leaves this integral commented out
+ * (nonlinear/rhs.py:76-89). */
+int sf_couple_rufrc(sf_ctx *ctx, int32_t precision, const void *ru, const void *rv, const void *u, const void *v,
+                    const double *sustr, const double *svstr, double rdrg, double *rufrc, double *rvfrc,
+                    void *stream);
+
+/* Coupling fields of one physics-mode substep (ROMS step2d with SOLVE3D,
+ * obsolete/step2d_LF_AM3.h.py:778-846 and :2377-2590), all FP64.  The first substep of
+ * a baroclinic step (first != 0) forms the forcing F = rufrc - rhs of its predictor into
+ * Fu/Fv and adds its Adams-Bashforth extrapolation over baroclinic steps (ab = 0: F;
+ * 1: 1.5 F - 0.5 F1; 2: 23/12 F - 16/12 F1 + 5/12 F2; Fu1/Fu2: F of the two previous
+ * baroclinic steps); every other stage adds F.  DU_avg2/DV_avg2 accumulate w2p x the
+ * fluxes of the predictor's state (set, not accumulated, on the first substep) and
+ * w2c x those of the predicted state. */
+typedef struct sf_couple_fields {
+    const double *rufrc, *rvfrc;
+    double *Fu, *Fv;
+    const double *Fu1, *Fv1, *Fu2, *Fv2;
+    double *DU_avg2, *DV_avg2;
+    double w2p, w2c;
+    int32_t ab;
+} sf_couple_fields;
+
+/* sf_phys_substep (four launches) coupled with the baroclinic step.  Same reference
+ * lineage as sf_phys_substep. */
+int sf_phys_substep_coupled(const sf_phys_fields *fields, const sf_couple_fields *cpl, int32_t precision,
+                            double dt, int32_t first, double cff1, void *stream);
+
+/* step3d_UV coupled with the barotropic mode (nonlinear/step3d_uv.py:155-180 and
+ * :365-545; replaces the reference kernel's tmpU-based mean correction and
+ * adjustBarotropicVelocity, step3D_uv.kernels.cpp:408-517, and makes the vbar_t2 = 0
+ * work-around of nonlinear/main3d.py:189 unnecessary): the implicit vertical viscosity of
+ * sf_step3d_uv, then per interior column u -= (sum DC u - DU_avg1) / D with
+ * DC = on_u Hz_u, ubar(t1) = ubar(t2) = DU_avg1 / D, and the mass flux
+ * Huon = 0.5 (Huon + u DC) corrected to sum to DU_avg2; v likewise.  3D fields, ubar,
+ * vbar and the scratch in `precision`; the averages FP64; column sums in FP64. */
+int sf_step3d_uv_coupled(sf_ctx *ctx, int32_t precision, void *u, void *v, const void *ru, const void *rv,
+                         void *ubar_t1, void *vbar_t1, void *ubar_t2, void *vbar_t2, const void *Hz, const void *Akv,
+                         const void *z_r, const double *DU_avg1, const double *DV_avg1, const double *DU_avg2,
+                         const double *DV_avg2, void *Huon, void *Hvom, const void *tmpU, const void *tmpV,
+                         int32_t iic, int32_t ntfirst, double lambda, double Dt, void *SD, void *D, void *RHS,
+                         void *stream);
+
 /* ---------------- ensembles of physics-mode basins ---------------------------------- */
 
 /* An ensemble of basins of any sizes advanced together: ONE launch per substep walks a

@@ -470,3 +470,104 @@ void sfo_step3d_UV(const sfo_grid3 *g, double *u, double *v, const double *ru, c
         vbar_t1[f] = vbar_t2[f];
     }
 }
+
+/* ---------------------------------------------------------------------------
+ * Coupling of the 2D and 3D momentum equations (ROMS split-explicit form).
+ *
+ * sfo_couple_rufrc: the 2D forcing of the baroclinic step, the vertical integral of
+ * the 3D right-hand side plus the surface and bottom stresses (nonlinear/rhs.py:76-89,
+ * the commented vertIntegral + stress terms; bottom stress rdrg * u of the bottom
+ * level):  rufrc = sum_K ru(K) + ((sustr - rdrg * u(0)) * om_u) * on_u, on the
+ * interior u points (i = 2..L-1, j = 1..M-1), and the same for v (j = 2..M-1,
+ * i = 1..L-1); 0 elsewhere.  Metrics as the physics substep derives them:
+ * on_u = 2/(pn(i-1)+pn(i)), om_u = 2/(pm(i-1)+pm(i)), om_v = 2/(pm(j-1)+pm(j)),
+ * on_v = 2/(pn(j-1)+pn(j)).
+ * ------------------------------------------------------------------------- */
+void sfo_couple_rufrc(const sfo_grid3 *g, const double *ru, const double *rv, const double *u, const double *v,
+                      const double *sustr, const double *svstr, double rdrg, double *rufrc, double *rvfrc)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N, W = g->szI;
+    const int L = g->szI - 1, M = g->szJ - 1;
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        const int i = (int)(f % W), j = (int)(f / W);
+        double su = 0.0, sv = 0.0;
+        if (i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1) {
+            for (ptrdiff_t K = 0; K < N; K++) su = su + ru[K * S2 + f];
+            const double onu = 2.0 / (g->pn[f - 1] + g->pn[f]), omu = 2.0 / (g->pm[f - 1] + g->pm[f]);
+            su = su + ((sustr[f] - rdrg * u[f]) * omu) * onu;
+        }
+        if (j >= 2 && j <= M - 1 && i >= 1 && i <= L - 1) {
+            for (ptrdiff_t K = 0; K < N; K++) sv = sv + rv[K * S2 + f];
+            const double omv = 2.0 / (g->pm[f - W] + g->pm[f]), onv = 2.0 / (g->pn[f - W] + g->pn[f]);
+            sv = sv + ((svstr[f] - rdrg * v[f]) * omv) * onv;
+        }
+        rufrc[f] = su;
+        rvfrc[f] = sv;
+    }
+}
+
+/* step3d_UV coupled with the barotropic mode (nonlinear/step3d_uv.py:155-180 and
+ * :365-545, ROMS step3d_uv "Couple 2D and 3D momentum equations"): the implicit
+ * vertical viscosity of sfo_step3d_UV (build_matrix + solve_tri, unchanged), then per
+ * interior u column (i = 2..L-1, j = 1..M-1; v: j = 2..M-1, i = 1..L-1), with
+ * DC(K) = on_u * Hz_u(K), D = sum DC(K):
+ *   CF = (sum DC(K) u(K) - DU_avg1) / D;  u(K) -= CF;  ubar(t1) = ubar(t2) = DU_avg1 / D
+ *   Huon(K) = 0.5 (Huon(K) + u(K) DC(K));  FC = (sum Huon(K) - DU_avg2) / D;
+ *   Huon(K) -= DC(K) FC
+ * so the new u carries the time-averaged barotropic transport DU_avg1 and the mass
+ * flux the filtered transport DU_avg2.  (The text's u section omits the FC sum that its
+ * v section has; both sections sum here, as ROMS does.)  This replaces the
+ * tmpU-based mean correction and adjustBarotropicVelocity of the live kernel. */
+static void couple_column(const sfo_grid3 *g, int vdir, ptrdiff_t f, double *x, const double *Hz,
+                          const double *Davg1, const double *Davg2, double *Hf, double *xbar1, double *xbar2)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N, W = g->szI;
+    const int i = (int)(f % W), j = (int)(f / W), L = g->szI - 1, M = g->szJ - 1;
+    if (!vdir && !(i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1)) return;
+    if (vdir && !(j >= 2 && j <= M - 1 && i >= 1 && i <= L - 1)) return;
+    const ptrdiff_t d = vdir ? W : 1;
+    const double met = vdir ? 2.0 / (g->pm[f - W] + g->pm[f]) : 2.0 / (g->pn[f - 1] + g->pn[f]);
+#define DC(K) (met * ((Hz[(K) * S2 + f] + Hz[(K) * S2 + f - d]) * 0.5))
+    double Ds = 0.0, CF = 0.0;
+    for (ptrdiff_t K = 0; K < N; K++) {
+        Ds = Ds + DC(K);
+        CF = CF + DC(K) * x[K * S2 + f];
+    }
+    const double rD = 1.0 / Ds;
+    CF = rD * (CF - Davg1[f]);
+    xbar2[f] = rD * Davg1[f];
+    xbar1[f] = xbar2[f];
+    for (ptrdiff_t K = 0; K < N; K++) x[K * S2 + f] = x[K * S2 + f] - CF;
+    double FC = 0.0;
+    for (ptrdiff_t K = 0; K < N; K++) {
+        Hf[K * S2 + f] = 0.5 * (Hf[K * S2 + f] + x[K * S2 + f] * DC(K));
+        FC = FC + Hf[K * S2 + f];
+    }
+    FC = rD * (FC - Davg2[f]);
+    for (ptrdiff_t K = 0; K < N; K++) Hf[K * S2 + f] = Hf[K * S2 + f] - DC(K) * FC;
+#undef DC
+}
+
+void sfo_step3d_uv_coupled(const sfo_grid3 *g, double *u, double *v, const double *ru, const double *rv,
+                           double *ubar_t1, double *vbar_t1, double *ubar_t2, double *vbar_t2, const double *Hz,
+                           const double *Akv, const double *z_r, const double *DU_avg1, const double *DV_avg1,
+                           const double *DU_avg2, const double *DV_avg2, double *Huon, double *Hvom,
+                           const double *tmpU, const double *tmpV, int iic, int ntfirst, double lambda, double Dt,
+                           double *SD, double *D, double *RHS)
+{
+    const ptrdiff_t S2 = S2_OF(g);
+    double cff;
+    if (iic == ntfirst) cff = 1.0;
+    else if (iic == ntfirst + 1) cff = 3.0 / 2.0;
+    else cff = 23.0 / 12.0;
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        build_matrix(g, 0, f, cff, Dt, lambda, SD, D, RHS, Hz, Akv, z_r, tmpU, ru);
+        solve_tri(g, f, SD, D, RHS, u);
+        build_matrix(g, 1, f, cff, Dt, lambda, SD, D, RHS, Hz, Akv, z_r, tmpV, rv);
+        solve_tri(g, f, SD, D, RHS, v);
+        couple_column(g, 0, f, u, Hz, DU_avg1, DU_avg2, Huon, ubar_t1, ubar_t2);
+        couple_column(g, 1, f, v, Hz, DV_avg1, DV_avg2, Hvom, vbar_t1, vbar_t2);
+    }
+}

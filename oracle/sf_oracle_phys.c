@@ -155,6 +155,38 @@ void sfo_phys_bc(const sfo_phys *p, double *z, double *u, double *v)
     }
 }
 
+/* Coupling of a substep with the baroclinic step (ROMS split-explicit form,
+ * obsolete/step2d_LF_AM3.h.py:2377-2590 and :778-846):
+ *  - the first substep of a baroclinic step (first != 0) turns the vertical integral
+ *    of the 3D right-hand side (rufrc) into the forcing F = rufrc - rhs_ubar of its
+ *    predictor, stores F, and adds the Adams-Bashforth extrapolation of F over the
+ *    baroclinic steps to that predictor (ab = 0: F; 1: 1.5 F - 0.5 F1;
+ *    2: 23/12 F - 16/12 F1 + 5/12 F2, F1/F2 the forcing of the two previous steps);
+ *  - every other stage adds F;
+ *  - DU_avg2/DV_avg2 accumulate the fluxes of the predictor's state (weight w2p;
+ *    the first substep sets them) and of the predicted state (weight w2c). */
+typedef struct sfo_couple {
+    const double *rufrc, *rvfrc;
+    double *Fu, *Fv;
+    const double *Fu1, *Fv1, *Fu2, *Fv2;
+    double *DU2, *DV2;
+    double w2p, w2c;
+    int ab;
+} sfo_couple;
+
+static double ab_forcing(int ab, double F, double F1, double F2)
+{
+    if (ab == 0) return F;
+    if (ab == 1) return 1.5 * F - 0.5 * F1;
+    return ((23.0 / 12.0) * F - (16.0 / 12.0) * F1) + (5.0 / 12.0) * F2;
+}
+
+static void phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
+                         const double *z0, const double *z1, double *z2, const double *u0, const double *u1,
+                         double *u2, const double *v0, const double *v1, double *v2, const double *rz0, double *rz1,
+                         const double *ru0, double *ru1, const double *rv0, double *rv1, double *zs, double *us,
+                         double *vs, double *gz, double *Zt, double *DUa, double *DVa, const sfo_couple *cp);
+
 /* One substep.  Slots after the caller's rotation: index 0 = n-1, 1 = n, 2 = new.
  * zs/us/vs: predicted-state scratch (S2 each).  first != 0: forward-Euler
  * predictor and backward-Euler corrector (no history).  Writes z2/u2/v2, the
@@ -164,6 +196,28 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
                       double *u2, const double *v0, const double *v1, double *v2, const double *rz0, double *rz1,
                       const double *ru0, double *ru1, const double *rv0, double *rv1, double *zs, double *us,
                       double *vs, double *gz, double *Zt, double *DUa, double *DVa)
+{
+    phys_substep(p, dt, first, cff1w, z0, z1, z2, u0, u1, u2, v0, v1, v2, rz0, rz1, ru0, ru1, rv0, rv1, zs, us,
+                 vs, gz, Zt, DUa, DVa, NULL);
+}
+
+/* The same substep coupled with the baroclinic step (sfo_couple above). */
+void sfo_phys_substep_coupled(const sfo_phys *p, double dt, int first, double cff1w,
+                              const double *z0, const double *z1, double *z2, const double *u0, const double *u1,
+                              double *u2, const double *v0, const double *v1, double *v2, const double *rz0,
+                              double *rz1, const double *ru0, double *ru1, const double *rv0, double *rv1,
+                              double *zs, double *us, double *vs, double *gz, double *Zt, double *DUa, double *DVa,
+                              const sfo_couple *cp)
+{
+    phys_substep(p, dt, first, cff1w, z0, z1, z2, u0, u1, u2, v0, v1, v2, rz0, rz1, ru0, ru1, rv0, rv1, zs, us,
+                 vs, gz, Zt, DUa, DVa, cp);
+}
+
+static void phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
+                         const double *z0, const double *z1, double *z2, const double *u0, const double *u1,
+                         double *u2, const double *v0, const double *v1, double *v2, const double *rz0, double *rz1,
+                         const double *ru0, double *ru1, const double *rv0, double *rv1, double *zs, double *us,
+                         double *vs, double *gz, double *Zt, double *DUa, double *DVa, const sfo_couple *cp)
 {
     const int W = p->szI, L = p->szI - 1, M = p->szJ - 1;
     const ptrdiff_t S2 = (ptrdiff_t)W * p->szJ;
@@ -178,6 +232,11 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
             rz1[f] = r;
             if (first) zs[f] = z1[f] + ((p->pm[f] * p->pn[f]) * dt) * r;
             else zs[f] = z0[f] + ((p->pm[f] * p->pn[f]) * (2.0 * dt)) * r;
+            if (cp) {   /* DU_avg2/DV_avg2: fluxes of the predictor's state */
+                const double a = first ? 0.0 : cp->DU2[f], b = first ? 0.0 : cp->DV2[f];
+                cp->DU2[f] = a + cp->w2p * DUon(p, z1, u1, W, j, i);
+                cp->DV2[f] = b + cp->w2p * DVom(p, z1, v1, W, j, i);
+            }
         }
     memcpy(us, u1, sizeof(double) * S2);
     memcpy(vs, v1, sizeof(double) * S2);
@@ -192,7 +251,16 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
     for (int j = 1; j < M; j++)
         for (int i = 2; i < L; i++) {
             const ptrdiff_t f = IDX(j, i);
-            const double r = sfo_phys_rhs_u(p, z1, u1, v1, gz, j, i);
+            double r = sfo_phys_rhs_u(p, z1, u1, v1, gz, j, i);
+            if (cp) {
+                if (first) {
+                    const double F = cp->rufrc[f] - r;
+                    cp->Fu[f] = F;
+                    r = r + ab_forcing(cp->ab, F, cp->Fu1[f], cp->Fu2[f]);
+                } else {
+                    r = r + cp->Fu[f];
+                }
+            }
             ru1[f] = r;
             const double cff = (p->pm[f] + p->pm[f - 1]) * (p->pn[f] + p->pn[f - 1]);
             const double fac = 1.0 / (Dat(p, zs, f) + Dat(p, zs, f - 1));
@@ -202,7 +270,16 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
     for (int j = 2; j < M; j++)
         for (int i = 1; i < L; i++) {
             const ptrdiff_t f = IDX(j, i);
-            const double r = sfo_phys_rhs_v(p, z1, u1, v1, gz, j, i);
+            double r = sfo_phys_rhs_v(p, z1, u1, v1, gz, j, i);
+            if (cp) {
+                if (first) {
+                    const double F = cp->rvfrc[f] - r;
+                    cp->Fv[f] = F;
+                    r = r + ab_forcing(cp->ab, F, cp->Fv1[f], cp->Fv2[f]);
+                } else {
+                    r = r + cp->Fv[f];
+                }
+            }
             rv1[f] = r;
             const double cff = (p->pm[f] + p->pm[f - W]) * (p->pn[f] + p->pn[f - W]);
             const double fac = 1.0 / (Dat(p, zs, f) + Dat(p, zs, f - W));
@@ -224,6 +301,10 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
             Zt[f] = Zt[f] + cff1w * zs[f];
             DUa[f] = DUa[f] + cff1w * DUon(p, zs, us, W, j, i);
             DVa[f] = DVa[f] + cff1w * DVom(p, zs, vs, W, j, i);
+            if (cp) {   /* DU_avg2/DV_avg2: fluxes of the predicted state */
+                cp->DU2[f] = cp->DU2[f] + cp->w2c * DUon(p, zs, us, W, j, i);
+                cp->DV2[f] = cp->DV2[f] + cp->w2c * DVom(p, zs, vs, W, j, i);
+            }
         }
     memcpy(u2, u1, sizeof(double) * S2);
     memcpy(v2, v1, sizeof(double) * S2);
@@ -237,7 +318,8 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
     for (int j = 1; j < M; j++)
         for (int i = 2; i < L; i++) {
             const ptrdiff_t f = IDX(j, i);
-            const double r = sfo_phys_rhs_u(p, zs, us, vs, gz, j, i);
+            double r = sfo_phys_rhs_u(p, zs, us, vs, gz, j, i);
+            if (cp) r = r + cp->Fu[f];
             const double cff = (p->pm[f] + p->pm[f - 1]) * (p->pn[f] + p->pn[f - 1]);
             const double fac = 1.0 / (Dat(p, z2, f) + Dat(p, z2, f - 1));
             const double tend = first ? (0.5 * dt) * r : (b1 * r + b2 * ru1[f]) - b3 * ru0[f];
@@ -247,7 +329,8 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
     for (int j = 2; j < M; j++)
         for (int i = 1; i < L; i++) {
             const ptrdiff_t f = IDX(j, i);
-            const double r = sfo_phys_rhs_v(p, zs, us, vs, gz, j, i);
+            double r = sfo_phys_rhs_v(p, zs, us, vs, gz, j, i);
+            if (cp) r = r + cp->Fv[f];
             const double cff = (p->pm[f] + p->pm[f - W]) * (p->pn[f] + p->pn[f - W]);
             const double fac = 1.0 / (Dat(p, z2, f) + Dat(p, z2, f - W));
             const double tend = first ? (0.5 * dt) * r : (b1 * r + b2 * rv1[f]) - b3 * rv0[f];

@@ -1,0 +1,168 @@
+"""Coupled split-explicit baroclinic step: the reference's 3D kernels (rhs3d,
+set_maxflux, omega, set_depth, the step3d_UV viscosity solve) with the barotropic mode
+in physics form (ROMS LF-AM3 with advection, Coriolis, wind, drag) and the 2D/3D
+coupling of ROMS that the port keeps only as text and comments:
+
+- the vertical integral of the 3D right-hand side plus surface and bottom stress
+  forces the barotropic substeps (nonlinear/rhs.py:76-89, commented in the reference;
+  obsolete/step2d_LF_AM3.h.py:2377-2590), with an Adams-Bashforth extrapolation over
+  baroclinic steps at the first substep;
+- the substeps also accumulate the filtered transport DU_avg2/DV_avg2
+  (step2d_LF_AM3.h.py:778-846);
+- step3d_UV replaces the vertical mean of u, v by the averaged barotropic transport
+  and corrects the mass fluxes Huon/Hvom to DU_avg2/DV_avg2
+  (nonlinear/step3d_uv.py:155-180, :365-545), so the vbar_t2 = 0 work-around of
+  nonlinear/main3d.py:189 is not needed.
+
+One baroclinic step (``coupled_step``): set_maxflux, omega, set_zeta, rhs3d, rufrc,
+nfast+1 coupled substeps (the first forward/backward Euler, averages reset on it),
+set_depth, coupled step3d_UV, setLateralUVBCs, omega.  FP64 or FP32 bulk state (the
+averages, forcing and coupling sums stay FP64).  The barotropic substep is the
+four-launch physics form (``sf_phys_substep_coupled``)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import mod_operators as ops
+from .main3d import initial_depths
+from .phys2d import PhysStepper, sf_phys_fields
+from .rhs import rhs3d
+
+P = C.c_void_p
+
+
+class sf_couple_fields(C.Structure):
+    _fields_ = [("rufrc", P), ("rvfrc", P), ("Fu", P), ("Fv", P), ("Fu1", P), ("Fv1", P), ("Fu2", P), ("Fv2", P),
+                ("DU_avg2", P), ("DV_avg2", P), ("w2p", C.c_double), ("w2c", C.c_double), ("ab", C.c_int32)]
+
+
+def _bind():
+    lib = _lib.load()
+    if not getattr(lib, "_sf_coupled_bound", False):
+        lib.sf_phys_substep_coupled.argtypes = [C.POINTER(sf_phys_fields), C.POINTER(sf_couple_fields), C.c_int32,
+                                                C.c_double, C.c_int32, C.c_double, P]
+        lib.sf_phys_substep_coupled.restype = C.c_int32
+        lib.sf_couple_rufrc.argtypes = [P, C.c_int32, P, P, P, P, P, P, C.c_double, P, P, P]
+        lib.sf_couple_rufrc.restype = C.c_int32
+        lib.sf_step3d_uv_coupled.argtypes = [P, C.c_int32] + [P] * 19 + [C.c_int32, C.c_int32, C.c_double,
+                                                                         C.c_double, P, P, P, P]
+        lib.sf_step3d_uv_coupled.restype = C.c_int32
+        lib._sf_coupled_bound = True
+    return lib
+
+
+def filtered_transport_weights(weight2, iif: int) -> tuple[float, float]:
+    """(w2p, w2c): weights of the predictor-state and predicted-state fluxes in
+    DU_avg2 at substep iif (1-based) of a barotropic loop, step2d_LF_AM3.h.py:778-846.
+    ROMS's weight(2, k) is weight2[k-1]; entries past the table are 0."""
+    def w(k):
+        return float(weight2[k - 1]) if 1 <= k <= len(weight2) else 0.0
+    if iif == 1:
+        return (-1.0 / 12.0) * w(2), w(1)
+    return (8.0 / 12.0) * w(iif) - (1.0 / 12.0) * w(iif + 1), (5.0 / 12.0) * w(iif)
+
+
+class CoupledState:
+    """Device fields of the coupling: rufrc/rvfrc, three forcing buffers per component
+    (this step and the two before it, rotated per baroclinic step), DU_avg2/DV_avg2,
+    the FP64 stresses, and the physics substep in its four-launch form."""
+
+    def __init__(self, GRID, OCEAN, rdrg: float = 3e-4, g: float | None = None):
+        _lib.require_gpu()
+        self.lib = _bind()
+        dev, S2 = GRID.device, GRID.szI * GRID.szJ
+        z = lambda: torch.zeros(S2, dtype=torch.float64, device=dev)
+        self.rufrc, self.rvfrc = z(), z()
+        self.Fu = [z() for _ in range(3)]
+        self.Fv = [z() for _ in range(3)]
+        self.DU_avg2, self.DV_avg2 = z(), z()
+        zeros = np.zeros(GRID.shape2D)
+        host = GRID.host
+        self.sustr = torch.from_numpy(np.ascontiguousarray(host.get("sustr", zeros)).reshape(-1)).to(dev)
+        self.svstr = torch.from_numpy(np.ascontiguousarray(host.get("svstr", zeros)).reshape(-1)).to(dev)
+        self.rdrg = float(rdrg)
+        kw = {} if g is None else {"g": g}
+        self.stepper = PhysStepper(GRID, OCEAN, rdrg=rdrg, fused=False, **kw)
+        from .main3d import Main3dState
+        self.m3 = Main3dState(GRID, OCEAN)
+        OCEAN.extra_state = {"DU_avg2": self.DU_avg2, "DV_avg2": self.DV_avg2,
+                             **{f"Fu{k}": t for k, t in enumerate(self.Fu)},
+                             **{f"Fv{k}": t for k, t in enumerate(self.Fv)}}
+
+    def couple_fields(self, n: int, iif: int, weight2) -> sf_couple_fields:
+        """Coupling fields of substep iif of baroclinic step n (iic - ntfirst): the
+        forcing buffers rotate per baroclinic step, the extrapolation order is that of
+        ROMS (first step F, second 3/2 -1/2, then 23/12 -16/12 5/12)."""
+        cur, h1, h2 = n % 3, (n - 1) % 3, (n - 2) % 3
+        w2p, w2c = filtered_transport_weights(weight2, iif)
+        ab = 0 if n == 0 else (1 if n == 1 else 2)
+        p = lambda t: t.data_ptr()
+        return sf_couple_fields(p(self.rufrc), p(self.rvfrc), p(self.Fu[cur]), p(self.Fv[cur]), p(self.Fu[h1]),
+                                p(self.Fv[h1]), p(self.Fu[h2]), p(self.Fv[h2]), p(self.DU_avg2), p(self.DV_avg2),
+                                w2p, w2c, ab)
+
+
+def coupled_step(compTimes, GRID, OCEAN, BOUNDARY, st: CoupledState, history=None):
+    """One baroclinic step of the coupled form."""
+    c = ops.ctx_of(GRID)
+    lib = st.lib
+    dm = BOUNDARY.device_maps()
+    BC = dm["bcIdxFieldIdx2"]
+    prec = _lib.SF_FP64 if OCEAN.u.dtype == torch.float64 else _lib.SF_FP32
+    stream = _lib.stream_handle()
+    Huon, Hvom, W = OCEAN.Huon.reshape(-1), OCEAN.Hvom.reshape(-1), OCEAN.W.reshape(-1)
+    Hz, z_w, z_r = GRID.Hz.reshape(-1), GRID.z_w.reshape(-1), GRID.z_r.reshape(-1)
+    ops.set_maxflux(None, None, (OCEAN.u_t2, OCEAN.v_t2, Huon, Hvom, Hz), ctx=c)
+    ops.omega(None, None, (W, OCEAN.u_t2, OCEAN.v_t2, Huon, Hvom, z_w, BC), ctx=c)
+    ops.set_zeta(None, None, (OCEAN.zeta_t1, OCEAN.zeta_t2, OCEAN.Zt_avg1), ctx=c)
+    rhs3d(GRID, OCEAN, BOUNDARY)
+    _lib.check(lib.sf_couple_rufrc(c.handle, prec, OCEAN.ru_t2.data_ptr(), OCEAN.rv_t2.data_ptr(),
+                                   OCEAN.u_t2.data_ptr(), OCEAN.v_t2.data_ptr(), st.sustr.data_ptr(),
+                                   st.svstr.data_ptr(), st.rdrg, st.rufrc.data_ptr(), st.rvfrc.data_ptr(), stream),
+               "sf_couple_rufrc")
+    compTimes.first2DTimeStep()
+    for _ in range(compTimes.nfast + 1):
+        if compTimes.isFirst2DStep():
+            OCEAN.Zt_avg1.zero_()
+            OCEAN.DU_avg1.zero_()
+            OCEAN.DV_avg1.zero_()
+        compTimes.next2DTimeStep()
+        OCEAN.cycleTimes2D()
+        a = st.stepper.fields(OCEAN)
+        cp = st.couple_fields(compTimes.iic - compTimes.ntfirst, compTimes.iif, compTimes.weight2)
+        _lib.check(lib.sf_phys_substep_coupled(C.byref(a), C.byref(cp), st.stepper.precision,
+                                               float(compTimes.dtfast), int(compTimes.iif == 1), compTimes.cff1(),
+                                               stream), "sf_phys_substep_coupled")
+    if history is not None:
+        history.maybe_write(compTimes, OCEAN)
+    initial_depths(GRID, OCEAN)
+    m3 = st.m3
+    m3.tmpU.copy_(OCEAN.u_t2)
+    m3.tmpV.copy_(OCEAN.v_t2)
+    q = lambda t: t.data_ptr()
+    _lib.check(lib.sf_step3d_uv_coupled(c.handle, prec, q(OCEAN.u_t2), q(OCEAN.v_t2), q(OCEAN.ru_t2),
+                                        q(OCEAN.rv_t2), q(OCEAN.ubar_t1), q(OCEAN.vbar_t1), q(OCEAN.ubar_t2),
+                                        q(OCEAN.vbar_t2), q(Hz), q(OCEAN.AKv), q(z_r), q(OCEAN.DU_avg1),
+                                        q(OCEAN.DV_avg1), q(st.DU_avg2), q(st.DV_avg2), q(Huon), q(Hvom),
+                                        q(m3.tmpU), q(m3.tmpV), int(compTimes.iic), int(compTimes.ntfirst), 1.0,
+                                        float(compTimes.dt), q(m3.eqSD), q(m3.eqD), q(m3.eqRHS), stream),
+               "sf_step3d_uv_coupled")
+    ops.setLateralUVBCs(None, None, (compTimes.time, OCEAN.u_t2, OCEAN.v_t2, dm["uFieldIdx2"], dm["vFieldIdx2"],
+                                     dm["uFieldType"], dm["vFieldType"]), ctx=c)
+    ops.omega(None, None, (W, OCEAN.u_t2, OCEAN.v_t2, Huon, Hvom, z_w, BC), ctx=c)
+    compTimes.nextTimeStep()
+
+
+def main3d_coupled(compTimes, GRID, OCEAN, BOUNDARY, nsteps: int, st: CoupledState | None = None,
+                   rdrg: float = 3e-4, history=None) -> CoupledState:
+    """``nsteps`` coupled baroclinic steps after the initial set_depth (main3d.py:62-63)."""
+    if st is None:
+        st = CoupledState(GRID, OCEAN, rdrg=rdrg)
+        initial_depths(GRID, OCEAN)
+    for _ in range(int(nsteps)):
+        coupled_step(compTimes, GRID, OCEAN, BOUNDARY, st, history=history)
+    return st

@@ -108,6 +108,29 @@ __device__ __forceinline__ void zeta_src(int szI, int szJ, int &i, int &j)
     j = j < 1 ? 1 : (j > M - 1 ? M - 1 : j);
 }
 
+// Coupling with the baroclinic step (sf_phys_substep_coupled; on == 0: plain substep).
+// The first substep of a baroclinic step turns the vertical integral of the 3D
+// right-hand side into the forcing F = rufrc - rhs of its predictor, stores F and adds
+// its Adams-Bashforth extrapolation over the baroclinic steps; every other stage adds F
+// (obsolete/step2d_LF_AM3.h.py:2377-2590).  DU2/DV2 accumulate the fluxes of the
+// predictor's state (w2p; set on the first substep) and of the predicted state (w2c)
+// (:778-846).  All in FP64 whatever the state precision.
+struct Cpl {
+    const double *rufrc, *rvfrc;
+    double *Fu, *Fv;
+    const double *Fu1, *Fv1, *Fu2, *Fv2;
+    double *DU2, *DV2;
+    double w2p, w2c;
+    int ab, on;
+};
+
+__device__ __forceinline__ double ab_forcing(int ab, double F, double F1, double F2)
+{
+    if (ab == 0) return F;
+    if (ab == 1) return 1.5 * F - 0.5 * F1;
+    return ((23.0 / 12.0) * F - (16.0 / 12.0) * F1) + (5.0 / 12.0) * F2;
+}
+
 template <typename T>
 struct Sub {
     PF<T> p;
@@ -120,7 +143,22 @@ struct Sub {
     T dt;
     int first;
     double w1;
+    Cpl c;
 };
+
+// predictor forcing of a u (or v) point: the first substep forms F from the point's
+// right-hand side r (and stores it where the point is its own, not a ghost's source)
+template <typename T>
+__device__ __forceinline__ T pred_forcing(const Sub<T> &a, bool vdir, long long f, T r, bool store)
+{
+    const double *frc = vdir ? a.c.rvfrc : a.c.rufrc;
+    double *Fo = vdir ? a.c.Fv : a.c.Fu;
+    if (!a.first) return r + T(Fo[f]);
+    const double F = frc[f] - (double)r;
+    if (store) Fo[f] = F;
+    const double *F1 = vdir ? a.c.Fv1 : a.c.Fu1, *F2 = vdir ? a.c.Fv2 : a.c.Fu2;
+    return r + T(ab_forcing(a.c.ab, F, F1[f], F2[f]));
+}
 
 // ---- P1: predicted zeta (+ predictor RHS), and the pressure surface gz ---------------
 template <typename T>
@@ -140,7 +178,14 @@ __global__ __launch_bounds__(256) void k_phys_pred_zeta(Sub<T> a)
     T zs;
     if (a.first) zs = a.z1[fs] + ((a.p.pm[fs] * a.p.pn[fs]) * a.dt) * r;
     else zs = a.z0[fs] + ((a.p.pm[fs] * a.p.pn[fs]) * (T(2) * a.dt)) * r;
-    if (!ghost) a.rz1[f] = r;
+    if (!ghost) {
+        a.rz1[f] = r;
+        if (a.c.on) {
+            const double du = a.first ? 0.0 : a.c.DU2[f], dv = a.first ? 0.0 : a.c.DV2[f];
+            a.c.DU2[f] = du + a.c.w2p * (double)DUon(a.p, a.z1, a.u1, f);
+            a.c.DV2[f] = dv + a.c.w2p * (double)DVom(a.p, a.z1, a.v1, f);
+        }
+    }
     a.zs[f] = zs;
     const T cff4 = T(4.0 / 25.0), cff5 = T(1) - T(2) * cff4;
     a.gz[f] = a.first ? T(0.5) * (a.z1[f] + zs) : cff5 * a.z1[f] + cff4 * (a.z0[f] + zs);
@@ -152,7 +197,8 @@ __device__ T pred_u(const Sub<T> &a, long long f, bool store_rhs)
 {
     const long long W = a.p.szI;
     State3<T> s1{a.z1, a.u1, a.v1};
-    const T r = rhs_u(a.p, s1, a.gz, f);
+    T r = rhs_u(a.p, s1, a.gz, f);
+    if (a.c.on) r = pred_forcing(a, false, f, r, store_rhs);
     if (store_rhs) a.ru1[f] = r;
     const T cff = (a.p.pm[f] + a.p.pm[f - 1]) * (a.p.pn[f] + a.p.pn[f - 1]);
     const T fac = sf::recip(Dat(a.p, (const T *)a.zs, f) + Dat(a.p, (const T *)a.zs, f - 1));
@@ -168,7 +214,8 @@ __device__ T pred_v(const Sub<T> &a, long long f, bool store_rhs)
 {
     const long long W = a.p.szI;
     State3<T> s1{a.z1, a.u1, a.v1};
-    const T r = rhs_v(a.p, s1, a.gz, f);
+    T r = rhs_v(a.p, s1, a.gz, f);
+    if (a.c.on) r = pred_forcing(a, true, f, r, store_rhs);
     if (store_rhs) a.rv1[f] = r;
     const T cff = (a.p.pm[f] + a.p.pm[f - W]) * (a.p.pn[f] + a.p.pn[f - W]);
     const T fac = sf::recip(Dat(a.p, (const T *)a.zs, f) + Dat(a.p, (const T *)a.zs, f - W));
@@ -234,6 +281,10 @@ __global__ __launch_bounds__(256) void k_phys_corr_zeta(Sub<T> a)
         a.Zt[f] = a.Zt[f] + w * (double)a.zs[f];
         a.DUa[f] = a.DUa[f] + w * (double)DUon(a.p, (const T *)a.zs, (const T *)a.us, f);
         a.DVa[f] = a.DVa[f] + w * (double)DVom(a.p, (const T *)a.zs, (const T *)a.vs, f);
+        if (a.c.on) {
+            a.c.DU2[f] = a.c.DU2[f] + a.c.w2c * (double)DUon(a.p, (const T *)a.zs, (const T *)a.us, f);
+            a.c.DV2[f] = a.c.DV2[f] + a.c.w2c * (double)DVom(a.p, (const T *)a.zs, (const T *)a.vs, f);
+        }
     }
 }
 
@@ -242,7 +293,8 @@ template <typename T>
 __device__ T corr_u(const Sub<T> &a, long long f)
 {
     State3<T> ss{a.zs, a.us, a.vs};
-    const T r = rhs_u(a.p, ss, a.gz, f);
+    T r = rhs_u(a.p, ss, a.gz, f);
+    if (a.c.on) r = r + T(a.c.Fu[f]);
     const T cff = (a.p.pm[f] + a.p.pm[f - 1]) * (a.p.pn[f] + a.p.pn[f - 1]);
     const T fac = sf::recip(Dat(a.p, (const T *)a.z2, f) + Dat(a.p, (const T *)a.z2, f - 1));
     const T b1 = T(0.5) * a.dt * T(5.0) / T(12.0), b2 = T(0.5) * a.dt * T(8.0) / T(12.0),
@@ -256,7 +308,8 @@ __device__ T corr_v(const Sub<T> &a, long long f)
 {
     const long long W = a.p.szI;
     State3<T> ss{a.zs, a.us, a.vs};
-    const T r = rhs_v(a.p, ss, a.gz, f);
+    T r = rhs_v(a.p, ss, a.gz, f);
+    if (a.c.on) r = r + T(a.c.Fv[f]);
     const T cff = (a.p.pm[f] + a.p.pm[f - W]) * (a.p.pn[f] + a.p.pn[f - W]);
     const T fac = sf::recip(Dat(a.p, (const T *)a.z2, f) + Dat(a.p, (const T *)a.z2, f - W));
     const T b1 = T(0.5) * a.dt * T(5.0) / T(12.0), b2 = T(0.5) * a.dt * T(8.0) / T(12.0),
@@ -286,9 +339,15 @@ __global__ __launch_bounds__(256) void k_phys_corr_uv(Sub<T> a)
 }
 
 template <typename T>
-int launch_phys(const sf_phys_fields *x, double dt, int first, double w1, void *stream)
+int launch_phys(const sf_phys_fields *x, double dt, int first, double w1, void *stream,
+                const sf_couple_fields *cp = nullptr)
 {
     Sub<T> a;
+    a.c = Cpl{};
+    if (cp) {
+        a.c = Cpl{cp->rufrc, cp->rvfrc, cp->Fu, cp->Fv, cp->Fu1, cp->Fv1, cp->Fu2, cp->Fv2, cp->DU_avg2,
+                  cp->DV_avg2, cp->w2p, cp->w2c, cp->ab, 1};
+    }
     a.p = PF<T>{(const T *)x->h, (const T *)x->pm, (const T *)x->pn, (const T *)x->fomn, (const T *)x->sustr,
                 (const T *)x->svstr, x->szI, x->szJ, T(x->g), T(x->rdrg)};
     a.z0 = (const T *)x->zeta[0];
@@ -343,5 +402,28 @@ extern "C" int sf_phys_substep(const sf_phys_fields *x, int32_t precision, doubl
     if (precision == SF_FP64) return launch_phys<double>(x, dt, first, cff1, stream);
     if (precision == SF_FP32) return launch_phys<float>(x, dt, first, cff1, stream);
     sf::set_error("sf_phys_substep: unknown precision %d", precision);
+    return SF_ERR_ARG;
+}
+
+extern "C" int sf_phys_substep_coupled(const sf_phys_fields *x, const sf_couple_fields *cp, int32_t precision,
+                                       double dt, int32_t first, double cff1, void *stream)
+{
+    SF_REQUIRE(x && cp, "sf_phys_substep_coupled: null argument");
+    SF_REQUIRE(x->szI >= 4 && x->szJ >= 4, "sf_phys_substep_coupled: grid %dx%d too small", x->szI, x->szJ);
+    SF_REQUIRE(x->h && x->pm && x->pn && x->fomn && x->sustr && x->svstr,
+               "sf_phys_substep_coupled: null grid field");
+    for (int k = 0; k < 3; k++)
+        SF_REQUIRE(x->zeta[k] && x->ubar[k] && x->vbar[k], "sf_phys_substep_coupled: null state slot");
+    SF_REQUIRE(x->rzeta[0] && x->rzeta[1] && x->rubar[0] && x->rubar[1] && x->rvbar[0] && x->rvbar[1],
+               "sf_phys_substep_coupled: null rhs slot");
+    for (int k = 0; k < 4; k++) SF_REQUIRE(x->scratch[k], "sf_phys_substep_coupled: null scratch");
+    SF_REQUIRE(x->Zt_avg1 && x->DU_avg1 && x->DV_avg1, "sf_phys_substep_coupled: null average");
+    SF_REQUIRE(cp->Fu && cp->Fv && cp->DU_avg2 && cp->DV_avg2, "sf_phys_substep_coupled: null coupling field");
+    SF_REQUIRE(!first || (cp->rufrc && cp->rvfrc && cp->Fu1 && cp->Fv1 && cp->Fu2 && cp->Fv2),
+               "sf_phys_substep_coupled: the first substep needs rufrc/rvfrc and the forcing history");
+    SF_REQUIRE(cp->ab >= 0 && cp->ab <= 2, "sf_phys_substep_coupled: ab=%d", cp->ab);
+    if (precision == SF_FP64) return launch_phys<double>(x, dt, first, cff1, stream, cp);
+    if (precision == SF_FP32) return launch_phys<float>(x, dt, first, cff1, stream, cp);
+    sf::set_error("sf_phys_substep_coupled: unknown precision %d", precision);
     return SF_ERR_ARG;
 }

@@ -774,6 +774,91 @@ __global__ __launch_bounds__(256) void k_step3d_uv(G3 g, UVArgs<T> a, double cff
     a.vbar_t1[f] = T(vb);
 }
 
+// ---- coupling of the 2D and 3D momentum equations (ROMS split-explicit form) ----------
+// rufrc / rvfrc: vertical integral of the 3D right-hand side plus the surface and bottom
+// stresses (nonlinear/rhs.py:76-89, commented in the reference; bottom stress rdrg * u of
+// the bottom level), on the interior u (v) points of the physics substep, 0 elsewhere;
+// summed in FP64 for either field precision.
+template <class T>
+__global__ __launch_bounds__(256) void k_couple_rufrc(G3 g, const T *ru, const T *rv, const T *u, const T *v,
+                                                      const double *sustr, const double *svstr, double rdrg,
+                                                      double *rufrc, double *rvfrc)
+{
+    COLUMN_INDEX
+    const long long S2 = g.S2, N = g.N, W = g.szI;
+    const int i = (int)(f % W), j = (int)(f / W), L = g.szI - 1, M = g.szJ - 1;
+    double su = 0.0, sv = 0.0;
+    if (i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1) {
+        for (long long K = 0; K < N; K++) su = su + (double)ru[K * S2 + f];
+        const double onu = 2.0 / (g.pn[f - 1] + g.pn[f]), omu = 2.0 / (g.pm[f - 1] + g.pm[f]);
+        su = su + ((sustr[f] - rdrg * (double)u[f]) * omu) * onu;
+    }
+    if (j >= 2 && j <= M - 1 && i >= 1 && i <= L - 1) {
+        for (long long K = 0; K < N; K++) sv = sv + (double)rv[K * S2 + f];
+        const double omv = 2.0 / (g.pm[f - W] + g.pm[f]), onv = 2.0 / (g.pn[f - W] + g.pn[f]);
+        sv = sv + ((svstr[f] - rdrg * (double)v[f]) * omv) * onv;
+    }
+    rufrc[f] = su;
+    rvfrc[f] = sv;
+}
+
+// step3d_UV coupled with the barotropic mode (nonlinear/step3d_uv.py:155-180, :365-545):
+// the implicit vertical viscosity of k_step3d_uv (same matrix and solve), then per
+// interior u (v) column, with DC(K) = on_u * Hz_u(K) and D = sum DC(K) formed in FP64:
+//   CF = (sum DC u - DU_avg1) / D;  u -= CF;  ubar(t1) = ubar(t2) = DU_avg1 / D;
+//   Huon = 0.5 (Huon + u DC);  FC = (sum Huon - DU_avg2) / D;  Huon -= DC FC.
+template <class T>
+__device__ void couple_column(const G3 &g, bool vdir, long long f, T *x, const T *Hz, const double *Davg1,
+                              const double *Davg2, T *Hf, T *xbar1, T *xbar2)
+{
+    const long long S2 = g.S2, N = g.N, W = g.szI;
+    const int i = (int)(f % W), j = (int)(f / W), L = g.szI - 1, M = g.szJ - 1;
+    if (!vdir && !(i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1)) return;
+    if (vdir && !(j >= 2 && j <= M - 1 && i >= 1 && i <= L - 1)) return;
+    const long long d = vdir ? W : 1;
+    const double met = vdir ? 2.0 / (g.pm[f - W] + g.pm[f]) : 2.0 / (g.pn[f - 1] + g.pn[f]);
+#define DC(K) (met * (double)((Hz[(K) * S2 + f] + Hz[(K) * S2 + f - d]) * T(0.5)))
+    double Ds = 0.0, CF = 0.0;
+    for (long long K = 0; K < N; K++) {
+        Ds = Ds + DC(K);
+        CF = CF + DC(K) * (double)x[K * S2 + f];
+    }
+    const double rD = 1.0 / Ds;
+    CF = rD * (CF - Davg1[f]);
+    const T xb = T(rD * Davg1[f]);
+    xbar2[f] = xb;
+    xbar1[f] = xb;
+    for (long long K = 0; K < N; K++) x[K * S2 + f] = T((double)x[K * S2 + f] - CF);
+    double FC = 0.0;
+    for (long long K = 0; K < N; K++) {
+        const T hf = T(0.5 * ((double)Hf[K * S2 + f] + (double)x[K * S2 + f] * DC(K)));
+        Hf[K * S2 + f] = hf;
+        FC = FC + (double)hf;
+    }
+    FC = rD * (FC - Davg2[f]);
+    for (long long K = 0; K < N; K++) Hf[K * S2 + f] = T((double)Hf[K * S2 + f] - DC(K) * FC);
+#undef DC
+}
+
+template <class T>
+struct CplUV {
+    const double *DU2, *DV2;
+    T *Huon, *Hvom;
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_step3d_uv_coupled(G3 g, UVArgs<T> a, CplUV<T> c, double cff, double lambda,
+                                                           double Dt)
+{
+    COLUMN_INDEX
+    build_matrix(g, false, f, cff, Dt, lambda, a.SD, a.D, a.RHS, a.Hz, a.Akv, a.z_r, a.tmpU, a.ru);
+    solve_tri(g, f, a.SD, a.D, a.RHS, a.u);
+    build_matrix(g, true, f, cff, Dt, lambda, a.SD, a.D, a.RHS, a.Hz, a.Akv, a.z_r, a.tmpV, a.rv);
+    solve_tri(g, f, a.SD, a.D, a.RHS, a.v);
+    couple_column(g, false, f, a.u, a.Hz, a.DU_avg1, c.DU2, c.Huon, a.ubar_t1, a.ubar_t2);
+    couple_column(g, true, f, a.v, a.Hz, a.DV_avg1, c.DV2, c.Hvom, a.vbar_t1, a.vbar_t2);
+}
+
 G3 grid3(const sf_ctx *c)
 {
     return G3{c->szI, c->szJ, c->szK, c->S2, c->on_u, c->om_v, c->pn, c->pm};
@@ -1022,4 +1107,75 @@ extern "C" int sf_step3d_uv_f32(sf_ctx *c, float *u, float *v, const float *ru, 
     UVArgs<float> a{u, v, ru, rv, ubar_t1, vbar_t1, ubar_t2, vbar_t2, Hz, Akv, z_r, DU_avg1, DV_avg1, tmpU, tmpV,
                     SD, D, RHS};
     LAUNCH_COLUMNS(k_step3d_uv<float>, c, stream, grid3(c), a, cff, lambda, Dt);
+}
+
+namespace {
+
+template <class T>
+int couple_rufrc_launch(sf_ctx *c, const void *ru, const void *rv, const void *u, const void *v, const double *sustr,
+                        const double *svstr, double rdrg, double *rufrc, double *rvfrc, void *stream)
+{
+    LAUNCH_COLUMNS(k_couple_rufrc<T>, c, stream, grid3(c), (const T *)ru, (const T *)rv, (const T *)u, (const T *)v,
+                   sustr, svstr, rdrg, rufrc, rvfrc);
+}
+
+template <class T>
+int step3d_uv_coupled_launch(sf_ctx *c, T *u, T *v, const T *ru, const T *rv, T *ubar_t1, T *vbar_t1, T *ubar_t2,
+                             T *vbar_t2, const T *Hz, const T *Akv, const T *z_r, const double *DU_avg1,
+                             const double *DV_avg1, const double *DU_avg2, const double *DV_avg2, T *Huon, T *Hvom,
+                             const T *tmpU, const T *tmpV, int32_t iic, int32_t ntfirst, double lambda, double Dt,
+                             T *SD, T *D, T *RHS, void *stream)
+{
+    double cff;
+    if (iic == ntfirst) cff = 1.0;
+    else if (iic == ntfirst + 1) cff = 3.0 / 2.0;
+    else cff = 23.0 / 12.0;
+    UVArgs<T> a{u, v, ru, rv, ubar_t1, vbar_t1, ubar_t2, vbar_t2, Hz, Akv, z_r, DU_avg1, DV_avg1, tmpU, tmpV,
+                SD, D, RHS};
+    CplUV<T> cp{DU_avg2, DV_avg2, Huon, Hvom};
+    LAUNCH_COLUMNS(k_step3d_uv_coupled<T>, c, stream, grid3(c), a, cp, cff, lambda, Dt);
+}
+
+}  // namespace
+
+extern "C" int sf_couple_rufrc(sf_ctx *c, int32_t precision, const void *ru, const void *rv, const void *u,
+                               const void *v, const double *sustr, const double *svstr, double rdrg, double *rufrc,
+                               double *rvfrc, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(ru && rv && u && v && sustr && svstr && rufrc && rvfrc, "sf_couple_rufrc: null field");
+    if (precision == SF_FP64)
+        return couple_rufrc_launch<double>(c, ru, rv, u, v, sustr, svstr, rdrg, rufrc, rvfrc, stream);
+    if (precision == SF_FP32)
+        return couple_rufrc_launch<float>(c, ru, rv, u, v, sustr, svstr, rdrg, rufrc, rvfrc, stream);
+    sf::set_error("sf_couple_rufrc: unknown precision %d", precision);
+    return SF_ERR_ARG;
+}
+
+extern "C" int sf_step3d_uv_coupled(sf_ctx *c, int32_t precision, void *u, void *v, const void *ru, const void *rv,
+                                    void *ubar_t1, void *vbar_t1, void *ubar_t2, void *vbar_t2, const void *Hz,
+                                    const void *Akv, const void *z_r, const double *DU_avg1, const double *DV_avg1,
+                                    const double *DU_avg2, const double *DV_avg2, void *Huon, void *Hvom,
+                                    const void *tmpU, const void *tmpV, int32_t iic, int32_t ntfirst, double lambda,
+                                    double Dt, void *SD, void *D, void *RHS, void *stream)
+{
+    CTX3(c);
+    SF_REQUIRE(u && v && ru && rv && ubar_t1 && vbar_t1 && ubar_t2 && vbar_t2 && Hz && Akv && z_r && DU_avg1 &&
+                   DV_avg1 && DU_avg2 && DV_avg2 && Huon && Hvom && tmpU && tmpV && SD && D && RHS,
+               "sf_step3d_uv_coupled: null field");
+    if (precision == SF_FP64)
+        return step3d_uv_coupled_launch<double>(
+            c, (double *)u, (double *)v, (const double *)ru, (const double *)rv, (double *)ubar_t1,
+            (double *)vbar_t1, (double *)ubar_t2, (double *)vbar_t2, (const double *)Hz, (const double *)Akv,
+            (const double *)z_r, DU_avg1, DV_avg1, DU_avg2, DV_avg2, (double *)Huon, (double *)Hvom,
+            (const double *)tmpU, (const double *)tmpV, iic, ntfirst, lambda, Dt, (double *)SD, (double *)D,
+            (double *)RHS, stream);
+    if (precision == SF_FP32)
+        return step3d_uv_coupled_launch<float>(
+            c, (float *)u, (float *)v, (const float *)ru, (const float *)rv, (float *)ubar_t1, (float *)vbar_t1,
+            (float *)ubar_t2, (float *)vbar_t2, (const float *)Hz, (const float *)Akv, (const float *)z_r, DU_avg1,
+            DV_avg1, DU_avg2, DV_avg2, (float *)Huon, (float *)Hvom, (const float *)tmpU, (const float *)tmpV, iic,
+            ntfirst, lambda, Dt, (float *)SD, (float *)D, (float *)RHS, stream);
+    sf::set_error("sf_step3d_uv_coupled: unknown precision %d", precision);
+    return SF_ERR_ARG;
 }

@@ -61,7 +61,8 @@ def build_model3d(cfg: dict, *, N: int, device="cuda", ndtfast: int = 30, dt: fl
     GRID = Grid(szI - 2, szJ - 2, N, device=device, Vtransform=Vtransform, Vstretching=Vstretching,
                 theta_s=theta_s, theta_b=theta_b, Tcline=Tcline)
     GRID.fix_q5 = bool(fix_q5)
-    GRID.set_fields(h=cfg["h"], pm=cfg["pm"], pn=cfg["pn"], f=cfg["f"])
+    extra = {k: cfg[k] for k in ("sustr", "svstr") if k in cfg}
+    GRID.set_fields(h=cfg["h"], pm=cfg["pm"], pn=cfg["pn"], f=cfg["f"], **extra)
     sc_r, sc_w, Cs_r, Cs_w = s_coordinates(N, Vstretching, theta_s, theta_b)
     hc = min(GRID.hmin, Tcline) if Vtransform == 1 else Tcline
     GRID.set_vertical(sc_r, sc_w, Cs_r, Cs_w, hc=hc, dtype=dtype)

@@ -1,0 +1,135 @@
+"""GPU parity of the coupled split-explicit step (sailfish_amd/coupled3d.py): the
+reference's 3D kernels with the ROMS 2D/3D coupling (rufrc forcing with its
+Adams-Bashforth start, DU_avg2, the coupled step3d_UV) against the oracle
+(oracle/oracle_coupled.py), bit for bit in FP64 over four baroclinic steps (all three
+extrapolation orders and a full turn of the forcing buffers); the FP32 bulk form
+against FP64 with the bounds written below; and the coupling's defining property on
+the GPU result: each interior column carries the averaged transport DU_avg1."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from sailfish_amd import configs
+from tests.helpers2d import bitwise_equal
+
+pytestmark = pytest.mark.gpu
+
+NSTEPS = 4
+
+
+def _cfg(szI, szJ, N, seed):
+    cfg = configs.random_smooth(szI, szJ, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    sm = lambda: configs.gaussian_smooth(rng.standard_normal((szJ, szI)), 1.5)
+    cfg["u0"] = np.stack([0.05 * sm() for _ in range(N + 1)])
+    cfg["v0"] = np.stack([0.05 * sm() for _ in range(N + 1)])
+    cfg["dtfast"] = 2.0
+    return cfg
+
+
+def _run_gpu(cfg, N, dtype, ndtfast=6, dt=12.0):
+    from sailfish_amd.coupled3d import main3d_coupled
+    from sailfish_amd.model2d import build_model3d
+    ct, GRID, OCEAN, BOUNDARY = build_model3d(cfg, N=N, ndtfast=ndtfast, dt=dt, fix_q5=True, dtype=dtype)
+    st = main3d_coupled(ct, GRID, OCEAN, BOUNDARY, NSTEPS)
+    return ct, GRID, OCEAN, BOUNDARY, st
+
+
+@pytest.mark.parametrize("szI,szJ,N", [(40, 24, 6), (67, 33, 5)])
+def test_coupled_fp64_bitwise_against_oracle(szI, szJ, N):
+    import torch
+    from oracle import oracle_coupled as OC
+    from sailfish_amd.mod_constants import g as G
+
+    print(f"[case] coupled FP64 {szI}x{szJ}x{N}, {NSTEPS} baroclinic steps", file=sys.stderr, flush=True)
+    cfg = _cfg(szI, szJ, N, seed=szI + N)
+    ct, GRID, OCEAN, BOUNDARY, st = _run_gpu(cfg, N, torch.float64)
+    torch.cuda.synchronize()
+
+    host = GRID.host
+    o = OC.OCoupled(cfg, N, host, host["sc_r"], host["sc_w"], host["Cs_r"], host["Cs_w"], GRID.hc,
+                    BOUNDARY.zetaBC.bcIdxFieldIdx2, BOUNDARY.uvelBC.bcIdxFieldType,
+                    BOUNDARY.vvelBC.bcIdxFieldType, rdrg=3e-4, g=G)
+    m = o.m
+    for k in range(3):
+        m.u[k][:] = np.broadcast_to(cfg["u0"], (N + 1, szJ, szI)).ravel()
+        m.v[k][:] = np.broadcast_to(cfg["v0"], (N + 1, szJ, szI)).ravel()
+    m.AKv[:] = OCEAN.AKv.detach().cpu().numpy().ravel()
+    o.s.Zt_avg1[:] = np.asarray(cfg["zeta0"], float).ravel()
+    m.set_depth(o.s)
+    time = 0.0
+    for iic in range(NSTEPS):
+        o.baroclinic_step(iic, 0, ct.nfast, ct.dtfast, ct.dt, ct.weight1, ct.weight2, time)
+        time += ct.dt
+
+    get = lambda t: t.detach().cpu().numpy().ravel()
+    bad = []
+    for name in ("u", "v", "ru", "rv"):
+        if not bitwise_equal(get(getattr(OCEAN, name)[2]), getattr(m, name)[2]):
+            bad.append(name)
+    for name in ("W", "Huon", "Hvom"):
+        if not bitwise_equal(get(getattr(OCEAN, name)), getattr(m, name)):
+            bad.append(name)
+    for name, arr in (("z_w", GRID.z_w), ("z_r", GRID.z_r), ("Hz", GRID.Hz)):
+        if not bitwise_equal(get(arr), getattr(m, name)):
+            bad.append(name)
+    for name in ("zeta", "ubar", "vbar"):
+        for k in range(3):
+            if not bitwise_equal(get(getattr(OCEAN, f"{name}_t{k}")), o.s.v(name, k)):
+                bad.append(f"{name}_t{k}")
+    for name in ("Zt_avg1", "DU_avg1", "DV_avg1"):
+        if not bitwise_equal(get(getattr(OCEAN, name)), getattr(o.s, name)):
+            bad.append(name)
+    for a, b, name in ((st.DU_avg2, o.DU2, "DU_avg2"), (st.DV_avg2, o.DV2, "DV_avg2"),
+                       (st.rufrc, o.rufrc, "rufrc"), (st.rvfrc, o.rvfrc, "rvfrc")):
+        if not bitwise_equal(get(a), b):
+            bad.append(name)
+    for k in range(3):
+        if not (bitwise_equal(get(st.Fu[k]), o.Fu[k]) and bitwise_equal(get(st.Fv[k]), o.Fv[k])):
+            bad.append(f"F{k}")
+    assert not bad, bad
+    u = get(OCEAN.u_t2)
+    assert np.isfinite(u).all() and np.abs(u).max() > 1e-3
+    assert np.abs(get(st.Fu[0])).max() > 0      # the forcing is not trivially zero
+
+    # each interior column carries the averaged barotropic transport after the step:
+    # sum_K on_u Hz_u(K) u(K) = DU_avg1 (before the lateral boundary values are reset)
+    Hz = get(GRID.Hz).reshape(N, szJ, szI)
+    uu = u.reshape(N + 1, szJ, szI)
+    pn = cfg["pn"]
+    DU = get(OCEAN.DU_avg1).reshape(szJ, szI)
+    j, i = szJ // 2, szI // 2
+    DC = 2 / (pn[j, i - 1] + pn[j, i]) * 0.5 * (Hz[:, j, i] + Hz[:, j, i - 1])
+    assert (DC * uu[:N, j, i]).sum() == pytest.approx(DU[j, i], rel=1e-11, abs=1e-9)
+
+
+# FP32 bulk coupled form against FP64 after four baroclinic steps:
+# max |difference| / max |FP64 field|
+FP32_BOUNDS = {"zeta": 1e-4, "ubar": 1e-3, "vbar": 1e-3, "u": 1e-3, "v": 1e-3, "W": 2e-3, "DU_avg2": 1e-4}
+
+
+def test_coupled_fp32_against_fp64():
+    import torch
+    szI, szJ, N = 96, 64, 8
+    print(f"[case] coupled FP32 vs FP64 {szI}x{szJ}x{N}", file=sys.stderr, flush=True)
+    cfg = _cfg(szI, szJ, N, seed=31)
+    S2 = szI * szJ
+
+    def fields(O, st):
+        g = lambda t: t.detach().cpu().numpy().astype(np.float64)
+        return {"zeta": g(O.zeta_t2), "ubar": g(O.ubar_t2), "vbar": g(O.vbar_t2),
+                "u": g(O.u_t2).reshape(N + 1, S2)[:N], "v": g(O.v_t2).reshape(N + 1, S2)[:N],
+                "W": g(O.W).reshape(N + 1, S2), "DU_avg2": g(st.DU_avg2)}
+
+    *_, O64, _B, st64 = _run_gpu(cfg, N, torch.float64)
+    ref = fields(O64, st64)
+    *_, O32, _B, st32 = _run_gpu(cfg, N, torch.float32)
+    got = fields(O32, st32)
+    errs = {k: float(np.abs(got[k] - ref[k]).max() / np.abs(ref[k]).max()) for k in ref}
+    print("[coupled fp32 error vs fp64] " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), file=sys.stderr,
+          flush=True)
+    over = {k: v for k, v in errs.items() if not v <= FP32_BOUNDS[k]}
+    assert not over, (over, errs)

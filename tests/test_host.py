@@ -181,3 +181,18 @@ def test_c_abi_rejects_bad_arguments_without_a_gpu():
     assert lib.sf_closure_stress(None, None, 0, None, None, None, None) == -1 and "null" in err()
     lib.sf_rhs3d.argtypes = [P] * 11 + [C.c_int64, C.c_int32, C.c_int32, P]
     assert lib.sf_rhs3d(*([None] * 11), 0, 1, 1, None) == -1 and "null context" in err()
+
+
+def test_filtered_transport_weights_sum_to_one():
+    """DU_avg2 weights of the coupled substeps (step2d_LF_AM3.h.py:778-846): over the
+    nfast+1 substeps of a barotropic loop they sum to the filter's total weight, 1."""
+    from sailfish_amd.coupled3d import filtered_transport_weights
+    from sailfish_amd.mod_comptimes import CompTimes
+    from sailfish_amd.set_weights import set_weights
+    for ndtfast in (4, 6, 30, 40):
+        ct = CompTimes(dt=300.0, ndtfast=ndtfast)
+        set_weights(ct)
+        tot = sum(sum(filtered_transport_weights(ct.weight2, k)) for k in range(1, ct.nfast + 2))
+        assert abs(tot - 1.0) <= 1e-12
+        w2p, w2c = filtered_transport_weights(ct.weight2, 1)
+        assert abs(w2p + ct.weight2[1] / 12.0) <= 1e-17 and w2c == ct.weight2[0]

@@ -22,7 +22,7 @@ import pytest
 ROOT = Path(__file__).resolve().parents[1]
 ORACLE = ROOT / "oracle"
 PIN_FILES = ["tests/test_oracle2d.py", "tests/test_oracle3d.py", "tests/test_oracle_phys.py",
-             "tests/test_oracle_volcons.py"]
+             "tests/test_oracle_volcons.py", "tests/test_oracle_coupled.py"]
 
 # (name, source file, text to replace, replacement): the text occurs exactly once
 MUTANTS = [
@@ -130,13 +130,40 @@ MUTANTS = [
     ("set_zeta_one_slot", "sf_oracle3d.c",
      "        zeta2[f] = Zt[f];\n", "        zeta2[f] = zeta1[f];\n"),
     ("step3d_ab3_second_factor", "sf_oracle3d.c",
-     "else if (iic == ntfirst + 1) cff = 3.0 / 2.0;", "else if (iic == ntfirst + 1) cff = 1.0;"),
+     "S2_OF(g), N = g->N;\n    double cff;\n    if (iic == ntfirst) cff = 1.0;\n    else if (iic == ntfirst + 1) cff = 3.0 / 2.0;",
+     "S2_OF(g), N = g->N;\n    double cff;\n    if (iic == ntfirst) cff = 1.0;\n    else if (iic == ntfirst + 1) cff = 1.0;"),
     ("step3d_viscous_sign", "sf_oracle3d.c",
      "SD[K * S2 + f] = (((-lambda) * Dt) * AKVU(K)) / DZ(K + 1);",
      "SD[K * S2 + f] = ((lambda * Dt) * AKVU(K)) / DZ(K + 1);"),
     ("step3d_mean_correction_sign", "sf_oracle3d.c",
      "for (ptrdiff_t K = 0; K < N; K++) x[K * S2 + f] = x[K * S2 + f] - diff;",
      "for (ptrdiff_t K = 0; K < N; K++) x[K * S2 + f] = x[K * S2 + f] + diff;"),
+    # --- coupled split-explicit form (ROMS 2D/3D coupling) ---
+    ("cpl_ab3_middle_coefficient", "sf_oracle_phys.c",
+     "(16.0 / 12.0) * F1", "(15.0 / 12.0) * F1"),
+    ("cpl_ab2_coefficient", "sf_oracle_phys.c",
+     "if (ab == 1) return 1.5 * F - 0.5 * F1;", "if (ab == 1) return 1.5 * F - 0.5 * F;"),
+    ("cpl_forcing_sign", "sf_oracle_phys.c",
+     "const double F = cp->rufrc[f] - r;", "const double F = r - cp->rufrc[f];"),
+    ("cpl_corrector_without_forcing", "sf_oracle_phys.c",
+     "if (cp) r = r + cp->Fu[f];", "if (cp) r = r + 0.0 * cp->Fu[f];"),
+    ("cpl_avg2_corrector_state", "sf_oracle_phys.c",
+     "cp->DU2[f] = cp->DU2[f] + cp->w2c * DUon(p, zs, us, W, j, i);",
+     "cp->DU2[f] = cp->DU2[f] + cp->w2c * DUon(p, z1, u1, W, j, i);"),
+    ("cpl_avg2_not_reset", "sf_oracle_phys.c",
+     "const double a = first ? 0.0 : cp->DU2[f]", "const double a = cp->DU2[f]"),
+    ("cpl_rufrc_stress_sign", "sf_oracle3d.c",
+     "su = su + ((sustr[f] - rdrg * u[f]) * omu) * onu;", "su = su + ((sustr[f] + rdrg * u[f]) * omu) * onu;"),
+    ("cpl_rufrc_skips_bottom", "sf_oracle3d.c",
+     "for (ptrdiff_t K = 0; K < N; K++) su = su + ru[K * S2 + f];",
+     "for (ptrdiff_t K = 1; K < N; K++) su = su + ru[K * S2 + f];"),
+    ("cpl_huon_half_dropped", "sf_oracle3d.c",
+     "Hf[K * S2 + f] = 0.5 * (Hf[K * S2 + f] + x[K * S2 + f] * DC(K));",
+     "Hf[K * S2 + f] = (Hf[K * S2 + f] + x[K * S2 + f] * DC(K));"),
+    ("cpl_transport_avg1_for_avg2", "sf_oracle3d.c",
+     "FC = rD * (FC - Davg2[f]);", "FC = rD * (FC - Davg1[f]);"),
+    ("cpl_ubar_without_depth", "sf_oracle3d.c",
+     "xbar2[f] = rD * Davg1[f];", "xbar2[f] = Davg1[f];"),
 ]
 
 
