@@ -392,6 +392,18 @@ int sf_obc_volcons(const void *zeta, const void *h, const void *pm, const void *
                    int32_t szI, int32_t szJ, int32_t precision, int32_t flags, int32_t apply, double *work,
                    double *out, void *stream);
 
+/* sf_obc_volcons without the work argument and without any allocation: the block
+ * partials live in a static device array of 4096 doubles (one per device, held until
+ * the process ends), so sf_obc_volcons_work_size(szI, szJ) must be <= 4096, i.e. at
+ * most 524 032 open-boundary points; larger grids get SF_ERR_ARG.  Same kernels and
+ * results as sf_obc_volcons.  Calls on one device share that array: calls that overlap
+ * in time on one device (different streams, or host threads) race.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (HIP graph
+ * capture, tight serving loops). */
+int sf_obc_volcons_noalloc(const void *zeta, const void *h, const void *pm, const void *pn, void *ubar,
+                           void *vbar, int32_t szI, int32_t szJ, int32_t precision, int32_t flags, int32_t apply,
+                           double *out, void *stream);
+
 /* ---------------- learned subgrid momentum closure ---------------------------------- */
 
 /* The closure network (45 -> 64 GELU -> 64 GELU -> 2) trained and evaluated on matrix

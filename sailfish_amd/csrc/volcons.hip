@@ -13,6 +13,14 @@
 // result is reproducible run to run.
 #include "sf_internal.h"
 
+#include <mutex>
+
+// Work space of sf_obc_volcons_noalloc: the block partials of the two FP64 sums, in a
+// file-scope device array instead of caller or library allocations.  Each device holds
+// its own copy until the process ends, shared by every call on that device.
+static constexpr int kVolScratchDoubles = 4096;
+static __device__ __align__(256) double scratch[kVolScratchDoubles];
+
 namespace {
 
 constexpr int VB = 256;
@@ -153,6 +161,29 @@ int run_volcons(const void *zeta, const void *h, const void *pm, const void *pn,
     return sf::check_launch("sf_obc_volcons");
 }
 
+// Device address of `scratch` per device.  The array's name in host code is only a
+// stand-in for the symbol; its address on a device comes from hipGetSymbolAddress with
+// that device current, once per device.
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[64];
+
+int scratch_for_current_device(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return sf::hip_fail(e, "sf_obc_volcons_noalloc: hipGetDevice");
+    SF_REQUIRE(dev >= 0 && dev < 64, "sf_obc_volcons_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return sf::hip_fail(e, "sf_obc_volcons_noalloc: hipGetSymbolAddress");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return SF_OK;
+}
+
 }  // namespace
 
 extern "C" int sf_obc_volcons_work_size(int32_t szI, int32_t szJ)
@@ -174,4 +205,23 @@ extern "C" int sf_obc_volcons(const void *zeta, const void *h, const void *pm, c
         return run_volcons<float>(zeta, h, pm, pn, ubar, vbar, szI, szJ, flags, apply, work, out, stream);
     sf::set_error("sf_obc_volcons: unknown precision %d", precision);
     return SF_ERR_ARG;
+}
+
+extern "C" int sf_obc_volcons_noalloc(const void *zeta, const void *h, const void *pm, const void *pn, void *ubar,
+                                      void *vbar, int32_t szI, int32_t szJ, int32_t precision, int32_t flags,
+                                      int32_t apply, double *out, void *stream)
+{
+    SF_REQUIRE(zeta && h && pm && pn && ubar && vbar && out, "sf_obc_volcons_noalloc: null argument");
+    SF_REQUIRE(szI >= 4 && szJ >= 4, "sf_obc_volcons_noalloc: grid %dx%d too small", szI, szJ);
+    const long long n = 2LL * (szJ - 2) + 2LL * (szI - 2);
+    const long long need = 2 * ((n + VB - 1) / VB + 1);
+    SF_REQUIRE(need <= kVolScratchDoubles,
+               "sf_obc_volcons_noalloc: grid %dx%d needs %lld doubles of work space, the static maximum is %d "
+               "(at most %d boundary points); use sf_obc_volcons with a caller work buffer",
+               szI, szJ, need, kVolScratchDoubles, (kVolScratchDoubles / 2 - 1) * VB);
+    double *work = nullptr;
+    const int rc = scratch_for_current_device(&work);
+    if (rc != SF_OK) return rc;
+    // every partial slot of the launch is written before it is read: nothing to clear
+    return sf_obc_volcons(zeta, h, pm, pn, ubar, vbar, szI, szJ, precision, flags, apply, work, out, stream);
 }
