@@ -300,6 +300,31 @@ int sf_phys_substep_fused(const sf_phys_fields *fields, int32_t precision, doubl
  * when 64 x 8 tiles would give fewer than 1024 workgroups). */
 int sf_phys_fused_default_tile(int32_t precision, int32_t szI, int32_t szJ);
 
+/* ---------------- baroclinic terms: state equation, pressure gradient, tracers ------- */
+
+/* Linear equation of state (utility/test.in:2130-2160) on the szK levels of T and S:
+ * rho = ((R0 - 1000) - R0 Tcoef (T - T0)) + R0 Scoef (S - S0), the density anomaly over
+ * 1000 kg/m3.  Fields in `precision` (SF_FP64 / SF_FP32), arithmetic in FP64. */
+int sf_rho_eos(sf_ctx *ctx, int32_t precision, const void *T, const void *S, double R0, double T0, double S0,
+               double Tcoef, double Scoef, void *rho, void *stream);
+
+/* Baroclinic pressure gradient (the prsgrd call nonlinear/rhs.py:31-32 leaves commented
+ * out): the standard density Jacobian on s-levels (Song 1998; ROMS prsgrd31), ADDED to
+ * ru/rv (levels 0 .. szK-1) on the interior u points (i = 2 .. szI-2, j = 1 .. szJ-2) and
+ * v points (j = 2 .. szJ-2, i = 1 .. szI-2); rho from sf_rho_eos, z_r/Hz with szK levels,
+ * z_w with szK+1, g and rho0 the gravity and Boussinesq density.  Column sums in FP64. */
+int sf_prsgrd(sf_ctx *ctx, int32_t precision, const void *rho, const void *z_r, const void *z_w, const void *Hz,
+              double g, double rho0, void *ru, void *rv, void *stream);
+
+/* Tracer step (nonlinear/step3d_t.py, flux form): first-order upstream horizontal and
+ * vertical advection by the mass fluxes Huon/Hvom, forward over dt, with each cell's
+ * vertical volume flux formed from the horizontal ones and the thickness change
+ * Hz_old -> Hz_new (a uniform tracer stays uniform); interior columns i = 1 .. szI-2,
+ * j = 1 .. szJ-2, ghost columns copied from their interior neighbours.  t and tn (new)
+ * have szK levels and must not alias. */
+int sf_step3d_t(sf_ctx *ctx, int32_t precision, const void *t, void *tn, const void *Huon, const void *Hvom,
+                const void *Hz_old, const void *Hz_new, double dt, void *stream);
+
 /* ---------------- coupled split-explicit step (ROMS 2D/3D coupling) ---------------- */
 
 /* Forcing of the barotropic mode by the baroclinic step: rufrc = sum over the szK levels
@@ -348,6 +373,30 @@ int sf_step3d_uv_coupled(sf_ctx *ctx, int32_t precision, void *u, void *v, const
                          const double *DV_avg2, void *Huon, void *Hvom, const void *tmpU, const void *tmpV,
                          int32_t iic, int32_t ntfirst, double lambda, double Dt, void *SD, void *D, void *RHS,
                          void *stream);
+
+/* ---------------- open boundaries of the physics substep ---------------------------- */
+
+/* Open edges of a physics-mode basin (flags: SF_OBC_WEST/EAST/SOUTH/NORTH below; the
+ * other edges stay closed walls).  zeta_ext[e] / vel_ext[e]: external free surface and
+ * normal barotropic velocity along edge e (0 west, 1 east, 2 south, 3 north; west/east
+ * indexed by j, south/north by i, szJ or szI doubles; NULL = 0).  volcons != 0:
+ * obc_volcons + conserve_mass (sf_obc_volcons) on the open edges' velocities. */
+typedef struct sf_open_bc {
+    int32_t flags;
+    int32_t volcons;
+    const double *zeta_ext[4], *vel_ext[4];
+} sf_open_bc;
+
+/* sf_phys_substep (four launches) with open edges: after each stage, the ghost free
+ * surface of every open edge takes the implicit Chapman radiation value (nonlinear/zetabc.py,
+ * Chapman) and the normal velocity on the edge face the Flather value
+ * (nonlinear/barotropicVelocityBC.py:66-98).  volcons != 0: after each of the two
+ * velocity stages (predicted and new state) the volume constraint
+ * (nonlinear/obc_volcons.py:1-72, sf_obc_volcons) removes the net flow through the open
+ * edges, so the free-surface updates conserve the volume (volcons_work:
+ * sf_obc_volcons_work_size doubles; volcons_out: area, flux, ubar_xs of the last). */
+int sf_phys_substep_open(const sf_phys_fields *fields, const sf_open_bc *obc, int32_t precision, double dt,
+                         int32_t first, double cff1, double *volcons_work, double *volcons_out, void *stream);
 
 /* ---------------- ensembles of physics-mode basins ---------------------------------- */
 

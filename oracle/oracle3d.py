@@ -33,6 +33,9 @@ O.declare({
     "sfo_setLateralUVBCs": ([_G3, F64, P, P, P, P, P, P], C.c_int),
     "sfo_step3d_UV": ([_G3, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, C.c_int, C.c_int, F64, F64,
                        P, P, P], None),
+    "sfo_rho_eos": ([_G3, P, P, F64, F64, F64, F64, F64, P], None),
+    "sfo_prsgrd": ([_G3, P, P, P, P, F64, F64, P, P], None),
+    "sfo_step3d_t": ([_G3, P, P, P, P, P, P, F64], None),
 })
 
 

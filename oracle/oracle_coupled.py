@@ -57,7 +57,10 @@ def ab_order(n: int) -> int:
 class OCoupled:
     """Host copies of the coupled model's state (FP64, flat arrays)."""
 
-    def __init__(self, cfg, N, metrics, sc_r, sc_w, Cs_r, Cs_w, hc, BCfield, uType, vType, rdrg=3e-4, g=10.0):
+    def __init__(self, cfg, N, metrics, sc_r, sc_w, Cs_r, Cs_w, hc, BCfield, uType, vType, rdrg=3e-4, g=10.0,
+                 tracers=None):
+        """tracers: None, or dict(T=..., S=..., eos=(R0, T0, S0, Tcoef, Scoef), rho0=...) for the
+        baroclinic pressure gradient (after rhs3d) and the tracer step (after step3d_UV)."""
         self.ph = OP.OPhys(cfg, metrics["fomn"], rdrg=rdrg, g=g)
         self.s = self.ph.st
         fields = dict(h=cfg["h"], pm=cfg["pm"], pn=cfg["pn"], on_u=metrics["on_u"], om_v=metrics["om_v"])
@@ -70,6 +73,13 @@ class OCoupled:
         self.Fu = [np.zeros(S2) for _ in range(3)]
         self.Fv = [np.zeros(S2) for _ in range(3)]
         self.DU2, self.DV2 = np.zeros(S2), np.zeros(S2)
+        self.g = float(g)
+        self.tr = None
+        if tracers is not None:
+            n = N * S2
+            self.tr = dict(T=np.ascontiguousarray(np.broadcast_to(tracers["T"], (n,)).astype(float)),
+                           S=np.ascontiguousarray(np.broadcast_to(tracers["S"], (n,)).astype(float)),
+                           eos=tracers["eos"], rho0=float(tracers["rho0"]), rho=np.zeros(n))
 
     def substep(self, dt, first, w1, cp):
         v = self.s.v
@@ -92,6 +102,11 @@ class OCoupled:
         L.sfo_omega(m.cp, p(m.W), p(m.Huon), p(m.Hvom), p(m.z_w), p(m.BC))
         L.sfo_set_zeta(m.cp, p(s.v("zeta", 1)), p(s.v("zeta", 2)), p(s.Zt_avg1))
         m.rhs3d()
+        tr = self.tr
+        if tr is not None:
+            L.sfo_rho_eos(m.cp, p(tr["T"]), p(tr["S"]), *tr["eos"], p(tr["rho"]))
+            L.sfo_prsgrd(m.cp, p(tr["rho"]), p(m.z_r), p(m.z_w), p(m.Hz), self.g, tr["rho0"], p(m.ru[2]),
+                         p(m.rv[2]))
         st = self.ph.static
         L.sfo_couple_rufrc(m.cp, p(m.ru[2]), p(m.rv[2]), p(u2), p(v2), p(st["sustr"]), p(st["svstr"]), self.rdrg,
                            p(self.rufrc), p(self.rvfrc))
@@ -109,12 +124,18 @@ class OCoupled:
                             p(self.Fv[h1]), p(self.Fu[h2]), p(self.Fv[h2]), p(self.DU2), p(self.DV2), w2p, w2c,
                             ab_order(n))
             self.substep(dtfast, iif == 1, float(weight1[k1]), cp)
+        Hz_old = m.Hz.copy()
         m.set_depth(s)
         tmpU, tmpV = u2.copy(), v2.copy()
         L.sfo_step3d_uv_coupled(m.cp, p(u2), p(v2), p(m.ru[2]), p(m.rv[2]), p(s.v("ubar", 1)), p(s.v("vbar", 1)),
                                 p(s.v("ubar", 2)), p(s.v("vbar", 2)), p(m.Hz), p(m.AKv), p(m.z_r), p(s.DU_avg1),
                                 p(s.DV_avg1), p(self.DU2), p(self.DV2), p(m.Huon), p(m.Hvom), p(tmpU), p(tmpV),
                                 int(iic), int(ntfirst), 1.0, float(dt), p(m.eqSD), p(m.eqD), p(m.eqRHS))
+        if tr is not None:
+            for name in ("T", "S"):
+                new = np.zeros_like(tr[name])
+                L.sfo_step3d_t(m.cp, p(tr[name]), p(new), p(m.Huon), p(m.Hvom), p(Hz_old), p(m.Hz), float(dt))
+                tr[name][:] = new
         rc = L.sfo_setLateralUVBCs(m.cp, time, p(u2), p(v2), p(m.BC), p(m.BC), p(m.uType), p(m.vType))
         assert rc == 0, "gradient v boundaries are not supported"
         L.sfo_omega(m.cp, p(m.W), p(m.Huon), p(m.Hvom), p(m.z_w), p(m.BC))

@@ -18,7 +18,14 @@ class sfo_phys(C.Structure):
 
 
 _PP = C.POINTER(sfo_phys)
+
+
+class sfo_open(C.Structure):
+    _fields_ = [("flags", C.c_int), ("volcons", C.c_int), ("ext_z", P * 4), ("ext_u", P * 4), ("out", P)]
+
+
 O.declare({
+    "sfo_phys_substep_open": ([_PP, F64, C.c_int, F64] + [P] * 22 + [C.POINTER(sfo_open), P], None),
     "sfo_phys_substep": ([_PP, F64, C.c_int, F64] + [P] * 22, None),
     "sfo_phys_rhs_zeta": ([_PP, P, P, P, C.c_int, C.c_int], F64),
     "sfo_phys_rhs_u": ([_PP, P, P, P, P, C.c_int, C.c_int], F64),
@@ -65,6 +72,43 @@ class OPhys:
                                p(v("rvbar", 0)), p(v("rvbar", 1)),
                                p(self.scr[0]), p(self.scr[1]), p(self.scr[2]), p(self.scr[3]),
                                p(self.st.Zt_avg1), p(self.st.DU_avg1), p(self.st.DV_avg1))
+
+    def open_bc(self, flags, volcons=False, ext_z=None, ext_u=None):
+        """Open edges for substep_open (ext_z / ext_u: 4 per-edge arrays or None)."""
+        ob = sfo_open(int(flags), int(bool(volcons)))
+        self._ext = []
+        for e in range(4):
+            for arrs, field in ((ext_z, "ext_z"), (ext_u, "ext_u")):
+                a = None if arrs is None else arrs[e]
+                if a is not None:
+                    a = np.ascontiguousarray(np.asarray(a, float))
+                    self._ext.append(a)
+                    getattr(ob, field)[e] = O.ptr(a)
+        self.ob = ob
+        self.volcons_out = np.zeros(3)
+        return ob
+
+    def substep_open(self, dt, first, w1):
+        v = self.st.v
+        p = O.ptr
+        lib().sfo_phys_substep_open(C.byref(self.c), dt, int(first), w1,
+                                    p(v("zeta", 0)), p(v("zeta", 1)), p(v("zeta", 2)),
+                                    p(v("ubar", 0)), p(v("ubar", 1)), p(v("ubar", 2)),
+                                    p(v("vbar", 0)), p(v("vbar", 1)), p(v("vbar", 2)),
+                                    p(v("rzeta", 0)), p(v("rzeta", 1)), p(v("rubar", 0)), p(v("rubar", 1)),
+                                    p(v("rvbar", 0)), p(v("rvbar", 1)),
+                                    p(self.scr[0]), p(self.scr[1]), p(self.scr[2]), p(self.scr[3]),
+                                    p(self.st.Zt_avg1), p(self.st.DU_avg1), p(self.st.DV_avg1),
+                                    C.byref(self.ob), p(self.volcons_out))
+
+    def run_open(self, nsteps, dt, weight1, start_first=True, iif0=0):
+        iif = iif0
+        for n in range(int(nsteps)):
+            iif += 1
+            self.st.cycleTimes2D()
+            k = min(max(iif - 1, 0), len(weight1) - 1)
+            self.substep_open(dt, start_first and n == 0, float(weight1[k]))
+        return iif
 
     def run(self, nsteps, dt, weight1, start_first=True, iif0=0):
         iif = iif0

@@ -571,3 +571,132 @@ void sfo_step3d_uv_coupled(const sfo_grid3 *g, double *u, double *v, const doubl
         couple_column(g, 1, f, v, Hz, DV_avg1, DV_avg2, Hvom, vbar_t1, vbar_t2);
     }
 }
+
+/* ---------------------------------------------------------------------------
+ * Baroclinic terms: linear equation of state, baroclinic pressure gradient and the
+ * tracer step.  The reference names them but leaves them out (nonlinear/rhs.py:31-32
+ * "prsgrd()" commented out; nonlinear/step3d_t.py Fortran text; the linear state
+ * equation of utility/test.in:2130-2160).
+ *
+ * sfo_rho_eos: density anomaly over 1000 kg/m3 of the N levels,
+ *   rho = ((R0 - 1000) - (R0 Tcoef)(T - T0)) + (R0 Scoef)(S - S0).
+ * ------------------------------------------------------------------------- */
+void sfo_rho_eos(const sfo_grid3 *g, const double *T, const double *S, double R0, double T0, double S0,
+                 double Tcoef, double Scoef, double *rho)
+{
+    const ptrdiff_t n = (ptrdiff_t)g->N * S2_OF(g);
+    const double a = R0 - 1000.0, bt = R0 * Tcoef, bs = R0 * Scoef;
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t b = 0; b < n; b++) rho[b] = (a - bt * (T[b] - T0)) + bs * (S[b] - S0);
+}
+
+/* Baroclinic pressure gradient by the standard density Jacobian on s-levels (Song 1998,
+ * the second-order scheme of ROMS prsgrd31), added to ru/rv (m4/s2) of levels 0..N-1 on
+ * the interior u points (i = 2..L-1, j = 1..M-1) and v points (j = 2..M-1, i = 1..L-1).
+ * With fac1 = 0.5 g/rho0, fac2 = 1000 g/rho0, fac3 = 0.25 g/rho0 and the neighbour
+ * p = i-1 (u) or j-1 (v), from the top level N-1 down:
+ *   phi(N-1) = fac1 (rho - rho_p)((z_w(N) - z_r) + (z_w(N) - z_r)_p)
+ *              + (fac2 + fac1 (rho + rho_p)) (z_w(N) - z_w(N)_p)
+ *   phi(k)   = phi(k+1) + fac3 (C1 C3 - C2 C4), with
+ *     gamma = 0.125 (dz_r(k) + dz_r(k+1)) (z_r(k+1) - z_r(k) - (z_r(k+1) - z_r(k))_p) /
+ *             ((z_r(k+1) - z_r(k)) (z_r(k+1) - z_r(k))_p),   dz_r(k) = z_r(k) - z_r(k)_p,
+ *     C1 = (1+gamma) drho(k+1) + (1-gamma) drho(k),   C2 = rho(k+1) + rho_p(k+1) - rho(k) - rho_p(k),
+ *     C3 = z_r(k+1) + z_r_p(k+1) - z_r(k) - z_r_p(k), C4 = (1+gamma) dz_r(k+1) + (1-gamma) dz_r(k)
+ *   ru(k) += -0.5 (Hz(k) + Hz_p(k)) phi(k) on_u      (on_u = 2/(pn+pn_p); v: om_v = 2/(pm+pm_p)). */
+static void pgf_column(const sfo_grid3 *g, int vdir, ptrdiff_t f, const double *rho, const double *z_r,
+                       const double *z_w, const double *Hz, double grav, double rho0, double *r)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N, W = g->szI;
+    const int i = (int)(f % W), j = (int)(f / W), L = g->szI - 1, M = g->szJ - 1;
+    if (!vdir && !(i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1)) return;
+    if (vdir && !(j >= 2 && j <= M - 1 && i >= 1 && i <= L - 1)) return;
+    const ptrdiff_t p = vdir ? f - W : f - 1;
+    const double met = vdir ? 2.0 / (g->pm[p] + g->pm[f]) : 2.0 / (g->pn[p] + g->pn[f]);
+    const double fac1 = 0.5 * grav / rho0, fac2 = 1000.0 * grav / rho0, fac3 = 0.25 * grav / rho0;
+#define R(k, x) rho[(ptrdiff_t)(k) * S2 + (x)]
+#define ZR(k, x) z_r[(ptrdiff_t)(k) * S2 + (x)]
+#define ZW(k, x) z_w[(ptrdiff_t)(k) * S2 + (x)]
+    ptrdiff_t k = N - 1;
+    const double c1 = (ZW(N, f) - ZR(k, f)) + (ZW(N, p) - ZR(k, p));
+    double phi = (fac1 * (R(k, f) - R(k, p))) * c1;
+    phi = phi + (fac2 + fac1 * (R(k, f) + R(k, p))) * (ZW(N, f) - ZW(N, p));
+    r[k * S2 + f] = r[k * S2 + f] + ((-0.5 * (Hz[k * S2 + f] + Hz[k * S2 + p])) * phi) * met;
+    for (k = N - 2; k >= 0; k--) {
+        const double dzf = ZR(k + 1, f) - ZR(k, f), dzp = ZR(k + 1, p) - ZR(k, p);
+        const double a2 = (ZR(k, f) - ZR(k, p)) + (ZR(k + 1, f) - ZR(k + 1, p));
+        const double a3 = dzf - dzp;
+        const double gam = ((0.125 * a2) * a3) / (dzf * dzp);
+        const double C1 = (1.0 + gam) * (R(k + 1, f) - R(k + 1, p)) + (1.0 - gam) * (R(k, f) - R(k, p));
+        const double C2 = (R(k + 1, f) + R(k + 1, p)) - (R(k, f) + R(k, p));
+        const double C3 = (ZR(k + 1, f) + ZR(k + 1, p)) - (ZR(k, f) + ZR(k, p));
+        const double C4 = (1.0 + gam) * (ZR(k + 1, f) - ZR(k + 1, p)) + (1.0 - gam) * (ZR(k, f) - ZR(k, p));
+        phi = phi + fac3 * (C1 * C3 - C2 * C4);
+        r[k * S2 + f] = r[k * S2 + f] + ((-0.5 * (Hz[k * S2 + f] + Hz[k * S2 + p])) * phi) * met;
+    }
+#undef R
+#undef ZR
+#undef ZW
+}
+
+void sfo_prsgrd(const sfo_grid3 *g, const double *rho, const double *z_r, const double *z_w, const double *Hz,
+                double grav, double rho0, double *ru, double *rv)
+{
+    const ptrdiff_t S2 = S2_OF(g);
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        pgf_column(g, 0, f, rho, z_r, z_w, Hz, grav, rho0, ru);
+        pgf_column(g, 1, f, rho, z_r, z_w, Hz, grav, rho0, rv);
+    }
+}
+
+/* Tracer step (nonlinear/step3d_t.py, flux form; first-order upstream fluxes, forward
+ * in time): on every interior rho column (i = 1..L-1, j = 1..M-1), from the bottom up,
+ *   Dh(k) = pm pn ((Huon(i+1) - Huon(i)) + (Hvom(j+1) - Hvom(j)))        [level k]
+ *   Wt(k) = Wt(k-1) - Dh(k) - (Hz_new(k) - Hz_old(k)) / dt,  Wt(-1) = 0  [volume flux, top of k]
+ *   FX(i) = Huon(i) * (Huon(i) > 0 ? t(i-1) : t(i)), FE likewise, FC(k) = Wt(k) * (Wt(k) > 0 ? t(k) : t(k+1))
+ *   t_new(k) = (Hz_old(k) t(k) - dt (pm pn ((FX(i+1) - FX(i)) + (FE(j+1) - FE(j))) + FC(k) - FC(k-1)))
+ *              / Hz_new(k),   FC(-1) = 0,  FC(N-1) = Wt(N-1) t(N-1)
+ * The vertical flux is formed from the same horizontal volume fluxes and thickness change,
+ * so a uniform tracer stays uniform; its value at the surface is whatever the column's
+ * volume budget leaves.  Ghost columns copy their interior neighbour (zero gradient). */
+void sfo_step3d_t(const sfo_grid3 *g, const double *t, double *tn, const double *Huon, const double *Hvom,
+                  const double *Hz_old, const double *Hz_new, double dt)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N, W = g->szI;
+    const int L = g->szI - 1, M = g->szJ - 1;
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t f = 0; f < S2; f++) {
+        const int i = (int)(f % W), j = (int)(f / W);
+        if (!(i >= 1 && i <= L - 1 && j >= 1 && j <= M - 1)) continue;
+        const double mn = g->pm[f] * g->pn[f];
+        double Wlow = 0.0, FClow = 0.0;
+        for (ptrdiff_t k = 0; k < N; k++) {
+            const ptrdiff_t b = k * S2 + f;
+            const double hu0 = Huon[b], hu1 = Huon[b + 1], hv0 = Hvom[b], hv1 = Hvom[b + W];
+            const double Dh = mn * ((hu1 - hu0) + (hv1 - hv0));
+            const double Wk = (Wlow - Dh) - (Hz_new[b] - Hz_old[b]) / dt;
+            const double FX0 = hu0 * (hu0 > 0.0 ? t[b - 1] : t[b]);
+            const double FX1 = hu1 * (hu1 > 0.0 ? t[b] : t[b + 1]);
+            const double FE0 = hv0 * (hv0 > 0.0 ? t[b - W] : t[b]);
+            const double FE1 = hv1 * (hv1 > 0.0 ? t[b] : t[b + W]);
+            const double FCk = k < N - 1 ? Wk * (Wk > 0.0 ? t[b] : t[b + S2]) : Wk * t[b];
+            const double div = mn * ((FX1 - FX0) + (FE1 - FE0)) + (FCk - FClow);
+            tn[b] = (Hz_old[b] * t[b] - dt * div) / Hz_new[b];
+            Wlow = Wk;
+            FClow = FCk;
+        }
+    }
+    /* ghost columns: zero gradient (rows, then columns) */
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t k = 0; k < N; k++) {
+        for (int i = 1; i <= L - 1; i++) {
+            tn[k * S2 + i] = tn[k * S2 + W + i];
+            tn[k * S2 + (ptrdiff_t)M * W + i] = tn[k * S2 + (ptrdiff_t)(M - 1) * W + i];
+        }
+        for (int j = 0; j <= M; j++) {
+            const int jj = j < 1 ? 1 : (j > M - 1 ? M - 1 : j);
+            tn[k * S2 + (ptrdiff_t)j * W] = tn[k * S2 + (ptrdiff_t)jj * W + 1];
+            tn[k * S2 + (ptrdiff_t)j * W + L] = tn[k * S2 + (ptrdiff_t)jj * W + L - 1];
+        }
+    }
+}

@@ -155,6 +155,74 @@ void sfo_phys_bc(const sfo_phys *p, double *z, double *u, double *v)
     }
 }
 
+/* Open boundaries (nonlinear/zetabc.py Chapman sketch, nonlinear/barotropicVelocityBC.py:66-98
+ * Flather, as in ROMS zetabc/u2dbc/v2dbc): edges whose bit is set in flags (1 west,
+ * 2 east, 4 south, 8 north) are open, the others stay closed walls.  After a stage has
+ * written the new free surface zn (with closed-wall ghosts), every open edge's ghost
+ * rho point takes the implicit Chapman value
+ *   zn(g) = (zb(g) + Ce zn(i)) / (1 + Ce),  Ce = dt_s 0.5 (m(g)+m(i)) sqrt(g0 (h(g) + zb(g))),
+ * (i the first interior point, zb the level the stage starts from, dt_s its time span,
+ * m = pm across west/east edges and pn across south/north ones).  After the stage's
+ * momentum, the normal velocity on the edge's face takes the Flather value from the new
+ * surface on both sides of the face,
+ *   west/south: u = ext_u - sqrt(g0/D) (0.5 (zn(g)+zn(i)) - ext_z),  east/north: + sqrt(g0/D)(...),
+ * D = 0.5 ((h(g)+zn(g)) + (h(i)+zn(i))), with its two ghost-row copies (zero gradient
+ * along the edge).  ext_z / ext_u: external free surface and normal velocity per edge
+ * (west/east indexed by j, south/north by i; NULL = 0). */
+typedef struct sfo_open {
+    int flags;
+    int volcons;
+    const double *ext_z[4], *ext_u[4];
+    double *out;    /* area, flux, ubar_xs of the corrector's volume constraint */
+} sfo_open;
+
+static inline double ext_at(const double *a, int k) { return a ? a[k] : 0.0; }
+
+static void open_zeta(const sfo_phys *p, const sfo_open *ob, double *zn, const double *zb, double dts)
+{
+    const int W = p->szI, L = p->szI - 1, M = p->szJ - 1;
+    for (int e = 0; e < 4; e++) {
+        if (!(ob->flags & (1 << e))) continue;
+        const int n = e < 2 ? M - 1 : L - 1;
+        for (int b = 1; b <= n; b++) {
+            ptrdiff_t fg, fi;
+            if (e == 0) { fg = IDX(b, 0); fi = IDX(b, 1); }
+            else if (e == 1) { fg = IDX(b, L); fi = IDX(b, L - 1); }
+            else if (e == 2) { fg = IDX(0, b); fi = IDX(1, b); }
+            else { fg = IDX(M, b); fi = IDX(M - 1, b); }
+            const double m = e < 2 ? 0.5 * (p->pm[fg] + p->pm[fi]) : 0.5 * (p->pn[fg] + p->pn[fi]);
+            const double Ce = (dts * m) * sqrt(p->g * (p->h[fg] + zb[fg]));
+            zn[fg] = (zb[fg] + Ce * zn[fi]) / (1.0 + Ce);
+        }
+    }
+}
+
+static void open_vel(const sfo_phys *p, const sfo_open *ob, const double *zn, double *un, double *vn)
+{
+    const int W = p->szI, L = p->szI - 1, M = p->szJ - 1;
+    for (int e = 0; e < 4; e++) {
+        if (!(ob->flags & (1 << e))) continue;
+        const int n = e < 2 ? M - 1 : L - 1;
+        double *x = e < 2 ? un : vn;
+        for (int b = 1; b <= n; b++) {
+            ptrdiff_t fg, fi, fu;
+            if (e == 0) { fg = IDX(b, 0); fi = IDX(b, 1); fu = IDX(b, 1); }
+            else if (e == 1) { fg = IDX(b, L); fi = IDX(b, L - 1); fu = IDX(b, L); }
+            else if (e == 2) { fg = IDX(0, b); fi = IDX(1, b); fu = IDX(1, b); }
+            else { fg = IDX(M, b); fi = IDX(M - 1, b); fu = IDX(M, b); }
+            const double D = 0.5 * ((p->h[fg] + zn[fg]) + (p->h[fi] + zn[fi]));
+            const double c = sqrt(p->g / D);
+            const double dz = 0.5 * (zn[fg] + zn[fi]) - ext_at(ob->ext_z[e], b);
+            x[fu] = (e == 0 || e == 2) ? ext_at(ob->ext_u[e], b) - c * dz : ext_at(ob->ext_u[e], b) + c * dz;
+        }
+        /* ghost rows / columns of the edge's face: zero gradient along the edge */
+        if (e == 0) { un[IDX(0, 1)] = un[IDX(1, 1)]; un[IDX(M, 1)] = un[IDX(M - 1, 1)]; }
+        else if (e == 1) { un[IDX(0, L)] = un[IDX(1, L)]; un[IDX(M, L)] = un[IDX(M - 1, L)]; }
+        else if (e == 2) { vn[IDX(1, 0)] = vn[IDX(1, 1)]; vn[IDX(1, L)] = vn[IDX(1, L - 1)]; }
+        else { vn[IDX(M, 0)] = vn[IDX(M, 1)]; vn[IDX(M, L)] = vn[IDX(M, L - 1)]; }
+    }
+}
+
 /* Coupling of a substep with the baroclinic step (ROMS split-explicit form,
  * obsolete/step2d_LF_AM3.h.py:2377-2590 and :778-846):
  *  - the first substep of a baroclinic step (first != 0) turns the vertical integral
@@ -185,7 +253,8 @@ static void phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
                          const double *z0, const double *z1, double *z2, const double *u0, const double *u1,
                          double *u2, const double *v0, const double *v1, double *v2, const double *rz0, double *rz1,
                          const double *ru0, double *ru1, const double *rv0, double *rv1, double *zs, double *us,
-                         double *vs, double *gz, double *Zt, double *DUa, double *DVa, const sfo_couple *cp);
+                         double *vs, double *gz, double *Zt, double *DUa, double *DVa, const sfo_couple *cp,
+                         const sfo_open *ob);
 
 /* One substep.  Slots after the caller's rotation: index 0 = n-1, 1 = n, 2 = new.
  * zs/us/vs: predicted-state scratch (S2 each).  first != 0: forward-Euler
@@ -198,7 +267,7 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
                       double *vs, double *gz, double *Zt, double *DUa, double *DVa)
 {
     phys_substep(p, dt, first, cff1w, z0, z1, z2, u0, u1, u2, v0, v1, v2, rz0, rz1, ru0, ru1, rv0, rv1, zs, us,
-                 vs, gz, Zt, DUa, DVa, NULL);
+                 vs, gz, Zt, DUa, DVa, NULL, NULL);
 }
 
 /* The same substep coupled with the baroclinic step (sfo_couple above). */
@@ -210,14 +279,35 @@ void sfo_phys_substep_coupled(const sfo_phys *p, double dt, int first, double cf
                               const sfo_couple *cp)
 {
     phys_substep(p, dt, first, cff1w, z0, z1, z2, u0, u1, u2, v0, v1, v2, rz0, rz1, ru0, ru1, rv0, rv1, zs, us,
-                 vs, gz, Zt, DUa, DVa, cp);
+                 vs, gz, Zt, DUa, DVa, cp, NULL);
+}
+
+void sfo_obc_volcons(int szI, int szJ, const double *z, const double *h, const double *pm, const double *pn,
+                     double *u, double *v, int flags, int apply, double *out);
+
+/* The substep with open edges (sfo_open above); volcons != 0 applies obc_volcons +
+ * conserve_mass (sfo_obc_volcons) to the velocities of each stage after their open-edge
+ * values (the predicted state, then the new one), so every flux the free-surface
+ * updates use has no net flow through the open edges and the volume is conserved. */
+void sfo_phys_substep_open(const sfo_phys *p, double dt, int first, double cff1w,
+                           const double *z0, const double *z1, double *z2, const double *u0, const double *u1,
+                           double *u2, const double *v0, const double *v1, double *v2, const double *rz0,
+                           double *rz1, const double *ru0, double *ru1, const double *rv0, double *rv1,
+                           double *zs, double *us, double *vs, double *gz, double *Zt, double *DUa, double *DVa,
+                           const sfo_open *ob, double *volcons_out)
+{
+    sfo_open o = *ob;
+    o.out = volcons_out;
+    phys_substep(p, dt, first, cff1w, z0, z1, z2, u0, u1, u2, v0, v1, v2, rz0, rz1, ru0, ru1, rv0, rv1, zs, us,
+                 vs, gz, Zt, DUa, DVa, NULL, &o);
 }
 
 static void phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
                          const double *z0, const double *z1, double *z2, const double *u0, const double *u1,
                          double *u2, const double *v0, const double *v1, double *v2, const double *rz0, double *rz1,
                          const double *ru0, double *ru1, const double *rv0, double *rv1, double *zs, double *us,
-                         double *vs, double *gz, double *Zt, double *DUa, double *DVa, const sfo_couple *cp)
+                         double *vs, double *gz, double *Zt, double *DUa, double *DVa, const sfo_couple *cp,
+                         const sfo_open *ob)
 {
     const int W = p->szI, L = p->szI - 1, M = p->szJ - 1;
     const ptrdiff_t S2 = (ptrdiff_t)W * p->szJ;
@@ -241,6 +331,7 @@ static void phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
     memcpy(us, u1, sizeof(double) * S2);
     memcpy(vs, v1, sizeof(double) * S2);
     sfo_phys_bc(p, zs, us, vs);   /* zeta ghosts of the predicted surface */
+    if (ob) open_zeta(p, ob, zs, first ? z1 : z0, first ? dt : 2.0 * dt);
 #pragma omp parallel for schedule(static)
     for (ptrdiff_t f = 0; f < S2; f++)
         gz[f] = first ? 0.5 * (z1[f] + zs[f]) : cff5 * z1[f] + cff4 * (z0[f] + zs[f]);
@@ -286,6 +377,13 @@ static void phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
             vs[f] = (vb[f] * (Dat(p, zb, f) + Dat(p, zb, f - W)) + (cff * c1) * r) * fac;
         }
     sfo_phys_bc(p, zs, us, vs);
+    if (ob) {
+        /* the closed-wall pass above rewrote the ghost surface: restore the open edges' */
+        open_zeta(p, ob, zs, first ? z1 : z0, first ? dt : 2.0 * dt);
+        open_vel(p, ob, zs, us, vs);
+        double tmp[3];
+        if (ob->volcons) sfo_obc_volcons(p->szI, p->szJ, zs, p->h, p->pm, p->pn, us, vs, ob->flags, 1, tmp);
+    }
     /* ---- corrector: zeta ---- */
     const double a1 = dt * 5.0 / 12.0, a2 = dt * 8.0 / 12.0, a3 = dt * 1.0 / 12.0;
     memcpy(z2, z1, sizeof(double) * S2);
@@ -309,6 +407,7 @@ static void phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
     memcpy(u2, u1, sizeof(double) * S2);
     memcpy(v2, v1, sizeof(double) * S2);
     sfo_phys_bc(p, z2, u2, v2);
+    if (ob) open_zeta(p, ob, z2, z1, dt);
     const double cf4 = 2.0 / 5.0, cf5 = 1.0 - cf4;
 #pragma omp parallel for schedule(static)
     for (ptrdiff_t f = 0; f < S2; f++) gz[f] = cf5 * z2[f] + cf4 * zs[f];
@@ -337,6 +436,11 @@ static void phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
             v2[f] = (v1[f] * (Dat(p, z1, f) + Dat(p, z1, f - W)) + cff * tend) * fac;
         }
     sfo_phys_bc(p, z2, u2, v2);
+    if (ob) {
+        open_zeta(p, ob, z2, z1, dt);
+        open_vel(p, ob, z2, u2, v2);
+        if (ob->volcons) sfo_obc_volcons(p->szI, p->szJ, z2, p->h, p->pm, p->pn, u2, v2, ob->flags, 1, ob->out);
+    }
 }
 
 /* obc_volcons + conserve_mass (nonlinear/obc_volcons.py:1-72) in FP64: sequential sums

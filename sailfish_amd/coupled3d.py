@@ -71,7 +71,7 @@ class CoupledState:
     (this step and the two before it, rotated per baroclinic step), DU_avg2/DV_avg2,
     the FP64 stresses, and the physics substep in its four-launch form."""
 
-    def __init__(self, GRID, OCEAN, rdrg: float = 3e-4, g: float | None = None):
+    def __init__(self, GRID, OCEAN, rdrg: float = 3e-4, g: float | None = None, tracers=None):
         _lib.require_gpu()
         self.lib = _bind()
         dev, S2 = GRID.device, GRID.szI * GRID.szJ
@@ -89,6 +89,9 @@ class CoupledState:
         self.stepper = PhysStepper(GRID, OCEAN, rdrg=rdrg, fused=False, **kw)
         from .main3d import Main3dState
         self.m3 = Main3dState(GRID, OCEAN)
+        # baroclinic.Tracers: pressure gradient of T, S added to the 3D right-hand side and
+        # a tracer step per baroclinic step (None: the reference's terms only)
+        self.tracers = tracers
         OCEAN.extra_state = {"DU_avg2": self.DU_avg2, "DV_avg2": self.DV_avg2,
                              **{f"Fu{k}": t for k, t in enumerate(self.Fu)},
                              **{f"Fv{k}": t for k, t in enumerate(self.Fv)}}
@@ -120,6 +123,8 @@ def coupled_step(compTimes, GRID, OCEAN, BOUNDARY, st: CoupledState, history=Non
     ops.omega(None, None, (W, OCEAN.u_t2, OCEAN.v_t2, Huon, Hvom, z_w, BC), ctx=c)
     ops.set_zeta(None, None, (OCEAN.zeta_t1, OCEAN.zeta_t2, OCEAN.Zt_avg1), ctx=c)
     rhs3d(GRID, OCEAN, BOUNDARY)
+    if st.tracers is not None:
+        st.tracers.pressure_gradient(GRID, OCEAN)
     _lib.check(lib.sf_couple_rufrc(c.handle, prec, OCEAN.ru_t2.data_ptr(), OCEAN.rv_t2.data_ptr(),
                                    OCEAN.u_t2.data_ptr(), OCEAN.v_t2.data_ptr(), st.sustr.data_ptr(),
                                    st.svstr.data_ptr(), st.rdrg, st.rufrc.data_ptr(), st.rvfrc.data_ptr(), stream),
@@ -139,6 +144,8 @@ def coupled_step(compTimes, GRID, OCEAN, BOUNDARY, st: CoupledState, history=Non
                                                stream), "sf_phys_substep_coupled")
     if history is not None:
         history.maybe_write(compTimes, OCEAN)
+    if st.tracers is not None:
+        st.tracers.keep_thickness(GRID)
     initial_depths(GRID, OCEAN)
     m3 = st.m3
     m3.tmpU.copy_(OCEAN.u_t2)
@@ -151,6 +158,8 @@ def coupled_step(compTimes, GRID, OCEAN, BOUNDARY, st: CoupledState, history=Non
                                         q(m3.tmpU), q(m3.tmpV), int(compTimes.iic), int(compTimes.ntfirst), 1.0,
                                         float(compTimes.dt), q(m3.eqSD), q(m3.eqD), q(m3.eqRHS), stream),
                "sf_step3d_uv_coupled")
+    if st.tracers is not None:
+        st.tracers.step(GRID, OCEAN, compTimes.dt)
     ops.setLateralUVBCs(None, None, (compTimes.time, OCEAN.u_t2, OCEAN.v_t2, dm["uFieldIdx2"], dm["vFieldIdx2"],
                                      dm["uFieldType"], dm["vFieldType"]), ctx=c)
     ops.omega(None, None, (W, OCEAN.u_t2, OCEAN.v_t2, Huon, Hvom, z_w, BC), ctx=c)
@@ -158,11 +167,20 @@ def coupled_step(compTimes, GRID, OCEAN, BOUNDARY, st: CoupledState, history=Non
 
 
 def main3d_coupled(compTimes, GRID, OCEAN, BOUNDARY, nsteps: int, st: CoupledState | None = None,
-                   rdrg: float = 3e-4, history=None) -> CoupledState:
-    """``nsteps`` coupled baroclinic steps after the initial set_depth (main3d.py:62-63)."""
+                   rdrg: float = 3e-4, history=None, tracers: bool | dict = False) -> CoupledState:
+    """``nsteps`` coupled baroclinic steps after the initial set_depth (main3d.py:62-63).
+    ``tracers``: True (or a dict of baroclinic.Tracers arguments, e.g. T=...) adds the
+    baroclinic pressure gradient and the tracer step; the default temperature is the
+    3D workload's stratification 4 + 16 exp(z/500) C at the initial depths."""
     if st is None:
-        st = CoupledState(GRID, OCEAN, rdrg=rdrg)
         initial_depths(GRID, OCEAN)
+        tr = None
+        if tracers:
+            from .baroclinic import Tracers, stratification
+            kw = dict(tracers) if isinstance(tracers, dict) else {}
+            kw.setdefault("T", stratification(GRID))
+            tr = Tracers(GRID, OCEAN, **kw)
+        st = CoupledState(GRID, OCEAN, rdrg=rdrg, tracers=tr)
     for _ in range(int(nsteps)):
         coupled_step(compTimes, GRID, OCEAN, BOUNDARY, st, history=history)
     return st

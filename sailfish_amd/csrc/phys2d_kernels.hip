@@ -338,9 +338,87 @@ __global__ __launch_bounds__(256) void k_phys_corr_uv(Sub<T> a)
     a.v2[f] = v;
 }
 
+// ---- open edges (sf_phys_substep_open) ----------------------------------------------
+// Chapman free surface on the ghost rho points and Flather normal velocity on the edge
+// faces of the open edges (flags: SF_OBC_WEST/EAST/SOUTH/NORTH), applied after the stage
+// kernels have written closed-wall values (nonlinear/zetabc.py, the Chapman sketch, and
+// nonlinear/barotropicVelocityBC.py:66-98, Flather; ROMS zetabc/u2dbc/v2dbc):
+//   ghost  zn(g) = (zb(g) + Ce zn(i)) / (1 + Ce),  Ce = dt_s 0.5 (m(g)+m(i)) sqrt(g (h(g)+zb(g)))
+//   face   u = ext_u -/+ sqrt(g/D) (0.5 (zn(g)+zn(i)) - ext_z)   (- west/south, + east/north)
+// zb: the level the stage starts from, dt_s its time span, zn the stage's new surface.
+// Edge points b = 1 .. n (n = szJ-2 for west/east, szI-2 for south/north); the threads of
+// b = 1 and b = n also write the face value into the two ghost rows (zero gradient).
+struct Open {
+    int flags;
+    const double *ez[4], *eu[4];
+    int volcons;              // obc_volcons + conserve_mass after each velocity stage
+    double *vc_work, *vc_out;
+};
+
+__device__ __forceinline__ void edge_points(int e, int b, int W, int L, int M, long long &fg, long long &fi,
+                                            long long &fu)
+{
+    if (e == 0) { fg = (long long)b * W; fi = fg + 1; fu = fi; }
+    else if (e == 1) { fg = (long long)b * W + L; fi = fg - 1; fu = fg; }
+    else if (e == 2) { fg = b; fi = (long long)W + b; fu = fi; }
+    else { fg = (long long)M * W + b; fi = fg - W; fu = fg; }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_open_zeta(Sub<T> a, Open ob, int corrector)
+{
+    const int e = blockIdx.y;
+    if (!(ob.flags & (1 << e))) return;
+    const int W = a.p.szI, L = a.p.szI - 1, M = a.p.szJ - 1;
+    const int n = e < 2 ? M - 1 : L - 1;
+    const int b = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b > n) return;
+    long long fg, fi, fu;
+    edge_points(e, b, W, L, M, fg, fi, fu);
+    T *zn = corrector ? a.z2 : a.zs;
+    const T *zb = corrector ? a.z1 : (a.first ? a.z1 : a.z0);
+    const T dts = corrector ? a.dt : (a.first ? a.dt : T(2) * a.dt);
+    const T m = e < 2 ? T(0.5) * (a.p.pm[fg] + a.p.pm[fi]) : T(0.5) * (a.p.pn[fg] + a.p.pn[fi]);
+    const T Ce = (dts * m) * sqrt(a.p.g * (a.p.h[fg] + zb[fg]));
+    const T z = (zb[fg] + Ce * zn[fi]) / (T(1) + Ce);
+    zn[fg] = z;
+    if (corrector) {
+        const T cf4 = T(2.0 / 5.0), cf5 = T(1) - cf4;
+        a.gz[fg] = cf5 * z + cf4 * a.zs[fg];
+    } else {
+        const T cff4 = T(4.0 / 25.0), cff5 = T(1) - T(2) * cff4;
+        a.gz[fg] = a.first ? T(0.5) * (a.z1[fg] + z) : cff5 * a.z1[fg] + cff4 * (a.z0[fg] + z);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_open_vel(Sub<T> a, Open ob, int corrector)
+{
+    const int e = blockIdx.y;
+    if (!(ob.flags & (1 << e))) return;
+    const int W = a.p.szI, L = a.p.szI - 1, M = a.p.szJ - 1;
+    const int n = e < 2 ? M - 1 : L - 1;
+    const int b = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b > n) return;
+    long long fg, fi, fu;
+    edge_points(e, b, W, L, M, fg, fi, fu);
+    const T *zn = corrector ? a.z2 : a.zs;
+    T *x = e < 2 ? (corrector ? a.u2 : a.us) : (corrector ? a.v2 : a.vs);
+    const T D = T(0.5) * ((a.p.h[fg] + zn[fg]) + (a.p.h[fi] + zn[fi]));
+    const T c = sqrt(a.p.g / D);
+    const T ez = ob.ez[e] ? T(ob.ez[e][b]) : T(0), eu = ob.eu[e] ? T(ob.eu[e][b]) : T(0);
+    const T dz = T(0.5) * (zn[fg] + zn[fi]) - ez;
+    const T val = (e == 0 || e == 2) ? eu - c * dz : eu + c * dz;
+    x[fu] = val;
+    // ghost rows (west/east) or columns (south/north) of the face
+    const long long step = e < 2 ? W : 1;
+    if (b == 1) x[fu - step] = val;
+    if (b == n) x[fu + step] = val;
+}
+
 template <typename T>
 int launch_phys(const sf_phys_fields *x, double dt, int first, double w1, void *stream,
-                const sf_couple_fields *cp = nullptr)
+                const sf_couple_fields *cp = nullptr, const Open *ob = nullptr)
 {
     Sub<T> a;
     a.c = Cpl{};
@@ -378,10 +456,30 @@ int launch_phys(const sf_phys_fields *x, double dt, int first, double w1, void *
     const long long S2 = (long long)x->szI * x->szJ;
     const dim3 grid(sf::grid1d(S2)), block(256);
     hipStream_t s = sf::as_stream(stream);
+    const int nedge = (x->szI > x->szJ ? x->szI : x->szJ);
+    const dim3 egrid(sf::grid1d(nedge), 4);
     hipLaunchKernelGGL(k_phys_pred_zeta<T>, grid, block, 0, s, a);
+    if (ob) hipLaunchKernelGGL(k_open_zeta<T>, egrid, block, 0, s, a, *ob, 0);
+    const int prec = sizeof(T) == 8 ? SF_FP64 : SF_FP32;
+    const bool vc = ob && ob->volcons && ob->flags;
     hipLaunchKernelGGL(k_phys_pred_uv<T>, grid, block, 0, s, a);
+    if (ob) hipLaunchKernelGGL(k_open_vel<T>, egrid, block, 0, s, a, *ob, 0);
+    // the volume constraint on the predicted velocities: the corrector's free surface
+    // then sees no net flow through the open edges
+    if (vc) {
+        const int rc = sf_obc_volcons(a.zs, x->h, x->pm, x->pn, a.us, a.vs, x->szI, x->szJ, prec, ob->flags, 1,
+                                      ob->vc_work, ob->vc_out, stream);
+        if (rc != SF_OK) return rc;
+    }
     hipLaunchKernelGGL(k_phys_corr_zeta<T>, grid, block, 0, s, a);
+    if (ob) hipLaunchKernelGGL(k_open_zeta<T>, egrid, block, 0, s, a, *ob, 1);
     hipLaunchKernelGGL(k_phys_corr_uv<T>, grid, block, 0, s, a);
+    if (ob) hipLaunchKernelGGL(k_open_vel<T>, egrid, block, 0, s, a, *ob, 1);
+    if (vc) {
+        const int rc = sf_obc_volcons(x->zeta[2], x->h, x->pm, x->pn, x->ubar[2], x->vbar[2], x->szI, x->szJ, prec,
+                                      ob->flags, 1, ob->vc_work, ob->vc_out, stream);
+        if (rc != SF_OK) return rc;
+    }
     return sf::check_launch("sf_phys_substep");
 }
 
@@ -425,5 +523,35 @@ extern "C" int sf_phys_substep_coupled(const sf_phys_fields *x, const sf_couple_
     if (precision == SF_FP64) return launch_phys<double>(x, dt, first, cff1, stream, cp);
     if (precision == SF_FP32) return launch_phys<float>(x, dt, first, cff1, stream, cp);
     sf::set_error("sf_phys_substep_coupled: unknown precision %d", precision);
+    return SF_ERR_ARG;
+}
+
+extern "C" int sf_phys_substep_open(const sf_phys_fields *x, const sf_open_bc *obc, int32_t precision, double dt,
+                                    int32_t first, double cff1, double *volcons_work, double *volcons_out,
+                                    void *stream)
+{
+    SF_REQUIRE(x && obc, "sf_phys_substep_open: null argument");
+    SF_REQUIRE(x->szI >= 4 && x->szJ >= 4, "sf_phys_substep_open: grid %dx%d too small", x->szI, x->szJ);
+    SF_REQUIRE(x->h && x->pm && x->pn && x->fomn && x->sustr && x->svstr, "sf_phys_substep_open: null grid field");
+    for (int k = 0; k < 3; k++)
+        SF_REQUIRE(x->zeta[k] && x->ubar[k] && x->vbar[k], "sf_phys_substep_open: null state slot");
+    SF_REQUIRE(x->rzeta[0] && x->rzeta[1] && x->rubar[0] && x->rubar[1] && x->rvbar[0] && x->rvbar[1],
+               "sf_phys_substep_open: null rhs slot");
+    for (int k = 0; k < 4; k++) SF_REQUIRE(x->scratch[k], "sf_phys_substep_open: null scratch");
+    SF_REQUIRE(x->Zt_avg1 && x->DU_avg1 && x->DV_avg1, "sf_phys_substep_open: null average");
+    SF_REQUIRE((obc->flags & ~15) == 0, "sf_phys_substep_open: unknown edge flags 0x%x", obc->flags);
+    SF_REQUIRE(!obc->volcons || (volcons_work && volcons_out), "sf_phys_substep_open: volcons needs work and out");
+    Open ob;
+    ob.flags = obc->flags;
+    for (int e = 0; e < 4; e++) {
+        ob.ez[e] = obc->zeta_ext[e];
+        ob.eu[e] = obc->vel_ext[e];
+    }
+    ob.volcons = obc->volcons;
+    ob.vc_work = volcons_work;
+    ob.vc_out = volcons_out;
+    if (precision == SF_FP64) return launch_phys<double>(x, dt, first, cff1, stream, nullptr, &ob);
+    if (precision == SF_FP32) return launch_phys<float>(x, dt, first, cff1, stream, nullptr, &ob);
+    sf::set_error("sf_phys_substep_open: unknown precision %d", precision);
     return SF_ERR_ARG;
 }

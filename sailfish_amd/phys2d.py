@@ -26,8 +26,15 @@ class sf_phys_fields(C.Structure):
     ]
 
 
+class sf_open_bc(C.Structure):
+    _fields_ = [("flags", C.c_int32), ("volcons", C.c_int32), ("zeta_ext", P * 4), ("vel_ext", P * 4)]
+
+
 def _bind():
     lib = _lib.load()
+    lib.sf_phys_substep_open.argtypes = [C.POINTER(sf_phys_fields), C.POINTER(sf_open_bc), C.c_int32, C.c_double,
+                                         C.c_int32, C.c_double, P, P, P]
+    lib.sf_phys_substep_open.restype = C.c_int32
     lib.sf_phys_substep.argtypes = [C.POINTER(sf_phys_fields), C.c_int32, C.c_double, C.c_int32, C.c_double, P]
     lib.sf_phys_substep.restype = C.c_int32
     lib.sf_phys_substep_fused.argtypes = [C.POINTER(sf_phys_fields), C.c_int32, C.c_double, C.c_int32, C.c_double,
@@ -42,10 +49,15 @@ class PhysStepper:
     """Static fields in the state precision plus the predicted-state scratch.
 
     ``fused``: one launch per substep (tile height ``tile_j``, 0 = library default)
-    instead of the four-launch form."""
+    instead of the four-launch form.
+
+    ``open_edges``: SF_OBC_* bits of the open edges (Chapman free surface, Flather normal
+    velocity; the four-launch form), ``volcons``: the volume constraint on them after
+    each substep, ``zeta_ext`` / ``vel_ext``: four per-edge host arrays (or None) of the
+    external free surface and normal velocity."""
 
     def __init__(self, GRID, OCEAN, rdrg: float = 3e-4, g: float = G_CONST, fused: bool = True,
-                 tile_j: int = 0):
+                 tile_j: int = 0, open_edges: int = 0, volcons: bool = False, zeta_ext=None, vel_ext=None):
         _lib.require_gpu()
         self.lib = _bind()
         dt = OCEAN.dtype
@@ -65,9 +77,24 @@ class PhysStepper:
         }
         self.szI, self.szJ = GRID.szI, GRID.szJ
         self.g, self.rdrg = float(g), float(rdrg)
-        self.fused, self.tile_j = bool(fused), int(tile_j)
+        self.open_edges = int(open_edges)
+        self.fused, self.tile_j = bool(fused) and not self.open_edges, int(tile_j)
         # predicted-state scratch of the four-launch form (the fused form keeps it in LDS)
         self.scratch = [] if self.fused else [torch.zeros(S2, dtype=dt, device=dev) for _ in range(4)]
+        if self.open_edges:
+            ob = sf_open_bc(self.open_edges, int(bool(volcons)))
+            self._ext = []
+            for e in range(4):
+                for arrs, field in ((zeta_ext, "zeta_ext"), (vel_ext, "vel_ext")):
+                    a = None if arrs is None else arrs[e]
+                    if a is not None:
+                        t = torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.float64))).to(dev)
+                        self._ext.append(t)
+                        getattr(ob, field)[e] = t.data_ptr()
+            self.open_bc = ob
+            n = int(self.lib.sf_obc_volcons_work_size(self.szI, self.szJ))
+            self.vc_work = torch.zeros(n, dtype=torch.float64, device=dev)
+            self.vc_out = torch.zeros(3, dtype=torch.float64, device=dev)
 
     def fields(self, OCEAN) -> sf_phys_fields:
         a = sf_phys_fields()
@@ -86,6 +113,13 @@ class PhysStepper:
 
     def step(self, compTimes, OCEAN, first: bool, stream=None):
         a = self.fields(OCEAN)
+        if self.open_edges:
+            rc = self.lib.sf_phys_substep_open(C.byref(a), C.byref(self.open_bc), self.precision,
+                                               float(compTimes.dtfast), int(bool(first)), compTimes.cff1(),
+                                               self.vc_work.data_ptr(), self.vc_out.data_ptr(),
+                                               _lib.stream_handle(stream))
+            _lib.check(rc, "sf_phys_substep_open")
+            return
         if self.fused:
             rc = self.lib.sf_phys_substep_fused(C.byref(a), self.precision, float(compTimes.dtfast),
                                                 int(bool(first)), compTimes.cff1(), self.tile_j,

@@ -32,8 +32,10 @@ def _oracle3(cfg, GRID, ct, N, fused_fix_q5=True):
     m = O3.OModel3(g, N, GRID.host["fomn"], GRID.host["sc_r"], GRID.host["sc_w"], GRID.host["Cs_r"],
                    GRID.host["Cs_w"], GRID.hc, GRID.Vtransform, *BOUNDARY_fields)
     for k in range(3):
-        m.u[k][:] = np.broadcast_to(cfg["u0"], (N + 1, cfg["szJ"], cfg["szI"])).ravel()
-        m.v[k][:] = np.broadcast_to(cfg["v0"], (N + 1, cfg["szJ"], cfg["szI"])).ravel()
+        if "u0" in cfg:
+            m.u[k][:] = np.broadcast_to(cfg["u0"], (N + 1, cfg["szJ"], cfg["szI"])).ravel()
+        if "v0" in cfg:
+            m.v[k][:] = np.broadcast_to(cfg["v0"], (N + 1, cfg["szJ"], cfg["szI"])).ravel()
     return t, g, s, b, m
 
 

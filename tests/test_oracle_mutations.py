@@ -22,7 +22,8 @@ import pytest
 ROOT = Path(__file__).resolve().parents[1]
 ORACLE = ROOT / "oracle"
 PIN_FILES = ["tests/test_oracle2d.py", "tests/test_oracle3d.py", "tests/test_oracle_phys.py",
-             "tests/test_oracle_volcons.py", "tests/test_oracle_coupled.py"]
+             "tests/test_oracle_volcons.py", "tests/test_oracle_coupled.py", "tests/test_oracle_open.py",
+             "tests/test_oracle_baroclinic.py"]
 
 # (name, source file, text to replace, replacement): the text occurs exactly once
 MUTANTS = [
@@ -164,6 +165,43 @@ MUTANTS = [
      "FC = rD * (FC - Davg2[f]);", "FC = rD * (FC - Davg1[f]);"),
     ("cpl_ubar_without_depth", "sf_oracle3d.c",
      "xbar2[f] = rD * Davg1[f];", "xbar2[f] = Davg1[f];"),
+    # --- open edges (Chapman, Flather, volume constraint) ---
+    ("open_flather_sign", "sf_oracle_phys.c",
+     "x[fu] = (e == 0 || e == 2) ? ext_at(ob->ext_u[e], b) - c * dz : ext_at(ob->ext_u[e], b) + c * dz;",
+     "x[fu] = (e == 0 || e == 2) ? ext_at(ob->ext_u[e], b) + c * dz : ext_at(ob->ext_u[e], b) - c * dz;"),
+    ("open_flather_celerity", "sf_oracle_phys.c",
+     "const double c = sqrt(p->g / D);", "const double c = sqrt(p->g * D);"),
+    ("open_flather_ext_level_ignored", "sf_oracle_phys.c",
+     "const double dz = 0.5 * (zn[fg] + zn[fi]) - ext_at(ob->ext_z[e], b);",
+     "const double dz = 0.5 * (zn[fg] + zn[fi]);"),
+    ("open_chapman_explicit_interior", "sf_oracle_phys.c",
+     "zn[fg] = (zb[fg] + Ce * zn[fi]) / (1.0 + Ce);", "zn[fg] = (zb[fg] + Ce * zb[fi]) / (1.0 + Ce);"),
+    ("open_chapman_frozen", "sf_oracle_phys.c",
+     "zn[fg] = (zb[fg] + Ce * zn[fi]) / (1.0 + Ce);", "zn[fg] = zb[fg];"),
+    ("open_volcons_corrector_only", "sf_oracle_phys.c",
+     "if (ob->volcons) sfo_obc_volcons(p->szI, p->szJ, zs, p->h, p->pm, p->pn, us, vs, ob->flags, 1, tmp);",
+     "(void)tmp;"),
+    # --- baroclinic terms (EOS, pressure gradient, tracer step) ---
+    ("eos_temperature_sign", "sf_oracle3d.c",
+     "rho[b] = (a - bt * (T[b] - T0)) + bs * (S[b] - S0);", "rho[b] = (a + bt * (T[b] - T0)) + bs * (S[b] - S0);"),
+    ("eos_salinity_dropped", "sf_oracle3d.c",
+     "rho[b] = (a - bt * (T[b] - T0)) + bs * (S[b] - S0);", "rho[b] = (a - bt * (T[b] - T0));"),
+    ("pgf_background_density", "sf_oracle3d.c",
+     "fac2 = 1000.0 * grav / rho0", "fac2 = 0.0 * grav / rho0"),
+    ("pgf_jacobian_weight", "sf_oracle3d.c",
+     "const double C1 = (1.0 + gam) * (R(k + 1, f) - R(k + 1, p)) + (1.0 - gam) * (R(k, f) - R(k, p));",
+     "const double C1 = (1.0 - gam) * (R(k + 1, f) - R(k + 1, p)) + (1.0 + gam) * (R(k, f) - R(k, p));"),
+    ("pgf_jacobian_sign", "sf_oracle3d.c",
+     "phi = phi + fac3 * (C1 * C3 - C2 * C4);", "phi = phi + fac3 * (C1 * C3 + C2 * C4);"),
+    ("pgf_top_half_depth", "sf_oracle3d.c",
+     "double phi = (fac1 * (R(k, f) - R(k, p))) * c1;", "double phi = (fac3 * (R(k, f) - R(k, p))) * c1;"),
+    ("tracer_downstream", "sf_oracle3d.c",
+     "const double FX0 = hu0 * (hu0 > 0.0 ? t[b - 1] : t[b]);", "const double FX0 = hu0 * (hu0 > 0.0 ? t[b] : t[b - 1]);"),
+    ("tracer_thickness_change_ignored", "sf_oracle3d.c",
+     "const double Wk = (Wlow - Dh) - (Hz_new[b] - Hz_old[b]) / dt;", "const double Wk = (Wlow - Dh);"),
+    ("tracer_vertical_flux_dropped", "sf_oracle3d.c",
+     "const double div = mn * ((FX1 - FX0) + (FE1 - FE0)) + (FCk - FClow);",
+     "const double div = mn * ((FX1 - FX0) + (FE1 - FE0));"),
 ]
 
 
