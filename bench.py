@@ -699,6 +699,76 @@ def run_3d_gpu(args):
     return res
 
 
+def run_3d_step_gpu(args, dtype_name: str, coupled: bool):
+    """The whole baroclinic step on the 1024 x 512 x 32 workload (configs.basin3: 40
+    barotropic substeps per step of 300 s): the reference's loop body (main3d:
+    set_maxflux, omega, set_zeta, rhs3d, 41 fused barotropic substeps, set_depth,
+    step3d_UV, lateral BCs, omega) or the coupled form with the baroclinic pressure
+    gradient and the tracer step (coupled3d).  Timed two ways: three steps issued from
+    the host, and three steps captured in a HIP graph and replayed (median of 3 replays
+    after 2 untimed ones; the three steps bring the slot rotation back).  A 3D
+    cell-step = one (i, j, k) rho cell x one baroclinic step."""
+    import torch
+    from sailfish_amd import configs
+    from sailfish_amd.main3d import Main3dState, baroclinic_step, initial_depths
+    from sailfish_amd.model2d import build_model3d
+    dtype = torch.float64 if dtype_name == "f64" else torch.float32
+    cfg = configs.basin3(args.szI3, args.szJ3, args.N3)
+    N = cfg["N"]
+    ct, GRID, OCEAN, BOUNDARY = build_model3d(cfg, N=N, ndtfast=cfg["ndtfast"], dt=cfg["dt"], fix_q5=True,
+                                              dtype=dtype)
+    if coupled:
+        from sailfish_amd.coupled3d import coupled_step, main3d_coupled
+        st = main3d_coupled(ct, GRID, OCEAN, BOUNDARY, 1, tracers=True)     # includes the initial set_depth
+        step = lambda: coupled_step(ct, GRID, OCEAN, BOUNDARY, st)
+    else:
+        m3 = Main3dState(GRID, OCEAN)
+        initial_depths(GRID, OCEAN)
+        step = lambda: baroclinic_step(ct, GRID, OCEAN, BOUNDARY, m3, fused=True)
+        step()
+    torch.cuda.synchronize()
+    a, b = _events(torch, 1)[0]
+    a.record()
+    for _ in range(3):
+        step()
+    b.record()
+    torch.cuda.synchronize()
+    eager = a.elapsed_time(b) / 3
+    res = {"ms_per_step_host_issued": eager}
+    try:
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                for _ in range(3):
+                    step()
+        torch.cuda.current_stream().wait_stream(s)
+        for _ in range(2):
+            g.replay()
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(3):
+            a, b = _events(torch, 1)[0]
+            a.record()
+            g.replay()
+            b.record()
+            torch.cuda.synchronize()
+            times.append(a.elapsed_time(b) / 3)
+        res["ms_per_step_graph"] = sorted(times)[1]
+        res["graph_replays_ms_per_step"] = times
+        ms = res["ms_per_step_graph"]
+    except Exception as e:      # a capture that fails leaves the host-issued figure
+        res["graph_error"] = f"{type(e).__name__}: {e}"
+        ms = eager
+    cells = N * GRID.szI * GRID.szJ
+    res.update(ms_per_step=ms, cell_steps_per_s_3d=cells / (ms * 1e-3), dtype=dtype_name,
+               finite_after_run=bool(torch.isfinite(OCEAN.u).all().item()),
+               workload=(f"{GRID.szI}x{GRID.szJ}x{N}, 40 substeps per 300 s step, "
+                         + ("coupled form + pressure gradient + tracers" if coupled else "reference loop body")))
+    return res
+
+
 def run_cpu_3d(GRID, BOUNDARY, host, N, budget_s):
     """The same 3D kernels (rhs3d with the reference's launch sizes, set_maxflux, omega)
     in the oracle's C restatement (FP64, OpenMP) on the same grid and state."""
@@ -879,6 +949,10 @@ def main():
     if not args.no_3d and args.mode == "phys":
         out["baroclinic_3d"] = run_3d_gpu(args)
         progress("3d")
+        out["baroclinic_3d"]["whole_step"] = {
+            f"{'coupled' if cp else 'reference'}_{d}": run_3d_step_gpu(args, d, cp)
+            for cp in (False, True) for d in ("f64", "f32")}
+        progress("3d whole step")
     if not args.no_cpu:
         out["cpu_baseline"] = (run_cpu_phys if args.mode == "phys" else run_cpu_faithful)(args, args.cpu_budget)
         progress("cpu baseline")

@@ -273,13 +273,21 @@ def set_zeta(grid, block, args, ctx=None):
     _lib.call(name, c.handle, *p, _f64(Zt), _stream())
 
 
+_lateral_checked: dict = {}
+
+
 def setLateralUVBCs(grid, block, args, ctx=None):
     """step3D_uv.kernels.cpp:314-368; args = (t, u, v, uIdx2, vIdx2, uType, vType)."""
     t, u, v, _ui, _vi, ut, vt = args
     c = ctx or ctx_of()
-    vt_host = vt.detach().cpu().numpy()
-    grad_only = (vt_host > 0) & ((vt_host & (1 << 4)) == 0) & ((vt_host & (1 << 3)) == 0) & ((vt_host & (1 << 6)) != 0)
-    if grad_only.any():
+    # the boundary types never change: check them once per array (no host copy, and so
+    # no synchronisation, on later calls, e.g. inside a HIP graph capture)
+    key = (vt.data_ptr(), vt.numel())
+    if key not in _lateral_checked:
+        vt_host = vt.detach().cpu().numpy()
+        _lateral_checked[key] = bool(((vt_host > 0) & ((vt_host & (1 << 4)) == 0) & ((vt_host & (1 << 3)) == 0)
+                                      & ((vt_host & (1 << 6)) != 0)).any())
+    if _lateral_checked[key]:
         raise _lib.SailfishError("setLateralUVBCs: gradient v boundaries index the 2D index field at 3D "
                                  "offsets in the reference (step3D_uv.kernels.cpp:363); not supported")
     name, p = _bulk("sf_set_lateral_uv_bcs", (u, v))
