@@ -359,19 +359,37 @@ typedef struct sf_couple_fields {
 int sf_phys_substep_coupled(const sf_phys_fields *fields, const sf_couple_fields *cpl, int32_t precision,
                             double dt, int32_t first, double cff1, void *stream);
 
-/* step3d_UV coupled with the barotropic mode (nonlinear/step3d_uv.py:155-180 and
- * :365-545; replaces the reference kernel's tmpU-based mean correction and
- * adjustBarotropicVelocity, step3D_uv.kernels.cpp:408-517, and makes the vbar_t2 = 0
- * work-around of nonlinear/main3d.py:189 unnecessary): the implicit vertical viscosity of
- * sf_step3d_uv, then per interior column u -= (sum DC u - DU_avg1) / D with
- * DC = on_u Hz_u, ubar(t1) = ubar(t2) = DU_avg1 / D, and the mass flux
- * Huon = 0.5 (Huon + u DC) corrected to sum to DU_avg2; v likewise.  3D fields, ubar,
- * vbar and the scratch in `precision`; the averages FP64; column sums in FP64. */
-int sf_step3d_uv_coupled(sf_ctx *ctx, int32_t precision, void *u, void *v, const void *ru, const void *rv,
-                         void *ubar_t1, void *vbar_t1, void *ubar_t2, void *vbar_t2, const void *Hz, const void *Akv,
-                         const void *z_r, const double *DU_avg1, const double *DV_avg1, const double *DU_avg2,
-                         const double *DV_avg2, void *Huon, void *Hvom, const void *tmpU, const void *tmpV,
-                         int32_t iic, int32_t ntfirst, double lambda, double Dt, void *SD, void *D, void *RHS,
+/* Fields of the coupled 3D momentum step.  3D fields, ubar/vbar, the right-hand-side
+ * history and the scratch in the step's precision; the averages and the stresses FP64. */
+typedef struct sf_uv_coupled {
+    void *u, *v;                          /* szK+1 levels; levels 0..szK-1 updated */
+    const void *ru, *rv;                  /* this step's right-hand side (rhs3d + pressure gradient) */
+    const void *ru1, *rv1, *ru2, *rv2;    /* right-hand sides of the two steps before (szK levels) */
+    void *ru_keep, *rv_keep;              /* receive ru, rv (szK levels); must not alias ru1..rv2 */
+    void *ubar_t1, *vbar_t1, *ubar_t2, *vbar_t2;
+    const void *Hz_old, *Hz, *Akv, *z_r;  /* thicknesses before and after set_depth; Akv szK+1 levels */
+    const double *DU_avg1, *DV_avg1, *DU_avg2, *DV_avg2;
+    void *Huon, *Hvom;                    /* mass fluxes, corrected to DU_avg2 / DV_avg2 */
+    const double *sustr, *svstr;          /* kinematic surface stress at u / v points */
+    double rdrg;                          /* linear bottom drag */
+    int32_t ab;                           /* 0: first step, 1: second, 2: later (Adams-Bashforth order) */
+    void *SD, *D, *RHS;                   /* scratch, szK * szI * szJ elements each */
+} sf_uv_coupled;
+
+/* The 3D momentum step of the coupled form (nonlinear/step3d_uv.py:155-180 and
+ * :365-545, ROMS step3d_uv with "Couple 2D and 3D momentum equations"; replaces the live
+ * kernel's solve, tmpU mean correction and adjustBarotropicVelocity,
+ * step3D_uv.kernels.cpp:408-707, and makes the vbar_t2 = 0 work-around of
+ * nonlinear/main3d.py:189 unnecessary).  On the interior u points (i = 2..szI-2,
+ * j = 1..szJ-2; v: j = 2..szJ-2, i = 1..szI-2) it solves
+ *   Hz_u,new u_new - dt d/dz(Akv du_new/dz) = Hz_u,old u + dt pm_u pn_u AB3(ru)
+ *                                             - dt rdrg u(0) [bottom] + dt sustr [top]
+ * (the reference's solve multiplies ru by Hz a second time and has no right-hand-side
+ * history, which is unstable once the coupling moves the 3D flow), then replaces the
+ * column mean: u -= (sum DC u - DU_avg1) / D with DC = on_u Hz_u, ubar(t1) = ubar(t2) =
+ * DU_avg1 / D, and the mass flux Huon = 0.5 (Huon + u DC) corrected to sum to DU_avg2;
+ * v likewise.  Other columns are left as they are.  Column sums in FP64. */
+int sf_step3d_uv_coupled(sf_ctx *ctx, int32_t precision, const sf_uv_coupled *fields, double lambda, double dt,
                          void *stream);
 
 /* ---------------- open boundaries of the physics substep ---------------------------- */

@@ -31,10 +31,17 @@ class sfo_couple(C.Structure):
                 ("DU2", P), ("DV2", P), ("w2p", F64), ("w2c", F64), ("ab", C.c_int)]
 
 
+class sfo_uv_coupled(C.Structure):
+    _fields_ = [(n, P) for n in ("u", "v", "ru", "rv", "ru1", "rv1", "ru2", "rv2", "ru_keep", "rv_keep", "ubar_t1",
+                                 "vbar_t1", "ubar_t2", "vbar_t2", "Hz_old", "Hz", "Akv", "z_r", "DU_avg1", "DV_avg1",
+                                 "DU_avg2", "DV_avg2", "Huon", "Hvom", "sustr", "svstr")] + \
+        [("rdrg", F64), ("ab", C.c_int), ("SD", P), ("D", P), ("RHS", P)]
+
+
 O.declare({
     "sfo_phys_substep_coupled": ([OP._PP, F64, C.c_int, F64] + [P] * 22 + [C.POINTER(sfo_couple)], None),
     "sfo_couple_rufrc": ([O3._G3, P, P, P, P, P, P, F64, P, P], None),
-    "sfo_step3d_uv_coupled": ([O3._G3] + [P] * 19 + [C.c_int, C.c_int, F64, F64, P, P, P], None),
+    "sfo_step3d_uv_coupled": ([O3._G3, C.POINTER(sfo_uv_coupled), F64, F64], None),
 })
 
 
@@ -73,6 +80,9 @@ class OCoupled:
         self.Fu = [np.zeros(S2) for _ in range(3)]
         self.Fv = [np.zeros(S2) for _ in range(3)]
         self.DU2, self.DV2 = np.zeros(S2), np.zeros(S2)
+        # right-hand sides of the 3D momentum of this step and the two before (rotated)
+        self.ruh = [np.zeros(N * S2) for _ in range(3)]
+        self.rvh = [np.zeros(N * S2) for _ in range(3)]
         self.g = float(g)
         self.tr = None
         if tracers is not None:
@@ -126,11 +136,13 @@ class OCoupled:
             self.substep(dtfast, iif == 1, float(weight1[k1]), cp)
         Hz_old = m.Hz.copy()
         m.set_depth(s)
-        tmpU, tmpV = u2.copy(), v2.copy()
-        L.sfo_step3d_uv_coupled(m.cp, p(u2), p(v2), p(m.ru[2]), p(m.rv[2]), p(s.v("ubar", 1)), p(s.v("vbar", 1)),
-                                p(s.v("ubar", 2)), p(s.v("vbar", 2)), p(m.Hz), p(m.AKv), p(m.z_r), p(s.DU_avg1),
-                                p(s.DV_avg1), p(self.DU2), p(self.DV2), p(m.Huon), p(m.Hvom), p(tmpU), p(tmpV),
-                                int(iic), int(ntfirst), 1.0, float(dt), p(m.eqSD), p(m.eqD), p(m.eqRHS))
+        st = self.ph.static
+        a = sfo_uv_coupled(p(u2), p(v2), p(m.ru[2]), p(m.rv[2]), p(self.ruh[h1]), p(self.rvh[h1]), p(self.ruh[h2]),
+                           p(self.rvh[h2]), p(self.ruh[cur]), p(self.rvh[cur]), p(s.v("ubar", 1)), p(s.v("vbar", 1)),
+                           p(s.v("ubar", 2)), p(s.v("vbar", 2)), p(Hz_old), p(m.Hz), p(m.AKv), p(m.z_r),
+                           p(s.DU_avg1), p(s.DV_avg1), p(self.DU2), p(self.DV2), p(m.Huon), p(m.Hvom),
+                           p(st["sustr"]), p(st["svstr"]), self.rdrg, ab_order(n), p(m.eqSD), p(m.eqD), p(m.eqRHS))
+        L.sfo_step3d_uv_coupled(m.cp, C.byref(a), 1.0, float(dt))
         if tr is not None:
             for name in ("T", "S"):
                 new = np.zeros_like(tr[name])

@@ -507,25 +507,110 @@ void sfo_couple_rufrc(const sfo_grid3 *g, const double *ru, const double *rv, co
     }
 }
 
-/* step3d_UV coupled with the barotropic mode (nonlinear/step3d_uv.py:155-180 and
- * :365-545, ROMS step3d_uv "Couple 2D and 3D momentum equations"): the implicit
- * vertical viscosity of sfo_step3d_UV (build_matrix + solve_tri, unchanged), then per
- * interior u column (i = 2..L-1, j = 1..M-1; v: j = 2..M-1, i = 1..L-1), with
- * DC(K) = on_u * Hz_u(K), D = sum DC(K):
+/* The 3D momentum step of the coupled form (ROMS step3d_uv with the "Couple 2D and 3D
+ * momentum equations" section, nonlinear/step3d_uv.py:155-180 and :365-545; the live
+ * kernel step3D_uv.kernels.cpp:568-707 is the uncoupled sfo_step3d_UV above).
+ *
+ * The reference's solve forms RHS = Hz_u (u + cff dt pm pn ru), but ru already carries
+ * Hz (ru = Hz f v area, the flux divergences of Huon u, W u), so every tendency is
+ * multiplied by the layer thickness a second time, and cff = 23/12 multiplies the
+ * current ru with no history.  Its velocity never moves in the reference's loop, where
+ * this stays hidden; with the coupling driving the 3D flow the extra factor Hz f dt
+ * (6.7 for the 116 m bottom layers of a 1000 m column with dt = 300 s) makes the step
+ * grow about 7x per step.  The coupled form therefore solves ROMS's equation
+ *   Hz_u,new u_new - dt d/dz (Akv du_new/dz) = Hz_u,old u + dt pm_u pn_u AB3(ru)
+ *                                              - dt rdrg u(0) [bottom] + dt sustr [top]
+ * on the interior u points (i = 2..L-1, j = 1..M-1, levels 0..N-1; v: j = 2..M-1,
+ * i = 1..L-1), the points the barotropic mode and rufrc cover (the other columns -- the
+ * walls and the ghost rows -- are left as they are: the reference's lateral boundary
+ * step does not zero the wall velocity, so a solve there would only accumulate the
+ * wind stress), with
+ *   SD(K) = (-lambda dt) Akv_u(K+1) / dz_u(K+1)   (K = 0..N-2, SD(N-1) = 0; Akv_u(K+1) at
+ *            the interface between levels K and K+1, dz_u the z_r difference at u),
+ *   D(K)  = (Hz_u,new(K) - SD(K)) - SD(K-1)       (SD(-1) = 0),
+ *   AB3(ru) = ru (ab 0), 1.5 ru - 0.5 ru1 (ab 1), (23/12 ru - 16/12 ru1) + 5/12 ru2 (ab 2),
+ * where ru1, ru2 are the right-hand sides of the two steps before; this step's ru is
+ * stored into ru_keep for the next.  The bottom stress rdrg u(0) and the surface stress
+ * sustr are the ones rufrc gives the barotropic mode.  Then per column, with
+ * DC(K) = on_u Hz_u,new(K), D = sum DC(K):
  *   CF = (sum DC(K) u(K) - DU_avg1) / D;  u(K) -= CF;  ubar(t1) = ubar(t2) = DU_avg1 / D
  *   Huon(K) = 0.5 (Huon(K) + u(K) DC(K));  FC = (sum Huon(K) - DU_avg2) / D;
  *   Huon(K) -= DC(K) FC
  * so the new u carries the time-averaged barotropic transport DU_avg1 and the mass
  * flux the filtered transport DU_avg2.  (The text's u section omits the FC sum that its
- * v section has; both sections sum here, as ROMS does.)  This replaces the
- * tmpU-based mean correction and adjustBarotropicVelocity of the live kernel. */
+ * v section has; both sections sum here, as ROMS does.) */
+typedef struct {
+    double *u, *v;                       /* szK+1 levels, updated (levels 0..N-1) */
+    const double *ru, *rv;               /* this step's right-hand side (levels 0..N-1 read) */
+    const double *ru1, *rv1, *ru2, *rv2; /* the two steps before (N levels) */
+    double *ru_keep, *rv_keep;           /* receive ru, rv (N levels) */
+    double *ubar_t1, *vbar_t1, *ubar_t2, *vbar_t2;
+    const double *Hz_old, *Hz, *Akv, *z_r;
+    const double *DU_avg1, *DV_avg1, *DU_avg2, *DV_avg2;
+    double *Huon, *Hvom;
+    const double *sustr, *svstr;
+    double rdrg;
+    int ab;
+    double *SD, *D, *RHS;                /* scratch, N * S2 each */
+} sfo_uv_coupled;
+
+static double ab3(int ab, double r, double r1, double r2)
+{
+    if (ab == 0) return r;
+    if (ab == 1) return 1.5 * r - 0.5 * r1;
+    return ((23.0 / 12.0) * r - (16.0 / 12.0) * r1) + (5.0 / 12.0) * r2;
+}
+
+/* the interior u (vdir 0) or v (vdir 1) points of the coupled form */
+static int coupled_point(const sfo_grid3 *g, int vdir, ptrdiff_t f)
+{
+    const int i = (int)(f % g->szI), j = (int)(f / g->szI), L = g->szI - 1, M = g->szJ - 1;
+    return vdir ? (j >= 2 && j <= M - 1 && i >= 1 && i <= L - 1) : (i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1);
+}
+
+static void momentum_column(const sfo_grid3 *g, int vdir, ptrdiff_t f, const sfo_uv_coupled *a, double lambda,
+                            double Dt)
+{
+    const ptrdiff_t S2 = S2_OF(g), N = g->N;
+    const ptrdiff_t d = vdir ? g->szI : 1;
+    double *x = vdir ? a->v : a->u, *keep = vdir ? a->rv_keep : a->ru_keep;
+    const double *r = vdir ? a->rv : a->ru, *r1 = vdir ? a->rv1 : a->ru1, *r2 = vdir ? a->rv2 : a->ru2;
+    const double *stress = vdir ? a->svstr : a->sustr;
+    double *SD = a->SD, *D = a->D, *RHS = a->RHS;
+    const double *Hz = a->Hz, *Ho = a->Hz_old, *Akv = a->Akv, *z_r = a->z_r;
+#define AVG(A, K) (((A)[(K) * S2 + f] + (A)[(K) * S2 + f - d]) * 0.5)
+#define DZ(K) (((z_r[(K) * S2 + f] - z_r[((K) - 1) * S2 + f]) + (z_r[(K) * S2 + f - d] - z_r[((K) - 1) * S2 + f - d])) * 0.5)
+    const double pmU = (g->pm[f] + g->pm[f - d]) * 0.5;
+    const double pnU = (g->pn[f] + g->pn[f - d]) * 0.5;
+    const double cdt_mn = Dt * (pmU * pnU);
+    for (ptrdiff_t K = 0; K < N - 1; K++) SD[K * S2 + f] = (((-lambda) * Dt) * AVG(Akv, K + 1)) / DZ(K + 1);
+    SD[(N - 1) * S2 + f] = 0.0;
+    for (ptrdiff_t K = 0; K < N; K++) {
+        const ptrdiff_t b = K * S2 + f;
+        const double rk = r[b];
+        const double rab = ab3(a->ab, rk, a->ab > 0 ? r1[b] : 0.0, a->ab > 1 ? r2[b] : 0.0);
+        keep[b] = rk;
+        RHS[b] = AVG(Ho, K) * x[b] + cdt_mn * rab;
+        D[b] = K == 0 ? AVG(Hz, 0) - SD[f] : (AVG(Hz, K) - SD[b]) - SD[b - S2];
+    }
+    RHS[f] = RHS[f] - (Dt * a->rdrg) * x[f];
+    RHS[(N - 1) * S2 + f] = RHS[(N - 1) * S2 + f] + Dt * stress[f];
+#undef AVG
+#undef DZ
+    for (ptrdiff_t k = 1; k < N; k++) {
+        const double w = SD[(k - 1) * S2 + f] / D[(k - 1) * S2 + f];
+        D[k * S2 + f] -= w * SD[(k - 1) * S2 + f];
+        RHS[k * S2 + f] -= w * RHS[(k - 1) * S2 + f];
+    }
+    x[(N - 1) * S2 + f] = RHS[(N - 1) * S2 + f] / D[(N - 1) * S2 + f];
+    for (ptrdiff_t k = N - 2; k >= 0; k--)
+        x[k * S2 + f] = (RHS[k * S2 + f] - SD[k * S2 + f] * x[(k + 1) * S2 + f]) / D[k * S2 + f];
+}
+
 static void couple_column(const sfo_grid3 *g, int vdir, ptrdiff_t f, double *x, const double *Hz,
                           const double *Davg1, const double *Davg2, double *Hf, double *xbar1, double *xbar2)
 {
     const ptrdiff_t S2 = S2_OF(g), N = g->N, W = g->szI;
-    const int i = (int)(f % W), j = (int)(f / W), L = g->szI - 1, M = g->szJ - 1;
-    if (!vdir && !(i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1)) return;
-    if (vdir && !(j >= 2 && j <= M - 1 && i >= 1 && i <= L - 1)) return;
     const ptrdiff_t d = vdir ? W : 1;
     const double met = vdir ? 2.0 / (g->pm[f - W] + g->pm[f]) : 2.0 / (g->pn[f - 1] + g->pn[f]);
 #define DC(K) (met * ((Hz[(K) * S2 + f] + Hz[(K) * S2 + f - d]) * 0.5))
@@ -549,26 +634,19 @@ static void couple_column(const sfo_grid3 *g, int vdir, ptrdiff_t f, double *x, 
 #undef DC
 }
 
-void sfo_step3d_uv_coupled(const sfo_grid3 *g, double *u, double *v, const double *ru, const double *rv,
-                           double *ubar_t1, double *vbar_t1, double *ubar_t2, double *vbar_t2, const double *Hz,
-                           const double *Akv, const double *z_r, const double *DU_avg1, const double *DV_avg1,
-                           const double *DU_avg2, const double *DV_avg2, double *Huon, double *Hvom,
-                           const double *tmpU, const double *tmpV, int iic, int ntfirst, double lambda, double Dt,
-                           double *SD, double *D, double *RHS)
+void sfo_step3d_uv_coupled(const sfo_grid3 *g, const sfo_uv_coupled *a, double lambda, double Dt)
 {
     const ptrdiff_t S2 = S2_OF(g);
-    double cff;
-    if (iic == ntfirst) cff = 1.0;
-    else if (iic == ntfirst + 1) cff = 3.0 / 2.0;
-    else cff = 23.0 / 12.0;
 #pragma omp parallel for schedule(static)
     for (ptrdiff_t f = 0; f < S2; f++) {
-        build_matrix(g, 0, f, cff, Dt, lambda, SD, D, RHS, Hz, Akv, z_r, tmpU, ru);
-        solve_tri(g, f, SD, D, RHS, u);
-        build_matrix(g, 1, f, cff, Dt, lambda, SD, D, RHS, Hz, Akv, z_r, tmpV, rv);
-        solve_tri(g, f, SD, D, RHS, v);
-        couple_column(g, 0, f, u, Hz, DU_avg1, DU_avg2, Huon, ubar_t1, ubar_t2);
-        couple_column(g, 1, f, v, Hz, DV_avg1, DV_avg2, Hvom, vbar_t1, vbar_t2);
+        if (coupled_point(g, 0, f)) {
+            momentum_column(g, 0, f, a, lambda, Dt);
+            couple_column(g, 0, f, a->u, a->Hz, a->DU_avg1, a->DU_avg2, a->Huon, a->ubar_t1, a->ubar_t2);
+        }
+        if (coupled_point(g, 1, f)) {
+            momentum_column(g, 1, f, a, lambda, Dt);
+            couple_column(g, 1, f, a->v, a->Hz, a->DV_avg1, a->DV_avg2, a->Hvom, a->vbar_t1, a->vbar_t2);
+        }
     }
 }
 

@@ -7,7 +7,8 @@ sf_step3d_t).
 ``Tracers`` holds temperature and salinity (N levels each, the bulk precision of the
 model) and the density; ``coupled3d.coupled_step`` calls ``pressure_gradient`` after
 rhs3d (so the gradient also enters the barotropic forcing) and ``step`` after the
-coupled step3d_UV with the thicknesses before and after set_depth."""
+coupled step3d_UV with the thicknesses before (``Hz_old``, kept by the coupled step) and
+after set_depth."""
 from __future__ import annotations
 
 import ctypes as C
@@ -82,10 +83,6 @@ class Tracers:
                                       GRID.z_w.data_ptr(), GRID.Hz.data_ptr(), self.g, self.rho0,
                                       OCEAN.ru_t2.data_ptr(), OCEAN.rv_t2.data_ptr(), _lib.stream_handle(stream)),
                    "sf_prsgrd")
-
-    def keep_thickness(self, GRID):
-        """Remember Hz before set_depth changes it (the tracer step's old thickness)."""
-        self.Hz_old.copy_(GRID.Hz)
 
     def step(self, GRID, OCEAN, dt: float, stream=None):
         """Advect T and S by the step's mass fluxes from Hz_old to the current Hz."""

@@ -1,5 +1,5 @@
 """Coupled split-explicit baroclinic step: the reference's 3D kernels (rhs3d,
-set_maxflux, omega, set_depth, the step3d_UV viscosity solve) with the barotropic mode
+set_maxflux, omega, set_depth) with the barotropic mode
 in physics form (ROMS LF-AM3 with advection, Coriolis, wind, drag) and the 2D/3D
 coupling of ROMS that the port keeps only as text and comments:
 
@@ -9,10 +9,13 @@ coupling of ROMS that the port keeps only as text and comments:
   baroclinic steps at the first substep;
 - the substeps also accumulate the filtered transport DU_avg2/DV_avg2
   (step2d_LF_AM3.h.py:778-846);
-- step3d_UV replaces the vertical mean of u, v by the averaged barotropic transport
-  and corrects the mass fluxes Huon/Hvom to DU_avg2/DV_avg2
-  (nonlinear/step3d_uv.py:155-180, :365-545), so the vbar_t2 = 0 work-around of
-  nonlinear/main3d.py:189 is not needed.
+- the 3D momentum step is ROMS's: Hz_new u_new = Hz_old u + dt pm pn AB3(ru) with
+  the implicit vertical viscosity, bottom drag and surface stress (the reference's
+  step3D_uv solve multiplies ru by Hz a second time and keeps no history of ru, which
+  grows by about Hz f dt per step once the coupling moves the 3D flow); it then
+  replaces the vertical mean of u, v by the averaged barotropic transport and corrects
+  the mass fluxes Huon/Hvom to DU_avg2/DV_avg2 (nonlinear/step3d_uv.py:155-180,
+  :365-545), so the vbar_t2 = 0 work-around of nonlinear/main3d.py:189 is not needed.
 
 One baroclinic step (``coupled_step``): set_maxflux, omega, set_zeta, rhs3d, rufrc,
 nfast+1 coupled substeps (the first forward/backward Euler, averages reset on it),
@@ -40,6 +43,14 @@ class sf_couple_fields(C.Structure):
                 ("DU_avg2", P), ("DV_avg2", P), ("w2p", C.c_double), ("w2c", C.c_double), ("ab", C.c_int32)]
 
 
+class sf_uv_coupled(C.Structure):
+    """Fields of the coupled 3D momentum step (sf_uv_coupled of include/sailfish_hip.h)."""
+    _fields_ = [(n, P) for n in ("u", "v", "ru", "rv", "ru1", "rv1", "ru2", "rv2", "ru_keep", "rv_keep", "ubar_t1",
+                                 "vbar_t1", "ubar_t2", "vbar_t2", "Hz_old", "Hz", "Akv", "z_r", "DU_avg1", "DV_avg1",
+                                 "DU_avg2", "DV_avg2", "Huon", "Hvom", "sustr", "svstr")] + \
+        [("rdrg", C.c_double), ("ab", C.c_int32), ("SD", P), ("D", P), ("RHS", P)]
+
+
 def _bind():
     lib = _lib.load()
     if not getattr(lib, "_sf_coupled_bound", False):
@@ -48,8 +59,7 @@ def _bind():
         lib.sf_phys_substep_coupled.restype = C.c_int32
         lib.sf_couple_rufrc.argtypes = [P, C.c_int32, P, P, P, P, P, P, C.c_double, P, P, P]
         lib.sf_couple_rufrc.restype = C.c_int32
-        lib.sf_step3d_uv_coupled.argtypes = [P, C.c_int32] + [P] * 19 + [C.c_int32, C.c_int32, C.c_double,
-                                                                         C.c_double, P, P, P, P]
+        lib.sf_step3d_uv_coupled.argtypes = [P, C.c_int32, C.POINTER(sf_uv_coupled), C.c_double, C.c_double, P]
         lib.sf_step3d_uv_coupled.restype = C.c_int32
         lib._sf_coupled_bound = True
     return lib
@@ -66,10 +76,18 @@ def filtered_transport_weights(weight2, iif: int) -> tuple[float, float]:
     return (8.0 / 12.0) * w(iif) - (1.0 / 12.0) * w(iif + 1), (5.0 / 12.0) * w(iif)
 
 
+def ab_order(n: int) -> int:
+    """Adams-Bashforth order index in baroclinic step n (iic - ntfirst): 0 (first step),
+    1 (3/2, -1/2), 2 (23/12, -16/12, 5/12)."""
+    return 0 if n == 0 else (1 if n == 1 else 2)
+
+
 class CoupledState:
     """Device fields of the coupling: rufrc/rvfrc, three forcing buffers per component
     (this step and the two before it, rotated per baroclinic step), DU_avg2/DV_avg2,
-    the FP64 stresses, and the physics substep in its four-launch form."""
+    three 3D right-hand sides per component for the Adams-Bashforth step of u, v (rotated
+    the same way), the thicknesses before set_depth, the FP64 stresses, and the physics
+    substep in its four-launch form."""
 
     def __init__(self, GRID, OCEAN, rdrg: float = 3e-4, g: float | None = None, tracers=None):
         _lib.require_gpu()
@@ -80,6 +98,11 @@ class CoupledState:
         self.Fu = [z() for _ in range(3)]
         self.Fv = [z() for _ in range(3)]
         self.DU_avg2, self.DV_avg2 = z(), z()
+        shape3 = (GRID.N,) + GRID.shape2D
+        z3 = lambda: torch.zeros(shape3, dtype=OCEAN.u.dtype, device=dev)
+        self.ruh = [z3() for _ in range(3)]
+        self.rvh = [z3() for _ in range(3)]
+        self.Hz_old = torch.zeros_like(GRID.Hz)
         zeros = np.zeros(GRID.shape2D)
         host = GRID.host
         self.sustr = torch.from_numpy(np.ascontiguousarray(host.get("sustr", zeros)).reshape(-1)).to(dev)
@@ -92,9 +115,13 @@ class CoupledState:
         # baroclinic.Tracers: pressure gradient of T, S added to the 3D right-hand side and
         # a tracer step per baroclinic step (None: the reference's terms only)
         self.tracers = tracers
-        OCEAN.extra_state = {"DU_avg2": self.DU_avg2, "DV_avg2": self.DV_avg2,
+        if tracers is not None:
+            tracers.Hz_old = self.Hz_old     # the tracer step reads the same old thicknesses
+        OCEAN.extra_state = {**getattr(OCEAN, "extra_state", {}), "DU_avg2": self.DU_avg2, "DV_avg2": self.DV_avg2,
                              **{f"Fu{k}": t for k, t in enumerate(self.Fu)},
-                             **{f"Fv{k}": t for k, t in enumerate(self.Fv)}}
+                             **{f"Fv{k}": t for k, t in enumerate(self.Fv)},
+                             **{f"ru_h{k}": t for k, t in enumerate(self.ruh)},
+                             **{f"rv_h{k}": t for k, t in enumerate(self.rvh)}}
 
     def couple_fields(self, n: int, iif: int, weight2) -> sf_couple_fields:
         """Coupling fields of substep iif of baroclinic step n (iic - ntfirst): the
@@ -102,7 +129,7 @@ class CoupledState:
         ROMS (first step F, second 3/2 -1/2, then 23/12 -16/12 5/12)."""
         cur, h1, h2 = n % 3, (n - 1) % 3, (n - 2) % 3
         w2p, w2c = filtered_transport_weights(weight2, iif)
-        ab = 0 if n == 0 else (1 if n == 1 else 2)
+        ab = ab_order(n)
         p = lambda t: t.data_ptr()
         return sf_couple_fields(p(self.rufrc), p(self.rvfrc), p(self.Fu[cur]), p(self.Fv[cur]), p(self.Fu[h1]),
                                 p(self.Fv[h1]), p(self.Fu[h2]), p(self.Fv[h2]), p(self.DU_avg2), p(self.DV_avg2),
@@ -144,19 +171,18 @@ def coupled_step(compTimes, GRID, OCEAN, BOUNDARY, st: CoupledState, history=Non
                                                stream), "sf_phys_substep_coupled")
     if history is not None:
         history.maybe_write(compTimes, OCEAN)
-    if st.tracers is not None:
-        st.tracers.keep_thickness(GRID)
+    st.Hz_old.copy_(GRID.Hz)
     initial_depths(GRID, OCEAN)
     m3 = st.m3
-    m3.tmpU.copy_(OCEAN.u_t2)
-    m3.tmpV.copy_(OCEAN.v_t2)
+    n = compTimes.iic - compTimes.ntfirst
+    cur, h1, h2 = n % 3, (n - 1) % 3, (n - 2) % 3
     q = lambda t: t.data_ptr()
-    _lib.check(lib.sf_step3d_uv_coupled(c.handle, prec, q(OCEAN.u_t2), q(OCEAN.v_t2), q(OCEAN.ru_t2),
-                                        q(OCEAN.rv_t2), q(OCEAN.ubar_t1), q(OCEAN.vbar_t1), q(OCEAN.ubar_t2),
-                                        q(OCEAN.vbar_t2), q(Hz), q(OCEAN.AKv), q(z_r), q(OCEAN.DU_avg1),
-                                        q(OCEAN.DV_avg1), q(st.DU_avg2), q(st.DV_avg2), q(Huon), q(Hvom),
-                                        q(m3.tmpU), q(m3.tmpV), int(compTimes.iic), int(compTimes.ntfirst), 1.0,
-                                        float(compTimes.dt), q(m3.eqSD), q(m3.eqD), q(m3.eqRHS), stream),
+    a = sf_uv_coupled(q(OCEAN.u_t2), q(OCEAN.v_t2), q(OCEAN.ru_t2), q(OCEAN.rv_t2), q(st.ruh[h1]), q(st.rvh[h1]),
+                      q(st.ruh[h2]), q(st.rvh[h2]), q(st.ruh[cur]), q(st.rvh[cur]), q(OCEAN.ubar_t1),
+                      q(OCEAN.vbar_t1), q(OCEAN.ubar_t2), q(OCEAN.vbar_t2), q(st.Hz_old), q(Hz), q(OCEAN.AKv), q(z_r),
+                      q(OCEAN.DU_avg1), q(OCEAN.DV_avg1), q(st.DU_avg2), q(st.DV_avg2), q(Huon), q(Hvom),
+                      q(st.sustr), q(st.svstr), st.rdrg, ab_order(n), q(m3.eqSD), q(m3.eqD), q(m3.eqRHS))
+    _lib.check(lib.sf_step3d_uv_coupled(c.handle, prec, C.byref(a), 1.0, float(compTimes.dt), stream),
                "sf_step3d_uv_coupled")
     if st.tracers is not None:
         st.tracers.step(GRID, OCEAN, compTimes.dt)

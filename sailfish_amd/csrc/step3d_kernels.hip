@@ -840,23 +840,91 @@ __device__ void couple_column(const G3 &g, bool vdir, long long f, T *x, const T
 #undef DC
 }
 
+// The 3D momentum step of the coupled form (ROMS step3d_uv): per interior u (v) column,
+//   Hz_u,new u_new - dt d/dz(Akv du_new/dz) = Hz_u,old u + dt pm_u pn_u AB3(ru)
+//                                             - dt rdrg u(0) [bottom] + dt sustr [top],
+// with SD(K) = (-lambda dt) Akv_u(K+1) / dz_u(K+1) (K = 0..N-2), D(K) = (Hz_u(K) - SD(K))
+// - SD(K-1), AB3 over this step's ru and the two stored before it (this ru is stored for
+// the next step), solved by the Thomas algorithm in the scratch columns; couple_column
+// then replaces the column mean.  The reference's solve (build_matrix above) multiplies
+// ru by Hz a second time and keeps no history: with the coupling moving the 3D flow that
+// form grows by about Hz f dt per step, so the coupled form solves ROMS's equation.
 template <class T>
-struct CplUV {
-    const double *DU2, *DV2;
+struct UVC {
+    T *u, *v;
+    const T *ru, *rv, *ru1, *rv1, *ru2, *rv2;
+    T *ru_keep, *rv_keep;
+    T *ubar_t1, *vbar_t1, *ubar_t2, *vbar_t2;
+    const T *Hz_old, *Hz, *Akv, *z_r;
+    const double *DU_avg1, *DV_avg1, *DU_avg2, *DV_avg2;
     T *Huon, *Hvom;
+    const double *sustr, *svstr;
+    double rdrg;
+    int ab;
+    T *SD, *D, *RHS;
 };
 
+__device__ inline bool coupled_point(const G3 &g, bool vdir, long long f)
+{
+    const int i = (int)(f % g.szI), j = (int)(f / g.szI), L = g.szI - 1, M = g.szJ - 1;
+    return vdir ? (j >= 2 && j <= M - 1 && i >= 1 && i <= L - 1) : (i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1);
+}
+
 template <class T>
-__global__ __launch_bounds__(256) void k_step3d_uv_coupled(G3 g, UVArgs<T> a, CplUV<T> c, double cff, double lambda,
-                                                           double Dt)
+__device__ void momentum_column(const G3 &g, bool vdir, long long f, const UVC<T> &a, double lambda, double Dt)
+{
+    const long long S2 = g.S2, N = g.N;
+    const long long d = vdir ? g.szI : 1;
+    T *x = vdir ? a.v : a.u, *keep = vdir ? a.rv_keep : a.ru_keep;
+    const T *r = vdir ? a.rv : a.ru, *r1 = vdir ? a.rv1 : a.ru1, *r2 = vdir ? a.rv2 : a.ru2;
+    const double stress = vdir ? a.svstr[f] : a.sustr[f];
+    T *SD = a.SD, *D = a.D, *RHS = a.RHS;
+    const T *Hz = a.Hz, *Ho = a.Hz_old, *Akv = a.Akv, *z_r = a.z_r;
+#define AVG(A, K) (((A)[(K) * S2 + f] + (A)[(K) * S2 + f - d]) * T(0.5))
+#define DZ(K) (((z_r[(K) * S2 + f] - z_r[((K) - 1) * S2 + f]) + (z_r[(K) * S2 + f - d] - z_r[((K) - 1) * S2 + f - d])) * T(0.5))
+    const double pmU = (g.pm[f] + g.pm[f - d]) * 0.5;
+    const double pnU = (g.pn[f] + g.pn[f - d]) * 0.5;
+    const T cdt_mn = T(Dt * (pmU * pnU));
+    const T mldt = T((-lambda) * Dt);
+    for (long long K = 0; K < N - 1; K++) SD[K * S2 + f] = (mldt * AVG(Akv, K + 1)) / DZ(K + 1);
+    SD[(N - 1) * S2 + f] = T(0);
+    for (long long K = 0; K < N; K++) {
+        const long long b = K * S2 + f;
+        const T rk = r[b];
+        T rab;
+        if (a.ab == 0) rab = rk;
+        else if (a.ab == 1) rab = T(1.5) * rk - T(0.5) * r1[b];
+        else rab = (T(23.0 / 12.0) * rk - T(16.0 / 12.0) * r1[b]) + T(5.0 / 12.0) * r2[b];
+        keep[b] = rk;
+        RHS[b] = AVG(Ho, K) * x[b] + cdt_mn * rab;
+        D[b] = K == 0 ? AVG(Hz, 0) - SD[f] : (AVG(Hz, K) - SD[b]) - SD[b - S2];
+    }
+    RHS[f] = RHS[f] - T(Dt * a.rdrg) * x[f];
+    RHS[(N - 1) * S2 + f] = RHS[(N - 1) * S2 + f] + T(Dt * stress);
+#undef AVG
+#undef DZ
+    for (long long k = 1; k < N; k++) {
+        const T w = SD[(k - 1) * S2 + f] / D[(k - 1) * S2 + f];
+        D[k * S2 + f] -= w * SD[(k - 1) * S2 + f];
+        RHS[k * S2 + f] -= w * RHS[(k - 1) * S2 + f];
+    }
+    x[(N - 1) * S2 + f] = RHS[(N - 1) * S2 + f] / D[(N - 1) * S2 + f];
+    for (long long k = N - 2; k >= 0; k--)
+        x[k * S2 + f] = (RHS[k * S2 + f] - SD[k * S2 + f] * x[(k + 1) * S2 + f]) / D[k * S2 + f];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_step3d_uv_coupled(G3 g, UVC<T> a, double lambda, double Dt)
 {
     COLUMN_INDEX
-    build_matrix(g, false, f, cff, Dt, lambda, a.SD, a.D, a.RHS, a.Hz, a.Akv, a.z_r, a.tmpU, a.ru);
-    solve_tri(g, f, a.SD, a.D, a.RHS, a.u);
-    build_matrix(g, true, f, cff, Dt, lambda, a.SD, a.D, a.RHS, a.Hz, a.Akv, a.z_r, a.tmpV, a.rv);
-    solve_tri(g, f, a.SD, a.D, a.RHS, a.v);
-    couple_column(g, false, f, a.u, a.Hz, a.DU_avg1, c.DU2, c.Huon, a.ubar_t1, a.ubar_t2);
-    couple_column(g, true, f, a.v, a.Hz, a.DV_avg1, c.DV2, c.Hvom, a.vbar_t1, a.vbar_t2);
+    if (coupled_point(g, false, f)) {
+        momentum_column(g, false, f, a, lambda, Dt);
+        couple_column(g, false, f, a.u, a.Hz, a.DU_avg1, a.DU_avg2, a.Huon, a.ubar_t1, a.ubar_t2);
+    }
+    if (coupled_point(g, true, f)) {
+        momentum_column(g, true, f, a, lambda, Dt);
+        couple_column(g, true, f, a.v, a.Hz, a.DV_avg1, a.DV_avg2, a.Hvom, a.vbar_t1, a.vbar_t2);
+    }
 }
 
 G3 grid3(const sf_ctx *c)
@@ -1120,20 +1188,14 @@ int couple_rufrc_launch(sf_ctx *c, const void *ru, const void *rv, const void *u
 }
 
 template <class T>
-int step3d_uv_coupled_launch(sf_ctx *c, T *u, T *v, const T *ru, const T *rv, T *ubar_t1, T *vbar_t1, T *ubar_t2,
-                             T *vbar_t2, const T *Hz, const T *Akv, const T *z_r, const double *DU_avg1,
-                             const double *DV_avg1, const double *DU_avg2, const double *DV_avg2, T *Huon, T *Hvom,
-                             const T *tmpU, const T *tmpV, int32_t iic, int32_t ntfirst, double lambda, double Dt,
-                             T *SD, T *D, T *RHS, void *stream)
+int step3d_uv_coupled_launch(sf_ctx *c, const sf_uv_coupled *p, double lambda, double Dt, void *stream)
 {
-    double cff;
-    if (iic == ntfirst) cff = 1.0;
-    else if (iic == ntfirst + 1) cff = 3.0 / 2.0;
-    else cff = 23.0 / 12.0;
-    UVArgs<T> a{u, v, ru, rv, ubar_t1, vbar_t1, ubar_t2, vbar_t2, Hz, Akv, z_r, DU_avg1, DV_avg1, tmpU, tmpV,
-                SD, D, RHS};
-    CplUV<T> cp{DU_avg2, DV_avg2, Huon, Hvom};
-    LAUNCH_COLUMNS(k_step3d_uv_coupled<T>, c, stream, grid3(c), a, cp, cff, lambda, Dt);
+    UVC<T> a{(T *)p->u, (T *)p->v, (const T *)p->ru, (const T *)p->rv, (const T *)p->ru1, (const T *)p->rv1,
+             (const T *)p->ru2, (const T *)p->rv2, (T *)p->ru_keep, (T *)p->rv_keep, (T *)p->ubar_t1,
+             (T *)p->vbar_t1, (T *)p->ubar_t2, (T *)p->vbar_t2, (const T *)p->Hz_old, (const T *)p->Hz,
+             (const T *)p->Akv, (const T *)p->z_r, p->DU_avg1, p->DV_avg1, p->DU_avg2, p->DV_avg2, (T *)p->Huon,
+             (T *)p->Hvom, p->sustr, p->svstr, p->rdrg, p->ab, (T *)p->SD, (T *)p->D, (T *)p->RHS};
+    LAUNCH_COLUMNS(k_step3d_uv_coupled<T>, c, stream, grid3(c), a, lambda, Dt);
 }
 
 }  // namespace
@@ -1152,30 +1214,23 @@ extern "C" int sf_couple_rufrc(sf_ctx *c, int32_t precision, const void *ru, con
     return SF_ERR_ARG;
 }
 
-extern "C" int sf_step3d_uv_coupled(sf_ctx *c, int32_t precision, void *u, void *v, const void *ru, const void *rv,
-                                    void *ubar_t1, void *vbar_t1, void *ubar_t2, void *vbar_t2, const void *Hz,
-                                    const void *Akv, const void *z_r, const double *DU_avg1, const double *DV_avg1,
-                                    const double *DU_avg2, const double *DV_avg2, void *Huon, void *Hvom,
-                                    const void *tmpU, const void *tmpV, int32_t iic, int32_t ntfirst, double lambda,
-                                    double Dt, void *SD, void *D, void *RHS, void *stream)
+extern "C" int sf_step3d_uv_coupled(sf_ctx *c, int32_t precision, const sf_uv_coupled *p, double lambda,
+                                    double Dt, void *stream)
 {
     CTX3(c);
-    SF_REQUIRE(u && v && ru && rv && ubar_t1 && vbar_t1 && ubar_t2 && vbar_t2 && Hz && Akv && z_r && DU_avg1 &&
-                   DV_avg1 && DU_avg2 && DV_avg2 && Huon && Hvom && tmpU && tmpV && SD && D && RHS,
+    SF_REQUIRE(p != nullptr, "sf_step3d_uv_coupled: null fields");
+    SF_REQUIRE(p->u && p->v && p->ru && p->rv && p->ru_keep && p->rv_keep && p->ubar_t1 && p->vbar_t1 &&
+                   p->ubar_t2 && p->vbar_t2 && p->Hz_old && p->Hz && p->Akv && p->z_r && p->DU_avg1 && p->DV_avg1 &&
+                   p->DU_avg2 && p->DV_avg2 && p->Huon && p->Hvom && p->sustr && p->svstr && p->SD && p->D &&
+                   p->RHS,
                "sf_step3d_uv_coupled: null field");
-    if (precision == SF_FP64)
-        return step3d_uv_coupled_launch<double>(
-            c, (double *)u, (double *)v, (const double *)ru, (const double *)rv, (double *)ubar_t1,
-            (double *)vbar_t1, (double *)ubar_t2, (double *)vbar_t2, (const double *)Hz, (const double *)Akv,
-            (const double *)z_r, DU_avg1, DV_avg1, DU_avg2, DV_avg2, (double *)Huon, (double *)Hvom,
-            (const double *)tmpU, (const double *)tmpV, iic, ntfirst, lambda, Dt, (double *)SD, (double *)D,
-            (double *)RHS, stream);
-    if (precision == SF_FP32)
-        return step3d_uv_coupled_launch<float>(
-            c, (float *)u, (float *)v, (const float *)ru, (const float *)rv, (float *)ubar_t1, (float *)vbar_t1,
-            (float *)ubar_t2, (float *)vbar_t2, (const float *)Hz, (const float *)Akv, (const float *)z_r, DU_avg1,
-            DV_avg1, DU_avg2, DV_avg2, (float *)Huon, (float *)Hvom, (const float *)tmpU, (const float *)tmpV, iic,
-            ntfirst, lambda, Dt, (float *)SD, (float *)D, (float *)RHS, stream);
+    SF_REQUIRE(p->ab >= 0 && p->ab <= 2, "sf_step3d_uv_coupled: ab must be 0, 1 or 2 (got %d)", p->ab);
+    SF_REQUIRE(p->ab < 1 || (p->ru1 && p->rv1), "sf_step3d_uv_coupled: ab %d needs ru1, rv1", p->ab);
+    SF_REQUIRE(p->ab < 2 || (p->ru2 && p->rv2), "sf_step3d_uv_coupled: ab 2 needs ru2, rv2");
+    SF_REQUIRE(p->ru_keep != p->ru1 && p->ru_keep != p->ru2 && p->rv_keep != p->rv1 && p->rv_keep != p->rv2,
+               "sf_step3d_uv_coupled: ru_keep / rv_keep alias the history");
+    if (precision == SF_FP64) return step3d_uv_coupled_launch<double>(c, p, lambda, Dt, stream);
+    if (precision == SF_FP32) return step3d_uv_coupled_launch<float>(c, p, lambda, Dt, stream);
     sf::set_error("sf_step3d_uv_coupled: unknown precision %d", precision);
     return SF_ERR_ARG;
 }

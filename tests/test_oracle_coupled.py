@@ -202,63 +202,179 @@ def test_avg2_of_a_nearly_frozen_state_is_its_flux():
     assert np.allclose(DV2[fv], dvom, rtol=1e-6, atol=1e-9 * np.abs(dvom).max())
 
 
-def test_coupled_step3d_carries_the_averaged_transports():
-    szI, szJ, N = 14, 11, 6
-    rng = np.random.default_rng(3)
+def _momentum_case(seed, N=6, szI=14, szJ=11):
+    """A sloping basin with new and old thicknesses that differ, random velocities, right-hand
+    sides (this step and two before), viscosity, stresses and averaged transports."""
     from sailfish_amd import configs
-    cfg = configs.random_smooth(szI, szJ, seed=6)
-    g, M, m = grid3(cfg, N)
+    rng = np.random.default_rng(seed)
+    cfg = configs.random_smooth(szI, szJ, seed=seed + 3)
+    g, M, _ = grid3(cfg, N)
     S2 = g.S2
-    Zt = 0.1 * rng.standard_normal(S2)
     st = O.OState(g)
-    st.Zt_avg1[:] = Zt
+    st.Zt_avg1[:] = 0.1 * rng.standard_normal(S2)
     M.set_depth(st)
-    n3 = (N + 1) * S2
-    u, v = 0.1 * rng.standard_normal(n3), 0.1 * rng.standard_normal(n3)
-    ru, rv = rng.standard_normal(n3), rng.standard_normal(n3)
-    M.AKv[:] = 1e-3 * (1 + rng.random(n3))
-    Hu0, Hv0 = 100 * rng.standard_normal(n3), 100 * rng.standard_normal(n3)
-    DU1, DV1, DU2, DV2 = (1e4 * rng.standard_normal(S2) for _ in range(4))
-    ub = [np.zeros(S2) for _ in range(4)]
-    L = O.lib()
+    n3, nk = (N + 1) * S2, N * S2
+    c = dict(cfg=cfg, g=g, M=M, N=N, szI=szI, szJ=szJ, S2=S2)
+    c["Hz_old"] = M.Hz[:nk] * (1.0 + 0.02 * rng.standard_normal(nk))
+    c["u"], c["v"] = 0.1 * rng.standard_normal(n3), 0.1 * rng.standard_normal(n3)
+    c["ru"], c["rv"] = 1e5 * rng.standard_normal(n3), 1e5 * rng.standard_normal(n3)
+    c["ru1"], c["rv1"], c["ru2"], c["rv2"] = (1e5 * rng.standard_normal(nk) for _ in range(4))
+    M.AKv[:] = 1e-2 * (1 + rng.random(n3))
+    c["Hu0"], c["Hv0"] = 100 * rng.standard_normal(n3), 100 * rng.standard_normal(n3)
+    c["DU1"], c["DV1"], c["DU2"], c["DV2"] = (1e4 * rng.standard_normal(S2) for _ in range(4))
+    c["sustr"], c["svstr"] = 1e-4 * rng.standard_normal(S2), 1e-4 * rng.standard_normal(S2)
+    return c
 
-    def run(coupled):
-        uu, vv, Hu, Hv = u.copy(), v.copy(), Hu0.copy(), Hv0.copy()
-        tmpU, tmpV = u.copy(), v.copy()
-        if coupled:
-            L.sfo_step3d_uv_coupled(M.cp, P(uu), P(vv), P(ru), P(rv), P(ub[0]), P(ub[1]), P(ub[2]), P(ub[3]),
-                                    P(M.Hz), P(M.AKv), P(M.z_r), P(DU1), P(DV1), P(DU2), P(DV2), P(Hu), P(Hv),
-                                    P(tmpU), P(tmpV), 3, 0, 1.0, 60.0, P(M.eqSD), P(M.eqD), P(M.eqRHS))
-        else:
-            bars = [np.zeros(S2) for _ in range(4)]     # kept alive across the call
-            L.sfo_step3d_UV(M.cp, P(uu), P(vv), P(ru), P(rv), P(bars[0]), P(bars[1]), P(bars[2]), P(bars[3]),
-                            P(M.Hz), P(M.AKv), P(M.z_r), P(DU1), P(DV1), P(tmpU), P(tmpV), 3, 0,
-                            1.0, 60.0, P(M.eqSD), P(M.eqD), P(M.eqRHS))
-        return uu.reshape(N + 1, szJ, szI), vv.reshape(N + 1, szJ, szI), Hu.reshape(N + 1, szJ, szI), \
-            Hv.reshape(N + 1, szJ, szI)
 
-    uc, vc, Huc, Hvc = run(True)
-    up, vp, _, _ = run(False)
-    Hz = M.Hz.reshape(N, szJ, szI)
+def _momentum_run(c, ab, lam, dt=300.0, rdrg=2e-3):
+    M, S2 = c["M"], c["S2"]
+    out = {k: c[k].copy() for k in ("u", "v", "Hu0", "Hv0")}
+    keep = [np.full(c["N"] * S2, np.nan), np.full(c["N"] * S2, np.nan)]
+    bars = [np.zeros(S2) for _ in range(4)]
+    a = OC.sfo_uv_coupled(P(out["u"]), P(out["v"]), P(c["ru"]), P(c["rv"]), P(c["ru1"]), P(c["rv1"]), P(c["ru2"]),
+                          P(c["rv2"]), P(keep[0]), P(keep[1]), P(bars[0]), P(bars[1]), P(bars[2]), P(bars[3]),
+                          P(c["Hz_old"]), P(M.Hz), P(M.AKv), P(M.z_r), P(c["DU1"]), P(c["DV1"]), P(c["DU2"]),
+                          P(c["DV2"]), P(out["Hu0"]), P(out["Hv0"]), P(c["sustr"]), P(c["svstr"]), rdrg, ab,
+                          P(M.eqSD), P(M.eqD), P(M.eqRHS))
+    O.lib().sfo_step3d_uv_coupled(M.cp, C.byref(a), lam, dt)
+    out.update(keep_u=keep[0], keep_v=keep[1], ubar1=bars[0], vbar1=bars[1], ubar2=bars[2], vbar2=bars[3])
+    return out
+
+
+# Adams-Bashforth coefficients of the current, previous and second previous right-hand side
+AB = {0: (1.0, 0.0, 0.0), 1: (1.5, -0.5, 0.0), 2: (23 / 12, -16 / 12, 5 / 12)}
+
+
+def _columns(c, vdir):
+    """(j, i, neighbour offset, metric across the face, pm pn at the point) of the interior
+    u (vdir 0) or v (vdir 1) points."""
+    szI, szJ, cfg = c["szI"], c["szJ"], c["cfg"]
     pm, pn = cfg["pm"], cfg["pn"]
-    D1u, D1v = DU1.reshape(szJ, szI), DV1.reshape(szJ, szI)
-    D2u, D2v = DU2.reshape(szJ, szI), DV2.reshape(szJ, szI)
-    for j in range(1, szJ - 1):
-        for i in range(2, szI - 1):
-            DC = 2 / (pn[j, i - 1] + pn[j, i]) * 0.5 * (Hz[:, j, i] + Hz[:, j, i - 1])
-            assert (DC * uc[:N, j, i]).sum() == pytest.approx(D1u[j, i], rel=1e-12, abs=1e-9)
-            assert ub[2].reshape(szJ, szI)[j, i] == pytest.approx(D1u[j, i] / DC.sum(), rel=1e-13)
-            assert ub[0].reshape(szJ, szI)[j, i] == ub[2].reshape(szJ, szI)[j, i]
-            assert Huc[:N, j, i].sum() == pytest.approx(D2u[j, i], rel=1e-12, abs=1e-9)
-            # Huon(K) = 0.5 (Huon0 + u DC) - DC FC with one FC per column
-            fc = (0.5 * (Hu0.reshape(N + 1, szJ, szI)[:N, j, i] + uc[:N, j, i] * DC) - Huc[:N, j, i]) / DC
+    pts = []
+    if not vdir:
+        for j in range(1, szJ - 1):
+            for i in range(2, szI - 1):
+                pts.append((j, i, (j, i - 1), 2 / (pn[j, i - 1] + pn[j, i]),
+                            0.5 * (pm[j, i] + pm[j, i - 1]) * 0.5 * (pn[j, i] + pn[j, i - 1])))
+    else:
+        for j in range(2, szJ - 1):
+            for i in range(1, szI - 1):
+                pts.append((j, i, (j - 1, i), 2 / (pm[j - 1, i] + pm[j, i]),
+                            0.5 * (pm[j, i] + pm[j - 1, i]) * 0.5 * (pn[j, i] + pn[j - 1, i])))
+    return pts
+
+
+@pytest.mark.parametrize("ab", [0, 1, 2])
+def test_coupled_momentum_without_viscosity_closed_form(ab):
+    """lambda = 0: each level is Hz_new u_new = Hz_old u + dt pm pn AB3(ru) (- dt rdrg u at the
+    bottom, + dt sustr at the top), shifted by one constant per column so that the column
+    carries DU_avg1; the mass flux sums to DU_avg2; this ru is kept for the next step; the
+    walls and ghost rows are untouched."""
+    c = _momentum_case(11 + ab)
+    N, szI, szJ, dt, rdrg = c["N"], c["szI"], c["szJ"], 300.0, 2e-3
+    out = _momentum_run(c, ab, 0.0, dt, rdrg)
+    sh3 = (N + 1, szJ, szI)
+    Hn = c["M"].Hz.reshape(N, szJ, szI)
+    Ho = c["Hz_old"].reshape(N, szJ, szI)
+    co = AB[ab]
+    for vdir, (x, r, r1, r2, st, D1, D2, Hf0, Hf, bar, keep) in enumerate((
+            ("u", "ru", "ru1", "ru2", "sustr", "DU1", "DU2", "Hu0", "Hu0", "ubar2", "keep_u"),
+            ("v", "rv", "rv1", "rv2", "svstr", "DV1", "DV2", "Hv0", "Hv0", "vbar2", "keep_v"))):
+        x0 = c[x].reshape(sh3)
+        xn = out[x].reshape(sh3)
+        R, R1, R2 = (c[k].reshape(-1, szJ, szI) for k in (r, r1, r2))
+        S = c[st].reshape(szJ, szI)
+        mask = np.zeros((szJ, szI), bool)
+        for j, i, (jp, ip), met, mn in _columns(c, vdir):
+            mask[j, i] = True
+            hn = 0.5 * (Hn[:, j, i] + Hn[:, jp, ip])
+            ho = 0.5 * (Ho[:, j, i] + Ho[:, jp, ip])
+            rab = co[0] * R[:N, j, i] + co[1] * R1[:, j, i] + co[2] * R2[:, j, i]
+            rhs = ho * x0[:N, j, i] + dt * mn * rab
+            rhs[0] -= dt * rdrg * x0[0, j, i]
+            rhs[N - 1] += dt * S[j, i]
+            pre = rhs / hn
+            shift = xn[:N, j, i] - pre
+            assert np.ptp(shift) <= 1e-12 * (np.abs(pre).max() + 1), (vdir, j, i)
+            DC = met * hn
+            assert (DC * xn[:N, j, i]).sum() == pytest.approx(c[D1].reshape(szJ, szI)[j, i], rel=1e-12, abs=1e-9)
+            assert out[bar].reshape(szJ, szI)[j, i] == pytest.approx(c[D1].reshape(szJ, szI)[j, i] / DC.sum(),
+                                                                     rel=1e-13)
+            Hfn = out[Hf].reshape(sh3)[:N, j, i]
+            assert Hfn.sum() == pytest.approx(c[D2].reshape(szJ, szI)[j, i], rel=1e-12, abs=1e-9)
+            fc = (0.5 * (c[Hf0].reshape(sh3)[:N, j, i] + xn[:N, j, i] * DC) - Hfn) / DC
             assert np.ptp(fc) <= 1e-9 * (abs(fc).max() + 1)
-            # one shift of the uncoupled solve's velocity per column
-            assert np.ptp(uc[:N, j, i] - up[:N, j, i]) <= 1e-12
-            assert Huc[N, j, i] == Hu0.reshape(N + 1, szJ, szI)[N, j, i]      # top level untouched
-    for j in range(2, szJ - 1):
-        for i in range(1, szI - 1):
-            DC = 2 / (pm[j - 1, i] + pm[j, i]) * 0.5 * (Hz[:, j, i] + Hz[:, j - 1, i])
-            assert (DC * vc[:N, j, i]).sum() == pytest.approx(D1v[j, i], rel=1e-12, abs=1e-9)
-            assert ub[3].reshape(szJ, szI)[j, i] == pytest.approx(D1v[j, i] / DC.sum(), rel=1e-13)
-            assert Hvc[:N, j, i].sum() == pytest.approx(D2v[j, i], rel=1e-12, abs=1e-9)
+            assert np.array_equal(out[keep].reshape(N, szJ, szI)[:, j, i], R[:N, j, i])
+        assert np.array_equal(xn[:, ~mask], x0[:, ~mask])            # walls and ghost rows untouched
+        assert np.array_equal(xn[N], x0[N])                          # level N untouched
+
+
+def test_coupled_momentum_viscosity_is_the_tridiagonal_solve():
+    """lambda = 1: the column solve equals a dense solve of
+    Hz_new(K) x(K) - dt [Akv(K+1) (x(K+1) - x(K)) / dz(K+1) - Akv(K) (x(K) - x(K-1)) / dz(K)] = rhs(K)
+    with Akv at the interfaces (K+1 between levels K and K+1; none through the bottom
+    and the top), before the column shift."""
+    c = _momentum_case(21)
+    N, szI, szJ, dt, rdrg = c["N"], c["szI"], c["szJ"], 300.0, 2e-3
+    out = _momentum_run(c, 2, 1.0, dt, rdrg)
+    M = c["M"]
+    Hn = M.Hz.reshape(N, szJ, szI)
+    Ho = c["Hz_old"].reshape(N, szJ, szI)
+    Akv = M.AKv.reshape(N + 1, szJ, szI)
+    z_r = M.z_r[:N * c["S2"]].reshape(N, szJ, szI)
+    co = AB[2]
+    x0 = c["u"].reshape(N + 1, szJ, szI)
+    xn = out["u"].reshape(N + 1, szJ, szI)
+    R, R1, R2 = (c[k].reshape(-1, szJ, szI) for k in ("ru", "ru1", "ru2"))
+    S = c["sustr"].reshape(szJ, szI)
+    worst = 0.0
+    for j, i, (jp, ip), met, mn in _columns(c, 0):
+        av = lambda A: 0.5 * (A[:, j, i] + A[:, jp, ip])
+        hn, ho, ak, zr = av(Hn), av(Ho), av(Akv), av(z_r)
+        kap = dt * ak[1:N] / np.diff(zr)               # interface K+1, K = 0..N-2
+        A = np.diag(hn.copy())
+        for K in range(N - 1):
+            A[K, K] += kap[K]
+            A[K + 1, K + 1] += kap[K]
+            A[K, K + 1] -= kap[K]
+            A[K + 1, K] -= kap[K]
+        rhs = ho * x0[:N, j, i] + dt * mn * (co[0] * R[:N, j, i] + co[1] * R1[:, j, i] + co[2] * R2[:, j, i])
+        rhs[0] -= dt * rdrg * x0[0, j, i]
+        rhs[N - 1] += dt * S[j, i]
+        pre = np.linalg.solve(A, rhs)
+        shift = xn[:N, j, i] - pre
+        worst = max(worst, np.ptp(shift) / (np.abs(pre).max() + 1))
+    assert worst <= 1e-11, worst
+
+
+def test_coupled_form_stays_bounded_with_thick_layers():
+    """The failure the reference's solve shows once the coupling moves the 3D flow: with
+    100 m layers, f = 1e-4 and dt = 300 s its extra factor Hz f dt grows the flow about
+    sevenfold per step.  The coupled form keeps a wind-driven basin of 125 m layers
+    bounded over 30 steps."""
+    from sailfish_amd import configs
+    from sailfish_amd.mod_boundary import Boundary
+    from sailfish_amd.mod_constants import g as G
+    from sailfish_amd.mod_grid import Grid
+    szI, szJ, N = 40, 24, 8
+    cfg = configs.basin3(szI, szJ, N)
+    GR = Grid(szI - 2, szJ - 2, N, device="cpu", Vtransform=2, Vstretching=4, theta_s=5.0, theta_b=0.4,
+              Tcline=50.0)
+    B = Boundary(GR, None, device="cpu")
+    m = update_metrics(cfg["pm"], cfg["pn"], cfg["f"])
+    sc_r, sc_w, Cs_r, Cs_w = s_coordinates(N, 4, 5.0, 0.4)
+    ct = CompTimes(dt=cfg["dt"], ndtfast=cfg["ndtfast"])
+    set_weights(ct)
+    o = OC.OCoupled(cfg, N, m, sc_r, sc_w, Cs_r, Cs_w, 50.0, B.zetaBC.bcIdxFieldIdx2, B.uvelBC.bcIdxFieldType,
+                    B.vvelBC.bcIdxFieldType, rdrg=3e-4, g=G)
+    o.m.AKv[:] = 2e-3
+    o.s.Zt_avg1[:] = np.asarray(cfg["zeta0"], float).ravel()
+    o.m.set_depth(o.s)
+    assert o.m.Hz.max() > 100.0
+    t = 0.0
+    for iic in range(30):
+        o.baroclinic_step(iic, 0, ct.nfast, ct.dtfast, ct.dt, ct.weight1, ct.weight2, t)
+        t += ct.dt
+    u = o.m.u[2].reshape(N + 1, szJ, szI)[:N]
+    assert np.isfinite(u).all()
+    assert 1e-4 < np.abs(u).max() < 0.5

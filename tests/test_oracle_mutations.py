@@ -165,6 +165,25 @@ MUTANTS = [
      "FC = rD * (FC - Davg2[f]);", "FC = rD * (FC - Davg1[f]);"),
     ("cpl_ubar_without_depth", "sf_oracle3d.c",
      "xbar2[f] = rD * Davg1[f];", "xbar2[f] = Davg1[f];"),
+    ("cpl3d_reference_rhs_form", "sf_oracle3d.c",
+     "RHS[b] = AVG(Ho, K) * x[b] + cdt_mn * rab;", "RHS[b] = AVG(Hz, K) * (x[b] + cdt_mn * rab);"),
+    ("cpl3d_new_thickness_for_old", "sf_oracle3d.c",
+     "RHS[b] = AVG(Ho, K) * x[b] + cdt_mn * rab;", "RHS[b] = AVG(Hz, K) * x[b] + cdt_mn * rab;"),
+    ("cpl3d_akv_interface_below", "sf_oracle3d.c",
+     "AVG(Akv, K + 1)) / DZ(K + 1);", "AVG(Akv, K)) / DZ(K + 1);"),
+    ("cpl3d_ab3_last_term", "sf_oracle3d.c",
+     "(5.0 / 12.0) * r2;", "(5.0 / 12.0) * r1;"),
+    ("cpl3d_ab2_coefficient", "sf_oracle3d.c",
+     "if (ab == 1) return 1.5 * r - 0.5 * r1;", "if (ab == 1) return 1.5 * r - 0.5 * r;"),
+    ("cpl3d_bottom_drag_dropped", "sf_oracle3d.c",
+     "RHS[f] = RHS[f] - (Dt * a->rdrg) * x[f];", "RHS[f] = RHS[f] - 0.0 * x[f];"),
+    ("cpl3d_surface_stress_sign", "sf_oracle3d.c",
+     "RHS[(N - 1) * S2 + f] = RHS[(N - 1) * S2 + f] + Dt * stress[f];",
+     "RHS[(N - 1) * S2 + f] = RHS[(N - 1) * S2 + f] - Dt * stress[f];"),
+    ("cpl3d_keeps_extrapolation", "sf_oracle3d.c",
+     "keep[b] = rk;", "keep[b] = rab;"),
+    ("cpl3d_walls_solved", "sf_oracle3d.c",
+     "        if (coupled_point(g, 0, f)) {", "        if (isUNode(g, f)) {"),
     # --- open edges (Chapman, Flather, volume constraint) ---
     ("open_flather_sign", "sf_oracle_phys.c",
      "x[fu] = (e == 0 || e == 2) ? ext_at(ob->ext_u[e], b) - c * dz : ext_at(ob->ext_u[e], b) + c * dz;",
