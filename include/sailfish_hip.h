@@ -192,7 +192,11 @@ int sf_set_lateral_uv_bcs_f32(sf_ctx *ctx, double t, float *u, float *v, const i
 
 /* step3d_UV (nonlinear/step3D_uv.kernels.cpp:646-707) with its scratch arrays SD, D,
  * RHS ((szK+1)*S2 doubles each, nonlinear/main3d.py:48-58).  sf_step3d_uv_f32 (declared
- * below) is its FP32 bulk form for the same call site (main3d.py:304-306). */
+ * below) is its FP32 bulk form for the same call site (main3d.py:304-306).  With at
+ * most SF_STEP3D_REG_LEVELS levels each column's system stays in registers (the scratch
+ * is not touched) and tmpU / tmpV may be u / v themselves, so the copies main3d.py:48-52
+ * makes before the call are not needed; deeper grids use the scratch and need the copies. */
+#define SF_STEP3D_REG_LEVELS 32
 int sf_step3d_uv(sf_ctx *ctx, double *u, double *v, const double *ru, const double *rv,
                  double *ubar_t1, double *vbar_t1, double *ubar_t2, double *vbar_t2,
                  const double *Hz, const double *Akv, const double *z_r, const double *DU_avg1,

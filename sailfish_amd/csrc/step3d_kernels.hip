@@ -774,6 +774,149 @@ __global__ __launch_bounds__(256) void k_step3d_uv(G3 g, UVArgs<T> a, double cff
     a.vbar_t1[f] = T(vb);
 }
 
+// step3d_UV with each column's tridiagonal system held in registers (same arithmetic,
+// same order as k_step3d_uv; N <= NMAX).  The matrix of a direction is built and
+// eliminated level by level (build_matrix and the forward sweep of solve_tri fused),
+// the back substitution stores the solution with correct_mean's shift applied, and the
+// depth sums of correct_mean and adjustBarotropicVelocity run in the reference's level
+// order (the barotropic sums re-read the stored velocities).  At u-but-not-v points the
+// v pass solves the u pass's eliminated system a second time, as solve_tri does on the
+// shared scratch.  Each thread reads tmpU / tmpV only before it writes u / v, so they may
+// alias u / v.
+template <class T, int NMAX>
+__global__ __launch_bounds__(256) void k_step3d_uv_reg(G3 g, UVArgs<T> a, double cff, double lambda, double Dt)
+{
+    COLUMN_INDEX
+    const long long S2 = g.S2;
+    const int N = g.N;
+    const bool un = isUNode(g, f), vn = isVNode(g, f);
+    if (!un && !vn) return;
+    T SD[NMAX], D[NMAX], R[NMAX];
+    const T mldt = T((-lambda) * Dt);
+    // build_matrix + forward elimination of direction vdir; Hs, Hx: correct_mean's sums.
+    // The operands of level K+1 are loaded before level K is computed.
+    auto build_elim = [&](bool vdir, const T *xold, const T *r, double &Hs, double &Hx) {
+        const long long d = vdir ? g.szI : 1;
+        const T pmU = (T(g.pm[f]) + T(g.pm[f - d])) * T(0.5);
+        const T pnU = (T(g.pn[f]) + T(g.pn[f - d])) * T(0.5);
+        const T cdt_mn = T(cff * Dt) * (pmU * pnU);
+        T zf0 = a.z_r[f], zd0 = a.z_r[f - d];
+        struct Lv {
+            T h0, h1, xo, rr, z0, z1, k0, k1;
+        };
+        auto load = [&](int K) {
+            Lv q{};
+            const long long b = K * S2 + f;
+            q.h0 = a.Hz[b];
+            q.h1 = a.Hz[b - d];
+            q.xo = xold[b];
+            q.rr = r[b];
+            if (K < N - 1) {
+                q.z0 = a.z_r[b + S2];
+                q.z1 = a.z_r[b + S2 - d];
+                q.k0 = a.Akv[b];
+                q.k1 = a.Akv[b - d];
+            }
+            return q;
+        };
+        Lv cur = load(0);
+        Hs = 0.0;
+        Hx = 0.0;
+#pragma unroll
+        for (int K = 0; K < NMAX; K++) {
+            if (K < N) {
+                Lv nxt{};
+                if (K + 1 < N) nxt = load(K + 1);
+                const T hz = (cur.h0 + cur.h1) * T(0.5);
+                const T xo = cur.xo;
+                Hs += (double)hz;
+                Hx += (double)xo * (double)hz;
+                T sd = T(0);
+                if (K < N - 1) {
+                    const T dz = ((cur.z0 - zf0) + (cur.z1 - zd0)) * T(0.5);
+                    sd = (mldt * ((cur.k0 + cur.k1) * T(0.5))) / dz;
+                    zf0 = cur.z0;
+                    zd0 = cur.z1;
+                }
+                SD[K] = sd;
+                if (K == 0) {
+                    D[0] = hz - T(2) * sd;
+                    R[0] = T(0);
+                } else {
+                    D[K] = (hz - sd) - SD[K - 1];
+                    R[K] = hz * (xo + cdt_mn * cur.rr);
+                    const T w = SD[K - 1] / D[K - 1];
+                    D[K] -= w * SD[K - 1];
+                    R[K] -= w * R[K - 1];
+                }
+                cur = nxt;
+            }
+        }
+    };
+    // the forward sweep of solve_tri on an already eliminated system
+    auto elim_again = [&]() {
+#pragma unroll
+        for (int K = 1; K < NMAX; K++) {
+            if (K < N) {
+                const T w = SD[K - 1] / D[K - 1];
+                D[K] -= w * SD[K - 1];
+                R[K] -= w * R[K - 1];
+            }
+        }
+    };
+    // back substitution; stores x - shift at every level of x
+    auto back_store = [&](T *x, double shift) {
+        T xn = T(0);
+#pragma unroll
+        for (int K = NMAX - 1; K >= 0; K--) {
+            if (K < N) {
+                const T xk = K == N - 1 ? R[K] / D[K] : (R[K] - SD[K] * xn) / D[K];
+                x[K * S2 + f] = T((double)xk - shift);
+                xn = xk;
+            }
+        }
+    };
+    double HU = 0.0, HuU = 0.0;
+    if (un) {
+        double Hs, Hx;
+        build_elim(false, a.tmpU, a.ru, Hs, Hx);
+        const double diff = (Hx - a.DU_avg1[f] / g.on_u[f]) / Hs;
+        back_store(a.u, diff);   // leaves SD, D, R as they are (reused below where !vn)
+    }
+    if (vn) {
+        double Hs, Hx;
+        build_elim(true, a.tmpV, a.rv, Hs, Hx);
+        const double diff = (Hx - a.DV_avg1[f] / g.om_v[f]) / Hs;
+        if (un) {
+            back_store(a.v, diff);
+        } else {
+            for (int K = 0; K < N; K++) a.v[K * S2 + f] = T((double)a.v[K * S2 + f] - diff);
+        }
+    } else {
+        elim_again();
+        back_store(a.v, 0.0);
+    }
+    if (!un) return;
+    double HV = 0.0, HvV = 0.0;
+    for (int K = 0; K < N; K++) {
+        const long long b = K * S2 + f;
+        const T hzu = (a.Hz[b] + a.Hz[b - 1]) * T(0.5);
+        const T hzv = (a.Hz[b] + a.Hz[b - g.szI]) * T(0.5);
+        HU += (double)hzu;
+        HV += (double)hzv;
+        HuU += (double)hzu * (double)a.u[b];
+        HvV += (double)hzv * (double)a.v[b];
+    }
+    const double uUp = (HuU - a.DU_avg1[f] / g.on_u[f]) / HU;
+    const double vUp = (HvV - a.DV_avg1[f] / g.om_v[f]) / HV;
+    const double ub = (a.DU_avg1[f] / g.on_u[f]) / HU + uUp;
+    const double vb = (a.DV_avg1[f] / g.om_v[f]) / HV + vUp;
+    a.ubar_t2[f] = T(ub);
+    a.vbar_t2[f] = T(vb);
+    a.ubar_t1[f] = T(ub);
+    a.vbar_t1[f] = T(vb);
+}
+
 // ---- coupling of the 2D and 3D momentum equations (ROMS split-explicit form) ----------
 // rufrc / rvfrc: vertical integral of the 3D right-hand side plus the surface and bottom
 // stresses (nonlinear/rhs.py:76-89, commented in the reference; bottom stress rdrg * u of
@@ -925,6 +1068,139 @@ __global__ __launch_bounds__(256) void k_step3d_uv_coupled(G3 g, UVC<T> a, doubl
         momentum_column(g, true, f, a, lambda, Dt);
         couple_column(g, true, f, a.v, a.Hz, a.DV_avg1, a.DV_avg2, a.Hvom, a.vbar_t1, a.vbar_t2);
     }
+}
+
+// The coupled momentum step with each column's system in registers (same arithmetic
+// and order as momentum_column + couple_column; N <= NMAX): the matrix is built and
+// eliminated level by level with the operands of the next level loaded ahead, the
+// solution stays in registers for the column sums, and the new thicknesses of the
+// mass-flux correction are kept in the slots of the eliminated diagonal.
+template <class T, int NMAX>
+__global__ __launch_bounds__(256) void k_step3d_uv_coupled_reg(G3 g, UVC<T> a, double lambda, double Dt)
+{
+    COLUMN_INDEX
+    const long long S2 = g.S2;
+    const int N = g.N;
+    __shared__ T sds[NMAX][256];   // the sub-diagonal of each column (thread)
+    T D[NMAX], R[NMAX];
+    const int tid = threadIdx.x;
+    const T mldt = T((-lambda) * Dt);
+    auto column = [&](bool vdir) {
+        const long long d = vdir ? g.szI : 1;
+        T *x = vdir ? a.v : a.u, *keep = vdir ? a.rv_keep : a.ru_keep, *Hf = vdir ? a.Hvom : a.Huon;
+        const T *r = vdir ? a.rv : a.ru, *r1 = vdir ? a.rv1 : a.ru1, *r2 = vdir ? a.rv2 : a.ru2;
+        const double stress = vdir ? a.svstr[f] : a.sustr[f];
+        const double pmU = (g.pm[f] + g.pm[f - d]) * 0.5;
+        const double pnU = (g.pn[f] + g.pn[f - d]) * 0.5;
+        const T cdt_mn = T(Dt * (pmU * pnU));
+        const int ab = a.ab;
+        struct Lv {
+            T hn0, hn1, ho0, ho1, xo, rk, rk1, rk2, z0, z1, k0, k1;
+        };
+        auto load = [&](int K) {
+            Lv q{};
+            const long long b = K * S2 + f;
+            q.hn0 = a.Hz[b];
+            q.hn1 = a.Hz[b - d];
+            q.ho0 = a.Hz_old[b];
+            q.ho1 = a.Hz_old[b - d];
+            q.xo = x[b];
+            q.rk = r[b];
+            if (ab > 0) q.rk1 = r1[b];
+            if (ab > 1) q.rk2 = r2[b];
+            if (K < N - 1) {
+                q.z0 = a.z_r[b + S2];
+                q.z1 = a.z_r[b + S2 - d];
+                q.k0 = a.Akv[b + S2];
+                q.k1 = a.Akv[b + S2 - d];
+            }
+            return q;
+        };
+        T zf0 = a.z_r[f], zd0 = a.z_r[f - d];
+        const T xbot = x[f];
+        T sdp = T(0);   // sub-diagonal of the level below
+        Lv cur = load(0);
+#pragma unroll
+        for (int K = 0; K < NMAX; K++) {
+            if (K < N) {
+                Lv nxt{};
+                if (K + 1 < N) nxt = load(K + 1);
+                const long long b = K * S2 + f;
+                T sd = T(0);
+                if (K < N - 1) {
+                    const T dz = ((cur.z0 - zf0) + (cur.z1 - zd0)) * T(0.5);
+                    sd = (mldt * ((cur.k0 + cur.k1) * T(0.5))) / dz;
+                    zf0 = cur.z0;
+                    zd0 = cur.z1;
+                }
+                sds[K][tid] = sd;
+                T rab;
+                if (ab == 0) rab = cur.rk;
+                else if (ab == 1) rab = T(1.5) * cur.rk - T(0.5) * cur.rk1;
+                else rab = (T(23.0 / 12.0) * cur.rk - T(16.0 / 12.0) * cur.rk1) + T(5.0 / 12.0) * cur.rk2;
+                keep[b] = cur.rk;
+                T rhs = ((cur.ho0 + cur.ho1) * T(0.5)) * cur.xo + cdt_mn * rab;
+                const T hn = (cur.hn0 + cur.hn1) * T(0.5);
+                if (K == 0) {
+                    D[0] = hn - sd;
+                    R[0] = rhs - T(Dt * a.rdrg) * xbot;
+                } else {
+                    D[K] = (hn - sd) - sdp;
+                    if (K == N - 1) rhs = rhs + T(Dt * stress);
+                    R[K] = rhs;
+                    const T w = sdp / D[K - 1];
+                    D[K] -= w * sdp;
+                    R[K] -= w * R[K - 1];
+                }
+                sdp = sd;
+                cur = nxt;
+            }
+        }
+        // back substitution: R(K) := x(K)
+#pragma unroll
+        for (int K = NMAX - 1; K >= 0; K--) {
+            if (K < N) R[K] = K == N - 1 ? R[K] / D[K] : (R[K] - sds[K][tid] * R[K + 1]) / D[K];
+        }
+        // couple_column: replace the column mean, correct the mass flux
+        const double met = vdir ? 2.0 / (g.pm[f - g.szI] + g.pm[f]) : 2.0 / (g.pn[f - 1] + g.pn[f]);
+        double Ds = 0.0, CF = 0.0;
+#pragma unroll
+        for (int K = 0; K < NMAX; K++) {
+            if (K < N) {
+                const long long b = K * S2 + f;
+                const T hn = (a.Hz[b] + a.Hz[b - d]) * T(0.5);
+                D[K] = hn;
+                const double dc = met * (double)hn;
+                Ds = Ds + dc;
+                CF = CF + dc * (double)R[K];
+            }
+        }
+        const double rD = 1.0 / Ds;
+        const double *Davg1 = vdir ? a.DV_avg1 : a.DU_avg1, *Davg2 = vdir ? a.DV_avg2 : a.DU_avg2;
+        CF = rD * (CF - Davg1[f]);
+        const T xb = T(rD * Davg1[f]);
+        (vdir ? a.vbar_t2 : a.ubar_t2)[f] = xb;
+        (vdir ? a.vbar_t1 : a.ubar_t1)[f] = xb;
+        double FC = 0.0;
+#pragma unroll
+        for (int K = 0; K < NMAX; K++) {
+            if (K < N) {
+                const long long b = K * S2 + f;
+                const T xk = T((double)R[K] - CF);
+                x[b] = xk;
+                const T hf = T(0.5 * ((double)Hf[b] + (double)xk * (met * (double)D[K])));
+                R[K] = hf;
+                FC = FC + (double)hf;
+            }
+        }
+        FC = rD * (FC - Davg2[f]);
+#pragma unroll
+        for (int K = 0; K < NMAX; K++) {
+            if (K < N) Hf[K * S2 + f] = T((double)R[K] - (met * (double)D[K]) * FC);
+        }
+    };
+    if (coupled_point(g, false, f)) column(false);
+    if (coupled_point(g, true, f)) column(true);
 }
 
 G3 grid3(const sf_ctx *c)
@@ -1139,6 +1415,20 @@ extern "C" int sf_set_lateral_uv_bcs_f32(sf_ctx *c, double t, float *u, float *v
     LAUNCH_COLUMNS(k_lateral_uv<float>, c, stream, grid3(c), t, u, v, uType, vType);
 }
 
+// step3d_UV: columns of up to 32 levels in registers (k_step3d_uv_reg), deeper ones
+// through the SD/D/RHS scratch (k_step3d_uv)
+template <class T>
+static int step3d_uv_launch(sf_ctx *c, const UVArgs<T> &a, double cff, double lambda, double Dt, void *stream)
+{
+    if (c->szK <= 16) {
+        LAUNCH_COLUMNS((k_step3d_uv_reg<T, 16>), c, stream, grid3(c), a, cff, lambda, Dt);
+    }
+    if (c->szK <= 32) {
+        LAUNCH_COLUMNS((k_step3d_uv_reg<T, 32>), c, stream, grid3(c), a, cff, lambda, Dt);
+    }
+    LAUNCH_COLUMNS(k_step3d_uv<T>, c, stream, grid3(c), a, cff, lambda, Dt);
+}
+
 extern "C" int sf_step3d_uv(sf_ctx *c, double *u, double *v, const double *ru, const double *rv, double *ubar_t1,
                             double *vbar_t1, double *ubar_t2, double *vbar_t2, const double *Hz, const double *Akv,
                             const double *z_r, const double *DU_avg1, const double *DV_avg1, const double *tmpU,
@@ -1155,7 +1445,7 @@ extern "C" int sf_step3d_uv(sf_ctx *c, double *u, double *v, const double *ru, c
     else cff = 23.0 / 12.0;
     UVArgs<double> a{u, v, ru, rv, ubar_t1, vbar_t1, ubar_t2, vbar_t2, Hz, Akv, z_r, DU_avg1, DV_avg1, tmpU, tmpV,
                      SD, D, RHS};
-    LAUNCH_COLUMNS(k_step3d_uv<double>, c, stream, grid3(c), a, cff, lambda, Dt);
+    return step3d_uv_launch(c, a, cff, lambda, Dt, stream);
 }
 
 extern "C" int sf_step3d_uv_f32(sf_ctx *c, float *u, float *v, const float *ru, const float *rv, float *ubar_t1,
@@ -1174,7 +1464,7 @@ extern "C" int sf_step3d_uv_f32(sf_ctx *c, float *u, float *v, const float *ru, 
     else cff = 23.0 / 12.0;
     UVArgs<float> a{u, v, ru, rv, ubar_t1, vbar_t1, ubar_t2, vbar_t2, Hz, Akv, z_r, DU_avg1, DV_avg1, tmpU, tmpV,
                     SD, D, RHS};
-    LAUNCH_COLUMNS(k_step3d_uv<float>, c, stream, grid3(c), a, cff, lambda, Dt);
+    return step3d_uv_launch(c, a, cff, lambda, Dt, stream);
 }
 
 namespace {
@@ -1195,6 +1485,12 @@ int step3d_uv_coupled_launch(sf_ctx *c, const sf_uv_coupled *p, double lambda, d
              (T *)p->vbar_t1, (T *)p->ubar_t2, (T *)p->vbar_t2, (const T *)p->Hz_old, (const T *)p->Hz,
              (const T *)p->Akv, (const T *)p->z_r, p->DU_avg1, p->DV_avg1, p->DU_avg2, p->DV_avg2, (T *)p->Huon,
              (T *)p->Hvom, p->sustr, p->svstr, p->rdrg, p->ab, (T *)p->SD, (T *)p->D, (T *)p->RHS};
+    if (c->szK <= 16) {
+        LAUNCH_COLUMNS((k_step3d_uv_coupled_reg<T, 16>), c, stream, grid3(c), a, lambda, Dt);
+    }
+    if (c->szK <= 32) {
+        LAUNCH_COLUMNS((k_step3d_uv_coupled_reg<T, 32>), c, stream, grid3(c), a, lambda, Dt);
+    }
     LAUNCH_COLUMNS(k_step3d_uv_coupled<T>, c, stream, grid3(c), a, lambda, Dt);
 }
 

@@ -38,7 +38,9 @@ def _run_gpu(cfg, N, dtype, ndtfast=6, dt=12.0):
     return ct, GRID, OCEAN, BOUNDARY, st
 
 
-@pytest.mark.parametrize("szI,szJ,N", [(40, 24, 6), (67, 33, 5)])
+# N <= 16 and 17..32 take the register forms of the coupled momentum step, N = 34 the
+# scratch form
+@pytest.mark.parametrize("szI,szJ,N", [(40, 24, 6), (67, 33, 5), (30, 20, 20), (28, 18, 34)])
 def test_coupled_fp64_bitwise_against_oracle(szI, szJ, N):
     import torch
     from oracle import oracle_coupled as OC

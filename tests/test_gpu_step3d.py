@@ -39,8 +39,10 @@ def _oracle3(cfg, GRID, ct, N, fused_fix_q5=True):
     return t, g, s, b, m
 
 
+# N <= 16 and 17..32 take the register forms of step3d_UV, N = 36 the scratch form
 @pytest.mark.parametrize("szI,szJ,N,fused", [(24, 14, 6, False), (40, 20, 8, True), (33, 17, 5, False),
-                                            (130, 21, 4, False), (36, 18, 17, True)])
+                                            (130, 21, 4, False), (36, 18, 17, True), (30, 16, 24, True),
+                                            (26, 14, 36, False)])
 def test_main3d_bitwise(szI, szJ, N, fused):
     import torch
     from oracle import oracle as O
