@@ -363,6 +363,15 @@ typedef struct sf_couple_fields {
 int sf_phys_substep_coupled(const sf_phys_fields *fields, const sf_couple_fields *cpl, int32_t precision,
                             double dt, int32_t first, double cff1, void *stream);
 
+/* sf_phys_substep_fused coupled with the baroclinic step: the coupling of
+ * sf_phys_substep_coupled inside the single-launch tile kernel (the forcing of the
+ * ghost-source rows kept in LDS, the DU_avg2 terms from the fluxes the tile already
+ * forms).  Tile height 4 or 8 (0: sf_phys_fused_default_tile).  FP64 results equal
+ * sf_phys_substep_coupled bit for bit; FP32 agrees to rounding (contraction on, as
+ * sf_phys_substep_fused). */
+int sf_phys_substep_coupled_fused(const sf_phys_fields *fields, const sf_couple_fields *cpl, int32_t precision,
+                                  double dt, int32_t first, double cff1, int32_t tile_j, void *stream);
+
 /* Fields of the coupled 3D momentum step.  3D fields, ubar/vbar, the right-hand-side
  * history and the scratch in the step's precision; the averages and the stresses FP64. */
 typedef struct sf_uv_coupled {

@@ -20,8 +20,9 @@ coupling of ROMS that the port keeps only as text and comments:
 One baroclinic step (``coupled_step``): set_maxflux, omega, set_zeta, rhs3d, rufrc,
 nfast+1 coupled substeps (the first forward/backward Euler, averages reset on it),
 set_depth, coupled step3d_UV, setLateralUVBCs, omega.  FP64 or FP32 bulk state (the
-averages, forcing and coupling sums stay FP64).  The barotropic substep is the
-four-launch physics form (``sf_phys_substep_coupled``)."""
+averages, forcing and coupling sums stay FP64).  The barotropic substep is the fused
+physics tile kernel with the coupling terms (``sf_phys_substep_coupled_fused``; the
+four-launch ``sf_phys_substep_coupled`` with ``fused=False``)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -57,6 +58,9 @@ def _bind():
         lib.sf_phys_substep_coupled.argtypes = [C.POINTER(sf_phys_fields), C.POINTER(sf_couple_fields), C.c_int32,
                                                 C.c_double, C.c_int32, C.c_double, P]
         lib.sf_phys_substep_coupled.restype = C.c_int32
+        lib.sf_phys_substep_coupled_fused.argtypes = [C.POINTER(sf_phys_fields), C.POINTER(sf_couple_fields),
+                                                      C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_int32, P]
+        lib.sf_phys_substep_coupled_fused.restype = C.c_int32
         lib.sf_couple_rufrc.argtypes = [P, C.c_int32, P, P, P, P, P, P, C.c_double, P, P, P]
         lib.sf_couple_rufrc.restype = C.c_int32
         lib.sf_step3d_uv_coupled.argtypes = [P, C.c_int32, C.POINTER(sf_uv_coupled), C.c_double, C.c_double, P]
@@ -87,9 +91,11 @@ class CoupledState:
     (this step and the two before it, rotated per baroclinic step), DU_avg2/DV_avg2,
     three 3D right-hand sides per component for the Adams-Bashforth step of u, v (rotated
     the same way), the thicknesses before set_depth, the FP64 stresses, and the physics
-    substep in its four-launch form."""
+    substep (``fused``: one launch per substep, sf_phys_substep_coupled_fused; else the
+    four-launch sf_phys_substep_coupled)."""
 
-    def __init__(self, GRID, OCEAN, rdrg: float = 3e-4, g: float | None = None, tracers=None):
+    def __init__(self, GRID, OCEAN, rdrg: float = 3e-4, g: float | None = None, tracers=None, fused: bool = True,
+                 tile_j: int = 0):
         _lib.require_gpu()
         self.lib = _bind()
         dev, S2 = GRID.device, GRID.szI * GRID.szJ
@@ -108,6 +114,8 @@ class CoupledState:
         self.sustr = torch.from_numpy(np.ascontiguousarray(host.get("sustr", zeros)).reshape(-1)).to(dev)
         self.svstr = torch.from_numpy(np.ascontiguousarray(host.get("svstr", zeros)).reshape(-1)).to(dev)
         self.rdrg = float(rdrg)
+        self.fused = bool(fused)
+        self.tile_j = int(tile_j)   # fused substep tile height (0: the library's default)
         kw = {} if g is None else {"g": g}
         self.stepper = PhysStepper(GRID, OCEAN, rdrg=rdrg, fused=False, **kw)
         from .main3d import Main3dState
@@ -166,9 +174,15 @@ def coupled_step(compTimes, GRID, OCEAN, BOUNDARY, st: CoupledState, history=Non
         OCEAN.cycleTimes2D()
         a = st.stepper.fields(OCEAN)
         cp = st.couple_fields(compTimes.iic - compTimes.ntfirst, compTimes.iif, compTimes.weight2)
-        _lib.check(lib.sf_phys_substep_coupled(C.byref(a), C.byref(cp), st.stepper.precision,
-                                               float(compTimes.dtfast), int(compTimes.iif == 1), compTimes.cff1(),
-                                               stream), "sf_phys_substep_coupled")
+        if st.fused:
+            _lib.check(lib.sf_phys_substep_coupled_fused(C.byref(a), C.byref(cp), st.stepper.precision,
+                                                         float(compTimes.dtfast), int(compTimes.iif == 1),
+                                                         compTimes.cff1(), st.tile_j, stream),
+                       "sf_phys_substep_coupled_fused")
+        else:
+            _lib.check(lib.sf_phys_substep_coupled(C.byref(a), C.byref(cp), st.stepper.precision,
+                                                   float(compTimes.dtfast), int(compTimes.iif == 1),
+                                                   compTimes.cff1(), stream), "sf_phys_substep_coupled")
     if history is not None:
         history.maybe_write(compTimes, OCEAN)
     st.Hz_old.copy_(GRID.Hz)
@@ -193,7 +207,8 @@ def coupled_step(compTimes, GRID, OCEAN, BOUNDARY, st: CoupledState, history=Non
 
 
 def main3d_coupled(compTimes, GRID, OCEAN, BOUNDARY, nsteps: int, st: CoupledState | None = None,
-                   rdrg: float = 3e-4, history=None, tracers: bool | dict = False) -> CoupledState:
+                   rdrg: float = 3e-4, history=None, tracers: bool | dict = False, fused: bool = True,
+                   tile_j: int = 0) -> CoupledState:
     """``nsteps`` coupled baroclinic steps after the initial set_depth (main3d.py:62-63).
     ``tracers``: True (or a dict of baroclinic.Tracers arguments, e.g. T=...) adds the
     baroclinic pressure gradient and the tracer step; the default temperature is the
@@ -206,7 +221,7 @@ def main3d_coupled(compTimes, GRID, OCEAN, BOUNDARY, nsteps: int, st: CoupledSta
             kw = dict(tracers) if isinstance(tracers, dict) else {}
             kw.setdefault("T", stratification(GRID))
             tr = Tracers(GRID, OCEAN, **kw)
-        st = CoupledState(GRID, OCEAN, rdrg=rdrg, tracers=tr)
+        st = CoupledState(GRID, OCEAN, rdrg=rdrg, tracers=tr, fused=fused, tile_j=tile_j)
     for _ in range(int(nsteps)):
         coupled_step(compTimes, GRID, OCEAN, BOUNDARY, st, history=history)
     return st

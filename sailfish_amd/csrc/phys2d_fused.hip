@@ -32,6 +32,30 @@ extern "C" int sf_phys_substep_fused(const sf_phys_fields *x, int32_t precision,
     return SF_ERR_ARG;
 }
 
+extern "C" int sf_phys_substep_coupled_fused(const sf_phys_fields *x, const sf_couple_fields *cp, int32_t precision,
+                                             double dt, int32_t first, double cff1, int32_t tile_j, void *stream)
+{
+    SF_REQUIRE(x && cp, "sf_phys_substep_coupled_fused: null argument");
+    SF_REQUIRE(x->szI >= 4 && x->szJ >= 4, "sf_phys_substep_coupled_fused: grid %dx%d too small", x->szI, x->szJ);
+    SF_REQUIRE(x->h && x->pm && x->pn && x->fomn && x->sustr && x->svstr,
+               "sf_phys_substep_coupled_fused: null grid field");
+    for (int k = 0; k < 3; k++)
+        SF_REQUIRE(x->zeta[k] && x->ubar[k] && x->vbar[k], "sf_phys_substep_coupled_fused: null state slot");
+    SF_REQUIRE(x->rzeta[0] && x->rzeta[1] && x->rubar[0] && x->rubar[1] && x->rvbar[0] && x->rvbar[1],
+               "sf_phys_substep_coupled_fused: null rhs slot");
+    SF_REQUIRE(x->Zt_avg1 && x->DU_avg1 && x->DV_avg1, "sf_phys_substep_coupled_fused: null average");
+    SF_REQUIRE(cp->Fu && cp->Fv && cp->DU_avg2 && cp->DV_avg2, "sf_phys_substep_coupled_fused: null coupling field");
+    SF_REQUIRE(!first || (cp->rufrc && cp->rvfrc && cp->Fu1 && cp->Fv1 && cp->Fu2 && cp->Fv2),
+               "sf_phys_substep_coupled_fused: the first substep needs rufrc/rvfrc and the forcing history");
+    SF_REQUIRE(cp->ab >= 0 && cp->ab <= 2, "sf_phys_substep_coupled_fused: ab=%d", cp->ab);
+    const int tj = tile_j > 0 ? tile_j : sf_phys_fused_default_tile(precision, x->szI, x->szJ);
+    SF_REQUIRE(tj == 4 || tj == 8, "sf_phys_substep_coupled_fused: tile height %d (4 or 8)", tj);
+    if (precision == SF_FP64) return dispatch_tj<double>(x, dt, first, cff1, tj, stream, cp);
+    if (precision == SF_FP32) return sf_phys_fused_dispatch_f32(x, dt, first, cff1, tj, stream, cp);
+    sf::set_error("sf_phys_substep_coupled_fused: unknown precision %d", precision);
+    return SF_ERR_ARG;
+}
+
 namespace {
 
 int ens_rebuild(sf_ens *e)

@@ -59,6 +59,7 @@ struct PFA {
     int nts;   // streaming (non-temporal) stores and average loads: set when one substep's
                // bytes exceed what the 256 MB last-level cache keeps for the next substep
     double w1;
+    sf_couple_fields c;   // coupling with the baroclinic step (CPL instances only)
 };
 
 // Visit the points of region [3-LO, 3+TI+HI) x [3-LO, 3+TJ+HI) of the frame: the
@@ -184,9 +185,9 @@ struct UVIn {
     T x0, z0c, z0b;             // predictor "before" velocity and zeta (t0) at point / back
 };
 
-template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS>
+template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS, bool CPL = false>
 __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*sm)[PhysLds<TI, TJ>::FN],
-                                          T *side)
+                                          T *side, double *sideF = nullptr)
 {
     constexpr int FW = TI + 5;
     constexpr int KC = TI * TJ / NT;
@@ -198,6 +199,9 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     T *const sPM = sm[9], *const sPN = sm[10];  // pm, pn over the frame (read-only after L)
     // predictor right-hand sides of the ghost-source row (j = 1, M-1) / column (i = 1, L-1)
     T *const sRUlo = side, *const sRUhi = side + TI, *const sRVlo = side + 2 * TI, *const sRVhi = side + 2 * TI + TJ;
+    // CPL: the forcing F of the same ghost-source points (formed in P2 on the first substep)
+    double *const sFUlo = sideF, *const sFUhi = sideF + TI, *const sFVlo = sideF + 2 * TI,
+                 *const sFVhi = sideF + 2 * TI + TJ;
 
     const int szI = P.szI, szJ = P.szJ, L = szI - 1, M = szJ - 1;
     const long long W = szI;
@@ -292,12 +296,30 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         x.cff = (x.pmc + x.pmb) * (x.pnc + x.pnb);
     };
 
+    // CPL: forcing of the predictor right-hand side r of a u (v) point at FP64 byte offset
+    // fd: the first substep forms F = rufrc - r and adds its Adams-Bashforth
+    // extrapolation over the baroclinic steps, the others add F (as pred_forcing of the
+    // four-launch form); F is returned in Fk
+    auto pred_forcing = [&](bool vdir, uint32_t fd, T r, double &Fk) {
+        const double *Fo = vdir ? P.c.Fv : P.c.Fu;
+        if constexpr (FIRST) {
+            const double F = ldo(vdir ? P.c.rvfrc : P.c.rufrc, 0, fd) - (double)r;
+            Fk = F;
+            const double F1 = P.c.ab > 0 ? ldo(vdir ? P.c.Fv1 : P.c.Fu1, 0, fd) : 0.0;
+            const double F2 = P.c.ab > 1 ? ldo(vdir ? P.c.Fv2 : P.c.Fu2, 0, fd) : 0.0;
+            return r + T(sf::ab_extrapolate(P.c.ab, F, F1, F2));
+        } else {
+            Fk = ldo(Fo, 0, fd);
+            return r + T(Fk);
+        }
+    };
+
     // ---- predictor momentum at LDS index q from inputs x (returns rhs in r) ----------
     const T cff4 = T(4.0 / 25.0), cff5 = T(1) - T(2) * cff4;
     const T c1dt = first ? T(0.5) * dt : dt;
     auto gzp = [&](int o, T z0) { return first ? T(0.5) * (sZ1[o] + sZS[o]) : cff5 * sZ1[o] + cff4 * (z0 + sZS[o]); };
     auto Dbp = [&](int o, T h, T z0) { return first ? h + sZ1[o] : h + z0; };
-    auto pred_u = [&](int q, const UVIn<T> &x, T &r) {
+    auto pred_u = [&](int q, const UVIn<T> &x, T &r, uint32_t fd = 0, double *Fk = nullptr) {
         const T onu = x.m1, omu = x.m2;
         const T g0 = gzp(q - 1, x.z0b), g1 = gzp(q, x.z0c);
         r = ((T(0.5) * P.g) * onu) * ((x.hb + x.hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
@@ -311,10 +333,15 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         r = r + T(0.5) * (cr1 + cr0);
         r = r - ((P.rdrg * x.xc) * omu) * onu;
         r = r + (x.st * omu) * onu;
+        if constexpr (CPL) {
+            double F;
+            r = pred_forcing(false, fd, r, F);
+            if (Fk) *Fk = F;
+        }
         const T fac = sf::recip((x.hc + sZS[q]) + (x.hb + sZS[q - 1]));
         return (x.x0 * (Dbp(q, x.hc, x.z0c) + Dbp(q - 1, x.hb, x.z0b)) + (x.cff * c1dt) * r) * fac;
     };
-    auto pred_v = [&](int q, const UVIn<T> &x, T &r) {
+    auto pred_v = [&](int q, const UVIn<T> &x, T &r, uint32_t fd = 0, double *Fk = nullptr) {
         const T omv = x.m1, onv = x.m2;
         const T g0 = gzp(q - FW, x.z0b), g1 = gzp(q, x.z0c);
         r = ((T(0.5) * P.g) * omv) * ((x.hb + x.hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
@@ -328,6 +355,11 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         r = r - T(0.5) * (cr1 + cr0);
         r = r - ((P.rdrg * x.xc) * omv) * onv;
         r = r + (x.st * omv) * onv;
+        if constexpr (CPL) {
+            double F;
+            r = pred_forcing(true, fd, r, F);
+            if (Fk) *Fk = F;
+        }
         const T fac = sf::recip((x.hc + sZS[q]) + (x.hb + sZS[q - FW]));
         return (x.x0 * (Dbp(q, x.hc, x.z0c) + Dbp(q - FW, x.hb, x.z0b)) + (x.cff * c1dt) * r) * fac;
     };
@@ -343,6 +375,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     T z0c_[KC], z0w_[KC], z0s_[KC], u0c_[KC], v0c_[KC], fo_[KC], fow_[KC], fos_[KC], su_[KC], sv_[KC];
     T rz0_[KC], ru0_[KC], rv0_[KC], z1w_[KC], z1s_[KC];
     double zt_[KC], dua_[KC], dva_[KC];
+    // CPL: stage-A fluxes of the core points, their forcing F and DU_avg2 / DV_avg2
+    double du1c[KC], dv1c[KC], fuc[KC], fvc[KC], du2_[KC], dv2_[KC];
 
     // fast-path predicates of a core point (its computations use only its own offset)
     auto rho_int = [&](int i, int j) { return i >= 1 && i <= L - 1 && j >= 1 && j <= M - 1; };
@@ -461,6 +495,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     cffuc[k] = (pm_[k] + pmw) * (pn_[k] + pnw);
                     du = ((T(0.5) * ((hw_[k] + sZ1[o - 1]) + (hc_[k] + sZ1[o]))) * u1c[k]) * onuc[k];
                 }
+                if constexpr (CPL) du1c[k] = (double)du;
                 if (j >= 1) {
                     const T pms = sPM[o - FW], pns = sPN[o - FW];
                     omvc[k] = T(2) * sf::recip(pms + pm_[k]);
@@ -468,6 +503,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     cffvc[k] = (pm_[k] + pms) * (pn_[k] + pns);
                     dv = ((T(0.5) * ((hs_[k] + sZ1[o - FW]) + (hc_[k] + sZ1[o]))) * v1c[k]) * omvc[k];
                 }
+                if constexpr (CPL) dv1c[k] = (double)dv;
             }
             sDU[o] = du;
             sDV[o] = dv;
@@ -622,17 +658,26 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     x.z0c = z0c_[k];
                     x.z0b = z0w_[k];
                     T r;
-                    us = pred_u(o, x, r);
+                    double F = 0.0;
+                    us = pred_u(o, x, r, offd(i, j), &F);
                     sto<NTS>(P.ru1, cf[k], r);
                     ruc[k] = r;
                     if (j == 1) sRUlo[a - 3] = r;
                     if (j == M - 1) sRUhi[a - 3] = r;
+                    if constexpr (CPL) {
+                        fuc[k] = F;
+                        if constexpr (FIRST) {
+                            sto<NTS>(P.c.Fu, offd(i, j), F);
+                            if (j == 1) sFUlo[a - 3] = F;
+                            if (j == M - 1) sFUhi[a - 3] = F;
+                        }
+                    }
                 } else if (i >= 2 && i <= L - 1) {   // ghost row: source (i, 1) or (i, M-1)
                     const int qj = clampi(j, 1, M - 1);
                     T r;
                     UVIn<T> x = load_u(off(i, qj), o + (qj - j) * FW);
                     metrics_u(x);
-                    us = pred_u(o + (qj - j) * FW, x, r);
+                    us = pred_u(o + (qj - j) * FW, x, r, offd(i, qj));
                 }
                 if (v_int(i, j)) {
                     UVIn<T> x;
@@ -657,17 +702,26 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     x.z0c = z0c_[k];
                     x.z0b = z0s_[k];
                     T r;
-                    vs = pred_v(o, x, r);
+                    double F = 0.0;
+                    vs = pred_v(o, x, r, offd(i, j), &F);
                     sto<NTS>(P.rv1, cf[k], r);
                     rvc[k] = r;
                     if (i == 1) sRVlo[b - 3] = r;
                     if (i == L - 1) sRVhi[b - 3] = r;
+                    if constexpr (CPL) {
+                        fvc[k] = F;
+                        if constexpr (FIRST) {
+                            sto<NTS>(P.c.Fv, offd(i, j), F);
+                            if (i == 1) sFVlo[b - 3] = F;
+                            if (i == L - 1) sFVhi[b - 3] = F;
+                        }
+                    }
                 } else if (j >= 2 && j <= M - 1) {   // ghost column: source (1, j) or (L-1, j)
                     const int qi = clampi(i, 1, L - 1);
                     T r;
                     UVIn<T> x = load_v(off(qi, j), o + (qi - i));
                     metrics_v(x);
-                    vs = pred_v(o + (qi - i), x, r);
+                    vs = pred_v(o + (qi - i), x, r, offd(qi, j));
                 }
             }
             sUS[o] = us;
@@ -686,6 +740,10 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 zt_[k] = ldavg<NTS>(P.Zt, fd);
                 dua_[k] = ldavg<NTS>(P.DUa, fd);
                 dva_[k] = ldavg<NTS>(P.DVa, fd);
+                if constexpr (CPL) {
+                    du2_[k] = FIRST ? 0.0 : ldavg<NTS>(P.c.DU_avg2, fd);
+                    dv2_[k] = FIRST ? 0.0 : ldavg<NTS>(P.c.DV_avg2, fd);
+                }
             }
             if (cin[k] && u_int(i, j)) {
                 if (!first) ru0_[k] = ldo(P.ru0, 0, f);
@@ -711,13 +769,13 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     const int qj = clampi(j, 1, M - 1);
                     UVIn<T> x = load_u(off(i, qj), o + (qj - j) * FW);
                     metrics_u(x);
-                    us = pred_u(o + (qj - j) * FW, x, r);
+                    us = pred_u(o + (qj - j) * FW, x, r, offd(i, qj));
                 }
                 if (j >= 2 && j <= M - 1) {
                     const int qi = clampi(i, 1, L - 1);
                     UVIn<T> x = load_v(off(qi, j), o + (qi - i));
                     metrics_v(x);
-                    vs = pred_v(o + (qi - i), x, r);
+                    vs = pred_v(o + (qi - i), x, r, offd(qi, j));
                 }
             }
             sUS[o] = us;
@@ -833,6 +891,10 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     sto<NTS>(P.Zt, fd, zt_[k] + w * (double)zsc[k]);
                     sto<NTS>(P.DUa, fd, dua_[k] + w * (double)duc[k]);
                     sto<NTS>(P.DVa, fd, dva_[k] + w * (double)dvc[k]);
+                    if constexpr (CPL) {
+                        sto<NTS>(P.c.DU_avg2, fd, (du2_[k] + P.c.w2p * du1c[k]) + P.c.w2c * (double)duc[k]);
+                        sto<NTS>(P.c.DV_avg2, fd, (dv2_[k] + P.c.w2p * dv1c[k]) + P.c.w2c * (double)dvc[k]);
+                    }
                 } else {
                     const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
                     const uint32_t fq = off(qi, qj);
@@ -865,7 +927,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     auto gzc = [&](int o) { return cf5 * sZ2[o] + cf4 * sZS[o]; };
     // corrected ubar at LDS index q: metrics m1 = onu, m2 = omu; statics hb/hc, fc/fb, st
     auto corr_u = [&](int q, T onu, T omu, T cff, T hb, T hc, T fc, T fb, T st, T ru1, T ru0, T z1c_, T z1b,
-                      T u1c_) {
+                      T u1c_, double F = 0.0) {
         const T g0 = gzc(q - 1), g1 = gzc(q);
         T r = ((T(0.5) * P.g) * onu) * ((hb + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
         const T uc = sUS[q];
@@ -879,6 +941,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         r = r + T(0.5) * (cr1 + cr0);
         r = r - ((P.rdrg * uc) * omu) * onu;
         r = r + (st * omu) * onu;
+        if constexpr (CPL) r = r + T(F);
         const T fac = sf::recip((hc + sZ2[q]) + (hb + sZ2[q - 1]));
         T tend;
         if (first) tend = (T(0.5) * dt) * r;
@@ -886,7 +949,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         return (u1c_ * ((hc + z1c_) + (hb + z1b)) + cff * tend) * fac;
     };
     auto corr_v = [&](int q, T omv, T onv, T cff, T hb, T hc, T fc, T fb, T st, T rv1, T rv0, T z1c_, T z1b,
-                      T v1c_) {
+                      T v1c_, double F = 0.0) {
         const T g0 = gzc(q - FW), g1 = gzc(q);
         T r = ((T(0.5) * P.g) * omv) * ((hb + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
         const T vc = sVS[q];
@@ -900,6 +963,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         r = r - T(0.5) * (cr1 + cr0);
         r = r - ((P.rdrg * vc) * omv) * onv;
         r = r + (st * omv) * onv;
+        if constexpr (CPL) r = r + T(F);
         const T fac = sf::recip((hc + sZ2[q]) + (hb + sZ2[q - FW]));
         T tend;
         if (first) tend = (T(0.5) * dt) * r;
@@ -915,28 +979,32 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         T u2 = T(0), v2 = T(0);
         if (u_int(i, j)) {
             u2 = corr_u(o, onuc[k], omuc[k], cffuc[k], hw_[k], hc_[k], fo_[k], fow_[k], su_[k], ruc[k], ru0_[k],
-                        z1c[k], z1w_[k], u1c[k]);
+                        z1c[k], z1w_[k], u1c[k], CPL ? fuc[k] : 0.0);
         } else if (i >= 2 && i <= L - 1) {   // ghost row j = 0 or M: source row 1 or M-1
             const int qj = clampi(j, 1, M - 1);
             const uint32_t f = off(i, qj);
             const T ru1 = qj == 1 ? sRUlo[a - 3] : sRUhi[a - 3];
             const T ru0 = first ? T(0) : ldo(P.ru0, 0, f);
+            double F = 0.0;
+            if constexpr (CPL) F = FIRST ? (qj == 1 ? sFUlo[a - 3] : sFUhi[a - 3]) : ldo(P.c.Fu, 0, offd(i, qj));
             u2 = corr_u(o + (qj - j) * FW, onu_at(f), omu_at(f), cffu_at(f), ldo(P.h, -1, f), ldo(P.h, 0, f),
                         ldo(P.fomn, 0, f), ldo(P.fomn, -1, f), ldo(P.sustr, 0, f), ru1, ru0, ldo(P.z1, 0, f),
-                        ldo(P.z1, -1, f), sU1[o + (qj - j) * FW]);
+                        ldo(P.z1, -1, f), sU1[o + (qj - j) * FW], F);
         }
         sto<NTS>(P.u2, cf[k], u2);
         if (v_int(i, j)) {
             v2 = corr_v(o, omvc[k], onvc[k], cffvc[k], hs_[k], hc_[k], fo_[k], fos_[k], sv_[k], rvc[k], rv0_[k],
-                        z1c[k], z1s_[k], v1c[k]);
+                        z1c[k], z1s_[k], v1c[k], CPL ? fvc[k] : 0.0);
         } else if (j >= 2 && j <= M - 1) {   // ghost column i = 0 or L: source column 1 or L-1
             const int qi = clampi(i, 1, L - 1);
             const uint32_t f = off(qi, j);
             const T rv1 = qi == 1 ? sRVlo[b - 3] : sRVhi[b - 3];
             const T rv0 = first ? T(0) : ldo(P.rv0, 0, f);
+            double F = 0.0;
+            if constexpr (CPL) F = FIRST ? (qi == 1 ? sFVlo[b - 3] : sFVhi[b - 3]) : ldo(P.c.Fv, 0, offd(qi, j));
             v2 = corr_v(o + (qi - i), omv_at(f), onv_at(f), cffv_at(f), ldo(P.h, -W, f), ldo(P.h, 0, f),
                         ldo(P.fomn, 0, f), ldo(P.fomn, -W, f), ldo(P.svstr, 0, f), rv1, rv0, ldo(P.z1, 0, f),
-                        ldo(P.z1, -W, f), sV1[o + (qi - i)]);
+                        ldo(P.z1, -W, f), sV1[o + (qi - i)], F);
         }
         sto<NTS>(P.v2, cf[k], v2);
     }
@@ -951,19 +1019,22 @@ __device__ __forceinline__ int xcd_band(int nblk)
     return b;
 }
 
-template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS>
+template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS, bool CPL = false>
 __global__ __launch_bounds__(NT) void k_phys_fused(PFA<T> P)
 {
     __shared__ T sm[PhysLds<TI, TJ>::PLANES][PhysLds<TI, TJ>::FN];
     __shared__ T side[2 * TI + 2 * TJ];
+    __shared__ double sideF[CPL ? 2 * TI + 2 * TJ : 1];
     const int b = xcd_band(P.tiles_i * P.tiles_j);
-    phys_tile<T, TI, TJ, NT, FIRST, NTS>(P, b % P.tiles_i, b / P.tiles_i, sm, side);
+    phys_tile<T, TI, TJ, NT, FIRST, NTS, CPL>(P, b % P.tiles_i, b / P.tiles_i, sm, side, sideF);
 }
 
 template <typename T, int TI, int TJ, int NT>
-int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void *stream)
+int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void *stream,
+                 const sf_couple_fields *cp = nullptr)
 {
     PFA<T> a;
+    a.c = cp ? *cp : sf_couple_fields{};
     a.h = (const T *)x->h;
     a.pm = (const T *)x->pm;
     a.pn = (const T *)x->pn;
@@ -1006,6 +1077,19 @@ int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void 
     SF_REQUIRE(nb < (1LL << 31), "sf_phys_substep_fused: grid too large");
     const dim3 grid((unsigned)nb), block(NT);
     hipStream_t s = sf::as_stream(stream);
+    if constexpr (TJ <= 8) {
+        if (cp) {
+            if (first && a.nts) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true, true, true>), grid, block, 0, s, a);
+            else if (first) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true, false, true>), grid, block, 0, s, a);
+            else if (a.nts) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, false, true, true>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, false, false, true>), grid, block, 0, s, a);
+            return sf::check_launch("sf_phys_substep_coupled_fused");
+        }
+    }
+    if (cp) {
+        sf::set_error("sf_phys_substep_coupled_fused: tile height %d (4 or 8)", TJ);
+        return SF_ERR_ARG;
+    }
     if (first && a.nts) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true, true>), grid, block, 0, s, a);
     else if (first) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true, false>), grid, block, 0, s, a);
     else if (a.nts) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, false, true>), grid, block, 0, s, a);
@@ -1014,12 +1098,13 @@ int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void 
 }
 
 template <typename T>
-int dispatch_tj(const sf_phys_fields *x, double dt, int first, double w1, int tj, void *stream)
+int dispatch_tj(const sf_phys_fields *x, double dt, int first, double w1, int tj, void *stream,
+                const sf_couple_fields *cp = nullptr)
 {
     switch (tj) {
-    case 4: return launch_fused<T, 64, 4, 256>(x, dt, first, w1, stream);
-    case 8: return launch_fused<T, 64, 8, 256>(x, dt, first, w1, stream);
-    case 16: return launch_fused<T, 64, 16, 512>(x, dt, first, w1, stream);
+    case 4: return launch_fused<T, 64, 4, 256>(x, dt, first, w1, stream, cp);
+    case 8: return launch_fused<T, 64, 8, 256>(x, dt, first, w1, stream, cp);
+    case 16: return launch_fused<T, 64, 16, 512>(x, dt, first, w1, stream, cp);
     default: break;
     }
     sf::set_error("sf_phys_substep_fused: unsupported tile height %d (4, 8 or 16)", tj);
@@ -1256,5 +1341,6 @@ void ens_launch(sf_ens *e, int first, double w1, int rot, hipStream_t s)
 }  // namespace
 
 // FP32 instances, compiled in phys2d_fused_f32.hip with contraction on
-int sf_phys_fused_dispatch_f32(const sf_phys_fields *x, double dt, int first, double w1, int tj, void *stream);
+int sf_phys_fused_dispatch_f32(const sf_phys_fields *x, double dt, int first, double w1, int tj, void *stream,
+                               const sf_couple_fields *cp = nullptr);
 void sf_ens_launch_f32(sf_ens *e, int first, double w1, int rot, hipStream_t s);

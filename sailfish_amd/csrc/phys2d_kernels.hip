@@ -124,13 +124,6 @@ struct Cpl {
     int ab, on;
 };
 
-__device__ __forceinline__ double ab_forcing(int ab, double F, double F1, double F2)
-{
-    if (ab == 0) return F;
-    if (ab == 1) return 1.5 * F - 0.5 * F1;
-    return ((23.0 / 12.0) * F - (16.0 / 12.0) * F1) + (5.0 / 12.0) * F2;
-}
-
 template <typename T>
 struct Sub {
     PF<T> p;
@@ -157,7 +150,7 @@ __device__ __forceinline__ T pred_forcing(const Sub<T> &a, bool vdir, long long 
     const double F = frc[f] - (double)r;
     if (store) Fo[f] = F;
     const double *F1 = vdir ? a.c.Fv1 : a.c.Fu1, *F2 = vdir ? a.c.Fv2 : a.c.Fu2;
-    return r + T(ab_forcing(a.c.ab, F, F1[f], F2[f]));
+    return r + T(sf::ab_extrapolate(a.c.ab, F, F1[f], F2[f]));
 }
 
 // ---- P1: predicted zeta (+ predictor RHS), and the pressure surface gz ---------------

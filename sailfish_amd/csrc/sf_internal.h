@@ -43,6 +43,15 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Adams-Bashforth extrapolation over baroclinic steps of the barotropic forcing of the
+// coupled substep: F (ab 0), 3/2 F - 1/2 F1 (ab 1), (23/12 F - 16/12 F1) + 5/12 F2 (ab 2)
+__device__ __forceinline__ double ab_extrapolate(int ab, double F, double F1, double F2)
+{
+    if (ab == 0) return F;
+    if (ab == 1) return 1.5 * F - 0.5 * F1;
+    return ((23.0 / 12.0) * F - (16.0 / 12.0) * F1) + (5.0 / 12.0) * F2;
+}
+
 constexpr int kBlock1D = 256;
 
 // Bytes one substep must move before the fused 2D substeps write their outputs with

@@ -30,11 +30,11 @@ def _cfg(szI, szJ, N, seed):
     return cfg
 
 
-def _run_gpu(cfg, N, dtype, ndtfast=6, dt=12.0):
+def _run_gpu(cfg, N, dtype, ndtfast=6, dt=12.0, fused=True, tile_j=0):
     from sailfish_amd.coupled3d import main3d_coupled
     from sailfish_amd.model2d import build_model3d
     ct, GRID, OCEAN, BOUNDARY = build_model3d(cfg, N=N, ndtfast=ndtfast, dt=dt, fix_q5=True, dtype=dtype)
-    st = main3d_coupled(ct, GRID, OCEAN, BOUNDARY, NSTEPS)
+    st = main3d_coupled(ct, GRID, OCEAN, BOUNDARY, NSTEPS, fused=fused, tile_j=tile_j)
     return ct, GRID, OCEAN, BOUNDARY, st
 
 
@@ -136,3 +136,27 @@ def test_coupled_fp32_against_fp64():
           flush=True)
     over = {k: v for k, v in errs.items() if not v <= FP32_BOUNDS[k]}
     assert not over, (over, errs)
+
+
+@pytest.mark.parametrize("szI,szJ,tile", [(70, 37, 8), (130, 66, 4)])
+def test_fused_coupled_substep_equals_four_launches(szI, szJ, tile):
+    """sf_phys_substep_coupled_fused against sf_phys_substep_coupled over four baroclinic
+    steps: FP64 bit for bit; FP32 within 2e-6 of each field's maximum (the fused FP32
+    kernel contracts multiply-adds)."""
+    import torch
+    N = 6
+    print(f"[case] fused vs four-launch coupled substep {szI}x{szJ}x{N}", file=sys.stderr, flush=True)
+    cfg = _cfg(szI, szJ, N, seed=szI + 5)
+    names = ("zeta_t2", "ubar_t2", "vbar_t2", "u_t2", "v_t2", "Zt_avg1", "DU_avg1", "DV_avg1")
+    for dt in (torch.float64, torch.float32):
+        res = []
+        for fused in (True, False):
+            *_, O, _B, st = _run_gpu(cfg, N, dt, fused=fused, tile_j=tile)
+            g = lambda t: t.detach().cpu().numpy().astype(np.float64)
+            res.append({**{n: g(getattr(O, n)) for n in names}, "DU_avg2": g(st.DU_avg2), "Fu0": g(st.Fu[0])})
+        for n in res[0]:
+            a, b = res[0][n], res[1][n]
+            if dt == torch.float64:
+                assert bitwise_equal(a, b), n
+            else:
+                assert np.abs(a - b).max() <= 2e-6 * max(np.abs(b).max(), 1e-30), (n, float(np.abs(a - b).max()))
