@@ -48,15 +48,18 @@ def init_params(seed: int = 7) -> np.ndarray:
     return pack(p)
 
 
-_erf = np.vectorize(math.erf)
+def _sig(x):
+    return 1.0 / (1.0 + np.exp(-1.702 * x))
 
 
 def gelu(x):
-    return 0.5 * x * (1.0 + _erf(x / math.sqrt(2.0)))
+    """x * sigmoid(1.702 x): the sigmoid form of GELU (Hendrycks & Gimpel 2016)."""
+    return x * _sig(x)
 
 
 def gelu_grad(x):
-    return 0.5 * (1.0 + _erf(x / math.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / math.sqrt(2.0 * math.pi)
+    sg = _sig(x)
+    return sg + 1.702 * x * sg * (1.0 - sg)
 
 
 def forward(theta, X):

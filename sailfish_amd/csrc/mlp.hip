@@ -15,9 +15,10 @@
 //   A (16 x 16, row-major M x K): lane holds A[r][4q .. 4q+3]
 //   B (16 x 16, K x N):           lane holds B[4q .. 4q+3][r]
 //   D (16 x 16):                  lane holds D[4q + t][r], t = 0..3
-// GELU is x * Phi(x) with Phi from the erfc approximation of Abramowitz & Stegun
-// (1964) eq. 7.1.26 (|error| <= 1.5e-7); in FP32 with the hardware reciprocal and
-// exponential, |GELU error| <= 4.3e-7 and |GELU' error| <= 3e-7 over the real line.
+// Activation: the sigmoid form of GELU, x * sigmoid(1.702 x) (Hendrycks & Gimpel 2016),
+// in FP32 with the hardware exponential and reciprocal: five VALU instructions, its
+// derivative six (the erfc form of x * Phi(x) took ~15).  Training and inference use
+// the same function.
 // BF16 rounding is round-to-nearest-even (v_cvt_pk_bf16_f32).
 #include "sf_internal.h"
 
@@ -45,27 +46,17 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi)
 __device__ __forceinline__ uint16_t f2bf(float x) { return (uint16_t)pack_bf16x2(x, 0.0f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// Phi(x) (standard normal CDF) and exp(-x^2 / 2).  erfc(y) ~ (a1 t + .. + a5 t^5) exp(-y^2)
-// with t = 1 / (1 + p y), y = |x| / sqrt 2; the factor 1/2 of Phi = erfc / 2 is folded
-// into the coefficients.
-struct PhiExp {
-    float phi, e;
-};
-__device__ __forceinline__ PhiExp phi_exp(float x)
+// sigmoid(1.702 x) = 1 / (1 + 2^(-1.702 log2(e) x)); for x -> -inf the power overflows to
+// +inf and the reciprocal gives 0
+__device__ __forceinline__ float sig1702(float x)
 {
-    constexpr float p = 0.3275911f * 0.70710678118654752f;
-    constexpr float c1 = 0.5f * 0.254829592f, c2 = 0.5f * -0.284496736f, c3 = 0.5f * 1.421413741f,
-                    c4 = 0.5f * -1.453152027f, c5 = 0.5f * 1.061405429f;
-    const float t = __builtin_amdgcn_rcpf(fmaf(p, fabsf(x), 1.0f));
-    const float e = __builtin_amdgcn_exp2f((x * x) * -0.72134752044448170f);   // 2^(-x^2 log2(e) / 2)
-    const float h = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, c5, c4), c3), c2), c1) * e;   // Phi(-|x|)
-    return {x >= 0.0f ? 1.0f - h : h, e};
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(x * -2.4554669595930157f));
 }
-__device__ __forceinline__ float gelu(float x) { return x * phi_exp(x).phi; }
+__device__ __forceinline__ float gelu(float x) { return x * sig1702(x); }
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    const PhiExp pe = phi_exp(x);
-    return pe.phi + x * pe.e * 0.39894228040143268f;
+    const float s = sig1702(x);
+    return fmaf((1.702f * x) * s, 1.0f - s, s);
 }
 
 // One 16x16 output tile (tm, tn) of A(M x K) * B(K x N) accumulated onto acc; A(m, k)
@@ -301,8 +292,13 @@ __global__ __launch_bounds__(NT) void k_mlp_fwd_bwd(const uint16_t *__restrict__
     }
 }
 
-// grad[p] = sum over blocks (fixed order); optional Adam step on the FP32 master and
-// refresh of the BF16 copy; loss_out = sum of squared errors / (batch * NOUT)
+// grad[p] = sum over the blocks' partials; optional Adam step on the FP32 master and
+// refresh of the BF16 copy; loss_out = sum of squared errors / (batch * NOUT).
+// A workgroup takes 64 parameters; its four waves each sum the partials of every
+// fourth block (wave w: blocks w, w+4, w+8, ... in increasing order), and the four sums
+// are added in wave order: a fixed order for any grid, so the result is deterministic.
+// Reads are coalesced (64 consecutive parameters of one block per wave instruction).
+constexpr int RP = 64;   // parameters per reduce workgroup
 __global__ __launch_bounds__(256) void k_mlp_reduce_adam(const float *__restrict__ partial,
                                                         const float *__restrict__ partial_loss, int nblk,
                                                         float *__restrict__ grad, float *__restrict__ w,
@@ -312,15 +308,21 @@ __global__ __launch_bounds__(256) void k_mlp_reduce_adam(const float *__restrict
                                                         float beta1, float beta2, float eps, float bc1, float bc2,
                                                         float inv_count, float *__restrict__ loss_out)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0 && loss_out) {
+    __shared__ float red[4][RP];
+    const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
+    const int p = blockIdx.x * RP + lane;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && loss_out) {
         float s = 0.0f;
         for (int b = 0; b < nblk; b++) s += partial_loss[b];
         loss_out[0] = s * inv_count;
     }
-    if (p >= NPARAM) return;
     float g = 0.0f;
-    for (int b = 0; b < nblk; b++) g += partial[(long long)b * NPARAM + p];
+    if (p < NPARAM)
+        for (int b = slice; b < nblk; b += 4) g += partial[(long long)b * NPARAM + p];
+    red[slice][lane] = g;
+    __syncthreads();
+    if (slice != 0 || p >= NPARAM) return;
+    g = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
     grad[p] = g;
     if (!adam) return;
     const float mm = beta1 * m[p] + (1.0f - beta1) * g;
@@ -659,7 +661,7 @@ extern "C" int sf_mlp_train_step(sf_mlp *p, const uint16_t *X, const float *Y, c
                        p->partial, p->ploss);
     const float bc1 = adam ? 1.0f - powf(beta1, (float)step) : 1.0f;
     const float bc2 = adam ? 1.0f - powf(beta2, (float)step) : 1.0f;
-    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, s, p->partial, p->ploss,
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPARAM + RP - 1) / RP), dim3(256), 0, s, p->partial, p->ploss,
                        nblocks, p->grad, p->w, p->m, p->v, p->wb, p->wpk, adam, lr, beta1, beta2, eps, bc1, bc2,
                        1.0f / (float)(batch * NOUT), p->loss);
     return sf::check_launch("sf_mlp_train_step");

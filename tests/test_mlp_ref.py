@@ -1,5 +1,6 @@
 """Pins of the FP64 CPU MLP (oracle/mlp_ref.py): central finite differences of the
-loss match the analytic gradient; GELU matches its closed form at known points;
+loss match the analytic gradient; the activation (sigmoid form of GELU) matches its
+closed form at known points and stays close to x Phi(x);
 the bf16 rounding helper is exact on representable values and rounds to nearest even."""
 from __future__ import annotations
 
@@ -48,7 +49,11 @@ def test_padding_gradients_are_zero():
 def test_gelu_closed_forms():
     assert R.gelu(np.array([0.0]))[0] == 0.0
     x = 1.0
-    assert abs(R.gelu(np.array([x]))[0] - 0.5 * (1 + math.erf(1 / math.sqrt(2)))) < 1e-15
+    assert abs(R.gelu(np.array([x]))[0] - 1.0 / (1.0 + math.exp(-1.702))) < 1e-15
+    # the sigmoid form stays within 0.021 of x * Phi(x) (Phi the standard normal CDF)
+    xs = np.linspace(-6, 6, 1201)
+    phi = 0.5 * (1.0 + np.vectorize(math.erf)(xs / math.sqrt(2.0)))
+    assert np.abs(R.gelu(xs) - xs * phi).max() < 0.021
     # gelu(x) - gelu(-x) = x
     xs = np.linspace(-4, 4, 17)
     assert np.allclose(R.gelu(xs) - R.gelu(-xs), xs, atol=1e-14)
