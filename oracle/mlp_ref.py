@@ -1,4 +1,4 @@
-"""FP64 CPU restatement of the closure MLP (45 -> 64 -> 64 -> 2, GELU) with its
+"""FP64 CPU restatement of the closure MLP (45 -> 64 -> 64 -> 2, ReLU) with its
 forward pass, backward pass (MSE loss) and Adam step.  TEST INFRASTRUCTURE ONLY:
 it is the reference for the GPU kernels in sailfish_amd/csrc/mlp.hip and is pinned by
 central finite differences in tests/test_mlp_ref.py.
@@ -48,27 +48,22 @@ def init_params(seed: int = 7) -> np.ndarray:
     return pack(p)
 
 
-def _sig(x):
-    return 1.0 / (1.0 + np.exp(-1.702 * x))
+def relu(x):
+    return np.maximum(x, 0.0)
 
 
-def gelu(x):
-    """x * sigmoid(1.702 x): the sigmoid form of GELU (Hendrycks & Gimpel 2016)."""
-    return x * _sig(x)
-
-
-def gelu_grad(x):
-    sg = _sig(x)
-    return sg + 1.702 * x * sg * (1.0 - sg)
+def relu_grad(x):
+    """Derivative of max(x, 0), taken as 0 at x = 0."""
+    return (np.asarray(x) > 0.0).astype(np.float64)
 
 
 def forward(theta, X):
     """X: (B, 48) (columns 45..47 zero).  Returns outputs (B, 2) and the cache."""
     p = unpack(theta)
     Z1 = X @ p["W1"] + p["b1"]
-    A1 = gelu(Z1)
+    A1 = relu(Z1)
     Z2 = A1 @ p["W2"] + p["b2"]
-    A2 = gelu(Z2)
+    A2 = relu(Z2)
     Z3 = A2 @ p["W3"] + p["b3"]
     return Z3[:, :NOUT], (X, Z1, A1, Z2, A2, Z3)
 
@@ -86,10 +81,10 @@ def grad(theta, X, Y):
     dZ3 = np.zeros_like(Z3)
     dZ3[:, :NOUT] = 2.0 * (out - Y) / (B * NOUT)
     g = {"W3": A2.T @ dZ3, "b3": dZ3.sum(0)}
-    dZ2 = (dZ3 @ p["W3"].T) * gelu_grad(Z2)
+    dZ2 = (dZ3 @ p["W3"].T) * relu_grad(Z2)
     g["W2"] = A1.T @ dZ2
     g["b2"] = dZ2.sum(0)
-    dZ1 = (dZ2 @ p["W2"].T) * gelu_grad(Z1)
+    dZ1 = (dZ2 @ p["W2"].T) * relu_grad(Z1)
     g["W1"] = X.T @ dZ1
     g["b1"] = dZ1.sum(0)
     return pack(g), float(np.mean((out - Y) ** 2))
@@ -114,33 +109,34 @@ def bf16_round(x) -> np.ndarray:
 
 # ---- the GPU kernels' rounding, emulated in FP64 (reference for the MFMA kernels) ----
 def forward_bf16(theta, X):
-    """Forward pass with the GPU's operand rounding: BF16 inputs, weights and hidden
-    activations, exact products and sums, FP32-level biases."""
+    """Forward pass with the GPU's operand rounding: BF16 inputs and weights, exact
+    products and sums, FP32-level biases, hidden activations relu(bf16(Z))."""
     p = unpack(theta)
     W1, W2, W3 = bf16_round(p["W1"]), bf16_round(p["W2"]), bf16_round(p["W3"])
     Xb = bf16_round(X)
     Z1 = Xb @ W1 + p["b1"]
-    A1 = bf16_round(gelu(Z1))
+    A1 = relu(bf16_round(Z1))
     Z2 = A1 @ W2 + p["b2"]
-    A2 = bf16_round(gelu(Z2))
+    A2 = relu(bf16_round(Z2))
     Z3 = A2 @ W3 + p["b3"]
     return Z3[:, :NOUT], (Xb, Z1, A1, Z2, A2, Z3, W1, W2, W3)
 
 
 def grad_bf16(theta, X, Y):
     """Gradient with the GPU's operand rounding (BF16 dZ operands for the products,
-    unrounded dZ for the bias sums)."""
+    unrounded dZ for the bias sums; the ReLU derivative is taken on the rounded
+    activation, [A > 0])."""
     out, (Xb, Z1, A1, Z2, A2, Z3, W1, W2, W3) = forward_bf16(theta, X)
     B = X.shape[0]
     dZ3 = np.zeros_like(Z3)
     dZ3[:, :NOUT] = 2.0 * (out - Y) / (B * NOUT)
     dZ3b = bf16_round(dZ3)
     g = {"W3": A2.T @ dZ3b, "b3": dZ3.sum(0)}
-    dZ2 = (dZ3b @ W3.T) * gelu_grad(Z2)
+    dZ2 = (dZ3b @ W3.T) * relu_grad(A2)
     dZ2b = bf16_round(dZ2)
     g["W2"] = A1.T @ dZ2b
     g["b2"] = dZ2.sum(0)
-    dZ1 = (dZ2b @ W2.T) * gelu_grad(Z1)
+    dZ1 = (dZ2b @ W2.T) * relu_grad(A1)
     dZ1b = bf16_round(dZ1)
     g["W1"] = Xb.T @ dZ1b
     g["b1"] = dZ1.sum(0)

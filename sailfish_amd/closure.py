@@ -124,9 +124,14 @@ class ClosureMLP:
         _lib.check(self.lib.sf_mlp_set_norm(self.handle, pa, pb, pc), "sf_mlp_set_norm")
 
     def train_step(self, Xb: torch.Tensor, Y: torch.Tensor, idx: torch.Tensor, adam: bool = True, lr: float = 1e-3,
-                   beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8, ntile: int = 1, stream=None):
-        """One step on the samples idx (a multiple of 64 * ntile)."""
+                   beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8, ntile: int | None = None,
+                   stream=None):
+        """One step on the samples idx (a multiple of 64 * ntile).  ntile None: the number of
+        64-sample tiles per workgroup that keeps the grid at <= 256 workgroups (one per CU),
+        so large batches run as few, long workgroups and their gradient partials stay few."""
         n = idx.numel()
+        if ntile is None:
+            ntile = default_ntile(n)
         assert n % (64 * ntile) == 0
         nblocks = n // (64 * ntile)
         if adam:
@@ -442,6 +447,17 @@ def pack_inputs(F: torch.Tensor, mu, sd) -> torch.Tensor:
     return to_bf16_bits(Xn)
 
 
+def default_ntile(n: int, blocks: int = 256) -> int:
+    """Tiles of 64 samples per training workgroup for a batch of n samples (n a multiple of
+    64): the largest divisor of n / 64 that is <= ceil(n / (64 * blocks))."""
+    assert n % 64 == 0 and n > 0
+    tiles = n // 64
+    t = max(1, -(-tiles // blocks))
+    while tiles % t:
+        t -= 1
+    return t
+
+
 def train(mlp: ClosureMLP, Xb: torch.Tensor, Yn: torch.Tensor, epochs: int, batch: int = 4096, lr: float = 1e-3,
           seed: int = 6, lr_final: float | None = None, noise: float = 0.0):
     """Adam over shuffled mini-batches (seeded); the learning rate follows a cosine from
@@ -508,7 +524,7 @@ def fine_coarse_ics(fine: Model, r: int):
 
 def run_experiment(coarse_shape=(512, 256), ratio: int = 4, nsteps: int = 5000, n_train_runs: int = 4,
                    snaps_per_run: int = 4, snap_spacing: int = 250, nsamples: int = 2_000_000, probe_dt: float = 0.01,
-                   epochs: int = 40, batch: int = 4096, lr: float = 2e-3, eval_every: int = 500, seed: int = 6,
+                   epochs: int = 300, batch: int = 131072, lr: float = 4e-3, eval_every: int = 500, seed: int = 6,
                    n_val: int | None = None, gains=(1.0, 0.5, 0.25, 0.1), lr_final: float | None = 1e-5,
                    noise: float = 1.0, target: str = "probe",
                    dtype=torch.float64, log=None) -> dict:
@@ -681,4 +697,36 @@ def speed(mlp: ClosureMLP, coarse_shape=(512, 256), ratio: int = 4, nsub: int = 
     for k in out:
         out[k]["s_per_sim_hour"] = 3600.0 / out[k]["dt"] * out[k]["ms_per_substep"] * 1e-3
     out["speedup_closure_vs_fine"] = out["fine"]["s_per_sim_hour"] / out["coarse_closure"]["s_per_sim_hour"]
+    out["training"] = train_rate(mlp.get_params())
     return out
+
+
+MLP_FLOP_PER_SAMPLE = 43008   # forward 16 384 + backward data 10 240 + weight gradients 16 384 (padded shapes)
+
+
+def train_rate(params: np.ndarray, batch: int = 131072, nstep: int = 50, n: int = 1 << 20) -> dict:
+    """Time per Adam step of the training kernels (forward + backward + block sums, then the
+    reduction + Adam) at `batch` random samples from n synthetic rows, on a copy of the
+    network; matrix-core FLOP rate from MLP_FLOP_PER_SAMPLE."""
+    g = torch.Generator(device="cpu").manual_seed(11)
+    X = torch.zeros(n, IN)
+    X[:, :NIN] = torch.randn(n, NIN, generator=g)
+    Xb = to_bf16_bits(X.cuda())
+    Y = torch.randn(n, NOUT, generator=g).cuda().contiguous()
+    idx = torch.randperm(n, generator=g)[:batch].to(torch.int32).cuda()
+    mlp = ClosureMLP(max_blocks=batch // 64, params=params)
+    for _ in range(5):
+        mlp.train_step(Xb, Y, idx, adam=True, lr=1e-4)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(nstep):
+        mlp.train_step(Xb, Y, idx, adam=True, lr=1e-4)
+    b.record()
+    torch.cuda.synchronize()
+    mlp.close()
+    us = a.elapsed_time(b) * 1e3 / nstep
+    tflops = batch * MLP_FLOP_PER_SAMPLE / (us * 1e-6) / 1e12
+    return dict(batch=batch, ntile=default_ntile(batch), us_per_step=us, samples_per_us=batch / us,
+                tflop_s=tflops, frac_bf16_peak=tflops / 2500.0)

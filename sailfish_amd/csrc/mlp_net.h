@@ -1,0 +1,228 @@
+// mlp_net.h -- the closure network on gfx950 matrix cores: layout, operands and the forward
+// pass shared by training (mlp.hip) and inference (closure_infer.hip).
+//
+// Network: 45 inputs (per field ubar, vbar, zeta, h, f: the value at a coarse point and the
+// 8 differences neighbour - point of its 3x3 stencil) -> 64 ReLU -> 64 ReLU -> 2 outputs
+// (ubar and vbar tendency residuals).  Padded shapes: input 48, output 16.  Parameter
+// vector (FP32 master copy, 8336 values):
+//   W1[48][64], b1[64], W2[64][64], b2[64], W3[64][16], b3[16]
+// Products on the matrix cores with BF16 operands and FP32 accumulation; biases, loss and
+// Adam in FP32.  Layer l: Z = A W + b (FP32), A' = relu(bf16(Z)), rounding to nearest
+// even and then the sign test on the packed values: a BF16 value is negative exactly when
+// its bit pattern is a negative 16-bit integer, so one v_pk_max_i16 applies the ReLU to
+// two values.  Backward: dZ = dA * [A > 0]; the products take bf16(dZ), the bias
+// gradients sum the FP32 dZ.
+//
+// Every product keeps the samples on its column index ("transposed" products
+// Z^T = W^T A^T), 16 samples per wave at a time, so that one layer's result is the next
+// layer's operand without leaving the registers.  Operand layouts (wave64, lane l,
+// r = l % 16, q = l / 16; 16 x 16 result D):
+//   16x16x16: lane holds A[r][4q + j] and B[4q + j][r], j = 0..3
+//   16x16x32: lane holds A[r][c(q, j)] and B[c(q, j)][r], j = 0..7, in the order
+//             c(q, j) = 4q + j (j < 4), 16 + 4q + j - 4 (j >= 4) chosen here: a product
+//             sums over K, so any order works that the A and B operands share
+//   D:        lane holds D[4q + t][r], t = 0..3
+// Two consecutive D tiles (result rows 32kb .. 32kb + 15 and 32kb + 16 .. 32kb + 31),
+// packed to BF16 as elements 0..3 and 4..7, are then exactly the B operand of a 16x16x32
+// product over those 32 rows.  The weight gradients sum over the samples, which the
+// registers hold on the column index; a product with the identity moves them to the
+// row index (exact: one product with 1.0 per sum).
+#pragma once
+
+#include "sf_internal.h"
+
+
+namespace {
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));    // 16x16x16 operand (BF16 bits)
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));   // 16x16x32 operand
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef short s16x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int NIN = 45, IN = 48, H = 64, NOUT = 2, OUT = 16;
+constexpr int OFF_W1 = 0, OFF_B1 = IN * H, OFF_W2 = OFF_B1 + H, OFF_B2 = OFF_W2 + H * H, OFF_W3 = OFF_B2 + H,
+              OFF_B3 = OFF_W3 + H * OUT, NPARAM = OFF_B3 + OUT;
+constexpr int BT = 64;    // samples per block tile (16 per wave)
+constexpr int NT = 256;   // threads per workgroup (4 waves)
+
+// two FP32 -> packed BF16 pair (lo in bits 0..15), round to nearest even
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t){lo, hi}, bf16x2_t));
+}
+__device__ __forceinline__ uint16_t f2bf(float x) { return (uint16_t)pack_bf16x2(x, 0.0f); }
+
+__device__ __forceinline__ f32x4_t mma16(bf16x4_t a, bf16x4_t b, f32x4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4_t mma32(bf16x8_t a, bf16x8_t b, f32x4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// a D tile as BF16 (4 values, D rows 4q .. 4q + 3), plain or through the ReLU
+__device__ __forceinline__ u32x2_t pack4(f32x4_t z)
+{
+    return (u32x2_t){pack_bf16x2(z[0], z[1]), pack_bf16x2(z[2], z[3])};
+}
+__device__ __forceinline__ uint32_t relu2(uint32_t p)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2_t, p), (s16x2_t){0, 0}));
+}
+__device__ __forceinline__ u32x2_t relu4(f32x4_t z)
+{
+    const u32x2_t p = pack4(z);
+    return (u32x2_t){relu2(p[0]), relu2(p[1])};
+}
+__device__ __forceinline__ bf16x8_t cat8(u32x2_t lo, u32x2_t hi)
+{
+    return __builtin_bit_cast(bf16x8_t, (u32x4_t){lo[0], lo[1], hi[0], hi[1]});
+}
+__device__ __forceinline__ bf16x4_t as4(u32x2_t p) { return __builtin_bit_cast(bf16x4_t, p); }
+
+// Forward operand layout (wpk, BF16 values, lane-major tiles of W^T):
+//   layer 1, column tile tn = 0..3: a 16x16x32 tile over inputs 0..31 (64 lanes x 8),
+//            then a 16x16x16 tile over inputs 32..47 (64 lanes x 4)
+//   layer 2, (tn, kb): 16x16x32 tiles over layer-1 units 32kb .. 32kb + 31
+//   layer 3, kb = 0, 1: 16x16x32 tiles
+// Element j of lane (r, q) in a tile of column tile tn and K base kbase holds
+// W[kbase + c(q, j)][16 tn + r] (16x16x16 tile: W[32 + 4q + j][16 tn + r]).
+constexpr int WPK_L1 = 0, WPK_L1_TN = 768, WPK_L2 = 3072, WPK_L3 = 7168, NWPK = 8192;
+
+// lane and element of K offset kk (0..31) within a 16x16x32 tile
+__device__ __forceinline__ int k32_slot(int r, int kk)
+{
+    const int q = (kk & 15) >> 2, j = (kk & 3) + ((kk >> 4) << 2);
+    return (r + 16 * q) * 8 + j;
+}
+
+// slot of parameter p in the packed layout, -1 for biases
+__device__ __forceinline__ int wpk_slot(int p)
+{
+    if (p < OFF_B1) {
+        const int k = p / H, n = p % H, tn = n >> 4, r = n & 15;
+        if (k < 32) return WPK_L1 + tn * WPK_L1_TN + k32_slot(r, k);
+        const int kk = k - 32;
+        return WPK_L1 + tn * WPK_L1_TN + 512 + (r + 16 * (kk >> 2)) * 4 + (kk & 3);
+    }
+    if (p >= OFF_W2 && p < OFF_B2) {
+        const int k = (p - OFF_W2) / H, n = (p - OFF_W2) % H;
+        return WPK_L2 + ((n >> 4) * 2 + (k >> 5)) * 512 + k32_slot(n & 15, k & 31);
+    }
+    if (p >= OFF_W3 && p < OFF_B3) {
+        const int k = (p - OFF_W3) / OUT, n = (p - OFF_W3) % OUT;
+        return WPK_L3 + (k >> 5) * 512 + k32_slot(n, k & 31);
+    }
+    return -1;
+}
+
+// the forward network in registers: weight operands and the biases in the D layout
+// (lane (r, q) holds b[16 tn + 4q + t])
+struct Fwd {
+    bf16x8_t w1a[4], w2[8], w3[2];
+    bf16x4_t w1b[4];
+    f32x4_t b1[4], b2[4], b3;
+};
+
+__device__ __forceinline__ void load_fwd(Fwd &W, const uint16_t *__restrict__ wpk, const float *__restrict__ w)
+{
+    const int lane = threadIdx.x & 63, q = lane >> 4;
+#pragma unroll
+    for (int tn = 0; tn < 4; tn++) {
+        W.w1a[tn] = *(const bf16x8_t *)(wpk + WPK_L1 + tn * WPK_L1_TN + lane * 8);
+        W.w1b[tn] = *(const bf16x4_t *)(wpk + WPK_L1 + tn * WPK_L1_TN + 512 + lane * 4);
+        W.b1[tn] = *(const f32x4_t *)(w + OFF_B1 + 16 * tn + 4 * q);
+        W.b2[tn] = *(const f32x4_t *)(w + OFF_B2 + 16 * tn + 4 * q);
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) W.w2[e] = *(const bf16x8_t *)(wpk + WPK_L2 + e * 512 + lane * 8);
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++) W.w3[kb] = *(const bf16x8_t *)(wpk + WPK_L3 + kb * 512 + lane * 8);
+    W.b3 = *(const f32x4_t *)(w + OFF_B3 + 4 * q);
+}
+
+// B operands of 16 samples from their BF16 input rows (lane (r, q): sample r's row)
+__device__ __forceinline__ void load_x(const uint16_t *row, bf16x8_t &xa, bf16x4_t &xb)
+{
+    const int q = (threadIdx.x & 63) >> 4;
+    xa = cat8(*(const u32x2_t *)(row + 4 * q), *(const u32x2_t *)(row + 16 + 4 * q));
+    xb = *(const bf16x4_t *)(row + 32 + 4 * q);
+}
+
+struct Acts {
+    u32x2_t a1[4], a2[4];   // BF16 activations, one 16-unit tile each (D rows 4q .. 4q + 3)
+    f32x4_t z3;             // lanes q = 0: outputs 0, 1 of sample r in z3[0], z3[1]
+};
+
+// forward pass of 16 samples; Z = b + (first K-32 part) + (next part), in that order
+__device__ __forceinline__ void forward16(const Fwd &W, bf16x8_t xa, bf16x4_t xb, Acts &A)
+{
+    f32x4_t z[4];
+#pragma unroll
+    for (int tn = 0; tn < 4; tn++) z[tn] = mma16(W.w1b[tn], xb, mma32(W.w1a[tn], xa, W.b1[tn]));
+#pragma unroll
+    for (int tn = 0; tn < 4; tn++) A.a1[tn] = relu4(z[tn]);
+    const bf16x8_t h0 = cat8(A.a1[0], A.a1[1]), h1 = cat8(A.a1[2], A.a1[3]);
+#pragma unroll
+    for (int tn = 0; tn < 4; tn++) z[tn] = mma32(W.w2[2 * tn + 1], h1, mma32(W.w2[2 * tn], h0, W.b2[tn]));
+#pragma unroll
+    for (int tn = 0; tn < 4; tn++) A.a2[tn] = relu4(z[tn]);
+    A.z3 = mma32(W.w3[1], cat8(A.a2[2], A.a2[3]), mma32(W.w3[0], cat8(A.a2[0], A.a2[1]), W.b3));
+}
+
+// outputs of NG groups of 16 samples, evaluated layer by layer across the groups (NG
+// independent product chains per layer); the same sums in the same order as forward16
+template <int NG>
+__device__ __forceinline__ void forward_groups(const Fwd &W, const bf16x8_t *xa, const bf16x4_t *xb, f32x4_t *z3)
+{
+    f32x4_t z[NG][4];
+    u32x2_t a[NG][4];
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+#pragma unroll
+        for (int tn = 0; tn < 4; tn++) z[g][tn] = mma16(W.w1b[tn], xb[g], mma32(W.w1a[tn], xa[g], W.b1[tn]));
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+#pragma unroll
+        for (int tn = 0; tn < 4; tn++) a[g][tn] = relu4(z[g][tn]);
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const bf16x8_t h0 = cat8(a[g][0], a[g][1]), h1 = cat8(a[g][2], a[g][3]);
+#pragma unroll
+        for (int tn = 0; tn < 4; tn++) z[g][tn] = mma32(W.w2[2 * tn + 1], h1, mma32(W.w2[2 * tn], h0, W.b2[tn]));
+    }
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+#pragma unroll
+        for (int tn = 0; tn < 4; tn++) a[g][tn] = relu4(z[g][tn]);
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+        z3[g] = mma32(W.w3[1], cat8(a[g][2], a[g][3]), mma32(W.w3[0], cat8(a[g][0], a[g][1]), W.b3));
+}
+
+// dZ = dA where the activation (BF16, >= 0) is positive, else 0
+__device__ __forceinline__ f32x4_t relu_mask(f32x4_t d, u32x2_t a)
+{
+    f32x4_t o;
+    o[0] = (a[0] & 0xFFFFu) ? d[0] : 0.0f;
+    o[1] = (a[0] >> 16) ? d[1] : 0.0f;
+    o[2] = (a[1] & 0xFFFFu) ? d[2] : 0.0f;
+    o[3] = (a[1] >> 16) ? d[3] : 0.0f;
+    return o;
+}
+
+}  // namespace
+
+struct sf_mlp {
+    float *w, *m, *v, *grad, *partial, *ploss, *loss;
+    uint16_t *wb;               // BF16 parameters, parameter order
+    uint16_t *wpk;              // BF16 weights, inference operand order (NWPK)
+    float *norm;                // mu[45], 1 / sd[45] (FP32)
+    float ynorm[4];
+    int max_blocks;
+};

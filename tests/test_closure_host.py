@@ -57,3 +57,15 @@ def test_block_mean_conserves_volume():
     zc, _, _ = coarse_grain(zf, u.clone(), u.clone(), r)
     # uniform cells: fine area = coarse area / r^2
     assert abs(float(zc[1:-1, 1:-1].sum()) * r * r - float(zf[1:-1, 1:-1].sum())) < 1e-9
+
+
+def test_default_ntile_keeps_grid_within_one_workgroup_per_cu():
+    from sailfish_amd.closure import default_ntile
+    for n in (64, 4096, 16384, 64 * 255, 64 * 256, 64 * 257, 131072, 64 * 3 * 1000, 2_000_000 // 64 * 64):
+        t = default_ntile(n)
+        tiles = n // 64
+        assert tiles % t == 0 and t >= 1
+        # the largest divisor of the tile count at or below ceil(tiles / 256)
+        cap = -(-tiles // 256)
+        assert t <= cap and all(tiles % d for d in range(t + 1, cap + 1))
+    assert default_ntile(131072) == 8 and default_ntile(4096) == 1

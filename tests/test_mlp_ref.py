@@ -1,6 +1,6 @@
 """Pins of the FP64 CPU MLP (oracle/mlp_ref.py): central finite differences of the
-loss match the analytic gradient; the activation (sigmoid form of GELU) matches its
-closed form at known points and stays close to x Phi(x);
+loss match the analytic gradient; closed forms of the ReLU network (affine when no unit
+is cut, bias-only when every layer-1 unit is cut) and of the BF16 rounding model;
 the bf16 rounding helper is exact on representable values and rounds to nearest even."""
 from __future__ import annotations
 
@@ -46,20 +46,56 @@ def test_padding_gradients_are_zero():
     assert np.all(G["W3"][:, R.NOUT:] == 0.0) and np.all(G["b3"][R.NOUT:] == 0.0)
 
 
-def test_gelu_closed_forms():
-    assert R.gelu(np.array([0.0]))[0] == 0.0
-    x = 1.0
-    assert abs(R.gelu(np.array([x]))[0] - 1.0 / (1.0 + math.exp(-1.702))) < 1e-15
-    # the sigmoid form stays within 0.021 of x * Phi(x) (Phi the standard normal CDF)
-    xs = np.linspace(-6, 6, 1201)
-    phi = 0.5 * (1.0 + np.vectorize(math.erf)(xs / math.sqrt(2.0)))
-    assert np.abs(R.gelu(xs) - xs * phi).max() < 0.021
-    # gelu(x) - gelu(-x) = x
-    xs = np.linspace(-4, 4, 17)
-    assert np.allclose(R.gelu(xs) - R.gelu(-xs), xs, atol=1e-14)
-    # derivative by central differences
-    h = 1e-6
-    assert np.allclose((R.gelu(xs + h) - R.gelu(xs - h)) / (2 * h), R.gelu_grad(xs), atol=1e-8)
+def _net(seed, scale_b=0.1, nonneg=False):
+    rng = np.random.default_rng(seed)
+    p = R.unpack(R.init_params(seed))
+    for k in ("b1", "b2"):
+        p[k] = scale_b * rng.standard_normal(p[k].shape)
+    p["b3"][:R.NOUT] = scale_b * rng.standard_normal(R.NOUT)
+    if nonneg:
+        for k in p:
+            p[k] = np.abs(p[k])
+    p["W1"][R.NIN:] = 0.0
+    p["W3"][:, R.NOUT:] = 0.0
+    p["b3"][R.NOUT:] = 0.0
+    return p
+
+
+def test_relu_network_closed_forms():
+    # the activation itself
+    xs = np.array([-2.0, -1e-300, 0.0, 1e-300, 3.5])
+    assert np.array_equal(R.relu(xs), [0.0, 0.0, 0.0, 1e-300, 3.5])
+    assert np.array_equal(R.relu_grad(xs), [0.0, 0.0, 0.0, 1.0, 1.0])
+    # non-negative inputs, weights and biases: no unit is cut, the network is affine
+    p = _net(3, nonneg=True)
+    X = np.abs(_data(8)[0])
+    out, _ = R.forward(R.pack(p), X)
+    lin = ((X @ p["W1"] + p["b1"]) @ p["W2"] + p["b2"]) @ p["W3"] + p["b3"]
+    assert np.allclose(out, lin[:, :R.NOUT], rtol=1e-13, atol=0)
+    # every layer-1 unit cut (b1 very negative): the output is relu(b2) W3 + b3 for every
+    # sample, and nothing flows back into W1, b1
+    p = _net(4, scale_b=1.0)
+    p["b1"][:] = -1e6
+    X, Y = _data(8)
+    out, _ = R.forward(R.pack(p), X)
+    ref = np.maximum(p["b2"], 0.0) @ p["W3"] + p["b3"]
+    assert np.allclose(out, np.broadcast_to(ref[:R.NOUT], out.shape), rtol=1e-14, atol=0)
+    assert (p["b2"] < 0).any() and (p["b2"] > 0).any()
+    g, _ = R.grad(R.pack(p), X, Y)
+    G = R.unpack(g)
+    assert np.all(G["W1"] == 0.0) and np.all(G["b1"] == 0.0)
+    # the GPU rounding model: activations are relu(bf16(Z)) = bf16(relu(Z)), and the
+    # derivative mask is taken on those activations
+    z = np.random.default_rng(6).standard_normal(1000) * 3.0
+    assert np.array_equal(R.relu(R.bf16_round(z)), R.bf16_round(R.relu(z)))
+    pb = _net(5, nonneg=True)
+    Xb = R.bf16_round(np.abs(_data(8)[0]))
+    outb, cache = R.forward_bf16(R.pack(pb), Xb)
+    A1 = cache[2]
+    W1, W2, W3 = (R.bf16_round(pb[k]) for k in ("W1", "W2", "W3"))
+    assert np.array_equal(A1, R.bf16_round(Xb @ W1 + pb["b1"]))
+    A2 = R.bf16_round(A1 @ W2 + pb["b2"])
+    assert np.allclose(outb, (A2 @ W3 + pb["b3"])[:, :R.NOUT], rtol=1e-14, atol=0)
 
 
 def test_bf16_rounding():
