@@ -486,7 +486,7 @@ int sf_obc_volcons_noalloc(const void *zeta, const void *h, const void *pm, cons
 
 /* ---------------- learned subgrid momentum closure ---------------------------------- */
 
-/* The closure network (45 -> 64 GELU -> 64 GELU -> 2) trained and evaluated on matrix
+/* The closure network (45 -> 64 ReLU -> 64 ReLU -> 2) trained and evaluated on matrix
  * cores (BF16 operands, FP32 accumulation).  No reference counterpart (the learned
  * closure is new); it enters the physics substep as an extra momentum stress through
  * the sustr/svstr fields (obsolete/step2d_LF_AM3.h.py:2595-2606 is where ROMS adds its
@@ -504,7 +504,8 @@ int sf_mlp_get_grad(sf_mlp *mlp, float *host_grad, float *host_loss);
 int sf_mlp_set_norm(sf_mlp *mlp, const float *mu45, const float *sd45, const float *ynorm4);
 /* One step on the batch idx[0 .. nblocks*ntile*64) of X (BF16 bits [n][48]) and Y
  * ([n][2]): forward + backward + deterministic gradient reduction, then Adam when
- * adam != 0 (step counts from 1). */
+ * adam != 0 (step counts from 1).  nblocks workgroups of 4 waves each take ntile tiles
+ * of 64 samples; nblocks <= 256 keeps one workgroup per CU. */
 int sf_mlp_train_step(sf_mlp *mlp, const uint16_t *X, const float *Y, const int32_t *idx, int32_t nblocks,
                       int32_t ntile, int32_t adam, float lr, float beta1, float beta2, float eps, int32_t step,
                       void *stream);
