@@ -517,6 +517,13 @@ int sf_closure_features(const sf_phys_fields *fields, int32_t precision, const v
 /* sustr_eff = sustr + R_u * D_u, svstr_eff = svstr + R_v * D_v from the t1 state. */
 int sf_closure_stress(sf_mlp *mlp, const sf_phys_fields *fields, int32_t precision, const void *f,
                       void *sustr_eff, void *svstr_eff, void *stream);
+/* One physics substep (sf_phys_substep_fused) with the closure evaluated inside the tile
+ * kernel: the corrector stage's surface stress is sf_closure_stress's sustr_eff /
+ * svstr_eff of the substep's t1 state, the predictor keeps sustr / svstr.  f: the
+ * Coriolis field in the state precision.  Tile height 4 or 8 (0: the default). */
+int sf_phys_substep_closure_fused(const sf_phys_fields *fields, const sf_mlp *mlp, const void *f,
+                                  int32_t precision, double dt, int32_t first, double cff1, int32_t tile_j,
+                                  void *stream);
 
 #ifdef __cplusplus
 }

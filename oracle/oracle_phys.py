@@ -27,6 +27,7 @@ class sfo_open(C.Structure):
 O.declare({
     "sfo_phys_substep_open": ([_PP, F64, C.c_int, F64] + [P] * 22 + [C.POINTER(sfo_open), P], None),
     "sfo_phys_substep": ([_PP, F64, C.c_int, F64] + [P] * 22, None),
+    "sfo_phys_substep_cstress": ([_PP, P, P, F64, C.c_int, F64] + [P] * 22, None),
     "sfo_phys_rhs_zeta": ([_PP, P, P, P, C.c_int, C.c_int], F64),
     "sfo_phys_rhs_u": ([_PP, P, P, P, P, C.c_int, C.c_int], F64),
     "sfo_phys_rhs_v": ([_PP, P, P, P, P, C.c_int, C.c_int], F64),
@@ -72,6 +73,20 @@ class OPhys:
                                p(v("rvbar", 0)), p(v("rvbar", 1)),
                                p(self.scr[0]), p(self.scr[1]), p(self.scr[2]), p(self.scr[3]),
                                p(self.st.Zt_avg1), p(self.st.DU_avg1), p(self.st.DV_avg1))
+
+    def substep_cstress(self, dt, first, w1, csustr, csvstr):
+        """substep with the corrector stage's surface stress taken from csustr / csvstr."""
+        v = self.st.v
+        p = O.ptr
+        self._cs = (np.ascontiguousarray(csustr, dtype=float), np.ascontiguousarray(csvstr, dtype=float))
+        lib().sfo_phys_substep_cstress(C.byref(self.c), p(self._cs[0]), p(self._cs[1]), dt, int(first), w1,
+                                       p(v("zeta", 0)), p(v("zeta", 1)), p(v("zeta", 2)),
+                                       p(v("ubar", 0)), p(v("ubar", 1)), p(v("ubar", 2)),
+                                       p(v("vbar", 0)), p(v("vbar", 1)), p(v("vbar", 2)),
+                                       p(v("rzeta", 0)), p(v("rzeta", 1)), p(v("rubar", 0)), p(v("rubar", 1)),
+                                       p(v("rvbar", 0)), p(v("rvbar", 1)),
+                                       p(self.scr[0]), p(self.scr[1]), p(self.scr[2]), p(self.scr[3]),
+                                       p(self.st.Zt_avg1), p(self.st.DU_avg1), p(self.st.DV_avg1))
 
     def open_bc(self, flags, volcons=False, ext_z=None, ext_u=None):
         """Open edges for substep_open (ext_z / ext_u: 4 per-edge arrays or None)."""

@@ -249,7 +249,7 @@ static double ab_forcing(int ab, double F, double F1, double F2)
     return ((23.0 / 12.0) * F - (16.0 / 12.0) * F1) + (5.0 / 12.0) * F2;
 }
 
-static void phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
+static void phys_substep(const sfo_phys *p, const sfo_phys *pc, double dt, int first, double cff1w,
                          const double *z0, const double *z1, double *z2, const double *u0, const double *u1,
                          double *u2, const double *v0, const double *v1, double *v2, const double *rz0, double *rz1,
                          const double *ru0, double *ru1, const double *rv0, double *rv1, double *zs, double *us,
@@ -266,7 +266,7 @@ void sfo_phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
                       const double *ru0, double *ru1, const double *rv0, double *rv1, double *zs, double *us,
                       double *vs, double *gz, double *Zt, double *DUa, double *DVa)
 {
-    phys_substep(p, dt, first, cff1w, z0, z1, z2, u0, u1, u2, v0, v1, v2, rz0, rz1, ru0, ru1, rv0, rv1, zs, us,
+    phys_substep(p, p, dt, first, cff1w, z0, z1, z2, u0, u1, u2, v0, v1, v2, rz0, rz1, ru0, ru1, rv0, rv1, zs, us,
                  vs, gz, Zt, DUa, DVa, NULL, NULL);
 }
 
@@ -278,8 +278,25 @@ void sfo_phys_substep_coupled(const sfo_phys *p, double dt, int first, double cf
                               double *zs, double *us, double *vs, double *gz, double *Zt, double *DUa, double *DVa,
                               const sfo_couple *cp)
 {
-    phys_substep(p, dt, first, cff1w, z0, z1, z2, u0, u1, u2, v0, v1, v2, rz0, rz1, ru0, ru1, rv0, rv1, zs, us,
+    phys_substep(p, p, dt, first, cff1w, z0, z1, z2, u0, u1, u2, v0, v1, v2, rz0, rz1, ru0, ru1, rv0, rv1, zs, us,
                  vs, gz, Zt, DUa, DVa, cp, NULL);
+}
+
+/* The substep with the corrector stage's surface stress taken from csustr / csvstr
+ * (the predictor keeps p->sustr / p->svstr): the form in which the learned closure's
+ * stress enters the coarse substep. */
+void sfo_phys_substep_cstress(const sfo_phys *p, const double *csustr, const double *csvstr, double dt, int first,
+                              double cff1w, const double *z0, const double *z1, double *z2, const double *u0,
+                              const double *u1, double *u2, const double *v0, const double *v1, double *v2,
+                              const double *rz0, double *rz1, const double *ru0, double *ru1, const double *rv0,
+                              double *rv1, double *zs, double *us, double *vs, double *gz, double *Zt, double *DUa,
+                              double *DVa)
+{
+    sfo_phys pc = *p;
+    pc.sustr = csustr;
+    pc.svstr = csvstr;
+    phys_substep(p, &pc, dt, first, cff1w, z0, z1, z2, u0, u1, u2, v0, v1, v2, rz0, rz1, ru0, ru1, rv0, rv1, zs, us,
+                 vs, gz, Zt, DUa, DVa, NULL, NULL);
 }
 
 void sfo_obc_volcons(int szI, int szJ, const double *z, const double *h, const double *pm, const double *pn,
@@ -298,11 +315,13 @@ void sfo_phys_substep_open(const sfo_phys *p, double dt, int first, double cff1w
 {
     sfo_open o = *ob;
     o.out = volcons_out;
-    phys_substep(p, dt, first, cff1w, z0, z1, z2, u0, u1, u2, v0, v1, v2, rz0, rz1, ru0, ru1, rv0, rv1, zs, us,
+    phys_substep(p, p, dt, first, cff1w, z0, z1, z2, u0, u1, u2, v0, v1, v2, rz0, rz1, ru0, ru1, rv0, rv1, zs, us,
                  vs, gz, Zt, DUa, DVa, NULL, &o);
 }
 
-static void phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
+/* pc: the parameters of the corrector's momentum right-hand sides (p itself, or p with
+ * the corrector-stage surface stress of sfo_phys_substep_cstress) */
+static void phys_substep(const sfo_phys *p, const sfo_phys *pc, double dt, int first, double cff1w,
                          const double *z0, const double *z1, double *z2, const double *u0, const double *u1,
                          double *u2, const double *v0, const double *v1, double *v2, const double *rz0, double *rz1,
                          const double *ru0, double *ru1, const double *rv0, double *rv1, double *zs, double *us,
@@ -417,7 +436,7 @@ static void phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
     for (int j = 1; j < M; j++)
         for (int i = 2; i < L; i++) {
             const ptrdiff_t f = IDX(j, i);
-            double r = sfo_phys_rhs_u(p, zs, us, vs, gz, j, i);
+            double r = sfo_phys_rhs_u(pc, zs, us, vs, gz, j, i);
             if (cp) r = r + cp->Fu[f];
             const double cff = (p->pm[f] + p->pm[f - 1]) * (p->pn[f] + p->pn[f - 1]);
             const double fac = 1.0 / (Dat(p, z2, f) + Dat(p, z2, f - 1));
@@ -428,7 +447,7 @@ static void phys_substep(const sfo_phys *p, double dt, int first, double cff1w,
     for (int j = 2; j < M; j++)
         for (int i = 1; i < L; i++) {
             const ptrdiff_t f = IDX(j, i);
-            double r = sfo_phys_rhs_v(p, zs, us, vs, gz, j, i);
+            double r = sfo_phys_rhs_v(pc, zs, us, vs, gz, j, i);
             if (cp) r = r + cp->Fv[f];
             const double cff = (p->pm[f] + p->pm[f - W]) * (p->pn[f] + p->pn[f - W]);
             const double fac = 1.0 / (Dat(p, z2, f) + Dat(p, z2, f - W));

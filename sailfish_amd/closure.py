@@ -12,9 +12,11 @@ in csrc/mlp.hip):
    T_coarse(CG(B)), each instantaneous tendency measured by a tiny forward/backward
    Euler probe substep;
 4. training with Adam on normalised features/targets (sf_mlp_train_step);
-5. coarse runs with the closure: before every substep, sf_closure_stress turns the
-   network output R into the extra momentum stress R * D added to the wind stress of
-   that substep; the plain coarse scheme is the same run without it.
+5. coarse runs with the closure: every substep evaluates the network inside the substep
+   kernel (sf_phys_substep_closure_fused) and adds the momentum stress R * D to the wind
+   stress of the corrector stage; the plain coarse scheme is the same run without it.
+   sf_closure_stress computes the same stresses in a separate launch (the tests compare
+   the two paths).
 """
 from __future__ import annotations
 
@@ -48,6 +50,8 @@ def _bind(lib):
         "sf_mlp_forward": [P, P, C.c_int32, P, P],
         "sf_closure_features": [C.POINTER(sf_phys_fields), C.c_int32, P, P, P],
         "sf_closure_stress": [P, C.POINTER(sf_phys_fields), C.c_int32, P, P, P, P],
+        "sf_phys_substep_closure_fused": [C.POINTER(sf_phys_fields), P, P, C.c_int32, C.c_double, C.c_int32,
+                                          C.c_double, C.c_int32, P],
     }
     for name, args in sig.items():
         getattr(lib, name).argtypes = args
@@ -291,16 +295,20 @@ class Model:
                 getattr(O, f"{name}_t{lev}").copy_(x.reshape(-1))
 
     def advance(self, closure: "ClosureMLP | None" = None, su=None, sv=None):
+        """One substep; with ``closure`` the network runs inside the substep kernel and its
+        stress enters the corrector stage (su, sv are not used: kept for callers that pass
+        the buffers of closure_buffers)."""
         st = self.stepper
         a = st.fields(self.OCEAN)
         if closure is not None:
-            rc = closure.lib.sf_closure_stress(closure.handle, C.byref(a), st.precision, self.f.data_ptr(),
-                                               su.data_ptr(), sv.data_ptr(), _lib.stream_handle())
-            _lib.check(rc, "sf_closure_stress")
-            a.sustr, a.svstr = su.data_ptr(), sv.data_ptr()
-        rc = st.lib.sf_phys_substep_fused(C.byref(a), st.precision, float(self.ct.dtfast), int(self.n == 0),
-                                          self.ct.cff1(), st.tile_j, _lib.stream_handle())
-        _lib.check(rc, "sf_phys_substep_fused")
+            rc = closure.lib.sf_phys_substep_closure_fused(
+                C.byref(a), closure.handle, self.f.data_ptr(), st.precision, float(self.ct.dtfast), int(self.n == 0),
+                self.ct.cff1(), st.tile_j, _lib.stream_handle())
+            _lib.check(rc, "sf_phys_substep_closure_fused")
+        else:
+            rc = st.lib.sf_phys_substep_fused(C.byref(a), st.precision, float(self.ct.dtfast), int(self.n == 0),
+                                              self.ct.cff1(), st.tile_j, _lib.stream_handle())
+            _lib.check(rc, "sf_phys_substep_fused")
         self.n += 1
 
     def features(self) -> torch.Tensor:

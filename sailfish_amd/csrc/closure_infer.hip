@@ -68,14 +68,6 @@ __global__ __launch_bounds__(256) void k_closure_features(ClosureArgs<T> a)
 // features and applying the stress) and keeps the network in registers; features are
 // normalised as (x - mu) * rsd, rsd = 1 / sd rounded to FP32, and rounded to BF16.
 // norm = {mu[45], rsd[45]}; wpk: the packed BF16 weights (NWPK values).
-constexpr int XS = 26;   // feature-tile row stride in BF16 pairs (104 B: conflict-free 8-B row writes)
-
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // element e of field base at byte offset off (< 2^32: the launcher checks the grid size),
 // addressed as a 64-bit scalar base plus a 32-bit vector offset
@@ -122,19 +114,14 @@ __global__ __launch_bounds__(NT) void k_closure_stress(const uint16_t *__restric
         if (ou) Du = T(0.5) * ((at(a.h, og - E) + at(a.z, og - E)) + (at(a.h, og) + at(a.z, og)));
         if (ov) Dv = T(0.5) * ((at(a.h, og - row) + at(a.z, og - row)) + (at(a.h, og) + at(a.z, og)));
         // features of the lane's point -> row `lane` of the wave's tile
-        float xv[IN];
+        float xv[NIN];
 #pragma unroll
         for (int k = 0; k < NIN; k++) {
             const int f = k / 9, dj = (k % 9) / 3 - 1, di = k % 3 - 1;
             const T cv = at(fld[f], oc);
-            const float x = (dj == 0 && di == 0) ? (float)cv : (float)(at(fld[f], oc + dj * row + di * E) - cv);
-            xv[k] = (x - nrm[k]) * nrm[NIN + k];
+            xv[k] = (dj == 0 && di == 0) ? (float)cv : (float)(at(fld[f], oc + dj * row + di * E) - cv);
         }
-        xv[45] = xv[46] = xv[47] = 0.0f;
-        u32x2_t *xrow = (u32x2_t *)(X + lane * XS);
-#pragma unroll
-        for (int e = 0; e < IN / 4; e++)
-            xrow[e] = (u32x2_t){pack_bf16x2(xv[4 * e], xv[4 * e + 1]), pack_bf16x2(xv[4 * e + 2], xv[4 * e + 3])};
+        closure_row(X, lane, xv, nrm);
         wave_lds_sync();
         bf16x8_t xa[4];
         bf16x4_t xb[4];

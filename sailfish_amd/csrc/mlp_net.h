@@ -175,6 +175,51 @@ __device__ __forceinline__ void forward16(const Fwd &W, bf16x8_t xa, bf16x4_t xb
     A.z3 = mma32(W.w3[1], cat8(A.a2[2], A.a2[3]), mma32(W.w3[0], cat8(A.a2[0], A.a2[1]), W.b3));
 }
 
+// Matrix-core operand hand-offs with explicit waits.  On gfx950 (this toolchain) a product
+// whose A/B operand registers were written by a VALU instruction a few cycles before, or
+// are overwritten by one shortly after the product issues, can read wrong operand values:
+// observed as wrong outputs for whole 16-sample groups, depending only on the instruction
+// schedule.  forward16_held keeps each layer's operands unchanged from 5 wait states
+// before its products until 16 after them (layer granularity: one wait before, one after).
+#define SF_SETTLE(...) asm volatile("s_nop 4" : __VA_ARGS__)
+#define SF_HOLD(...)                                        \
+    do {                                                    \
+        __builtin_amdgcn_sched_barrier(0);                  \
+        asm volatile("s_nop 7\n\ts_nop 7" ::__VA_ARGS__);  \
+        __builtin_amdgcn_sched_barrier(0);                  \
+    } while (0)
+
+// forward16 with the waits above around each layer's products (same sums, same results)
+__device__ __forceinline__ void forward16_held(const Fwd &W, bf16x8_t xa, bf16x4_t xb, Acts &A)
+{
+    bf16x8_t a0 = W.w1a[0], a1 = W.w1a[1], a2 = W.w1a[2], a3 = W.w1a[3];
+    bf16x4_t c0 = W.w1b[0], c1 = W.w1b[1], c2 = W.w1b[2], c3 = W.w1b[3];
+    SF_SETTLE("+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3), "+v"(xa), "+v"(xb));
+    f32x4_t z[4];
+    z[0] = mma16(c0, xb, mma32(a0, xa, W.b1[0]));
+    z[1] = mma16(c1, xb, mma32(a1, xa, W.b1[1]));
+    z[2] = mma16(c2, xb, mma32(a2, xa, W.b1[2]));
+    z[3] = mma16(c3, xb, mma32(a3, xa, W.b1[3]));
+    SF_HOLD("v"(a0), "v"(a1), "v"(a2), "v"(a3), "v"(c0), "v"(c1), "v"(c2), "v"(c3), "v"(xa), "v"(xb));
+#pragma unroll
+    for (int tn = 0; tn < 4; tn++) A.a1[tn] = relu4(z[tn]);
+    bf16x8_t h0 = cat8(A.a1[0], A.a1[1]), h1 = cat8(A.a1[2], A.a1[3]);
+    bf16x8_t w0 = W.w2[0], w1 = W.w2[1], w2 = W.w2[2], w3 = W.w2[3], w4 = W.w2[4], w5 = W.w2[5], w6 = W.w2[6],
+             w7 = W.w2[7];
+    SF_SETTLE("+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5), "+v"(w6), "+v"(w7), "+v"(h0), "+v"(h1));
+    z[0] = mma32(w1, h1, mma32(w0, h0, W.b2[0]));
+    z[1] = mma32(w3, h1, mma32(w2, h0, W.b2[1]));
+    z[2] = mma32(w5, h1, mma32(w4, h0, W.b2[2]));
+    z[3] = mma32(w7, h1, mma32(w6, h0, W.b2[3]));
+    SF_HOLD("v"(w0), "v"(w1), "v"(w2), "v"(w3), "v"(w4), "v"(w5), "v"(w6), "v"(w7), "v"(h0), "v"(h1));
+#pragma unroll
+    for (int tn = 0; tn < 4; tn++) A.a2[tn] = relu4(z[tn]);
+    bf16x8_t g0 = cat8(A.a2[0], A.a2[1]), g1 = cat8(A.a2[2], A.a2[3]), v0 = W.w3[0], v1 = W.w3[1];
+    SF_SETTLE("+v"(v0), "+v"(v1), "+v"(g0), "+v"(g1));
+    A.z3 = mma32(v1, g1, mma32(v0, g0, W.b3));
+    SF_HOLD("v"(v0), "v"(v1), "v"(g0), "v"(g1));
+}
+
 // outputs of NG groups of 16 samples, evaluated layer by layer across the groups (NG
 // independent product chains per layer); the same sums in the same order as forward16
 template <int NG>
@@ -203,6 +248,53 @@ __device__ __forceinline__ void forward_groups(const Fwd &W, const bf16x8_t *xa,
 #pragma unroll
     for (int g = 0; g < NG; g++)
         z3[g] = mma32(W.w3[1], cat8(a[g][2], a[g][3]), mma32(W.w3[0], cat8(a[g][0], a[g][1]), W.b3));
+}
+
+// Per-wave feature tile of 64 samples: row = sample, 48 BF16 inputs, row stride XS BF16
+// pairs (104 B: conflict-free 8-B row writes); the rows are read back as B operands.
+constexpr int XS = 26;
+
+// LDS writes of a wave visible to its other lanes
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Features of a point as BF16 row `lane` of the wave's tile X: xv[0..44] normalised as
+// (x - mu) * rsd with nrm = {mu[45], rsd[45]}, inputs 45..47 zero.
+__device__ __forceinline__ void closure_row(uint32_t *X, int lane, const float *xraw, const float *nrm)
+{
+    u32x2_t *row = (u32x2_t *)(X + lane * XS);
+#pragma unroll
+    for (int e = 0; e < IN / 4; e++) {
+        float x4[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int k = 4 * e + t;
+            x4[t] = k < NIN ? (xraw[k] - nrm[k]) * nrm[NIN + k] : 0.0f;
+        }
+        row[e] = (u32x2_t){pack_bf16x2(x4[0], x4[1]), pack_bf16x2(x4[2], x4[3])};
+    }
+}
+
+// Network outputs of the wave's 64 samples (tile X, written by closure_row): lane l gets
+// outputs 0 and 1 of sample l.  The four 16-sample groups run one after another.
+__device__ __forceinline__ void closure_eval(const Fwd &W, uint32_t *X, float &zu, float &zv)
+{
+    const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+    wave_lds_sync();
+    zu = zv = 0.0f;
+#pragma unroll
+    for (int gq = 0; gq < 4; gq++) {
+        const u32x2_t *xr = (const u32x2_t *)(X + (16 * gq + r) * XS);
+        Acts A;
+        forward16_held(W, cat8(xr[q], xr[4 + q]), as4(xr[8 + q]), A);
+        const float tu = __shfl(A.z3[0], r), tv = __shfl(A.z3[1], r);   // lanes q = 0 hold the outputs
+        if (q == gq) zu = tu, zv = tv;
+    }
+    wave_lds_sync();   // the tile is rewritten by the next chunk
 }
 
 // dZ = dA where the activation (BF16, >= 0) is positive, else 0

@@ -56,6 +56,28 @@ extern "C" int sf_phys_substep_coupled_fused(const sf_phys_fields *x, const sf_c
     return SF_ERR_ARG;
 }
 
+extern "C" int sf_phys_substep_closure_fused(const sf_phys_fields *x, const sf_mlp *mlp, const void *f,
+                                             int32_t precision, double dt, int32_t first, double cff1,
+                                             int32_t tile_j, void *stream)
+{
+    SF_REQUIRE(x && mlp && f, "sf_phys_substep_closure_fused: null argument");
+    SF_REQUIRE(x->szI >= 4 && x->szJ >= 4, "sf_phys_substep_closure_fused: grid %dx%d too small", x->szI, x->szJ);
+    SF_REQUIRE(x->h && x->pm && x->pn && x->fomn && x->sustr && x->svstr,
+               "sf_phys_substep_closure_fused: null grid field");
+    for (int k = 0; k < 3; k++)
+        SF_REQUIRE(x->zeta[k] && x->ubar[k] && x->vbar[k], "sf_phys_substep_closure_fused: null state slot");
+    SF_REQUIRE(x->rzeta[0] && x->rzeta[1] && x->rubar[0] && x->rubar[1] && x->rvbar[0] && x->rvbar[1],
+               "sf_phys_substep_closure_fused: null rhs slot");
+    SF_REQUIRE(x->Zt_avg1 && x->DU_avg1 && x->DV_avg1, "sf_phys_substep_closure_fused: null average");
+    const int tj = tile_j > 0 ? tile_j : sf_phys_fused_default_tile(precision, x->szI, x->szJ);
+    SF_REQUIRE(tj == 4 || tj == 8, "sf_phys_substep_closure_fused: tile height %d (4 or 8)", tj);
+    const ClosureIn cl{mlp, f};
+    if (precision == SF_FP64) return dispatch_tj<double>(x, dt, first, cff1, tj, stream, nullptr, &cl);
+    if (precision == SF_FP32) return sf_phys_fused_dispatch_f32(x, dt, first, cff1, tj, stream, nullptr, &cl);
+    sf::set_error("sf_phys_substep_closure_fused: unknown precision %d", precision);
+    return SF_ERR_ARG;
+}
+
 namespace {
 
 int ens_rebuild(sf_ens *e)

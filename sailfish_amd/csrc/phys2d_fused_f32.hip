@@ -11,9 +11,9 @@
 #include "phys2d_fused_impl.h"
 
 int sf_phys_fused_dispatch_f32(const sf_phys_fields *x, double dt, int first, double w1, int tj, void *stream,
-                               const sf_couple_fields *cp)
+                               const sf_couple_fields *cp, const ClosureIn *cl)
 {
-    return dispatch_tj<float>(x, dt, first, w1, tj, stream, cp);
+    return dispatch_tj<float>(x, dt, first, w1, tj, stream, cp, cl);
 }
 
 void sf_ens_launch_f32(sf_ens *e, int first, double w1, int rot, hipStream_t s)

@@ -39,9 +39,16 @@
 // compiled with contraction on and agree with the four-launch FP32 form to rounding.
 #pragma once
 
+#include "mlp_net.h"
 #include "sf_internal.h"
 
 #include <vector>
+
+// the learned closure of a fused substep: the network and its Coriolis field
+struct ClosureIn {
+    const sf_mlp *mlp;
+    const void *f;
+};
 
 namespace {
 
@@ -60,6 +67,12 @@ struct PFA {
                // bytes exceed what the 256 MB last-level cache keeps for the next substep
     double w1;
     sf_couple_fields c;   // coupling with the baroclinic step (CPL instances only)
+    // the learned closure evaluated in the tile (CLO instances only): network operands,
+    // input normalisation {mu[45], 1/sd[45]}, output de-normalisation, Coriolis field
+    const uint16_t *cl_wpk;
+    const float *cl_w, *cl_norm;
+    const T *cl_f;
+    float cl_ynorm[4];
 };
 
 // Visit the points of region [3-LO, 3+TI+HI) x [3-LO, 3+TJ+HI) of the frame: the
@@ -185,9 +198,10 @@ struct UVIn {
     T x0, z0c, z0b;             // predictor "before" velocity and zeta (t0) at point / back
 };
 
-template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS, bool CPL = false>
+template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS, bool CPL = false, bool CLO = false>
 __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*sm)[PhysLds<TI, TJ>::FN],
-                                          T *side, double *sideF = nullptr)
+                                          T *side, double *sideF = nullptr, uint32_t *clX = nullptr,
+                                          float *clN = nullptr, T *clS = nullptr)
 {
     constexpr int FW = TI + 5;
     constexpr int KC = TI * TJ / NT;
@@ -383,6 +397,9 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     auto u_int = [&](int i, int j) { return i >= 2 && i <= L - 1 && j >= 1 && j <= M - 1; };
     auto v_int = [&](int i, int j) { return i >= 1 && i <= L - 1 && j >= 2 && j <= M - 1; };
 
+    if constexpr (CLO)   // input normalisation of the closure (read after L's barrier)
+        if (tid < 2 * NIN) clN[tid] = P.cl_norm[tid];
+
     // ---- G0 + L: z1 over the frame; core statics -----------------------------------------
 #pragma unroll
     for (int k = 0; k < KC; k++) {
@@ -452,6 +469,60 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             }
     }
     lds_barrier();
+
+    // ---- CL: the learned closure's corrector-stage stresses at the core points ----------
+    // Each wave evaluates the network on its core row(s) of 64 points (features from the
+    // t1 frame planes, h and f from memory) and keeps sustr + R_u D_u / svstr + R_v D_v of
+    // its points for C2; those of the ghost-source rows / columns (j = 1, M-1 / i = 1, L-1)
+    // also go to clS for the ghost points of C2.  The same arithmetic as sf_closure_stress.
+    T csu[KC], csv[KC];
+    if constexpr (CLO) {
+#pragma clang fp contract(off)   // the same roundings as sf_closure_stress in both precisions
+        Fwd Wn;
+        load_fwd(Wn, P.cl_wpk, P.cl_w);
+        uint32_t *X = clX + (tid >> 6) * (64 * XS);
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            int a, b;
+            core_ab(k, a, b);
+            const int i = fi0 + a, j = fj0 + b, o = b * FW + a;
+            const bool in = rho_int(i, j);
+            const uint32_t f = cf[k];
+            float xv[NIN];
+#pragma unroll
+            for (int kk = 0; kk < NIN; kk++) {
+                const int fld = kk / 9, dj = (kk % 9) / 3 - 1, di = kk % 3 - 1;
+                const int d = dj * FW + di;
+                T cv, nv;
+                if (fld < 3) {
+                    const T *pl = fld == 0 ? sU1 : fld == 1 ? sV1 : sZ1;
+                    cv = pl[o];
+                    nv = pl[o + d];
+                } else {
+                    const T *src = fld == 3 ? P.h : P.cl_f;
+                    cv = in ? ldo(src, 0, f) : T(0);
+                    nv = in ? ldo(src, dj * W + di, f) : T(0);
+                }
+                xv[kk] = (dj == 0 && di == 0) ? (float)cv : (float)(nv - cv);
+            }
+            closure_row(X, tid & 63, xv, clN);
+            float zu, zv;
+            closure_eval(Wn, X, zu, zv);
+            csu[k] = csv[k] = T(0);   // read in C2 only at u / v interior points
+            if (u_int(i, j)) {
+                const T Du = T(0.5) * ((hw_[k] + sZ1[o - 1]) + (hc_[k] + sZ1[o]));
+                csu[k] = ldo(P.sustr, 0, f) + T(zu * P.cl_ynorm[1] + P.cl_ynorm[0]) * Du;
+                if (j == 1) clS[a - 3] = csu[k];
+                if (j == M - 1) clS[TI + a - 3] = csu[k];
+            }
+            if (v_int(i, j)) {
+                const T Dv = T(0.5) * ((hs_[k] + sZ1[o - FW]) + (hc_[k] + sZ1[o]));
+                csv[k] = ldo(P.svstr, 0, f) + T(zv * P.cl_ynorm[3] + P.cl_ynorm[2]) * Dv;
+                if (i == 1) clS[2 * TI + b - 3] = csv[k];
+                if (i == L - 1) clS[2 * TI + TJ + b - 3] = csv[k];
+            }
+        }
+    }
 
     // ---- A: DU1 at u points, DV1 at v points ---------------------------------------------
     {
@@ -978,8 +1049,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         if (!cin[k]) continue;
         T u2 = T(0), v2 = T(0);
         if (u_int(i, j)) {
-            u2 = corr_u(o, onuc[k], omuc[k], cffuc[k], hw_[k], hc_[k], fo_[k], fow_[k], su_[k], ruc[k], ru0_[k],
-                        z1c[k], z1w_[k], u1c[k], CPL ? fuc[k] : 0.0);
+            u2 = corr_u(o, onuc[k], omuc[k], cffuc[k], hw_[k], hc_[k], fo_[k], fow_[k], CLO ? csu[k] : su_[k], ruc[k],
+                        ru0_[k], z1c[k], z1w_[k], u1c[k], CPL ? fuc[k] : 0.0);
         } else if (i >= 2 && i <= L - 1) {   // ghost row j = 0 or M: source row 1 or M-1
             const int qj = clampi(j, 1, M - 1);
             const uint32_t f = off(i, qj);
@@ -987,14 +1058,15 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const T ru0 = first ? T(0) : ldo(P.ru0, 0, f);
             double F = 0.0;
             if constexpr (CPL) F = FIRST ? (qj == 1 ? sFUlo[a - 3] : sFUhi[a - 3]) : ldo(P.c.Fu, 0, offd(i, qj));
+            const T st = CLO ? clS[(qj == 1 ? 0 : TI) + a - 3] : ldo(P.sustr, 0, f);
             u2 = corr_u(o + (qj - j) * FW, onu_at(f), omu_at(f), cffu_at(f), ldo(P.h, -1, f), ldo(P.h, 0, f),
-                        ldo(P.fomn, 0, f), ldo(P.fomn, -1, f), ldo(P.sustr, 0, f), ru1, ru0, ldo(P.z1, 0, f),
-                        ldo(P.z1, -1, f), sU1[o + (qj - j) * FW], F);
+                        ldo(P.fomn, 0, f), ldo(P.fomn, -1, f), st, ru1, ru0, ldo(P.z1, 0, f), ldo(P.z1, -1, f),
+                        sU1[o + (qj - j) * FW], F);
         }
         sto<NTS>(P.u2, cf[k], u2);
         if (v_int(i, j)) {
-            v2 = corr_v(o, omvc[k], onvc[k], cffvc[k], hs_[k], hc_[k], fo_[k], fos_[k], sv_[k], rvc[k], rv0_[k],
-                        z1c[k], z1s_[k], v1c[k], CPL ? fvc[k] : 0.0);
+            v2 = corr_v(o, omvc[k], onvc[k], cffvc[k], hs_[k], hc_[k], fo_[k], fos_[k], CLO ? csv[k] : sv_[k], rvc[k],
+                        rv0_[k], z1c[k], z1s_[k], v1c[k], CPL ? fvc[k] : 0.0);
         } else if (j >= 2 && j <= M - 1) {   // ghost column i = 0 or L: source column 1 or L-1
             const int qi = clampi(i, 1, L - 1);
             const uint32_t f = off(qi, j);
@@ -1002,9 +1074,10 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const T rv0 = first ? T(0) : ldo(P.rv0, 0, f);
             double F = 0.0;
             if constexpr (CPL) F = FIRST ? (qi == 1 ? sFVlo[b - 3] : sFVhi[b - 3]) : ldo(P.c.Fv, 0, offd(qi, j));
+            const T st = CLO ? clS[2 * TI + (qi == 1 ? 0 : TJ) + b - 3] : ldo(P.svstr, 0, f);
             v2 = corr_v(o + (qi - i), omv_at(f), onv_at(f), cffv_at(f), ldo(P.h, -W, f), ldo(P.h, 0, f),
-                        ldo(P.fomn, 0, f), ldo(P.fomn, -W, f), ldo(P.svstr, 0, f), rv1, rv0, ldo(P.z1, 0, f),
-                        ldo(P.z1, -W, f), sV1[o + (qi - i)], F);
+                        ldo(P.fomn, 0, f), ldo(P.fomn, -W, f), st, rv1, rv0, ldo(P.z1, 0, f), ldo(P.z1, -W, f),
+                        sV1[o + (qi - i)], F);
         }
         sto<NTS>(P.v2, cf[k], v2);
     }
@@ -1019,22 +1092,31 @@ __device__ __forceinline__ int xcd_band(int nblk)
     return b;
 }
 
-template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS, bool CPL = false>
+template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS, bool CPL = false, bool CLO = false>
 __global__ __launch_bounds__(NT) void k_phys_fused(PFA<T> P)
 {
     __shared__ T sm[PhysLds<TI, TJ>::PLANES][PhysLds<TI, TJ>::FN];
     __shared__ T side[2 * TI + 2 * TJ];
     __shared__ double sideF[CPL ? 2 * TI + 2 * TJ : 1];
+    __shared__ __attribute__((aligned(16))) uint32_t clX[CLO ? (NT / 64) * 64 * XS : 1];
+    __shared__ float clN[CLO ? 2 * NIN : 1];
+    __shared__ T clS[CLO ? 2 * TI + 2 * TJ : 1];
     const int b = xcd_band(P.tiles_i * P.tiles_j);
-    phys_tile<T, TI, TJ, NT, FIRST, NTS, CPL>(P, b % P.tiles_i, b / P.tiles_i, sm, side, sideF);
+    phys_tile<T, TI, TJ, NT, FIRST, NTS, CPL, CLO>(P, b % P.tiles_i, b / P.tiles_i, sm, side, sideF, clX, clN, clS);
 }
+
 
 template <typename T, int TI, int TJ, int NT>
 int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void *stream,
-                 const sf_couple_fields *cp = nullptr)
+                 const sf_couple_fields *cp = nullptr, const ClosureIn *cl = nullptr)
 {
     PFA<T> a;
     a.c = cp ? *cp : sf_couple_fields{};
+    a.cl_wpk = cl ? cl->mlp->wpk : nullptr;
+    a.cl_w = cl ? cl->mlp->w : nullptr;
+    a.cl_norm = cl ? cl->mlp->norm : nullptr;
+    a.cl_f = cl ? (const T *)cl->f : nullptr;
+    for (int k = 0; k < 4; k++) a.cl_ynorm[k] = cl ? cl->mlp->ynorm[k] : 0.0f;
     a.h = (const T *)x->h;
     a.pm = (const T *)x->pm;
     a.pn = (const T *)x->pn;
@@ -1078,6 +1160,14 @@ int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void 
     const dim3 grid((unsigned)nb), block(NT);
     hipStream_t s = sf::as_stream(stream);
     if constexpr (TJ <= 8) {
+        if (cl) {
+            SF_REQUIRE(!cp, "sf_phys_substep_closure_fused: coupling and closure together are not supported");
+            if (first && a.nts) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true, true, false, true>), grid, block, 0, s, a);
+            else if (first) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true, false, false, true>), grid, block, 0, s, a);
+            else if (a.nts) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, false, true, false, true>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, false, false, false, true>), grid, block, 0, s, a);
+            return sf::check_launch("sf_phys_substep_closure_fused");
+        }
         if (cp) {
             if (first && a.nts) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true, true, true>), grid, block, 0, s, a);
             else if (first) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true, false, true>), grid, block, 0, s, a);
@@ -1086,8 +1176,9 @@ int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void 
             return sf::check_launch("sf_phys_substep_coupled_fused");
         }
     }
-    if (cp) {
-        sf::set_error("sf_phys_substep_coupled_fused: tile height %d (4 or 8)", TJ);
+    if (cp || cl) {
+        sf::set_error("%s: tile height %d (4 or 8)",
+                      cp ? "sf_phys_substep_coupled_fused" : "sf_phys_substep_closure_fused", TJ);
         return SF_ERR_ARG;
     }
     if (first && a.nts) hipLaunchKernelGGL((k_phys_fused<T, TI, TJ, NT, true, true>), grid, block, 0, s, a);
@@ -1099,12 +1190,12 @@ int launch_fused(const sf_phys_fields *x, double dt, int first, double w1, void 
 
 template <typename T>
 int dispatch_tj(const sf_phys_fields *x, double dt, int first, double w1, int tj, void *stream,
-                const sf_couple_fields *cp = nullptr)
+                const sf_couple_fields *cp = nullptr, const ClosureIn *cl = nullptr)
 {
     switch (tj) {
-    case 4: return launch_fused<T, 64, 4, 256>(x, dt, first, w1, stream, cp);
-    case 8: return launch_fused<T, 64, 8, 256>(x, dt, first, w1, stream, cp);
-    case 16: return launch_fused<T, 64, 16, 512>(x, dt, first, w1, stream, cp);
+    case 4: return launch_fused<T, 64, 4, 256>(x, dt, first, w1, stream, cp, cl);
+    case 8: return launch_fused<T, 64, 8, 256>(x, dt, first, w1, stream, cp, cl);
+    case 16: return launch_fused<T, 64, 16, 512>(x, dt, first, w1, stream, cp, cl);
     default: break;
     }
     sf::set_error("sf_phys_substep_fused: unsupported tile height %d (4, 8 or 16)", tj);
@@ -1342,5 +1433,5 @@ void ens_launch(sf_ens *e, int first, double w1, int rot, hipStream_t s)
 
 // FP32 instances, compiled in phys2d_fused_f32.hip with contraction on
 int sf_phys_fused_dispatch_f32(const sf_phys_fields *x, double dt, int first, double w1, int tj, void *stream,
-                               const sf_couple_fields *cp = nullptr);
+                               const sf_couple_fields *cp = nullptr, const ClosureIn *cl = nullptr);
 void sf_ens_launch_f32(sf_ens *e, int first, double w1, int rot, hipStream_t s);

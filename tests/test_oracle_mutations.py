@@ -88,6 +88,8 @@ MUTANTS = [
      "else zs[f] = z0[f] + ((p->pm[f] * p->pn[f]) * dt) * r;"),
     ("am3_zeta_weight", "sf_oracle_phys.c",
      "a2 = dt * 8.0 / 12.0", "a2 = dt * 7.0 / 12.0"),
+    ("corrector_stress_ignored", "sf_oracle_phys.c",
+     "double r = sfo_phys_rhs_u(pc, zs, us, vs, gz, j, i);", "double r = sfo_phys_rhs_u(p, zs, us, vs, gz, j, i);"),
     ("am3_momentum_weight", "sf_oracle_phys.c",
      "b1 = 0.5 * dt * 5.0 / 12.0", "b1 = 0.5 * dt * 4.0 / 12.0"),
     ("predictor_surface_weight", "sf_oracle_phys.c",

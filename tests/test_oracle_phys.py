@@ -332,3 +332,35 @@ def test_predictor_pressure_surface_weights():
     us = 4 * dt * r / (DX * DY * zs_u2)
     got = o.scr[1].reshape(szJ, szI)[sl]
     assert np.allclose(got, np.broadcast_to(us, (40, 48))[sl], rtol=1e-10)
+
+
+def test_corrector_stage_stress_closed_form():
+    """Stress only in the corrector stage (sfo_phys_substep_cstress) on a flat basin at
+    rest: the predictor leaves the state at rest, so the corrector's right-hand side is the
+    stress term alone and the first substep gives u = dt*tau/D, v = dt*tau_v/D at every
+    interior point, walls included -- where the same stress in both stages adds the
+    predicted flow's drag.  With the plain stress as the corrector stress the substep
+    equals the plain one bit for bit."""
+    szI, szJ, dt = 40, 30, 20.0
+    cfg = flat_basin(szI, szJ, DX, H0)
+    cfg["pn"] = np.full((szJ, szI), 1 / DY)
+    cfg["zeta0"] = np.zeros((szJ, szI))
+    tau, tauv = 1e-4, -3e-5
+    cs_u, cs_v = np.full(szI * szJ, tau), np.full(szI * szJ, tauv)
+    o = oracle_for(cfg, rdrg=3e-4)
+    o.st.cycleTimes2D()
+    o.substep_cstress(dt, True, 0.0, cs_u, cs_v)
+    u = o.st.v("ubar", 2).reshape(szJ, szI)
+    v = o.st.v("vbar", 2).reshape(szJ, szI)
+    assert np.allclose(u[1:-1, 2:-1], dt * tau / H0, rtol=1e-13, atol=0)
+    assert np.allclose(v[2:-1, 1:-1], dt * tauv / H0, rtol=1e-13, atol=0)
+    assert np.all(u[:, :2] == 0) and np.all(v[:2, :] == 0)
+    # plain stress passed as the corrector stress: identical to the plain substep
+    wind = dict(sustr=np.full((szJ, szI), tau), svstr=np.full((szJ, szI), tauv))
+    a, b = oracle_for(dict(cfg, **wind), rdrg=3e-4), oracle_for(dict(cfg, **wind), rdrg=3e-4)
+    for oo in (a, b):
+        oo.st.cycleTimes2D()
+    a.substep(dt, True, 0.0)
+    b.substep_cstress(dt, True, 0.0, cs_u, cs_v)
+    for name in ("zeta", "ubar", "vbar"):
+        assert np.array_equal(a.st.v(name, 2), b.st.v(name, 2))
