@@ -66,7 +66,9 @@ __global__ __launch_bounds__(256) void k_closure_features(ClosureArgs<T> a)
 // network's (de-normalised) tendency residual and D the depth at the velocity point.
 // Each wave owns chunks of 64 consecutive grid points (lane = point while gathering the
 // features and applying the stress) and keeps the network in registers; features are
-// normalised as (x - mu) * rsd, rsd = 1 / sd rounded to FP32, and rounded to BF16.
+// normalised as (x - mu) * rsd, rsd = 1 / sd rounded to FP32, and rounded to BF16.  The
+// network runs as in the fused substep (closure_eval: groups one after another, waits
+// around each layer's products).
 // norm = {mu[45], rsd[45]}; wpk: the packed BF16 weights (NWPK values).
 
 // element e of field base at byte offset off (< 2^32: the launcher checks the grid size),
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(NT) void k_closure_stress(const uint16_t *__restric
 {
     __shared__ __attribute__((aligned(16))) uint32_t Xl[NT / 64][64 * XS];
     __shared__ float nrml[NT / 64][2 * NIN];   // per wave: mu, then rsd (uniform LDS reads)
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float *nrm = nrml[wave];
     nrm[lane] = norm[lane];
     if (lane < 2 * NIN - 64) nrm[64 + lane] = norm[64 + lane];
@@ -122,25 +124,8 @@ __global__ __launch_bounds__(NT) void k_closure_stress(const uint16_t *__restric
             xv[k] = (dj == 0 && di == 0) ? (float)cv : (float)(at(fld[f], oc + dj * row + di * E) - cv);
         }
         closure_row(X, lane, xv, nrm);
-        wave_lds_sync();
-        bf16x8_t xa[4];
-        bf16x4_t xb[4];
-#pragma unroll
-        for (int gq = 0; gq < 4; gq++) {
-            const u32x2_t *xr = (const u32x2_t *)(X + (16 * gq + r) * XS);
-            xa[gq] = cat8(xr[q], xr[4 + q]);
-            xb[gq] = as4(xr[8 + q]);
-        }
-        wave_lds_sync();   // the tile is rewritten by the next chunk
-        f32x4_t z3[4];
-        forward_groups<4>(W, xa, xb, z3);
-        // lanes q = 0 hold outputs 0 and 1 of sample r of group gq
-        float zu = 0.0f, zv = 0.0f;
-#pragma unroll
-        for (int gq = 0; gq < 4; gq++) {
-            const float tu = __shfl(z3[gq][0], r), tv = __shfl(z3[gq][1], r);
-            if (q == gq) zu = tu, zv = tv;
-        }
+        float zu, zv;
+        closure_eval(W, X, zu, zv);
         if (g < S2) {
             if (ou) su = su + T(zu * a.sd_u + a.mu_u) * Du;
             if (ov) sv = sv + T(zv * a.sd_v + a.mu_v) * Dv;

@@ -220,36 +220,6 @@ __device__ __forceinline__ void forward16_held(const Fwd &W, bf16x8_t xa, bf16x4
     SF_HOLD("v"(v0), "v"(v1), "v"(g0), "v"(g1));
 }
 
-// outputs of NG groups of 16 samples, evaluated layer by layer across the groups (NG
-// independent product chains per layer); the same sums in the same order as forward16
-template <int NG>
-__device__ __forceinline__ void forward_groups(const Fwd &W, const bf16x8_t *xa, const bf16x4_t *xb, f32x4_t *z3)
-{
-    f32x4_t z[NG][4];
-    u32x2_t a[NG][4];
-#pragma unroll
-    for (int g = 0; g < NG; g++)
-#pragma unroll
-        for (int tn = 0; tn < 4; tn++) z[g][tn] = mma16(W.w1b[tn], xb[g], mma32(W.w1a[tn], xa[g], W.b1[tn]));
-#pragma unroll
-    for (int g = 0; g < NG; g++)
-#pragma unroll
-        for (int tn = 0; tn < 4; tn++) a[g][tn] = relu4(z[g][tn]);
-#pragma unroll
-    for (int g = 0; g < NG; g++) {
-        const bf16x8_t h0 = cat8(a[g][0], a[g][1]), h1 = cat8(a[g][2], a[g][3]);
-#pragma unroll
-        for (int tn = 0; tn < 4; tn++) z[g][tn] = mma32(W.w2[2 * tn + 1], h1, mma32(W.w2[2 * tn], h0, W.b2[tn]));
-    }
-#pragma unroll
-    for (int g = 0; g < NG; g++)
-#pragma unroll
-        for (int tn = 0; tn < 4; tn++) a[g][tn] = relu4(z[g][tn]);
-#pragma unroll
-    for (int g = 0; g < NG; g++)
-        z3[g] = mma32(W.w3[1], cat8(a[g][2], a[g][3]), mma32(W.w3[0], cat8(a[g][0], a[g][1]), W.b3));
-}
-
 // Per-wave feature tile of 64 samples: row = sample, 48 BF16 inputs, row stride XS BF16
 // pairs (104 B: conflict-free 8-B row writes); the rows are read back as B operands.
 constexpr int XS = 26;
