@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_closure_features(ClosureArgs<T> a)
 // Each wave owns chunks of 64 consecutive grid points (lane = point while gathering the
 // features and applying the stress) and keeps the network in registers; features are
 // normalised as (x - mu) * rsd, rsd = 1 / sd rounded to FP32, and rounded to BF16.  The
-// network runs as in the fused substep (closure_eval: groups one after another, waits
+// network runs as in the fused substep (closure_eval: the four groups in two pairs, waits
 // around each layer's products).
 // norm = {mu[45], rsd[45]}; wpk: the packed BF16 weights (NWPK values).
 

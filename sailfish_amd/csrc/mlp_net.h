@@ -179,7 +179,7 @@ __device__ __forceinline__ void forward16(const Fwd &W, bf16x8_t xa, bf16x4_t xb
 // whose A/B operand registers were written by a VALU instruction a few cycles before, or
 // are overwritten by one shortly after the product issues, can read wrong operand values:
 // observed as wrong outputs for whole 16-sample groups, depending only on the instruction
-// schedule.  forward16_held keeps each layer's operands unchanged from 5 wait states
+// schedule.  forward32_held keeps each layer's operands unchanged from 5 wait states
 // before its products until 16 after them (layer granularity: one wait before, one after).
 #define SF_SETTLE(...) asm volatile("s_nop 4" : __VA_ARGS__)
 #define SF_HOLD(...)                                        \
@@ -189,35 +189,54 @@ __device__ __forceinline__ void forward16(const Fwd &W, bf16x8_t xa, bf16x4_t xb
         __builtin_amdgcn_sched_barrier(0);                  \
     } while (0)
 
-// forward16 with the waits above around each layer's products (same sums, same results)
-__device__ __forceinline__ void forward16_held(const Fwd &W, bf16x8_t xa, bf16x4_t xb, Acts &A)
+// forward16 for two 16-sample groups at once (inputs xa/xb and ya/yb; network outputs zx,
+// zy in forward16's z3 layout), with the waits above: one before and one after each
+// layer's products of both groups, which issue back to back.  Each group's sums are those
+// of forward16, so its outputs are too.
+__device__ __forceinline__ void forward32_held(const Fwd &W, bf16x8_t xa, bf16x4_t xb, bf16x8_t ya, bf16x4_t yb,
+                                               f32x4_t &zx, f32x4_t &zy)
 {
     bf16x8_t a0 = W.w1a[0], a1 = W.w1a[1], a2 = W.w1a[2], a3 = W.w1a[3];
     bf16x4_t c0 = W.w1b[0], c1 = W.w1b[1], c2 = W.w1b[2], c3 = W.w1b[3];
-    SF_SETTLE("+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3), "+v"(xa), "+v"(xb));
-    f32x4_t z[4];
+    SF_SETTLE("+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3), "+v"(xa), "+v"(xb),
+              "+v"(ya), "+v"(yb));
+    f32x4_t z[4], y[4];
     z[0] = mma16(c0, xb, mma32(a0, xa, W.b1[0]));
+    y[0] = mma16(c0, yb, mma32(a0, ya, W.b1[0]));
     z[1] = mma16(c1, xb, mma32(a1, xa, W.b1[1]));
+    y[1] = mma16(c1, yb, mma32(a1, ya, W.b1[1]));
     z[2] = mma16(c2, xb, mma32(a2, xa, W.b1[2]));
+    y[2] = mma16(c2, yb, mma32(a2, ya, W.b1[2]));
     z[3] = mma16(c3, xb, mma32(a3, xa, W.b1[3]));
-    SF_HOLD("v"(a0), "v"(a1), "v"(a2), "v"(a3), "v"(c0), "v"(c1), "v"(c2), "v"(c3), "v"(xa), "v"(xb));
+    y[3] = mma16(c3, yb, mma32(a3, ya, W.b1[3]));
+    SF_HOLD("v"(a0), "v"(a1), "v"(a2), "v"(a3), "v"(c0), "v"(c1), "v"(c2), "v"(c3), "v"(xa), "v"(xb), "v"(ya),
+            "v"(yb));
+    u32x2_t p[4], s[4];
 #pragma unroll
-    for (int tn = 0; tn < 4; tn++) A.a1[tn] = relu4(z[tn]);
-    bf16x8_t h0 = cat8(A.a1[0], A.a1[1]), h1 = cat8(A.a1[2], A.a1[3]);
+    for (int tn = 0; tn < 4; tn++) p[tn] = relu4(z[tn]), s[tn] = relu4(y[tn]);
+    bf16x8_t h0 = cat8(p[0], p[1]), h1 = cat8(p[2], p[3]), k0 = cat8(s[0], s[1]), k1 = cat8(s[2], s[3]);
     bf16x8_t w0 = W.w2[0], w1 = W.w2[1], w2 = W.w2[2], w3 = W.w2[3], w4 = W.w2[4], w5 = W.w2[5], w6 = W.w2[6],
              w7 = W.w2[7];
-    SF_SETTLE("+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5), "+v"(w6), "+v"(w7), "+v"(h0), "+v"(h1));
+    SF_SETTLE("+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5), "+v"(w6), "+v"(w7), "+v"(h0), "+v"(h1),
+              "+v"(k0), "+v"(k1));
     z[0] = mma32(w1, h1, mma32(w0, h0, W.b2[0]));
+    y[0] = mma32(w1, k1, mma32(w0, k0, W.b2[0]));
     z[1] = mma32(w3, h1, mma32(w2, h0, W.b2[1]));
+    y[1] = mma32(w3, k1, mma32(w2, k0, W.b2[1]));
     z[2] = mma32(w5, h1, mma32(w4, h0, W.b2[2]));
+    y[2] = mma32(w5, k1, mma32(w4, k0, W.b2[2]));
     z[3] = mma32(w7, h1, mma32(w6, h0, W.b2[3]));
-    SF_HOLD("v"(w0), "v"(w1), "v"(w2), "v"(w3), "v"(w4), "v"(w5), "v"(w6), "v"(w7), "v"(h0), "v"(h1));
+    y[3] = mma32(w7, k1, mma32(w6, k0, W.b2[3]));
+    SF_HOLD("v"(w0), "v"(w1), "v"(w2), "v"(w3), "v"(w4), "v"(w5), "v"(w6), "v"(w7), "v"(h0), "v"(h1), "v"(k0),
+            "v"(k1));
 #pragma unroll
-    for (int tn = 0; tn < 4; tn++) A.a2[tn] = relu4(z[tn]);
-    bf16x8_t g0 = cat8(A.a2[0], A.a2[1]), g1 = cat8(A.a2[2], A.a2[3]), v0 = W.w3[0], v1 = W.w3[1];
-    SF_SETTLE("+v"(v0), "+v"(v1), "+v"(g0), "+v"(g1));
-    A.z3 = mma32(v1, g1, mma32(v0, g0, W.b3));
-    SF_HOLD("v"(v0), "v"(v1), "v"(g0), "v"(g1));
+    for (int tn = 0; tn < 4; tn++) p[tn] = relu4(z[tn]), s[tn] = relu4(y[tn]);
+    bf16x8_t g0 = cat8(p[0], p[1]), g1 = cat8(p[2], p[3]), e0 = cat8(s[0], s[1]), e1 = cat8(s[2], s[3]);
+    bf16x8_t v0 = W.w3[0], v1 = W.w3[1];
+    SF_SETTLE("+v"(v0), "+v"(v1), "+v"(g0), "+v"(g1), "+v"(e0), "+v"(e1));
+    zx = mma32(v1, g1, mma32(v0, g0, W.b3));
+    zy = mma32(v1, e1, mma32(v0, e0, W.b3));
+    SF_HOLD("v"(v0), "v"(v1), "v"(g0), "v"(g1), "v"(e0), "v"(e1));
 }
 
 // Per-wave feature tile of 64 samples: row = sample, 48 BF16 inputs, row stride XS BF16
@@ -250,19 +269,21 @@ __device__ __forceinline__ void closure_row(uint32_t *X, int lane, const float *
 }
 
 // Network outputs of the wave's 64 samples (tile X, written by closure_row): lane l gets
-// outputs 0 and 1 of sample l.  The four 16-sample groups run one after another.
+// outputs 0 and 1 of sample l.  The 16-sample groups run in pairs (0, 1), then (2, 3).
 __device__ __forceinline__ void closure_eval(const Fwd &W, uint32_t *X, float &zu, float &zv)
 {
     const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
     wave_lds_sync();
     zu = zv = 0.0f;
 #pragma unroll
-    for (int gq = 0; gq < 4; gq++) {
-        const u32x2_t *xr = (const u32x2_t *)(X + (16 * gq + r) * XS);
-        Acts A;
-        forward16_held(W, cat8(xr[q], xr[4 + q]), as4(xr[8 + q]), A);
-        const float tu = __shfl(A.z3[0], r), tv = __shfl(A.z3[1], r);   // lanes q = 0 hold the outputs
+    for (int gq = 0; gq < 4; gq += 2) {
+        const u32x2_t *xr = (const u32x2_t *)(X + (16 * gq + r) * XS), *yr = xr + 16 * XS / 2;
+        f32x4_t zx, zy;
+        forward32_held(W, cat8(xr[q], xr[4 + q]), as4(xr[8 + q]), cat8(yr[q], yr[4 + q]), as4(yr[8 + q]), zx, zy);
+        // lanes q = 0 hold the outputs of sample r of each group
+        const float tu = __shfl(zx[0], r), tv = __shfl(zx[1], r), su = __shfl(zy[0], r), sv = __shfl(zy[1], r);
         if (q == gq) zu = tu, zv = tv;
+        if (q == gq + 1) zu = su, zv = sv;
     }
     wave_lds_sync();   // the tile is rewritten by the next chunk
 }

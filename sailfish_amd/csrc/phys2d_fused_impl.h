@@ -382,6 +382,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     bool cin[KC];
     uint32_t cf[KC];
     T z1c[KC], u1c[KC], v1c[KC], hc_[KC], hw_[KC], hs_[KC];
+    T clh[CLO ? KC : 1][9], clf[CLO ? KC : 1][9];   // CLO: h and f over each core point's 3x3 stencil
+    T clsu[CLO ? KC : 1], clsv[CLO ? KC : 1];       // CLO: wind stress at the core point
     T pm_[KC], pn_[KC];
     T onuc[KC], omuc[KC], cffuc[KC], omvc[KC], onvc[KC], cffvc[KC], pmnc[KC];
     T zsc[KC], ruc[KC], rvc[KC], duc[KC], dvc[KC];
@@ -420,6 +422,17 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             pn_[k] = ldo(P.pn, 0, f);
             if (i >= 1) hw_[k] = ldo(P.h, -1, f);
             if (j >= 1) hs_[k] = ldo(P.h, -W, f);
+        }
+        if constexpr (CLO) {   // the closure's h and f stencils and wind stress, in flight with the frame loads
+            const bool in = rho_int(i, j);
+#pragma unroll
+            for (int d = 0; d < 9; d++) {
+                const long long e = (long long)(d / 3 - 1) * W + (d % 3 - 1);
+                clh[k][d] = in ? ldo(P.h, e, f) : T(0);
+                clf[k][d] = in ? ldo(P.cl_f, e, f) : T(0);
+            }
+            clsu[k] = u_int(i, j) ? ldo(P.sustr, 0, f) : T(0);
+            clsv[k] = v_int(i, j) ? ldo(P.svstr, 0, f) : T(0);
         }
     }
     {
@@ -472,8 +485,9 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
 
     // ---- CL: the learned closure's corrector-stage stresses at the core points ----------
     // Each wave evaluates the network on its core row(s) of 64 points (features from the
-    // t1 frame planes, h and f from memory) and keeps sustr + R_u D_u / svstr + R_v D_v of
-    // its points for C2; those of the ghost-source rows / columns (j = 1, M-1 / i = 1, L-1)
+    // t1 frame planes; h, f and the wind stress loaded with the frame) and keeps
+    // sustr + R_u D_u / svstr + R_v D_v of its points for C2; those of the ghost-source
+    // rows / columns (j = 1, M-1 / i = 1, L-1)
     // also go to clS for the ghost points of C2.  The same arithmetic as sf_closure_stress.
     T csu[KC], csv[KC];
     if constexpr (CLO) {
@@ -499,9 +513,9 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     cv = pl[o];
                     nv = pl[o + d];
                 } else {
-                    const T *src = fld == 3 ? P.h : P.cl_f;
-                    cv = in ? ldo(src, 0, f) : T(0);
-                    nv = in ? ldo(src, dj * W + di, f) : T(0);
+                    const T(&st9)[9] = fld == 3 ? clh[k] : clf[k];
+                    cv = st9[4];
+                    nv = st9[(dj + 1) * 3 + (di + 1)];
                 }
                 xv[kk] = (dj == 0 && di == 0) ? (float)cv : (float)(nv - cv);
             }
@@ -511,13 +525,13 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             csu[k] = csv[k] = T(0);   // read in C2 only at u / v interior points
             if (u_int(i, j)) {
                 const T Du = T(0.5) * ((hw_[k] + sZ1[o - 1]) + (hc_[k] + sZ1[o]));
-                csu[k] = ldo(P.sustr, 0, f) + T(zu * P.cl_ynorm[1] + P.cl_ynorm[0]) * Du;
+                csu[k] = clsu[k] + T(zu * P.cl_ynorm[1] + P.cl_ynorm[0]) * Du;
                 if (j == 1) clS[a - 3] = csu[k];
                 if (j == M - 1) clS[TI + a - 3] = csu[k];
             }
             if (v_int(i, j)) {
                 const T Dv = T(0.5) * ((hs_[k] + sZ1[o - FW]) + (hc_[k] + sZ1[o]));
-                csv[k] = ldo(P.svstr, 0, f) + T(zv * P.cl_ynorm[3] + P.cl_ynorm[2]) * Dv;
+                csv[k] = clsv[k] + T(zv * P.cl_ynorm[3] + P.cl_ynorm[2]) * Dv;
                 if (i == 1) clS[2 * TI + b - 3] = csv[k];
                 if (i == L - 1) clS[2 * TI + TJ + b - 3] = csv[k];
             }
