@@ -568,13 +568,14 @@ def run_closure_gpu(args):
     points, dx=312.5 m, dt=1.25 s) coarse-grained to 512x256 (dx=1.25 km, dt=5 s),
     2e6 samples, BF16-MFMA training, then 5000 coarse steps plain and with the
     closure against the coarse-grained fine run; speed of fine vs coarse+closure
-    (FP32, graph replays)."""
+    (FP32 and FP64, graph replays)."""
     import torch
     from sailfish_amd import closure as CL
 
     res = CL.run_experiment(dtype=torch.float64, log=lambda *a: print("[closure]", *a, file=sys.stderr, flush=True))
     mlp = res.pop("mlp")
     sp = CL.speed(mlp, dtype=torch.float32)
+    sp64 = CL.speed(mlp, dtype=torch.float64, training=False)
     mlp.close()
     last = res["series"][-1]
     return {
@@ -591,6 +592,7 @@ def run_closure_gpu(args):
                                   "signal_max": last["signal"], "order": ["zeta_m", "ubar_m_s", "vbar_m_s"]},
         "selected_model": res["validation"]["selected"],
         "speed_fp32": sp,
+        "speed_fp64": sp64,
         "experiment_s": res["total_s"],
     }
 

@@ -680,9 +680,11 @@ def compare_runs(fcfg: dict, mlp: ClosureMLP | None, ratio: int, nsteps: int, ev
     return series
 
 
-def speed(mlp: ClosureMLP, coarse_shape=(512, 256), ratio: int = 4, nsub: int = 300, dtype=torch.float32) -> dict:
+def speed(mlp: ClosureMLP, coarse_shape=(512, 256), ratio: int = 4, nsub: int = 300, dtype=torch.float32,
+          training: bool = True) -> dict:
     """GPU time per substep of the fine run and of the coarse run with the closure
-    (graph replays), and the wall-time ratio for the same simulated period."""
+    (graph replays), the closure's share of the coarse substep, the wall-time ratio for the
+    same simulated period and (``training``) the training-step rate."""
     fcfg = fine_config(*coarse_shape, ratio)
     ccfg = coarse_config(fcfg)
     out = {}
@@ -705,7 +707,9 @@ def speed(mlp: ClosureMLP, coarse_shape=(512, 256), ratio: int = 4, nsub: int = 
     for k in out:
         out[k]["s_per_sim_hour"] = 3600.0 / out[k]["dt"] * out[k]["ms_per_substep"] * 1e-3
     out["speedup_closure_vs_fine"] = out["fine"]["s_per_sim_hour"] / out["coarse_closure"]["s_per_sim_hour"]
-    out["training"] = train_rate(mlp.get_params())
+    out["closure_adds"] = out["coarse_closure"]["ms_per_substep"] / out["coarse_plain"]["ms_per_substep"] - 1.0
+    if training:
+        out["training"] = train_rate(mlp.get_params())
     return out
 
 

@@ -121,6 +121,12 @@ struct PhysLds {
     static constexpr int FW = TI + 5, FH = TJ + 5, FN = FW * FH, PLANES = 11;
 };
 
+// FP64 closure instances keep the network's feature tiles in frame planes 1..6 (written
+// only from stage A on, after a barrier that ends the closure phase): 27 KB less LDS, so
+// two 64 x 4 workgroups fit a CU.  FP32 planes are too small; the tiles get their own array.
+template <typename T, bool CLO>
+constexpr bool cl_tiles_in_frame = CLO && sizeof(T) == 8;
+
 // Element (f + d) of p, given the byte offset boff of element f: the address is the
 // array's scalar base plus a 32-bit vector offset (global_load saddr form).  The
 // neighbour offset boff + d*sizeof(T) is computed in 32-bit arithmetic, so it is one
@@ -537,6 +543,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             }
         }
     }
+
+    if constexpr (cl_tiles_in_frame<T, CLO>) lds_barrier();   // feature tiles read; planes 1..6 free
 
     // ---- A: DU1 at u points, DV1 at v points ---------------------------------------------
     {
@@ -1112,11 +1120,15 @@ __global__ __launch_bounds__(NT) void k_phys_fused(PFA<T> P)
     __shared__ T sm[PhysLds<TI, TJ>::PLANES][PhysLds<TI, TJ>::FN];
     __shared__ T side[2 * TI + 2 * TJ];
     __shared__ double sideF[CPL ? 2 * TI + 2 * TJ : 1];
-    __shared__ __attribute__((aligned(16))) uint32_t clX[CLO ? (NT / 64) * 64 * XS : 1];
+    constexpr bool IN_FRAME = cl_tiles_in_frame<T, CLO>;
+    static_assert(!IN_FRAME || 6 * PhysLds<TI, TJ>::FN * sizeof(T) >= (NT / 64) * 64 * XS * sizeof(uint32_t),
+                  "frame planes 1..6 hold the feature tiles");
+    __shared__ __attribute__((aligned(16))) uint32_t clX[CLO && !IN_FRAME ? (NT / 64) * 64 * XS : 1];
     __shared__ float clN[CLO ? 2 * NIN : 1];
     __shared__ T clS[CLO ? 2 * TI + 2 * TJ : 1];
     const int b = xcd_band(P.tiles_i * P.tiles_j);
-    phys_tile<T, TI, TJ, NT, FIRST, NTS, CPL, CLO>(P, b % P.tiles_i, b / P.tiles_i, sm, side, sideF, clX, clN, clS);
+    phys_tile<T, TI, TJ, NT, FIRST, NTS, CPL, CLO>(P, b % P.tiles_i, b / P.tiles_i, sm, side, sideF,
+                                                   IN_FRAME ? (uint32_t *)sm[1] : clX, clN, clS);
 }
 
 
