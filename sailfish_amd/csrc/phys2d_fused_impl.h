@@ -204,11 +204,25 @@ struct UVIn {
     T x0, z0c, z0b;             // predictor "before" velocity and zeta (t0) at point / back
 };
 
+// The tile's arguments.  Closure instances (launched only as k_phys_fused, whose one kernel
+// argument is P) read them through the kernel-argument segment pointer, which an empty asm
+// statement re-derives after the closure phase; other instances use P itself.
+template <typename T, bool CLO>
+__device__ __forceinline__ auto arg_ptr(const PFA<T> &P)
+{
+    if constexpr (CLO) return (const __attribute__((address_space(4))) PFA<T> *)__builtin_amdgcn_kernarg_segment_ptr();
+    else return &P;
+}
+
 template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS, bool CPL = false, bool CLO = false>
 __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*sm)[PhysLds<TI, TJ>::FN],
                                           T *side, double *sideF = nullptr, uint32_t *clX = nullptr,
                                           float *clN = nullptr, T *clS = nullptr)
 {
+    // the arguments through a pointer that the closure instances re-derive after the closure
+    // phase, so that pointers first needed from stage A on are loaded there again instead of
+    // being held across the phase
+    auto Pp = arg_ptr<T, CLO>(P);
     constexpr int FW = TI + 5;
     constexpr int KC = TI * TJ / NT;
     constexpr uint32_t ES = sizeof(T);
@@ -223,11 +237,11 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     double *const sFUlo = sideF, *const sFUhi = sideF + TI, *const sFVlo = sideF + 2 * TI,
                  *const sFVhi = sideF + 2 * TI + TJ;
 
-    const int szI = P.szI, szJ = P.szJ, L = szI - 1, M = szJ - 1;
+    const int szI = Pp->szI, szJ = Pp->szJ, L = szI - 1, M = szJ - 1;
     const long long W = szI;
-    const int fi0 = ti * TI - P.off_i - 3, fj0 = tj * TJ - P.off_j - 3;   // frame origin
+    const int fi0 = ti * TI - Pp->off_i - 3, fj0 = tj * TJ - Pp->off_j - 3;   // frame origin
     const int tid = threadIdx.x;
-    const T dt = P.dt;
+    const T dt = Pp->dt;
     constexpr bool first = FIRST;
 
     auto inside = [&](int i, int j) { return i >= 0 && i <= L && j >= 0 && j <= M; };
@@ -241,15 +255,15 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     };
 
     // metric factors of a u point / v point at byte offset o
-    auto onu_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(P.pn, -1, o) + ldo(P.pn, 0, o)); };
-    auto omu_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(P.pm, -1, o) + ldo(P.pm, 0, o)); };
-    auto omv_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(P.pm, -W, o) + ldo(P.pm, 0, o)); };
-    auto onv_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(P.pn, -W, o) + ldo(P.pn, 0, o)); };
+    auto onu_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(Pp->pn, -1, o) + ldo(Pp->pn, 0, o)); };
+    auto omu_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(Pp->pm, -1, o) + ldo(Pp->pm, 0, o)); };
+    auto omv_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(Pp->pm, -W, o) + ldo(Pp->pm, 0, o)); };
+    auto onv_at = [&](uint32_t o) { return T(2) * sf::recip(ldo(Pp->pn, -W, o) + ldo(Pp->pn, 0, o)); };
     auto cffu_at = [&](uint32_t o) {
-        return (ldo(P.pm, 0, o) + ldo(P.pm, -1, o)) * (ldo(P.pn, 0, o) + ldo(P.pn, -1, o));
+        return (ldo(Pp->pm, 0, o) + ldo(Pp->pm, -1, o)) * (ldo(Pp->pn, 0, o) + ldo(Pp->pn, -1, o));
     };
     auto cffv_at = [&](uint32_t o) {
-        return (ldo(P.pm, 0, o) + ldo(P.pm, -W, o)) * (ldo(P.pn, 0, o) + ldo(P.pn, -W, o));
+        return (ldo(Pp->pm, 0, o) + ldo(Pp->pm, -W, o)) * (ldo(Pp->pn, 0, o) + ldo(Pp->pn, -W, o));
     };
 
     // generic (load-on-demand) inputs of the u / v right-hand side at offset f
@@ -259,23 +273,23 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         x.pmb = sPM[q - 1];
         x.pnc = sPN[q];
         x.pnb = sPN[q - 1];
-        x.hb = ldo(P.h, -1, f);
-        x.hc = ldo(P.h, 0, f);
+        x.hb = ldo(Pp->h, -1, f);
+        x.hc = ldo(Pp->h, 0, f);
         x.xb = sU1[q - 1];
         x.xc = sU1[q];
         x.xf = sU1[q + 1];
         x.yf = sU1[q + FW];
         x.yb = sU1[q - FW];
-        x.fc = ldo(P.fomn, 0, f);
-        x.fb = ldo(P.fomn, -1, f);
+        x.fc = ldo(Pp->fomn, 0, f);
+        x.fb = ldo(Pp->fomn, -1, f);
         x.oc = sV1[q];
         x.of = sV1[q + FW];
         x.ob = sV1[q - 1];
         x.obf = sV1[q + FW - 1];
-        x.st = ldo(P.sustr, 0, f);
-        x.x0 = first ? x.xc : ldo(P.u0, 0, f);
-        x.z0c = first ? T(0) : ldo(P.z0, 0, f);
-        x.z0b = first ? T(0) : ldo(P.z0, -1, f);
+        x.st = ldo(Pp->sustr, 0, f);
+        x.x0 = first ? x.xc : ldo(Pp->u0, 0, f);
+        x.z0c = first ? T(0) : ldo(Pp->z0, 0, f);
+        x.z0b = first ? T(0) : ldo(Pp->z0, -1, f);
         return x;
     };
     auto load_v = [&](uint32_t f, int q) {
@@ -284,23 +298,23 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         x.pmb = sPM[q - FW];
         x.pnc = sPN[q];
         x.pnb = sPN[q - FW];
-        x.hb = ldo(P.h, -W, f);
-        x.hc = ldo(P.h, 0, f);
+        x.hb = ldo(Pp->h, -W, f);
+        x.hc = ldo(Pp->h, 0, f);
         x.xb = sV1[q - FW];
         x.xc = sV1[q];
         x.xf = sV1[q + FW];
         x.yf = sV1[q + 1];
         x.yb = sV1[q - 1];
-        x.fc = ldo(P.fomn, 0, f);
-        x.fb = ldo(P.fomn, -W, f);
+        x.fc = ldo(Pp->fomn, 0, f);
+        x.fb = ldo(Pp->fomn, -W, f);
         x.oc = sU1[q];
         x.of = sU1[q + 1];
         x.ob = sU1[q - FW];
         x.obf = sU1[q + 1 - FW];
-        x.st = ldo(P.svstr, 0, f);
-        x.x0 = first ? x.xc : ldo(P.v0, 0, f);
-        x.z0c = first ? T(0) : ldo(P.z0, 0, f);
-        x.z0b = first ? T(0) : ldo(P.z0, -W, f);
+        x.st = ldo(Pp->svstr, 0, f);
+        x.x0 = first ? x.xc : ldo(Pp->v0, 0, f);
+        x.z0c = first ? T(0) : ldo(Pp->z0, 0, f);
+        x.z0b = first ? T(0) : ldo(Pp->z0, -W, f);
         return x;
     };
 
@@ -321,13 +335,13 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     // extrapolation over the baroclinic steps, the others add F (as pred_forcing of the
     // four-launch form); F is returned in Fk
     auto pred_forcing = [&](bool vdir, uint32_t fd, T r, double &Fk) {
-        const double *Fo = vdir ? P.c.Fv : P.c.Fu;
+        const double *Fo = vdir ? Pp->c.Fv : Pp->c.Fu;
         if constexpr (FIRST) {
-            const double F = ldo(vdir ? P.c.rvfrc : P.c.rufrc, 0, fd) - (double)r;
+            const double F = ldo(vdir ? Pp->c.rvfrc : Pp->c.rufrc, 0, fd) - (double)r;
             Fk = F;
-            const double F1 = P.c.ab > 0 ? ldo(vdir ? P.c.Fv1 : P.c.Fu1, 0, fd) : 0.0;
-            const double F2 = P.c.ab > 1 ? ldo(vdir ? P.c.Fv2 : P.c.Fu2, 0, fd) : 0.0;
-            return r + T(sf::ab_extrapolate(P.c.ab, F, F1, F2));
+            const double F1 = Pp->c.ab > 0 ? ldo(vdir ? Pp->c.Fv1 : Pp->c.Fu1, 0, fd) : 0.0;
+            const double F2 = Pp->c.ab > 1 ? ldo(vdir ? Pp->c.Fv2 : Pp->c.Fu2, 0, fd) : 0.0;
+            return r + T(sf::ab_extrapolate(Pp->c.ab, F, F1, F2));
         } else {
             Fk = ldo(Fo, 0, fd);
             return r + T(Fk);
@@ -342,7 +356,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     auto pred_u = [&](int q, const UVIn<T> &x, T &r, uint32_t fd = 0, double *Fk = nullptr) {
         const T onu = x.m1, omu = x.m2;
         const T g0 = gzp(q - 1, x.z0b), g1 = gzp(q, x.z0c);
-        r = ((T(0.5) * P.g) * onu) * ((x.hb + x.hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
+        r = ((T(0.5) * Pp->g) * onu) * ((x.hb + x.hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
         const T UFx1 = T(0.25) * (sDU[q] + sDU[q + 1]) * (x.xc + x.xf);
         const T UFx0 = T(0.25) * (sDU[q - 1] + sDU[q]) * (x.xb + x.xc);
         const T UFe1 = T(0.25) * (sDV[q + FW] + sDV[q + FW - 1]) * (x.yf + x.xc);
@@ -351,7 +365,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         const T cr1 = ((T(0.5) * (x.hc + sZ1[q])) * x.fc) * (x.oc + x.of);
         const T cr0 = ((T(0.5) * (x.hb + sZ1[q - 1])) * x.fb) * (x.ob + x.obf);
         r = r + T(0.5) * (cr1 + cr0);
-        r = r - ((P.rdrg * x.xc) * omu) * onu;
+        r = r - ((Pp->rdrg * x.xc) * omu) * onu;
         r = r + (x.st * omu) * onu;
         if constexpr (CPL) {
             double F;
@@ -364,7 +378,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     auto pred_v = [&](int q, const UVIn<T> &x, T &r, uint32_t fd = 0, double *Fk = nullptr) {
         const T omv = x.m1, onv = x.m2;
         const T g0 = gzp(q - FW, x.z0b), g1 = gzp(q, x.z0c);
-        r = ((T(0.5) * P.g) * omv) * ((x.hb + x.hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
+        r = ((T(0.5) * Pp->g) * omv) * ((x.hb + x.hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
         const T VFx1 = T(0.25) * (sDU[q + 1] + sDU[q + 1 - FW]) * (x.yf + x.xc);
         const T VFx0 = T(0.25) * (sDU[q] + sDU[q - FW]) * (x.xc + x.yb);
         const T VFe1 = T(0.25) * (sDV[q] + sDV[q + FW]) * (x.xc + x.xf);
@@ -373,7 +387,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         const T cr1 = ((T(0.5) * (x.hc + sZ1[q])) * x.fc) * (x.oc + x.of);
         const T cr0 = ((T(0.5) * (x.hb + sZ1[q - FW])) * x.fb) * (x.ob + x.obf);
         r = r - T(0.5) * (cr1 + cr0);
-        r = r - ((P.rdrg * x.xc) * omv) * onv;
+        r = r - ((Pp->rdrg * x.xc) * omv) * onv;
         r = r + (x.st * omv) * onv;
         if constexpr (CPL) {
             double F;
@@ -406,7 +420,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     auto v_int = [&](int i, int j) { return i >= 1 && i <= L - 1 && j >= 2 && j <= M - 1; };
 
     if constexpr (CLO)   // input normalisation of the closure (read after L's barrier)
-        if (tid < 2 * NIN) clN[tid] = P.cl_norm[tid];
+        if (tid < 2 * NIN) clN[tid] = Pp->cl_norm[tid];
 
     // ---- G0 + L: z1 over the frame; core statics -----------------------------------------
 #pragma unroll
@@ -420,25 +434,25 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         // values below are read only under the predicate they are loaded under
         z1c[k] = T(0);
         if (cin[k]) {
-            z1c[k] = ldo(P.z1, 0, f);
-            u1c[k] = ldo(P.u1, 0, f);
-            v1c[k] = ldo(P.v1, 0, f);
-            hc_[k] = ldo(P.h, 0, f);
-            pm_[k] = ldo(P.pm, 0, f);
-            pn_[k] = ldo(P.pn, 0, f);
-            if (i >= 1) hw_[k] = ldo(P.h, -1, f);
-            if (j >= 1) hs_[k] = ldo(P.h, -W, f);
+            z1c[k] = ldo(Pp->z1, 0, f);
+            u1c[k] = ldo(Pp->u1, 0, f);
+            v1c[k] = ldo(Pp->v1, 0, f);
+            hc_[k] = ldo(Pp->h, 0, f);
+            pm_[k] = ldo(Pp->pm, 0, f);
+            pn_[k] = ldo(Pp->pn, 0, f);
+            if (i >= 1) hw_[k] = ldo(Pp->h, -1, f);
+            if (j >= 1) hs_[k] = ldo(Pp->h, -W, f);
         }
         if constexpr (CLO) {   // the closure's h and f stencils and wind stress, in flight with the frame loads
             const bool in = rho_int(i, j);
 #pragma unroll
             for (int d = 0; d < 9; d++) {
                 const long long e = (long long)(d / 3 - 1) * W + (d % 3 - 1);
-                clh[k][d] = in ? ldo(P.h, e, f) : T(0);
-                clf[k][d] = in ? ldo(P.cl_f, e, f) : T(0);
+                clh[k][d] = in ? ldo(Pp->h, e, f) : T(0);
+                clf[k][d] = in ? ldo(Pp->cl_f, e, f) : T(0);
             }
-            clsu[k] = u_int(i, j) ? ldo(P.sustr, 0, f) : T(0);
-            clsv[k] = v_int(i, j) ? ldo(P.svstr, 0, f) : T(0);
+            clsu[k] = u_int(i, j) ? ldo(Pp->sustr, 0, f) : T(0);
+            clsv[k] = v_int(i, j) ? ldo(Pp->svstr, 0, f) : T(0);
         }
     }
     {
@@ -458,11 +472,11 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 ro[s] = b * FW + a;
                 if (inside(i, j)) {
                     const uint32_t f = off(i, j);
-                    rz[s] = ldo(P.z1, 0, f);
-                    ru[s] = ldo(P.u1, 0, f);
-                    rv[s] = ldo(P.v1, 0, f);
-                    rm[s] = ldo(P.pm, 0, f);
-                    rn[s] = ldo(P.pn, 0, f);
+                    rz[s] = ldo(Pp->z1, 0, f);
+                    ru[s] = ldo(Pp->u1, 0, f);
+                    rv[s] = ldo(Pp->v1, 0, f);
+                    rm[s] = ldo(Pp->pm, 0, f);
+                    rn[s] = ldo(Pp->pn, 0, f);
                 }
             }
         }
@@ -499,7 +513,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     if constexpr (CLO) {
 #pragma clang fp contract(off)   // the same roundings as sf_closure_stress in both precisions
         Fwd Wn;
-        load_fwd(Wn, P.cl_wpk, P.cl_w);
+        load_fwd(Wn, Pp->cl_wpk, Pp->cl_w);
         uint32_t *X = clX + (tid >> 6) * (64 * XS);
 #pragma unroll
         for (int k = 0; k < KC; k++) {
@@ -531,13 +545,13 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             csu[k] = csv[k] = T(0);   // read in C2 only at u / v interior points
             if (u_int(i, j)) {
                 const T Du = T(0.5) * ((hw_[k] + sZ1[o - 1]) + (hc_[k] + sZ1[o]));
-                csu[k] = clsu[k] + T(zu * P.cl_ynorm[1] + P.cl_ynorm[0]) * Du;
+                csu[k] = clsu[k] + T(zu * Pp->cl_ynorm[1] + Pp->cl_ynorm[0]) * Du;
                 if (j == 1) clS[a - 3] = csu[k];
                 if (j == M - 1) clS[TI + a - 3] = csu[k];
             }
             if (v_int(i, j)) {
                 const T Dv = T(0.5) * ((hs_[k] + sZ1[o - FW]) + (hc_[k] + sZ1[o]));
-                csv[k] = clsv[k] + T(zv * P.cl_ynorm[3] + P.cl_ynorm[2]) * Dv;
+                csv[k] = clsv[k] + T(zv * Pp->cl_ynorm[3] + Pp->cl_ynorm[2]) * Dv;
                 if (i == 1) clS[2 * TI + b - 3] = csv[k];
                 if (i == L - 1) clS[2 * TI + TJ + b - 3] = csv[k];
             }
@@ -545,6 +559,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     }
 
     if constexpr (cl_tiles_in_frame<T, CLO>) lds_barrier();   // feature tiles read; planes 1..6 free
+    if constexpr (CLO) asm volatile("" : "+s"(Pp));
 
     // ---- A: DU1 at u points, DV1 at v points ---------------------------------------------
     {
@@ -566,9 +581,9 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     ri[s] = i;
                     rj[s] = j;
                     const uint32_t f = off(i, j);
-                    rh[s] = ldo(P.h, 0, f);
-                    if (i >= 1) rhw[s] = ldo(P.h, -1, f);
-                    if (j >= 1) rhs[s] = ldo(P.h, -W, f);
+                    rh[s] = ldo(Pp->h, 0, f);
+                    if (i >= 1) rhw[s] = ldo(Pp->h, -1, f);
+                    if (j >= 1) rhs[s] = ldo(Pp->h, -W, f);
                 }
             }
         }
@@ -628,24 +643,24 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const uint32_t f = cf[k];
             const bool ui = cin[k] && u_int(i, j), vi = cin[k] && v_int(i, j), ri_ = cin[k] && rho_int(i, j);
             if (ri_ || ui || vi) {
-                if (!first) z0c_[k] = ldo(P.z0, 0, f);
-                fo_[k] = ldo(P.fomn, 0, f);
+                if (!first) z0c_[k] = ldo(Pp->z0, 0, f);
+                fo_[k] = ldo(Pp->fomn, 0, f);
             }
             if (ui) {
                 if (!first) {
-                    z0w_[k] = ldo(P.z0, -1, f);
-                    u0c_[k] = ldo(P.u0, 0, f);
+                    z0w_[k] = ldo(Pp->z0, -1, f);
+                    u0c_[k] = ldo(Pp->u0, 0, f);
                 }
-                fow_[k] = ldo(P.fomn, -1, f);
-                su_[k] = ldo(P.sustr, 0, f);
+                fow_[k] = ldo(Pp->fomn, -1, f);
+                su_[k] = ldo(Pp->sustr, 0, f);
             }
             if (vi) {
                 if (!first) {
-                    z0s_[k] = ldo(P.z0, -W, f);
-                    v0c_[k] = ldo(P.v0, 0, f);
+                    z0s_[k] = ldo(Pp->z0, -W, f);
+                    v0c_[k] = ldo(Pp->v0, 0, f);
                 }
-                fos_[k] = ldo(P.fomn, -W, f);
-                sv_[k] = ldo(P.svstr, 0, f);
+                fos_[k] = ldo(Pp->fomn, -W, f);
+                sv_[k] = ldo(Pp->svstr, 0, f);
             }
         }
     lds_barrier();
@@ -669,7 +684,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 if (inside(i, j)) {
                     const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
                     rq[s] = ro[s] + (qj - j) * FW + (qi - i);
-                    if (!first) rz0[s] = ldo(P.z0, 0, off(qi, qj));
+                    if (!first) rz0[s] = ldo(Pp->z0, 0, off(qi, qj));
                 } else {
                     ro[s] = -2 - ro[s];   // outside: store zeros
                 }
@@ -689,13 +704,13 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             if (cin[k]) {
                 if (rho_int(i, j)) {
                     zs = zs_of(o, pmnc[k], z0c_[k], r);
-                    sto<NTS>(P.rz1, cf[k], r);
+                    sto<NTS>(Pp->rz1, cf[k], r);
                 } else {
                     const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
                     const uint32_t fq = off(qi, qj);
                     const int oq = o + (qj - j) * FW + (qi - i);
                     const T pmn = sPM[oq] * sPN[oq];
-                    const T z0 = first ? T(0) : ldo(P.z0, 0, fq);
+                    const T z0 = first ? T(0) : ldo(Pp->z0, 0, fq);
                     zs = zs_of(oq, pmn, z0, r);
                 }
             }
@@ -753,14 +768,14 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     T r;
                     double F = 0.0;
                     us = pred_u(o, x, r, offd(i, j), &F);
-                    sto<NTS>(P.ru1, cf[k], r);
+                    sto<NTS>(Pp->ru1, cf[k], r);
                     ruc[k] = r;
                     if (j == 1) sRUlo[a - 3] = r;
                     if (j == M - 1) sRUhi[a - 3] = r;
                     if constexpr (CPL) {
                         fuc[k] = F;
                         if constexpr (FIRST) {
-                            sto<NTS>(P.c.Fu, offd(i, j), F);
+                            sto<NTS>(Pp->c.Fu, offd(i, j), F);
                             if (j == 1) sFUlo[a - 3] = F;
                             if (j == M - 1) sFUhi[a - 3] = F;
                         }
@@ -797,14 +812,14 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     T r;
                     double F = 0.0;
                     vs = pred_v(o, x, r, offd(i, j), &F);
-                    sto<NTS>(P.rv1, cf[k], r);
+                    sto<NTS>(Pp->rv1, cf[k], r);
                     rvc[k] = r;
                     if (i == 1) sRVlo[b - 3] = r;
                     if (i == L - 1) sRVhi[b - 3] = r;
                     if constexpr (CPL) {
                         fvc[k] = F;
                         if constexpr (FIRST) {
-                            sto<NTS>(P.c.Fv, offd(i, j), F);
+                            sto<NTS>(Pp->c.Fv, offd(i, j), F);
                             if (i == 1) sFVlo[b - 3] = F;
                             if (i == L - 1) sFVhi[b - 3] = F;
                         }
@@ -829,21 +844,21 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const uint32_t f = cf[k];
             if (cin[k] && rho_int(i, j)) {
                 const uint32_t fd = offd(i, j);
-                if (!first) rz0_[k] = ldo(P.rz0, 0, f);
-                zt_[k] = ldavg<NTS>(P.Zt, fd);
-                dua_[k] = ldavg<NTS>(P.DUa, fd);
-                dva_[k] = ldavg<NTS>(P.DVa, fd);
+                if (!first) rz0_[k] = ldo(Pp->rz0, 0, f);
+                zt_[k] = ldavg<NTS>(Pp->Zt, fd);
+                dua_[k] = ldavg<NTS>(Pp->DUa, fd);
+                dva_[k] = ldavg<NTS>(Pp->DVa, fd);
                 if constexpr (CPL) {
-                    du2_[k] = FIRST ? 0.0 : ldavg<NTS>(P.c.DU_avg2, fd);
-                    dv2_[k] = FIRST ? 0.0 : ldavg<NTS>(P.c.DV_avg2, fd);
+                    du2_[k] = FIRST ? 0.0 : ldavg<NTS>(Pp->c.DU_avg2, fd);
+                    dv2_[k] = FIRST ? 0.0 : ldavg<NTS>(Pp->c.DV_avg2, fd);
                 }
             }
             if (cin[k] && u_int(i, j)) {
-                if (!first) ru0_[k] = ldo(P.ru0, 0, f);
+                if (!first) ru0_[k] = ldo(Pp->ru0, 0, f);
                 z1w_[k] = sZ1[o - 1];   // the Z1 plane holds z1 until C1
             }
             if (cin[k] && v_int(i, j)) {
-                if (!first) rv0_[k] = ldo(P.rv0, 0, f);
+                if (!first) rv0_[k] = ldo(Pp->rv0, 0, f);
                 z1s_[k] = sZ1[o - FW];
             }
         }
@@ -897,9 +912,9 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     ri[s] = i;
                     rj[s] = j;
                     const uint32_t f = off(i, j);
-                    rh[s] = ldo(P.h, 0, f);
-                    if (i >= 1) rhw[s] = ldo(P.h, -1, f);
-                    if (j >= 1) rhs[s] = ldo(P.h, -W, f);
+                    rh[s] = ldo(Pp->h, 0, f);
+                    if (i >= 1) rhw[s] = ldo(Pp->h, -1, f);
+                    if (j >= 1) rhs[s] = ldo(Pp->h, -W, f);
                 }
             }
         }
@@ -963,8 +978,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
                     rq[s] = ro[s] + (qj - j) * FW + (qi - i);
                     const uint32_t fq = off(qi, qj);
-                    rz1[s] = ldo(P.z1, 0, fq);
-                    if (!first) rrz0[s] = ldo(P.rz0, 0, fq);
+                    rz1[s] = ldo(Pp->z1, 0, fq);
+                    if (!first) rrz0[s] = ldo(Pp->rz0, 0, fq);
                 } else {
                     ro[s] = -2 - ro[s];
                 }
@@ -979,24 +994,24 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             if (cin[k]) {
                 if (rho_int(i, j)) {
                     z2 = z2_of(o, pmnc[k], z1c[k], rz0_[k]);
-                    const double w = P.w1;
+                    const double w = Pp->w1;
                     const uint32_t fd = offd(i, j);
-                    sto<NTS>(P.Zt, fd, zt_[k] + w * (double)zsc[k]);
-                    sto<NTS>(P.DUa, fd, dua_[k] + w * (double)duc[k]);
-                    sto<NTS>(P.DVa, fd, dva_[k] + w * (double)dvc[k]);
+                    sto<NTS>(Pp->Zt, fd, zt_[k] + w * (double)zsc[k]);
+                    sto<NTS>(Pp->DUa, fd, dua_[k] + w * (double)duc[k]);
+                    sto<NTS>(Pp->DVa, fd, dva_[k] + w * (double)dvc[k]);
                     if constexpr (CPL) {
-                        sto<NTS>(P.c.DU_avg2, fd, (du2_[k] + P.c.w2p * du1c[k]) + P.c.w2c * (double)duc[k]);
-                        sto<NTS>(P.c.DV_avg2, fd, (dv2_[k] + P.c.w2p * dv1c[k]) + P.c.w2c * (double)dvc[k]);
+                        sto<NTS>(Pp->c.DU_avg2, fd, (du2_[k] + Pp->c.w2p * du1c[k]) + Pp->c.w2c * (double)duc[k]);
+                        sto<NTS>(Pp->c.DV_avg2, fd, (dv2_[k] + Pp->c.w2p * dv1c[k]) + Pp->c.w2c * (double)dvc[k]);
                     }
                 } else {
                     const int qi = clampi(i, 1, L - 1), qj = clampi(j, 1, M - 1);
                     const uint32_t fq = off(qi, qj);
                     const int oq = o + (qj - j) * FW + (qi - i);
                     const T pmn = sPM[oq] * sPN[oq];
-                    const T rz0 = first ? T(0) : ldo(P.rz0, 0, fq);
-                    z2 = z2_of(oq, pmn, ldo(P.z1, 0, fq), rz0);
+                    const T rz0 = first ? T(0) : ldo(Pp->rz0, 0, fq);
+                    z2 = z2_of(oq, pmn, ldo(Pp->z1, 0, fq), rz0);
                 }
-                sto<NTS>(P.z2, cf[k], z2);
+                sto<NTS>(Pp->z2, cf[k], z2);
             }
             // Z2 reuses the Z1 plane: every reader of Z1 finished before the last barrier
             sZ2[o] = z2;
@@ -1022,7 +1037,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     auto corr_u = [&](int q, T onu, T omu, T cff, T hb, T hc, T fc, T fb, T st, T ru1, T ru0, T z1c_, T z1b,
                       T u1c_, double F = 0.0) {
         const T g0 = gzc(q - 1), g1 = gzc(q);
-        T r = ((T(0.5) * P.g) * onu) * ((hb + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
+        T r = ((T(0.5) * Pp->g) * onu) * ((hb + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
         const T uc = sUS[q];
         const T UFx1 = T(0.25) * (sDU[q] + sDU[q + 1]) * (uc + sUS[q + 1]);
         const T UFx0 = T(0.25) * (sDU[q - 1] + sDU[q]) * (sUS[q - 1] + uc);
@@ -1032,7 +1047,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         const T cr1 = ((T(0.5) * (hc + sZS[q])) * fc) * (sVS[q] + sVS[q + FW]);
         const T cr0 = ((T(0.5) * (hb + sZS[q - 1])) * fb) * (sVS[q - 1] + sVS[q - 1 + FW]);
         r = r + T(0.5) * (cr1 + cr0);
-        r = r - ((P.rdrg * uc) * omu) * onu;
+        r = r - ((Pp->rdrg * uc) * omu) * onu;
         r = r + (st * omu) * onu;
         if constexpr (CPL) r = r + T(F);
         const T fac = sf::recip((hc + sZ2[q]) + (hb + sZ2[q - 1]));
@@ -1044,7 +1059,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     auto corr_v = [&](int q, T omv, T onv, T cff, T hb, T hc, T fc, T fb, T st, T rv1, T rv0, T z1c_, T z1b,
                       T v1c_, double F = 0.0) {
         const T g0 = gzc(q - FW), g1 = gzc(q);
-        T r = ((T(0.5) * P.g) * omv) * ((hb + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
+        T r = ((T(0.5) * Pp->g) * omv) * ((hb + hc) * (g0 - g1) + (g0 * g0 - g1 * g1));
         const T vc = sVS[q];
         const T VFx1 = T(0.25) * (sDU[q + 1] + sDU[q + 1 - FW]) * (sVS[q + 1] + vc);
         const T VFx0 = T(0.25) * (sDU[q] + sDU[q - FW]) * (vc + sVS[q - 1]);
@@ -1054,7 +1069,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         const T cr1 = ((T(0.5) * (hc + sZS[q])) * fc) * (sUS[q] + sUS[q + 1]);
         const T cr0 = ((T(0.5) * (hb + sZS[q - FW])) * fb) * (sUS[q - FW] + sUS[q - FW + 1]);
         r = r - T(0.5) * (cr1 + cr0);
-        r = r - ((P.rdrg * vc) * omv) * onv;
+        r = r - ((Pp->rdrg * vc) * omv) * onv;
         r = r + (st * omv) * onv;
         if constexpr (CPL) r = r + T(F);
         const T fac = sf::recip((hc + sZ2[q]) + (hb + sZ2[q - FW]));
@@ -1077,15 +1092,15 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const int qj = clampi(j, 1, M - 1);
             const uint32_t f = off(i, qj);
             const T ru1 = qj == 1 ? sRUlo[a - 3] : sRUhi[a - 3];
-            const T ru0 = first ? T(0) : ldo(P.ru0, 0, f);
+            const T ru0 = first ? T(0) : ldo(Pp->ru0, 0, f);
             double F = 0.0;
-            if constexpr (CPL) F = FIRST ? (qj == 1 ? sFUlo[a - 3] : sFUhi[a - 3]) : ldo(P.c.Fu, 0, offd(i, qj));
-            const T st = CLO ? clS[(qj == 1 ? 0 : TI) + a - 3] : ldo(P.sustr, 0, f);
-            u2 = corr_u(o + (qj - j) * FW, onu_at(f), omu_at(f), cffu_at(f), ldo(P.h, -1, f), ldo(P.h, 0, f),
-                        ldo(P.fomn, 0, f), ldo(P.fomn, -1, f), st, ru1, ru0, ldo(P.z1, 0, f), ldo(P.z1, -1, f),
+            if constexpr (CPL) F = FIRST ? (qj == 1 ? sFUlo[a - 3] : sFUhi[a - 3]) : ldo(Pp->c.Fu, 0, offd(i, qj));
+            const T st = CLO ? clS[(qj == 1 ? 0 : TI) + a - 3] : ldo(Pp->sustr, 0, f);
+            u2 = corr_u(o + (qj - j) * FW, onu_at(f), omu_at(f), cffu_at(f), ldo(Pp->h, -1, f), ldo(Pp->h, 0, f),
+                        ldo(Pp->fomn, 0, f), ldo(Pp->fomn, -1, f), st, ru1, ru0, ldo(Pp->z1, 0, f), ldo(Pp->z1, -1, f),
                         sU1[o + (qj - j) * FW], F);
         }
-        sto<NTS>(P.u2, cf[k], u2);
+        sto<NTS>(Pp->u2, cf[k], u2);
         if (v_int(i, j)) {
             v2 = corr_v(o, omvc[k], onvc[k], cffvc[k], hs_[k], hc_[k], fo_[k], fos_[k], CLO ? csv[k] : sv_[k], rvc[k],
                         rv0_[k], z1c[k], z1s_[k], v1c[k], CPL ? fvc[k] : 0.0);
@@ -1093,15 +1108,15 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             const int qi = clampi(i, 1, L - 1);
             const uint32_t f = off(qi, j);
             const T rv1 = qi == 1 ? sRVlo[b - 3] : sRVhi[b - 3];
-            const T rv0 = first ? T(0) : ldo(P.rv0, 0, f);
+            const T rv0 = first ? T(0) : ldo(Pp->rv0, 0, f);
             double F = 0.0;
-            if constexpr (CPL) F = FIRST ? (qi == 1 ? sFVlo[b - 3] : sFVhi[b - 3]) : ldo(P.c.Fv, 0, offd(qi, j));
-            const T st = CLO ? clS[2 * TI + (qi == 1 ? 0 : TJ) + b - 3] : ldo(P.svstr, 0, f);
-            v2 = corr_v(o + (qi - i), omv_at(f), onv_at(f), cffv_at(f), ldo(P.h, -W, f), ldo(P.h, 0, f),
-                        ldo(P.fomn, 0, f), ldo(P.fomn, -W, f), st, rv1, rv0, ldo(P.z1, 0, f), ldo(P.z1, -W, f),
+            if constexpr (CPL) F = FIRST ? (qi == 1 ? sFVlo[b - 3] : sFVhi[b - 3]) : ldo(Pp->c.Fv, 0, offd(qi, j));
+            const T st = CLO ? clS[2 * TI + (qi == 1 ? 0 : TJ) + b - 3] : ldo(Pp->svstr, 0, f);
+            v2 = corr_v(o + (qi - i), omv_at(f), onv_at(f), cffv_at(f), ldo(Pp->h, -W, f), ldo(Pp->h, 0, f),
+                        ldo(Pp->fomn, 0, f), ldo(Pp->fomn, -W, f), st, rv1, rv0, ldo(Pp->z1, 0, f), ldo(Pp->z1, -W, f),
                         sV1[o + (qi - i)], F);
         }
-        sto<NTS>(P.v2, cf[k], v2);
+        sto<NTS>(Pp->v2, cf[k], v2);
     }
 }
 
