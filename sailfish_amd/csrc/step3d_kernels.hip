@@ -1075,18 +1075,44 @@ __global__ __launch_bounds__(256) void k_step3d_uv_coupled(G3 g, UVC<T> a, doubl
 // eliminated level by level with the operands of the next level loaded ahead, the
 // solution stays in registers for the column sums, and the new thicknesses of the
 // mass-flux correction are kept in the slots of the eliminated diagonal.
+//
+// Addressing: every access is an array's scalar base plus a 32-bit byte offset held in a
+// VGPR (the launcher checks that (N + 1) levels fit in 2^32 bytes).  The offset of each
+// level is the previous one plus the level stride, passed through an empty asm statement
+// so that the compiler does not fold the unrolled levels into per-level 64-bit scalar
+// bases (dozens of arrays x levels, which it spilled to VGPR lanes and restored at every
+// use).
+template <class T>
+__device__ __forceinline__ T ldb(const T *p, uint32_t o)
+{
+    return *(const __attribute__((address_space(1))) T *)((const __attribute__((address_space(1))) char *)p + o);
+}
+template <class T>
+__device__ __forceinline__ void stb(T *p, uint32_t o, T v)
+{
+    *(__attribute__((address_space(1))) T *)((__attribute__((address_space(1))) char *)p + o) = v;
+}
+__device__ __forceinline__ uint32_t next_level(uint32_t o, uint32_t stride)
+{
+    o += stride;
+    asm("" : "+v"(o));
+    return o;
+}
+
 template <class T, int NMAX>
 __global__ __launch_bounds__(256) void k_step3d_uv_coupled_reg(G3 g, UVC<T> a, double lambda, double Dt)
 {
     COLUMN_INDEX
-    const long long S2 = g.S2;
     const int N = g.N;
+    const uint32_t SB = (uint32_t)(g.S2 * (long long)sizeof(T));   // level stride in bytes
+    const uint32_t o0 = (uint32_t)(f * (long long)sizeof(T));      // level 0 of the column
     __shared__ T sds[NMAX][256];   // the sub-diagonal of each column (thread)
     T D[NMAX], R[NMAX];
     const int tid = threadIdx.x;
     const T mldt = T((-lambda) * Dt);
     auto column = [&](bool vdir) {
         const long long d = vdir ? g.szI : 1;
+        const uint32_t dB = (uint32_t)(d * (long long)sizeof(T));
         T *x = vdir ? a.v : a.u, *keep = vdir ? a.rv_keep : a.ru_keep, *Hf = vdir ? a.Hvom : a.Huon;
         const T *r = vdir ? a.rv : a.ru, *r1 = vdir ? a.rv1 : a.ru1, *r2 = vdir ? a.rv2 : a.ru2;
         const double stress = vdir ? a.svstr[f] : a.sustr[f];
@@ -1097,35 +1123,36 @@ __global__ __launch_bounds__(256) void k_step3d_uv_coupled_reg(G3 g, UVC<T> a, d
         struct Lv {
             T hn0, hn1, ho0, ho1, xo, rk, rk1, rk2, z0, z1, k0, k1;
         };
-        auto load = [&](int K) {
+        // operands of the level at byte offset ob (z_r and Akv of the level above)
+        auto load = [&](uint32_t ob, bool top) {
             Lv q{};
-            const long long b = K * S2 + f;
-            q.hn0 = a.Hz[b];
-            q.hn1 = a.Hz[b - d];
-            q.ho0 = a.Hz_old[b];
-            q.ho1 = a.Hz_old[b - d];
-            q.xo = x[b];
-            q.rk = r[b];
-            if (ab > 0) q.rk1 = r1[b];
-            if (ab > 1) q.rk2 = r2[b];
-            if (K < N - 1) {
-                q.z0 = a.z_r[b + S2];
-                q.z1 = a.z_r[b + S2 - d];
-                q.k0 = a.Akv[b + S2];
-                q.k1 = a.Akv[b + S2 - d];
+            q.hn0 = ldb(a.Hz, ob);
+            q.hn1 = ldb(a.Hz, ob - dB);
+            q.ho0 = ldb(a.Hz_old, ob);
+            q.ho1 = ldb(a.Hz_old, ob - dB);
+            q.xo = ldb((const T *)x, ob);
+            q.rk = ldb(r, ob);
+            if (ab > 0) q.rk1 = ldb(r1, ob);
+            if (ab > 1) q.rk2 = ldb(r2, ob);
+            if (!top) {
+                q.z0 = ldb(a.z_r, ob + SB);
+                q.z1 = ldb(a.z_r, ob + SB - dB);
+                q.k0 = ldb(a.Akv, ob + SB);
+                q.k1 = ldb(a.Akv, ob + SB - dB);
             }
             return q;
         };
-        T zf0 = a.z_r[f], zd0 = a.z_r[f - d];
-        const T xbot = x[f];
+        T zf0 = ldb(a.z_r, o0), zd0 = ldb(a.z_r, o0 - dB);
+        const T xbot = ldb((const T *)x, o0);
         T sdp = T(0);   // sub-diagonal of the level below
-        Lv cur = load(0);
+        uint32_t ob = o0;
+        Lv cur = load(ob, N - 1 == 0);
 #pragma unroll
         for (int K = 0; K < NMAX; K++) {
             if (K < N) {
+                const uint32_t on = next_level(ob, SB);
                 Lv nxt{};
-                if (K + 1 < N) nxt = load(K + 1);
-                const long long b = K * S2 + f;
+                if (K + 1 < N) nxt = load(on, K + 1 >= N - 1);
                 T sd = T(0);
                 if (K < N - 1) {
                     const T dz = ((cur.z0 - zf0) + (cur.z1 - zd0)) * T(0.5);
@@ -1138,7 +1165,7 @@ __global__ __launch_bounds__(256) void k_step3d_uv_coupled_reg(G3 g, UVC<T> a, d
                 if (ab == 0) rab = cur.rk;
                 else if (ab == 1) rab = T(1.5) * cur.rk - T(0.5) * cur.rk1;
                 else rab = (T(23.0 / 12.0) * cur.rk - T(16.0 / 12.0) * cur.rk1) + T(5.0 / 12.0) * cur.rk2;
-                keep[b] = cur.rk;
+                stb(keep, ob, cur.rk);
                 T rhs = ((cur.ho0 + cur.ho1) * T(0.5)) * cur.xo + cdt_mn * rab;
                 const T hn = (cur.hn0 + cur.hn1) * T(0.5);
                 if (K == 0) {
@@ -1154,6 +1181,7 @@ __global__ __launch_bounds__(256) void k_step3d_uv_coupled_reg(G3 g, UVC<T> a, d
                 }
                 sdp = sd;
                 cur = nxt;
+                ob = on;
             }
         }
         // back substitution: R(K) := x(K)
@@ -1164,15 +1192,16 @@ __global__ __launch_bounds__(256) void k_step3d_uv_coupled_reg(G3 g, UVC<T> a, d
         // couple_column: replace the column mean, correct the mass flux
         const double met = vdir ? 2.0 / (g.pm[f - g.szI] + g.pm[f]) : 2.0 / (g.pn[f - 1] + g.pn[f]);
         double Ds = 0.0, CF = 0.0;
+        ob = o0;
 #pragma unroll
         for (int K = 0; K < NMAX; K++) {
             if (K < N) {
-                const long long b = K * S2 + f;
-                const T hn = (a.Hz[b] + a.Hz[b - d]) * T(0.5);
+                const T hn = (ldb(a.Hz, ob) + ldb(a.Hz, ob - dB)) * T(0.5);
                 D[K] = hn;
                 const double dc = met * (double)hn;
                 Ds = Ds + dc;
                 CF = CF + dc * (double)R[K];
+                ob = next_level(ob, SB);
             }
         }
         const double rD = 1.0 / Ds;
@@ -1182,21 +1211,26 @@ __global__ __launch_bounds__(256) void k_step3d_uv_coupled_reg(G3 g, UVC<T> a, d
         (vdir ? a.vbar_t2 : a.ubar_t2)[f] = xb;
         (vdir ? a.vbar_t1 : a.ubar_t1)[f] = xb;
         double FC = 0.0;
+        ob = o0;
 #pragma unroll
         for (int K = 0; K < NMAX; K++) {
             if (K < N) {
-                const long long b = K * S2 + f;
                 const T xk = T((double)R[K] - CF);
-                x[b] = xk;
-                const T hf = T(0.5 * ((double)Hf[b] + (double)xk * (met * (double)D[K])));
+                stb(x, ob, xk);
+                const T hf = T(0.5 * ((double)ldb((const T *)Hf, ob) + (double)xk * (met * (double)D[K])));
                 R[K] = hf;
                 FC = FC + (double)hf;
+                ob = next_level(ob, SB);
             }
         }
         FC = rD * (FC - Davg2[f]);
+        ob = o0;
 #pragma unroll
         for (int K = 0; K < NMAX; K++) {
-            if (K < N) Hf[K * S2 + f] = T((double)R[K] - (met * (double)D[K]) * FC);
+            if (K < N) {
+                stb(Hf, ob, T((double)R[K] - (met * (double)D[K]) * FC));
+                ob = next_level(ob, SB);
+            }
         }
     };
     if (coupled_point(g, false, f)) column(false);
@@ -1485,10 +1519,12 @@ int step3d_uv_coupled_launch(sf_ctx *c, const sf_uv_coupled *p, double lambda, d
              (T *)p->vbar_t1, (T *)p->ubar_t2, (T *)p->vbar_t2, (const T *)p->Hz_old, (const T *)p->Hz,
              (const T *)p->Akv, (const T *)p->z_r, p->DU_avg1, p->DV_avg1, p->DU_avg2, p->DV_avg2, (T *)p->Huon,
              (T *)p->Hvom, p->sustr, p->svstr, p->rdrg, p->ab, (T *)p->SD, (T *)p->D, (T *)p->RHS};
-    if (c->szK <= 16) {
+    // register columns address every level with 32-bit byte offsets
+    const bool fits = (unsigned long long)(c->szK + 1) * (unsigned long long)c->S2 * sizeof(T) < (1ull << 32);
+    if (fits && c->szK <= 16) {
         LAUNCH_COLUMNS((k_step3d_uv_coupled_reg<T, 16>), c, stream, grid3(c), a, lambda, Dt);
     }
-    if (c->szK <= 32) {
+    if (fits && c->szK <= 32) {
         LAUNCH_COLUMNS((k_step3d_uv_coupled_reg<T, 32>), c, stream, grid3(c), a, lambda, Dt);
     }
     LAUNCH_COLUMNS(k_step3d_uv_coupled<T>, c, stream, grid3(c), a, lambda, Dt);
