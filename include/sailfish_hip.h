@@ -236,7 +236,10 @@ enum {
  * nonlinear/zetabc.py:5-12 and nonlinear/barotropicVelocityBC.py:7-31 (same encoding
  * as sf_bc_apply_map; host arrays).  Rejects (SF_ERR_UNSUPPORTED) maps the fused
  * kernel cannot apply inside a tile: zeta sources must be neighbours in the same
- * tile, velocity maps must be constants (closed/clamped walls). */
+ * tile; velocity maps hold constants (closed/clamped walls) or copies (gradient walls)
+ * from a neighbour that the map leaves unchanged and that lies in the same tile region
+ * as the destination.  (The reference's vbar gradient copy, which reads ubar's source
+ * list at barotropicVelocityBC.py:31, is rejected: its sources are not neighbours.) */
 int sf_ctx_set_bc_maps(sf_ctx *ctx, const int32_t *zeta_dst, const int32_t *zeta_src, int32_t nz,
                        const int32_t *ubar_dst, const int32_t *ubar_src, int32_t nu,
                        const int32_t *vbar_dst, const int32_t *vbar_src, int32_t nv);

@@ -21,6 +21,7 @@ struct sf_ctx {
     // allocated by sf_ctx_set_bc_maps; see step2d_fused.hip
     int32_t *bc_zeta, *bc_ubar, *bc_vbar;
     int bc_ready;
+    int bc_vel_copy;          // a velocity map copies from a neighbour (gradient boundary)
 };
 
 namespace sf {

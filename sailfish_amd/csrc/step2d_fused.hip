@@ -39,6 +39,9 @@
 #include "sf_internal.h"
 #include "step2d_fused.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace {
 
 constexpr int TI = SF_FUSED_TI;
@@ -111,7 +114,10 @@ __device__ __forceinline__ double ldavg(const double *p)
     return *p;
 }
 
-template <typename T, bool NTS>
+// VC: a velocity boundary map copies from neighbours (gradient boundaries).  Stage E
+// then keeps the predicted velocities before the map in two more LDS planes, and a
+// destination takes its source's value from them after a barrier.
+template <typename T, bool NTS, bool VC>
 __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
 {
     // six LDS planes on the R2 frame, reused as their contents die:
@@ -122,6 +128,7 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
     //   P5  C-D: predicted zeta before BC   F-G: corrected zeta before BC
     //   P6  D-E: gz = (1-w)*predicted zeta + w*zeta_t0
     __shared__ T P1[N2], P2[N2], P3[N2], P4[N2], P5[N2], P6[N2];
+    __shared__ T P7[VC ? N2 : 1], P8[VC ? N2 : 1];   // VC: ubar, vbar of E before the map
     T *const sD1 = P1, *const sD2 = P1, *const sH = P2;
     T *const sFX = P3, *const sPX = P3, *const sFY = P4, *const sPY = P4;
     T *const sZq = P5, *const sZc = P5, *const sGZ = P6;
@@ -281,6 +288,34 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
     const T a = T(1) - w;
     const T gc = T(10.0);
     const double cff1 = P.cff1;
+    // the boundary map at (i, j), the corrector's face fluxes of the predicted state and
+    // the time averages; VC copies read the pre-map values of E in P7 / P8
+    auto e_tail = [&](int k, int r, int i, int j, long long f, bool inT, T u2, T v2) {
+        if (band(i, j)) {
+            const int cu = F.bc_ubar[f], cv = F.bc_vbar[f];
+            if (cu == kZero) u2 = T(0);
+            else if (cu == kClamp) u2 = T(P.clamp);
+            else if (VC && cu >= 0) u2 = P7[lidx(cu % szI - gi0, cu / szI - gj0)];
+            if (cv == kZero) v2 = T(0);
+            else if (cv == kClamp) v2 = T(P.clamp);
+            else if (VC && cv >= 0) v2 = P8[lidx(cv % szI - gi0, cv / szI - gj0)];
+        }
+        up[k] = u2;
+        vp[k] = v2;
+        // corrector fluxes of the predicted state: DU = ubar*RtoU(D), DV = vbar*RtoV(D)
+        const T D = sD2[r];
+        const T Dl = sD2[r - 1];
+        const T Dd = sD2[r - W2];
+        const T DU = u2 * ((D + Dl) * T(0.5));
+        const T DV = v2 * ((D + Dd) * T(0.5));
+        if (inT && interior(i, j)) {
+            stv<NTS>(F.Zt_avg1 + f, zt[k] + cff1 * (double)zp[k]);
+            stv<NTS>(F.DU_avg1 + f, du[k] + (cff1 * (double)DU) * (double)onu[k]);
+            stv<NTS>(F.DV_avg1 + f, dv[k] + (cff1 * (double)DV) * (double)omv[k]);
+        }
+        sPX[r] = DU * onu[k];
+        sPY[r] = DV * omv[k];
+    };
 #pragma unroll
     for (int k = 0; k < KP; k++) {
         const int r = tid + k * NT;
@@ -289,7 +324,6 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
         up[k] = vp[k] = T(0);
         if (li < 2 || li > TI + 2 || lj < 2 || lj > TJ + 2) continue;
         const int i = gi0 + li, j = gj0 + lj;
-        T px = T(0), py = T(0);
         if (inside(i, j)) {
             const long long f = (long long)j * szI + i;
             const bool inT = li <= TI + 1 && lj <= TJ + 1;
@@ -323,31 +357,31 @@ __global__ __launch_bounds__(NT) void k_step2d_fused(Fields<T> F, Params P)
                     stv<NTS>(F.rubar_t1 + f, (gc * onu[k]) * (hu * (gz - gzl) + T(0.5) * (gz * gz - gzl * gzl)));
                 }
             }
-            if (band(i, j)) {
-                const int cu = F.bc_ubar[f], cv = F.bc_vbar[f];
-                if (cu == kZero) u2 = T(0);
-                else if (cu == kClamp) u2 = T(P.clamp);
-                if (cv == kZero) v2 = T(0);
-                else if (cv == kClamp) v2 = T(P.clamp);
+            if constexpr (VC) {
+                up[k] = u2;   // the map is applied after the barrier below
+                vp[k] = v2;
+                P7[r] = u2;
+                P8[r] = v2;
+            } else {
+                e_tail(k, r, i, j, f, inT, u2, v2);
             }
-            up[k] = u2;
-            vp[k] = v2;
-            // corrector fluxes of the predicted state: DU = ubar*RtoU(D), DV = vbar*RtoV(D)
-            const T D = sD2[r];
-            const T Dl = sD2[r - 1];
-            const T Dd = sD2[r - W2];
-            const T DU = u2 * ((D + Dl) * T(0.5));
-            const T DV = v2 * ((D + Dd) * T(0.5));
-            px = DU * onu[k];
-            py = DV * omv[k];
-            if (inT && interior(i, j)) {
-                stv<NTS>(F.Zt_avg1 + f, zt[k] + cff1 * (double)zp[k]);
-                stv<NTS>(F.DU_avg1 + f, du[k] + (cff1 * (double)DU) * (double)onu[k]);
-                stv<NTS>(F.DV_avg1 + f, dv[k] + (cff1 * (double)DV) * (double)omv[k]);
-            }
+        } else {
+            sPX[r] = T(0);
+            sPY[r] = T(0);
         }
-        sPX[r] = px;
-        sPY[r] = py;
+    }
+    if constexpr (VC) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KP; k++) {
+            const int r = tid + k * NT;
+            if (r >= N2) continue;
+            const int li = r % W2, lj = r / W2;
+            if (li < 2 || li > TI + 2 || lj < 2 || lj > TJ + 2) continue;
+            const int i = gi0 + li, j = gj0 + lj;
+            if (!inside(i, j)) continue;
+            e_tail(k, r, i, j, (long long)j * szI + i, li <= TI + 1 && lj <= TJ + 1, up[k], vp[k]);
+        }
     }
     __syncthreads();
 
@@ -418,11 +452,16 @@ __global__ __launch_bounds__(256) void k_step2d_masked(Fields<T> F, Params P)
     const bool band = i <= 1 || i >= szI - 2 || j <= 1 || j >= szJ - 2;
     T up = F.ubar_t2[f], vp = F.vbar_t2[f];
     if (band) {
+        // a copy's source is no destination of its map, so its stored value is its
+        // value before the map (masked sources keep theirs; vbar sources are only zeroed
+        // under zero_vbar, which zeroes this point too)
         const int cu = F.bc_ubar[f], cv = F.bc_vbar[f];
         if (cu == kZero) up = T(0);
         else if (cu == kClamp) up = T(P.clamp);
+        else if (cu >= 0) up = F.ubar_t2[cu];
         if (cv == kZero) vp = T(0);
         else if (cv == kClamp) vp = T(P.clamp);
+        else if (cv >= 0) vp = F.vbar_t2[cv];
     }
     F.ubar_t2[f] = up;
     F.vbar_t2[f] = P.zero_vbar ? T(0) : vp;
@@ -483,10 +522,13 @@ int launch(sf_ctx *c, const sf_step2d_fields *a, double dtfast, double cff1, dou
     if (parts & 1) {
         // 11 bulk fields read and 5 written, 3 FP64 averages read and written
         const bool nts = (double)c->S2 * (16.0 * sizeof(T) + 6.0 * 8) > sf::kStreamingBytes;
-        if (nts) {
-            hipLaunchKernelGGL((k_step2d_fused<T, true>), dim3(P.tiles_i * P.tiles_j), dim3(NT), 0, s, F, P);
+        const dim3 grid(P.tiles_i * P.tiles_j);
+        if (c->bc_vel_copy) {
+            if (nts) hipLaunchKernelGGL((k_step2d_fused<T, true, true>), grid, dim3(NT), 0, s, F, P);
+            else hipLaunchKernelGGL((k_step2d_fused<T, false, true>), grid, dim3(NT), 0, s, F, P);
         } else {
-            hipLaunchKernelGGL((k_step2d_fused<T, false>), dim3(P.tiles_i * P.tiles_j), dim3(NT), 0, s, F, P);
+            if (nts) hipLaunchKernelGGL((k_step2d_fused<T, true, false>), grid, dim3(NT), 0, s, F, P);
+            else hipLaunchKernelGGL((k_step2d_fused<T, false, false>), grid, dim3(NT), 0, s, F, P);
         }
         int rc = sf::check_launch("sf_step2d_substep (tiles)");
         if (rc != SF_OK) return rc;
@@ -561,17 +603,45 @@ extern "C" int sf_step2d_check_maps(int32_t szI, int32_t szJ, int32_t q5_stride,
                 }
             }
     }
+    // Velocity maps: constants anywhere on the band; copies (gradient boundaries) from a
+    // neighbour that is no destination of the same map, and that lies in every stage-E
+    // region (tile + right column + top row: [x0, x0+TI] x [y0, y0+TJ]) that holds the
+    // destination, where the kernel reads its value before the map from LDS.
     const int32_t *dd[2] = {ud, vd}, *ss[2] = {us, vs};
     const int nn[2] = {nu, nv};
-    for (int q = 0; q < 2; q++)
+    const int nti = (szI + off_i + TI - 1) / TI, ntj = (szJ + off_j + TJ - 1) / TJ;
+    std::vector<char> isdst((size_t)S2);
+    for (int q = 0; q < 2; q++) {
+        std::fill(isdst.begin(), isdst.end(), 0);
+        for (int k = 0; k < nn[q]; k++)
+            if (dd[q][k] >= 0 && dd[q][k] < S2) isdst[dd[q][k]] = 1;
         for (int k = 0; k < nn[q]; k++) {
-            const int d = dd[q][k];
-            if (d < 0 || d >= S2 || !band(d % szI, d / szI) || ss[q][k] >= 0) {
-                sf::set_error("velocity map entry %d is not a boundary constant (the fused substep supports "
-                              "closed/clamped velocity boundaries only)", d);
+            const int d = dd[q][k], s = ss[q][k];
+            if (d < 0 || d >= S2 || !band(d % szI, d / szI)) {
+                sf::set_error("velocity map destination %d is not on the boundary band", d);
                 return SF_ERR_UNSUPPORTED;
             }
+            if (s < 0) continue;   // a constant
+            const int di = d % szI, dj = d / szI, si = s % szI, sj = s / szI;
+            if (s >= S2 || abs(si - di) > 1 || abs(sj - dj) > 1 || isdst[s]) {
+                sf::set_error("velocity map entry %d <- %d: the fused substep copies only from a neighbour "
+                              "that the map leaves unchanged", d, s);
+                return SF_ERR_UNSUPPORTED;
+            }
+            for (int tj = tjx(dj) - 1; tj <= tjx(dj); tj++)
+                for (int ti = tix(di) - 1; ti <= tix(di); ti++) {
+                    if (ti < 0 || tj < 0 || ti >= nti || tj >= ntj) continue;
+                    const int x0 = ti * TI - off_i, y0 = tj * TJ - off_j;
+                    const bool dIn = di >= x0 && di <= x0 + TI && dj >= y0 && dj <= y0 + TJ;
+                    const bool sIn = si >= x0 && si <= x0 + TI && sj >= y0 && sj <= y0 + TJ;
+                    if (dIn && !sIn) {
+                        sf::set_error("velocity map source %d of %d is outside the region of tile (%d,%d)", s, d,
+                                      ti, tj);
+                        return SF_ERR_UNSUPPORTED;
+                    }
+                }
         }
+    }
     if (q5_stride != szI && (szI % q5_stride) != 0) {
         sf::set_error("fused substep: the i%%szJ mask admits column-0 points on a %dx%d grid; "
                       "use the one-kernel-per-step path or SF_FLAG_FIX_Q5_MASK", szI, szJ);
@@ -593,6 +663,9 @@ extern "C" int sf_ctx_set_bc_maps(sf_ctx *c, const int32_t *zeta_dst, const int3
     int rc = sf_step2d_check_maps(c->szI, c->szJ, c->q5_stride, zeta_dst, zeta_src, nz, ubar_dst, ubar_src, nu,
                                   vbar_dst, vbar_src, nv);
     if (rc != SF_OK) return rc;
+    c->bc_vel_copy = 0;
+    for (int k = 0; k < nu; k++) c->bc_vel_copy |= ubar_src[k] >= 0;
+    for (int k = 0; k < nv; k++) c->bc_vel_copy |= vbar_src[k] >= 0;
     hipError_t e;
     int32_t **fields[3] = {&c->bc_zeta, &c->bc_ubar, &c->bc_vbar};
     for (auto p : fields)
@@ -615,19 +688,19 @@ extern "C" int sf_ctx_set_bc_maps(sf_ctx *c, const int32_t *zeta_dst, const int3
             e = hipMemcpy(tmp, ds[q], sizeof(int32_t) * n[q], hipMemcpyHostToDevice);
             if (e == hipSuccess) e = hipMemcpy(tmp + nmax, sr[q], sizeof(int32_t) * n[q], hipMemcpyHostToDevice);
             if (e != hipSuccess) {
-                hipFree(tmp);
+                (void)hipFree(tmp);
                 return sf::hip_fail(e, "sf_ctx_set_bc_maps: copy");
             }
             hipLaunchKernelGGL(k_scatter_codes, dim3(sf::grid1d(n[q])), dim3(256), 0, 0, dptr, tmp, tmp + nmax, n[q]);
             e = hipDeviceSynchronize();
             if (e != hipSuccess) {
-                hipFree(tmp);
+                (void)hipFree(tmp);
                 return sf::hip_fail(e, "sf_ctx_set_bc_maps: scatter");
             }
         }
     }
     e = hipDeviceSynchronize();
-    if (tmp) hipFree(tmp);
+    if (tmp) (void)hipFree(tmp);
     if (e != hipSuccess) return sf::hip_fail(e, "sf_ctx_set_bc_maps");
     c->bc_ready = 1;
     return SF_OK;

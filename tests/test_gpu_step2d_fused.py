@@ -63,12 +63,42 @@ def test_fused_clamped_velocity_boundary():
     assert not compare_fused(prod, orc, 70, 20, 70)
 
 
-def test_fused_rejects_gradient_velocity_boundary():
+# gradient (copy) velocity boundaries: west / east ubar copies, clamped and closed sides;
+# grids with a shifted tile origin (szI % 64 == 1), several tiles, and the reference's
+# i%szJ mask (masked interior columns finished by the masked-points kernel)
+GRAD_CASES = [
+    (70, 20, True, {"ubar": ["GRA", "CLO", "CLO", "CLO"], "zeta": ["GRA", "CLO", "GRA", "CLO"]}),
+    (129, 33, True, {"ubar": ["GRA", "CLO", "GRA", "CLO"]}),
+    (130, 17, True, {"ubar": ["CLA", "CLO", "GRA", "CLO"], "zeta": ["GRA", "CLO", "GRA", "CLO"]}),
+    (128, 64, False, {"ubar": ["GRA", "CLO", "GRA", "CLO"]}),
+    (96, 32, False, {"ubar": ["GRA", "CLO", "CLO", "CLO"]}),
+]
+
+
+@pytest.mark.parametrize("szI,szJ,fix_q5,lbc", GRAD_CASES)
+@pytest.mark.parametrize("mode", ["main2d", "main3d"])
+def test_fused_gradient_velocity_boundary(szI, szJ, fix_q5, lbc, mode):
+    print(f"[case] fused gradient ubar {mode} {szI}x{szJ} fix_q5={fix_q5} {lbc}", file=sys.stderr, flush=True)
+    from sailfish_amd.step2d_fused import check_maps
+    from sailfish_amd.model2d import build_model
+    cfg = configs.random_smooth(szI, szJ, seed=szI + 3 * szJ)
+    _, GRID, _, BOUNDARY = build_model(cfg, lbc=lbc, ndtfast=4, fix_q5=fix_q5)
+    ok, why = check_maps(szI, szJ, szI if fix_q5 else szJ, BOUNDARY)
+    assert ok, why
+    prod, orc = _run(cfg, 6 if mode == "main2d" else 2, mode, fix_q5, lbc=lbc)
+    bad = compare_fused(prod, orc, szI, szJ, szI if fix_q5 else szJ)
+    assert not bad, bad
+
+
+def test_fused_rejects_far_velocity_copy():
+    """The reference's vbar gradient copy reads ubar's source list
+    (barotropicVelocityBC.py:31): its sources are not neighbours, so the fused
+    substep rejects the maps (the caller keeps the one-kernel-per-step path)."""
     import torch
     from sailfish_amd import _lib
     from sailfish_amd.model2d import build_model
     from sailfish_amd.step2d_fused import FusedStepper
-    lbc = {"ubar": ["GRA", "CLO", "CLO", "CLO"]}
+    lbc = {"ubar": ["CLA", "CLO", "GRA", "CLO"], "vbar": ["GRA", "CLO", "CLO", "CLO"]}
     cfg = configs.random_smooth(40, 20, seed=3)
     ct, GRID, OCEAN, BOUNDARY = build_model(cfg, lbc=lbc, ndtfast=4, fix_q5=True)
     with pytest.raises(_lib.SailfishError):

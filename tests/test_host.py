@@ -112,7 +112,8 @@ FUSED_SIZES = [(130, 17), (64, 16), (67, 29), (128, 64), (96, 32), (200, 100), (
 
 
 @pytest.mark.parametrize("szI,szJ", FUSED_SIZES)
-@pytest.mark.parametrize("lbc", [None, {"ubar": ["CLA", "CLO", "CLO", "CLO"], "zeta": ["GRA", "CLO", "GRA", "CLO"]}])
+@pytest.mark.parametrize("lbc", [None, {"ubar": ["CLA", "CLO", "CLO", "CLO"], "zeta": ["GRA", "CLO", "GRA", "CLO"]},
+                                 {"ubar": ["GRA", "CLO", "GRA", "CLO"], "zeta": ["GRA", "CLO", "GRA", "CLO"]}])
 def test_fused_maps_accepted(szI, szJ, lbc):
     from sailfish_amd.mod_boundary import Boundary
     from sailfish_amd.step2d_fused import check_maps
@@ -122,11 +123,33 @@ def test_fused_maps_accepted(szI, szJ, lbc):
 
 
 def test_fused_maps_rejections():
+    from types import SimpleNamespace
     from sailfish_amd.mod_boundary import Boundary
     from sailfish_amd.step2d_fused import check_maps
-    b = Boundary(_G(40, 20), {"ubar": ["GRA", "CLO", "CLO", "CLO"]})
+    # the reference's vbar gradient copy reads ubar's source list: far sources
+    b = Boundary(_G(40, 20), {"ubar": ["CLA", "CLO", "GRA", "CLO"], "vbar": ["GRA", "CLO", "CLO", "CLO"]})
     ok, msg = check_maps(40, 20, 40, b)
     assert not ok and "velocity" in msg
+    # hand-made ubar copies on a 140 x 20 grid (64-wide tiles from i = 0): a neighbour
+    # copy inside one tile region is accepted; a source that is itself a destination, a
+    # source two points away, and a copy across a tile's left edge are not
+    z = (np.zeros(0, np.int32), np.zeros(0, np.int32))
+    v = z
+    f = lambda i, j: j * 140 + i
+
+    def chk(pairs):
+        d = np.array([p[0] for p in pairs], np.int32)
+        s_ = np.array([p[1] for p in pairs], np.int32)
+        return check_maps(140, 20, 140, SimpleNamespace(zeta_map=z, ubar_map2=(d, s_), vbar_map2=v))
+
+    assert chk([(f(139, 5), f(138, 5))])[0]
+    assert chk([(f(1, 5), f(2, 5)), (f(139, 5), f(138, 5))])[0]
+    ok, msg = chk([(f(1, 5), f(2, 5)), (f(2, 5), f(3, 5))])       # f(2,5) is both
+    assert not ok and "neighbour" in msg
+    ok, msg = chk([(f(1, 5), f(3, 5))])                            # two points away
+    assert not ok and "neighbour" in msg
+    ok, msg = chk([(f(64, 1), f(63, 1))])   # tile 1's region [64, 128] holds 64, not 63
+    assert not ok and "outside the region" in msg
     b = Boundary(_G(30, 20), None)
     ok, msg = check_maps(30, 20, 20, b)     # i%szJ mask with 30 % 20 != 0
     assert not ok and "column-0" in msg
