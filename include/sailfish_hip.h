@@ -393,7 +393,7 @@ typedef struct sf_uv_coupled {
 } sf_uv_coupled;
 
 /* The 3D momentum step of the coupled form (nonlinear/step3d_uv.py:155-180 and
- * :365-545, ROMS step3d_uv with "Couple 2D and 3D momentum equations"; replaces the live
+ * :365-544, ROMS step3d_uv with "Couple 2D and 3D momentum equations"; replaces the live
  * kernel's solve, tmpU mean correction and adjustBarotropicVelocity,
  * step3D_uv.kernels.cpp:408-707, and makes the vbar_t2 = 0 work-around of
  * nonlinear/main3d.py:189 unnecessary).  On the interior u points (i = 2..szI-2,

@@ -6,7 +6,7 @@ the physics-mode barotropic substep coupled to them (sfo_phys_substep_coupled).
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 
 One baroclinic step (nonlinear/main3d.py:87-325 with the coupling of
-obsolete/step2d_LF_AM3.h.py:778-846, :2377-2590 and nonlinear/step3d_uv.py:365-545):
+obsolete/step2d_LF_AM3.h.py:778-846, :2377-2590 and nonlinear/step3d_uv.py:365-544):
 set_maxflux, omega, set_zeta, rhs3d, rufrc/rvfrc, nfast+1 coupled physics substeps (the
 first forward/backward Euler, averages reset on it), set_depth, the coupled
 step3d_UV, setLateralUVBCs, omega.  No vbar = 0 after the substeps (the reference's

@@ -15,7 +15,7 @@ coupling of ROMS that the port keeps only as text and comments:
   grows by about Hz f dt per step once the coupling moves the 3D flow); it then
   replaces the vertical mean of u, v by the averaged barotropic transport and corrects
   the mass fluxes Huon/Hvom to DU_avg2/DV_avg2 (nonlinear/step3d_uv.py:155-180,
-  :365-545), so the vbar_t2 = 0 work-around of nonlinear/main3d.py:189 is not needed.
+  :365-544), so the vbar_t2 = 0 work-around of nonlinear/main3d.py:189 is not needed.
 
 One baroclinic step (``coupled_step``): set_maxflux, omega, set_zeta, rhs3d, rufrc,
 nfast+1 coupled substeps (the first forward/backward Euler, averages reset on it),

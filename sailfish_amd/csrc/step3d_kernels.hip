@@ -945,7 +945,7 @@ __global__ __launch_bounds__(256) void k_couple_rufrc(G3 g, const T *ru, const T
     rvfrc[f] = sv;
 }
 
-// step3d_UV coupled with the barotropic mode (nonlinear/step3d_uv.py:155-180, :365-545):
+// step3d_UV coupled with the barotropic mode (nonlinear/step3d_uv.py:155-180, :365-544):
 // the implicit vertical viscosity of k_step3d_uv (same matrix and solve), then per
 // interior u (v) column, with DC(K) = on_u * Hz_u(K) and D = sum DC(K) formed in FP64:
 //   CF = (sum DC u - DU_avg1) / D;  u -= CF;  ubar(t1) = ubar(t2) = DU_avg1 / D;
