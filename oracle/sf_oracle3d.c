@@ -508,7 +508,7 @@ void sfo_couple_rufrc(const sfo_grid3 *g, const double *ru, const double *rv, co
 }
 
 /* The 3D momentum step of the coupled form (ROMS step3d_uv with the "Couple 2D and 3D
- * momentum equations" section, nonlinear/step3d_uv.py:155-180 and :365-545; the live
+ * momentum equations" section, nonlinear/step3d_uv.py:155-180 and :365-544; the live
  * kernel step3D_uv.kernels.cpp:568-707 is the uncoupled sfo_step3d_UV above).
  *
  * The reference's solve forms RHS = Hz_u (u + cff dt pm pn ru), but ru already carries
