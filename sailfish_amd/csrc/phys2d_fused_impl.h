@@ -217,7 +217,8 @@ __device__ __forceinline__ auto arg_ptr(const PFA<T> &P)
 template <typename T, int TI, int TJ, int NT, bool FIRST, bool NTS, bool CPL = false, bool CLO = false>
 __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*sm)[PhysLds<TI, TJ>::FN],
                                           T *side, double *sideF = nullptr, uint32_t *clX = nullptr,
-                                          float *clN = nullptr, T *clS = nullptr)
+                                          float *clN = nullptr, T *clS = nullptr, T *clH = nullptr,
+                                          T *clF = nullptr)
 {
     // the arguments through a pointer that the closure instances re-derive after the closure
     // phase, so that pointers first needed from stage A on are loaded there again instead of
@@ -402,7 +403,7 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
     bool cin[KC];
     uint32_t cf[KC];
     T z1c[KC], u1c[KC], v1c[KC], hc_[KC], hw_[KC], hs_[KC];
-    T clh[CLO ? KC : 1][9], clf[CLO ? KC : 1][9];   // CLO: h and f over each core point's 3x3 stencil
+    T clf0[CLO ? KC : 1];                            // CLO: f at the core point
     T clsu[CLO ? KC : 1], clsv[CLO ? KC : 1];       // CLO: wind stress at the core point
     T pm_[KC], pn_[KC];
     T onuc[KC], omuc[KC], cffuc[KC], omvc[KC], onvc[KC], cffvc[KC], pmnc[KC];
@@ -443,14 +444,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             if (i >= 1) hw_[k] = ldo(Pp->h, -1, f);
             if (j >= 1) hs_[k] = ldo(Pp->h, -W, f);
         }
-        if constexpr (CLO) {   // the closure's h and f stencils and wind stress, in flight with the frame loads
-            const bool in = rho_int(i, j);
-#pragma unroll
-            for (int d = 0; d < 9; d++) {
-                const long long e = (long long)(d / 3 - 1) * W + (d % 3 - 1);
-                clh[k][d] = in ? ldo(Pp->h, e, f) : T(0);
-                clf[k][d] = in ? ldo(Pp->cl_f, e, f) : T(0);
-            }
+        if constexpr (CLO) {   // the closure's f and wind stress, in flight with the frame loads
+            clf0[k] = cin[k] ? ldo(Pp->cl_f, 0, f) : T(0);
             clsu[k] = u_int(i, j) ? ldo(Pp->sustr, 0, f) : T(0);
             clsv[k] = v_int(i, j) ? ldo(Pp->svstr, 0, f) : T(0);
         }
@@ -459,11 +454,13 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
         constexpr int RING = ring_size<TI, TJ, 3, 2>();
         constexpr int RS = (RING + NT - 1) / NT;
         T rz[RS], ru[RS], rv[RS], rm[RS], rn[RS];
+        T rh[CLO ? RS : 1], rf[CLO ? RS : 1];   // CLO: h and f of the ring for the closure's stencils
         int ro[RS];
 #pragma unroll
         for (int s = 0; s < RS; s++) {
             const int t = tid + s * NT;
             rz[s] = ru[s] = rv[s] = rm[s] = rn[s] = T(0);
+            if constexpr (CLO) rh[s] = rf[s] = T(0);
             ro[s] = -1;
             if (t < RING) {
                 int a, b;
@@ -477,6 +474,10 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                     rv[s] = ldo(Pp->v1, 0, f);
                     rm[s] = ldo(Pp->pm, 0, f);
                     rn[s] = ldo(Pp->pn, 0, f);
+                    if constexpr (CLO) {
+                        rh[s] = ldo(Pp->h, 0, f);
+                        rf[s] = ldo(Pp->cl_f, 0, f);
+                    }
                 }
             }
         }
@@ -490,6 +491,10 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             sV1[o] = cin[k] ? v1c[k] : T(0);
             sPM[o] = cin[k] ? pm_[k] : T(0);
             sPN[o] = cin[k] ? pn_[k] : T(0);
+            if constexpr (CLO) {
+                clH[o] = cin[k] ? hc_[k] : T(0);
+                clF[o] = clf0[k];
+            }
         }
 #pragma unroll
         for (int s = 0; s < RS; s++)
@@ -499,13 +504,17 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
                 sV1[ro[s]] = rv[s];
                 sPM[ro[s]] = rm[s];
                 sPN[ro[s]] = rn[s];
+                if constexpr (CLO) {
+                    clH[ro[s]] = rh[s];
+                    clF[ro[s]] = rf[s];
+                }
             }
     }
     lds_barrier();
 
     // ---- CL: the learned closure's corrector-stage stresses at the core points ----------
     // Each wave evaluates the network on its core row(s) of 64 points (features from the
-    // t1 frame planes; h, f and the wind stress loaded with the frame) and keeps
+    // t1 frame planes and the h, f planes staged with them) and keeps
     // sustr + R_u D_u / svstr + R_v D_v of its points for C2; those of the ghost-source
     // rows / columns (j = 1, M-1 / i = 1, L-1)
     // also go to clS for the ghost points of C2.  The same arithmetic as sf_closure_stress.
@@ -527,16 +536,8 @@ __device__ __forceinline__ void phys_tile(const PFA<T> &P, int ti, int tj, T (*s
             for (int kk = 0; kk < NIN; kk++) {
                 const int fld = kk / 9, dj = (kk % 9) / 3 - 1, di = kk % 3 - 1;
                 const int d = dj * FW + di;
-                T cv, nv;
-                if (fld < 3) {
-                    const T *pl = fld == 0 ? sU1 : fld == 1 ? sV1 : sZ1;
-                    cv = pl[o];
-                    nv = pl[o + d];
-                } else {
-                    const T(&st9)[9] = fld == 3 ? clh[k] : clf[k];
-                    cv = st9[4];
-                    nv = st9[(dj + 1) * 3 + (di + 1)];
-                }
+                const T *pl = fld == 0 ? sU1 : fld == 1 ? sV1 : fld == 2 ? sZ1 : fld == 3 ? clH : clF;
+                const T cv = pl[o], nv = pl[o + d];
                 xv[kk] = (dj == 0 && di == 0) ? (float)cv : (float)(nv - cv);
             }
             closure_row(X, tid & 63, xv, clN);
@@ -1141,9 +1142,10 @@ __global__ __launch_bounds__(NT) void k_phys_fused(PFA<T> P)
     __shared__ __attribute__((aligned(16))) uint32_t clX[CLO && !IN_FRAME ? (NT / 64) * 64 * XS : 1];
     __shared__ float clN[CLO ? 2 * NIN : 1];
     __shared__ T clS[CLO ? 2 * TI + 2 * TJ : 1];
+    __shared__ T clH[CLO ? PhysLds<TI, TJ>::FN : 1], clF[CLO ? PhysLds<TI, TJ>::FN : 1];   // h, f over the frame
     const int b = xcd_band(P.tiles_i * P.tiles_j);
     phys_tile<T, TI, TJ, NT, FIRST, NTS, CPL, CLO>(P, b % P.tiles_i, b / P.tiles_i, sm, side, sideF,
-                                                   IN_FRAME ? (uint32_t *)sm[1] : clX, clN, clS);
+                                                   IN_FRAME ? (uint32_t *)sm[1] : clX, clN, clS, clH, clF);
 }
 
 
