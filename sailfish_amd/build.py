@@ -44,9 +44,18 @@ def _hipcc() -> str:
     return exe
 
 
+# Matrix-core products of the closure network's inference write their results to
+# ordinary VGPRs (no accumulation-register copies before the ReLU packs) in the units
+# that evaluate the network inside other kernels.
+UNIT_FLAGS = {
+    name: ["-mllvm", "-amdgpu-mfma-vgpr-form"]
+    for name in ("phys2d_fused.hip", "phys2d_fused_f32.hip", "closure_infer.hip")
+}
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [_hipcc(), *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
+    cmd = [_hipcc(), *HIPCC_FLAGS, *UNIT_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src.name}:\n{res.stderr}")
