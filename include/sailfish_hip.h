@@ -523,7 +523,8 @@ int sf_closure_stress(sf_mlp *mlp, const sf_phys_fields *fields, int32_t precisi
 /* One physics substep (sf_phys_substep_fused) with the closure evaluated inside the tile
  * kernel: the corrector stage's surface stress is sf_closure_stress's sustr_eff /
  * svstr_eff of the substep's t1 state, the predictor keeps sustr / svstr.  f: the
- * Coriolis field in the state precision.  Tile height 4 or 8 (0: the default). */
+ * Coriolis field in the state precision.  Tile height 4 or 8 (0: 4, the height whose
+ * closure instances keep two waves per SIMD). */
 int sf_phys_substep_closure_fused(const sf_phys_fields *fields, const sf_mlp *mlp, const void *f,
                                   int32_t precision, double dt, int32_t first, double cff1, int32_t tile_j,
                                   void *stream);

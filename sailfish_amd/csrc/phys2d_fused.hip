@@ -69,7 +69,9 @@ extern "C" int sf_phys_substep_closure_fused(const sf_phys_fields *x, const sf_m
     SF_REQUIRE(x->rzeta[0] && x->rzeta[1] && x->rubar[0] && x->rubar[1] && x->rvbar[0] && x->rvbar[1],
                "sf_phys_substep_closure_fused: null rhs slot");
     SF_REQUIRE(x->Zt_avg1 && x->DU_avg1 && x->DV_avg1, "sf_phys_substep_closure_fused: null average");
-    const int tj = tile_j > 0 ? tile_j : sf_phys_fused_default_tile(precision, x->szI, x->szJ);
+    // default 64 x 4: those closure instances keep two waves per SIMD (the 64 x 8 ones need
+    // more registers than that allows, on any grid size)
+    const int tj = tile_j > 0 ? tile_j : 4;
     SF_REQUIRE(tj == 4 || tj == 8, "sf_phys_substep_closure_fused: tile height %d (4 or 8)", tj);
     const ClosureIn cl{mlp, f};
     if (precision == SF_FP64) return dispatch_tj<double>(x, dt, first, cff1, tj, stream, nullptr, &cl);
